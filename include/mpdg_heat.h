@@ -1,0 +1,168 @@
+/*
+ * mpdg_heat.h -- C ABI of the MI355X-native matrix-free heat operator path.
+ *
+ * Drop-in for MeltPoolDG's implicit heat operator, its preconditioners and the
+ * Krylov/Newton/predictor stack that drives it (citations are relative to the
+ * MeltPoolDG repository root). Plain C: device pointers are `void*`/typed
+ * pointers to device memory, streams are hipStream_t passed as `void*`
+ * (NULL = the default stream). Every call returns MPDG_OK (0) or a negative
+ * MPDG_ERR_* code; no exception crosses this boundary.
+ *
+ * Discretisation: structured hexahedral mesh of n_cells[0] x n_cells[1] x
+ * n_cells[2] cells on [0,extent[0]] x [0,extent[1]] x [0,extent[2]],
+ * continuous Q_k (k = 1..4) on Gauss-Lobatto nodes, QGauss(k+1). DoF vectors
+ * are lexicographic: i = ix + nx*(iy + ny*iz), nx = k*n_cells[0]+1. Per-qp
+ * arrays are cell-major: cell*(k+1)^3 + qx + (k+1)*(qy + (k+1)*qz) with
+ * cell = cx + n_cells[0]*(cy + n_cells[1]*cz). Faces: bit 0 x-, 1 x+, 2 y-,
+ * 3 y+, 4 z-, 5 z+ (deal.II's colorized hyper_rectangle boundary ids).
+ * Handles are not internally synchronised: use one handle per stream.
+ */
+#ifndef MPDG_HEAT_H
+#define MPDG_HEAT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPDG_OK 0
+#define MPDG_ERR_INVALID (-1)          /* invalid argument / unsupported configuration */
+#define MPDG_ERR_NOMEM (-2)            /* device allocation failed */
+#define MPDG_ERR_NO_CONVERGENCE (-3)   /* SolverControl::NoConvergence / ExcNewtonDidNotConverge */
+#define MPDG_ERR_NONFINITE (-4)        /* NaN/Inf met in a residual */
+#define MPDG_ERR_DEVICE (-5)           /* HIP runtime error */
+#define MPDG_ERR_ZERO_TIME_INCREMENT (-6) /* ExcZeroTimeIncrement (heat_diffuse_operator.cpp:788) */
+
+#define MPDG_FP64 0
+#define MPDG_FP32 1
+
+#define MPDG_KAPPA_CONST 0 /* one conductivity for the whole mesh */
+#define MPDG_KAPPA_QP 1    /* one value per quadrature point */
+#define MPDG_KAPPA_CELL 2  /* one value per cell */
+#define MPDG_KAPPA_LAW 3   /* k(T) from a temperature DoF vector at every quadrature point */
+
+#define MPDG_LAW_TANH 1  /* k0*(1 + a*tanh((T - T0)/w)), params {k0, a, T0, w} */
+#define MPDG_LAW_MUSHY 2 /* liquid_solid mushy zone, params {k_liquid, k_solid, T_solidus, T_liquidus}
+                            (include/meltpooldg/core/material.templates.hpp:296-330, :546-598, :765-778) */
+
+typedef struct mpdg_heat_desc {
+  int degree;          /* k, 1..4 */
+  int n_cells[3];
+  double extent[3];
+  int precision;       /* MPDG_FP64 or MPDG_FP32 (the number type of the operator's vectors) */
+  int dirichlet_faces; /* homogeneous Dirichlet faces (bitmask); constrained rows of J are 0 */
+  double rho_cp;       /* volumetric heat capacity rho*c_p */
+} mpdg_heat_desc;
+
+typedef struct mpdg_heat_op mpdg_heat_op;
+
+/* --- operator: HeatDiffuseMultiPhaseOperator<3,double> as an OperatorMatrixFree
+       (include/meltpooldg/core/operator_base.hpp:189-282) --- */
+int mpdg_heat_op_create(const mpdg_heat_desc *desc, mpdg_heat_op **op);
+int mpdg_heat_op_destroy(mpdg_heat_op *op);
+long mpdg_heat_op_n_dofs(const mpdg_heat_op *op);
+long mpdg_heat_op_n_cells(const mpdg_heat_op *op);
+/* OperatorBase::reset_time_increment (operator_base.hpp:42-46): mass coefficient rho_cp/dt */
+int mpdg_heat_op_reset_time_increment(mpdg_heat_op *op, double dt);
+/* conductivity data (Material::compute_parameters, heat_diffuse_operator.cpp:1172-1210);
+   arrays are device memory of the operator's precision and stay owned by the caller */
+int mpdg_heat_op_set_conductivity_constant(mpdg_heat_op *op, double k);
+int mpdg_heat_op_set_conductivity_qp(mpdg_heat_op *op, const void *k_qp);
+int mpdg_heat_op_set_conductivity_cell(mpdg_heat_op *op, const void *k_cell);
+int mpdg_heat_op_set_conductivity_law(mpdg_heat_op *op, int law, const double *params4,
+                                      const void *T_dofs);
+/* convection/radiation boundary faces (heat_diffuse_operator.cpp:1090-1129, :721-781);
+   T (device, FP64) is the temperature the radiation term is linearised about */
+int mpdg_heat_op_set_robin(mpdg_heat_op *op, int face_mask, double alpha, double T_inf_conv,
+                           double emissivity, double T_inf_rad, const double *T);
+/* Gaussian surface flux on face z+ used by create_rhs: q = peak*exp(-((x-x0-vx*t)^2+(y-y0)^2)/(2 sigma^2)),
+   p = {peak, sigma, x0, y0, vx, t}; peak = 0 switches it off */
+int mpdg_heat_op_set_surface_flux(mpdg_heat_op *op, const double *p6);
+
+/* vmult (heat_diffuse_operator.cpp:181-197): dst = J src, dst is overwritten */
+int mpdg_heat_op_vmult(const mpdg_heat_op *op, double *dst, const double *src, void *stream);
+int mpdg_heat_op_vmult_f32(const mpdg_heat_op *op, float *dst, const float *src, void *stream);
+/* compute_inverse_diagonal_from_matrixfree (heat_diffuse_operator.cpp:273-284):
+   inv = |d| > 1e-10 ? 1/d : 1, constrained rows 1 */
+int mpdg_heat_op_compute_inverse_diagonal(const mpdg_heat_op *op, double *inv_diag, void *stream);
+/* diagonal (not inverted) in the operator's precision */
+int mpdg_heat_op_compute_diagonal(const mpdg_heat_op *op, void *diag, void *stream);
+/* create_rhs (heat_diffuse_operator.cpp:785-806): dst += -R(T) with T_old and an optional
+   volumetric source q (DoF vector, may be NULL); conductivity laws are evaluated at T */
+int mpdg_heat_op_create_rhs(const mpdg_heat_op *op, double *dst, const double *T,
+                            const double *T_old, const double *q, void *stream);
+/* VectorTools::compute_norm (source/utilities/vector_tools.cpp:170-200): L2 norm of the FE function */
+int mpdg_heat_op_l2_norm(const mpdg_heat_op *op, const double *u, double *norm, void *stream);
+
+/* --- preconditioners: the Preconditioner concept update()/vmult()/reinit()
+       (include/meltpooldg/linear_algebra/preconditioner.hpp:19-37) --- */
+typedef struct mpdg_precond mpdg_precond;
+#define MPDG_PRECOND_IDENTITY 0
+#define MPDG_PRECOND_DIAGONAL 1 /* JacobiPreconditioner (preconditioner_jacobi.hpp:17-89) */
+#define MPDG_PRECOND_GMG 2      /* geometric multigrid V-cycle (recipe unit_tests/gmg.cc:343-383, :447-635) */
+
+typedef struct mpdg_gmg_params {
+  int precision;          /* MPDG_FP32 (mixed precision) or MPDG_FP64 level operators */
+  int n_cells_coarse;     /* stop h-coarsening at this many cells in x */
+  int coarse_degree;      /* p-coarsening to this degree on the coarsest mesh (0: keep) */
+  int smoother_degree;    /* Chebyshev degree */
+  double smoothing_range; /* Chebyshev smoothing range */
+  int eig_cg_n_iterations;/* CG steps of the eigenvalue estimate */
+} mpdg_gmg_params;
+
+int mpdg_precond_create(mpdg_heat_op *op, int type, const mpdg_gmg_params *gmg, mpdg_precond **p);
+int mpdg_precond_update(mpdg_precond *p, void *stream);
+int mpdg_precond_vmult(mpdg_precond *p, double *dst, const double *src, void *stream);
+int mpdg_precond_destroy(mpdg_precond *p);
+int mpdg_gmg_n_levels(const mpdg_precond *p);
+int mpdg_gmg_level_info(const mpdg_precond *p, int level, int *degree, int *n_cells,
+                        double *lambda_max);
+
+/* --- LinearSolver::solve (include/meltpooldg/linear_algebra/linear_solver.hpp:21-134) --- */
+#define MPDG_SOLVER_CG 0
+#define MPDG_SOLVER_FCG 1 /* flexible CG (Polak-Ribiere beta) for variable preconditioners */
+#define MPDG_CRIT_REDUCTION 0 /* ReductionControl: ||r|| <= max(abs_tol, rel_tol*||r0||) */
+#define MPDG_CRIT_RHS 1       /* ||b - J x|| <= rel_tol*||b|| */
+typedef struct mpdg_solver_params {
+  int solver_type;
+  int max_iterations;
+  double rel_tolerance;
+  double abs_tolerance;
+  int criterion;
+} mpdg_solver_params;
+/* x holds the initial guess on entry; iterations = last_step(); res = final ||r|| */
+int mpdg_linear_solve(mpdg_heat_op *op, mpdg_precond *p, const mpdg_solver_params *params,
+                      const double *b, double *x, int *iterations, double *res, void *stream);
+
+/* --- HeatDiffuseOperation: predictor + Newton-Raphson + linear solver per time step
+       (source/heat/heat_diffuse_operation.cpp:215-298, newton_raphson_solver.cpp:24-152,
+        include/meltpooldg/linear_algebra/predictor.hpp:118-160) --- */
+typedef struct mpdg_heat_operation mpdg_heat_operation;
+#define MPDG_PREDICTOR_NONE 0
+#define MPDG_PREDICTOR_ZERO 1
+#define MPDG_PREDICTOR_LINEAR 2
+#define MPDG_PREDICTOR_LEARNED 3
+typedef struct mpdg_heat_operation_params {
+  int predictor;
+  mpdg_solver_params linear;
+  int max_nonlinear_iterations;
+  double residual_tolerance;
+  double field_correction_tolerance;
+} mpdg_heat_operation_params;
+int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p,
+                               const mpdg_heat_operation_params *params,
+                               mpdg_heat_operation **h);
+int mpdg_heat_operation_destroy(mpdg_heat_operation *h);
+/* initial condition: T_{n} = T_{n-1} = T0 (device FP64) */
+int mpdg_heat_operation_set_initial(mpdg_heat_operation *h, const double *T0, void *stream);
+/* one implicit time step of size dt; lin_iters = accumulated Krylov iterations,
+   nl_iters = Newton iterations */
+int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_iters,
+                                   int *nl_iters, void *stream);
+/* device pointers to the current / previous solution (owned by the handle) */
+const double *mpdg_heat_operation_solution(const mpdg_heat_operation *h);
+const double *mpdg_heat_operation_old_solution(const mpdg_heat_operation *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,405 @@
+"""MI355X-native matrix-free heat operator path of MeltPoolDG.
+
+Python mirror of the reference's operator / preconditioner / solver interfaces,
+calling the HIP library ``libmpdg_heat.so`` through its C ABI
+(include/mpdg_heat.h). Vectors are torch tensors on the GPU (device memory and
+streams only); every computation happens in the library's HIP kernels. There is
+no CPU fallback: if the library or a GPU is missing, construction raises.
+
+Reference interfaces mirrored (paths relative to the MeltPoolDG repository):
+  HeatDiffuseMultiPhaseOperator  source/heat/heat_diffuse_operator.cpp:181-806
+  OperatorMatrixFree             include/meltpooldg/core/operator_base.hpp:189-282
+  JacobiPreconditioner           include/meltpooldg/linear_algebra/preconditioner_jacobi.hpp:17-89
+  LinearSolver / LinearSolverData include/meltpooldg/linear_algebra/linear_solver.hpp:21-134,
+                                 linear_solver_data.hpp:7-60
+  HeatDiffuseOperation + Newton  source/heat/heat_diffuse_operation.cpp:215-298,
+                                 source/linear_algebra/newton_raphson_solver.cpp:24-152
+  Predictor                      include/meltpooldg/linear_algebra/predictor.hpp:118-160
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmpdg_heat.so")
+
+MPDG_OK = 0
+ERRORS = {-1: "invalid argument", -2: "out of device memory", -3: "no convergence",
+          -4: "non-finite value", -5: "HIP runtime error", -6: "zero time increment"}
+FP64, FP32 = 0, 1
+KAPPA_CONST, KAPPA_QP, KAPPA_CELL, KAPPA_LAW = 0, 1, 2, 3
+LAW_TANH, LAW_MUSHY = 1, 2
+PRECOND_IDENTITY, PRECOND_DIAGONAL, PRECOND_GMG = 0, 1, 2
+SOLVER_CG, SOLVER_FCG = 0, 1
+CRIT_REDUCTION, CRIT_RHS = 0, 1
+PREDICTOR_NONE, PREDICTOR_ZERO, PREDICTOR_LINEAR, PREDICTOR_LEARNED = 0, 1, 2, 3
+
+
+class MpdgError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__("%s failed: %s (%d)" % (what, ERRORS.get(code, "error"), code))
+        self.code = code
+
+
+class NoConvergence(MpdgError):
+    """SolverControl::NoConvergence / ExcNewtonDidNotConverge."""
+
+
+class HeatDesc(C.Structure):
+    _fields_ = [("degree", C.c_int), ("n_cells", C.c_int * 3), ("extent", C.c_double * 3),
+                ("precision", C.c_int), ("dirichlet_faces", C.c_int), ("rho_cp", C.c_double)]
+
+
+class GMGParams(C.Structure):
+    _fields_ = [("precision", C.c_int), ("n_cells_coarse", C.c_int), ("coarse_degree", C.c_int),
+                ("smoother_degree", C.c_int), ("smoothing_range", C.c_double),
+                ("eig_cg_n_iterations", C.c_int)]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("solver_type", C.c_int), ("max_iterations", C.c_int), ("rel_tolerance", C.c_double),
+                ("abs_tolerance", C.c_double), ("criterion", C.c_int)]
+
+
+class OperationParams(C.Structure):
+    _fields_ = [("predictor", C.c_int), ("linear", SolverParams), ("max_nonlinear_iterations", C.c_int),
+                ("residual_tolerance", C.c_double), ("field_correction_tolerance", C.c_double)]
+
+
+_V, _I, _D, _L = C.c_void_p, C.c_int, C.c_double, C.c_long
+_SIGNATURES = {
+    "mpdg_heat_op_create": ([C.POINTER(HeatDesc), C.POINTER(_V)], _I),
+    "mpdg_heat_op_destroy": ([_V], _I),
+    "mpdg_heat_op_n_dofs": ([_V], _L),
+    "mpdg_heat_op_n_cells": ([_V], _L),
+    "mpdg_heat_op_reset_time_increment": ([_V, _D], _I),
+    "mpdg_heat_op_set_conductivity_constant": ([_V, _D], _I),
+    "mpdg_heat_op_set_conductivity_qp": ([_V, _V], _I),
+    "mpdg_heat_op_set_conductivity_cell": ([_V, _V], _I),
+    "mpdg_heat_op_set_conductivity_law": ([_V, _I, C.POINTER(_D), _V], _I),
+    "mpdg_heat_op_set_robin": ([_V, _I, _D, _D, _D, _D, _V], _I),
+    "mpdg_heat_op_set_surface_flux": ([_V, C.POINTER(_D)], _I),
+    "mpdg_heat_op_vmult": ([_V, _V, _V, _V], _I),
+    "mpdg_heat_op_vmult_f32": ([_V, _V, _V, _V], _I),
+    "mpdg_heat_op_compute_inverse_diagonal": ([_V, _V, _V], _I),
+    "mpdg_heat_op_compute_diagonal": ([_V, _V, _V], _I),
+    "mpdg_heat_op_create_rhs": ([_V, _V, _V, _V, _V, _V], _I),
+    "mpdg_heat_op_l2_norm": ([_V, _V, C.POINTER(_D), _V], _I),
+    "mpdg_precond_create": ([_V, _I, C.POINTER(GMGParams), C.POINTER(_V)], _I),
+    "mpdg_precond_update": ([_V, _V], _I),
+    "mpdg_precond_vmult": ([_V, _V, _V, _V], _I),
+    "mpdg_precond_destroy": ([_V], _I),
+    "mpdg_gmg_n_levels": ([_V], _I),
+    "mpdg_gmg_level_info": ([_V, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)], _I),
+    "mpdg_linear_solve": ([_V, _V, C.POINTER(SolverParams), _V, _V, C.POINTER(_I), C.POINTER(_D), _V], _I),
+    "mpdg_heat_operation_create": ([_V, _V, C.POINTER(OperationParams), C.POINTER(_V)], _I),
+    "mpdg_heat_operation_destroy": ([_V], _I),
+    "mpdg_heat_operation_set_initial": ([_V, _V, _V], _I),
+    "mpdg_heat_operation_solve_step": ([_V, _D, C.POINTER(_I), C.POINTER(_I), _V], _I),
+    "mpdg_heat_operation_solution": ([_V], _V),
+    "mpdg_heat_operation_old_solution": ([_V], _V),
+}
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libmpdg_heat.so and bind every exported entry point (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError("libmpdg_heat.so not built: run __graft_entry__.build() (%s)" % path)
+    L = C.CDLL(path)
+    for name, (args, res) in _SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)
+
+
+def _check(rc, what):
+    if rc != MPDG_OK:
+        if rc == -3:
+            raise NoConvergence(rc, what)
+        raise MpdgError(rc, what)
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("meltpooldg_amd needs a GPU: torch.cuda.is_available() is False")
+    return torch
+
+
+def _stream():
+    torch = _torch()
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t, dtype=None):
+    if t is None:
+        return None
+    if dtype is not None and t.dtype != dtype:
+        raise TypeError("expected %s, got %s" % (dtype, t.dtype))
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError("expected a contiguous GPU tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+class HeatDiffuseMultiPhaseOperator:
+    """J = (rho c_p / dt) M + K_k (+ Robin faces) on a structured Q_k hex mesh.
+
+    Mirrors OperatorMatrixFree<3,double>: vmult, create_rhs,
+    compute_inverse_diagonal_from_matrixfree, reset_time_increment.
+    Attached arrays (conductivity, temperature) stay owned by the caller; this
+    object keeps a reference so they outlive the operator.
+    """
+
+    def __init__(self, degree, n_cells, extent=(1.0, 1.0, 1.0), rho_cp=1.0, precision=FP64,
+                 dirichlet_faces=0):
+        self.lib = load_library()
+        _torch()
+        d = HeatDesc()
+        d.degree = degree
+        d.n_cells[:] = list(n_cells)
+        d.extent[:] = list(extent)
+        d.precision = precision
+        d.dirichlet_faces = dirichlet_faces
+        d.rho_cp = rho_cp
+        h = C.c_void_p()
+        _check(self.lib.mpdg_heat_op_create(C.byref(d), C.byref(h)), "mpdg_heat_op_create")
+        self.h = h
+        self.degree, self.n_cells, self.extent, self.precision = degree, tuple(n_cells), tuple(extent), precision
+        self.n_dofs = self.lib.mpdg_heat_op_n_dofs(h)
+        self.n_cells_total = self.lib.mpdg_heat_op_n_cells(h)
+        self.n_q_points = self.n_cells_total * (degree + 1) ** 3
+        self._keep = {}
+
+    @property
+    def dtype(self):
+        torch = _torch()
+        return torch.float64 if self.precision == FP64 else torch.float32
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.mpdg_heat_op_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def reset_time_increment(self, dt):
+        _check(self.lib.mpdg_heat_op_reset_time_increment(self.h, dt), "reset_time_increment")
+
+    def set_conductivity_constant(self, k):
+        _check(self.lib.mpdg_heat_op_set_conductivity_constant(self.h, k), "set_conductivity")
+
+    def set_conductivity_qp(self, k_qp):
+        assert k_qp.numel() == self.n_q_points
+        self._keep["k"] = k_qp
+        _check(self.lib.mpdg_heat_op_set_conductivity_qp(self.h, _ptr(k_qp, self.dtype)), "set_conductivity_qp")
+
+    def set_conductivity_cell(self, k_cell):
+        assert k_cell.numel() == self.n_cells_total
+        self._keep["k"] = k_cell
+        _check(self.lib.mpdg_heat_op_set_conductivity_cell(self.h, _ptr(k_cell, self.dtype)),
+               "set_conductivity_cell")
+
+    def set_conductivity_law(self, law, params, T):
+        assert T.numel() == self.n_dofs
+        self._keep["T"] = T
+        p = (C.c_double * 4)(*params)
+        _check(self.lib.mpdg_heat_op_set_conductivity_law(self.h, law, p, _ptr(T, self.dtype)),
+               "set_conductivity_law")
+
+    def set_robin(self, face_mask, alpha=0.0, T_inf_conv=0.0, emissivity=0.0, T_inf_rad=0.0, T=None):
+        self._keep["Trobin"] = T
+        _check(self.lib.mpdg_heat_op_set_robin(self.h, face_mask, alpha, T_inf_conv, emissivity, T_inf_rad,
+                                               _ptr(T)), "set_robin")
+
+    def set_surface_flux(self, peak, sigma, x0, y0, vx, t):
+        p = (C.c_double * 6)(peak, sigma, x0, y0, vx, t)
+        _check(self.lib.mpdg_heat_op_set_surface_flux(self.h, p), "set_surface_flux")
+
+    def vmult(self, dst, src):
+        if self.precision == FP64:
+            _check(self.lib.mpdg_heat_op_vmult(self.h, _ptr(dst, self.dtype), _ptr(src, self.dtype), _stream()),
+                   "vmult")
+        else:
+            _check(self.lib.mpdg_heat_op_vmult_f32(self.h, _ptr(dst, self.dtype), _ptr(src, self.dtype),
+                                                   _stream()), "vmult_f32")
+
+    def compute_inverse_diagonal_from_matrixfree(self, diag):
+        _check(self.lib.mpdg_heat_op_compute_inverse_diagonal(self.h, _ptr(diag), _stream()),
+               "compute_inverse_diagonal")
+
+    def compute_diagonal(self, diag):
+        _check(self.lib.mpdg_heat_op_compute_diagonal(self.h, _ptr(diag, self.dtype), _stream()),
+               "compute_diagonal")
+
+    def create_rhs(self, dst, T, T_old, q=None):
+        _check(self.lib.mpdg_heat_op_create_rhs(self.h, _ptr(dst), _ptr(T), _ptr(T_old), _ptr(q), _stream()),
+               "create_rhs")
+
+    def l2_norm(self, u):
+        out = C.c_double()
+        _check(self.lib.mpdg_heat_op_l2_norm(self.h, _ptr(u), C.byref(out), _stream()), "l2_norm")
+        return out.value
+
+    def new_vector(self):
+        torch = _torch()
+        return torch.zeros(self.n_dofs, dtype=self.dtype, device="cuda")
+
+
+class Preconditioner:
+    """Type-erased preconditioner (include/meltpooldg/linear_algebra/preconditioner.hpp:41-203)."""
+
+    def __init__(self, op, type_, gmg=None):
+        self.lib = op.lib
+        self.op = op
+        h = C.c_void_p()
+        _check(self.lib.mpdg_precond_create(op.h, type_, C.byref(gmg) if gmg is not None else None, C.byref(h)),
+               "mpdg_precond_create")
+        self.h = h
+        self.type = type_
+
+    def update(self):
+        _check(self.lib.mpdg_precond_update(self.h, _stream()), "precond update")
+
+    def vmult(self, dst, src):
+        _check(self.lib.mpdg_precond_vmult(self.h, _ptr(dst), _ptr(src), _stream()), "precond vmult")
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.mpdg_precond_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class JacobiPreconditioner(Preconditioner):
+    def __init__(self, op):
+        super().__init__(op, PRECOND_DIAGONAL)
+
+
+class GMGPreconditioner(Preconditioner):
+    """Geometric multigrid V-cycle (recipe of unit_tests/gmg.cc) with FP32 or FP64 levels."""
+
+    def __init__(self, op, precision=FP32, n_cells_coarse=4, coarse_degree=0, smoother_degree=3,
+                 smoothing_range=20.0, eig_cg_n_iterations=20):
+        p = GMGParams()
+        p.precision, p.n_cells_coarse, p.coarse_degree = precision, n_cells_coarse, coarse_degree
+        p.smoother_degree, p.smoothing_range, p.eig_cg_n_iterations = smoother_degree, smoothing_range, \
+            eig_cg_n_iterations
+        super().__init__(op, PRECOND_GMG, p)
+
+    def levels(self):
+        out = []
+        for l in range(self.lib.mpdg_gmg_n_levels(self.h)):
+            deg, nc, lam = C.c_int(), (C.c_int * 3)(), C.c_double()
+            _check(self.lib.mpdg_gmg_level_info(self.h, l, C.byref(deg), nc, C.byref(lam)), "level_info")
+            out.append((deg.value, tuple(nc), lam.value))
+        return out
+
+
+@dataclass
+class LinearSolverData:
+    """include/meltpooldg/linear_algebra/linear_solver_data.hpp:36-56 (CG on this path)."""
+    solver_type: int = SOLVER_CG
+    max_iterations: int = 10000
+    rel_tolerance: float = 1e-12
+    abs_tolerance: float = 1e-20
+    criterion: int = CRIT_REDUCTION
+
+    def c(self):
+        p = SolverParams()
+        p.solver_type, p.max_iterations = self.solver_type, self.max_iterations
+        p.rel_tolerance, p.abs_tolerance, p.criterion = self.rel_tolerance, self.abs_tolerance, self.criterion
+        return p
+
+
+class LinearSolver:
+    @staticmethod
+    def solve(op, solution, rhs, data=None, preconditioner=None):
+        """Returns the iteration count (solver_control.last_step()); raises NoConvergence."""
+        data = data or LinearSolverData()
+        it, res = C.c_int(), C.c_double()
+        rc = op.lib.mpdg_linear_solve(op.h, preconditioner.h if preconditioner is not None else None,
+                                      C.byref(data.c()), _ptr(rhs), _ptr(solution), C.byref(it), C.byref(res),
+                                      _stream())
+        LinearSolver.last_residual = res.value
+        _check(rc, "LinearSolver::solve")
+        return it.value
+
+
+@dataclass
+class NonlinearSolverData:
+    """include/meltpooldg/linear_algebra/nonlinear_solver_data.hpp:13-19"""
+    max_nonlinear_iterations: int = 10
+    field_correction_tolerance: float = 1e-10
+    residual_tolerance: float = 1e-9
+
+
+class HeatDiffuseOperation:
+    """Implicit heat time step: predictor, preconditioner update, Newton-Raphson with CG."""
+
+    def __init__(self, op, preconditioner=None, predictor=PREDICTOR_LINEAR, linear=None, nlsolve=None):
+        self.op = op
+        self.lib = op.lib
+        self.pc = preconditioner
+        linear = linear or LinearSolverData()
+        nlsolve = nlsolve or NonlinearSolverData()
+        p = OperationParams()
+        p.predictor = predictor
+        p.linear = linear.c()
+        p.max_nonlinear_iterations = nlsolve.max_nonlinear_iterations
+        p.residual_tolerance = nlsolve.residual_tolerance
+        p.field_correction_tolerance = nlsolve.field_correction_tolerance
+        h = C.c_void_p()
+        _check(self.lib.mpdg_heat_operation_create(op.h, preconditioner.h if preconditioner else None,
+                                                   C.byref(p), C.byref(h)), "heat_operation_create")
+        self.h = h
+
+    def set_initial_condition(self, T0):
+        _check(self.lib.mpdg_heat_operation_set_initial(self.h, _ptr(T0), _stream()), "set_initial")
+
+    def solve(self, dt):
+        li, ni = C.c_int(), C.c_int()
+        _check(self.lib.mpdg_heat_operation_solve_step(self.h, dt, C.byref(li), C.byref(ni), _stream()),
+               "HeatDiffuseOperation::solve")
+        return li.value, ni.value
+
+    def solution(self):
+        """A copy of the current temperature (device)."""
+        torch = _torch()
+        out = torch.empty(self.op.n_dofs, dtype=torch.float64, device="cuda")
+        ptr = self.lib.mpdg_heat_operation_solution(self.h)
+        torch.cuda.synchronize()
+        _copy_device(out, ptr, self.op.n_dofs)
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.mpdg_heat_operation_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def _copy_device(dst, src_ptr, n):
+    """dst (torch, fp64) <- n doubles at the device address src_ptr."""
+    import torch
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    rc = hip.hipMemcpy(C.c_void_p(dst.data_ptr()), C.c_void_p(src_ptr), n * 8, 3)
+    if rc != 0:
+        raise MpdgError(-5, "hipMemcpy")
+    torch.cuda.synchronize()

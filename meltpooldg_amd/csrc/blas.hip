@@ -1,0 +1,470 @@
+// Vector kernels of the Krylov solvers and the multigrid smoother.
+//
+// Reductions are deterministic: every block writes one partial per quantity,
+// the last block to finish (atomic ticket) sums the partials in block order and
+// updates the solver scalars on the device, so the host never has to read a
+// dot product to launch the next kernel. Kernels of an iteration return early
+// once the device-side "done" flag is set, which lets the host enqueue several
+// iterations before it looks at the convergence state.
+#include "blas.hpp"
+
+namespace mpdg {
+
+template <typename T>
+__global__ __launch_bounds__(256) void fill_kernel(T *x, T v, long n) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) x[i] = v;
+}
+template <typename T>
+int launch_fill(T *x, T v, long n, hipStream_t s) {
+  const long nb = std::min<long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(fill_kernel<T>, dim3(std::max<long>(nb, 1)), dim3(256), 0, s, x, v, n);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+template int launch_fill<double>(double *, double, long, hipStream_t);
+template int launch_fill<float>(float *, float, long, hipStream_t);
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void convert_kernel(const TA *a, TB *b, long n) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) b[i] = (TB)a[i];
+}
+template <typename TA, typename TB>
+int launch_convert(const TA *a, TB *b, long n, hipStream_t s) {
+  const long nb = std::min<long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL((convert_kernel<TA, TB>), dim3(std::max<long>(nb, 1)), dim3(256), 0, s, a, b, n);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+template int launch_convert<double, float>(const double *, float *, long, hipStream_t);
+template int launch_convert<float, double>(const float *, double *, long, hipStream_t);
+template int launch_convert<double, double>(const double *, double *, long, hipStream_t);
+
+__global__ __launch_bounds__(256) void axpby_kernel(double *y, double a, const double *x, double b, long n) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) y[i] = a * x[i] + b * y[i];
+}
+int launch_axpby(double *y, double a, const double *x, double b, long n, hipStream_t s) {
+  const long nb = std::min<long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(axpby_kernel, dim3(std::max<long>(nb, 1)), dim3(256), 0, s, y, a, x, b, n);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void invert_diag_kernel(T *d, long n) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const T v = d[i];
+    d[i] = fabs((double)v) > 1.0e-10 ? T(1) / v : T(1);  // heat_diffuse_operator.cpp:282-283
+  }
+}
+template <typename T>
+int launch_invert_diagonal(T *d, long n, hipStream_t s) {
+  const long nb = std::min<long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(invert_diag_kernel<T>, dim3(std::max<long>(nb, 1)), dim3(256), 0, s, d, n);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+template int launch_invert_diagonal<double>(double *, long, hipStream_t);
+template int launch_invert_diagonal<float>(float *, long, hipStream_t);
+
+// ---- block reduction helpers ----
+template <int NQ>
+__device__ __forceinline__ void block_reduce(double (&v)[NQ], double *sh) {
+  // sh: 256*NQ doubles
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) sh[q * 256 + threadIdx.x] = v[q];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) sh[q * 256 + threadIdx.x] += sh[q * 256 + threadIdx.x + s];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) v[q] = sh[q * 256];
+}
+
+// Write the block partials; returns true in the last block, whose v then holds the totals.
+template <int NQ>
+__device__ __forceinline__ bool reduce_to_last_block(double (&v)[NQ], double *partials, unsigned *counter) {
+  __shared__ double sh[256 * NQ];
+  __shared__ bool is_last;
+  block_reduce<NQ>(v, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) partials[q * kRedBlocks + blockIdx.x] = v[q];
+    __threadfence();
+    const unsigned t = atomicAdd(counter, 1u);
+    is_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+  double w[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) w[q] = 0.0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += 256)
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) w[q] += __builtin_nontemporal_load(partials + q * kRedBlocks + b);
+  // fixed-order tree over the 256 thread sums (deterministic for a fixed grid)
+  block_reduce<NQ>(w, sh);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) v[q] = w[q];
+  if (threadIdx.x == 0) *counter = 0;
+  return true;
+}
+
+inline int red_blocks(long n) { return (int)std::max<long>(1, std::min<long>(kRedBlocks, (n + 1023) / 1024)); }
+
+// ---- generic dot product into a device scalar ----
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void dot_kernel(const TA *a, const TB *b, long n, double *partials,
+                                                  unsigned *counter, double *out) {
+  double v[1] = {0.0};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+    v[0] += (double)a[i] * (double)b[i];
+  if (reduce_to_last_block<1>(v, partials, counter) && threadIdx.x == 0) *out = v[0];
+}
+template <typename TA, typename TB>
+int launch_dot(const TA *a, const TB *b, long n, const Reduction &r, double *out, hipStream_t s) {
+  hipLaunchKernelGGL((dot_kernel<TA, TB>), dim3(red_blocks(n)), dim3(256), 0, s, a, b, n, r.partials, r.counter,
+                     out);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+template int launch_dot<double, double>(const double *, const double *, long, const Reduction &, double *, hipStream_t);
+template int launch_dot<float, float>(const float *, const float *, long, const Reduction &, double *, hipStream_t);
+
+// ================= preconditioned CG =================
+// k1: alpha = rz / (p . q)
+__global__ __launch_bounds__(256) void cg_alpha_kernel(const double *p, const double *q, long n, double *partials,
+                                                       unsigned *counter, CGState *st) {
+  if (st->done) return;
+  double v[1] = {0.0};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) v[0] += p[i] * q[i];
+  if (reduce_to_last_block<1>(v, partials, counter) && threadIdx.x == 0) {
+    st->pq = v[0];
+    st->alpha = st->rz / v[0];
+  }
+}
+
+// k2: x += alpha p, r -= alpha q, (Jacobi: z = inv r), reductions of r.r and r.z (or r.z_old)
+template <bool JACOBI>
+__global__ __launch_bounds__(256) void cg_update_kernel(double *x, double *r, const double *p, const double *q,
+                                                        double *z, const double *inv, long n, double *partials,
+                                                        unsigned *counter, CGState *st) {
+  if (st->done) return;
+  const double alpha = st->alpha;
+  double v[2] = {0.0, 0.0};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    v[0] += ri * ri;
+    if (JACOBI) {
+      const double zi = inv ? inv[i] * ri : ri;
+      z[i] = zi;
+      v[1] += ri * zi;
+    } else {
+      v[1] += ri * z[i];  // r_new . z_old for the flexible beta
+    }
+  }
+  if (reduce_to_last_block<2>(v, partials, counter) && threadIdx.x == 0) {
+    st->rr = v[0];
+    st->iters += 1;
+    const bool conv = v[0] <= st->tol2;
+    const bool bad = !(v[0] == v[0]) || !(fabs(v[0]) <= 1.7e308);
+    if (JACOBI) {
+      st->beta = v[1] / st->rz;
+      st->rz = v[1];
+    } else {
+      st->rzold_new = v[1];
+    }
+    if (conv || bad || st->iters >= st->max_it) st->done = bad ? 2 : 1;
+  }
+}
+
+// k3 (external preconditioner): beta from r.z_new
+__global__ __launch_bounds__(256) void cg_beta_kernel(const double *r, const double *z, long n, double *partials,
+                                                      unsigned *counter, CGState *st) {
+  if (st->done) return;
+  double v[1] = {0.0};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) v[0] += r[i] * z[i];
+  if (reduce_to_last_block<1>(v, partials, counter) && threadIdx.x == 0) {
+    const double num = st->flexible ? v[0] - st->rzold_new : v[0];
+    st->beta = num / st->rz;
+    st->rz = v[0];
+  }
+}
+
+// k4: p = z + beta p
+__global__ __launch_bounds__(256) void cg_direction_kernel(double *p, const double *z, long n, const CGState *st) {
+  if (st->done) return;
+  const double beta = st->beta;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = z[i] + beta * p[i];
+}
+
+// initial: r = b - Ax (Ax in q), z = P r (Jacobi / identity), rr, rz, bb; p = z
+template <bool JACOBI>
+__global__ __launch_bounds__(256) void cg_init_kernel(const double *b, double *r, const double *q, double *z,
+                                                      double *p, const double *inv, long n, double *partials,
+                                                      unsigned *counter, CGState *st, double rel_tol,
+                                                      double abs_tol, int criterion) {
+  double v[3] = {0.0, 0.0, 0.0};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const double ri = b[i] - q[i];
+    r[i] = ri;
+    v[0] += ri * ri;
+    v[2] += b[i] * b[i];
+    if (JACOBI) {
+      const double zi = inv ? inv[i] * ri : ri;
+      z[i] = zi;
+      p[i] = zi;
+      v[1] += ri * zi;
+    }
+  }
+  if (reduce_to_last_block<3>(v, partials, counter) && threadIdx.x == 0) {
+    st->rr = v[0];
+    st->rr0 = v[0];
+    st->bb = v[2];
+    st->rz = v[1];
+    st->iters = 0;
+    st->rzold_new = 0.0;
+    double tol;
+    if (criterion == MPDG_CRIT_RHS) tol = rel_tol * sqrt(v[2]);
+    else tol = fmax(abs_tol, rel_tol * sqrt(v[0]));
+    st->tol2 = tol * tol;
+    const bool bad = !(v[0] == v[0]) || !(fabs(v[0]) <= 1.7e308);
+    st->done = bad ? 2 : (v[0] <= st->tol2 ? 1 : 0);
+  }
+}
+
+__global__ void cg_copy_kernel(double *p, const double *z, long n, const CGState *st) {
+  if (st->done) return;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = z[i];
+}
+__global__ __launch_bounds__(256) void cg_rz0_kernel(const double *r, const double *z, long n, double *partials,
+                                                     unsigned *counter, CGState *st) {
+  if (st->done) return;
+  double v[1] = {0.0};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) v[0] += r[i] * z[i];
+  if (reduce_to_last_block<1>(v, partials, counter) && threadIdx.x == 0) st->rz = v[0];
+}
+
+int cg_init(const double *b, double *r, const double *q, double *z, double *p, const double *inv, bool jacobi,
+            long n, const Reduction &red, CGState *st, double rel_tol, double abs_tol, int criterion, hipStream_t s) {
+  if (jacobi)
+    hipLaunchKernelGGL(cg_init_kernel<true>, dim3(red_blocks(n)), dim3(256), 0, s, b, r, q, z, p, inv, n,
+                       red.partials, red.counter, st, rel_tol, abs_tol, criterion);
+  else
+    hipLaunchKernelGGL(cg_init_kernel<false>, dim3(red_blocks(n)), dim3(256), 0, s, b, r, q, z, p, inv, n,
+                       red.partials, red.counter, st, rel_tol, abs_tol, criterion);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+int cg_after_external_precond_init(const double *r, const double *z, double *p, long n, const Reduction &red,
+                                   CGState *st, hipStream_t s) {
+  hipLaunchKernelGGL(cg_rz0_kernel, dim3(red_blocks(n)), dim3(256), 0, s, r, z, n, red.partials, red.counter, st);
+  hipLaunchKernelGGL(cg_copy_kernel, dim3(std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, s, p, z, n, st);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+int cg_alpha(const double *p, const double *q, long n, const Reduction &red, CGState *st, hipStream_t s) {
+  hipLaunchKernelGGL(cg_alpha_kernel, dim3(red_blocks(n)), dim3(256), 0, s, p, q, n, red.partials, red.counter, st);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+int cg_update(double *x, double *r, const double *p, const double *q, double *z, const double *inv, bool jacobi,
+              long n, const Reduction &red, CGState *st, hipStream_t s) {
+  if (jacobi)
+    hipLaunchKernelGGL(cg_update_kernel<true>, dim3(red_blocks(n)), dim3(256), 0, s, x, r, p, q, z, inv, n,
+                       red.partials, red.counter, st);
+  else
+    hipLaunchKernelGGL(cg_update_kernel<false>, dim3(red_blocks(n)), dim3(256), 0, s, x, r, p, q, z, inv, n,
+                       red.partials, red.counter, st);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+int cg_beta(const double *r, const double *z, long n, const Reduction &red, CGState *st, hipStream_t s) {
+  hipLaunchKernelGGL(cg_beta_kernel, dim3(red_blocks(n)), dim3(256), 0, s, r, z, n, red.partials, red.counter, st);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+int cg_direction(double *p, const double *z, long n, const CGState *st, hipStream_t s) {
+  hipLaunchKernelGGL(cg_direction_kernel, dim3(std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, s, p, z, n, st);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+// ================= multigrid smoother / transfer =================
+// Chebyshev step: r = b - Ax; d = c1 d + c2 inv r; x += d   (first step: d = inv b / theta, x = d)
+template <typename T>
+__global__ __launch_bounds__(256) void cheb_kernel(T *x, T *d, const T *b, const T *Ax, const T *inv, long n,
+                                                   T c1, T c2, int first) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const T r = first ? b[i] : b[i] - Ax[i];
+    const T di = first ? c2 * inv[i] * r : c1 * d[i] + c2 * inv[i] * r;
+    d[i] = di;
+    x[i] = first ? di : x[i] + di;
+  }
+}
+template <typename T>
+int launch_chebyshev_step(T *x, T *d, const T *b, const T *Ax, const T *inv, long n, double c1, double c2,
+                          bool first, hipStream_t s) {
+  hipLaunchKernelGGL(cheb_kernel<T>, dim3(std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, s, x, d, b, Ax, inv,
+                     n, (T)c1, (T)c2, first ? 1 : 0);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+template int launch_chebyshev_step<float>(float *, float *, const float *, const float *, const float *, long,
+                                          double, double, bool, hipStream_t);
+template int launch_chebyshev_step<double>(double *, double *, const double *, const double *, const double *, long,
+                                           double, double, bool, hipStream_t);
+
+// r = b - Ax
+template <typename T>
+__global__ __launch_bounds__(256) void residual_kernel(T *r, const T *b, const T *Ax, long n) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) r[i] = b[i] - Ax[i];
+}
+template <typename T>
+int launch_residual(T *r, const T *b, const T *Ax, long n, hipStream_t s) {
+  hipLaunchKernelGGL(residual_kernel<T>, dim3(std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, s, r, b, Ax, n);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+template int launch_residual<float>(float *, const float *, const float *, long, hipStream_t);
+template int launch_residual<double>(double *, const double *, const double *, long, hipStream_t);
+
+// x += y
+template <typename T>
+__global__ __launch_bounds__(256) void add_kernel(T *x, const T *y, long n) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) x[i] += y[i];
+}
+template <typename T>
+int launch_add(T *x, const T *y, long n, hipStream_t s) {
+  hipLaunchKernelGGL(add_kernel<T>, dim3(std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, s, x, y, n);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+template int launch_add<float>(float *, const float *, long, hipStream_t);
+template int launch_add<double>(double *, const double *, long, hipStream_t);
+
+// 1D ELL matrix applied along direction `dir` of a lexicographic 3D array:
+// out(i with i_dir = row) = sum_e val[row][e] in(i with i_dir = col[row][e])
+template <typename T>
+__global__ __launch_bounds__(256) void ell_dir_kernel(const T *__restrict__ in, T *__restrict__ out, int o0, int o1,
+                                                      int o2, int i0, int i1, int dir, int width,
+                                                      const int *__restrict__ col, const T *__restrict__ val) {
+  const long n = (long)o0 * o1 * o2;
+  for (long g = (long)blockIdx.x * 256 + threadIdx.x; g < n; g += (long)gridDim.x * 256) {
+    const int x = g % o0, y = (g / o0) % o1, z = g / ((long)o0 * o1);
+    const int row = dir == 0 ? x : (dir == 1 ? y : z);
+    T acc = 0;
+    for (int e = 0; e < width; ++e) {
+      const int c = col[row * width + e];
+      if (c < 0) continue;
+      const int xx = dir == 0 ? c : x, yy = dir == 1 ? c : y, zz = dir == 2 ? c : z;
+      acc += val[row * width + e] * in[xx + (long)i0 * (yy + (long)i1 * zz)];
+    }
+    out[g] = acc;
+  }
+}
+template <typename T>
+int launch_ell_dir(const T *in, T *out, const int *odims, const int *idims, int dir, int width, const int *col,
+                   const T *val, hipStream_t s) {
+  const long n = (long)odims[0] * odims[1] * odims[2];
+  hipLaunchKernelGGL(ell_dir_kernel<T>, dim3(std::min<long>((n + 255) / 256, 16384)), dim3(256), 0, s, in, out,
+                     odims[0], odims[1], odims[2], idims[0], idims[1], dir, width, col, val);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+template int launch_ell_dir<float>(const float *, float *, const int *, const int *, int, int, const int *,
+                                   const float *, hipStream_t);
+template int launch_ell_dir<double>(const double *, double *, const int *, const int *, int, int, const int *,
+                                    const double *, hipStream_t);
+
+// y = A x for a dense column-major FP64 matrix (coarse-grid solve with the inverse)
+__global__ __launch_bounds__(256) void dense_gemv_kernel(const double *__restrict__ A, const double *__restrict__ x,
+                                                         double *__restrict__ y, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int j = 0; j < n; ++j) acc += A[(long)j * n + i] * x[j];
+  y[i] = acc;
+}
+int launch_dense_gemv(const double *A, const double *x, double *y, int n, hipStream_t s) {
+  hipLaunchKernelGGL(dense_gemv_kernel, dim3((n + 255) / 256), dim3(256), 0, s, A, x, y, n);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+// deterministic start vector of the eigenvalue estimate (constrained entries 0)
+template <typename T>
+__global__ __launch_bounds__(256) void eig_start_kernel(T *v, int nd0, int nd1, int nd2, int mask) {
+  const long n = (long)nd0 * nd1 * nd2;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const int ix = i % nd0, iy = (i / nd0) % nd1, iz = i / ((long)nd0 * nd1);
+    const bool con = ((mask & 1) && ix == 0) || ((mask & 2) && ix == nd0 - 1) || ((mask & 4) && iy == 0) ||
+                     ((mask & 8) && iy == nd1 - 1) || ((mask & 16) && iz == 0) || ((mask & 32) && iz == nd2 - 1);
+    v[i] = con ? T(0) : (T)(1.0 + 0.5 * sin(0.9 * (double)(i % 1000003)));
+  }
+}
+template <typename T>
+int launch_eig_start(T *v, const int *nd, int mask, hipStream_t s) {
+  const long n = (long)nd[0] * nd[1] * nd[2];
+  hipLaunchKernelGGL(eig_start_kernel<T>, dim3(std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, s, v, nd[0],
+                     nd[1], nd[2], mask);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+template int launch_eig_start<float>(float *, const int *, int, hipStream_t);
+template int launch_eig_start<double>(double *, const int *, int, hipStream_t);
+
+// CG step kernels of the eigenvalue estimate in the level precision, scalars in FP64:
+// alpha_k = rz/(p.Ap) recorded in hist[2k]; x += alpha p; r -= alpha Ap; z = inv r;
+// beta_k = rz_new/rz recorded in hist[2k+1]; p = z + beta p.
+template <typename T>
+__global__ __launch_bounds__(256) void eig_alpha_kernel(const T *p, const T *q, long n, double *partials,
+                                                        unsigned *counter, double *sc, double *hist, int k) {
+  double v[1] = {0.0};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) v[0] += (double)p[i] * q[i];
+  if (reduce_to_last_block<1>(v, partials, counter) && threadIdx.x == 0) {
+    sc[1] = sc[0] / v[0];
+    hist[2 * k] = sc[1];
+  }
+}
+template <typename T>
+__global__ __launch_bounds__(256) void eig_update_kernel(T *r, T *z, const T *q, const T *inv, long n,
+                                                         double *partials, unsigned *counter, double *sc,
+                                                         double *hist, int k) {
+  const T alpha = (T)sc[1];
+  double v[1] = {0.0};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const T ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    const T zi = inv[i] * ri;
+    z[i] = zi;
+    v[0] += (double)ri * zi;
+  }
+  if (reduce_to_last_block<1>(v, partials, counter) && threadIdx.x == 0) {
+    hist[2 * k + 1] = v[0] / sc[0];
+    sc[2] = v[0] / sc[0];
+    sc[0] = v[0];
+  }
+}
+template <typename T>
+__global__ __launch_bounds__(256) void eig_dir_kernel(T *p, const T *z, long n, const double *sc) {
+  const T beta = (T)sc[2];
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = z[i] + beta * p[i];
+}
+template <typename T>
+__global__ __launch_bounds__(256) void eig_init_kernel(T *r, T *z, T *p, const T *inv, long n, double *partials,
+                                                       unsigned *counter, double *sc) {
+  double v[1] = {0.0};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const T zi = inv[i] * r[i];
+    z[i] = zi;
+    p[i] = zi;
+    v[0] += (double)r[i] * zi;
+  }
+  if (reduce_to_last_block<1>(v, partials, counter) && threadIdx.x == 0) sc[0] = v[0];
+}
+template <typename T>
+int launch_eig_step(int phase, T *r, T *z, T *p, const T *q, const T *inv, long n, const Reduction &red, double *sc,
+                    double *hist, int k, hipStream_t s) {
+  const dim3 g(red_blocks(n)), b(256);
+  if (phase == 0) hipLaunchKernelGGL(eig_init_kernel<T>, g, b, 0, s, r, z, p, inv, n, red.partials, red.counter, sc);
+  if (phase == 1) hipLaunchKernelGGL(eig_alpha_kernel<T>, g, b, 0, s, p, q, n, red.partials, red.counter, sc, hist, k);
+  if (phase == 2)
+    hipLaunchKernelGGL(eig_update_kernel<T>, g, b, 0, s, r, z, q, inv, n, red.partials, red.counter, sc, hist, k);
+  if (phase == 3) hipLaunchKernelGGL(eig_dir_kernel<T>, dim3(std::min<long>((n + 255) / 256, 8192)), b, 0, s, p, z, n, sc);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+template int launch_eig_step<float>(int, float *, float *, float *, const float *, const float *, long,
+                                    const Reduction &, double *, double *, int, hipStream_t);
+template int launch_eig_step<double>(int, double *, double *, double *, const double *, const double *, long,
+                                     const Reduction &, double *, double *, int, hipStream_t);
+
+}  // namespace mpdg

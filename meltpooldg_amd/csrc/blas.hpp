@@ -1,0 +1,56 @@
+// Vector kernels, deterministic reductions and the device-side CG state.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace mpdg {
+
+constexpr int kRedBlocks = 1024;
+
+// scratch of the two-stage reductions
+struct Reduction {
+  double *partials = nullptr;  // 4 * kRedBlocks doubles
+  unsigned *counter = nullptr; // zero-initialised ticket
+};
+
+// Krylov scalars kept on the device between kernels
+struct CGState {
+  double rz, pq, alpha, beta, rr, rr0, bb, tol2, rzold_new;
+  int done;  // 0 running, 1 converged or out of iterations, 2 non-finite
+  int iters;
+  int max_it;
+  int flexible;
+};
+
+template <typename T> int launch_fill(T *x, T v, long n, hipStream_t s);
+template <typename TA, typename TB> int launch_convert(const TA *a, TB *b, long n, hipStream_t s);
+int launch_axpby(double *y, double a, const double *x, double b, long n, hipStream_t s);
+template <typename T> int launch_invert_diagonal(T *d, long n, hipStream_t s);
+template <typename TA, typename TB>
+int launch_dot(const TA *a, const TB *b, long n, const Reduction &r, double *out, hipStream_t s);
+
+int cg_init(const double *b, double *r, const double *q, double *z, double *p, const double *inv, bool jacobi,
+            long n, const Reduction &red, CGState *st, double rel_tol, double abs_tol, int criterion, hipStream_t s);
+int cg_after_external_precond_init(const double *r, const double *z, double *p, long n, const Reduction &red,
+                                   CGState *st, hipStream_t s);
+int cg_alpha(const double *p, const double *q, long n, const Reduction &red, CGState *st, hipStream_t s);
+int cg_update(double *x, double *r, const double *p, const double *q, double *z, const double *inv, bool jacobi,
+              long n, const Reduction &red, CGState *st, hipStream_t s);
+int cg_beta(const double *r, const double *z, long n, const Reduction &red, CGState *st, hipStream_t s);
+int cg_direction(double *p, const double *z, long n, const CGState *st, hipStream_t s);
+
+template <typename T>
+int launch_chebyshev_step(T *x, T *d, const T *b, const T *Ax, const T *inv, long n, double c1, double c2,
+                          bool first, hipStream_t s);
+template <typename T> int launch_residual(T *r, const T *b, const T *Ax, long n, hipStream_t s);
+template <typename T> int launch_add(T *x, const T *y, long n, hipStream_t s);
+template <typename T>
+int launch_ell_dir(const T *in, T *out, const int *odims, const int *idims, int dir, int width, const int *col,
+                   const T *val, hipStream_t s);
+int launch_dense_gemv(const double *A, const double *x, double *y, int n, hipStream_t s);
+template <typename T> int launch_eig_start(T *v, const int *nd, int mask, hipStream_t s);
+template <typename T>
+int launch_eig_step(int phase, T *r, T *z, T *p, const T *q, const T *inv, long n, const Reduction &red, double *sc,
+                    double *hist, int k, hipStream_t s);
+
+}  // namespace mpdg

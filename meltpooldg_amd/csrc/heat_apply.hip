@@ -1,0 +1,401 @@
+// Matrix-free apply of the implicit heat operator J = c_m M + K_kappa on a
+// structured Q_k hex mesh, for CDNA4 (gfx950).
+//
+// Replaces HeatDiffuseMultiPhaseOperator::tangent_cell_loop /
+// tangent_local_cell_operation (source/heat/heat_diffuse_operator.cpp:201-241,
+// :957-1086) run through deal.II's MatrixFree::loop + FEEvaluation.
+//
+// Work decomposition
+//   * A workgroup (256 threads, 4 wavefronts) owns a column of BY x BZ cells in
+//     y-z and walks it along x in chunks of BX cells. Each cell is handled by
+//     N1 = k+1 consecutive lanes of one wavefront; lane l of a cell holds the
+//     z-plane l of the cell's DoFs (N1 x N1 values in registers).
+//   * Sum factorisation: the x and y sweeps are register-local in this "plane"
+//     layout; the z sweeps run after a wavefront-local transpose through LDS
+//     into the "y-plane" layout (lane s holds qy = s). Gradients use the
+//     collocation derivative on the Gauss points (n_q = k+1).
+//   * Scatter: cell results are summed into an LDS copy of the chunk's DoF
+//     brick (ds_add). The brick is then written once: DoFs interior to the
+//     column are stored, DoFs on the column's y/z faces (shared with the
+//     neighbouring columns) are added with global atomics, and the x-face
+//     between consecutive chunks stays in LDS and is carried to the next chunk,
+//     so no DoF row is split across x. A pre-pass zeroes the shared face rows.
+#include "kernels.hpp"
+
+namespace mpdg {
+
+template <int N1> struct ChunkShape;
+template <> struct ChunkShape<2> { static constexpr int BX = 8, BY = 4, BZ = 4; };
+template <> struct ChunkShape<3> { static constexpr int BX = 7, BY = 4, BZ = 3; };
+template <> struct ChunkShape<4> { static constexpr int BX = 4, BY = 4, BZ = 4; };
+template <> struct ChunkShape<5> { static constexpr int BX = 4, BY = 4, BZ = 3; };
+
+template <int N1> struct ApplyGeom {
+  static constexpr int K = N1 - 1;
+  static constexpr int CPW = 64 / N1;                 // cells per wavefront
+  static constexpr int CPB = 4 * CPW;                 // cells per chunk
+  static constexpr int BX = ChunkShape<N1>::BX, BY = ChunkShape<N1>::BY, BZ = ChunkShape<N1>::BZ;
+  static_assert(BX * BY * BZ == CPB, "chunk shape must match the cells per workgroup");
+  static constexpr int SQY = N1 * N1 + 1;             // padded qy stride of a cell's transpose slot
+  static constexpr int CS = N1 * SQY;                 // transpose slot per cell
+  static constexpr int AX = K * BX + 1, AY = K * BY + 1, AZ = K * BZ + 1;
+  static constexpr int ACC = AX * AY * AZ;            // DoF brick of a chunk
+  static constexpr int TBUF = 4 * CPW * CS;           // transpose buffers of the 4 wavefronts
+  static constexpr int LDS = TBUF > ACC ? TBUF : ACC;
+  static constexpr int CARRY = AY * AZ;
+  static constexpr int WPE = N1 <= 3 ? 4 : (N1 == 4 ? 3 : 2);  // wavefronts per SIMD the register budget targets
+};
+
+// ---- register-level 1D contractions on an N1 x N1 array a[row][col] ----
+// along columns (second index): out[r][q] = sum_i M[q][i] in[r][i]
+template <int N1, bool TR, typename Number>
+__device__ __forceinline__ void sweep_col(const Number (&M)[N1][N1], const Number (&in)[N1][N1],
+                                          Number (&out)[N1][N1]) {
+#pragma unroll
+  for (int r = 0; r < N1; ++r)
+#pragma unroll
+    for (int q = 0; q < N1; ++q) {
+      Number acc = 0;
+#pragma unroll
+      for (int i = 0; i < N1; ++i) acc = fma(TR ? M[i][q] : M[q][i], in[r][i], acc);
+      out[r][q] = acc;
+    }
+}
+// along rows (first index): out[q][c] = sum_i M[q][i] in[i][c]
+template <int N1, bool TR, typename Number>
+__device__ __forceinline__ void sweep_row(const Number (&M)[N1][N1], const Number (&in)[N1][N1],
+                                          Number (&out)[N1][N1]) {
+#pragma unroll
+  for (int q = 0; q < N1; ++q)
+#pragma unroll
+    for (int c = 0; c < N1; ++c) {
+      Number acc = 0;
+#pragma unroll
+      for (int i = 0; i < N1; ++i) acc = fma(TR ? M[i][q] : M[q][i], in[i][c], acc);
+      out[q][c] = acc;
+    }
+}
+template <int N1, bool TR, typename Number>
+__device__ __forceinline__ void sweep_row_add(const Number (&M)[N1][N1], const Number (&in)[N1][N1],
+                                              Number (&out)[N1][N1]) {
+#pragma unroll
+  for (int q = 0; q < N1; ++q)
+#pragma unroll
+    for (int c = 0; c < N1; ++c) {
+      Number acc = out[q][c];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) acc = fma(TR ? M[i][q] : M[q][i], in[i][c], acc);
+      out[q][c] = acc;
+    }
+}
+template <int N1, bool TR, typename Number>
+__device__ __forceinline__ void sweep_col_add(const Number (&M)[N1][N1], const Number (&in)[N1][N1],
+                                              Number (&out)[N1][N1]) {
+#pragma unroll
+  for (int r = 0; r < N1; ++r)
+#pragma unroll
+    for (int q = 0; q < N1; ++q) {
+      Number acc = out[r][q];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) acc = fma(TR ? M[i][q] : M[q][i], in[r][i], acc);
+      out[r][q] = acc;
+    }
+}
+
+// Wavefront-local LDS ordering: all lanes' LDS writes are complete and visible
+// to the other lanes of the same wavefront.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// plane layout (lane = z-plane l, a[qy][qx]) -> y-plane layout (lane = qy s, b[qz][qx])
+template <int N1, typename Number>
+__device__ __forceinline__ void transpose_zy(Number *slot, int l, bool on, const Number (&a)[N1][N1],
+                                             Number (&b)[N1][N1]) {
+  using G = ApplyGeom<N1>;
+  wave_lds_sync();
+  if (on) {
+#pragma unroll
+    for (int qy = 0; qy < N1; ++qy)
+#pragma unroll
+      for (int qx = 0; qx < N1; ++qx) slot[l + N1 * qx + G::SQY * qy] = a[qy][qx];
+  }
+  wave_lds_sync();
+  if (on) {
+#pragma unroll
+    for (int qz = 0; qz < N1; ++qz)
+#pragma unroll
+      for (int qx = 0; qx < N1; ++qx) b[qz][qx] = slot[qz + N1 * qx + G::SQY * l];
+  }
+}
+// y-plane layout (lane = qy s, b[iz][qx]) -> plane layout (lane = z-plane l, a[qy][qx])
+template <int N1, typename Number>
+__device__ __forceinline__ void transpose_yz(Number *slot, int s, bool on, const Number (&b)[N1][N1],
+                                             Number (&a)[N1][N1]) {
+  using G = ApplyGeom<N1>;
+  wave_lds_sync();
+  if (on) {
+#pragma unroll
+    for (int iz = 0; iz < N1; ++iz)
+#pragma unroll
+      for (int qx = 0; qx < N1; ++qx) slot[iz + N1 * qx + G::SQY * s] = b[iz][qx];
+  }
+  wave_lds_sync();
+  if (on) {
+#pragma unroll
+    for (int qy = 0; qy < N1; ++qy)
+#pragma unroll
+      for (int qx = 0; qx < N1; ++qx) a[qy][qx] = slot[s + N1 * qx + G::SQY * qy];
+  }
+}
+
+template <typename Number>
+__device__ __forceinline__ Number law_kappa(int law, const Number *p, Number T) {
+  if (law == LAW_TANH) {
+    return p[0] * (Number(1) + p[1] * tanh((T - p[2]) / p[3]));
+  }
+  // mushy zone (liquid_solid): p = {k_liquid, k_solid, T_solidus, T_liquidus}
+  const Number inv = Number(1) / (p[3] - p[2]);
+  Number fs = (p[3] - T) * inv;
+  fs = fs < Number(0) ? Number(0) : (fs > Number(1) ? Number(1) : fs);
+  return p[0] + (p[1] - p[0]) * (Number(3) - Number(2) * fs) * fs * fs;
+}
+
+template <int N1, typename Number, int KMODE>
+__global__ __launch_bounds__(256) void heat_apply_kernel(const ApplyArgs<Number, N1> P,
+                                                         const Number *__restrict__ src,
+                                                         Number *__restrict__ dst) {
+  using G = ApplyGeom<N1>;
+  constexpr int K = G::K;
+  __shared__ Number lds[G::LDS];
+  __shared__ Number carry[G::CARRY];
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int cw = lane / N1, l = lane - cw * N1;
+  const bool lane_on = cw < G::CPW;
+  const int lc = wave * G::CPW + cw;
+  const int lx = lc % G::BX, ly = (lc / G::BX) % G::BY, lz = lc / (G::BX * G::BY);
+  Number *slot = lds + (wave * G::CPW + (lane_on ? cw : 0)) * G::CS;
+
+  // column of this workgroup (consecutive XCDs get consecutive column ranges)
+  int col = blockIdx.x;
+  if (P.xcd_cols > 0) col = (blockIdx.x & 7) * P.xcd_cols + (blockIdx.x >> 3);
+  const int col_y = col % P.ncol[0], col_z = col / P.ncol[0];
+  const int cy = col_y * G::BY + ly, cz = col_z * G::BZ + lz;
+  const int Y0 = K * G::BY * col_y, Z0 = K * G::BZ * col_z;
+  const int DY = K * min(G::BY, P.nc[1] - col_y * G::BY) + 1;
+  const int DZ = K * min(G::BZ, P.nc[2] - col_z * G::BZ) + 1;
+  const long sy = P.nd[0], sz = (long)P.nd[0] * P.nd[1];
+  const int nchunks = (P.nc[0] + G::BX - 1) / G::BX;
+
+  for (int chunk = 0; chunk < nchunks; ++chunk) {
+    const int cx = chunk * G::BX + lx;
+    const bool valid = lane_on && cx < P.nc[0] && cy < P.nc[1] && cz < P.nc[2];
+    const long cell = (long)cx + (long)P.nc[0] * (cy + (long)P.nc[1] * cz);
+
+    // ---- gather the z-plane l of the cell (read_dof_values with homogeneous DBC) ----
+    Number u[N1][N1];
+    const long base = (long)K * cx + sy * (K * cy) + sz * (K * cz + l);
+#pragma unroll
+    for (int j = 0; j < N1; ++j)
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        Number v = 0;
+        if (valid) {
+          v = src[base + i + sy * j];
+          if (P.dmask) {
+            const int ix = K * cx + i, iy = K * cy + j, iz = K * cz + l;
+            if (((P.dmask & 1) && ix == 0) || ((P.dmask & 2) && ix == P.nd[0] - 1) ||
+                ((P.dmask & 4) && iy == 0) || ((P.dmask & 8) && iy == P.nd[1] - 1) ||
+                ((P.dmask & 16) && iz == 0) || ((P.dmask & 32) && iz == P.nd[2] - 1))
+              v = 0;
+          }
+        }
+        u[j][i] = v;
+      }
+
+    // ---- conductivity law: T at the quadrature points in the y-plane layout ----
+    Number kq[N1][N1];
+    if (KMODE == KAPPA_LAW) {
+      Number T0[N1][N1], w1[N1][N1], t1[N1][N1];
+#pragma unroll
+      for (int j = 0; j < N1; ++j)
+#pragma unroll
+        for (int i = 0; i < N1; ++i) T0[j][i] = valid ? P.T[base + i + sy * j] : Number(0);
+      sweep_col<N1, false>(P.S, T0, w1);
+      sweep_row<N1, false>(P.S, w1, t1);
+      transpose_zy<N1>(slot, l, lane_on, t1, w1);
+      sweep_row<N1, false>(P.S, w1, T0);
+#pragma unroll
+      for (int qz = 0; qz < N1; ++qz)
+#pragma unroll
+        for (int qx = 0; qx < N1; ++qx) kq[qz][qx] = law_kappa<Number>(P.law, P.law_p, T0[qz][qx]);
+    }
+
+    // ---- forward, plane layout: t = Sy Sx u, b = Dy t ----
+    Number t[N1][N1], b[N1][N1];
+    {
+      Number w1[N1][N1];
+      sweep_col<N1, false>(P.S, u, w1);
+      sweep_row<N1, false>(P.S, w1, t);
+      sweep_row<N1, false>(P.D, t, b);
+    }
+    // ---- y-plane layout (lane = qy): v = Sz t, gy = Sz b ----
+    Number v[N1][N1], gy[N1][N1];
+    transpose_zy<N1>(slot, l, lane_on, t, u);
+    sweep_row<N1, false>(P.S, u, v);
+    transpose_zy<N1>(slot, l, lane_on, b, u);
+    sweep_row<N1, false>(P.S, u, gy);
+
+    // ---- quadrature point operation: val = c_m u, flux = kappa grad u, integrated
+    //      along z and x right away: e = c W v + Dz^T(kW gz) + Dx^T(kW gx) ----
+    const Number wy = P.w[l];
+    Number kc = P.kappa;
+    if (KMODE == KAPPA_CELL) kc = valid ? P.kappa_arr[cell] : Number(0);
+    Number kw[N1][N1], e[N1][N1];
+#pragma unroll
+    for (int qz = 0; qz < N1; ++qz)
+#pragma unroll
+      for (int qx = 0; qx < N1; ++qx) {
+        const Number W = P.w[qx] * wy * P.w[qz];
+        Number kap = kc;
+        if (KMODE == KAPPA_QP) kap = valid ? P.kappa_arr[cell * (N1 * N1 * N1) + qx + N1 * (l + N1 * qz)] : Number(0);
+        if (KMODE == KAPPA_LAW) kap = kq[qz][qx];
+        kw[qz][qx] = kap * W;
+        e[qz][qx] = P.cm * W * v[qz][qx];
+        gy[qz][qx] *= kw[qz][qx] * P.ks[1];
+      }
+    {
+      Number g[N1][N1];
+      sweep_row<N1, false>(P.D, v, g);  // gz
+#pragma unroll
+      for (int qz = 0; qz < N1; ++qz)
+#pragma unroll
+        for (int qx = 0; qx < N1; ++qx) g[qz][qx] *= kw[qz][qx] * P.ks[2];
+      sweep_row_add<N1, true>(P.D, g, e);
+      sweep_col<N1, false>(P.D, v, g);  // gx
+#pragma unroll
+      for (int qz = 0; qz < N1; ++qz)
+#pragma unroll
+        for (int qx = 0; qx < N1; ++qx) g[qz][qx] *= kw[qz][qx] * P.ks[0];
+      sweep_col_add<N1, true>(P.D, g, e);
+    }
+    // ---- back to the plane layout: e' = Sz^T e, ey = Sz^T (kW gy) ----
+    sweep_row<N1, true>(P.S, e, v);
+    transpose_yz<N1>(slot, l, lane_on, v, t);
+    sweep_row<N1, true>(P.S, gy, v);
+    transpose_yz<N1>(slot, l, lane_on, v, b);
+    // ---- plane layout: out = Sx^T Sy^T (e' + Dy^T ey) ----
+    sweep_row_add<N1, true>(P.D, b, t);
+    sweep_row<N1, true>(P.S, t, b);
+    sweep_col<N1, true>(P.S, b, u);
+
+    // ---- scatter: sum the chunk's cells in the LDS DoF brick ----
+    __syncthreads();
+    const int X0 = K * G::BX * chunk;
+    const int DX = K * min(G::BX, P.nc[0] - chunk * G::BX) + 1;
+    for (int idx = tid; idx < G::ACC; idx += 256) {
+      const int x = idx % G::AX, yz = idx / G::AX;
+      lds[idx] = (x == 0 && chunk > 0) ? carry[yz] : Number(0);
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int j = 0; j < N1; ++j)
+#pragma unroll
+        for (int i = 0; i < N1; ++i)
+          atomicAdd(&lds[(K * lx + i) + G::AX * ((K * ly + j) + G::AY * (K * lz + l))], u[j][i]);
+    }
+    __syncthreads();
+    const bool last = chunk == nchunks - 1;
+    for (int idx = tid; idx < G::ACC; idx += 256) {
+      const int x = idx % G::AX, y = (idx / G::AX) % G::AY, z = idx / (G::AX * G::AY);
+      if (x >= DX || y >= DY || z >= DZ) continue;
+      const Number val = lds[idx];
+      if (x == DX - 1 && !last) {
+        carry[y + G::AY * z] = val;
+        continue;
+      }
+      const int ix = X0 + x, iy = Y0 + y, iz = Z0 + z;
+      const long g = ix + sy * iy + sz * iz;
+      const bool face = (y == 0 || y == DY - 1 || z == 0 || z == DZ - 1);
+      bool con = false;
+      if (P.dmask)
+        con = ((P.dmask & 1) && ix == 0) || ((P.dmask & 2) && ix == P.nd[0] - 1) ||
+              ((P.dmask & 4) && iy == 0) || ((P.dmask & 8) && iy == P.nd[1] - 1) ||
+              ((P.dmask & 16) && iz == 0) || ((P.dmask & 32) && iz == P.nd[2] - 1);
+      if (face) {
+        if (!con) atomicAdd(dst + g, val);
+      } else {
+        dst[g] = con ? Number(0) : val;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Zero the DoF rows shared between neighbouring columns (y or z on a column face).
+template <typename Number>
+__global__ __launch_bounds__(256) void zero_face_rows_kernel(Number *__restrict__ dst, int nd0, int nd1,
+                                                             int nd2, int py, int pz) {
+  const long nrows = (long)nd1 * nd2;
+  for (long r = blockIdx.y; r < nrows; r += gridDim.y) {
+    const int iy = r % nd1, iz = r / nd1;
+    const bool face = (iy % py == 0) || (iy == nd1 - 1) || (iz % pz == 0) || (iz == nd2 - 1);
+    if (!face) continue;
+    for (int ix = blockIdx.x * 256 + threadIdx.x; ix < nd0; ix += gridDim.x * 256) dst[ix + r * nd0] = 0;
+  }
+}
+
+template <int N1, typename Number>
+int launch_apply_n1(const OperatorData<Number> &op, const Number *src, Number *dst, hipStream_t stream) {
+  using G = ApplyGeom<N1>;
+  ApplyArgs<Number, N1> A;
+  fill_apply_args(op, A);
+  A.ncol[0] = (op.nc[1] + G::BY - 1) / G::BY;
+  A.ncol[1] = (op.nc[2] + G::BZ - 1) / G::BZ;
+  const int ncols = A.ncol[0] * A.ncol[1];
+  A.xcd_cols = (ncols % 8 == 0 && ncols >= 64) ? ncols / 8 : 0;
+  {
+    const long nrows = (long)op.nd[1] * op.nd[2];
+    dim3 grid((op.nd[0] + 255) / 256, (unsigned)std::min<long>(nrows, 65535));
+    hipLaunchKernelGGL(zero_face_rows_kernel<Number>, grid, dim3(256), 0, stream, dst, op.nd[0],
+                       op.nd[1], op.nd[2], (N1 - 1) * G::BY, (N1 - 1) * G::BZ);
+  }
+  switch (op.kappa_mode) {
+    case KAPPA_CONST:
+      hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CONST>), dim3(ncols), dim3(256), 0, stream, A, src, dst);
+      break;
+    case KAPPA_QP:
+      hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_QP>), dim3(ncols), dim3(256), 0, stream, A, src, dst);
+      break;
+    case KAPPA_CELL:
+      hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CELL>), dim3(ncols), dim3(256), 0, stream, A, src, dst);
+      break;
+    case KAPPA_LAW:
+      hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_LAW>), dim3(ncols), dim3(256), 0, stream, A, src, dst);
+      break;
+    default:
+      return MPDG_ERR_INVALID;
+  }
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+template <typename Number>
+int launch_apply(const OperatorData<Number> &op, const Number *src, Number *dst, hipStream_t stream) {
+  switch (op.degree) {
+    case 1: return launch_apply_n1<2, Number>(op, src, dst, stream);
+    case 2: return launch_apply_n1<3, Number>(op, src, dst, stream);
+    case 3: return launch_apply_n1<4, Number>(op, src, dst, stream);
+    case 4: return launch_apply_n1<5, Number>(op, src, dst, stream);
+    default: return MPDG_ERR_INVALID;
+  }
+}
+
+template int launch_apply<double>(const OperatorData<double> &, const double *, double *, hipStream_t);
+template int launch_apply<float>(const OperatorData<float> &, const float *, float *, hipStream_t);
+
+}  // namespace mpdg

@@ -1,0 +1,312 @@
+// Auxiliary kernels of the heat operator: diagonal, conductivity at quadrature
+// points, cell averages for the multigrid levels, and the boundary-face terms.
+//
+// These run once per time step or once per Newton iteration, not once per
+// Krylov iteration, so they favour simple indexing over tiling.
+#include "kernels.hpp"
+
+namespace mpdg {
+
+constexpr double kStefanBoltzmann = 5.670374419e-8;  // include/meltpooldg/utilities/physical_constants.hpp:6
+
+template <typename Number>
+struct AuxTables {
+  Number S[kMaxN1][kMaxN1], G[kMaxN1][kMaxN1], w[kMaxN1];
+};
+template <typename Number>
+AuxTables<Number> make_aux_tables(int k) {
+  FE1D fe(k);
+  AuxTables<Number> t{};
+  for (int a = 0; a < fe.n1; ++a) {
+    t.w[a] = (Number)fe.qw[a];
+    for (int b = 0; b < fe.n1; ++b) {
+      t.S[a][b] = (Number)fe.S[a][b];
+      t.G[a][b] = (Number)fe.G[a][b];
+    }
+  }
+  return t;
+}
+
+template <typename Number>
+__device__ __forceinline__ Number law_value(int law, const Number *p, Number T) {
+  if (law == LAW_TANH) return p[0] * (Number(1) + p[1] * tanh((T - p[2]) / p[3]));
+  const Number inv = Number(1) / (p[3] - p[2]);
+  Number fs = (p[3] - T) * inv;
+  fs = fs < Number(0) ? Number(0) : (fs > Number(1) ? Number(1) : fs);
+  return p[0] + (p[1] - p[0]) * (Number(3) - Number(2) * fs) * fs * fs;
+}
+
+// conductivity at every quadrature point of every cell (for KAPPA_LAW: k(T_q))
+template <typename Number>
+__global__ __launch_bounds__(256) void kappa_qp_kernel(AuxTables<Number> t, int n1, int nc0, int nc1,
+                                                       long ncells, int nd0, int nd1, int mode, Number kappa,
+                                                       const Number *__restrict__ karr, int law, Number p0,
+                                                       Number p1, Number p2, Number p3,
+                                                       const Number *__restrict__ T, Number *__restrict__ kq) {
+  const int nq = n1 * n1 * n1;
+  const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+  if (gid >= ncells * nq) return;
+  const long cell = gid / nq;
+  const int q = gid - cell * nq;
+  const int qx = q % n1, qy = (q / n1) % n1, qz = q / (n1 * n1);
+  Number k = kappa;
+  if (mode == KAPPA_QP) k = karr[gid];
+  else if (mode == KAPPA_CELL) k = karr[cell];
+  else if (mode == KAPPA_LAW) {
+    const int cx = cell % nc0, cy = (cell / nc0) % nc1, cz = cell / ((long)nc0 * nc1);
+    const int K = n1 - 1;
+    Number Tq = 0;
+    for (int l = 0; l < n1; ++l)
+      for (int j = 0; j < n1; ++j) {
+        Number r = 0;
+        const long row = (long)K * cx + (long)nd0 * ((K * cy + j) + (long)nd1 * (K * cz + l));
+        for (int i = 0; i < n1; ++i) r += t.S[qx][i] * T[row + i];
+        Tq += t.S[qy][j] * t.S[qz][l] * r;
+      }
+    const Number p[4] = {p0, p1, p2, p3};
+    k = law_value<Number>(law, p, Tq);
+  }
+  kq[gid] = k;
+}
+
+template <typename Number>
+int launch_kappa_qp(const OperatorData<Number> &op, Number *kq, hipStream_t stream) {
+  const int n1 = op.degree + 1;
+  const long n = op.n_cells * n1 * n1 * n1;
+  hipLaunchKernelGGL(kappa_qp_kernel<Number>, dim3((n + 255) / 256), dim3(256), 0, stream,
+                     make_aux_tables<Number>(op.degree), n1, op.nc[0], op.nc[1], op.n_cells, op.nd[0],
+                     op.nd[1], op.kappa_mode, (Number)op.kappa, op.kappa_arr, op.law, (Number)op.law_p[0],
+                     (Number)op.law_p[1], (Number)op.law_p[2], (Number)op.law_p[3], op.T, kq);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+// diag_I += sum_q W_q (c_m vol phi_I^2 + k_q sum_d vol/h_d^2 (d_d phi_I)^2)
+template <typename Number>
+__global__ __launch_bounds__(256) void diagonal_kernel(AuxTables<Number> t, int n1, int nc0, int nc1,
+                                                       long ncells, int nd0, int nd1, Number cm, Number ks0,
+                                                       Number ks1, Number ks2, const Number *__restrict__ kq,
+                                                       Number *__restrict__ diag) {
+  const int nloc = n1 * n1 * n1;
+  const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+  if (gid >= ncells * nloc) return;
+  const long cell = gid / nloc;
+  const int a = gid - cell * nloc;
+  const int i = a % n1, j = (a / n1) % n1, l = a / (n1 * n1);
+  const int cx = cell % nc0, cy = (cell / nc0) % nc1, cz = cell / ((long)nc0 * nc1);
+  const int K = n1 - 1;
+  Number acc = 0;
+  for (int qz = 0; qz < n1; ++qz)
+    for (int qy = 0; qy < n1; ++qy)
+      for (int qx = 0; qx < n1; ++qx) {
+        const Number W = t.w[qx] * t.w[qy] * t.w[qz];
+        const Number sx = t.S[qx][i], sy = t.S[qy][j], sz = t.S[qz][l];
+        const Number gx = t.G[qx][i], gy = t.G[qy][j], gz = t.G[qz][l];
+        const Number k = kq[cell * nloc + qx + n1 * (qy + n1 * qz)];
+        const Number m = sx * sy * sz;
+        acc += W * (cm * m * m + k * (ks0 * gx * gx * sy * sy * sz * sz + ks1 * sx * sx * gy * gy * sz * sz +
+                                      ks2 * sx * sx * sy * sy * gz * gz));
+      }
+  const long g = (long)(K * cx + i) + (long)nd0 * ((K * cy + j) + (long)nd1 * (K * cz + l));
+  atomicAdd(diag + g, acc);
+}
+
+template <typename Number>
+__global__ __launch_bounds__(256) void diag_dirichlet_kernel(Number *__restrict__ d, int nd0, int nd1, int nd2,
+                                                             int mask) {
+  const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+  if (gid >= (long)nd0 * nd1 * nd2) return;
+  const int ix = gid % nd0, iy = (gid / nd0) % nd1, iz = gid / ((long)nd0 * nd1);
+  const bool con = ((mask & 1) && ix == 0) || ((mask & 2) && ix == nd0 - 1) || ((mask & 4) && iy == 0) ||
+                   ((mask & 8) && iy == nd1 - 1) || ((mask & 16) && iz == 0) || ((mask & 32) && iz == nd2 - 1);
+  if (con) d[gid] = Number(1);
+}
+
+// Boundary faces, thread per (boundary cell face, face DoF I).
+// mode 0: Robin tangent applied to src: dst_I += sum_q phi_I (alpha + 4 eps sigma T_q^3) u_q JxW_q
+//         (heat_diffuse_operator.cpp:1090-1129)
+// mode 1: its diagonal
+// mode 2: boundary residual -alpha (T - T_inf) - eps sigma (T^4 - T_inf^4) on Robin faces plus the
+//         Gaussian surface flux on z+ (heat_diffuse_operator.cpp:721-781)
+struct FaceArgs {
+  AuxTables<double> t;
+  double qp[kMaxN1];
+  int n1, nc[3], nd[3], robin_mask, face_mask, dmask, mode, flux_on;
+  double h[3], alpha, emissivity, T_inf_conv, T_inf_rad;
+  double flux[6];  // peak, sigma, x0, y0, vx, t
+};
+
+__global__ __launch_bounds__(256) void boundary_face_kernel(FaceArgs A, const double *__restrict__ src,
+                                                            const double *__restrict__ T, double *__restrict__ dst) {
+  const int n1 = A.n1, K = n1 - 1, nf = n1 * n1;
+  long gid = (long)blockIdx.x * 256 + threadIdx.x;
+  for (int f = 0; f < 6; ++f) {
+    if (!((A.face_mask >> f) & 1)) continue;
+    const int d = f / 2, s = f % 2;
+    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    const long count = (long)A.nc[ta] * A.nc[tb] * nf;
+    if (gid >= count) { gid -= count; continue; }
+    const long fc = gid / nf;
+    const int a = gid - fc * nf;
+    const int ia = a % n1, ib = a / n1;
+    int c[3];
+    c[d] = s ? A.nc[d] - 1 : 0;
+    c[ta] = fc % A.nc[ta];
+    c[tb] = fc / A.nc[ta];
+    auto dof = [&](int xa, int xb) {
+      int idx[3];
+      idx[d] = K * c[d] + s * K;
+      idx[ta] = K * c[ta] + xa;
+      idx[tb] = K * c[tb] + xb;
+      return (long)idx[0] + (long)A.nd[0] * (idx[1] + (long)A.nd[1] * idx[2]);
+    };
+    auto constrained = [&](long g) {
+      const int ix = g % A.nd[0], iy = (g / A.nd[0]) % A.nd[1], iz = g / ((long)A.nd[0] * A.nd[1]);
+      const int m = A.dmask;
+      return ((m & 1) && ix == 0) || ((m & 2) && ix == A.nd[0] - 1) || ((m & 4) && iy == 0) ||
+             ((m & 8) && iy == A.nd[1] - 1) || ((m & 16) && iz == 0) || ((m & 32) && iz == A.nd[2] - 1);
+    };
+    const long gI = dof(ia, ib);
+    if (constrained(gI)) return;
+    const bool robin = (A.robin_mask >> f) & 1;
+    double acc = 0.0;
+    for (int qb = 0; qb < n1; ++qb)
+      for (int qa = 0; qa < n1; ++qa) {
+        double Tq = 0.0, uq = 0.0;
+        for (int xb = 0; xb < n1; ++xb)
+          for (int xa = 0; xa < n1; ++xa) {
+            const double phi = A.t.S[qa][xa] * A.t.S[qb][xb];
+            const long g = dof(xa, xb);
+            if (T) Tq += phi * T[g];
+            if (A.mode == 0 && !constrained(g)) uq += phi * src[g];
+          }
+        const double JxW = A.h[ta] * A.h[tb] * A.t.w[qa] * A.t.w[qb];
+        const double phiI = A.t.S[qa][ia] * A.t.S[qb][ib];
+        double val = 0.0;
+        if (A.mode == 2) {
+          if (robin && A.emissivity > 0) {
+            const double T4 = Tq * Tq * Tq * Tq, Ti = A.T_inf_rad;
+            val -= A.emissivity * kStefanBoltzmann * (T4 - Ti * Ti * Ti * Ti);
+          }
+          if (robin && A.alpha > 0) val -= A.alpha * (Tq - A.T_inf_conv);
+          if (A.flux_on && f == 5) {
+            double pos[3];
+            pos[d] = (c[d] + s) * A.h[d];
+            pos[ta] = (c[ta] + A.qp[qa]) * A.h[ta];
+            pos[tb] = (c[tb] + A.qp[qb]) * A.h[tb];
+            const double dx = pos[0] - A.flux[2] - A.flux[4] * A.flux[5], dy = pos[1] - A.flux[3];
+            val += A.flux[0] * exp(-(dx * dx + dy * dy) / (2.0 * A.flux[1] * A.flux[1]));
+          }
+        } else {
+          double coef = 0.0;
+          if (A.alpha > 0) coef += A.alpha;
+          if (A.emissivity > 0) coef += 4.0 * A.emissivity * kStefanBoltzmann * Tq * Tq * Tq;
+          val = coef * (A.mode == 0 ? uq : phiI);
+        }
+        acc += phiI * val * JxW;
+      }
+    atomicAdd(dst + gI, acc);
+    return;
+  }
+}
+
+int launch_boundary(const OperatorData<double> &op, int mode, const double *flux, double *dst,
+                    const double *src, const double *T, hipStream_t stream) {
+  FaceArgs A{};
+  A.t = make_aux_tables<double>(op.degree);
+  FE1D fe(op.degree);
+  for (int i = 0; i < fe.n1; ++i) A.qp[i] = fe.qp[i];
+  A.n1 = op.degree + 1;
+  for (int d = 0; d < 3; ++d) {
+    A.nc[d] = op.nc[d];
+    A.nd[d] = op.nd[d];
+    A.h[d] = op.h[d];
+  }
+  A.robin_mask = op.robin_mask;
+  A.flux_on = (mode == 2 && flux && flux[0] != 0.0) ? 1 : 0;
+  A.face_mask = op.robin_mask | (A.flux_on ? 32 : 0);
+  if (!A.face_mask) return MPDG_OK;
+  A.dmask = op.dmask;
+  A.mode = mode;
+  A.alpha = op.alpha;
+  A.emissivity = op.emissivity;
+  A.T_inf_conv = op.T_inf_conv;
+  A.T_inf_rad = op.T_inf_rad;
+  if (flux)
+    for (int i = 0; i < 6; ++i) A.flux[i] = flux[i];
+  long total = 0;
+  for (int f = 0; f < 6; ++f)
+    if ((A.face_mask >> f) & 1) {
+      const int d = f / 2, ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+      total += (long)op.nc[ta] * op.nc[tb] * A.n1 * A.n1;
+    }
+  hipLaunchKernelGGL(boundary_face_kernel, dim3((total + 255) / 256), dim3(256), 0, stream, A, src, T, dst);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+template <typename Number>
+int launch_diagonal(const OperatorData<Number> &op, Number *diag, Number *kq, hipStream_t stream) {
+  const int n1 = op.degree + 1;
+  int rc = launch_kappa_qp<Number>(op, kq, stream);
+  if (rc) return rc;
+  hipMemsetAsync(diag, 0, sizeof(Number) * op.n_dofs, stream);
+  const double vol = op.h[0] * op.h[1] * op.h[2];
+  const long n = op.n_cells * n1 * n1 * n1;
+  hipLaunchKernelGGL(diagonal_kernel<Number>, dim3((n + 255) / 256), dim3(256), 0, stream,
+                     make_aux_tables<Number>(op.degree), n1, op.nc[0], op.nc[1], op.n_cells, op.nd[0], op.nd[1],
+                     (Number)(op.c_mass * vol), (Number)(vol / (op.h[0] * op.h[0])),
+                     (Number)(vol / (op.h[1] * op.h[1])), (Number)(vol / (op.h[2] * op.h[2])), kq, diag);
+  if (op.dmask)
+    hipLaunchKernelGGL(diag_dirichlet_kernel<Number>, dim3((op.n_dofs + 255) / 256), dim3(256), 0, stream, diag,
+                       op.nd[0], op.nd[1], op.nd[2], op.dmask);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+// quadrature-weighted cell averages of the conductivity
+template <typename Number>
+__global__ __launch_bounds__(256) void cell_average_kernel(AuxTables<Number> t, int n1, long ncells,
+                                                           const Number *__restrict__ kq, Number *__restrict__ kc) {
+  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncells) return;
+  const int nq = n1 * n1 * n1;
+  Number s = 0;
+  for (int q = 0; q < nq; ++q) s += t.w[q % n1] * t.w[(q / n1) % n1] * t.w[q / (n1 * n1)] * kq[c * nq + q];
+  kc[c] = s;
+}
+template <typename Number>
+int launch_cell_average(const OperatorData<Number> &op, const Number *kq, Number *kc, hipStream_t stream) {
+  hipLaunchKernelGGL(cell_average_kernel<Number>, dim3((op.n_cells + 255) / 256), dim3(256), 0, stream,
+                     make_aux_tables<Number>(op.degree), op.degree + 1, op.n_cells, kq, kc);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+// average of the 8 children of every coarse cell
+template <typename Number>
+__global__ __launch_bounds__(256) void coarsen_cells_kernel(int f0, int f1, int f2, const Number *__restrict__ kf,
+                                                            Number *__restrict__ kc) {
+  const int c0 = f0 / 2, c1 = f1 / 2, c2 = f2 / 2;
+  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  if (c >= (long)c0 * c1 * c2) return;
+  const int x = c % c0, y = (c / c0) % c1, z = c / ((long)c0 * c1);
+  Number s = 0;
+  for (int i = 0; i < 8; ++i)
+    s += kf[(2 * x + (i & 1)) + (long)f0 * ((2 * y + ((i >> 1) & 1)) + (long)f1 * (2 * z + (i >> 2)))];
+  kc[c] = Number(0.125) * s;
+}
+template <typename Number>
+int launch_coarsen_cells(const int *ncf, const Number *kf, Number *kc, hipStream_t stream) {
+  const long n = (long)(ncf[0] / 2) * (ncf[1] / 2) * (ncf[2] / 2);
+  hipLaunchKernelGGL(coarsen_cells_kernel<Number>, dim3((n + 255) / 256), dim3(256), 0, stream, ncf[0], ncf[1],
+                     ncf[2], kf, kc);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+template int launch_kappa_qp<double>(const OperatorData<double> &, double *, hipStream_t);
+template int launch_kappa_qp<float>(const OperatorData<float> &, float *, hipStream_t);
+template int launch_diagonal<double>(const OperatorData<double> &, double *, double *, hipStream_t);
+template int launch_diagonal<float>(const OperatorData<float> &, float *, float *, hipStream_t);
+template int launch_cell_average<double>(const OperatorData<double> &, const double *, double *, hipStream_t);
+template int launch_cell_average<float>(const OperatorData<float> &, const float *, float *, hipStream_t);
+template int launch_coarsen_cells<double>(const int *, const double *, double *, hipStream_t);
+template int launch_coarsen_cells<float>(const int *, const float *, float *, hipStream_t);
+
+}  // namespace mpdg

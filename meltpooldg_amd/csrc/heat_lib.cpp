@@ -1,0 +1,988 @@
+// Host side of the MI355X heat operator library and its C ABI (include/mpdg_heat.h).
+#include "heat_lib.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <type_traits>
+
+using namespace mpdg;
+
+namespace {
+
+inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+inline int dev_ok() { return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE; }
+
+}  // namespace
+
+// ===========================================================================
+// operator
+// ===========================================================================
+int mpdg_heat_op::vmult_cells(double *dst, const double *src, hipStream_t s) const {
+  if (precision != MPDG_FP64) return MPDG_ERR_INVALID;
+  return launch_apply<double>(d64, src, dst, s);
+}
+
+// HeatDiffuseMultiPhaseOperator::vmult: cell loop + boundary loop, dst zeroed
+// (heat_diffuse_operator.cpp:181-197)
+int mpdg_heat_op::vmult(double *dst, const double *src, hipStream_t s) const {
+  MPDG_TRY(vmult_cells(dst, src, s));
+  if (d64.robin_mask) MPDG_TRY(launch_boundary(d64, 0, nullptr, dst, src, T_robin, s));
+  return MPDG_OK;
+}
+
+int mpdg_heat_op::vmult_f32(float *dst, const float *src, hipStream_t s) const {
+  if (precision != MPDG_FP32) return MPDG_ERR_INVALID;
+  return launch_apply<float>(d32, src, dst, s);
+}
+
+int mpdg_heat_op::diagonal64(double *diag, hipStream_t s) const {
+  const int n1 = d64.degree + 1;
+  MPDG_TRY(kq.resize((size_t)d64.n_cells * n1 * n1 * n1));
+  MPDG_TRY(launch_diagonal<double>(d64, diag, kq.get(), s));
+  if (d64.robin_mask) MPDG_TRY(launch_boundary(d64, 1, nullptr, diag, nullptr, T_robin, s));
+  return MPDG_OK;
+}
+
+// compute_inverse_diagonal_from_matrixfree (heat_diffuse_operator.cpp:273-284)
+int mpdg_heat_op::inverse_diagonal(double *inv, hipStream_t s) const {
+  MPDG_TRY(diagonal64(inv, s));
+  return launch_invert_diagonal<double>(inv, d64.n_dofs, s);
+}
+
+// create_rhs (heat_diffuse_operator.cpp:785-806, cell part :395-529, faces :721-781):
+//   dst += -[(c_m (T - T_old) - q) N + k(T) grad T . grad N] + boundary terms
+// evaluated as  dst += M (c_m T_old + q) - J_cells(T) T + boundary residual,
+// with J_cells the cell part of the tangent at T (the same quadrature).
+int mpdg_heat_op::create_rhs(double *dst, const double *T, const double *T_old, const double *q,
+                             hipStream_t s) const {
+  if (precision != MPDG_FP64) return MPDG_ERR_INVALID;
+  if (dt == 0.0) return MPDG_ERR_ZERO_TIME_INCREMENT;
+  const long n = d64.n_dofs;
+  MPDG_TRY(tmp1.resize(n));
+  MPDG_TRY(tmp2.resize(n));
+  // tmp2 = c_m T_old + q ; tmp1 = M tmp2
+  MPDG_TRY(launch_convert<double, double>(T_old, tmp2.get(), n, s));
+  MPDG_TRY(launch_axpby(tmp2.get(), 0.0, tmp2.get(), d64.c_mass, n, s));
+  if (q) MPDG_TRY(launch_axpby(tmp2.get(), 1.0, q, 1.0, n, s));
+  OperatorData<double> mass = d64;
+  mass.c_mass = 1.0;
+  mass.kappa_mode = KAPPA_CONST;
+  mass.kappa = 0.0;
+  mass.robin_mask = 0;
+  MPDG_TRY(launch_apply<double>(mass, tmp2.get(), tmp1.get(), s));
+  MPDG_TRY(launch_axpby(dst, 1.0, tmp1.get(), 1.0, n, s));
+  // tmp1 = J_cells(T) T, conductivity law evaluated at T
+  OperatorData<double> jt = d64;
+  if (jt.kappa_mode == KAPPA_LAW) jt.T = T;
+  MPDG_TRY(launch_apply<double>(jt, T, tmp1.get(), s));
+  MPDG_TRY(launch_axpby(dst, -1.0, tmp1.get(), 1.0, n, s));
+  // boundary residuals and surface flux
+  MPDG_TRY(launch_boundary(d64, 2, flux, dst, nullptr, T, s));
+  return MPDG_OK;
+}
+
+// ||u||_L2^2 = u . (M u) with the operator's quadrature (VectorTools::compute_norm)
+int mpdg_heat_op::l2_norm(const double *u, double *norm, hipStream_t s) const {
+  if (precision != MPDG_FP64) return MPDG_ERR_INVALID;
+  const long n = d64.n_dofs;
+  MPDG_TRY(tmp1.resize(n));
+  MPDG_TRY(scal.resize(4));
+  MPDG_TRY(red.init());
+  OperatorData<double> mass = d64;
+  mass.c_mass = 1.0;
+  mass.kappa_mode = KAPPA_CONST;
+  mass.kappa = 0.0;
+  mass.dmask = 0;
+  MPDG_TRY(launch_apply<double>(mass, u, tmp1.get(), s));
+  MPDG_TRY(launch_dot<double, double>(u, tmp1.get(), n, red.red(), scal.get(), s));
+  double h = 0.0;
+  if (hipMemcpyAsync(&h, scal.get(), sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) return MPDG_ERR_DEVICE;
+  if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+  *norm = std::sqrt(std::max(h, 0.0));
+  return MPDG_OK;
+}
+
+// ===========================================================================
+// preconditioners
+// ===========================================================================
+namespace {
+
+struct IdentityPrecond : mpdg_precond {
+  int update(hipStream_t) override { return MPDG_OK; }
+  int vmult(double *dst, const double *src, hipStream_t s) override {
+    return launch_convert<double, double>(src, dst, op->n_dofs(), s);
+  }
+};
+
+// JacobiPreconditioner (include/meltpooldg/linear_algebra/preconditioner_jacobi.hpp:17-89)
+struct DiagonalPrecond : mpdg_precond {
+  DeviceArray<double> inv;
+  int update(hipStream_t s) override {
+    MPDG_TRY(inv.resize(op->n_dofs()));
+    return op->inverse_diagonal(inv.get(), s);
+  }
+  int vmult(double *dst, const double *src, hipStream_t s) override;
+  const double *inverse_diagonal() const override { return inv.get(); }
+};
+
+__global__ void scale_kernel(double *dst, const double *a, const double *b, long n) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) dst[i] = a[i] * b[i];
+}
+int DiagonalPrecond::vmult(double *dst, const double *src, hipStream_t s) {
+  const long n = op->n_dofs();
+  hipLaunchKernelGGL(scale_kernel, dim3(std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, s, dst, src,
+                     inv.get(), n);
+  return dev_ok();
+}
+
+// Largest eigenvalue of a symmetric tridiagonal matrix (cyclic Jacobi rotations).
+double tridiag_max_eig(const std::vector<double> &d, const std::vector<double> &e) {
+  const int n = (int)d.size();
+  if (n == 0) return 0.0;
+  std::vector<double> A(n * n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    A[i * n + i] = d[i];
+    if (i + 1 < n) A[i * n + i + 1] = A[(i + 1) * n + i] = e[i];
+  }
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) off += A[p * n + q] * A[p * n + q];
+    if (off < 1e-30) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A[p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        const double th = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < n; ++k) {
+          const double a = A[k * n + p], b = A[k * n + q];
+          A[k * n + p] = c * a - sn * b;
+          A[k * n + q] = sn * a + c * b;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double a = A[p * n + k], b = A[q * n + k];
+          A[p * n + k] = c * a - sn * b;
+          A[q * n + k] = sn * a + c * b;
+        }
+      }
+  }
+  double mx = A[0];
+  for (int i = 1; i < n; ++i) mx = std::max(mx, A[i * n + i]);
+  return mx;
+}
+
+// 1D transfer between two structured levels, from (kc, ncc) to (kf, ncf), ncf in {ncc, 2 ncc}:
+// prolongation rows are fine nodes holding the coarse Lagrange basis at the node;
+// rows of fine nodes on Dirichlet ends and columns of coarse nodes on Dirichlet ends are dropped.
+struct Transfer1DHost {
+  int nf = 0, nc = 0, pw = 0, rw = 0;
+  std::vector<int> pcol, rcol;
+  std::vector<double> pval, rval;
+  Transfer1DHost(int kc, int ncc, int kf, int ncf, bool dir_lo, bool dir_hi) {
+    FE1D fc(kc), ff(kf);
+    nf = kf * ncf + 1;
+    nc = kc * ncc + 1;
+    pw = kc + 1;
+    pcol.assign((size_t)nf * pw, -1);
+    pval.assign((size_t)nf * pw, 0.0);
+    std::vector<std::vector<std::pair<int, double>>> cols(nc);
+    for (int g = 0; g < nf; ++g) {
+      const int cf = std::min(g / kf, ncf - 1), i = g - kf * cf;
+      const int cc = (int)((long)cf * ncc / ncf);
+      const double x = (cf + ff.nodes[i]) * (double)ncc / (double)ncf - cc;
+      double v[kMaxN1];
+      FE1D::lagrange(kc + 1, fc.nodes, x, v, nullptr);
+      const bool drop_row = (dir_lo && g == 0) || (dir_hi && g == nf - 1);
+      for (int j = 0; j < pw; ++j) {
+        const int c = kc * cc + j;
+        const bool drop_col = (dir_lo && c == 0) || (dir_hi && c == nc - 1);
+        if (drop_row || drop_col) continue;
+        pcol[(size_t)g * pw + j] = c;
+        pval[(size_t)g * pw + j] = v[j];
+        cols[c].push_back({g, v[j]});
+      }
+    }
+    rw = 1;
+    for (auto &c : cols) rw = std::max<int>(rw, (int)c.size());
+    rcol.assign((size_t)nc * rw, -1);
+    rval.assign((size_t)nc * rw, 0.0);
+    for (int c = 0; c < nc; ++c)
+      for (size_t e = 0; e < cols[c].size(); ++e) {
+        rcol[(size_t)c * rw + e] = cols[c][e].first;
+        rval[(size_t)c * rw + e] = cols[c][e].second;
+      }
+  }
+};
+
+template <typename T>
+struct Level {
+  OperatorData<T> od;
+  DeviceArray<T> x, b, r, d, Ax, inv, kcell, kqp, Tlev, kq_scratch;
+  double lambda_max = 0.0;
+  // transfer from the next coarser level (unused on the coarsest)
+  DeviceArray<int> pcol[3], rcol[3];
+  DeviceArray<T> pval[3], rval[3];
+  int pw[3] = {0, 0, 0}, rw[3] = {0, 0, 0};
+  DeviceArray<T> t1, t2;
+  std::vector<double> cheb_c1, cheb_c2;
+};
+
+template <typename T>
+int upload(DeviceArray<T> &dst, const std::vector<double> &src) {
+  MPDG_TRY(dst.resize(src.size()));
+  std::vector<T> h(src.begin(), src.end());
+  return hipMemcpy(dst.get(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess
+             ? MPDG_OK
+             : MPDG_ERR_DEVICE;
+}
+int upload_int(DeviceArray<int> &dst, const std::vector<int> &src) {
+  MPDG_TRY(dst.resize(src.size()));
+  return hipMemcpy(dst.get(), src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess
+             ? MPDG_OK
+             : MPDG_ERR_DEVICE;
+}
+
+template <typename T>
+int level_apply(const OperatorData<T> &od, const T *src, T *dst, hipStream_t s) {
+  return launch_apply<T>(od, src, dst, s);
+}
+
+// Geometric multigrid V-cycle: recipe of unit_tests/gmg.cc:343-383, :447-635
+// (global coarsening, Chebyshev smoother with the inverse diagonal, eigenvalue
+// estimate by CG), with level operators in precision T, an optional p-coarsening
+// on the coarsest mesh and an FP64 direct coarse solve.
+template <typename T>
+struct GMGPrecond : mpdg_precond {
+  mpdg_gmg_params prm{};
+  std::vector<std::unique_ptr<Level<T>>> lev;  // 0 = coarsest
+  DeviceArray<double> coarse_inv, cvec_in, cvec_out, dscratch, eig_sc, eig_hist, kcell64;
+  ReductionScratch red;
+  int nc_coarse_dofs = 0;
+
+  int build() {
+    const auto &f = op->d64;
+    std::vector<OperatorData<T>> ods;
+    OperatorData<T> top;
+    top.set_mesh(f.degree, f.nc, f.L);
+    top.c_mass = f.c_mass;
+    top.kappa_mode = f.kappa_mode;
+    top.kappa = f.kappa;
+    top.law = f.law;
+    for (int i = 0; i < 4; ++i) top.law_p[i] = f.law_p[i];
+    top.dmask = f.dmask;
+    ods.push_back(top);
+    while (true) {
+      const OperatorData<T> &p = ods.back();
+      if (p.nc[0] <= prm.n_cells_coarse || p.nc[0] % 2 || p.nc[1] % 2 || p.nc[2] % 2) break;
+      OperatorData<T> c = p;
+      const int nc[3] = {p.nc[0] / 2, p.nc[1] / 2, p.nc[2] / 2};
+      c.set_mesh(p.degree, nc, p.L);
+      c.kappa_mode = KAPPA_CELL;
+      ods.push_back(c);
+    }
+    if (prm.coarse_degree > 0 && prm.coarse_degree < f.degree) {
+      OperatorData<T> c = ods.back();
+      c.set_mesh(prm.coarse_degree, ods.back().nc, ods.back().L);
+      c.kappa_mode = KAPPA_CELL;
+      ods.push_back(c);
+    }
+    lev.clear();
+    for (auto it = ods.rbegin(); it != ods.rend(); ++it) {
+      auto L = std::make_unique<Level<T>>();
+      L->od = *it;
+      lev.push_back(std::move(L));
+    }
+    for (size_t l = 0; l < lev.size(); ++l) {
+      Level<T> &L = *lev[l];
+      const long n = L.od.n_dofs;
+      MPDG_TRY(L.x.resize(n));
+      MPDG_TRY(L.b.resize(n));
+      MPDG_TRY(L.r.resize(n));
+      MPDG_TRY(L.d.resize(n));
+      MPDG_TRY(L.Ax.resize(n));
+      MPDG_TRY(L.inv.resize(n));
+      MPDG_TRY(L.kcell.resize(L.od.n_cells));
+      if (L.od.kappa_mode == KAPPA_CELL) L.od.kappa_arr = L.kcell.get();
+      if (l == 0) continue;
+      const Level<T> &C = *lev[l - 1];
+      long t1 = 0, t2 = 0;
+      for (int d = 0; d < 3; ++d) {
+        Transfer1DHost tr(C.od.degree, C.od.nc[d], L.od.degree, L.od.nc[d], (f.dmask >> (2 * d)) & 1,
+                          (f.dmask >> (2 * d + 1)) & 1);
+        L.pw[d] = tr.pw;
+        L.rw[d] = tr.rw;
+        MPDG_TRY(upload_int(L.pcol[d], tr.pcol));
+        MPDG_TRY(upload_int(L.rcol[d], tr.rcol));
+        MPDG_TRY(upload<T>(L.pval[d], tr.pval));
+        MPDG_TRY(upload<T>(L.rval[d], tr.rval));
+      }
+      t1 = (long)L.od.nd[0] * C.od.nd[1] * C.od.nd[2];
+      t2 = (long)L.od.nd[0] * L.od.nd[1] * C.od.nd[2];
+      MPDG_TRY(L.t1.resize(t1));
+      MPDG_TRY(L.t2.resize(t2));
+    }
+    MPDG_TRY(red.init());
+    MPDG_TRY(eig_sc.resize(4));
+    MPDG_TRY(eig_hist.resize(2 * std::max(prm.eig_cg_n_iterations, 1) + 2));
+    nc_coarse_dofs = (int)lev[0]->od.n_dofs;
+    MPDG_TRY(coarse_inv.resize((size_t)nc_coarse_dofs * nc_coarse_dofs));
+    MPDG_TRY(cvec_in.resize(nc_coarse_dofs));
+    MPDG_TRY(cvec_out.resize(nc_coarse_dofs));
+    MPDG_TRY(dscratch.resize(kMaxN1 * kMaxN1 + kMaxN1));
+    MPDG_TRY(kcell64.resize(f.n_cells));
+    return MPDG_OK;
+  }
+
+  // level data that depends on the fine operator's coefficient and time increment
+  int update(hipStream_t s) override {
+    const auto &f = op->d64;
+    Level<T> &top = *lev.back();
+    for (auto &L : lev) L->od.c_mass = f.c_mass;
+    // finest level: the fine operator's conductivity in precision T
+    top.od.kappa_mode = f.kappa_mode;
+    top.od.kappa = f.kappa;
+    if (f.kappa_mode == KAPPA_QP) {
+      const long nq = f.n_cells * (long)((f.degree + 1) * (f.degree + 1) * (f.degree + 1));
+      if (std::is_same<T, double>::value) top.od.kappa_arr = (const T *)f.kappa_arr;
+      else {
+        MPDG_TRY(top.kqp.resize(nq));
+        MPDG_TRY(launch_convert<double, T>(f.kappa_arr, top.kqp.get(), nq, s));
+        top.od.kappa_arr = top.kqp.get();
+      }
+    } else if (f.kappa_mode == KAPPA_CELL) {
+      MPDG_TRY(launch_convert<double, T>(f.kappa_arr, top.kcell.get(), f.n_cells, s));
+      top.od.kappa_arr = top.kcell.get();
+    } else if (f.kappa_mode == KAPPA_LAW) {
+      if (std::is_same<T, double>::value) top.od.T = (const T *)f.T;
+      else {
+        MPDG_TRY(top.Tlev.resize(f.n_dofs));
+        MPDG_TRY(launch_convert<double, T>(f.T, top.Tlev.get(), f.n_dofs, s));
+        top.od.T = top.Tlev.get();
+      }
+    }
+    // quadrature-weighted cell averages of the fine conductivity
+    if (lev.size() > 1) MPDG_TRY(launch_op_cell_average(f, kcell64.get(), dscratch.get(), s));
+    // the coarsened averages level by level (FP64 on the host is plenty for these sizes)
+    if (lev.size() > 1) {
+      std::vector<double> kc((size_t)f.n_cells);
+      if (hipMemcpyAsync(kc.data(), kcell64.get(), kc.size() * sizeof(double), hipMemcpyDeviceToHost, s) !=
+          hipSuccess)
+        return MPDG_ERR_DEVICE;
+      if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+      int nc[3] = {f.nc[0], f.nc[1], f.nc[2]};
+      for (int l = (int)lev.size() - 2; l >= 0; --l) {
+        Level<T> &L = *lev[l];
+        if (L.od.nc[0] != nc[0]) {
+          std::vector<double> k2((size_t)L.od.n_cells);
+          for (int z = 0; z < L.od.nc[2]; ++z)
+            for (int y = 0; y < L.od.nc[1]; ++y)
+              for (int x = 0; x < L.od.nc[0]; ++x) {
+                double sum = 0.0;
+                for (int c = 0; c < 8; ++c)
+                  sum += kc[(2 * x + (c & 1)) + (size_t)nc[0] * ((2 * y + ((c >> 1) & 1)) + (size_t)nc[1] * (2 * z + (c >> 2)))];
+                k2[x + (size_t)L.od.nc[0] * (y + (size_t)L.od.nc[1] * z)] = 0.125 * sum;
+              }
+          kc.swap(k2);
+          for (int d = 0; d < 3; ++d) nc[d] = L.od.nc[d];
+        }
+        MPDG_TRY(upload<T>(L.kcell, kc));
+        L.od.kappa_arr = L.kcell.get();
+        if (l == 0) MPDG_TRY(assemble_coarse(kc, s));
+      }
+    } else {
+      std::vector<double> kc;
+      MPDG_TRY(assemble_coarse(kc, s));
+    }
+    // inverse diagonals and eigenvalue estimates
+    for (size_t l = 1; l < lev.size(); ++l) {
+      Level<T> &L = *lev[l];
+      const int n1 = L.od.degree + 1;
+      MPDG_TRY(L.kq_scratch.resize((size_t)L.od.n_cells * n1 * n1 * n1));
+      MPDG_TRY(launch_diagonal<T>(L.od, L.inv.get(), L.kq_scratch.get(), s));
+      MPDG_TRY(launch_invert_diagonal<T>(L.inv.get(), L.od.n_dofs, s));
+      L.kq_scratch.release();
+      MPDG_TRY(estimate_eigenvalue(L, s));
+      set_chebyshev(L);
+    }
+    return MPDG_OK;
+  }
+
+  int estimate_eigenvalue(Level<T> &L, hipStream_t s) {
+    const long n = L.od.n_dofs;
+    const int it = prm.eig_cg_n_iterations;
+    // r = start vector (b, x0 = 0), z = inv r, p = z
+    MPDG_TRY(launch_eig_start<T>(L.r.get(), L.od.nd, L.od.dmask, s));
+    MPDG_TRY(launch_eig_step<T>(0, L.r.get(), L.b.get(), L.d.get(), nullptr, L.inv.get(), n, red.red(), eig_sc.get(),
+                                eig_hist.get(), 0, s));
+    for (int k = 0; k < it; ++k) {
+      MPDG_TRY(level_apply<T>(L.od, L.d.get(), L.Ax.get(), s));
+      MPDG_TRY(launch_eig_step<T>(1, L.r.get(), L.b.get(), L.d.get(), L.Ax.get(), L.inv.get(), n, red.red(),
+                                  eig_sc.get(), eig_hist.get(), k, s));
+      MPDG_TRY(launch_eig_step<T>(2, L.r.get(), L.b.get(), L.d.get(), L.Ax.get(), L.inv.get(), n, red.red(),
+                                  eig_sc.get(), eig_hist.get(), k, s));
+      MPDG_TRY(launch_eig_step<T>(3, L.r.get(), L.b.get(), L.d.get(), L.Ax.get(), L.inv.get(), n, red.red(),
+                                  eig_sc.get(), eig_hist.get(), k, s));
+    }
+    std::vector<double> h(2 * it);
+    if (hipMemcpyAsync(h.data(), eig_hist.get(), h.size() * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
+      return MPDG_ERR_DEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+    std::vector<double> alphas, betas;
+    for (int k = 0; k < it; ++k) {
+      const double a = h[2 * k], b = h[2 * k + 1];
+      if (!(a == a) || !(b == b)) break;
+      alphas.push_back(a);
+      betas.push_back(b);
+      if (b <= 0.0) break;
+    }
+    const int m = (int)alphas.size();
+    std::vector<double> dd(m), ee(m > 0 ? m - 1 : 0);
+    for (int j = 0; j < m; ++j) {
+      dd[j] = 1.0 / alphas[j] + (j > 0 ? betas[j - 1] / alphas[j - 1] : 0.0);
+      if (j + 1 < m) ee[j] = std::sqrt(betas[j]) / alphas[j];
+    }
+    L.lambda_max = 1.2 * tridiag_max_eig(dd, ee);
+    return MPDG_OK;
+  }
+
+  // PreconditionChebyshev coefficients on [lambda_max/range, lambda_max]
+  void set_chebyshev(Level<T> &L) {
+    const double lmax = L.lambda_max, lmin = lmax / prm.smoothing_range;
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    L.cheb_c1.assign(prm.smoother_degree, 0.0);
+    L.cheb_c2.assign(prm.smoother_degree, 0.0);
+    L.cheb_c2[0] = 1.0 / theta;
+    for (int k = 1; k < prm.smoother_degree; ++k) {
+      const double rn = 1.0 / (2.0 * sigma - rho);
+      L.cheb_c1[k] = rn * rho;
+      L.cheb_c2[k] = 2.0 * rn / delta;
+      rho = rn;
+    }
+  }
+
+  // dense coarse matrix from element matrices (FP64), Dirichlet rows/columns identity, inverted on the device
+  int assemble_coarse(const std::vector<double> &kcell, hipStream_t s) {
+    const OperatorData<T> &od = lev[0]->od;
+    const auto &f = op->d64;
+    const int k = od.degree, n1 = k + 1, nloc = n1 * n1 * n1;
+    const long n = od.n_dofs;
+    FE1D fe(k);
+    std::vector<double> A((size_t)n * n, 0.0), Ae((size_t)nloc * nloc);
+    const double vol = od.h[0] * od.h[1] * od.h[2];
+    const double cm = f.c_mass * vol, ks[3] = {vol / (od.h[0] * od.h[0]), vol / (od.h[1] * od.h[1]),
+                                                vol / (od.h[2] * od.h[2])};
+    std::vector<double> kc = kcell;
+    if (kc.empty()) {
+      // single-level hierarchy: the fine operator itself is the coarse operator
+      kc.assign((size_t)od.n_cells, f.kappa);
+      if (f.kappa_mode != KAPPA_CONST) {
+        MPDG_TRY(launch_op_cell_average(f, kcell64.get(), dscratch.get(), s));
+        if (hipMemcpyAsync(kc.data(), kcell64.get(), kc.size() * sizeof(double), hipMemcpyDeviceToHost, s) !=
+            hipSuccess)
+          return MPDG_ERR_DEVICE;
+        if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+      }
+    }
+    for (int cz = 0; cz < od.nc[2]; ++cz)
+      for (int cy = 0; cy < od.nc[1]; ++cy)
+        for (int cx = 0; cx < od.nc[0]; ++cx) {
+          const double kap = kc[cx + (size_t)od.nc[0] * (cy + (size_t)od.nc[1] * cz)];
+          std::fill(Ae.begin(), Ae.end(), 0.0);
+          for (int qz = 0; qz < n1; ++qz)
+            for (int qy = 0; qy < n1; ++qy)
+              for (int qx = 0; qx < n1; ++qx) {
+                const double W = fe.qw[qx] * fe.qw[qy] * fe.qw[qz];
+                for (int I = 0; I < nloc; ++I) {
+                  const int i0 = I % n1, i1 = (I / n1) % n1, i2 = I / (n1 * n1);
+                  const double pI = fe.S[qx][i0] * fe.S[qy][i1] * fe.S[qz][i2];
+                  const double gI[3] = {fe.G[qx][i0] * fe.S[qy][i1] * fe.S[qz][i2],
+                                        fe.S[qx][i0] * fe.G[qy][i1] * fe.S[qz][i2],
+                                        fe.S[qx][i0] * fe.S[qy][i1] * fe.G[qz][i2]};
+                  for (int J = 0; J < nloc; ++J) {
+                    const int j0 = J % n1, j1 = (J / n1) % n1, j2 = J / (n1 * n1);
+                    const double pJ = fe.S[qx][j0] * fe.S[qy][j1] * fe.S[qz][j2];
+                    const double gJ[3] = {fe.G[qx][j0] * fe.S[qy][j1] * fe.S[qz][j2],
+                                          fe.S[qx][j0] * fe.G[qy][j1] * fe.S[qz][j2],
+                                          fe.S[qx][j0] * fe.S[qy][j1] * fe.G[qz][j2]};
+                    Ae[(size_t)I * nloc + J] +=
+                        W * (cm * pI * pJ + kap * (ks[0] * gI[0] * gJ[0] + ks[1] * gI[1] * gJ[1] + ks[2] * gI[2] * gJ[2]));
+                  }
+                }
+              }
+          std::vector<long> idx(nloc);
+          for (int I = 0; I < nloc; ++I)
+            idx[I] = (long)(k * cx + I % n1) + (long)od.nd[0] * ((k * cy + (I / n1) % n1) + (long)od.nd[1] * (k * cz + I / (n1 * n1)));
+          for (int I = 0; I < nloc; ++I)
+            for (int J = 0; J < nloc; ++J) A[(size_t)idx[I] * n + idx[J]] += Ae[(size_t)I * nloc + J];
+        }
+    if (od.dmask)
+      for (long i = 0; i < n; ++i) {
+        const int ix = i % od.nd[0], iy = (i / od.nd[0]) % od.nd[1], iz = i / ((long)od.nd[0] * od.nd[1]);
+        const int m = od.dmask;
+        const bool con = ((m & 1) && ix == 0) || ((m & 2) && ix == od.nd[0] - 1) || ((m & 4) && iy == 0) ||
+                         ((m & 8) && iy == od.nd[1] - 1) || ((m & 16) && iz == 0) || ((m & 32) && iz == od.nd[2] - 1);
+        if (!con) continue;
+        for (long j = 0; j < n; ++j) A[(size_t)i * n + j] = A[(size_t)j * n + i] = 0.0;
+        A[(size_t)i * n + i] = 1.0;
+      }
+    if (hipMemcpyAsync(coarse_inv.get(), A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
+      return MPDG_ERR_DEVICE;
+    MPDG_TRY(launch_gauss_jordan_inverse(coarse_inv.get(), (int)n, s));
+    // the inverse of a symmetric matrix is symmetric: row-major == column-major for the gemv
+    return hipStreamSynchronize(s) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+  }
+
+  int prolongate_add(Level<T> &F, const Level<T> &C, hipStream_t s) {
+    // C.x (coarse) -> F.Ax (fine) via x, y, z passes, then F.x += F.Ax
+    int o[3] = {F.od.nd[0], C.od.nd[1], C.od.nd[2]}, i[3] = {C.od.nd[0], C.od.nd[1], C.od.nd[2]};
+    MPDG_TRY(launch_ell_dir<T>(C.x.get(), F.t1.get(), o, i, 0, F.pw[0], F.pcol[0].get(), F.pval[0].get(), s));
+    int o2[3] = {F.od.nd[0], F.od.nd[1], C.od.nd[2]};
+    MPDG_TRY(launch_ell_dir<T>(F.t1.get(), F.t2.get(), o2, o, 1, F.pw[1], F.pcol[1].get(), F.pval[1].get(), s));
+    int o3[3] = {F.od.nd[0], F.od.nd[1], F.od.nd[2]};
+    MPDG_TRY(launch_ell_dir<T>(F.t2.get(), F.Ax.get(), o3, o2, 2, F.pw[2], F.pcol[2].get(), F.pval[2].get(), s));
+    return launch_add<T>(F.x.get(), F.Ax.get(), F.od.n_dofs, s);
+  }
+  int restrict_residual(const Level<T> &F, Level<T> &C, hipStream_t s) {
+    // F.r (fine) -> C.b (coarse) via z, y, x passes
+    int i3[3] = {F.od.nd[0], F.od.nd[1], F.od.nd[2]}, o2[3] = {F.od.nd[0], F.od.nd[1], C.od.nd[2]};
+    MPDG_TRY(launch_ell_dir<T>(F.r.get(), F.t2.get(), o2, i3, 2, F.rw[2], F.rcol[2].get(), F.rval[2].get(), s));
+    int o1[3] = {F.od.nd[0], C.od.nd[1], C.od.nd[2]};
+    MPDG_TRY(launch_ell_dir<T>(F.t2.get(), F.t1.get(), o1, o2, 1, F.rw[1], F.rcol[1].get(), F.rval[1].get(), s));
+    int o0[3] = {C.od.nd[0], C.od.nd[1], C.od.nd[2]};
+    return launch_ell_dir<T>(F.t1.get(), C.b.get(), o0, o1, 0, F.rw[0], F.rcol[0].get(), F.rval[0].get(), s);
+  }
+
+  int smooth(Level<T> &L, bool x_zero, hipStream_t s) {
+    const long n = L.od.n_dofs;
+    for (int k = 0; k < prm.smoother_degree; ++k) {
+      const bool first = (k == 0);
+      if (first && x_zero) {
+        MPDG_TRY(launch_chebyshev_step<T>(L.x.get(), L.d.get(), L.b.get(), L.Ax.get(), L.inv.get(), n, 0.0,
+                                          L.cheb_c2[0], true, s));
+        continue;
+      }
+      MPDG_TRY(level_apply<T>(L.od, L.x.get(), L.Ax.get(), s));
+      MPDG_TRY(launch_chebyshev_step<T>(L.x.get(), L.d.get(), L.b.get(), L.Ax.get(), L.inv.get(), n, L.cheb_c1[k],
+                                        L.cheb_c2[k], false, s));
+    }
+    return MPDG_OK;
+  }
+
+  int vcycle(int l, hipStream_t s) {
+    Level<T> &L = *lev[l];
+    if (l == 0) {
+      MPDG_TRY(launch_convert<T, double>(L.b.get(), cvec_in.get(), L.od.n_dofs, s));
+      MPDG_TRY(launch_dense_gemv(coarse_inv.get(), cvec_in.get(), cvec_out.get(), (int)L.od.n_dofs, s));
+      return launch_convert<double, T>(cvec_out.get(), L.x.get(), L.od.n_dofs, s);
+    }
+    MPDG_TRY(smooth(L, true, s));
+    MPDG_TRY(level_apply<T>(L.od, L.x.get(), L.Ax.get(), s));
+    MPDG_TRY(launch_residual<T>(L.r.get(), L.b.get(), L.Ax.get(), L.od.n_dofs, s));
+    Level<T> &C = *lev[l - 1];
+    MPDG_TRY(restrict_residual(L, C, s));
+    MPDG_TRY(vcycle(l - 1, s));
+    MPDG_TRY(prolongate_add(L, C, s));
+    return smooth(L, false, s);
+  }
+
+  int vmult(double *dst, const double *src, hipStream_t s) override {
+    Level<T> &top = *lev.back();
+    const long n = top.od.n_dofs;
+    MPDG_TRY(launch_convert<double, T>(src, top.b.get(), n, s));
+    MPDG_TRY(vcycle((int)lev.size() - 1, s));
+    return launch_convert<T, double>(top.x.get(), dst, n, s);
+  }
+
+  int n_levels() const override { return (int)lev.size(); }
+  int level_info(int l, int *degree, int *nc, double *lambda) const override {
+    if (l < 0 || l >= (int)lev.size()) return MPDG_ERR_INVALID;
+    *degree = lev[l]->od.degree;
+    for (int d = 0; d < 3; ++d) nc[d] = lev[l]->od.nc[d];
+    *lambda = lev[l]->lambda_max;
+    return MPDG_OK;
+  }
+};
+
+}  // namespace
+
+// ===========================================================================
+// Krylov solver: SolverCG driven as LinearSolver::solve (linear_solver.hpp:21-134)
+// ===========================================================================
+namespace {
+
+struct KrylovWork {
+  DeviceArray<double> r, z, p, q;
+  DeviceArray<CGState> st;
+  ReductionScratch red;
+  int init(long n) {
+    MPDG_TRY(r.resize(n));
+    MPDG_TRY(z.resize(n));
+    MPDG_TRY(p.resize(n));
+    MPDG_TRY(q.resize(n));
+    MPDG_TRY(st.resize(1));
+    return red.init();
+  }
+};
+
+int krylov_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &prm, const double *b, double *x,
+                 KrylovWork &w, int *iterations, double *res, hipStream_t s) {
+  const long n = op->n_dofs();
+  MPDG_TRY(w.init(n));
+  const bool external = pc && pc->type == MPDG_PRECOND_GMG;
+  const double *inv = (pc && pc->type == MPDG_PRECOND_DIAGONAL) ? pc->inverse_diagonal() : nullptr;
+  const bool jacobi = !external;
+  CGState h{};
+  h.max_it = prm.max_iterations;
+  h.flexible = prm.solver_type == MPDG_SOLVER_FCG ? 1 : 0;
+  if (hipMemcpyAsync(w.st.get(), &h, sizeof(CGState), hipMemcpyHostToDevice, s) != hipSuccess) return MPDG_ERR_DEVICE;
+  // r = b - A x
+  MPDG_TRY(op->vmult(w.q.get(), x, s));
+  MPDG_TRY(cg_init(b, w.r.get(), w.q.get(), w.z.get(), w.p.get(), inv, jacobi, n, w.red.red(), w.st.get(),
+                   prm.rel_tolerance, prm.abs_tolerance, prm.criterion, s));
+  if (external) {
+    MPDG_TRY(pc->vmult(w.z.get(), w.r.get(), s));
+    MPDG_TRY(cg_after_external_precond_init(w.r.get(), w.z.get(), w.p.get(), n, w.red.red(), w.st.get(), s));
+  }
+  // The done flag makes every kernel of a finished solve a no-op, so a batch of
+  // iterations can be enqueued before the host reads the state back.
+  const int batch = n > 4000000 ? 2 : 8;
+  int it = 0;
+  while (true) {
+    for (int k = 0; k < batch; ++k) {
+      MPDG_TRY(op->vmult(w.q.get(), w.p.get(), s));
+      MPDG_TRY(cg_alpha(w.p.get(), w.q.get(), n, w.red.red(), w.st.get(), s));
+      MPDG_TRY(cg_update(x, w.r.get(), w.p.get(), w.q.get(), w.z.get(), inv, jacobi, n, w.red.red(), w.st.get(), s));
+      if (external) {
+        MPDG_TRY(pc->vmult(w.z.get(), w.r.get(), s));
+        MPDG_TRY(cg_beta(w.r.get(), w.z.get(), n, w.red.red(), w.st.get(), s));
+      }
+      MPDG_TRY(cg_direction(w.p.get(), w.z.get(), n, w.st.get(), s));
+    }
+    it += batch;
+    if (hipMemcpyAsync(&h, w.st.get(), sizeof(CGState), hipMemcpyDeviceToHost, s) != hipSuccess) return MPDG_ERR_DEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+    if (h.done || it > prm.max_iterations + batch) break;
+  }
+  *iterations = h.iters;
+  *res = std::sqrt(std::max(h.rr, 0.0));
+  if (h.done == 2) return MPDG_ERR_NONFINITE;
+  if (h.rr > h.tol2) return MPDG_ERR_NO_CONVERGENCE;
+  return MPDG_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// HeatDiffuseOperation (source/heat/heat_diffuse_operation.cpp:215-298)
+// ===========================================================================
+struct mpdg_heat_operation {
+  mpdg_heat_op *op = nullptr;
+  mpdg_precond *pc = nullptr;
+  mpdg_heat_operation_params prm{};
+  DeviceArray<double> buf[3], rhs, du;
+  int cur = 0, old = 1, spare = 2;
+  double dt_old = 0.0;
+  KrylovWork work;
+  ReductionScratch red;
+  DeviceArray<double> scal;
+};
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int mpdg_heat_op_create(const mpdg_heat_desc *desc, mpdg_heat_op **out) {
+  if (!desc || !out) return MPDG_ERR_INVALID;
+  if (desc->degree < 1 || desc->degree > 4) return MPDG_ERR_INVALID;
+  for (int d = 0; d < 3; ++d)
+    if (desc->n_cells[d] < 1 || !(desc->extent[d] > 0.0)) return MPDG_ERR_INVALID;
+  if (desc->precision != MPDG_FP64 && desc->precision != MPDG_FP32) return MPDG_ERR_INVALID;
+  auto *op = new (std::nothrow) mpdg_heat_op;
+  if (!op) return MPDG_ERR_NOMEM;
+  op->desc = *desc;
+  op->precision = desc->precision;
+  op->d64.set_mesh(desc->degree, desc->n_cells, desc->extent);
+  op->d32.set_mesh(desc->degree, desc->n_cells, desc->extent);
+  op->d64.dmask = op->d32.dmask = desc->dirichlet_faces & 63;
+  op->d64.c_mass = op->d32.c_mass = desc->rho_cp;  // dt = 1 until reset_time_increment
+  *out = op;
+  return MPDG_OK;
+}
+
+int mpdg_heat_op_destroy(mpdg_heat_op *op) {
+  delete op;
+  return MPDG_OK;
+}
+long mpdg_heat_op_n_dofs(const mpdg_heat_op *op) { return op ? op->n_dofs() : -1; }
+long mpdg_heat_op_n_cells(const mpdg_heat_op *op) { return op ? op->n_cells() : -1; }
+
+int mpdg_heat_op_reset_time_increment(mpdg_heat_op *op, double dt) {
+  if (!op) return MPDG_ERR_INVALID;
+  if (dt == 0.0) return MPDG_ERR_ZERO_TIME_INCREMENT;
+  op->dt = dt;
+  op->d64.c_mass = op->d32.c_mass = op->desc.rho_cp / dt;
+  return MPDG_OK;
+}
+
+int mpdg_heat_op_set_conductivity_constant(mpdg_heat_op *op, double k) {
+  if (!op) return MPDG_ERR_INVALID;
+  op->d64.kappa_mode = op->d32.kappa_mode = KAPPA_CONST;
+  op->d64.kappa = op->d32.kappa = k;
+  return MPDG_OK;
+}
+int mpdg_heat_op_set_conductivity_qp(mpdg_heat_op *op, const void *k) {
+  if (!op || !k) return MPDG_ERR_INVALID;
+  op->d64.kappa_mode = op->d32.kappa_mode = KAPPA_QP;
+  op->d64.kappa_arr = (const double *)k;
+  op->d32.kappa_arr = (const float *)k;
+  return MPDG_OK;
+}
+int mpdg_heat_op_set_conductivity_cell(mpdg_heat_op *op, const void *k) {
+  if (!op || !k) return MPDG_ERR_INVALID;
+  op->d64.kappa_mode = op->d32.kappa_mode = KAPPA_CELL;
+  op->d64.kappa_arr = (const double *)k;
+  op->d32.kappa_arr = (const float *)k;
+  return MPDG_OK;
+}
+int mpdg_heat_op_set_conductivity_law(mpdg_heat_op *op, int law, const double *p, const void *T) {
+  if (!op || !p || !T || (law != MPDG_LAW_TANH && law != MPDG_LAW_MUSHY)) return MPDG_ERR_INVALID;
+  op->d64.kappa_mode = op->d32.kappa_mode = KAPPA_LAW;
+  op->d64.law = op->d32.law = law;
+  for (int i = 0; i < 4; ++i) op->d64.law_p[i] = op->d32.law_p[i] = p[i];
+  op->d64.T = (const double *)T;
+  op->d32.T = (const float *)T;
+  return MPDG_OK;
+}
+int mpdg_heat_op_set_robin(mpdg_heat_op *op, int mask, double alpha, double T_inf_conv, double emissivity,
+                           double T_inf_rad, const double *T) {
+  if (!op || op->precision != MPDG_FP64) return MPDG_ERR_INVALID;
+  if (emissivity > 0.0 && !T) return MPDG_ERR_INVALID;
+  op->d64.robin_mask = mask & 63;
+  op->d64.alpha = alpha;
+  op->d64.T_inf_conv = T_inf_conv;
+  op->d64.emissivity = emissivity;
+  op->d64.T_inf_rad = T_inf_rad;
+  op->T_robin = T;
+  return MPDG_OK;
+}
+int mpdg_heat_op_set_surface_flux(mpdg_heat_op *op, const double *p) {
+  if (!op || !p) return MPDG_ERR_INVALID;
+  for (int i = 0; i < 6; ++i) op->flux[i] = p[i];
+  return MPDG_OK;
+}
+
+int mpdg_heat_op_vmult(const mpdg_heat_op *op, double *dst, const double *src, void *stream) {
+  if (!op || !dst || !src) return MPDG_ERR_INVALID;
+  return op->vmult(dst, src, S(stream));
+}
+int mpdg_heat_op_vmult_f32(const mpdg_heat_op *op, float *dst, const float *src, void *stream) {
+  if (!op || !dst || !src) return MPDG_ERR_INVALID;
+  return op->vmult_f32(dst, src, S(stream));
+}
+int mpdg_heat_op_compute_inverse_diagonal(const mpdg_heat_op *op, double *inv, void *stream) {
+  if (!op || !inv || op->precision != MPDG_FP64) return MPDG_ERR_INVALID;
+  return op->inverse_diagonal(inv, S(stream));
+}
+int mpdg_heat_op_compute_diagonal(const mpdg_heat_op *op, void *diag, void *stream) {
+  if (!op || !diag) return MPDG_ERR_INVALID;
+  if (op->precision == MPDG_FP64) return op->diagonal64((double *)diag, S(stream));
+  const int n1 = op->d32.degree + 1;
+  DeviceArray<float> kq;
+  MPDG_TRY(kq.resize((size_t)op->d32.n_cells * n1 * n1 * n1));
+  MPDG_TRY(launch_diagonal<float>(op->d32, (float *)diag, kq.get(), S(stream)));
+  return hipStreamSynchronize(S(stream)) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+int mpdg_heat_op_create_rhs(const mpdg_heat_op *op, double *dst, const double *T, const double *T_old,
+                            const double *q, void *stream) {
+  if (!op || !dst || !T || !T_old) return MPDG_ERR_INVALID;
+  return op->create_rhs(dst, T, T_old, q, S(stream));
+}
+int mpdg_heat_op_l2_norm(const mpdg_heat_op *op, const double *u, double *norm, void *stream) {
+  if (!op || !u || !norm) return MPDG_ERR_INVALID;
+  return op->l2_norm(u, norm, S(stream));
+}
+
+int mpdg_precond_create(mpdg_heat_op *op, int type, const mpdg_gmg_params *gmg, mpdg_precond **out) {
+  if (!op || !out || op->precision != MPDG_FP64) return MPDG_ERR_INVALID;
+  mpdg_precond *p = nullptr;
+  if (type == MPDG_PRECOND_IDENTITY) p = new (std::nothrow) IdentityPrecond;
+  else if (type == MPDG_PRECOND_DIAGONAL) p = new (std::nothrow) DiagonalPrecond;
+  else if (type == MPDG_PRECOND_GMG) {
+    if (!gmg || gmg->smoother_degree < 1 || gmg->smoothing_range <= 1.0 || gmg->eig_cg_n_iterations < 1)
+      return MPDG_ERR_INVALID;
+    if (gmg->precision == MPDG_FP32) {
+      auto *g = new (std::nothrow) GMGPrecond<float>;
+      if (!g) return MPDG_ERR_NOMEM;
+      g->prm = *gmg;
+      g->op = op;
+      g->type = type;
+      const int rc = g->build();
+      if (rc) { delete g; return rc; }
+      p = g;
+    } else if (gmg->precision == MPDG_FP64) {
+      auto *g = new (std::nothrow) GMGPrecond<double>;
+      if (!g) return MPDG_ERR_NOMEM;
+      g->prm = *gmg;
+      g->op = op;
+      g->type = type;
+      const int rc = g->build();
+      if (rc) { delete g; return rc; }
+      p = g;
+    } else {
+      return MPDG_ERR_INVALID;
+    }
+  } else {
+    return MPDG_ERR_INVALID;
+  }
+  if (!p) return MPDG_ERR_NOMEM;
+  p->op = op;
+  p->type = type;
+  *out = p;
+  return MPDG_OK;
+}
+int mpdg_precond_update(mpdg_precond *p, void *stream) {
+  if (!p) return MPDG_ERR_INVALID;
+  MPDG_TRY(p->update(S(stream)));
+  return hipStreamSynchronize(S(stream)) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+int mpdg_precond_vmult(mpdg_precond *p, double *dst, const double *src, void *stream) {
+  if (!p || !dst || !src) return MPDG_ERR_INVALID;
+  return p->vmult(dst, src, S(stream));
+}
+int mpdg_precond_destroy(mpdg_precond *p) {
+  delete p;
+  return MPDG_OK;
+}
+int mpdg_gmg_n_levels(const mpdg_precond *p) { return p ? p->n_levels() : 0; }
+int mpdg_gmg_level_info(const mpdg_precond *p, int level, int *degree, int *nc, double *lambda) {
+  if (!p || !degree || !nc || !lambda) return MPDG_ERR_INVALID;
+  return p->level_info(level, degree, nc, lambda);
+}
+
+int mpdg_linear_solve(mpdg_heat_op *op, mpdg_precond *p, const mpdg_solver_params *prm, const double *b, double *x,
+                      int *iterations, double *res, void *stream) {
+  if (!op || !prm || !b || !x || !iterations || !res || op->precision != MPDG_FP64) return MPDG_ERR_INVALID;
+  if (prm->solver_type != MPDG_SOLVER_CG && prm->solver_type != MPDG_SOLVER_FCG) return MPDG_ERR_INVALID;
+  KrylovWork w;
+  return krylov_solve(op, p, *prm, b, x, w, iterations, res, S(stream));
+}
+
+int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p, const mpdg_heat_operation_params *prm,
+                               mpdg_heat_operation **out) {
+  if (!op || !prm || !out || op->precision != MPDG_FP64) return MPDG_ERR_INVALID;
+  if (prm->predictor < MPDG_PREDICTOR_NONE || prm->predictor > MPDG_PREDICTOR_LINEAR) return MPDG_ERR_INVALID;
+  auto *h = new (std::nothrow) mpdg_heat_operation;
+  if (!h) return MPDG_ERR_NOMEM;
+  h->op = op;
+  h->pc = p;
+  h->prm = *prm;
+  const long n = op->n_dofs();
+  int rc = MPDG_OK;
+  for (auto &b : h->buf) rc = rc ? rc : b.resize(n);
+  rc = rc ? rc : h->rhs.resize(n);
+  rc = rc ? rc : h->du.resize(n);
+  rc = rc ? rc : h->red.init();
+  rc = rc ? rc : h->scal.resize(4);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return MPDG_OK;
+}
+int mpdg_heat_operation_destroy(mpdg_heat_operation *h) {
+  delete h;
+  return MPDG_OK;
+}
+int mpdg_heat_operation_set_initial(mpdg_heat_operation *h, const double *T0, void *stream) {
+  if (!h || !T0) return MPDG_ERR_INVALID;
+  const long n = h->op->n_dofs();
+  MPDG_TRY(launch_convert<double, double>(T0, h->buf[h->cur].get(), n, S(stream)));
+  MPDG_TRY(launch_convert<double, double>(T0, h->buf[h->old].get(), n, S(stream)));
+  h->dt_old = 0.0;
+  return hipStreamSynchronize(S(stream)) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+// HeatDiffuseOperation::init_time_advance + solve (heat_diffuse_operation.cpp:215-289)
+int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_iters, int *nl_iters, void *stream) {
+  if (!h) return MPDG_ERR_INVALID;
+  hipStream_t s = S(stream);
+  mpdg_heat_op *op = h->op;
+  const long n = op->n_dofs();
+  MPDG_TRY(mpdg_heat_op_reset_time_increment(op, dt));
+  // Predictor::vmult (predictor.hpp:118-160): pred from T_n (cur) and T_{n-1} (old)
+  double *pred = h->buf[h->spare].get();
+  const double *Tn = h->buf[h->cur].get(), *Tnm1 = h->buf[h->old].get();
+  switch (h->prm.predictor) {
+    case MPDG_PREDICTOR_NONE: MPDG_TRY(launch_convert<double, double>(Tn, pred, n, s)); break;
+    case MPDG_PREDICTOR_ZERO: MPDG_TRY(launch_fill<double>(pred, 0.0, n, s)); break;
+    case MPDG_PREDICTOR_LINEAR: {
+      // compute_linear_predictor (source/linear_algebra/predictor_linear.cpp:76-97)
+      if (std::fabs(h->dt_old) < 1e-12) MPDG_TRY(launch_convert<double, double>(Tn, pred, n, s));
+      else {
+        const double f = dt / h->dt_old;
+        MPDG_TRY(launch_convert<double, double>(Tn, pred, n, s));
+        MPDG_TRY(launch_axpby(pred, -f, Tnm1, 1.0 + f, n, s));
+      }
+      break;
+    }
+    default: return MPDG_ERR_INVALID;
+  }
+  // commit_old_solutions + swap: old <- T_n, cur <- prediction
+  const int new_old = h->cur, new_cur = h->spare, new_spare = h->old;
+  h->old = new_old;
+  h->cur = new_cur;
+  h->spare = new_spare;
+  double *T = h->buf[h->cur].get();
+  const double *T_old = h->buf[h->old].get();
+  if (op->d64.kappa_mode == KAPPA_LAW) op->d64.T = T;
+  if (op->d64.robin_mask) op->T_robin = T;
+  // preconditioner.update() at the predicted state (heat_diffuse_operation.cpp:276)
+  if (h->pc) MPDG_TRY(h->pc->update(s));
+  // NewtonRaphsonSolver::solve (newton_raphson_solver.cpp:24-66)
+  int acc = 0, it = 0;
+  bool converged = false;
+  const int max_it = std::max(1, h->prm.max_nonlinear_iterations);
+  for (it = 0; it < max_it; ++it) {
+    MPDG_TRY(launch_fill<double>(h->rhs.get(), 0.0, n, s));
+    MPDG_TRY(op->create_rhs(h->rhs.get(), T, T_old, nullptr, s));
+    MPDG_TRY(launch_fill<double>(h->du.get(), 0.0, n, s));
+    int li = 0;
+    double res = 0.0;
+    const int rc = krylov_solve(op, h->pc, h->prm.linear, h->rhs.get(), h->du.get(), h->work, &li, &res, s);
+    acc += li;
+    if (rc == MPDG_ERR_NONFINITE) return rc;
+    if (rc != MPDG_OK && rc != MPDG_ERR_NO_CONVERGENCE) return rc;
+    if (rc == MPDG_ERR_NO_CONVERGENCE) return rc;  // SolverControl::NoConvergence is rethrown (linear_solver.hpp:124-128)
+    // is_converged (newton_raphson_solver.cpp:104-127)
+    MPDG_TRY(launch_dot<double, double>(h->rhs.get(), h->rhs.get(), n, h->red.red(), h->scal.get(), s));
+    MPDG_TRY(launch_dot<double, double>(h->du.get(), h->du.get(), n, h->red.red(), h->scal.get() + 1, s));
+    double nr[2];
+    if (hipMemcpyAsync(nr, h->scal.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
+      return MPDG_ERR_DEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+    if (!(nr[0] == nr[0]) || !(nr[1] == nr[1])) return MPDG_ERR_NONFINITE;
+    if (std::sqrt(nr[0]) < h->prm.residual_tolerance && std::sqrt(nr[1]) < h->prm.field_correction_tolerance) {
+      converged = true;
+      break;
+    }
+    MPDG_TRY(launch_axpby(T, 1.0, h->du.get(), 1.0, n, s));
+  }
+  h->dt_old = dt;
+  if (lin_iters) *lin_iters = acc;
+  if (nl_iters) *nl_iters = it + 1;
+  if (!converged) return MPDG_ERR_NO_CONVERGENCE;  // ExcNewtonDidNotConverge
+  return hipStreamSynchronize(s) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+const double *mpdg_heat_operation_solution(const mpdg_heat_operation *h) { return h ? h->buf[h->cur].get() : nullptr; }
+const double *mpdg_heat_operation_old_solution(const mpdg_heat_operation *h) {
+  return h ? h->buf[h->old].get() : nullptr;
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// Host-side objects behind the C ABI: the operator, the preconditioners
+// (inverse diagonal, geometric multigrid), the Krylov solver and the implicit
+// heat time step. They mirror MeltPoolDG's OperatorMatrixFree,
+// Preconditioner, LinearSolver, NewtonRaphsonSolver, Predictor and
+// HeatDiffuseOperation (see the citations on each class).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "blas.hpp"
+
+namespace mpdg {
+
+int launch_op_cell_average(const OperatorData<double> &op, double *kc, double *table_scratch, hipStream_t stream);
+int launch_gauss_jordan_inverse(double *A, int n, hipStream_t s);
+
+// device buffer owned by the library
+template <typename T>
+struct DeviceArray {
+  T *ptr = nullptr;
+  size_t n = 0;
+  DeviceArray() = default;
+  DeviceArray(const DeviceArray &) = delete;
+  DeviceArray &operator=(const DeviceArray &) = delete;
+  ~DeviceArray() { release(); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+  int resize(size_t count) {
+    if (count == n && ptr) return MPDG_OK;
+    release();
+    if (count == 0) return MPDG_OK;
+    if (hipMalloc((void **)&ptr, count * sizeof(T)) != hipSuccess) {
+      ptr = nullptr;
+      return MPDG_ERR_NOMEM;
+    }
+    n = count;
+    return MPDG_OK;
+  }
+  T *get() const { return ptr; }
+};
+
+struct ReductionScratch {
+  DeviceArray<double> partials;
+  DeviceArray<unsigned> counter;
+  int init() {
+    int rc = partials.resize(4 * kRedBlocks);
+    if (rc) return rc;
+    rc = counter.resize(1);
+    if (rc) return rc;
+    return hipMemset(counter.get(), 0, sizeof(unsigned)) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+  }
+  Reduction red() const {
+    Reduction r;
+    r.partials = partials.get();
+    r.counter = counter.get();
+    return r;
+  }
+};
+
+#define MPDG_TRY(...)            \
+  do {                           \
+    const int rc__ = (__VA_ARGS__); \
+    if (rc__ != MPDG_OK) return rc__; \
+  } while (0)
+
+}  // namespace mpdg
+
+// ---------------------------------------------------------------------------
+// HeatDiffuseMultiPhaseOperator (source/heat/heat_diffuse_operator.cpp) as an
+// OperatorMatrixFree (include/meltpooldg/core/operator_base.hpp:189-282).
+// ---------------------------------------------------------------------------
+struct mpdg_heat_op {
+  mpdg_heat_desc desc{};
+  int precision = MPDG_FP64;
+  double dt = 0.0;
+  mpdg::OperatorData<double> d64;
+  mpdg::OperatorData<float> d32;
+  double flux[6] = {0, 0, 0, 0, 0, 0};
+  const double *T_robin = nullptr;
+  mutable mpdg::DeviceArray<double> kq, tmp1, tmp2, scal, tables;
+  mutable mpdg::ReductionScratch red;
+
+  long n_dofs() const { return d64.n_dofs; }
+  long n_cells() const { return d64.n_cells; }
+  int vmult(double *dst, const double *src, hipStream_t s) const;
+  int vmult_cells(double *dst, const double *src, hipStream_t s) const;  // without boundary faces
+  int vmult_f32(float *dst, const float *src, hipStream_t s) const;
+  int diagonal64(double *diag, hipStream_t s) const;
+  int inverse_diagonal(double *inv, hipStream_t s) const;
+  int create_rhs(double *dst, const double *T, const double *T_old, const double *q, hipStream_t s) const;
+  int l2_norm(const double *u, double *norm, hipStream_t s) const;
+};
+
+// ---------------------------------------------------------------------------
+// Preconditioner concept update()/vmult()/reinit()
+// (include/meltpooldg/linear_algebra/preconditioner.hpp:19-37)
+// ---------------------------------------------------------------------------
+struct mpdg_precond {
+  virtual ~mpdg_precond() = default;
+  int type = MPDG_PRECOND_IDENTITY;
+  mpdg_heat_op *op = nullptr;
+  virtual int update(hipStream_t s) = 0;
+  virtual int vmult(double *dst, const double *src, hipStream_t s) = 0;
+  virtual const double *inverse_diagonal() const { return nullptr; }
+  virtual int n_levels() const { return 0; }
+  virtual int level_info(int, int *, int *, double *) const { return MPDG_ERR_INVALID; }
+};
+
+struct mpdg_heat_operation;

@@ -1,0 +1,109 @@
+// Shared declarations of the HIP kernels and their host launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "../../include/mpdg_heat.h"
+#include "fe1d.hpp"
+
+namespace mpdg {
+
+enum KappaMode { KAPPA_CONST = MPDG_KAPPA_CONST, KAPPA_QP = MPDG_KAPPA_QP, KAPPA_CELL = MPDG_KAPPA_CELL, KAPPA_LAW = MPDG_KAPPA_LAW };
+enum LawId { LAW_TANH = MPDG_LAW_TANH, LAW_MUSHY = MPDG_LAW_MUSHY };
+
+// Everything a level operator needs on the device (pointers are device memory).
+template <typename Number>
+struct OperatorData {
+  int degree = 1;
+  int nc[3] = {1, 1, 1};
+  int nd[3] = {2, 2, 2};
+  double L[3] = {1, 1, 1};
+  double h[3] = {1, 1, 1};
+  long n_dofs = 0, n_cells = 0;
+  double c_mass = 1.0;
+  int kappa_mode = KAPPA_CONST;
+  double kappa = 1.0;
+  const Number *kappa_arr = nullptr;
+  int law = 0;
+  double law_p[4] = {0, 0, 0, 1};
+  const Number *T = nullptr;
+  int dmask = 0;
+  int robin_mask = 0;
+  double alpha = 0, emissivity = 0, T_inf_conv = 0, T_inf_rad = 0;
+  const double *T_robin = nullptr;  // temperature for the radiation linearisation (FP64)
+  void set_mesh(int k, const int *ncells, const double *ext) {
+    degree = k;
+    for (int d = 0; d < 3; ++d) {
+      nc[d] = ncells[d];
+      nd[d] = k * ncells[d] + 1;
+      L[d] = ext[d];
+      h[d] = ext[d] / ncells[d];
+    }
+    n_dofs = (long)nd[0] * nd[1] * nd[2];
+    n_cells = (long)nc[0] * nc[1] * nc[2];
+  }
+};
+
+template <typename Number, int N1>
+struct ApplyArgs {
+  Number S[N1][N1];  // shape values at the Gauss points
+  Number D[N1][N1];  // collocation derivative on the Gauss points
+  Number w[N1];      // Gauss weights
+  int nc[3], nd[3], ncol[2];
+  int dmask, xcd_cols, law;
+  Number cm;         // c_mass * hx hy hz
+  Number ks[3];      // hx hy hz / h_d^2
+  Number kappa;
+  Number law_p[4];
+  const Number *kappa_arr;
+  const Number *T;
+};
+
+template <typename Number, int N1>
+void fill_apply_args(const OperatorData<Number> &op, ApplyArgs<Number, N1> &A) {
+  FE1D fe(N1 - 1);
+  for (int a = 0; a < N1; ++a) {
+    A.w[a] = (Number)fe.qw[a];
+    for (int b = 0; b < N1; ++b) {
+      A.S[a][b] = (Number)fe.S[a][b];
+      A.D[a][b] = (Number)fe.D[a][b];
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    A.nc[d] = op.nc[d];
+    A.nd[d] = op.nd[d];
+  }
+  const double vol = op.h[0] * op.h[1] * op.h[2];
+  A.cm = (Number)(op.c_mass * vol);
+  for (int d = 0; d < 3; ++d) A.ks[d] = (Number)(vol / (op.h[d] * op.h[d]));
+  A.dmask = op.dmask;
+  A.kappa = (Number)op.kappa;
+  A.kappa_arr = op.kappa_arr;
+  A.T = op.T;
+  A.law = op.law;
+  for (int i = 0; i < 4; ++i) A.law_p[i] = (Number)op.law_p[i];
+  A.xcd_cols = 0;
+  A.ncol[0] = A.ncol[1] = 1;
+}
+
+// heat_apply.hip
+template <typename Number>
+int launch_apply(const OperatorData<Number> &op, const Number *src, Number *dst, hipStream_t stream);
+
+// heat_aux.hip: diagonal, boundary faces, residual, norms, conductivity
+template <typename Number>
+int launch_diagonal(const OperatorData<Number> &op, Number *diag, Number *kq_scratch, hipStream_t stream);
+// boundary faces: mode 0 Robin tangent on src, 1 its diagonal, 2 boundary residual (+ surface flux)
+int launch_boundary(const OperatorData<double> &op, int mode, const double *flux, double *dst,
+                    const double *src, const double *T, hipStream_t stream);
+template <typename Number>
+int launch_kappa_qp(const OperatorData<Number> &op, Number *kq, hipStream_t stream);
+template <typename Number>
+int launch_cell_average(const OperatorData<Number> &op, const Number *kq, Number *kcell, hipStream_t stream);
+template <typename Number>
+int launch_coarsen_cells(const int *nc_fine, const Number *kf, Number *kc, hipStream_t stream);
+
+}  // namespace mpdg

@@ -1,0 +1,911 @@
+// CPU oracle for the matrix-free heat operator path -- TEST INFRASTRUCTURE ONLY.
+//
+// A restatement, in plain FP64 C++, of the algorithm MeltPoolDG runs through
+// deal.II for its implicit heat step. It is the checker for the HIP library in
+// meltpooldg_amd/ and the CPU baseline that bench.py times; the product path
+// never links it. Each function cites the reference file:line it follows
+// (paths relative to the MeltPoolDG repository root).
+//
+// Parity status of each part (how it is pinned, see tests/test_oracle.py):
+//   vmult / cell loop      pinned: brute-force element matrices, closed forms
+//                          (K*1 = 0, 1'M1 = |Omega| c_mass), exact polynomial
+//                          integrals, symmetry, 2D golden norm sequences.
+//   diagonal               pinned: diagonal of the brute-force matrix.
+//   create_rhs, Robin,     pinned: the reference's golden ||T||_L2 sequences of
+//   Newton, predictor,     unidirectional_heat_transfer*.mpirun=4.output.
+//   L2 norm
+//   conductivity laws      pinned: unit_tests/material.output (mushy zone) and
+//                          the closed form of the tanh law.
+//   CG                     pinned: scipy.sparse.linalg.cg on the brute-force matrix.
+//   GMG pieces             pinned: transfer exactness on polynomials, two-grid
+//                          contraction, Chebyshev polynomial identity; the V-cycle
+//                          as a whole is "parity unpinned" against the reference
+//                          (unit_tests/gmg.cc is 2D with hanging nodes).
+#include "mpdg_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr int MAXN = 8;
+constexpr double PI = 3.14159265358979323846;
+// include/meltpooldg/utilities/physical_constants.hpp:6
+constexpr double STEFAN_BOLTZMANN = 5.670374419e-8;
+
+// ---------------------------------------------------------------------------
+// 1D finite element data: FE_Q(k) on Gauss-Lobatto support points and
+// QGauss(k+1) (source/utilities/fe_util.cpp:18-58, finite_element_data.cpp:62-65).
+// ---------------------------------------------------------------------------
+void legendre(int n, double x, double &p, double &dp) {
+  double p0 = 1.0, p1 = x;
+  if (n == 0) { p = 1.0; dp = 0.0; return; }
+  for (int m = 2; m <= n; ++m) {
+    const double p2 = ((2.0 * m - 1.0) * x * p1 - (m - 1.0) * p0) / m;
+    p0 = p1;
+    p1 = p2;
+  }
+  p = p1;
+  dp = n * (x * p1 - p0) / (x * x - 1.0);
+}
+
+void gauss_points(int n, double *x, double *w) {
+  for (int j = 0; j < n; ++j) {
+    double z = std::cos(PI * (j + 0.75) / (n + 0.5));
+    for (int it = 0; it < 100; ++it) {
+      double p, dp;
+      legendre(n, z, p, dp);
+      const double dz = p / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-17) break;
+    }
+    double p, dp;
+    legendre(n, z, p, dp);
+    // ascending order on [0,1]
+    x[n - 1 - j] = 0.5 * (z + 1.0);
+    w[n - 1 - j] = 1.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+void gauss_lobatto_points(int n, double *x) {
+  x[0] = 0.0;
+  x[n - 1] = 1.0;
+  const int m = n - 1;  // interior points: roots of P_m'
+  for (int j = 1; j < n - 1; ++j) {
+    double z = -std::cos(PI * j / m);
+    for (int it = 0; it < 100; ++it) {
+      double p, dp;
+      legendre(m, z, p, dp);
+      // (1-z^2) P'' = 2 z P' - m(m+1) P
+      const double d2p = (2.0 * z * dp - m * (m + 1.0) * p) / (1.0 - z * z);
+      const double dz = dp / d2p;
+      z -= dz;
+      if (std::fabs(dz) < 1e-17) break;
+    }
+    x[j] = 0.5 * (z + 1.0);
+  }
+}
+
+void lagrange(int n, const double *nodes, double xx, double *val, double *der) {
+  for (int i = 0; i < n; ++i) {
+    double v = 1.0;
+    for (int j = 0; j < n; ++j)
+      if (j != i) v *= (xx - nodes[j]) / (nodes[i] - nodes[j]);
+    double d = 0.0;
+    for (int m = 0; m < n; ++m) {
+      if (m == i) continue;
+      double t = 1.0 / (nodes[i] - nodes[m]);
+      for (int j = 0; j < n; ++j)
+        if (j != i && j != m) t *= (xx - nodes[j]) / (nodes[i] - nodes[j]);
+      d += t;
+    }
+    if (val) val[i] = v;
+    if (der) der[i] = d;
+  }
+}
+
+struct FE1D {
+  int k = 1, n = 2;
+  double nodes[MAXN], qp[MAXN], qw[MAXN];
+  double S[MAXN][MAXN], G[MAXN][MAXN], D[MAXN][MAXN];
+  explicit FE1D(int degree) : k(degree), n(degree + 1) {
+    gauss_lobatto_points(n, nodes);
+    gauss_points(n, qp, qw);
+    for (int q = 0; q < n; ++q) {
+      double v[MAXN], d[MAXN];
+      lagrange(n, nodes, qp[q], v, d);
+      for (int i = 0; i < n; ++i) { S[q][i] = v[i]; G[q][i] = d[i]; }
+      lagrange(n, qp, qp[q], nullptr, d);
+      for (int p = 0; p < n; ++p) D[q][p] = d[p];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Mesh helpers
+// ---------------------------------------------------------------------------
+struct Mesh {
+  int k, n1, nc[3], nd[3];
+  double h[3];
+  long ndofs, ncells;
+  explicit Mesh(const orc_op *op) {
+    k = op->degree;
+    n1 = k + 1;
+    for (int d = 0; d < 3; ++d) {
+      nc[d] = op->nc[d];
+      nd[d] = k * nc[d] + 1;
+      h[d] = op->L[d] / nc[d];
+    }
+    ndofs = (long)nd[0] * nd[1] * nd[2];
+    ncells = (long)nc[0] * nc[1] * nc[2];
+  }
+  long dof(int ix, int iy, int iz) const { return ix + (long)nd[0] * (iy + (long)nd[1] * iz); }
+  long cell_dof(int cx, int cy, int cz, int i, int j, int l) const {
+    return dof(k * cx + i, k * cy + j, k * cz + l);
+  }
+  long cell(int cx, int cy, int cz) const { return cx + (long)nc[0] * (cy + (long)nc[1] * cz); }
+  bool constrained(int mask, int ix, int iy, int iz) const {
+    if (!mask) return false;
+    if ((mask & 1) && ix == 0) return true;
+    if ((mask & 2) && ix == nd[0] - 1) return true;
+    if ((mask & 4) && iy == 0) return true;
+    if ((mask & 8) && iy == nd[1] - 1) return true;
+    if ((mask & 16) && iz == 0) return true;
+    if ((mask & 32) && iz == nd[2] - 1) return true;
+    return false;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Conductivity laws (a4).
+// tanh law: k0 (1 + a tanh((T - T0)/w)) -- the law the benchmark configurations
+// specify; not in the reference, pinned by its closed form.
+// mushy zone: include/meltpooldg/core/material.templates.hpp:296-330 (liquid_solid),
+// :546-559 (cubic blend), :585-598 (derivative), :765-778 (solid fraction);
+// include/meltpooldg/level_set/level_set_tools.hpp:72-89 (interpolate_cubic).
+// ---------------------------------------------------------------------------
+inline void kappa_law(int law, const double *p, double T, double &k, double &dk) {
+  if (law == ORC_LAW_TANH) {
+    const double th = std::tanh((T - p[2]) / p[3]);
+    k = p[0] * (1.0 + p[1] * th);
+    dk = p[0] * p[1] * (1.0 - th * th) / p[3];
+  } else if (law == ORC_LAW_MUSHY) {
+    const double kl = p[0], ks = p[1], Ts = p[2], Tl = p[3];
+    const double inv = 1.0 / (Tl - Ts);
+    const double fs = std::min(1.0, std::max(0.0, (Tl - T) * inv));
+    if (fs == 0.0) { k = kl; dk = 0.0; return; }
+    if (fs == 1.0) { k = ks; dk = 0.0; return; }
+    k = kl + (ks - kl) * (-2.0 * fs * fs * fs + 3.0 * fs * fs);
+    dk = -1.0 * inv * (ks - kl) * (-6.0 * fs * fs + 6.0 * fs);
+  } else {
+    k = 0.0;
+    dk = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sum factorisation on one cell (scalar), n = n1 points per direction.
+// Array layout a[x + n*(y + n*z)].
+// ---------------------------------------------------------------------------
+template <bool TRANSPOSE, bool ADD>
+inline void sweep(int n, int dir, const double M[MAXN][MAXN], const double *in, double *out) {
+  const int s = dir == 0 ? 1 : (dir == 1 ? n : n * n);
+  for (int a = 0; a < n * n; ++a) {
+    int base;
+    if (dir == 0) base = a * n;
+    else if (dir == 1) base = (a % n) + (a / n) * n * n;
+    else base = a;
+    for (int q = 0; q < n; ++q) {
+      double acc = 0.0;
+      for (int i = 0; i < n; ++i) acc += (TRANSPOSE ? M[i][q] : M[q][i]) * in[base + i * s];
+      if (ADD) out[base + q * s] += acc;
+      else out[base + q * s] = acc;
+    }
+  }
+}
+
+// values at quadrature points: v = (S x S x S) u
+inline void eval_values(const FE1D &fe, const double *u, double *v, double *tmp) {
+  const int n = fe.n;
+  sweep<false, false>(n, 0, fe.S, u, v);
+  sweep<false, false>(n, 1, fe.S, v, tmp);
+  sweep<false, false>(n, 2, fe.S, tmp, v);
+}
+// reference gradient at quadrature points via collocation derivative
+inline void eval_gradients(const FE1D &fe, const double *v, double *g0, double *g1, double *g2) {
+  const int n = fe.n;
+  sweep<false, false>(n, 0, fe.D, v, g0);
+  sweep<false, false>(n, 1, fe.D, v, g1);
+  sweep<false, false>(n, 2, fe.D, v, g2);
+}
+
+// Per-cell geometry at quadrature points: Jinv[q][a][b] = d xi_a / d x_b, JxW[q]
+struct CellGeom {
+  double Jinv[512][3][3];
+  double JxW[512];
+};
+
+void cell_geometry(const FE1D &fe, const Mesh &m, const double *coords, int cx, int cy, int cz,
+                   CellGeom &g) {
+  const int n = fe.n, nq = n * n * n;
+  if (!coords) {
+    const double det = m.h[0] * m.h[1] * m.h[2];
+    for (int qz = 0; qz < n; ++qz)
+      for (int qy = 0; qy < n; ++qy)
+        for (int qx = 0; qx < n; ++qx) {
+          const int q = qx + n * (qy + n * qz);
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) g.Jinv[q][a][b] = (a == b) ? 1.0 / m.h[a] : 0.0;
+          g.JxW[q] = det * fe.qw[qx] * fe.qw[qy] * fe.qw[qz];
+        }
+    return;
+  }
+  // J[a][b] = d x_a / d xi_b from the mapping nodes (MappingQ(k) with the FE's nodes)
+  double X[512], v[512], tmp[512], gr[3][512];
+  double J[512][3][3];
+  for (int a = 0; a < 3; ++a) {
+    for (int l = 0; l < n; ++l)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i)
+          X[i + n * (j + n * l)] = coords[a * m.ndofs + m.cell_dof(cx, cy, cz, i, j, l)];
+    eval_values(fe, X, v, tmp);
+    eval_gradients(fe, v, gr[0], gr[1], gr[2]);
+    for (int q = 0; q < nq; ++q)
+      for (int b = 0; b < 3; ++b) J[q][a][b] = gr[b][q];
+  }
+  for (int qz = 0; qz < n; ++qz)
+    for (int qy = 0; qy < n; ++qy)
+      for (int qx = 0; qx < n; ++qx) {
+        const int q = qx + n * (qy + n * qz);
+        const double(*A)[3] = J[q];
+        const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+                           A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                           A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+        const double id = 1.0 / det;
+        double(*B)[3] = g.Jinv[q];
+        B[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) * id;
+        B[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) * id;
+        B[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * id;
+        B[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) * id;
+        B[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) * id;
+        B[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * id;
+        B[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) * id;
+        B[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) * id;
+        B[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * id;
+        g.JxW[q] = std::fabs(det) * fe.qw[qx] * fe.qw[qy] * fe.qw[qz];
+      }
+}
+
+// Conductivity at the quadrature points of one cell
+void cell_kappa(const orc_op *op, const FE1D &fe, const Mesh &m, int cx, int cy, int cz,
+                double *kq) {
+  const int n = fe.n, nq = n * n * n;
+  const long c = m.cell(cx, cy, cz);
+  switch (op->kappa_mode) {
+    case ORC_KAPPA_CONST:
+      for (int q = 0; q < nq; ++q) kq[q] = op->kappa;
+      break;
+    case ORC_KAPPA_QP:
+      for (int q = 0; q < nq; ++q) kq[q] = op->kappa_arr[c * nq + q];
+      break;
+    case ORC_KAPPA_CELL:
+      for (int q = 0; q < nq; ++q) kq[q] = op->kappa_arr[c];
+      break;
+    case ORC_KAPPA_LAW: {
+      double u[512], v[512], tmp[512];
+      for (int l = 0; l < n; ++l)
+        for (int j = 0; j < n; ++j)
+          for (int i = 0; i < n; ++i) u[i + n * (j + n * l)] = op->T[m.cell_dof(cx, cy, cz, i, j, l)];
+      eval_values(fe, u, v, tmp);
+      for (int q = 0; q < nq; ++q) {
+        double k, dk;
+        kappa_law(op->law, op->law_p, v[q], k, dk);
+        kq[q] = k;
+      }
+      break;
+    }
+  }
+}
+
+// Local tangent cell operation (heat_diffuse_operator.cpp:957-1086, for the
+// terms active in the benchmark configurations: val = (rho cp/dt) u,
+// flux = k grad u, lines :1034-1036, :1076-1079).
+void local_cell_apply(const orc_op *op, const FE1D &fe, const Mesh &m, int cx, int cy, int cz,
+                      const double *u, double *out, CellGeom &geo) {
+  const int n = fe.n, nq = n * n * n;
+  double v[512], tmp[512], g[3][512], kq[512];
+  cell_geometry(fe, m, op->coords, cx, cy, cz, geo);
+  cell_kappa(op, fe, m, cx, cy, cz, kq);
+  eval_values(fe, u, v, tmp);
+  eval_gradients(fe, v, g[0], g[1], g[2]);
+  for (int q = 0; q < nq; ++q) {
+    const double(*B)[3] = geo.Jinv[q];
+    double gp[3];
+    for (int b = 0; b < 3; ++b) gp[b] = B[0][b] * g[0][q] + B[1][b] * g[1][q] + B[2][b] * g[2][q];
+    const double s = kq[q] * geo.JxW[q];
+    for (int a = 0; a < 3; ++a)
+      g[a][q] = s * (B[a][0] * gp[0] + B[a][1] * gp[1] + B[a][2] * gp[2]);
+    v[q] *= op->c_mass * geo.JxW[q];
+  }
+  // integrate: out = S^T (v + D^T g)
+  sweep<true, true>(n, 0, fe.D, g[0], v);
+  sweep<true, true>(n, 1, fe.D, g[1], v);
+  sweep<true, true>(n, 2, fe.D, g[2], v);
+  sweep<true, false>(n, 0, fe.S, v, tmp);
+  sweep<true, false>(n, 1, fe.S, tmp, v);
+  sweep<true, false>(n, 2, fe.S, v, out);
+}
+
+// ---------------------------------------------------------------------------
+// Boundary faces (heat_diffuse_operator.cpp:245-269, :1090-1129 tangent;
+// :721-781 residual). Face f: direction d = f/2, side s = f%2.
+// ---------------------------------------------------------------------------
+struct FaceGeom {
+  double JxW[64];
+  double x[64], y[64];  // physical x, y at face quadrature points (laser)
+};
+
+// tangential directions of a face normal to d
+inline void face_dirs(int d, int &a, int &b) {
+  a = d == 0 ? 1 : 0;
+  b = d == 2 ? 1 : 2;
+}
+
+void face_geometry(const FE1D &fe, const Mesh &m, const double *coords, int cx, int cy, int cz,
+                   int f, FaceGeom &g) {
+  const int n = fe.n, d = f / 2, s = f % 2;
+  int ta, tb;
+  face_dirs(d, ta, tb);
+  const int cc[3] = {cx, cy, cz};
+  for (int qb = 0; qb < n; ++qb)
+    for (int qa = 0; qa < n; ++qa) {
+      const int q = qa + n * qb;
+      if (!coords) {
+        g.JxW[q] = m.h[ta] * m.h[tb] * fe.qw[qa] * fe.qw[qb];
+        double p[3];
+        p[d] = (cc[d] + s) * m.h[d];
+        p[ta] = (cc[ta] + fe.qp[qa]) * m.h[ta];
+        p[tb] = (cc[tb] + fe.qp[qb]) * m.h[tb];
+        g.x[q] = p[0];
+        g.y[q] = p[1];
+        continue;
+      }
+      // tangential derivatives of the mapping on the face
+      double ta_v[3] = {0, 0, 0}, tb_v[3] = {0, 0, 0}, pos[3] = {0, 0, 0};
+      for (int ib = 0; ib < n; ++ib)
+        for (int ia = 0; ia < n; ++ia) {
+          int idx[3];
+          idx[d] = s * fe.k;
+          idx[ta] = ia;
+          idx[tb] = ib;
+          const long dof = m.cell_dof(cx, cy, cz, idx[0], idx[1], idx[2]);
+          for (int c = 0; c < 3; ++c) {
+            const double X = coords[c * m.ndofs + dof];
+            ta_v[c] += X * fe.G[qa][ia] * fe.S[qb][ib];
+            tb_v[c] += X * fe.S[qa][ia] * fe.G[qb][ib];
+            pos[c] += X * fe.S[qa][ia] * fe.S[qb][ib];
+          }
+        }
+      const double cr[3] = {ta_v[1] * tb_v[2] - ta_v[2] * tb_v[1],
+                            ta_v[2] * tb_v[0] - ta_v[0] * tb_v[2],
+                            ta_v[0] * tb_v[1] - ta_v[1] * tb_v[0]};
+      g.JxW[q] = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]) * fe.qw[qa] * fe.qw[qb];
+      g.x[q] = pos[0];
+      g.y[q] = pos[1];
+    }
+}
+
+// gather the face DoFs of cell (cx,cy,cz) on face f: out[ia + n*ib]
+inline void face_gather(const FE1D &fe, const Mesh &m, int cx, int cy, int cz, int f,
+                        const double *vec, double *out, long *dofs) {
+  const int n = fe.n, d = f / 2, s = f % 2;
+  int ta, tb;
+  face_dirs(d, ta, tb);
+  for (int ib = 0; ib < n; ++ib)
+    for (int ia = 0; ia < n; ++ia) {
+      int idx[3];
+      idx[d] = s * fe.k;
+      idx[ta] = ia;
+      idx[tb] = ib;
+      const long dof = m.cell_dof(cx, cy, cz, idx[0], idx[1], idx[2]);
+      if (dofs) dofs[ia + n * ib] = dof;
+      if (out) out[ia + n * ib] = vec ? vec[dof] : 0.0;
+    }
+}
+
+inline void face_values(const FE1D &fe, const double *u, double *v) {
+  const int n = fe.n;
+  double t[64];
+  for (int ib = 0; ib < n; ++ib)
+    for (int qa = 0; qa < n; ++qa) {
+      double acc = 0;
+      for (int ia = 0; ia < n; ++ia) acc += fe.S[qa][ia] * u[ia + n * ib];
+      t[qa + n * ib] = acc;
+    }
+  for (int qb = 0; qb < n; ++qb)
+    for (int qa = 0; qa < n; ++qa) {
+      double acc = 0;
+      for (int ib = 0; ib < n; ++ib) acc += fe.S[qb][ib] * t[qa + n * ib];
+      v[qa + n * qb] = acc;
+    }
+}
+inline void face_integrate(const FE1D &fe, const double *v, double *out) {
+  const int n = fe.n;
+  double t[64];
+  for (int qb = 0; qb < n; ++qb)
+    for (int ia = 0; ia < n; ++ia) {
+      double acc = 0;
+      for (int qa = 0; qa < n; ++qa) acc += fe.S[qa][ia] * v[qa + n * qb];
+      t[ia + n * qb] = acc;
+    }
+  for (int ib = 0; ib < n; ++ib)
+    for (int ia = 0; ia < n; ++ia) {
+      double acc = 0;
+      for (int qb = 0; qb < n; ++qb) acc += fe.S[qb][ib] * t[ia + n * qb];
+      out[ia + n * ib] = acc;
+    }
+}
+
+template <typename F>
+void for_each_boundary_face(const Mesh &m, int mask, F &&fn) {
+  for (int f = 0; f < 6; ++f) {
+    if (!(mask & (1 << f))) continue;
+    const int d = f / 2, s = f % 2;
+    int ta, tb;
+    face_dirs(d, ta, tb);
+    for (int b = 0; b < m.nc[tb]; ++b)
+      for (int a = 0; a < m.nc[ta]; ++a) {
+        int c[3];
+        c[d] = s ? m.nc[d] - 1 : 0;
+        c[ta] = a;
+        c[tb] = b;
+        fn(f, c[0], c[1], c[2]);
+      }
+  }
+}
+
+// Robin tangent on boundary faces: (alpha + 4 eps sigma T^3) u
+// (heat_diffuse_operator.cpp:1112-1127)
+void boundary_tangent_add(const orc_op *op, const FE1D &fe, const Mesh &m, double *dst,
+                          const double *src) {
+  if (!op->robin_mask) return;
+  const int n = fe.n;
+  for_each_boundary_face(m, op->robin_mask, [&](int f, int cx, int cy, int cz) {
+    double u[64], uq[64], Tq[64], Tv[64], out[64];
+    long dofs[64];
+    FaceGeom g;
+    face_geometry(fe, m, op->coords, cx, cy, cz, f, g);
+    face_gather(fe, m, cx, cy, cz, f, src, u, dofs);
+    for (int i = 0; i < n * n; ++i) {
+      const int ix = dofs[i] % m.nd[0], iy = (dofs[i] / m.nd[0]) % m.nd[1],
+                iz = dofs[i] / ((long)m.nd[0] * m.nd[1]);
+      if (m.constrained(op->dirichlet_mask, ix, iy, iz)) u[i] = 0.0;
+    }
+    face_values(fe, u, uq);
+    if (op->emissivity > 0.0) {
+      face_gather(fe, m, cx, cy, cz, f, op->T, Tv, nullptr);
+      face_values(fe, Tv, Tq);
+    }
+    for (int q = 0; q < n * n; ++q) {
+      double val = 0.0;
+      if (op->alpha > 0.0) val += op->alpha * uq[q];
+      if (op->emissivity > 0.0)
+        val += 4.0 * op->emissivity * STEFAN_BOLTZMANN * Tq[q] * Tq[q] * Tq[q] * uq[q];
+      uq[q] = val * g.JxW[q];
+    }
+    face_integrate(fe, uq, out);
+    for (int i = 0; i < n * n; ++i) {
+      const int ix = dofs[i] % m.nd[0], iy = (dofs[i] / m.nd[0]) % m.nd[1],
+                iz = dofs[i] / ((long)m.nd[0] * m.nd[1]);
+      if (!m.constrained(op->dirichlet_mask, ix, iy, iz)) dst[dofs[i]] += out[i];
+    }
+  });
+}
+
+// Cell loop with 8-colour scheduling so threads never write the same DoF.
+template <typename F>
+void colored_cell_loop(const Mesh &m, int n_threads, F &&fn) {
+  for (int col = 0; col < 8; ++col) {
+    const int px = col & 1, py = (col >> 1) & 1, pz = (col >> 2) & 1;
+    const int nx = (m.nc[0] - px + 1) / 2, ny = (m.nc[1] - py + 1) / 2, nz = (m.nc[2] - pz + 1) / 2;
+    const long total = (long)nx * ny * nz;
+    if (total <= 0) continue;
+#pragma omp parallel for schedule(static) num_threads(n_threads)
+    for (long t = 0; t < total; ++t) {
+      const int ax = t % nx, ay = (t / nx) % ny, az = t / ((long)nx * ny);
+      fn(px + 2 * ax, py + 2 * ay, pz + 2 * az);
+    }
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+void orc_fe1d(int degree, double *nodes, double *qp, double *qw, double *S, double *G,
+              double *D) {
+  FE1D fe(degree);
+  const int n = fe.n;
+  for (int i = 0; i < n; ++i) {
+    nodes[i] = fe.nodes[i];
+    qp[i] = fe.qp[i];
+    qw[i] = fe.qw[i];
+    for (int j = 0; j < n; ++j) {
+      S[i * n + j] = fe.S[i][j];
+      G[i * n + j] = fe.G[i][j];
+      D[i * n + j] = fe.D[i][j];
+    }
+  }
+}
+
+long orc_n_dofs(const orc_op *op) { return Mesh(op).ndofs; }
+long orc_n_cells(const orc_op *op) { return Mesh(op).ncells; }
+
+void orc_make_coords(const orc_op *op, int kind, double amp, double *coords) {
+  Mesh m(op);
+  FE1D fe(op->degree);
+  // node position along one direction: cell c, local i -> (c + nodes[i]) h, for
+  // global index g = k c + i (any cell containing the node gives the same value)
+  auto pos = [&](int d, int g) {
+    const int c = std::min(g / m.k, m.nc[d] - 1);
+    return (c + fe.nodes[g - m.k * c]) * m.h[d];
+  };
+  for (int iz = 0; iz < m.nd[2]; ++iz)
+    for (int iy = 0; iy < m.nd[1]; ++iy)
+      for (int ix = 0; ix < m.nd[0]; ++ix) {
+        const long i = m.dof(ix, iy, iz);
+        const double x = pos(0, ix), y = pos(1, iy), z = pos(2, iz);
+        double X = x;
+        if (kind == 1) X += amp * std::sin(2.0 * PI * y / op->L[1]) * std::sin(2.0 * PI * z / op->L[2]);
+        coords[i] = X;
+        coords[m.ndofs + i] = y;
+        coords[2 * m.ndofs + i] = z;
+      }
+}
+
+// HeatDiffuseMultiPhaseOperator::vmult (heat_diffuse_operator.cpp:181-197):
+// MatrixFree::loop with zero_dst, cell loop :201-241 (read_dof_values honours the
+// homogeneous Dirichlet constraints, distribute_local_to_global skips them),
+// boundary loop :245-269.
+void orc_vmult(const orc_op *op, double *dst, const double *src, int n_threads) {
+  Mesh m(op);
+  FE1D fe(op->degree);
+  const int n = fe.n;
+  std::fill(dst, dst + m.ndofs, 0.0);
+  colored_cell_loop(m, n_threads, [&](int cx, int cy, int cz) {
+    static thread_local CellGeom geo;
+    double u[512], out[512];
+    long idx[512];
+    bool con[512];
+    for (int l = 0; l < n; ++l)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          const int a = i + n * (j + n * l);
+          const int ix = m.k * cx + i, iy = m.k * cy + j, iz = m.k * cz + l;
+          idx[a] = m.dof(ix, iy, iz);
+          con[a] = m.constrained(op->dirichlet_mask, ix, iy, iz);
+          u[a] = con[a] ? 0.0 : src[idx[a]];
+        }
+    local_cell_apply(op, fe, m, cx, cy, cz, u, out, geo);
+    for (int a = 0; a < n * n * n; ++a)
+      if (!con[a]) dst[idx[a]] += out[a];
+  });
+  boundary_tangent_add(op, fe, m, dst, src);
+}
+
+// Explicit element matrices: A_e[I][J] = sum_q JxW (c phi_I phi_J + k grad phi_I . grad phi_J),
+// with phi and grad phi evaluated directly from the 1D Lagrange tables and the
+// Jacobian summed directly from the node coordinates (no sum factorisation).
+void orc_vmult_brute(const orc_op *op, double *dst, const double *src) {
+  Mesh m(op);
+  FE1D fe(op->degree);
+  const int n = fe.n, nd = n * n * n;
+  std::fill(dst, dst + m.ndofs, 0.0);
+  std::vector<double> A(nd * nd);
+  std::vector<double> phi(nd), dphi(3 * nd);
+  for (int cz = 0; cz < m.nc[2]; ++cz)
+    for (int cy = 0; cy < m.nc[1]; ++cy)
+      for (int cx = 0; cx < m.nc[0]; ++cx) {
+        std::fill(A.begin(), A.end(), 0.0);
+        long idx[512];
+        bool con[512];
+        for (int l = 0; l < n; ++l)
+          for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+              const int a = i + n * (j + n * l);
+              const int ix = m.k * cx + i, iy = m.k * cy + j, iz = m.k * cz + l;
+              idx[a] = m.dof(ix, iy, iz);
+              con[a] = m.constrained(op->dirichlet_mask, ix, iy, iz);
+            }
+        const long c = m.cell(cx, cy, cz);
+        for (int qz = 0; qz < n; ++qz)
+          for (int qy = 0; qy < n; ++qy)
+            for (int qx = 0; qx < n; ++qx) {
+              const int q = qx + n * (qy + n * qz);
+              for (int l = 0; l < n; ++l)
+                for (int j = 0; j < n; ++j)
+                  for (int i = 0; i < n; ++i) {
+                    const int a = i + n * (j + n * l);
+                    phi[a] = fe.S[qx][i] * fe.S[qy][j] * fe.S[qz][l];
+                    dphi[3 * a + 0] = fe.G[qx][i] * fe.S[qy][j] * fe.S[qz][l];
+                    dphi[3 * a + 1] = fe.S[qx][i] * fe.G[qy][j] * fe.S[qz][l];
+                    dphi[3 * a + 2] = fe.S[qx][i] * fe.S[qy][j] * fe.G[qz][l];
+                  }
+              double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+              if (op->coords) {
+                for (int a = 0; a < nd; ++a)
+                  for (int r = 0; r < 3; ++r) {
+                    const double X = op->coords[r * m.ndofs + idx[a]];
+                    for (int b = 0; b < 3; ++b) J[r][b] += X * dphi[3 * a + b];
+                  }
+              } else {
+                for (int r = 0; r < 3; ++r) J[r][r] = m.h[r];
+              }
+              const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                                 J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                                 J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+              double B[3][3];  // inverse Jacobian
+              B[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+              B[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+              B[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+              B[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+              B[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+              B[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+              B[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+              B[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+              B[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+              const double JxW = std::fabs(det) * fe.qw[qx] * fe.qw[qy] * fe.qw[qz];
+              double kap = 0.0;
+              switch (op->kappa_mode) {
+                case ORC_KAPPA_CONST: kap = op->kappa; break;
+                case ORC_KAPPA_QP: kap = op->kappa_arr[c * nd + q]; break;
+                case ORC_KAPPA_CELL: kap = op->kappa_arr[c]; break;
+                case ORC_KAPPA_LAW: {
+                  double Tq = 0.0;
+                  for (int a = 0; a < nd; ++a) Tq += op->T[idx[a]] * phi[a];
+                  double dk;
+                  kappa_law(op->law, op->law_p, Tq, kap, dk);
+                  break;
+                }
+              }
+              // physical gradients
+              std::vector<double> gp(3 * nd);
+              for (int a = 0; a < nd; ++a)
+                for (int r = 0; r < 3; ++r)
+                  gp[3 * a + r] = B[0][r] * dphi[3 * a + 0] + B[1][r] * dphi[3 * a + 1] +
+                                  B[2][r] * dphi[3 * a + 2];
+              for (int I = 0; I < nd; ++I)
+                for (int Jj = 0; Jj < nd; ++Jj)
+                  A[I * nd + Jj] += JxW * (op->c_mass * phi[I] * phi[Jj] +
+                                           kap * (gp[3 * I] * gp[3 * Jj] + gp[3 * I + 1] * gp[3 * Jj + 1] +
+                                                  gp[3 * I + 2] * gp[3 * Jj + 2]));
+            }
+        for (int I = 0; I < nd; ++I) {
+          if (con[I]) continue;
+          double acc = 0.0;
+          for (int Jj = 0; Jj < nd; ++Jj)
+            if (!con[Jj]) acc += A[I * nd + Jj] * src[idx[Jj]];
+          dst[idx[I]] += acc;
+        }
+      }
+  boundary_tangent_add(op, fe, m, dst, src);
+}
+
+// Diagonal of the operator: what MatrixFreeTools::compute_diagonal produces for the
+// local operation (heat_diffuse_operator.cpp:332-358), computed here element-wise
+// from the same quantities: d_I = sum_q JxW (c phi_I^2 + k |grad phi_I|^2) + face terms.
+void orc_diagonal(const orc_op *op, double *diag) {
+  Mesh m(op);
+  FE1D fe(op->degree);
+  const int n = fe.n, nd = n * n * n;
+  std::fill(diag, diag + m.ndofs, 0.0);
+  colored_cell_loop(m, 1, [&](int cx, int cy, int cz) {
+    static thread_local CellGeom geo;
+    double kq[512];
+    cell_geometry(fe, m, op->coords, cx, cy, cz, geo);
+    cell_kappa(op, fe, m, cx, cy, cz, kq);
+    for (int l = 0; l < n; ++l)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          double acc = 0.0;
+          for (int qz = 0; qz < n; ++qz)
+            for (int qy = 0; qy < n; ++qy)
+              for (int qx = 0; qx < n; ++qx) {
+                const int q = qx + n * (qy + n * qz);
+                const double ph = fe.S[qx][i] * fe.S[qy][j] * fe.S[qz][l];
+                const double dr[3] = {fe.G[qx][i] * fe.S[qy][j] * fe.S[qz][l],
+                                      fe.S[qx][i] * fe.G[qy][j] * fe.S[qz][l],
+                                      fe.S[qx][i] * fe.S[qy][j] * fe.G[qz][l]};
+                const double(*B)[3] = geo.Jinv[q];
+                double gp2 = 0.0;
+                for (int r = 0; r < 3; ++r) {
+                  const double gr = B[0][r] * dr[0] + B[1][r] * dr[1] + B[2][r] * dr[2];
+                  gp2 += gr * gr;
+                }
+                acc += geo.JxW[q] * (op->c_mass * ph * ph + kq[q] * gp2);
+              }
+          diag[m.cell_dof(cx, cy, cz, i, j, l)] += acc;
+        }
+  });
+  (void)nd;
+  if (op->robin_mask) {
+    for_each_boundary_face(m, op->robin_mask, [&](int f, int cx, int cy, int cz) {
+      double Tq[64], Tv[64];
+      long dofs[64];
+      FaceGeom g;
+      face_geometry(fe, m, op->coords, cx, cy, cz, f, g);
+      face_gather(fe, m, cx, cy, cz, f, nullptr, nullptr, dofs);
+      if (op->emissivity > 0.0) {
+        face_gather(fe, m, cx, cy, cz, f, op->T, Tv, nullptr);
+        face_values(fe, Tv, Tq);
+      }
+      for (int ib = 0; ib < n; ++ib)
+        for (int ia = 0; ia < n; ++ia) {
+          double acc = 0.0;
+          for (int qb = 0; qb < n; ++qb)
+            for (int qa = 0; qa < n; ++qa) {
+              const int q = qa + n * qb;
+              double c = 0.0;
+              if (op->alpha > 0.0) c += op->alpha;
+              if (op->emissivity > 0.0) c += 4.0 * op->emissivity * STEFAN_BOLTZMANN * Tq[q] * Tq[q] * Tq[q];
+              const double ph = fe.S[qa][ia] * fe.S[qb][ib];
+              acc += c * ph * ph * g.JxW[q];
+            }
+          diag[dofs[ia + n * ib]] += acc;
+        }
+    });
+  }
+  if (op->dirichlet_mask)
+    for (int iz = 0; iz < m.nd[2]; ++iz)
+      for (int iy = 0; iy < m.nd[1]; ++iy)
+        for (int ix = 0; ix < m.nd[0]; ++ix)
+          if (m.constrained(op->dirichlet_mask, ix, iy, iz)) diag[m.dof(ix, iy, iz)] = 1.0;
+}
+
+// compute_inverse_diagonal_from_matrixfree (heat_diffuse_operator.cpp:273-284)
+void orc_inverse_diagonal(const orc_op *op, double *inv) {
+  Mesh m(op);
+  orc_diagonal(op, inv);
+  for (long i = 0; i < m.ndofs; ++i) inv[i] = std::fabs(inv[i]) > 1.0e-10 ? 1.0 / inv[i] : 1.0;
+}
+
+// create_rhs (heat_diffuse_operator.cpp:785-806): rhs_cell_loop :395-529 and
+// rhs_boundary_loop :721-781; dst is accumulated (zero_dst = false).
+void orc_create_rhs(const orc_op *op, double *dst, const double *T, const double *T_old,
+                    const double *q_vol) {
+  Mesh m(op);
+  FE1D fe(op->degree);
+  const int n = fe.n, nq = n * n * n;
+  orc_op opT = *op;
+  opT.T = T;  // conductivity law evaluated at the current temperature
+  colored_cell_loop(m, 1, [&](int cx, int cy, int cz) {
+    static thread_local CellGeom geo;
+    double t[512], to[512], qs[512], v[512], vo[512], vq[512], tmp[512], g[3][512], kq[512],
+        out[512];
+    long idx[512];
+    bool con[512];
+    for (int l = 0; l < n; ++l)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          const int a = i + n * (j + n * l);
+          const int ix = m.k * cx + i, iy = m.k * cy + j, iz = m.k * cz + l;
+          idx[a] = m.dof(ix, iy, iz);
+          con[a] = m.constrained(op->dirichlet_mask, ix, iy, iz);
+          t[a] = T[idx[a]];  // read_dof_values_plain
+          to[a] = T_old[idx[a]];
+          qs[a] = q_vol ? q_vol[idx[a]] : 0.0;
+        }
+    cell_geometry(fe, m, op->coords, cx, cy, cz, geo);
+    cell_kappa(&opT, fe, m, cx, cy, cz, kq);
+    eval_values(fe, t, v, tmp);
+    eval_gradients(fe, v, g[0], g[1], g[2]);
+    eval_values(fe, to, vo, tmp);
+    eval_values(fe, qs, vq, tmp);
+    for (int q = 0; q < nq; ++q) {
+      const double(*B)[3] = geo.Jinv[q];
+      double gp[3];
+      for (int b = 0; b < 3; ++b) gp[b] = B[0][b] * g[0][q] + B[1][b] * g[1][q] + B[2][b] * g[2][q];
+      const double s = -kq[q] * geo.JxW[q];
+      for (int a = 0; a < 3; ++a) g[a][q] = s * (B[a][0] * gp[0] + B[a][1] * gp[1] + B[a][2] * gp[2]);
+      const double val = op->c_mass * (v[q] - vo[q]) - vq[q];
+      v[q] = -val * geo.JxW[q];
+    }
+    sweep<true, true>(n, 0, fe.D, g[0], v);
+    sweep<true, true>(n, 1, fe.D, g[1], v);
+    sweep<true, true>(n, 2, fe.D, g[2], v);
+    sweep<true, false>(n, 0, fe.S, v, tmp);
+    sweep<true, false>(n, 1, fe.S, tmp, v);
+    sweep<true, false>(n, 2, fe.S, v, out);
+    for (int a = 0; a < nq; ++a)
+      if (!con[a]) dst[idx[a]] += out[a];
+  });
+  int mask = op->robin_mask | (op->laser_on ? 32 : 0);
+  if (!mask) return;
+  for_each_boundary_face(m, mask, [&](int f, int cx, int cy, int cz) {
+    double Tv[64], Tq[64], val[64], out[64];
+    long dofs[64];
+    FaceGeom g;
+    face_geometry(fe, m, op->coords, cx, cy, cz, f, g);
+    face_gather(fe, m, cx, cy, cz, f, T, Tv, dofs);
+    face_values(fe, Tv, Tq);
+    const bool robin = op->robin_mask & (1 << f);
+    for (int q = 0; q < n * n; ++q) {
+      double v = 0.0;
+      if (op->laser_on && f == 5) {
+        const double *p = op->laser_p;
+        const double dx = g.x[q] - p[2] - p[4] * p[5], dy = g.y[q] - p[3];
+        v += p[0] * std::exp(-(dx * dx + dy * dy) / (2.0 * p[1] * p[1]));
+      }
+      if (robin && op->emissivity > 0.0) {
+        const double T4 = Tq[q] * Tq[q] * Tq[q] * Tq[q];
+        const double Ti = op->T_inf_rad;
+        v -= op->emissivity * STEFAN_BOLTZMANN * (T4 - Ti * Ti * Ti * Ti);
+      }
+      if (robin && op->alpha > 0.0) v -= op->alpha * (Tq[q] - op->T_inf_conv);
+      val[q] = v * g.JxW[q];
+    }
+    face_integrate(fe, val, out);
+    for (int i = 0; i < n * n; ++i) {
+      const int ix = dofs[i] % m.nd[0], iy = (dofs[i] / m.nd[0]) % m.nd[1],
+                iz = dofs[i] / ((long)m.nd[0] * m.nd[1]);
+      if (!m.constrained(op->dirichlet_mask, ix, iy, iz)) dst[dofs[i]] += out[i];
+    }
+  });
+}
+
+// VectorTools::compute_norm (source/utilities/vector_tools.cpp:170-200): L2 norm of
+// the FE function with the operator's quadrature and mapping.
+double orc_l2_norm(const orc_op *op, const double *u) {
+  Mesh m(op);
+  FE1D fe(op->degree);
+  const int n = fe.n, nq = n * n * n;
+  double sum = 0.0;
+  CellGeom *geo = new CellGeom;
+  for (int cz = 0; cz < m.nc[2]; ++cz)
+    for (int cy = 0; cy < m.nc[1]; ++cy)
+      for (int cx = 0; cx < m.nc[0]; ++cx) {
+        double a[512], v[512], tmp[512];
+        for (int l = 0; l < n; ++l)
+          for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) a[i + n * (j + n * l)] = u[m.cell_dof(cx, cy, cz, i, j, l)];
+        eval_values(fe, a, v, tmp);
+        cell_geometry(fe, m, op->coords, cx, cy, cz, *geo);
+        for (int q = 0; q < nq; ++q) sum += v[q] * v[q] * geo->JxW[q];
+      }
+  delete geo;
+  return std::sqrt(sum);
+}
+
+void orc_kappa_law(int law, const double *p, double T, double *out) {
+  kappa_law(law, p, T, out[0], out[1]);
+}
+
+void orc_interpolate_qp(const orc_op *op, const double *u, double *u_qp) {
+  Mesh m(op);
+  FE1D fe(op->degree);
+  const int n = fe.n, nq = n * n * n;
+  for (int cz = 0; cz < m.nc[2]; ++cz)
+    for (int cy = 0; cy < m.nc[1]; ++cy)
+      for (int cx = 0; cx < m.nc[0]; ++cx) {
+        double a[512], tmp[512];
+        for (int l = 0; l < n; ++l)
+          for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) a[i + n * (j + n * l)] = u[m.cell_dof(cx, cy, cz, i, j, l)];
+        eval_values(fe, a, u_qp + m.cell(cx, cy, cz) * nq, tmp);
+      }
+}
+
+}  // extern "C"
+
+#include "mpdg_oracle_solvers.inc"

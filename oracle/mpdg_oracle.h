@@ -1,0 +1,131 @@
+/*
+ * CPU oracle for the matrix-free heat operator path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker: it restates, on the
+ * CPU and in FP64, the algorithm of MeltPoolDG's implicit heat operator
+ * (HeatDiffuseMultiPhaseOperator, source/heat/heat_diffuse_operator.cpp) and
+ * of the solver stack around it (LinearSolver, JacobiPreconditioner,
+ * NewtonRaphsonSolver, Predictor, the GMG recipe of unit_tests/gmg.cc).
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it. The product path (meltpooldg_amd) never links or calls it.
+ *
+ * Discretisation: structured hexahedral mesh of nc[0] x nc[1] x nc[2] cells on
+ * the box [0,L0]x[0,L1]x[0,L2] (optionally mapped through node coordinates),
+ * continuous Q_k on Gauss-Lobatto nodes, QGauss(k+1) quadrature, lexicographic
+ * DoF numbering i = ix + nx*(iy + ny*iz) with nx = k*nc[0]+1 (x fastest).
+ * Faces are numbered 0:x-, 1:x+, 2:y-, 3:y+, 4:z-, 5:z+ (the colorized
+ * hyper_rectangle boundary ids of deal.II, which the reference uses).
+ */
+#ifndef MPDG_ORACLE_H
+#define MPDG_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* conductivity modes */
+#define ORC_KAPPA_CONST 0
+#define ORC_KAPPA_QP 1
+#define ORC_KAPPA_CELL 2
+#define ORC_KAPPA_LAW 3
+
+/* conductivity laws k(T) */
+#define ORC_LAW_TANH 1   /* k0*(1 + a*tanh((T-T0)/w)); p = {k0, a, T0, w} */
+#define ORC_LAW_MUSHY 2  /* liquid_solid mushy zone; p = {k_liquid, k_solid, T_solidus, T_liquidus} */
+
+typedef struct orc_op {
+  int degree;            /* k */
+  int nc[3];             /* cells per direction */
+  double L[3];           /* box extents */
+  const double *coords;  /* NULL: affine box; else 3*n_dofs node coordinates (X block, Y block, Z block) */
+  int geom_kind;         /* how coords were made (orc_make_coords kind), used to build coarse levels */
+  double geom_amp;       /* amplitude of the kind-1 transform */
+  double c_mass;         /* rho*c_p/dt */
+  int kappa_mode;        /* ORC_KAPPA_* */
+  double kappa;          /* constant conductivity */
+  const double *kappa_arr; /* per qp (n_cells*n1^3) or per cell (n_cells) */
+  int law;               /* ORC_LAW_* for ORC_KAPPA_LAW */
+  double law_p[4];
+  const double *T;       /* temperature DoF vector: k(T) and the radiation linearisation */
+  int dirichlet_mask;    /* bit f: homogeneous Dirichlet on face f */
+  int robin_mask;        /* bit f: convection/radiation on face f */
+  double alpha;          /* convection coefficient */
+  double emissivity;     /* radiation emissivity */
+  double T_inf_conv;     /* convection ambient temperature */
+  double T_inf_rad;      /* radiation ambient temperature */
+  int laser_on;          /* Gaussian surface flux on face 5 (z+) in create_rhs */
+  double laser_p[6];     /* peak, sigma, x0, y0, vx, t : q = peak*exp(-((x-x0-vx*t)^2+(y-y0)^2)/(2 sigma^2)) */
+} orc_op;
+
+/* 1D tables: nodes[n1], qp[n1], qw[n1], S[q*n1+i] = phi_i(qp_q), G = phi_i'(qp_q),
+   D[q*n1+p] = derivative of the Gauss-point Lagrange basis l_p at qp_q. */
+void orc_fe1d(int degree, double *nodes, double *qp, double *qw, double *S, double *G, double *D);
+
+long orc_n_dofs(const orc_op *op);
+long orc_n_cells(const orc_op *op);
+
+/* node coordinates of the mapped box: X = x + amp*sin(2 pi y/L1)*sin(2 pi z/L2) (kind 1), identity (kind 0) */
+void orc_make_coords(const orc_op *op, int kind, double amp, double *coords);
+
+/* dst = J src (dst zeroed), sum factorised per cell, OpenMP over coloured cells */
+void orc_vmult(const orc_op *op, double *dst, const double *src, int n_threads);
+/* dst = J src from explicitly assembled element matrices (small meshes only) */
+void orc_vmult_brute(const orc_op *op, double *dst, const double *src);
+/* diagonal of J (constrained rows: 1) */
+void orc_diagonal(const orc_op *op, double *diag);
+/* inverse diagonal with the reference rule |d| > 1e-10 ? 1/d : 1 */
+void orc_inverse_diagonal(const orc_op *op, double *inv_diag);
+/* dst += -R(T): -(c_mass (T - T_old) - q) N - k grad T . grad N  + boundary residuals */
+void orc_create_rhs(const orc_op *op, double *dst, const double *T, const double *T_old,
+                    const double *q_vol);
+/* L2 norm of the FE function u over the mesh with QGauss(k+1) */
+double orc_l2_norm(const orc_op *op, const double *u);
+/* conductivity law at scalar T: out[0] = k, out[1] = dk/dT */
+void orc_kappa_law(int law, const double *p, double T, double *out);
+/* T interpolated at all quadrature points, layout cell*n1^3 + qx + n1*(qy + n1*qz) */
+void orc_interpolate_qp(const orc_op *op, const double *u, double *u_qp);
+
+/* Krylov solve J x = b.
+   precond: 0 identity, 1 inverse diagonal, 2 GMG (orc_gmg handle in gmg).
+   criterion: 0 reference ReductionControl (||r|| <= max(abs_tol, rel_tol*||r0||)),
+              1 ||b - J x|| <= rel_tol*||b||.
+   Returns iterations (negative if not converged); res_out = final ||r||. */
+int orc_cg(const orc_op *op, const double *b, double *x, double rel_tol, double abs_tol,
+           int max_it, int precond, void *gmg, int criterion, double *res_out, int n_threads);
+
+/* Geometric multigrid (FP64): h-coarsening by 2 down to nc_coarse cells in x
+   (all directions scaled alike), optional p-coarsening to coarse_degree on the
+   coarsest mesh, Chebyshev(degree cheb_degree, smoothing range, eig_iters CG
+   steps) with inverse diagonal, dense Cholesky coarse solve. Level
+   conductivity: cell averages of the fine operator's conductivity. */
+void *orc_gmg_create(const orc_op *fine, int nc_coarse, int coarse_degree, int cheb_degree,
+                     double smoothing_range, int eig_iters, int n_threads);
+void orc_gmg_vcycle(void *gmg, double *x, const double *b);
+int orc_gmg_n_levels(void *gmg);
+void orc_gmg_level_info(void *gmg, int level, int *degree, int *nc, double *lambda_max);
+void orc_gmg_destroy(void *gmg);
+/* building blocks exposed for tests: transfer between two structured levels */
+void orc_prolongate(int kc, const int *ncc, int kf, const int *ncf, const double *coarse,
+                    double *fine);
+void orc_restrict(int kc, const int *ncc, int kf, const int *ncf, const double *fine,
+                  double *coarse);
+/* Chebyshev smoother step on one operator: x <- smoothed x (x_is_zero: skip first residual) */
+void orc_chebyshev(const orc_op *op, const double *inv_diag, double lambda_max,
+                   double smoothing_range, int degree, double *x, const double *b,
+                   int x_is_zero);
+/* largest eigenvalue estimate of D^-1 J by eig_iters CG steps (Lanczos tridiagonal) */
+double orc_estimate_lambda_max(const orc_op *op, const double *inv_diag, int eig_iters);
+
+/* Implicit heat time stepping as HeatDiffuseOperation::solve with Newton-Raphson
+   (source/linear_algebra/newton_raphson_solver.cpp) and CG + Jacobi.
+   predictor: 0 none, 1 zero, 2 linear_extrapolation. T (in/out) holds T_0 on
+   entry, T_n on exit; norms[step] = ||T||_L2 after each step;
+   lin_iters[step] = accumulated Krylov iterations of that step. */
+int orc_heat_run(orc_op *op, double rho_cp, double dt, int n_steps, int predictor,
+                 double lin_rel_tol, double nl_res_tol, double nl_corr_tol, int nl_max_it,
+                 double *T, double *norms, int *lin_iters, int n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
