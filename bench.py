@@ -1,0 +1,176 @@
+"""Benchmark of the matrix-free heat operator path on one MI355X.
+
+A step is one FP64 operator apply (vmult) of the Q3 heat operator
+J = (rho c_p/dt) M + K_k on 128^3 cells (57,066,625 DoFs, 134,217,728
+quadrature points) with the conductivity k = k0 (1 + 0.5 tanh((T - 1)/0.1))
+given per quadrature point from T ~ U[0,2] (seed 2) and src ~ U[-1,1] (seed 3),
+inputs resident in HBM. Prints one JSON line (see the repository's
+benchmark contract): value = GDoF/s of the apply; roofline of the apply
+kernel from algorithmic bytes (16 B per DoF for src/dst + 8 B per
+quadrature point for k) over the kernel time measured with HIP events on the
+launch stream; cpu_baseline = the CPU oracle's apply (same algorithm, OpenMP)
+on a bounded sample. With --solve (default) it also times one heat-step
+solve to ||b - J x|| <= 1e-10 ||b|| with FP64 CG + FP64 GMG and with
+flexible CG + FP32 GMG.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cells", type=int, default=128)
+    ap.add_argument("--degree", type=int, default=3)
+    ap.add_argument("--solve", type=int, default=1)
+    ap.add_argument("--cpu", type=int, default=1)
+    ap.add_argument("--cpu-cells", type=int, default=40)
+    return ap.parse_args()
+
+
+def make_operator(torch, M, k, N, dt=1e-3):
+    n1 = k + 1
+    op = M.HeatDiffuseMultiPhaseOperator(k, (N, N, N), (1.0, 1.0, 1.0), rho_cp=1.0)
+    op.reset_time_increment(dt)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2)
+    Tq = torch.rand(op.n_q_points, dtype=torch.float64, device="cuda", generator=g) * 2.0
+    kq = 1.0 * (1.0 + 0.5 * torch.tanh((Tq - 1.0) / 0.1))
+    del Tq
+    op.set_conductivity_qp(kq)
+    return op, kq
+
+
+def cpu_baseline(k, N, threads):
+    """CPU oracle apply (sum factorisation, OpenMP over coloured cells) on a bounded sample."""
+    import numpy as np
+    from oracle import oracle as O
+    n1 = k + 1
+    rng = np.random.default_rng(2)
+    Tq = rng.uniform(0.0, 2.0, N ** 3 * n1 ** 3)
+    op = O.Operator(k, (N, N, N), kappa_qp=1.0 + 0.5 * np.tanh((Tq - 1.0) / 0.1), c_mass=1e3)
+    x = np.random.default_rng(3).uniform(-1.0, 1.0, op.n_dofs)
+    op.vmult(x, threads=threads)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        op.vmult(x, threads=threads)
+        reps += 1
+        el = time.perf_counter() - t0
+        if (reps >= 3 and el > 8.0) or reps >= 50:
+            break
+    return op.n_dofs * reps / el / 1e9, reps, el, op.n_dofs
+
+
+def solve_timings(torch, M, op, reps=2):
+    """Time-to-solve of one heat step (x0 = 0) to ||b - J x|| <= 1e-10 ||b||."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4)
+    b = torch.rand(op.n_dofs, dtype=torch.float64, device="cuda", generator=g) * 2.0 - 1.0
+    out = {}
+    for name, prec, stype in (("fp64_cg_gmg64", M.FP64, M.SOLVER_CG), ("fp64_fcg_gmg32", M.FP32, M.SOLVER_FCG)):
+        pc = M.GMGPreconditioner(op, precision=prec, n_cells_coarse=4, coarse_degree=2, smoother_degree=3)
+        t0 = time.perf_counter()
+        pc.update()
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        data = M.LinearSolverData(solver_type=stype, rel_tolerance=1e-10, criterion=M.CRIT_RHS, max_iterations=500)
+        best, its = None, None
+        for _ in range(reps):
+            x = torch.zeros_like(b)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            its = M.LinearSolver.solve(op, x, b, data, pc)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            best = el if best is None else min(best, el)
+        r = torch.empty_like(b)
+        op.vmult(r, x)
+        rel = float(torch.linalg.norm(b - r) / torch.linalg.norm(b))
+        out[name] = {"ms": round(best * 1e3, 3), "iterations": its, "final_rel_residual": rel,
+                     "precond_setup_s": round(setup, 3)}
+        del pc
+    out["speedup_fp32_gmg_vs_fp64_gmg"] = round(out["fp64_cg_gmg64"]["ms"] / out["fp64_fcg_gmg32"]["ms"], 3)
+    return out
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported (see DESIGN.md)" % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import meltpooldg_amd as M
+    k, N = a.degree, a.cells
+    op, kq = make_operator(torch, M, k, N)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    src = torch.rand(op.n_dofs, dtype=torch.float64, device="cuda", generator=g) * 2.0 - 1.0
+    dst = torch.empty_like(src)
+    for _ in range(a.warmup):
+        op.vmult(dst, src)
+    torch.cuda.synchronize()
+    op.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        op.vmult(dst, src)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    kern_ms, nlaunch = op.get_timing()
+    op.set_timing(False)
+    ms_step = el * 1e3 / a.steps
+    kern_avg = kern_ms / max(nlaunch, 1)
+    gdofs = op.n_dofs / (ms_step * 1e-3) / 1e9
+    bytes_alg = 16.0 * op.n_dofs + 8.0 * op.n_q_points
+    achieved = bytes_alg / (kern_avg * 1e-3) / 1e9
+    line = {
+        "metric": "GDoF/s operator apply; time-to-solve per heat step at 1e-10 (learned+FP32 vs FP64)",
+        "value": round(gdofs, 3),
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (torch.Generator seeds 2 and 3 on the GPU)",
+        "config": {"workload": "FP64 heat operator apply, Q%d hex, %d^3 cells, %d DoFs, QGauss(%d), k per quadrature "
+                               "point = k0(1+0.5 tanh((T-1)/0.1)), T~U[0,2]" % (k, N, op.n_dofs, k + 1),
+                   "degree": k, "cells": N, "n_dofs": op.n_dofs, "n_q_points": op.n_q_points},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "heat_apply_kernel<4,double,KAPPA_QP>", "kernel_avg_ms": round(kern_avg, 5),
+                     "algorithmic_bytes": bytes_alg},
+    }
+    del kq
+    if a.solve and k == 3:
+        try:
+            line["solve"] = solve_timings(torch, M, op)
+        except Exception as e:  # report, do not hide the apply line
+            line["solve"] = {"error": str(e)}
+    if a.cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        threads = max(1, min(threads, os.cpu_count() or 1))
+        v, reps, el, nd = cpu_baseline(k, a.cpu_cells, threads)
+        line["cpu_baseline"] = {"value": round(v, 5), "unit": "GDoF/s", "cores": threads, "kind": "port",
+                                "sample": "CPU oracle apply (same sum-factorised algorithm, OpenMP, FP64), Q%d "
+                                          "%d^3 cells (%d DoFs), k per quadrature point, %d applies in %.1f s"
+                                          % (k, a.cpu_cells, nd, reps, el)}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

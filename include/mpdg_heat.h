@@ -93,6 +93,12 @@ int mpdg_heat_op_create_rhs(const mpdg_heat_op *op, double *dst, const double *T
 /* VectorTools::compute_norm (source/utilities/vector_tools.cpp:170-200): L2 norm of the FE function */
 int mpdg_heat_op_l2_norm(const mpdg_heat_op *op, const double *u, double *norm, void *stream);
 
+/* HIP-event timing of the cell-apply kernels launched by this operator (for
+   benchmarks): enable resets the record; get returns the summed elapsed time
+   of the recorded launches (waits for them) and clears the record */
+int mpdg_heat_op_set_timing(mpdg_heat_op *op, int enable);
+int mpdg_heat_op_get_timing(mpdg_heat_op *op, double *total_ms, int *count);
+
 /* --- preconditioners: the Preconditioner concept update()/vmult()/reinit()
        (include/meltpooldg/linear_algebra/preconditioner.hpp:19-37) --- */
 typedef struct mpdg_precond mpdg_precond;

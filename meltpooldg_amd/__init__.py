@@ -85,6 +85,8 @@ _SIGNATURES = {
     "mpdg_heat_op_compute_diagonal": ([_V, _V, _V], _I),
     "mpdg_heat_op_create_rhs": ([_V, _V, _V, _V, _V, _V], _I),
     "mpdg_heat_op_l2_norm": ([_V, _V, C.POINTER(_D), _V], _I),
+    "mpdg_heat_op_set_timing": ([_V, _I], _I),
+    "mpdg_heat_op_get_timing": ([_V, C.POINTER(_D), C.POINTER(_I)], _I),
     "mpdg_precond_create": ([_V, _I, C.POINTER(GMGParams), C.POINTER(_V)], _I),
     "mpdg_precond_update": ([_V, _V], _I),
     "mpdg_precond_vmult": ([_V, _V, _V, _V], _I),
@@ -251,6 +253,16 @@ class HeatDiffuseMultiPhaseOperator:
         out = C.c_double()
         _check(self.lib.mpdg_heat_op_l2_norm(self.h, _ptr(u), C.byref(out), _stream()), "l2_norm")
         return out.value
+
+    def set_timing(self, enable=True):
+        """Record HIP events around every cell-apply kernel launch of this operator."""
+        _check(self.lib.mpdg_heat_op_set_timing(self.h, 1 if enable else 0), "set_timing")
+
+    def get_timing(self):
+        """(summed kernel milliseconds, number of launches) since set_timing."""
+        ms, n = C.c_double(), C.c_int()
+        _check(self.lib.mpdg_heat_op_get_timing(self.h, C.byref(ms), C.byref(n)), "get_timing")
+        return ms.value, n.value
 
     def new_vector(self):
         torch = _torch()

@@ -381,6 +381,28 @@ int launch_dense_gemv(const double *A, const double *x, double *y, int n, hipStr
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
+// v = 0 on the DoFs of homogeneous Dirichlet faces
+template <typename T>
+__global__ __launch_bounds__(256) void zero_constrained_kernel(T *v, int nd0, int nd1, int nd2, int mask) {
+  const long n = (long)nd0 * nd1 * nd2;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const int ix = i % nd0, iy = (i / nd0) % nd1, iz = i / ((long)nd0 * nd1);
+    const bool con = ((mask & 1) && ix == 0) || ((mask & 2) && ix == nd0 - 1) || ((mask & 4) && iy == 0) ||
+                     ((mask & 8) && iy == nd1 - 1) || ((mask & 16) && iz == 0) || ((mask & 32) && iz == nd2 - 1);
+    if (con) v[i] = T(0);
+  }
+}
+template <typename T>
+int launch_zero_constrained(T *v, const int *nd, int mask, hipStream_t s) {
+  if (!mask) return MPDG_OK;
+  const long n = (long)nd[0] * nd[1] * nd[2];
+  hipLaunchKernelGGL(zero_constrained_kernel<T>, dim3(std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, s, v,
+                     nd[0], nd[1], nd[2], mask);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+template int launch_zero_constrained<float>(float *, const int *, int, hipStream_t);
+template int launch_zero_constrained<double>(double *, const int *, int, hipStream_t);
+
 // deterministic start vector of the eigenvalue estimate (constrained entries 0)
 template <typename T>
 __global__ __launch_bounds__(256) void eig_start_kernel(T *v, int nd0, int nd1, int nd2, int mask) {

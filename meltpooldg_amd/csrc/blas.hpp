@@ -48,6 +48,7 @@ template <typename T>
 int launch_ell_dir(const T *in, T *out, const int *odims, const int *idims, int dir, int width, const int *col,
                    const T *val, hipStream_t s);
 int launch_dense_gemv(const double *A, const double *x, double *y, int n, hipStream_t s);
+template <typename T> int launch_zero_constrained(T *v, const int *nd, int mask, hipStream_t s);
 template <typename T> int launch_eig_start(T *v, const int *nd, int mask, hipStream_t s);
 template <typename T>
 int launch_eig_step(int phase, T *r, T *z, T *p, const T *q, const T *inv, long n, const Reduction &red, double *sc,

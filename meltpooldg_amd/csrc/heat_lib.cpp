@@ -20,7 +20,17 @@ inline int dev_ok() { return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ER
 // ===========================================================================
 int mpdg_heat_op::vmult_cells(double *dst, const double *src, hipStream_t s) const {
   if (precision != MPDG_FP64) return MPDG_ERR_INVALID;
-  return launch_apply<double>(d64, src, dst, s);
+  if (!timing_on) return launch_apply<double>(d64, src, dst, s);
+  if (timing_used == timing_events.size()) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return MPDG_ERR_DEVICE;
+    timing_events.push_back({a, b});
+  }
+  auto &ev = timing_events[timing_used++];
+  hipEventRecord(ev.first, s);
+  const int rc = launch_apply<double>(d64, src, dst, s);
+  hipEventRecord(ev.second, s);
+  return rc;
 }
 
 // HeatDiffuseMultiPhaseOperator::vmult: cell loop + boundary loop, dst zeroed
@@ -403,6 +413,8 @@ struct GMGPrecond : mpdg_precond {
       MPDG_TRY(L.kq_scratch.resize((size_t)L.od.n_cells * n1 * n1 * n1));
       MPDG_TRY(launch_diagonal<T>(L.od, L.inv.get(), L.kq_scratch.get(), s));
       MPDG_TRY(launch_invert_diagonal<T>(L.inv.get(), L.od.n_dofs, s));
+      // smoothing leaves the constrained DoFs at zero
+      MPDG_TRY(launch_zero_constrained<T>(L.inv.get(), L.od.nd, L.od.dmask, s));
       L.kq_scratch.release();
       MPDG_TRY(estimate_eigenvalue(L, s));
       set_chebyshev(L);
@@ -773,6 +785,28 @@ int mpdg_heat_op_set_robin(mpdg_heat_op *op, int mask, double alpha, double T_in
 int mpdg_heat_op_set_surface_flux(mpdg_heat_op *op, const double *p) {
   if (!op || !p) return MPDG_ERR_INVALID;
   for (int i = 0; i < 6; ++i) op->flux[i] = p[i];
+  return MPDG_OK;
+}
+
+int mpdg_heat_op_set_timing(mpdg_heat_op *op, int enable) {
+  if (!op) return MPDG_ERR_INVALID;
+  op->timing_on = enable != 0;
+  op->timing_used = 0;
+  return MPDG_OK;
+}
+int mpdg_heat_op_get_timing(mpdg_heat_op *op, double *total_ms, int *count) {
+  if (!op || !total_ms || !count) return MPDG_ERR_INVALID;
+  double t = 0.0;
+  for (size_t i = 0; i < op->timing_used; ++i) {
+    if (hipEventSynchronize(op->timing_events[i].second) != hipSuccess) return MPDG_ERR_DEVICE;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, op->timing_events[i].first, op->timing_events[i].second) != hipSuccess)
+      return MPDG_ERR_DEVICE;
+    t += ms;
+  }
+  *total_ms = t;
+  *count = (int)op->timing_used;
+  op->timing_used = 0;
   return MPDG_OK;
 }
 

@@ -83,6 +83,16 @@ struct mpdg_heat_op {
   const double *T_robin = nullptr;
   mutable mpdg::DeviceArray<double> kq, tmp1, tmp2, scal, tables;
   mutable mpdg::ReductionScratch red;
+  // optional HIP-event timing of the cell-apply kernel (bench / profiling)
+  mutable std::vector<std::pair<hipEvent_t, hipEvent_t>> timing_events;
+  mutable size_t timing_used = 0;
+  bool timing_on = false;
+  ~mpdg_heat_op() {
+    for (auto &e : timing_events) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+  }
 
   long n_dofs() const { return d64.n_dofs; }
   long n_cells() const { return d64.n_cells; }

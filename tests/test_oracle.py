@@ -1,0 +1,300 @@
+"""Pins the CPU oracle (the checker of the HIP library) without a GPU.
+
+Each function of the oracle is checked against something that fixes it
+independently of its own formula: the reference's golden outputs, brute-force
+element matrices, closed forms of the mathematics, or a textbook library
+routine (scipy) on the assembled matrix.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle import oracle as O
+
+
+# ---------------------------------------------------------------- 1D tables
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+def test_fe1d_tables(k):
+    t = O.fe1d(k)
+    n = k + 1
+    # Gauss rule with n points integrates x^m exactly for m <= 2n-1 on [0,1]
+    for m in range(2 * n):
+        assert abs(np.dot(t["qw"], t["qp"] ** m) - 1.0 / (m + 1)) < 1e-14
+    # Gauss-Lobatto nodes: end points and roots of P'_k (symmetric, interior)
+    assert t["nodes"][0] == 0.0 and t["nodes"][-1] == 1.0
+    assert np.allclose(t["nodes"] + t["nodes"][::-1], 1.0, atol=1e-15)
+    if k >= 2:
+        dP = np.polynomial.legendre.Legendre.basis(k).deriv()
+        assert np.allclose(dP(2 * t["nodes"][1:-1] - 1), 0.0, atol=1e-12)
+    # partition of unity and exact derivative of x^j
+    assert np.allclose(t["S"].sum(axis=1), 1.0, atol=1e-14)
+    assert np.allclose(t["G"].sum(axis=1), 0.0, atol=1e-12)
+    for j in range(1, k + 1):
+        assert np.allclose(t["G"] @ t["nodes"] ** j, j * t["qp"] ** (j - 1), atol=1e-12)
+    # collocation derivative composed with S equals the shape gradient
+    assert np.allclose(t["D"] @ t["S"], t["G"], atol=1e-12)
+
+
+# ---------------------------------------------------- operator vs brute force
+CASES = [
+    dict(degree=1, nc=(3, 2, 2)),
+    dict(degree=2, nc=(2, 3, 2), kappa=2.5),
+    dict(degree=3, nc=(2, 2, 1), kappa_qp=True),
+    dict(degree=4, nc=(1, 2, 1), kappa_cell=True),
+    dict(degree=2, nc=(2, 2, 2), law="tanh"),
+    dict(degree=3, nc=(1, 1, 2), law="mushy"),
+    dict(degree=2, nc=(2, 2, 2), mapped=0.05),
+    dict(degree=3, nc=(2, 1, 1), mapped=0.08, kappa_qp=True),
+    dict(degree=2, nc=(2, 2, 2), dirichlet=0b101101),
+    dict(degree=1, nc=(3, 3, 1), robin=True),
+]
+
+
+def build(case, L=(1.0, 0.8, 1.2), seed=0):
+    rng = np.random.default_rng(seed)
+    k, nc = case["degree"], case["nc"]
+    n1 = k + 1
+    nq = nc[0] * nc[1] * nc[2] * n1 ** 3
+    kw = dict(c_mass=3.0, dirichlet_mask=case.get("dirichlet", 0))
+    if case.get("kappa_qp"):
+        kw["kappa_qp"] = rng.uniform(0.5, 1.5, nq)
+    elif case.get("kappa_cell"):
+        kw["kappa_cell"] = rng.uniform(0.5, 1.5, nc[0] * nc[1] * nc[2])
+    elif case.get("law"):
+        tmp = O.Operator(k, nc, L)
+        kw["T"] = rng.uniform(0.0, 2.0, tmp.n_dofs)
+        kw["law"] = O.LAW_TANH if case["law"] == "tanh" else O.LAW_MUSHY
+        kw["law_p"] = (1.0, 0.5, 1.0, 0.1) if case["law"] == "tanh" else (40.0, 30.0, 0.5, 1.5)
+    else:
+        kw["kappa"] = case.get("kappa", 1.0)
+    if case.get("mapped"):
+        kw["mapped_amp"] = case["mapped"]
+    if case.get("robin"):
+        tmp = O.Operator(k, nc, L)
+        kw.update(robin_mask=2 | 16, alpha=3.0, emissivity=0.5, T=rng.uniform(200, 400, tmp.n_dofs))
+    return O.Operator(k, nc, L, **kw)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[str(i) for i in range(len(CASES))])
+def test_vmult_equals_assembled_element_matrices(case):
+    op = build(case)
+    x = np.random.default_rng(1).standard_normal(op.n_dofs)
+    a, b = op.vmult(x), op.vmult_brute(x)
+    assert np.linalg.norm(a - b) <= 1e-13 * np.linalg.norm(b)
+
+
+def assemble(op):
+    n = op.n_dofs
+    cols = []
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        cols.append(op.vmult_brute(e))
+    return np.array(cols).T
+
+
+@pytest.mark.parametrize("case", CASES[:6] + CASES[8:], ids=[str(i) for i in [0, 1, 2, 3, 4, 5, 8, 9]])
+def test_diagonal_and_symmetry(case):
+    op = build(case)
+    A = assemble(op)
+    assert np.allclose(A, A.T, rtol=0, atol=1e-12 * np.abs(A).max())
+    d = np.diag(A).copy()
+    dm = case.get("dirichlet", 0)
+    if dm:
+        d[np.all(A == 0, axis=1)] = 1.0
+    assert np.allclose(op.diagonal(), d, rtol=1e-13, atol=1e-14)
+    inv = op.inverse_diagonal()
+    assert np.allclose(inv, np.where(np.abs(d) > 1e-10, 1.0 / d, 1.0), rtol=1e-13)
+    # positive definite on the unconstrained space
+    free = ~np.all(A == 0, axis=1)
+    assert np.linalg.eigvalsh(A[np.ix_(free, free)]).min() > 0.0
+
+
+def test_closed_forms():
+    """K 1 = 0 and 1' M 1 = |Omega| (exact quadrature on affine cells), for every degree."""
+    L = (0.3, 0.5, 0.7)
+    for k in (1, 2, 3, 4):
+        nc = (3, 2, 2)
+        stiff = O.Operator(k, nc, L, c_mass=0.0, kappa_qp=np.random.default_rng(k).uniform(1, 2, 12 * (k + 1) ** 3))
+        one = np.ones(stiff.n_dofs)
+        assert np.abs(stiff.vmult(one)).max() < 1e-12
+        mass = O.Operator(k, nc, L, c_mass=1.0, kappa=0.0)
+        assert abs(one @ mass.vmult(one) - L[0] * L[1] * L[2]) < 1e-14
+        # mapped mesh: the transform keeps every column's x-extent, so the volume is unchanged
+        mm = O.Operator(k, nc, L, c_mass=1.0, kappa=0.0, mapped_amp=0.04)
+        assert abs(one @ mm.vmult(one) - L[0] * L[1] * L[2]) < 1e-12
+        assert np.abs(O.Operator(k, nc, L, c_mass=0.0, kappa=1.0, mapped_amp=0.04).vmult(one)).max() < 1e-11
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_exact_polynomial_integrals(k):
+    """u' J v equals the analytic integral for u, v in Q_k on the unit-extent box."""
+    L = (1.0, 2.0, 0.5)
+    op = O.Operator(k, (2, 3, 2), L, c_mass=2.0, kappa=3.0)
+    x, y, z = op.node_coords()
+    u = x ** k + y * z
+    v = x * y ** min(k, 2) + 1.0
+    val = u @ op.vmult(v)
+    # analytic: 2 * int u v + 3 * int grad u . grad v over [0,1]x[0,2]x[0,0.5]
+    import itertools
+    g = np.polynomial.legendre.leggauss(8)
+    pts = [(0.5 * (g[0] + 1) * Lx, 0.5 * g[1] * Lx) for Lx in L]
+    s = 0.0
+    m = min(k, 2)
+    for (xi, wx), (yi, wy), (zi, wz) in itertools.product(*[list(zip(*p)) for p in pts]):
+        uu = xi ** k + yi * zi
+        vv = xi * yi ** m + 1.0
+        gu = (k * xi ** (k - 1), zi, yi)
+        gv = (yi ** m, m * xi * yi ** (m - 1) if m > 0 else 0.0, 0.0)
+        s += wx * wy * wz * (2.0 * uu * vv + 3.0 * (gu[0] * gv[0] + gu[1] * gv[1] + gu[2] * gv[2]))
+    assert abs(val - s) < 1e-11 * abs(s)
+
+
+def test_l2_norm_exact():
+    for k in (1, 2, 3):
+        op = O.Operator(k, (2, 2, 3), (1.0, 1.0, 1.0))
+        x, y, z = op.node_coords()
+        u = x + 2 * y * z
+        # int (x + 2yz)^2 over the unit cube = 1/3 + 2*(1/2)(1/2)*2 ... evaluated exactly:
+        exact = 1.0 / 3.0 + 4.0 * (1.0 / 2.0) * (1.0 / 4.0) + 4.0 * (1.0 / 3.0) * (1.0 / 3.0)
+        assert abs(op.l2_norm(u) ** 2 - exact) < 1e-13
+
+
+# ---------------------------------------------------------- material laws
+def test_mushy_zone_law_matches_reference_material_output():
+    """unit_tests/material.output, row liquid_solid: (T = 20 -> k 31.04, dk/dT 0.096),
+    (T = 60 -> k 36.48, dk/dT 0.144) for k_l = 40, k_s = 30, T_s = 0, T_l = 100."""
+    p = (40.0, 30.0, 0.0, 100.0)
+    k, dk = O.kappa_law(O.LAW_MUSHY, p, 20.0)
+    assert abs(k - 31.04) < 1e-12 and abs(dk - 0.096) < 1e-12
+    k, dk = O.kappa_law(O.LAW_MUSHY, p, 60.0)
+    assert abs(k - 36.48) < 1e-12 and abs(dk - 0.144) < 1e-12
+    # the same law on the other columns of that row (rho: l 8000, s 7000 -> 7104, 7648)
+    k, _ = O.kappa_law(O.LAW_MUSHY, (8000.0, 7000.0, 0.0, 100.0), 20.0)
+    assert abs(k - 7104.0) < 1e-9
+    assert O.kappa_law(O.LAW_MUSHY, p, -5.0)[0] == 30.0 and O.kappa_law(O.LAW_MUSHY, p, 150.0)[0] == 40.0
+
+
+def test_tanh_law_closed_form():
+    p = (2.0, 0.5, 1.0, 0.1)
+    for T in (0.0, 0.95, 1.0, 1.3, 2.0):
+        k, dk = O.kappa_law(O.LAW_TANH, p, T)
+        assert abs(k - 2.0 * (1 + 0.5 * np.tanh((T - 1.0) / 0.1))) < 1e-14
+        h = 1e-6
+        fd = (O.kappa_law(O.LAW_TANH, p, T + h)[0] - O.kappa_law(O.LAW_TANH, p, T - h)[0]) / (2 * h)
+        assert abs(dk - fd) < 1e-6
+
+
+# ------------------------------------------------------------------ solvers
+def test_cg_matches_scipy():
+    op = build(dict(degree=2, nc=(3, 3, 2), kappa_qp=True))
+    A = sp.csr_matrix(assemble(op))
+    b = np.random.default_rng(3).uniform(-1, 1, op.n_dofs)
+    x, it, _ = op.cg(b, rel_tol=1e-12, precond=0, criterion=1, threads=1)
+    xs = np.linalg.solve(A.toarray(), b)
+    assert np.linalg.norm(x - xs) <= 1e-9 * np.linalg.norm(xs)
+    # scipy's CG (same start, same Krylov space) needs the same number of steps
+    count = [0]
+    spla.cg(A, b, rtol=1e-12, atol=0.0, maxiter=10000, callback=lambda _: count.__setitem__(0, count[0] + 1))
+    assert abs(count[0] - it) <= 1
+
+
+def test_transfers_exact_on_polynomials():
+    """Prolongation reproduces coarse Q_k functions; restriction is its transpose."""
+    rng = np.random.default_rng(0)
+    for kc, ncc, kf, ncf in [(2, (2, 3, 2), 2, (4, 6, 4)), (3, (2, 2, 2), 3, (4, 4, 4)), (2, (3, 2, 2), 3, (3, 2, 2)),
+                             (1, (2, 2, 1), 1, (4, 4, 2))]:
+        oc = O.Operator(kc, ncc)
+        of = O.Operator(kf, ncf)
+        xc, yc, zc = oc.node_coords()
+        xf, yf, zf = of.node_coords()
+        f = lambda x, y, z: x ** min(kc, 2) + x * y - 2 * z ** kc + 0.5
+        assert np.allclose(O.prolongate(kc, ncc, kf, ncf, f(xc, yc, zc)), f(xf, yf, zf), atol=1e-12)
+        u, v = rng.standard_normal(oc.n_dofs), rng.standard_normal(of.n_dofs)
+        assert abs(v @ O.prolongate(kc, ncc, kf, ncf, u) - u @ O.restrict(kc, ncc, kf, ncf, v)) < 1e-10
+
+
+def test_chebyshev_is_the_chebyshev_polynomial():
+    """One smoothing sweep from x = 0 applies p(D^-1 A) D^-1 with 1 - t p(t) the scaled Chebyshev polynomial."""
+    op = build(dict(degree=2, nc=(2, 2, 2), kappa=1.0))
+    A = assemble(op)
+    inv = op.inverse_diagonal()
+    lam = O.kappa_law  # noqa: keep namespace use explicit below
+    lmax = op_lmax = float(np.linalg.eigvals(np.diag(inv) @ A).real.max()) * 1.2
+    deg, rng_ = 3, 20.0
+    b = np.random.default_rng(5).standard_normal(op.n_dofs)
+    x = np.zeros(op.n_dofs)
+    O.lib().orc_chebyshev(op.ref, O.ptr(inv), lmax, rng_, deg, O.ptr(x), O.ptr(b), 1)
+    lmin = lmax / rng_
+    theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+    # error propagation E = I - p(DA)DA must equal T_deg((theta - t)/delta)/T_deg(theta/delta)
+    Bm = np.diag(inv) @ A
+    w, V = np.linalg.eig(Bm)
+    Tn = lambda s: np.cos(deg * np.arccos(np.clip(s, -1, 1))) if abs(s) <= 1 else np.cosh(deg * np.arccosh(abs(s))) * (np.sign(s) ** deg)
+    Ew = np.array([Tn((theta - t.real) / delta) / Tn(theta / delta) for t in w])
+    E = (V @ np.diag(Ew) @ np.linalg.inv(V)).real
+    xe = np.linalg.solve(A, b)
+    assert np.allclose(xe - x, E @ xe, atol=1e-10 * np.abs(xe).max())
+
+
+def test_eigenvalue_estimate():
+    op = build(dict(degree=2, nc=(3, 2, 2), kappa_qp=True))
+    A = assemble(op)
+    inv = op.inverse_diagonal()
+    exact = np.linalg.eigvals(np.diag(inv) @ A).real.max()
+    est = O.lib().orc_estimate_lambda_max(op.ref, O.ptr(inv), 40) / 1.2
+    assert abs(est - exact) < 1e-6 * exact
+
+
+def test_two_grid_contraction():
+    """The V-cycle is a good preconditioner: CG + GMG needs few iterations, independent of h."""
+    its = []
+    for N in (8, 16):
+        op = O.Operator(2, (N, N, N), c_mass=10.0, kappa_qp=np.random.default_rng(N).uniform(0.5, 1.5, N ** 3 * 27))
+        g = O.GMG(op, nc_coarse=2)
+        b = np.random.default_rng(1).uniform(-1, 1, op.n_dofs)
+        _, it, _ = op.cg(b, rel_tol=1e-10, precond=2, gmg=g, criterion=1)
+        its.append(it)
+    assert max(its) <= 15 and abs(its[1] - its[0]) <= 3
+
+
+# -------------------------------------------------------- golden norm runs
+GOLDEN = {
+    "plain": [5.64198e-03, 5.53556e-03, 5.44694e-03, 5.37273e-03, 5.31056e-03],
+    "conv": [1.00000e+02, 9.99999e+01, 9.99999e+01, 9.99998e+01, 9.99998e+01],
+    "rad": [9.99998e+01, 9.99997e+01, 9.99995e+01, 9.99994e+01, 9.99992e+01],
+    "conv_rad": [9.99998e+01, 9.99996e+01, 9.99994e+01, 9.99992e+01, 9.99990e+01],
+}
+
+
+@pytest.mark.parametrize("case", sorted(GOLDEN))
+def test_reference_golden_norm_sequences(case):
+    """applications/mp-heat-transfer/cases/unidirectional_heat_transfer/tests/
+    unidirectional_heat_transfer{,_with_convection,_with_radiation,_with_radiation_and_convection}.mpirun=4.output:
+    2D Q1 on [0,0.1]^2 refined 5 times, run here as 32x32x1 cells of thickness 1 (the 3D
+    problem restricted to z-constant fields is exactly the 2D one)."""
+    nc, L = (32, 32, 1), (0.1, 0.1, 1.0)
+    if case == "plain":
+        op = O.Operator(1, nc, L, kappa=1.0)
+        T0, rc, dt = op.node_coords()[0].copy(), 1.0, 1e-4
+    else:
+        kw = dict(robin_mask=2, T_inf_conv=300.0)
+        if "conv" in case:
+            kw["alpha"] = 25.0
+        if "rad" in case:
+            kw.update(emissivity=0.98, T_inf_rad=300.0)
+        op = O.Operator(1, nc, L, kappa=55.563, **kw)
+        T0, rc, dt = np.full(op.n_dofs, 1000.0), 7850.0 * 460.0, 1e-2
+    rc_, _, norms, _ = O.heat_run(op, rc, dt, 5, T0, predictor=2, nl_res_tol=5e-9)
+    assert rc_ == 0
+    assert ["%.5e" % v for v in norms] == ["%.5e" % v for v in GOLDEN[case]]
+
+
+def test_planted_mistake_is_caught():
+    """A dropped stiffness term (c_mass only) or a wrong sign must fail the golden sequence."""
+    nc, L = (32, 32, 1), (0.1, 0.1, 1.0)
+    op = O.Operator(1, nc, L, kappa=0.5)  # wrong conductivity
+    T0 = op.node_coords()[0].copy()
+    _, _, norms, _ = O.heat_run(op, 1.0, 1e-4, 5, T0, predictor=2, nl_res_tol=5e-9)
+    assert ["%.5e" % v for v in norms] != ["%.5e" % v for v in GOLDEN["plain"]]
