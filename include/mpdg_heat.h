@@ -147,12 +147,17 @@ typedef struct mpdg_heat_operation mpdg_heat_operation;
 #define MPDG_PREDICTOR_ZERO 1
 #define MPDG_PREDICTOR_LINEAR 2
 #define MPDG_PREDICTOR_LEARNED 3
+#define MPDG_STEP_NEWTON 0 /* Newton-Raphson as the reference (>= 2 linear solves per step) */
+#define MPDG_STEP_LINEAR 1 /* one linear solve J(T_n) T_{n+1} = b per step, coefficient frozen at T_n,
+                              stopped at ||b - J T|| <= rel_tolerance ||b|| (no radiation faces) */
 typedef struct mpdg_heat_operation_params {
   int predictor;
   mpdg_solver_params linear;
   int max_nonlinear_iterations;
   double residual_tolerance;
   double field_correction_tolerance;
+  int step_mode;          /* MPDG_STEP_NEWTON or MPDG_STEP_LINEAR */
+  int guess_check;        /* learned predictor: fall back to T_n when its residual is not smaller */
 } mpdg_heat_operation_params;
 int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p,
                                const mpdg_heat_operation_params *params,
@@ -164,9 +169,41 @@ int mpdg_heat_operation_set_initial(mpdg_heat_operation *h, const double *T0, vo
    nl_iters = Newton iterations */
 int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_iters,
                                    int *nl_iters, void *stream);
+/* learned predictor (MPDG_PREDICTOR_LEARNED): the network and the device DoF field of the heat
+   source it reads (the caller refreshes the field before each step) */
+typedef struct mpdg_learned mpdg_learned;
+int mpdg_heat_operation_set_learned(mpdg_heat_operation *h, mpdg_learned *net, const double *q_field);
+/* statistics of the last step: 1 if the learned guess was replaced by T_n, initial and final
+   relative residuals ||b - J T||/||b|| of the step's linear solve (MPDG_STEP_LINEAR) */
+int mpdg_heat_operation_last_step_info(const mpdg_heat_operation *h, int *fell_back, double *rel_res0,
+                                       double *rel_res);
 /* device pointers to the current / previous solution (owned by the handle) */
 const double *mpdg_heat_operation_solution(const mpdg_heat_operation *h);
 const double *mpdg_heat_operation_old_solution(const mpdg_heat_operation *h);
+
+/* nodal field of the z+ surface flux of create_rhs (zero off the z+ face), for the network input */
+int mpdg_heat_op_surface_flux_field(const mpdg_heat_op *op, double *out, void *stream);
+
+/* --- learned initial guess: 3D CNN (3 -> 16 -> 16 -> 16 -> 1 channels, 3x3x3 taps, ReLU) on the
+       DoF grid, BF16 MFMA kernels, FP32 master weights, Adam. Enters the predictor slot of
+       include/meltpooldg/linear_algebra/predictor.hpp:118-160 as a new predictor type. --- */
+int mpdg_learned_create(const int *dims, double qscale, unsigned long long seed, mpdg_learned **net);
+int mpdg_learned_destroy(mpdg_learned *net);
+int mpdg_learned_n_params(const mpdg_learned *net);
+/* weights as one FP32 host array: per layer W[co][ci][tap] then b[co] */
+int mpdg_learned_get_weights(const mpdg_learned *net, float *host);
+int mpdg_learned_set_weights(mpdg_learned *net, const float *host);
+/* out = T_n + s (c0 + y(T_{n-1}, T_n, q)) (device FP64 DoF vectors; q may be NULL) */
+int mpdg_learned_predict(mpdg_learned *net, const double *T_nm1, const double *T_n, const double *q,
+                         double *out, void *stream);
+/* one Adam step on the sample (T_{n-1}, T_n, q) -> T_{n+1}; loss = mean squared error before the step */
+int mpdg_learned_train_step(mpdg_learned *net, const double *T_nm1, const double *T_n, const double *q,
+                            const double *T_np1, double lr, double *loss, void *stream);
+/* gradients of the loss (host FP32, weight layout) without an update */
+int mpdg_learned_gradients(mpdg_learned *net, const double *T_nm1, const double *T_n, const double *q,
+                           const double *T_np1, float *host_grad, double *loss, void *stream);
+/* layer activations of the last forward pass as bf16 bit patterns [voxel][16] (layer 0 = input) */
+int mpdg_learned_activation(const mpdg_learned *net, int layer, unsigned short *host);
 
 #ifdef __cplusplus
 }

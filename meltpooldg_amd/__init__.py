@@ -63,7 +63,11 @@ class SolverParams(C.Structure):
 
 class OperationParams(C.Structure):
     _fields_ = [("predictor", C.c_int), ("linear", SolverParams), ("max_nonlinear_iterations", C.c_int),
-                ("residual_tolerance", C.c_double), ("field_correction_tolerance", C.c_double)]
+                ("residual_tolerance", C.c_double), ("field_correction_tolerance", C.c_double),
+                ("step_mode", C.c_int), ("guess_check", C.c_int)]
+
+
+STEP_NEWTON, STEP_LINEAR = 0, 1
 
 
 _V, _I, _D, _L = C.c_void_p, C.c_int, C.c_double, C.c_long
@@ -100,6 +104,18 @@ _SIGNATURES = {
     "mpdg_heat_operation_solve_step": ([_V, _D, C.POINTER(_I), C.POINTER(_I), _V], _I),
     "mpdg_heat_operation_solution": ([_V], _V),
     "mpdg_heat_operation_old_solution": ([_V], _V),
+    "mpdg_heat_operation_set_learned": ([_V, _V, _V], _I),
+    "mpdg_heat_operation_last_step_info": ([_V, C.POINTER(_I), C.POINTER(_D), C.POINTER(_D)], _I),
+    "mpdg_heat_op_surface_flux_field": ([_V, _V, _V], _I),
+    "mpdg_learned_create": ([C.POINTER(_I), _D, C.c_ulonglong, C.POINTER(_V)], _I),
+    "mpdg_learned_destroy": ([_V], _I),
+    "mpdg_learned_n_params": ([_V], _I),
+    "mpdg_learned_get_weights": ([_V, _V], _I),
+    "mpdg_learned_set_weights": ([_V, _V], _I),
+    "mpdg_learned_predict": ([_V, _V, _V, _V, _V, _V], _I),
+    "mpdg_learned_train_step": ([_V, _V, _V, _V, _V, _D, C.POINTER(_D), _V], _I),
+    "mpdg_learned_gradients": ([_V, _V, _V, _V, _V, _V, C.POINTER(_D), _V], _I),
+    "mpdg_learned_activation": ([_V, _I, _V], _I),
 }
 
 _lib = None
@@ -264,6 +280,11 @@ class HeatDiffuseMultiPhaseOperator:
         _check(self.lib.mpdg_heat_op_get_timing(self.h, C.byref(ms), C.byref(n)), "get_timing")
         return ms.value, n.value
 
+    def surface_flux_field(self, out):
+        """Nodal values of the z+ surface flux set by set_surface_flux (zero off that face)."""
+        _check(self.lib.mpdg_heat_op_surface_flux_field(self.h, _ptr(out, _torch().float64), _stream()),
+               "surface_flux_field")
+
     def new_vector(self):
         torch = _torch()
         return torch.zeros(self.n_dofs, dtype=self.dtype, device="cuda")
@@ -362,10 +383,12 @@ class NonlinearSolverData:
 class HeatDiffuseOperation:
     """Implicit heat time step: predictor, preconditioner update, Newton-Raphson with CG."""
 
-    def __init__(self, op, preconditioner=None, predictor=PREDICTOR_LINEAR, linear=None, nlsolve=None):
+    def __init__(self, op, preconditioner=None, predictor=PREDICTOR_LINEAR, linear=None, nlsolve=None,
+                 step_mode=STEP_NEWTON, learned=None, q_field=None, guess_check=True):
         self.op = op
         self.lib = op.lib
         self.pc = preconditioner
+        self.learned, self.q_field = learned, q_field
         linear = linear or LinearSolverData()
         nlsolve = nlsolve or NonlinearSolverData()
         p = OperationParams()
@@ -374,10 +397,27 @@ class HeatDiffuseOperation:
         p.max_nonlinear_iterations = nlsolve.max_nonlinear_iterations
         p.residual_tolerance = nlsolve.residual_tolerance
         p.field_correction_tolerance = nlsolve.field_correction_tolerance
+        p.step_mode = step_mode
+        p.guess_check = 1 if guess_check else 0
         h = C.c_void_p()
         _check(self.lib.mpdg_heat_operation_create(op.h, preconditioner.h if preconditioner else None,
                                                    C.byref(p), C.byref(h)), "heat_operation_create")
         self.h = h
+        if learned is not None:
+            _check(self.lib.mpdg_heat_operation_set_learned(h, learned.h, _ptr(q_field)), "set_learned")
+
+    def last_step_info(self):
+        """(learned guess replaced by T_n, initial and final ||b - J T||/||b|| of the step)"""
+        fb, r0, r = C.c_int(), C.c_double(), C.c_double()
+        _check(self.lib.mpdg_heat_operation_last_step_info(self.h, C.byref(fb), C.byref(r0), C.byref(r)), "info")
+        return bool(fb.value), r0.value, r.value
+
+    def old_solution(self):
+        torch = _torch()
+        out = torch.empty(self.op.n_dofs, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        _copy_device(out, self.lib.mpdg_heat_operation_old_solution(self.h), self.op.n_dofs)
+        return out
 
     def set_initial_condition(self, T0):
         _check(self.lib.mpdg_heat_operation_set_initial(self.h, _ptr(T0), _stream()), "set_initial")
@@ -404,6 +444,71 @@ class HeatDiffuseOperation:
                 self.h = None
         except Exception:
             pass
+
+
+class LearnedPredictor:
+    """Learned initial guess: 3D CNN on the DoF grid predicting T_{n+1} from (T_{n-1}, T_n, q).
+
+    A new predictor type for the slot of include/meltpooldg/linear_algebra/predictor.hpp:118-160;
+    training (Adam, FP32 masters) and inference run in the library's BF16 MFMA kernels.
+    """
+
+    def __init__(self, dims, qscale=1.0, seed=1234):
+        self.lib = load_library()
+        _torch()
+        h = C.c_void_p()
+        d = (C.c_int * 3)(*dims)
+        _check(self.lib.mpdg_learned_create(d, qscale, seed, C.byref(h)), "mpdg_learned_create")
+        self.h = h
+        self.dims = tuple(dims)
+        self.n_params = self.lib.mpdg_learned_n_params(h)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.mpdg_learned_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def predict(self, T_nm1, T_n, q, out):
+        _check(self.lib.mpdg_learned_predict(self.h, _ptr(T_nm1), _ptr(T_n), _ptr(q), _ptr(out), _stream()),
+               "learned predict")
+
+    def train_step(self, T_nm1, T_n, q, T_np1, lr):
+        loss = C.c_double()
+        _check(self.lib.mpdg_learned_train_step(self.h, _ptr(T_nm1), _ptr(T_n), _ptr(q), _ptr(T_np1), lr,
+                                                C.byref(loss), _stream()), "learned train_step")
+        return loss.value
+
+    def gradients(self, T_nm1, T_n, q, T_np1):
+        import numpy as np
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = C.c_double()
+        _check(self.lib.mpdg_learned_gradients(self.h, _ptr(T_nm1), _ptr(T_n), _ptr(q), _ptr(T_np1),
+                                               g.ctypes.data_as(C.c_void_p), C.byref(loss), _stream()),
+               "learned gradients")
+        return loss.value, g
+
+    def get_weights(self):
+        import numpy as np
+        w = np.zeros(self.n_params, dtype=np.float32)
+        _check(self.lib.mpdg_learned_get_weights(self.h, w.ctypes.data_as(C.c_void_p)), "get_weights")
+        return w
+
+    def set_weights(self, w):
+        import numpy as np
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.n_params
+        _check(self.lib.mpdg_learned_set_weights(self.h, w.ctypes.data_as(C.c_void_p)), "set_weights")
+
+    def save(self, path):
+        """Checkpoint: the FP32 weights as a raw little-endian binary."""
+        self.get_weights().tofile(path)
+
+    def load(self, path):
+        import numpy as np
+        self.set_weights(np.fromfile(path, dtype=np.float32))
 
 
 def _copy_device(dst, src_ptr, n):

@@ -1,0 +1,361 @@
+// BF16 MFMA kernels of the learned initial guess: a 3D convolutional network
+// (3x3x3 taps, 16 channels, zero padding) on the nodal grid of the heat
+// problem, trained and evaluated on the GPU.
+//
+// Activations are channel-last bf16 tensors act[voxel][16] (voxel = x + nx*(y
+// + ny*z)); a layer with fewer than 16 input or output channels is padded with
+// zero channels. One v_mfma_f32_16x16x16_bf16 computes, for one of the 27 taps,
+// 16 output channels x 16 voxels (a run of 16 voxels along x) with the 16 input
+// channels as the contraction; the 27 taps accumulate in the same registers.
+//   A operand (M = out channel, K = in channel): lane l holds W[tap][m = l%16][k = 4(l/16)..+3]
+//   B operand (K = in channel, N = voxel):       lane l holds act[voxel n = l%16][k = 4(l/16)..+3]
+//   D (M = out channel, N = voxel), f32:          lane l holds D[m = 4(l/16)+i][n = l%16], i = 0..3
+// so every lane moves 8 contiguous bytes per operand and writes 4 contiguous
+// output channels of one voxel.
+#include "cnn.hpp"
+
+namespace mpdg {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short f32_to_bf16(float f) {
+  // round to nearest even
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+  const unsigned r = 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)((u + r) >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
+
+__device__ __forceinline__ bf16x4 load4(const unsigned short *p) {
+  const uint2 v = *reinterpret_cast<const uint2 *>(p);
+  bf16x4 r;
+  r[0] = (short)(v.x & 0xffff);
+  r[1] = (short)(v.x >> 16);
+  r[2] = (short)(v.y & 0xffff);
+  r[3] = (short)(v.y >> 16);
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// forward / data-gradient convolution: out[v][co] = act(bias[co] + sum_tap,ci W[tap][co][ci] in[v+off(tap)][ci])
+// mode 0: linear, 1: ReLU, 2: multiply by the ReLU mask of `mask` (backward: g * (a > 0))
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void conv3_kernel(const unsigned short *__restrict__ in,
+                                                    unsigned short *__restrict__ out,
+                                                    const unsigned short *__restrict__ W, const float *__restrict__ bias,
+                                                    int nx, int ny, int nz, int mode,
+                                                    const unsigned short *__restrict__ mask) {
+  const int lane = threadIdx.x & 63;
+  const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int nwaves = (gridDim.x * 256) >> 6;
+  const int n = lane & 15, kg = lane >> 4;
+  bf16x4 a[27];
+#pragma unroll
+  for (int t = 0; t < 27; ++t) a[t] = load4(W + (t * 16 + n) * 16 + 4 * kg);
+  float bv[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) bv[i] = bias ? bias[4 * kg + i] : 0.0f;
+  const int tx = (nx + 15) >> 4;
+  const long ntiles = (long)tx * ny * nz;
+  for (long tile = wave; tile < ntiles; tile += nwaves) {
+    const int x0 = (int)(tile % tx) * 16, y = (int)((tile / tx) % ny), z = (int)(tile / ((long)tx * ny));
+    const int x = x0 + n;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+      const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+      const int xs = x + dx, ys = y + dy, zs = z + dz;
+      bf16x4 b = {0, 0, 0, 0};
+      if (xs >= 0 && xs < nx && ys >= 0 && ys < ny && zs >= 0 && zs < nz)
+        b = load4(in + ((long)xs + (long)nx * (ys + (long)ny * zs)) * 16 + 4 * kg);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], b, acc, 0, 0, 0);
+    }
+    if (x < nx) {
+      const long v = (long)x + (long)nx * (y + (long)ny * z);
+      unsigned short o[4];
+      uint2 mk = make_uint2(0, 0);
+      if (mode == 2) mk = *reinterpret_cast<const uint2 *>(mask + v * 16 + 4 * kg);
+      const unsigned short mv[4] = {(unsigned short)(mk.x & 0xffff), (unsigned short)(mk.x >> 16),
+                                    (unsigned short)(mk.y & 0xffff), (unsigned short)(mk.y >> 16)};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float r = acc[i] + bv[i];
+        if (mode == 1) r = r > 0.f ? r : 0.f;
+        if (mode == 2) r = bf16_to_f32(mv[i]) > 0.f ? r : 0.f;
+        o[i] = f32_to_bf16(r);
+      }
+      *reinterpret_cast<uint2 *>(out + v * 16 + 4 * kg) =
+          make_uint2((unsigned)o[0] | ((unsigned)o[1] << 16), (unsigned)o[2] | ((unsigned)o[3] << 16));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// weight gradient: dW[tap][co][ci] = sum_v g[v][co] in[v+off(tap)][ci], db[co] = sum_v g[v][co]
+// (g = gradient of the layer's pre-activation). MFMA with the voxels as the
+// contraction: A (M = co, K = voxel): lane holds g[v = x0+4(l/16)+j][co = l%16];
+// B (K = voxel, N = ci): lane holds in[v+off][ci = l%16]. Block partials are
+// written to part[block][27*256 + 16] and summed in fixed order by wgrad_reduce.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void wgrad_kernel(const unsigned short *__restrict__ g,
+                                                    const unsigned short *__restrict__ in, int nx, int ny, int nz,
+                                                    float *__restrict__ part) {
+  __shared__ float red[27 * 256 + 16];
+  for (int i = threadIdx.x; i < 27 * 256 + 16; i += 256) red[i] = 0.f;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int nwaves = (gridDim.x * 256) >> 6;
+  const int n = lane & 15, kg = lane >> 4;
+  f32x4 acc[27];
+#pragma unroll
+  for (int t = 0; t < 27; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.f;
+  const int tx = (nx + 15) >> 4;
+  const long ntiles = (long)tx * ny * nz;
+  for (long tile = wave; tile < ntiles; tile += nwaves) {
+    const int x0 = (int)(tile % tx) * 16, y = (int)((tile / tx) % ny), z = (int)(tile / ((long)tx * ny));
+    bf16x4 a;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int x = x0 + 4 * kg + j;
+      short val = 0;
+      if (x < nx) val = (short)g[((long)x + (long)nx * (y + (long)ny * z)) * 16 + n];
+      a[j] = val;
+      bsum += bf16_to_f32((unsigned short)val);
+    }
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+      const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+      const int ys = y + dy, zs = z + dz;
+      bf16x4 b = {0, 0, 0, 0};
+      if (ys >= 0 && ys < ny && zs >= 0 && zs < nz) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int x = x0 + 4 * kg + j, xs = x + dx;
+          if (x < nx && xs >= 0 && xs < nx) b[j] = (short)in[((long)xs + (long)nx * (ys + (long)ny * zs)) * 16 + n];
+        }
+      }
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+    }
+  }
+  // D[m = co = 4 kg + i][n = ci]
+#pragma unroll
+  for (int t = 0; t < 27; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) atomicAdd(&red[t * 256 + (4 * kg + i) * 16 + n], acc[t][i]);
+  // bias: lanes with the same co (= n here, since a holds g[.][co = n]) summed over kg
+  bsum += __shfl_xor(bsum, 16);
+  bsum += __shfl_xor(bsum, 32);
+  if (kg == 0) atomicAdd(&red[27 * 256 + n], bsum);
+  __syncthreads();
+  for (int i = threadIdx.x; i < 27 * 256 + 16; i += 256) part[(long)blockIdx.x * (27 * 256 + 16) + i] = red[i];
+}
+
+__global__ __launch_bounds__(256) void wgrad_reduce_kernel(const float *__restrict__ part, int nblocks,
+                                                           float *__restrict__ dW, float *__restrict__ db) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= 27 * 256 + 16) return;
+  float s = 0.f;
+  for (int b = 0; b < nblocks; ++b) s += part[(long)b * (27 * 256 + 16) + i];
+  if (i < 27 * 256) dW[i] = s;
+  else db[i - 27 * 256] = s;
+}
+
+// bf16 weights for the forward pass (W[tap][co][ci], channel counts padded to 16)
+// and for the data gradient (Wt[tap'][ci][co] with the tap mirrored), from FP32 masters
+// stored as M[co][ci][tap] over the real channel counts.
+__global__ void pack_weights_kernel(const float *__restrict__ master, int cin, int cout,
+                                    unsigned short *__restrict__ W, unsigned short *__restrict__ Wt) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= 27 * 256) return;
+  const int t = i / 256, co = (i / 16) % 16, ci = i % 16;
+  float v = 0.f;
+  if (co < cout && ci < cin) v = master[(co * cin + ci) * 27 + t];
+  W[i] = f32_to_bf16(v);
+  Wt[(26 - t) * 256 + ci * 16 + co] = f32_to_bf16(v);
+}
+
+// Adam step on the FP32 masters (weights M[co][ci][tap] and biases), gradient
+// taken from the padded dW[tap][co][ci] / db[co].
+__global__ void adam_kernel(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v, const float *__restrict__ dW,
+                            const float *__restrict__ db, float *__restrict__ b, float *__restrict__ mb,
+                            float *__restrict__ vb, int cin, int cout, float lr, float b1, float b2, float eps,
+                            float c1, float c2, float gscale) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int nw = cout * cin * 27;
+  if (i < nw) {
+    const int co = i / (cin * 27), ci = (i / 27) % cin, t = i % 27;
+    const float g = gscale * dW[t * 256 + co * 16 + ci];
+    m[i] = b1 * m[i] + (1.f - b1) * g;
+    v[i] = b2 * v[i] + (1.f - b2) * g * g;
+    w[i] -= lr * (m[i] / c1) / (sqrtf(v[i] / c2) + eps);
+  } else if (i < nw + cout) {
+    const int co = i - nw;
+    const float g = gscale * db[co];
+    mb[co] = b1 * mb[co] + (1.f - b1) * g;
+    vb[co] = b2 * vb[co] + (1.f - b2) * g * g;
+    b[co] -= lr * (mb[co] / c1) / (sqrtf(vb[co] / c2) + eps);
+  }
+}
+
+// ---- features and targets (FP64 fields -> bf16 channels) ----
+// channels: 0 (Tn - Tnm1)/s, 1 q*qscale/s, 2 (Tn - tmean)/tscale
+__global__ void features_kernel(const double *__restrict__ Tnm1, const double *__restrict__ Tn,
+                                const double *__restrict__ q, long n, const double *__restrict__ stats, double qscale,
+                                unsigned short *__restrict__ x) {
+  const double s = stats[0], tmean = stats[1], ts = stats[2];
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const float c0 = (float)((Tn[i] - Tnm1[i]) / s);
+    const float c1 = q ? (float)(q[i] * qscale / s) : 0.f;
+    const float c2 = (float)((Tn[i] - tmean) / ts);
+    unsigned short o[16] = {0};
+    o[0] = f32_to_bf16(c0);
+    o[1] = f32_to_bf16(c1);
+    o[2] = f32_to_bf16(c2);
+    uint4 *dst = reinterpret_cast<uint4 *>(x + i * 16);
+    dst[0] = make_uint4(o[0] | (o[1] << 16), o[2], 0, 0);
+    dst[1] = make_uint4(0, 0, 0, 0);
+  }
+}
+
+// loss gradient on channel 0: target c* = (Tnp1 - Tn)/s - c0, pred = y[v][0];
+// g = 2 (pred - c*) / n * gain; partial sums of the squared error
+__global__ __launch_bounds__(256) void loss_kernel(const unsigned short *__restrict__ y, const double *__restrict__ Tnm1,
+                                                   const double *__restrict__ Tn, const double *__restrict__ Tnp1,
+                                                   long n, const double *__restrict__ stats, float gain,
+                                                   unsigned short *__restrict__ g, double *__restrict__ part) {
+  __shared__ double sh[256];
+  const double s = stats[0];
+  double acc = 0.0;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const double c0 = (Tn[i] - Tnm1[i]) / s;
+    const double tgt = (Tnp1[i] - Tn[i]) / s - c0;
+    const double e = (double)bf16_to_f32(y[i * 16]) - tgt;
+    acc += e * e;
+    uint4 *dst = reinterpret_cast<uint4 *>(g + i * 16);
+    dst[0] = make_uint4(f32_to_bf16((float)(2.0 * e * gain / (double)n)), 0, 0, 0);
+    dst[1] = make_uint4(0, 0, 0, 0);
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// guess = Tn + s (c0 + y[v][0])
+__global__ void predict_kernel(const unsigned short *__restrict__ y, const double *__restrict__ Tnm1,
+                               const double *__restrict__ Tn, long n, const double *__restrict__ stats,
+                               double *__restrict__ out) {
+  const double s = stats[0];
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const double c0 = (Tn[i] - Tnm1[i]) / s;
+    out[i] = Tn[i] + s * (c0 + (double)bf16_to_f32(y[i * 16]));
+  }
+}
+
+// statistics of a sample: s = rms(Tn - Tnm1) (+ tiny), mean(Tn), rms(Tn - mean) (+ tiny)
+__global__ __launch_bounds__(256) void stats_partial_kernel(const double *__restrict__ Tnm1, const double *__restrict__ Tn,
+                                                            long n, double *__restrict__ part) {
+  __shared__ double sh[3][256];
+  double a = 0, b = 0, c = 0;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const double d = Tn[i] - Tnm1[i];
+    a += d * d;
+    b += Tn[i];
+    c += Tn[i] * Tn[i];
+  }
+  sh[0][threadIdx.x] = a;
+  sh[1][threadIdx.x] = b;
+  sh[2][threadIdx.x] = c;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k)
+      for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 3; ++q) part[blockIdx.x * 3 + q] = sh[q][0];
+}
+__global__ void stats_final_kernel(const double *__restrict__ part, int nb, long n, double *__restrict__ stats) {
+  double a = 0, b = 0, c = 0;
+  for (int i = 0; i < nb; ++i) {
+    a += part[3 * i];
+    b += part[3 * i + 1];
+    c += part[3 * i + 2];
+  }
+  const double mean = b / n;
+  stats[0] = sqrt(a / n) + 1e-30;
+  stats[1] = mean;
+  stats[2] = sqrt(fmax(c / n - mean * mean, 0.0)) + 1e-12 + 1e-6 * fabs(mean);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline int cnn_grid(long ntiles) {
+  return (int)std::max<long>(1, std::min<long>((ntiles + 3) / 4, 2048));
+}
+
+int cnn_conv(const unsigned short *in, unsigned short *out, const unsigned short *W, const float *bias, const int *dims,
+             int mode, const unsigned short *mask, hipStream_t s) {
+  const long ntiles = (long)((dims[0] + 15) / 16) * dims[1] * dims[2];
+  hipLaunchKernelGGL(conv3_kernel, dim3(cnn_grid(ntiles)), dim3(256), 0, s, in, out, W, bias, dims[0], dims[1],
+                     dims[2], mode, mask);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+int cnn_wgrad(const unsigned short *g, const unsigned short *in, const int *dims, float *part, int nblocks, float *dW,
+              float *db, hipStream_t s) {
+  hipLaunchKernelGGL(wgrad_kernel, dim3(nblocks), dim3(256), 0, s, g, in, dims[0], dims[1], dims[2], part);
+  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((27 * 256 + 16 + 255) / 256), dim3(256), 0, s, part, nblocks, dW, db);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+int cnn_pack(const float *master, int cin, int cout, unsigned short *W, unsigned short *Wt, hipStream_t s) {
+  hipLaunchKernelGGL(pack_weights_kernel, dim3(27), dim3(256), 0, s, master, cin, cout, W, Wt);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+int cnn_adam(float *w, float *m, float *v, const float *dW, const float *db, float *b, float *mb, float *vb, int cin,
+             int cout, float lr, float b1, float b2, float eps, int step, float gscale, hipStream_t s) {
+  const int n = cout * cin * 27 + cout;
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(adam_kernel, dim3((n + 255) / 256), dim3(256), 0, s, w, m, v, dW, db, b, mb, vb, cin, cout, lr,
+                     b1, b2, eps, c1, c2, gscale);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+int cnn_stats(const double *Tnm1, const double *Tn, long n, double *part, double *stats, hipStream_t s) {
+  const int nb = (int)std::min<long>(1024, (n + 255) / 256);
+  hipLaunchKernelGGL(stats_partial_kernel, dim3(nb), dim3(256), 0, s, Tnm1, Tn, n, part);
+  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(1), 0, s, part, nb, n, stats);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+int cnn_features(const double *Tnm1, const double *Tn, const double *q, long n, const double *stats, double qscale,
+                 unsigned short *x, hipStream_t s) {
+  hipLaunchKernelGGL(features_kernel, dim3((int)std::min<long>(8192, (n + 255) / 256)), dim3(256), 0, s, Tnm1, Tn, q,
+                     n, stats, qscale, x);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+int cnn_loss(const unsigned short *y, const double *Tnm1, const double *Tn, const double *Tnp1, long n,
+             const double *stats, float gain, unsigned short *g, double *part, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(loss_kernel, dim3(nblocks), dim3(256), 0, s, y, Tnm1, Tn, Tnp1, n, stats, gain, g, part);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+int cnn_predict(const unsigned short *y, const double *Tnm1, const double *Tn, long n, const double *stats,
+                double *out, hipStream_t s) {
+  hipLaunchKernelGGL(predict_kernel, dim3((int)std::min<long>(8192, (n + 255) / 256)), dim3(256), 0, s, y, Tnm1, Tn,
+                     n, stats, out);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+}  // namespace mpdg

@@ -300,6 +300,32 @@ int launch_coarsen_cells(const int *ncf, const Number *kf, Number *kc, hipStream
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
+// nodal values of the z+ surface flux q = peak exp(-((x - x0 - vx t)^2 + (y - y0)^2)/(2 sigma^2)), 0 elsewhere
+__global__ __launch_bounds__(256) void surface_flux_field_kernel(double *__restrict__ out, int nd0, int nd1, int nd2,
+                                                                 int k, double h0, double h1, FluxNodes fn) {
+  const long n = (long)nd0 * nd1 * nd2;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const int ix = i % nd0, iy = (i / nd0) % nd1, iz = i / ((long)nd0 * nd1);
+    double v = 0.0;
+    if (iz == nd2 - 1) {
+      const int cx = min(ix / k, (nd0 - 1) / k - 1), cy = min(iy / k, (nd1 - 1) / k - 1);
+      const double x = (cx + fn.nodes[ix - k * cx]) * h0, y = (cy + fn.nodes[iy - k * cy]) * h1;
+      const double dx = x - fn.p[2] - fn.p[4] * fn.p[5], dy = y - fn.p[3];
+      v = fn.p[0] * exp(-(dx * dx + dy * dy) / (2.0 * fn.p[1] * fn.p[1]));
+    }
+    out[i] = v;
+  }
+}
+int launch_surface_flux_field(const OperatorData<double> &op, const double *p6, double *out, hipStream_t s) {
+  FluxNodes fn{};
+  FE1D fe(op.degree);
+  for (int i = 0; i < fe.n1; ++i) fn.nodes[i] = fe.nodes[i];
+  for (int i = 0; i < 6; ++i) fn.p[i] = p6[i];
+  hipLaunchKernelGGL(surface_flux_field_kernel, dim3(std::min<long>((op.n_dofs + 255) / 256, 8192)), dim3(256), 0, s,
+                     out, op.nd[0], op.nd[1], op.nd[2], op.degree, op.h[0], op.h[1], fn);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
 template int launch_kappa_qp<double>(const OperatorData<double> &, double *, hipStream_t);
 template int launch_kappa_qp<float>(const OperatorData<float> &, float *, hipStream_t);
 template int launch_diagonal<double>(const OperatorData<double> &, double *, double *, hipStream_t);

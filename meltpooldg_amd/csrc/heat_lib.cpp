@@ -698,9 +698,14 @@ struct mpdg_heat_operation {
   DeviceArray<double> buf[3], rhs, du;
   int cur = 0, old = 1, spare = 2;
   double dt_old = 0.0;
+  double dt_precond = 0.0;
   KrylovWork work;
   ReductionScratch red;
-  DeviceArray<double> scal;
+  DeviceArray<double> scal, bvec, zero, alt;
+  mpdg_learned *net = nullptr;
+  const double *q_field = nullptr;
+  int fell_back = 0;
+  double rel_res0 = 0.0, rel_res = 0.0;
 };
 
 // ===========================================================================
@@ -909,7 +914,8 @@ int mpdg_linear_solve(mpdg_heat_op *op, mpdg_precond *p, const mpdg_solver_param
 int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p, const mpdg_heat_operation_params *prm,
                                mpdg_heat_operation **out) {
   if (!op || !prm || !out || op->precision != MPDG_FP64) return MPDG_ERR_INVALID;
-  if (prm->predictor < MPDG_PREDICTOR_NONE || prm->predictor > MPDG_PREDICTOR_LINEAR) return MPDG_ERR_INVALID;
+  if (prm->predictor < MPDG_PREDICTOR_NONE || prm->predictor > MPDG_PREDICTOR_LEARNED) return MPDG_ERR_INVALID;
+  if (prm->step_mode != MPDG_STEP_NEWTON && prm->step_mode != MPDG_STEP_LINEAR) return MPDG_ERR_INVALID;
   auto *h = new (std::nothrow) mpdg_heat_operation;
   if (!h) return MPDG_ERR_NOMEM;
   h->op = op;
@@ -922,6 +928,10 @@ int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p, const mpdg_hea
   rc = rc ? rc : h->du.resize(n);
   rc = rc ? rc : h->red.init();
   rc = rc ? rc : h->scal.resize(4);
+  rc = rc ? rc : h->bvec.resize(n);
+  rc = rc ? rc : h->zero.resize(n);
+  rc = rc ? rc : h->alt.resize(n);
+  if (!rc) rc = hipMemset(h->zero.get(), 0, n * sizeof(double)) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
   if (rc) {
     delete h;
     return rc;
@@ -940,6 +950,56 @@ int mpdg_heat_operation_set_initial(mpdg_heat_operation *h, const double *T0, vo
   MPDG_TRY(launch_convert<double, double>(T0, h->buf[h->old].get(), n, S(stream)));
   h->dt_old = 0.0;
   return hipStreamSynchronize(S(stream)) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+// One linear implicit step with the coefficient frozen at T_n:
+// J(T_n) T_{n+1} = b,  b = M (c_m T_n + q) + surface flux, from the predicted T.
+static int linear_step(mpdg_heat_operation *h, double dt, double *T, const double *T_old, int *lin_iters,
+                       int *nl_iters, hipStream_t s) {
+  mpdg_heat_op *op = h->op;
+  const long n = op->n_dofs();
+  if (op->d64.emissivity > 0.0) return MPDG_ERR_INVALID;
+  if (op->d64.kappa_mode == KAPPA_LAW) op->d64.T = T_old;
+  if (op->d64.robin_mask) op->T_robin = T_old;
+  const bool coeff_changes = op->d64.kappa_mode == KAPPA_LAW;
+  if (h->pc && (coeff_changes || dt != h->dt_precond)) {
+    MPDG_TRY(h->pc->update(s));
+    h->dt_precond = dt;
+  }
+  MPDG_TRY(launch_fill<double>(h->bvec.get(), 0.0, n, s));
+  MPDG_TRY(op->create_rhs(h->bvec.get(), h->zero.get(), T_old, nullptr, s));
+  if (h->prm.predictor == MPDG_PREDICTOR_LEARNED && h->prm.guess_check) {
+    // residuals of the learned guess (T) and of the previous solution (T_old)
+    MPDG_TRY(op->vmult(h->alt.get(), T, s));
+    MPDG_TRY(launch_axpby(h->alt.get(), 1.0, h->bvec.get(), -1.0, n, s));
+    MPDG_TRY(launch_dot<double, double>(h->alt.get(), h->alt.get(), n, h->red.red(), h->scal.get(), s));
+    MPDG_TRY(op->vmult(h->alt.get(), T_old, s));
+    MPDG_TRY(launch_axpby(h->alt.get(), 1.0, h->bvec.get(), -1.0, n, s));
+    MPDG_TRY(launch_dot<double, double>(h->alt.get(), h->alt.get(), n, h->red.red(), h->scal.get() + 1, s));
+    double rr[2];
+    if (hipMemcpyAsync(rr, h->scal.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
+      return MPDG_ERR_DEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+    if (!(rr[0] == rr[0]) || !(rr[0] < rr[1])) {
+      MPDG_TRY(launch_convert<double, double>(T_old, T, n, s));
+      h->fell_back = 1;
+    }
+  }
+  mpdg_solver_params lp = h->prm.linear;
+  lp.criterion = MPDG_CRIT_RHS;
+  int li = 0;
+  double res = 0.0;
+  const int rc = krylov_solve(op, h->pc, lp, h->bvec.get(), T, h->work, &li, &res, s);
+  CGState st{};
+  if (hipMemcpyAsync(&st, h->work.st.get(), sizeof(CGState), hipMemcpyDeviceToHost, s) != hipSuccess)
+    return MPDG_ERR_DEVICE;
+  if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+  h->rel_res0 = std::sqrt(st.rr0 / st.bb);
+  h->rel_res = std::sqrt(st.rr / st.bb);
+  h->dt_old = dt;
+  if (lin_iters) *lin_iters = li;
+  if (nl_iters) *nl_iters = 1;
+  return rc;
 }
 
 // HeatDiffuseOperation::init_time_advance + solve (heat_diffuse_operation.cpp:215-289)
@@ -965,6 +1025,11 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
       }
       break;
     }
+    case MPDG_PREDICTOR_LEARNED: {
+      if (!h->net) return MPDG_ERR_INVALID;
+      MPDG_TRY(mpdg_learned_predict(h->net, Tnm1, Tn, h->q_field, pred, s));
+      break;
+    }
     default: return MPDG_ERR_INVALID;
   }
   // commit_old_solutions + swap: old <- T_n, cur <- prediction
@@ -976,6 +1041,8 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
   const double *T_old = h->buf[h->old].get();
   if (op->d64.kappa_mode == KAPPA_LAW) op->d64.T = T;
   if (op->d64.robin_mask) op->T_robin = T;
+  h->fell_back = 0;
+  if (h->prm.step_mode == MPDG_STEP_LINEAR) return linear_step(h, dt, T, T_old, lin_iters, nl_iters, s);
   // preconditioner.update() at the predicted state (heat_diffuse_operation.cpp:276)
   if (h->pc) MPDG_TRY(h->pc->update(s));
   // NewtonRaphsonSolver::solve (newton_raphson_solver.cpp:24-66)
@@ -1012,6 +1079,24 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
   if (nl_iters) *nl_iters = it + 1;
   if (!converged) return MPDG_ERR_NO_CONVERGENCE;  // ExcNewtonDidNotConverge
   return hipStreamSynchronize(s) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+int mpdg_heat_operation_set_learned(mpdg_heat_operation *h, mpdg_learned *net, const double *q_field) {
+  if (!h) return MPDG_ERR_INVALID;
+  h->net = net;
+  h->q_field = q_field;
+  return MPDG_OK;
+}
+int mpdg_heat_operation_last_step_info(const mpdg_heat_operation *h, int *fell_back, double *r0, double *r) {
+  if (!h) return MPDG_ERR_INVALID;
+  if (fell_back) *fell_back = h->fell_back;
+  if (r0) *r0 = h->rel_res0;
+  if (r) *r = h->rel_res;
+  return MPDG_OK;
+}
+int mpdg_heat_op_surface_flux_field(const mpdg_heat_op *op, double *out, void *stream) {
+  if (!op || !out) return MPDG_ERR_INVALID;
+  return launch_surface_flux_field(op->d64, op->flux, out, S(stream));
 }
 
 const double *mpdg_heat_operation_solution(const mpdg_heat_operation *h) { return h ? h->buf[h->cur].get() : nullptr; }

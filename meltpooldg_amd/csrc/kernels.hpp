@@ -99,6 +99,11 @@ int launch_diagonal(const OperatorData<Number> &op, Number *diag, Number *kq_scr
 // boundary faces: mode 0 Robin tangent on src, 1 its diagonal, 2 boundary residual (+ surface flux)
 int launch_boundary(const OperatorData<double> &op, int mode, const double *flux, double *dst,
                     const double *src, const double *T, hipStream_t stream);
+struct FluxNodes {
+  double nodes[kMaxN1];
+  double p[6];
+};
+int launch_surface_flux_field(const OperatorData<double> &op, const double *p6, double *out, hipStream_t s);
 template <typename Number>
 int launch_kappa_qp(const OperatorData<Number> &op, Number *kq, hipStream_t stream);
 template <typename Number>

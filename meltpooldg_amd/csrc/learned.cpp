@@ -1,0 +1,244 @@
+// Learned initial guess for the implicit heat step: a 4-layer 3D convolutional
+// network (3 -> 16 -> 16 -> 16 -> 1 channels, 3x3x3 taps, ReLU) on the nodal
+// grid predicts T_{n+1} from (T_{n-1}, T_n, heat source). It plugs into the
+// reference's predictor slot (include/meltpooldg/linear_algebra/predictor.hpp:118-160)
+// as a new predictor type; training and inference run in the BF16 MFMA kernels
+// of cnn.hip with FP32 master weights and Adam.
+//
+// Features (per sample, s = rms(T_n - T_{n-1})): c0 = (T_n - T_{n-1})/s,
+// c1 = q * qscale / s, c2 = (T_n - mean T_n)/rms. The network output y is the
+// normalised change of the increment: guess = T_n + s (c0 + y), i.e. linear
+// extrapolation plus a learned correction.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "cnn.hpp"
+#include "heat_lib.hpp"
+
+using namespace mpdg;
+
+namespace {
+constexpr int kLayers = 4;
+constexpr int kCin[kLayers] = {3, 16, 16, 16};
+constexpr int kCout[kLayers] = {16, 16, 16, 1};
+constexpr int kWgradBlocks = 256;
+inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// deterministic Gaussian numbers (splitmix64 + Box-Muller)
+struct Rng {
+  unsigned long long x;
+  unsigned long long next() {
+    unsigned long long z = (x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+  double normal() { return std::sqrt(-2.0 * std::log(uniform())) * std::cos(6.283185307179586 * uniform()); }
+};
+}  // namespace
+
+struct mpdg_learned {
+  int dims[3] = {0, 0, 0};
+  long n = 0;
+  double qscale = 1.0;
+  int step = 0;
+  DeviceArray<float> w[kLayers], b[kLayers], mw[kLayers], vw[kLayers], mb[kLayers], vb[kLayers];
+  DeviceArray<unsigned short> W[kLayers], Wt[kLayers], act[kLayers + 1], g0, g1;
+  DeviceArray<float> dW[kLayers], db[kLayers], part;
+  DeviceArray<double> stats, spart, lpart;
+
+  int init(const int *d, double qs, unsigned long long seed) {
+    for (int i = 0; i < 3; ++i) dims[i] = d[i];
+    n = (long)d[0] * d[1] * d[2];
+    qscale = qs;
+    Rng rng{seed};
+    for (int l = 0; l < kLayers; ++l) {
+      const int nw = kCout[l] * kCin[l] * 27;
+      std::vector<float> hw(nw), hb(kCout[l], 0.0f);
+      const double sd = std::sqrt(2.0 / (kCin[l] * 27.0)) * (l == kLayers - 1 ? 0.1 : 1.0);
+      for (auto &v : hw) v = (float)(sd * rng.normal());
+      MPDG_TRY(w[l].resize(nw));
+      MPDG_TRY(b[l].resize(kCout[l]));
+      MPDG_TRY(mw[l].resize(nw));
+      MPDG_TRY(vw[l].resize(nw));
+      MPDG_TRY(mb[l].resize(kCout[l]));
+      MPDG_TRY(vb[l].resize(kCout[l]));
+      MPDG_TRY(W[l].resize(27 * 256));
+      MPDG_TRY(Wt[l].resize(27 * 256));
+      MPDG_TRY(dW[l].resize(27 * 256));
+      MPDG_TRY(db[l].resize(16));
+      hipMemcpy(w[l].get(), hw.data(), nw * sizeof(float), hipMemcpyHostToDevice);
+      hipMemcpy(b[l].get(), hb.data(), hb.size() * sizeof(float), hipMemcpyHostToDevice);
+      hipMemset(mw[l].get(), 0, nw * sizeof(float));
+      hipMemset(vw[l].get(), 0, nw * sizeof(float));
+      hipMemset(mb[l].get(), 0, kCout[l] * sizeof(float));
+      hipMemset(vb[l].get(), 0, kCout[l] * sizeof(float));
+    }
+    for (int l = 0; l <= kLayers; ++l) MPDG_TRY(act[l].resize((size_t)n * 16));
+    MPDG_TRY(g0.resize((size_t)n * 16));
+    MPDG_TRY(g1.resize((size_t)n * 16));
+    MPDG_TRY(part.resize((size_t)kWgradBlocks * (27 * 256 + 16)));
+    MPDG_TRY(stats.resize(4));
+    MPDG_TRY(spart.resize(3 * 1024));
+    MPDG_TRY(lpart.resize(1024));
+    return pack(nullptr);
+  }
+  int pack(hipStream_t s) {
+    for (int l = 0; l < kLayers; ++l) MPDG_TRY(cnn_pack(w[l].get(), kCin[l], kCout[l], W[l].get(), Wt[l].get(), s));
+    return MPDG_OK;
+  }
+  int forward(const double *Tnm1, const double *Tn, const double *q, hipStream_t s) {
+    MPDG_TRY(cnn_stats(Tnm1, Tn, n, spart.get(), stats.get(), s));
+    MPDG_TRY(cnn_features(Tnm1, Tn, q, n, stats.get(), qscale, act[0].get(), s));
+    for (int l = 0; l < kLayers; ++l)
+      MPDG_TRY(cnn_conv(act[l].get(), act[l + 1].get(), W[l].get(), b[l].get(), dims, l == kLayers - 1 ? 0 : 1,
+                        nullptr, s));
+    return MPDG_OK;
+  }
+  // loss and gradients of all layers (dW, db); returns the mean squared error
+  int backward(const double *Tnm1, const double *Tn, const double *Tnp1, double *loss, hipStream_t s) {
+    const int nb = 512;
+    MPDG_TRY(cnn_loss(act[kLayers].get(), Tnm1, Tn, Tnp1, n, stats.get(), 1.0f, g0.get(), lpart.get(), nb, s));
+    unsigned short *g = g0.get(), *gn = g1.get();
+    for (int l = kLayers - 1; l >= 0; --l) {
+      MPDG_TRY(cnn_wgrad(g, act[l].get(), dims, part.get(), kWgradBlocks, dW[l].get(), db[l].get(), s));
+      if (l > 0) {
+        // gradient of the previous layer's pre-activation: (W^T conv g) * relu'(act[l])
+        MPDG_TRY(cnn_conv(g, gn, Wt[l].get(), nullptr, dims, 2, act[l].get(), s));
+        std::swap(g, gn);
+      }
+    }
+    if (loss) {
+      std::vector<double> h(nb);
+      if (hipMemcpyAsync(h.data(), lpart.get(), nb * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
+        return MPDG_ERR_DEVICE;
+      if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+      double sum = 0.0;
+      for (double v : h) sum += v;
+      *loss = sum / (double)n;
+    }
+    return MPDG_OK;
+  }
+};
+
+extern "C" {
+
+int mpdg_learned_create(const int *dims, double qscale, unsigned long long seed, mpdg_learned **out) {
+  if (!dims || !out || dims[0] < 1 || dims[1] < 1 || dims[2] < 1) return MPDG_ERR_INVALID;
+  auto *h = new (std::nothrow) mpdg_learned;
+  if (!h) return MPDG_ERR_NOMEM;
+  const int rc = h->init(dims, qscale, seed);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  if (hipDeviceSynchronize() != hipSuccess) {
+    delete h;
+    return MPDG_ERR_DEVICE;
+  }
+  *out = h;
+  return MPDG_OK;
+}
+
+int mpdg_learned_destroy(mpdg_learned *h) {
+  delete h;
+  return MPDG_OK;
+}
+
+int mpdg_learned_n_params(const mpdg_learned *h) {
+  if (!h) return -1;
+  int n = 0;
+  for (int l = 0; l < kLayers; ++l) n += kCout[l] * kCin[l] * 27 + kCout[l];
+  return n;
+}
+
+// weights as one FP32 array: per layer W[co][ci][tap] then b[co]
+int mpdg_learned_get_weights(const mpdg_learned *h, float *host) {
+  if (!h || !host) return MPDG_ERR_INVALID;
+  size_t o = 0;
+  for (int l = 0; l < kLayers; ++l) {
+    const int nw = kCout[l] * kCin[l] * 27;
+    if (hipMemcpy(host + o, h->w[l].get(), nw * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return MPDG_ERR_DEVICE;
+    o += nw;
+    if (hipMemcpy(host + o, h->b[l].get(), kCout[l] * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+      return MPDG_ERR_DEVICE;
+    o += kCout[l];
+  }
+  return MPDG_OK;
+}
+
+int mpdg_learned_set_weights(mpdg_learned *h, const float *host) {
+  if (!h || !host) return MPDG_ERR_INVALID;
+  size_t o = 0;
+  for (int l = 0; l < kLayers; ++l) {
+    const int nw = kCout[l] * kCin[l] * 27;
+    if (hipMemcpy(h->w[l].get(), host + o, nw * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return MPDG_ERR_DEVICE;
+    o += nw;
+    if (hipMemcpy(h->b[l].get(), host + o, kCout[l] * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+      return MPDG_ERR_DEVICE;
+    o += kCout[l];
+  }
+  MPDG_TRY(h->pack(nullptr));
+  return hipDeviceSynchronize() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+int mpdg_learned_predict(mpdg_learned *h, const double *Tnm1, const double *Tn, const double *q, double *out,
+                         void *stream) {
+  if (!h || !Tnm1 || !Tn || !out) return MPDG_ERR_INVALID;
+  MPDG_TRY(h->forward(Tnm1, Tn, q, S(stream)));
+  return cnn_predict(h->act[kLayers].get(), Tnm1, Tn, h->n, h->stats.get(), out, S(stream));
+}
+
+// gradients (FP32, same layout as the weights) and the loss of one sample, no update
+int mpdg_learned_gradients(mpdg_learned *h, const double *Tnm1, const double *Tn, const double *q,
+                           const double *Tnp1, float *host_grad, double *loss, void *stream) {
+  if (!h || !Tnm1 || !Tn || !Tnp1 || !host_grad) return MPDG_ERR_INVALID;
+  hipStream_t s = S(stream);
+  MPDG_TRY(h->forward(Tnm1, Tn, q, s));
+  MPDG_TRY(h->backward(Tnm1, Tn, Tnp1, loss, s));
+  size_t o = 0;
+  for (int l = 0; l < kLayers; ++l) {
+    std::vector<float> dw(27 * 256), db(16);
+    if (hipMemcpyAsync(dw.data(), h->dW[l].get(), dw.size() * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(db.data(), h->db[l].get(), db.size() * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return MPDG_ERR_DEVICE;
+    for (int co = 0; co < kCout[l]; ++co)
+      for (int ci = 0; ci < kCin[l]; ++ci)
+        for (int t = 0; t < 27; ++t) host_grad[o++] = dw[t * 256 + co * 16 + ci];
+    for (int co = 0; co < kCout[l]; ++co) host_grad[o++] = db[co];
+  }
+  return MPDG_OK;
+}
+
+// one Adam step on one sample (T_{n-1}, T_n, q) -> T_{n+1}; loss = mean squared error
+// of the normalised correction before the update
+int mpdg_learned_train_step(mpdg_learned *h, const double *Tnm1, const double *Tn, const double *q,
+                            const double *Tnp1, double lr, double *loss, void *stream) {
+  if (!h || !Tnm1 || !Tn || !Tnp1) return MPDG_ERR_INVALID;
+  hipStream_t s = S(stream);
+  MPDG_TRY(h->forward(Tnm1, Tn, q, s));
+  MPDG_TRY(h->backward(Tnm1, Tn, Tnp1, loss, s));
+  h->step += 1;
+  for (int l = 0; l < kLayers; ++l)
+    MPDG_TRY(cnn_adam(h->w[l].get(), h->mw[l].get(), h->vw[l].get(), h->dW[l].get(), h->db[l].get(), h->b[l].get(),
+                      h->mb[l].get(), h->vb[l].get(), kCin[l], kCout[l], (float)lr, 0.9f, 0.999f, 1e-8f, h->step,
+                      1.0f, s));
+  MPDG_TRY(h->pack(s));
+  if (loss && !std::isfinite(*loss)) return MPDG_ERR_NONFINITE;
+  return MPDG_OK;
+}
+
+// raw network input/output of the last forward pass (bf16 bit patterns, [voxel][16])
+int mpdg_learned_activation(const mpdg_learned *h, int layer, unsigned short *host) {
+  if (!h || !host || layer < 0 || layer > kLayers) return MPDG_ERR_INVALID;
+  return hipMemcpy(host, h->act[layer].get(), (size_t)h->n * 16 * sizeof(unsigned short), hipMemcpyDeviceToHost) ==
+                 hipSuccess
+             ? MPDG_OK
+             : MPDG_ERR_DEVICE;
+}
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+"""CPU oracle of the learned initial guess network -- TEST INFRASTRUCTURE ONLY.
+
+numpy restatement of the network the HIP library trains (meltpooldg_amd/csrc/cnn.hip,
+learned.cpp): features from (T_{n-1}, T_n, q), four 3x3x3 convolutions with zero
+padding (3 -> 16 -> 16 -> 16 -> 1 channels, ReLU on the first three), output =
+normalised correction of the linear extrapolation, loss = mean squared error,
+and its exact backward pass. With bf16=True the features, weights, activations
+and back-propagated gradients are rounded to bfloat16 (round to nearest even)
+at the same places as the GPU kernels, so the two agree up to the FP32
+accumulation order; with bf16=False everything is FP64.
+
+Weights use the library's layout: per layer W[co][ci][tap] then b[co], tap =
+(dx+1) + 3 (dy+1) + 9 (dz+1). Grids are (nz, ny, nx) arrays of the DoF grid.
+"""
+import numpy as np
+
+CIN = (3, 16, 16, 16)
+COUT = (16, 16, 16, 1)
+
+
+def bf16(a):
+    """Round float64/float32 values to bfloat16 (nearest even), returned as float64."""
+    f = np.asarray(a, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def unpack(flat):
+    out, o = [], 0
+    for ci, co in zip(CIN, COUT):
+        nw = co * ci * 27
+        W = np.asarray(flat[o:o + nw], dtype=np.float64).reshape(co, ci, 27)
+        o += nw
+        b = np.asarray(flat[o:o + co], dtype=np.float64)
+        o += co
+        out.append((W, b))
+    return out
+
+
+def pack(layers):
+    return np.concatenate([np.concatenate([W.ravel(), b]) for W, b in layers])
+
+
+def _shift(a, dx, dy, dz):
+    """a[:, z+dz, y+dy, x+dx] with zero padding, for a (C, nz, ny, nx) array."""
+    C, nz, ny, nx = a.shape
+    p = np.zeros((C, nz + 2, ny + 2, nx + 2))
+    p[:, 1:-1, 1:-1, 1:-1] = a
+    return p[:, 1 + dz:1 + dz + nz, 1 + dy:1 + dy + ny, 1 + dx:1 + dx + nx]
+
+
+def taps():
+    for t in range(27):
+        yield t, t % 3 - 1, (t // 3) % 3 - 1, t // 9 - 1
+
+
+def conv(x, W, b):
+    out = np.zeros((W.shape[0],) + x.shape[1:])
+    for t, dx, dy, dz in taps():
+        out += np.tensordot(W[:, :, t], _shift(x, dx, dy, dz), axes=(1, 0))
+    if b is not None:
+        out += b[:, None, None, None]
+    return out
+
+
+def features(Tnm1, Tn, q, qscale):
+    d = Tn - Tnm1
+    n = Tn.size
+    s = np.sqrt(np.sum(d * d) / n) + 1e-30
+    mean = np.sum(Tn) / n
+    ts = np.sqrt(max(np.sum(Tn * Tn) / n - mean * mean, 0.0)) + 1e-12 + 1e-6 * abs(mean)
+    c = np.zeros((3,) + Tn.shape)
+    c[0] = d / s
+    c[1] = (q * qscale / s) if q is not None else 0.0
+    c[2] = (Tn - mean) / ts
+    return c, s
+
+
+def forward(flat, Tnm1, Tn, q, qscale=1.0, bf=True):
+    rnd = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
+    layers = [(rnd(W), np.asarray(b, dtype=np.float32).astype(np.float64)) for W, b in unpack(flat)]
+    x, s = features(Tnm1, Tn, q, qscale)
+    acts = [rnd(x)]
+    for l, (W, b) in enumerate(layers):
+        y = conv(acts[-1], W, b)
+        if l < 3:
+            y = np.maximum(y, 0.0)
+        acts.append(rnd(y))
+    return acts, s, layers
+
+
+def predict(flat, Tnm1, Tn, q, qscale=1.0, bf=True):
+    acts, s, _ = forward(flat, Tnm1, Tn, q, qscale, bf)
+    c0 = (Tn - Tnm1) / s
+    return Tn + s * (c0 + acts[-1][0])
+
+
+def loss_and_gradients(flat, Tnm1, Tn, q, Tnp1, qscale=1.0, bf=True):
+    rnd = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
+    acts, s, layers = forward(flat, Tnm1, Tn, q, qscale, bf)
+    c0 = (Tn - Tnm1) / s
+    tgt = (Tnp1 - Tn) / s - c0
+    e = acts[-1][0] - tgt
+    n = Tn.size
+    loss = float(np.sum(e * e) / n)
+    g = np.zeros_like(acts[-1])
+    g[0] = 2.0 * e / n
+    g = rnd(g)
+    grads = [None] * 4
+    for l in range(3, -1, -1):
+        W, _ = layers[l]
+        a_in = acts[l]
+        dW = np.zeros_like(W)
+        for t, dx, dy, dz in taps():
+            dW[:, :, t] = np.tensordot(g, _shift(a_in, dx, dy, dz), axes=([1, 2, 3], [1, 2, 3]))
+        db = g.sum(axis=(1, 2, 3))
+        grads[l] = (dW, db)
+        if l > 0:
+            gin = np.zeros_like(a_in)
+            for t, dx, dy, dz in taps():
+                gin += np.tensordot(W[:, :, t].T, _shift(g, -dx, -dy, -dz), axes=(1, 0))
+            g = rnd(gin * (a_in > 0))
+    return loss, pack(grads)

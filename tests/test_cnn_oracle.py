@@ -1,0 +1,80 @@
+"""Pins the numpy oracle of the learned-guess network (CPU only): the backward
+pass against central finite differences of the FP64 loss, the convolution
+against a direct loop, and the bf16 rounding against known bit patterns."""
+import numpy as np
+
+from oracle import cnn_oracle as N
+
+
+def sample(shape=(5, 6, 7), seed=0):
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(*[np.linspace(0, 1, s) for s in shape], indexing="ij")
+    Tnm1 = np.sin(3 * x) * np.cos(2 * y) + z
+    Tn = Tnm1 + 0.1 * np.exp(-((x - 0.4) ** 2 + (y - 0.5) ** 2) / 0.05) + 0.01 * rng.standard_normal(shape)
+    Tnp1 = Tn + 0.12 * np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2) / 0.05)
+    q = np.where(np.arange(shape[0])[:, None, None] == shape[0] - 1, np.exp(-((x - 0.5) ** 2) / 0.02), 0.0)
+    return Tnm1, Tn, q, Tnp1
+
+
+def init(seed=1):
+    rng = np.random.default_rng(seed)
+    parts = []
+    for ci, co in zip(N.CIN, N.COUT):
+        parts.append(rng.standard_normal(co * ci * 27) * np.sqrt(2.0 / (27 * ci)))
+        parts.append(0.1 * rng.standard_normal(co))
+    return np.concatenate(parts)
+
+
+def test_bf16_rounding():
+    assert N.bf16(1.0) == 1.0
+    assert N.bf16(1.0 + 2 ** -8) == 1.0          # tie rounds to even
+    assert N.bf16(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -6
+    assert N.bf16(-3.14159) == -3.140625
+
+
+def test_conv_matches_direct_loop():
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((3, 4, 5, 6))
+    W = rng.standard_normal((2, 3, 27))
+    b = rng.standard_normal(2)
+    out = N.conv(x, W, b)
+    ref = np.zeros_like(out)
+    C, nz, ny, nx = x.shape
+    for co in range(2):
+        for z in range(nz):
+            for yy in range(ny):
+                for xx in range(nx):
+                    s = b[co]
+                    for t in range(27):
+                        dx, dy, dz = t % 3 - 1, (t // 3) % 3 - 1, t // 9 - 1
+                        if 0 <= xx + dx < nx and 0 <= yy + dy < ny and 0 <= z + dz < nz:
+                            s += W[co, :, t] @ x[:, z + dz, yy + dy, xx + dx]
+                    ref[co, z, yy, xx] = s
+    assert np.allclose(out, ref, atol=1e-12)
+
+
+def test_backward_matches_finite_differences():
+    Tnm1, Tn, q, Tnp1 = sample()
+    w = init()
+    loss, g = N.loss_and_gradients(w, Tnm1, Tn, q, Tnp1, bf=False)
+    rng = np.random.default_rng(3)
+    idx = rng.choice(w.size, 40, replace=False)
+    h = 1e-6
+    for i in idx:
+        wp, wm = w.copy(), w.copy()
+        wp[i] += h
+        wm[i] -= h
+        fp = N.loss_and_gradients(wp, Tnm1, Tn, q, Tnp1, bf=False)[0]
+        fm = N.loss_and_gradients(wm, Tnm1, Tn, q, Tnp1, bf=False)[0]
+        fd = (fp - fm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g).max())
+
+
+def test_prediction_is_linear_extrapolation_plus_output():
+    Tnm1, Tn, q, _ = sample()
+    w = init()
+    w[-1] = 0.0  # last bias
+    nlast = N.COUT[3] * N.CIN[3] * 27
+    w[-1 - nlast:-1] = 0.0  # last weights: output 0
+    p = N.predict(w, Tnm1, Tn, q, bf=False)
+    assert np.allclose(p, 2 * Tn - Tnm1, atol=1e-12)
