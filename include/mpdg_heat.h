@@ -74,6 +74,11 @@ int mpdg_heat_op_set_conductivity_law(mpdg_heat_op *op, int law, const double *p
    T (device, FP64) is the temperature the radiation term is linearised about */
 int mpdg_heat_op_set_robin(mpdg_heat_op *op, int face_mask, double alpha, double T_inf_conv,
                            double emissivity, double T_inf_rad, const double *T);
+/* mapped geometry: node coordinates of MappingQ(k) on the FE's support points (device FP64,
+   3*n_dofs: X block, Y block, Z block; read when called and by multigrid set-up, kept by the
+   caller). Stores the per-quadrature-point metric JxW and JxW J^-1 J^-T like MatrixFree does for
+   non-affine cells (scratch_data.cpp:192-266); boundary-face terms need the affine box. */
+int mpdg_heat_op_set_mapping(mpdg_heat_op *op, const double *coords);
 /* Gaussian surface flux on face z+ used by create_rhs: q = peak*exp(-((x-x0-vx*t)^2+(y-y0)^2)/(2 sigma^2)),
    p = {peak, sigma, x0, y0, vx, t}; peak = 0 switches it off */
 int mpdg_heat_op_set_surface_flux(mpdg_heat_op *op, const double *p6);

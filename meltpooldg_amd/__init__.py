@@ -83,6 +83,7 @@ _SIGNATURES = {
     "mpdg_heat_op_set_conductivity_law": ([_V, _I, C.POINTER(_D), _V], _I),
     "mpdg_heat_op_set_robin": ([_V, _I, _D, _D, _D, _D, _V], _I),
     "mpdg_heat_op_set_surface_flux": ([_V, C.POINTER(_D)], _I),
+    "mpdg_heat_op_set_mapping": ([_V, _V], _I),
     "mpdg_heat_op_vmult": ([_V, _V, _V, _V], _I),
     "mpdg_heat_op_vmult_f32": ([_V, _V, _V, _V], _I),
     "mpdg_heat_op_compute_inverse_diagonal": ([_V, _V, _V], _I),
@@ -240,6 +241,12 @@ class HeatDiffuseMultiPhaseOperator:
         self._keep["Trobin"] = T
         _check(self.lib.mpdg_heat_op_set_robin(self.h, face_mask, alpha, T_inf_conv, emissivity, T_inf_rad,
                                                _ptr(T)), "set_robin")
+
+    def set_mapping(self, coords):
+        """Mapped mesh: node coordinates (device FP64, X block, Y block, Z block)."""
+        assert coords.numel() == 3 * self.n_dofs
+        self._keep["X"] = coords
+        _check(self.lib.mpdg_heat_op_set_mapping(self.h, _ptr(coords, _torch().float64)), "set_mapping")
 
     def set_surface_flux(self, peak, sigma, x0, y0, vx, t):
         p = (C.c_double * 6)(peak, sigma, x0, y0, vx, t)

@@ -163,7 +163,7 @@ __device__ __forceinline__ Number law_kappa(int law, const Number *p, Number T) 
   return p[0] + (p[1] - p[0]) * (Number(3) - Number(2) * fs) * fs * fs;
 }
 
-template <int N1, typename Number, int KMODE>
+template <int N1, typename Number, int KMODE, bool MAPPED>
 __global__ __launch_bounds__(256) void heat_apply_kernel(const ApplyArgs<Number, N1> P,
                                                          const Number *__restrict__ src,
                                                          Number *__restrict__ dst) {
@@ -250,39 +250,49 @@ __global__ __launch_bounds__(256) void heat_apply_kernel(const ApplyArgs<Number,
     transpose_zy<N1>(slot, l, lane_on, b, u);
     sweep_row<N1, false>(P.S, u, gy);
 
-    // ---- quadrature point operation: val = c_m u, flux = kappa grad u, integrated
-    //      along z and x right away: e = c W v + Dz^T(kW gz) + Dx^T(kW gx) ----
+    // ---- quadrature point operation (heat_diffuse_operator.cpp:1029-1077):
+    //      val = c_m u and flux = kappa grad u at every point, then the z and x parts of the
+    //      integration right away: e = c W v + Dz^T(kW gz) + Dx^T(kW gx), in place of v ----
     const Number wy = P.w[l];
     Number kc = P.kappa;
     if (KMODE == KAPPA_CELL) kc = valid ? P.kappa_arr[cell] : Number(0);
-    Number kw[N1][N1], e[N1][N1];
-#pragma unroll
-    for (int qz = 0; qz < N1; ++qz)
-#pragma unroll
-      for (int qx = 0; qx < N1; ++qx) {
-        const Number W = P.w[qx] * wy * P.w[qz];
-        Number kap = kc;
-        if (KMODE == KAPPA_QP) kap = valid ? P.kappa_arr[cell * (N1 * N1 * N1) + qx + N1 * (l + N1 * qz)] : Number(0);
-        if (KMODE == KAPPA_LAW) kap = kq[qz][qx];
-        kw[qz][qx] = kap * W;
-        e[qz][qx] = P.cm * W * v[qz][qx];
-        gy[qz][qx] *= kw[qz][qx] * P.ks[1];
-      }
     {
-      Number g[N1][N1];
-      sweep_row<N1, false>(P.D, v, g);  // gz
+      Number gx[N1][N1], gz[N1][N1];
+      sweep_row<N1, false>(P.D, v, gz);
+      sweep_col<N1, false>(P.D, v, gx);
 #pragma unroll
       for (int qz = 0; qz < N1; ++qz)
 #pragma unroll
-        for (int qx = 0; qx < N1; ++qx) g[qz][qx] *= kw[qz][qx] * P.ks[2];
-      sweep_row_add<N1, true>(P.D, g, e);
-      sweep_col<N1, false>(P.D, v, g);  // gx
-#pragma unroll
-      for (int qz = 0; qz < N1; ++qz)
-#pragma unroll
-        for (int qx = 0; qx < N1; ++qx) g[qz][qx] *= kw[qz][qx] * P.ks[0];
-      sweep_col_add<N1, true>(P.D, g, e);
+        for (int qx = 0; qx < N1; ++qx) {
+          constexpr int NQ = N1 * N1 * N1;
+          const int q = qx + N1 * (l + N1 * qz);
+          Number kap = kc;
+          if (KMODE == KAPPA_QP) kap = valid ? P.kappa_arr[cell * NQ + q] : Number(0);
+          if (KMODE == KAPPA_LAW) kap = kq[qz][qx];
+          if (MAPPED) {
+            // flux in reference coordinates: kappa G grad_xi u, G = JxW J^-1 J^-T
+            const Number *M = P.metric + (valid ? cell : 0) * 7 * NQ + q;
+            const Number jxw = valid ? M[0] : Number(0);
+            const Number g00 = M[NQ], g01 = M[2 * NQ], g11 = M[3 * NQ], g02 = M[4 * NQ], g12 = M[5 * NQ],
+                         g22 = M[6 * NQ];
+            const Number a = gx[qz][qx], b = gy[qz][qx], c = gz[qz][qx];
+            v[qz][qx] *= P.cm * jxw;
+            gx[qz][qx] = kap * (g00 * a + g01 * b + g02 * c);
+            gy[qz][qx] = kap * (g01 * a + g11 * b + g12 * c);
+            gz[qz][qx] = kap * (g02 * a + g12 * b + g22 * c);
+          } else {
+            const Number W = P.w[qx] * wy * P.w[qz];
+            const Number kw = kap * W;
+            v[qz][qx] *= P.cm * W;
+            gx[qz][qx] *= kw * P.ks[0];
+            gy[qz][qx] *= kw * P.ks[1];
+            gz[qz][qx] *= kw * P.ks[2];
+          }
+        }
+      sweep_row_add<N1, true>(P.D, gz, v);
+      sweep_col_add<N1, true>(P.D, gx, v);
     }
+    Number (&e)[N1][N1] = v;
     // ---- back to the plane layout: e' = Sz^T e, ey = Sz^T (kW gy) ----
     sweep_row<N1, true>(P.S, e, v);
     transpose_yz<N1>(slot, l, lane_on, v, t);
@@ -365,21 +375,22 @@ int launch_apply_n1(const OperatorData<Number> &op, const Number *src, Number *d
     hipLaunchKernelGGL(zero_face_rows_kernel<Number>, grid, dim3(256), 0, stream, dst, op.nd[0],
                        op.nd[1], op.nd[2], (N1 - 1) * G::BY, (N1 - 1) * G::BZ);
   }
-  switch (op.kappa_mode) {
-    case KAPPA_CONST:
-      hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CONST>), dim3(ncols), dim3(256), 0, stream, A, src, dst);
-      break;
-    case KAPPA_QP:
-      hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_QP>), dim3(ncols), dim3(256), 0, stream, A, src, dst);
-      break;
-    case KAPPA_CELL:
-      hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CELL>), dim3(ncols), dim3(256), 0, stream, A, src, dst);
-      break;
-    case KAPPA_LAW:
-      hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_LAW>), dim3(ncols), dim3(256), 0, stream, A, src, dst);
-      break;
-    default:
-      return MPDG_ERR_INVALID;
+  if (op.metric) {
+    switch (op.kappa_mode) {
+      case KAPPA_CONST: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CONST, true>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
+      case KAPPA_QP: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_QP, true>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
+      case KAPPA_CELL: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CELL, true>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
+      case KAPPA_LAW: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_LAW, true>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
+      default: return MPDG_ERR_INVALID;
+    }
+  } else {
+    switch (op.kappa_mode) {
+      case KAPPA_CONST: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CONST, false>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
+      case KAPPA_QP: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_QP, false>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
+      case KAPPA_CELL: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CELL, false>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
+      case KAPPA_LAW: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_LAW, false>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
+      default: return MPDG_ERR_INVALID;
+    }
   }
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }

@@ -85,6 +85,7 @@ template <typename Number>
 __global__ __launch_bounds__(256) void diagonal_kernel(AuxTables<Number> t, int n1, int nc0, int nc1,
                                                        long ncells, int nd0, int nd1, Number cm, Number ks0,
                                                        Number ks1, Number ks2, const Number *__restrict__ kq,
+                                                       const Number *__restrict__ metric,
                                                        Number *__restrict__ diag) {
   const int nloc = n1 * n1 * n1;
   const long gid = (long)blockIdx.x * 256 + threadIdx.x;
@@ -101,10 +102,19 @@ __global__ __launch_bounds__(256) void diagonal_kernel(AuxTables<Number> t, int 
         const Number W = t.w[qx] * t.w[qy] * t.w[qz];
         const Number sx = t.S[qx][i], sy = t.S[qy][j], sz = t.S[qz][l];
         const Number gx = t.G[qx][i], gy = t.G[qy][j], gz = t.G[qz][l];
-        const Number k = kq[cell * nloc + qx + n1 * (qy + n1 * qz)];
+        const int q = qx + n1 * (qy + n1 * qz);
+        const Number k = kq[cell * nloc + q];
         const Number m = sx * sy * sz;
-        acc += W * (cm * m * m + k * (ks0 * gx * gx * sy * sy * sz * sz + ks1 * sx * sx * gy * gy * sz * sz +
-                                      ks2 * sx * sx * sy * sy * gz * gz));
+        if (metric) {
+          const Number *M = metric + cell * 7 * nloc + q;
+          const Number d0 = gx * sy * sz, d1 = sx * gy * sz, d2 = sx * sy * gz;
+          acc += cm * M[0] * m * m +
+                 k * (M[nloc] * d0 * d0 + M[3 * nloc] * d1 * d1 + M[6 * nloc] * d2 * d2 +
+                      Number(2) * (M[2 * nloc] * d0 * d1 + M[4 * nloc] * d0 * d2 + M[5 * nloc] * d1 * d2));
+        } else {
+          acc += W * (cm * m * m + k * (ks0 * gx * gx * sy * sy * sz * sz + ks1 * sx * sx * gy * gy * sz * sz +
+                                        ks2 * sx * sx * sy * sy * gz * gz));
+        }
       }
   const long g = (long)(K * cx + i) + (long)nd0 * ((K * cy + j) + (long)nd1 * (K * cz + l));
   atomicAdd(diag + g, acc);
@@ -225,6 +235,7 @@ int launch_boundary(const OperatorData<double> &op, int mode, const double *flux
   A.flux_on = (mode == 2 && flux && flux[0] != 0.0) ? 1 : 0;
   A.face_mask = op.robin_mask | (A.flux_on ? 32 : 0);
   if (!A.face_mask) return MPDG_OK;
+  if (op.metric) return MPDG_ERR_INVALID;  // boundary faces are implemented for the affine box
   A.dmask = op.dmask;
   A.mode = mode;
   A.alpha = op.alpha;
@@ -249,12 +260,12 @@ int launch_diagonal(const OperatorData<Number> &op, Number *diag, Number *kq, hi
   int rc = launch_kappa_qp<Number>(op, kq, stream);
   if (rc) return rc;
   hipMemsetAsync(diag, 0, sizeof(Number) * op.n_dofs, stream);
-  const double vol = op.h[0] * op.h[1] * op.h[2];
+  const double vol = op.metric ? 1.0 : op.h[0] * op.h[1] * op.h[2];
   const long n = op.n_cells * n1 * n1 * n1;
   hipLaunchKernelGGL(diagonal_kernel<Number>, dim3((n + 255) / 256), dim3(256), 0, stream,
                      make_aux_tables<Number>(op.degree), n1, op.nc[0], op.nc[1], op.n_cells, op.nd[0], op.nd[1],
                      (Number)(op.c_mass * vol), (Number)(vol / (op.h[0] * op.h[0])),
-                     (Number)(vol / (op.h[1] * op.h[1])), (Number)(vol / (op.h[2] * op.h[2])), kq, diag);
+                     (Number)(vol / (op.h[1] * op.h[1])), (Number)(vol / (op.h[2] * op.h[2])), kq, op.metric, diag);
   if (op.dmask)
     hipLaunchKernelGGL(diag_dirichlet_kernel<Number>, dim3((op.n_dofs + 255) / 256), dim3(256), 0, stream, diag,
                        op.nd[0], op.nd[1], op.nd[2], op.dmask);
@@ -299,6 +310,70 @@ int launch_coarsen_cells(const int *ncf, const Number *kf, Number *kc, hipStream
                      ncf[2], kf, kc);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
+
+// Geometry of mapped cells at every quadrature point from the mapping node coordinates
+// (MappingQ(k) with the FE's support points, source/utilities/fe_util.cpp:18-28):
+// J[a][b] = d x_a / d xi_b, B = J^-1, JxW = |det J| w, G = JxW B B^T.
+template <typename Number>
+__global__ __launch_bounds__(256) void metric_kernel(AuxTables<double> t, int n1, int nc0, int nc1, long ncells,
+                                                     int nd0, int nd1, long ndofs, const double *__restrict__ X,
+                                                     Number *__restrict__ metric) {
+  const int nq = n1 * n1 * n1, K = n1 - 1;
+  const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+  if (gid >= ncells * nq) return;
+  const long cell = gid / nq;
+  const int q = gid - cell * nq;
+  const int qx = q % n1, qy = (q / n1) % n1, qz = q / (n1 * n1);
+  const int cx = cell % nc0, cy = (cell / nc0) % nc1, cz = cell / ((long)nc0 * nc1);
+  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int l = 0; l < n1; ++l)
+    for (int j = 0; j < n1; ++j)
+      for (int i = 0; i < n1; ++i) {
+        const long g = (long)(K * cx + i) + (long)nd0 * ((K * cy + j) + (long)nd1 * (K * cz + l));
+        const double d0 = t.G[qx][i] * t.S[qy][j] * t.S[qz][l], d1 = t.S[qx][i] * t.G[qy][j] * t.S[qz][l],
+                     d2 = t.S[qx][i] * t.S[qy][j] * t.G[qz][l];
+        for (int a = 0; a < 3; ++a) {
+          const double x = X[a * ndofs + g];
+          J[a][0] += x * d0;
+          J[a][1] += x * d1;
+          J[a][2] += x * d2;
+        }
+      }
+  const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                     J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  const double id = 1.0 / det;
+  double B[3][3];
+  B[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
+  B[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+  B[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+  B[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+  B[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+  B[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+  B[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+  B[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+  B[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+  const double jxw = fabs(det) * t.w[qx] * t.w[qy] * t.w[qz];
+  auto g = [&](int a, int b) { return jxw * (B[a][0] * B[b][0] + B[a][1] * B[b][1] + B[a][2] * B[b][2]); };
+  Number *M = metric + cell * 7 * nq + q;
+  M[0] = (Number)jxw;
+  M[nq] = (Number)g(0, 0);
+  M[2 * nq] = (Number)g(0, 1);
+  M[3 * nq] = (Number)g(1, 1);
+  M[4 * nq] = (Number)g(0, 2);
+  M[5 * nq] = (Number)g(1, 2);
+  M[6 * nq] = (Number)g(2, 2);
+}
+
+template <typename Number>
+int launch_metric(const OperatorData<Number> &op, const double *coords, Number *metric, hipStream_t s) {
+  const int n1 = op.degree + 1;
+  const long n = op.n_cells * n1 * n1 * n1;
+  hipLaunchKernelGGL(metric_kernel<Number>, dim3((n + 255) / 256), dim3(256), 0, s, make_aux_tables<double>(op.degree),
+                     n1, op.nc[0], op.nc[1], op.n_cells, op.nd[0], op.nd[1], op.n_dofs, coords, metric);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+template int launch_metric<double>(const OperatorData<double> &, const double *, double *, hipStream_t);
+template int launch_metric<float>(const OperatorData<float> &, const double *, float *, hipStream_t);
 
 // nodal values of the z+ surface flux q = peak exp(-((x - x0 - vx t)^2 + (y - y0)^2)/(2 sigma^2)), 0 elsewhere
 __global__ __launch_bounds__(256) void surface_flux_field_kernel(double *__restrict__ out, int nd0, int nd1, int nd2,

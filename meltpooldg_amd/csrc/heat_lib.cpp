@@ -227,6 +227,33 @@ struct Transfer1DHost {
   }
 };
 
+// 1D evaluation of a fine Q_kf function at the nodes of (kc, ncc): carries the mapping
+// node coordinates from a level to the next coarser one.
+struct Interp1DHost {
+  int nf = 0, nc = 0, w = 0;
+  std::vector<int> col;
+  std::vector<double> val;
+  Interp1DHost(int kf, int ncf, int kc, int ncc) {
+    FE1D fc(kc), ff(kf);
+    nf = kf * ncf + 1;
+    nc = kc * ncc + 1;
+    w = kf + 1;
+    col.assign((size_t)nc * w, -1);
+    val.assign((size_t)nc * w, 0.0);
+    for (int g = 0; g < nc; ++g) {
+      const int cc = std::min(g / kc, ncc - 1), i = g - kc * cc;
+      const double xi = (cc + fc.nodes[i]) * (double)ncf / (double)ncc;
+      const int cf = std::min((int)std::floor(xi), ncf - 1);
+      double v[kMaxN1];
+      FE1D::lagrange(kf + 1, ff.nodes, xi - cf, v, nullptr);
+      for (int j = 0; j < w; ++j) {
+        col[(size_t)g * w + j] = kf * cf + j;
+        val[(size_t)g * w + j] = v[j];
+      }
+    }
+  }
+};
+
 template <typename T>
 struct Level {
   OperatorData<T> od;
@@ -238,6 +265,8 @@ struct Level {
   int pw[3] = {0, 0, 0}, rw[3] = {0, 0, 0};
   DeviceArray<T> t1, t2;
   std::vector<double> cheb_c1, cheb_c2;
+  DeviceArray<double> coords;  // mapping node coordinates of this level (mapped meshes)
+  DeviceArray<T> metric;
 };
 
 template <typename T>
@@ -334,6 +363,7 @@ struct GMGPrecond : mpdg_precond {
       MPDG_TRY(L.t1.resize(t1));
       MPDG_TRY(L.t2.resize(t2));
     }
+    if (op->coords) MPDG_TRY(build_geometry());
     MPDG_TRY(red.init());
     MPDG_TRY(eig_sc.resize(4));
     MPDG_TRY(eig_hist.resize(2 * std::max(prm.eig_cg_n_iterations, 1) + 2));
@@ -346,12 +376,59 @@ struct GMGPrecond : mpdg_precond {
     return MPDG_OK;
   }
 
+  // mapped meshes: every level's node coordinates are the next finer level's mapping
+  // evaluated at its nodes; each level keeps its metric in precision T
+  int build_geometry() {
+    const auto &f = op->d64;
+    Level<T> &top = *lev.back();
+    MPDG_TRY(top.coords.resize(3 * (size_t)f.n_dofs));
+    if (hipMemcpy(top.coords.get(), op->coords, 3 * f.n_dofs * sizeof(double), hipMemcpyDeviceToDevice) != hipSuccess)
+      return MPDG_ERR_DEVICE;
+    for (int l = (int)lev.size() - 2; l >= 0; --l) {
+      Level<T> &P = *lev[l + 1];
+      Level<T> &L = *lev[l];
+      DeviceArray<int> col[3];
+      DeviceArray<double> val[3];
+      int w[3];
+      for (int d = 0; d < 3; ++d) {
+        Interp1DHost e(P.od.degree, P.od.nc[d], L.od.degree, L.od.nc[d]);
+        w[d] = e.w;
+        MPDG_TRY(upload_int(col[d], e.col));
+        MPDG_TRY(upload<double>(val[d], e.val));
+      }
+      MPDG_TRY(L.coords.resize(3 * (size_t)L.od.n_dofs));
+      DeviceArray<double> a, b;
+      const int f3[3] = {P.od.nd[0], P.od.nd[1], P.od.nd[2]};
+      const int o1[3] = {L.od.nd[0], P.od.nd[1], P.od.nd[2]}, o2[3] = {L.od.nd[0], L.od.nd[1], P.od.nd[2]};
+      const int o3[3] = {L.od.nd[0], L.od.nd[1], L.od.nd[2]};
+      MPDG_TRY(a.resize((size_t)o1[0] * o1[1] * o1[2]));
+      MPDG_TRY(b.resize((size_t)o2[0] * o2[1] * o2[2]));
+      for (int c = 0; c < 3; ++c) {
+        MPDG_TRY(launch_ell_dir<double>(P.coords.get() + c * P.od.n_dofs, a.get(), o1, f3, 0, w[0], col[0].get(),
+                                        val[0].get(), nullptr));
+        MPDG_TRY(launch_ell_dir<double>(a.get(), b.get(), o2, o1, 1, w[1], col[1].get(), val[1].get(), nullptr));
+        MPDG_TRY(launch_ell_dir<double>(b.get(), L.coords.get() + c * L.od.n_dofs, o3, o2, 2, w[2], col[2].get(),
+                                        val[2].get(), nullptr));
+      }
+      if (hipDeviceSynchronize() != hipSuccess) return MPDG_ERR_DEVICE;
+    }
+    for (auto &L : lev) {
+      const int n1 = L->od.degree + 1;
+      MPDG_TRY(L->metric.resize((size_t)L->od.n_cells * 7 * n1 * n1 * n1));
+      MPDG_TRY(launch_metric<T>(L->od, L->coords.get(), L->metric.get(), nullptr));
+      L->od.metric = L->metric.get();
+      L->od.coords = L->coords.get();
+    }
+    return hipDeviceSynchronize() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+  }
+
   // level data that depends on the fine operator's coefficient and time increment
   int update(hipStream_t s) override {
     const auto &f = op->d64;
     Level<T> &top = *lev.back();
     for (auto &L : lev) L->od.c_mass = f.c_mass;
     // finest level: the fine operator's conductivity in precision T
+    if (op->coords && !top.od.metric) MPDG_TRY(build_geometry());
     top.od.kappa_mode = f.kappa_mode;
     top.od.kappa = f.kappa;
     if (f.kappa_mode == KAPPA_QP) {
@@ -499,15 +576,36 @@ struct GMGPrecond : mpdg_precond {
         if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
       }
     }
+    std::vector<double> met;
+    if (od.metric) {
+      std::vector<T> mt((size_t)od.n_cells * 7 * nloc);
+      if (hipMemcpy(mt.data(), od.metric, mt.size() * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess)
+        return MPDG_ERR_DEVICE;
+      met.assign(mt.begin(), mt.end());
+    }
     for (int cz = 0; cz < od.nc[2]; ++cz)
       for (int cy = 0; cy < od.nc[1]; ++cy)
         for (int cx = 0; cx < od.nc[0]; ++cx) {
-          const double kap = kc[cx + (size_t)od.nc[0] * (cy + (size_t)od.nc[1] * cz)];
+          const size_t cell = cx + (size_t)od.nc[0] * (cy + (size_t)od.nc[1] * cz);
+          const double kap = kc[cell];
           std::fill(Ae.begin(), Ae.end(), 0.0);
           for (int qz = 0; qz < n1; ++qz)
             for (int qy = 0; qy < n1; ++qy)
               for (int qx = 0; qx < n1; ++qx) {
                 const double W = fe.qw[qx] * fe.qw[qy] * fe.qw[qz];
+                const int q = qx + n1 * (qy + n1 * qz);
+                double Gm[3][3] = {{ks[0] * W, 0, 0}, {0, ks[1] * W, 0}, {0, 0, ks[2] * W}};
+                double mw = cm * W;
+                if (od.metric) {
+                  const double *M = met.data() + cell * 7 * nloc + q;
+                  mw = f.c_mass * M[0];
+                  Gm[0][0] = M[nloc];
+                  Gm[0][1] = Gm[1][0] = M[2 * nloc];
+                  Gm[1][1] = M[3 * nloc];
+                  Gm[0][2] = Gm[2][0] = M[4 * nloc];
+                  Gm[1][2] = Gm[2][1] = M[5 * nloc];
+                  Gm[2][2] = M[6 * nloc];
+                }
                 for (int I = 0; I < nloc; ++I) {
                   const int i0 = I % n1, i1 = (I / n1) % n1, i2 = I / (n1 * n1);
                   const double pI = fe.S[qx][i0] * fe.S[qy][i1] * fe.S[qz][i2];
@@ -520,8 +618,10 @@ struct GMGPrecond : mpdg_precond {
                     const double gJ[3] = {fe.G[qx][j0] * fe.S[qy][j1] * fe.S[qz][j2],
                                           fe.S[qx][j0] * fe.G[qy][j1] * fe.S[qz][j2],
                                           fe.S[qx][j0] * fe.S[qy][j1] * fe.G[qz][j2]};
-                    Ae[(size_t)I * nloc + J] +=
-                        W * (cm * pI * pJ + kap * (ks[0] * gI[0] * gJ[0] + ks[1] * gI[1] * gJ[1] + ks[2] * gI[2] * gJ[2]));
+                    double gg = 0.0;
+                    for (int a = 0; a < 3; ++a)
+                      for (int b = 0; b < 3; ++b) gg += gI[a] * Gm[a][b] * gJ[b];
+                    Ae[(size_t)I * nloc + J] += mw * pI * pJ + kap * gg;
                   }
                 }
               }
@@ -787,6 +887,24 @@ int mpdg_heat_op_set_robin(mpdg_heat_op *op, int mask, double alpha, double T_in
   op->T_robin = T;
   return MPDG_OK;
 }
+int mpdg_heat_op_set_mapping(mpdg_heat_op *op, const double *coords) {
+  if (!op || !coords) return MPDG_ERR_INVALID;
+  const int n1 = op->d64.degree + 1;
+  const size_t nm = (size_t)op->d64.n_cells * 7 * n1 * n1 * n1;
+  op->coords = coords;
+  op->d64.coords = op->d32.coords = coords;
+  if (op->precision == MPDG_FP64) {
+    MPDG_TRY(op->metric64.resize(nm));
+    MPDG_TRY(launch_metric<double>(op->d64, coords, op->metric64.get(), nullptr));
+    op->d64.metric = op->metric64.get();
+  } else {
+    MPDG_TRY(op->metric32.resize(nm));
+    MPDG_TRY(launch_metric<float>(op->d32, coords, op->metric32.get(), nullptr));
+    op->d32.metric = op->metric32.get();
+  }
+  return hipDeviceSynchronize() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
 int mpdg_heat_op_set_surface_flux(mpdg_heat_op *op, const double *p) {
   if (!op || !p) return MPDG_ERR_INVALID;
   for (int i = 0; i < 6; ++i) op->flux[i] = p[i];

@@ -81,7 +81,9 @@ struct mpdg_heat_op {
   mpdg::OperatorData<float> d32;
   double flux[6] = {0, 0, 0, 0, 0, 0};
   const double *T_robin = nullptr;
-  mutable mpdg::DeviceArray<double> kq, tmp1, tmp2, scal, tables;
+  mutable mpdg::DeviceArray<double> kq, tmp1, tmp2, scal, tables, metric64;
+  mpdg::DeviceArray<float> metric32;
+  const double *coords = nullptr;
   mutable mpdg::ReductionScratch red;
   // optional HIP-event timing of the cell-apply kernel (bench / profiling)
   mutable std::vector<std::pair<hipEvent_t, hipEvent_t>> timing_events;

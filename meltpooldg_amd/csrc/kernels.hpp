@@ -34,6 +34,9 @@ struct OperatorData {
   int robin_mask = 0;
   double alpha = 0, emissivity = 0, T_inf_conv = 0, T_inf_rad = 0;
   const double *T_robin = nullptr;  // temperature for the radiation linearisation (FP64)
+  // mapped geometry: per quadrature point [cell][7][q]: JxW, then G = JxW J^-1 J^-T as (00, 01, 11, 02, 12, 22)
+  const Number *metric = nullptr;
+  const double *coords = nullptr;   // mapping node coordinates (X block, Y block, Z block), FP64
   void set_mesh(int k, const int *ncells, const double *ext) {
     degree = k;
     for (int d = 0; d < 3; ++d) {
@@ -60,6 +63,7 @@ struct ApplyArgs {
   Number law_p[4];
   const Number *kappa_arr;
   const Number *T;
+  const Number *metric;
 };
 
 template <typename Number, int N1>
@@ -76,7 +80,8 @@ void fill_apply_args(const OperatorData<Number> &op, ApplyArgs<Number, N1> &A) {
     A.nc[d] = op.nc[d];
     A.nd[d] = op.nd[d];
   }
-  const double vol = op.h[0] * op.h[1] * op.h[2];
+  const double vol = op.metric ? 1.0 : op.h[0] * op.h[1] * op.h[2];
+  A.metric = op.metric;
   A.cm = (Number)(op.c_mass * vol);
   for (int d = 0; d < 3; ++d) A.ks[d] = (Number)(vol / (op.h[d] * op.h[d]));
   A.dmask = op.dmask;
@@ -103,6 +108,8 @@ struct FluxNodes {
   double nodes[kMaxN1];
   double p[6];
 };
+template <typename Number>
+int launch_metric(const OperatorData<Number> &op, const double *coords, Number *metric, hipStream_t s);
 int launch_surface_flux_field(const OperatorData<double> &op, const double *p6, double *out, hipStream_t s);
 template <typename Number>
 int launch_kappa_qp(const OperatorData<Number> &op, Number *kq, hipStream_t stream);

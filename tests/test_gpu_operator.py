@@ -39,6 +39,24 @@ def test_vmult_matches_oracle(cuda, degree, nc, mode, dmask):
     assert rel_err(yd.cpu().numpy(), ref) < 1e-12
 
 
+@pytest.mark.parametrize("degree,nc,mode,dmask", [(3, (6, 6, 6), "const", 0), (3, (5, 4, 7), "qp", 0),
+                                                 (2, (6, 5, 4), "law", 0b001100), (4, (4, 4, 3), "cell", 0)])
+def test_vmult_mapped(cuda, degree, nc, mode, dmask):
+    """Mesh mapped by x + 0.05 sin(2 pi y) sin(2 pi z) on all MappingQ_k nodes."""
+    torch = cuda
+    orc, gop, _ = make_pair(torch, degree, nc, mode=mode, dirichlet=dmask, mapped=0.05, seed=5)
+    x = np.random.default_rng(8).uniform(-1.0, 1.0, orc.n_dofs)
+    ref = orc.vmult(x)
+    y = torch.empty(orc.n_dofs, dtype=torch.float64, device="cuda")
+    gop.vmult(y, torch.tensor(x, device="cuda"))
+    torch.cuda.synchronize()
+    assert rel_err(y.cpu().numpy(), ref) < 1e-12
+    d = torch.empty_like(y)
+    gop.compute_inverse_diagonal_from_matrixfree(d)
+    torch.cuda.synchronize()
+    assert rel_err(d.cpu().numpy(), orc.inverse_diagonal()) < 1e-12
+
+
 @pytest.mark.parametrize("degree,nc,mode", [(2, (6, 6, 6), "qp"), (3, (5, 4, 7), "law"), (4, (4, 4, 4), "const")])
 def test_vmult_fp32(cuda, degree, nc, mode):
     torch = cuda

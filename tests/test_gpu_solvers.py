@@ -33,11 +33,14 @@ def test_cg_jacobi_q2_16(cuda):
     (3, (8, 8, 8), "qp", 0, 2),
     (3, (12, 12, 12), "law", 0, 2),
     (2, (8, 8, 8), "cell", 0b000011, 0),
+    (3, (8, 8, 8), "qp", 0, -2),  # mapped mesh, coarse Q2
 ])
 def test_gmg_vcycle_fp64_matches_oracle(cuda, degree, nc, mode, dmask, coarse_degree):
     torch = cuda
     import meltpooldg_amd as M
-    orc, gop, _ = make_pair(torch, degree, nc, mode=mode, dirichlet=dmask, seed=4)
+    mapped = 0.05 if coarse_degree < 0 else None
+    coarse_degree = abs(coarse_degree)
+    orc, gop, _ = make_pair(torch, degree, nc, mode=mode, dirichlet=dmask, seed=4, mapped=mapped)
     og = O.GMG(orc, nc_coarse=2, coarse_degree=coarse_degree, cheb_degree=3, smoothing_range=20.0, eig_iters=20)
     pc = M.GMGPreconditioner(gop, precision=M.FP64, n_cells_coarse=2, coarse_degree=coarse_degree,
                              smoother_degree=3, smoothing_range=20.0, eig_cg_n_iterations=20)
