@@ -294,10 +294,10 @@ __global__ __launch_bounds__(256) void heat_apply_kernel(const ApplyArgs<Number,
     }
     Number (&e)[N1][N1] = v;
     // ---- back to the plane layout: e' = Sz^T e, ey = Sz^T (kW gy) ----
-    sweep_row<N1, true>(P.S, e, v);
-    transpose_yz<N1>(slot, l, lane_on, v, t);
-    sweep_row<N1, true>(P.S, gy, v);
-    transpose_yz<N1>(slot, l, lane_on, v, b);
+    sweep_row<N1, true>(P.S, e, u);
+    transpose_yz<N1>(slot, l, lane_on, u, t);
+    sweep_row<N1, true>(P.S, gy, u);
+    transpose_yz<N1>(slot, l, lane_on, u, b);
     // ---- plane layout: out = Sx^T Sy^T (e' + Dy^T ey) ----
     sweep_row_add<N1, true>(P.D, b, t);
     sweep_row<N1, true>(P.S, t, b);
@@ -361,8 +361,10 @@ __global__ __launch_bounds__(256) void zero_face_rows_kernel(Number *__restrict_
 }
 
 template <int N1, typename Number>
-int launch_apply_n1(const OperatorData<Number> &op, const Number *src, Number *dst, hipStream_t stream) {
+int launch_apply_n1(const OperatorData<Number> &op_in, const Number *src, Number *dst, hipStream_t stream) {
   using G = ApplyGeom<N1>;
+  const OperatorData<Number> op = kernel_view(op_in);
+  if (op.metric == nullptr && op_in.metric != nullptr) return MPDG_ERR_INVALID;  // metric_k not computed
   ApplyArgs<Number, N1> A;
   fill_apply_args(op, A);
   A.ncol[0] = (op.nc[1] + G::BY - 1) / G::BY;
@@ -378,10 +380,8 @@ int launch_apply_n1(const OperatorData<Number> &op, const Number *src, Number *d
   if (op.metric) {
     switch (op.kappa_mode) {
       case KAPPA_CONST: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CONST, true>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
-      case KAPPA_QP: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_QP, true>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
       case KAPPA_CELL: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CELL, true>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
-      case KAPPA_LAW: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_LAW, true>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
-      default: return MPDG_ERR_INVALID;
+      default: return MPDG_ERR_INVALID;  // per-qp conductivity on mapped meshes runs through metric_k
     }
   } else {
     switch (op.kappa_mode) {

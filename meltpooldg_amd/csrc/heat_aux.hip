@@ -255,7 +255,9 @@ int launch_boundary(const OperatorData<double> &op, int mode, const double *flux
 }
 
 template <typename Number>
-int launch_diagonal(const OperatorData<Number> &op, Number *diag, Number *kq, hipStream_t stream) {
+int launch_diagonal(const OperatorData<Number> &op_in, Number *diag, Number *kq, hipStream_t stream) {
+  const OperatorData<Number> op = kernel_view(op_in);
+  if (op.metric == nullptr && op_in.metric != nullptr) return MPDG_ERR_INVALID;
   const int n1 = op.degree + 1;
   int rc = launch_kappa_qp<Number>(op, kq, stream);
   if (rc) return rc;
@@ -372,6 +374,28 @@ int launch_metric(const OperatorData<Number> &op, const double *coords, Number *
                      n1, op.nc[0], op.nc[1], op.n_cells, op.nd[0], op.nd[1], op.n_dofs, coords, metric);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
+// metric_k[c][0][q] = metric[c][0][q] (JxW), metric_k[c][i][q] = kq[c][q] metric[c][i][q] (i = 1..6)
+template <typename Number>
+__global__ __launch_bounds__(256) void scale_metric_kernel(const Number *__restrict__ metric, const Number *__restrict__ kq,
+                                                           Number *__restrict__ out, long ncells, int nq) {
+  const long n = ncells * 7 * nq;
+  for (long g = (long)blockIdx.x * 256 + threadIdx.x; g < n; g += (long)gridDim.x * 256) {
+    const long c = g / (7 * nq);
+    const int r = g - c * 7 * nq, comp = r / nq, q = r - comp * nq;
+    out[g] = comp == 0 ? metric[g] : kq[c * nq + q] * metric[g];
+  }
+}
+template <typename Number>
+int launch_scale_metric(const OperatorData<Number> &op, const Number *kq, Number *metric_k, hipStream_t s) {
+  const int n1 = op.degree + 1, nq = n1 * n1 * n1;
+  const long n = op.n_cells * 7 * nq;
+  hipLaunchKernelGGL(scale_metric_kernel<Number>, dim3(std::min<long>((n + 255) / 256, 16384)), dim3(256), 0, s,
+                     op.metric, kq, metric_k, op.n_cells, nq);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+template int launch_scale_metric<double>(const OperatorData<double> &, const double *, double *, hipStream_t);
+template int launch_scale_metric<float>(const OperatorData<float> &, const float *, float *, hipStream_t);
+
 template int launch_metric<double>(const OperatorData<double> &, const double *, double *, hipStream_t);
 template int launch_metric<float>(const OperatorData<float> &, const double *, float *, hipStream_t);
 

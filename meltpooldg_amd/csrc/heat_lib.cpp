@@ -20,6 +20,7 @@ inline int dev_ok() { return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ER
 // ===========================================================================
 int mpdg_heat_op::vmult_cells(double *dst, const double *src, hipStream_t s) const {
   if (precision != MPDG_FP64) return MPDG_ERR_INVALID;
+  if (d64.metric && d64.kappa_mode == KAPPA_LAW) MPDG_TRY(refresh_metric_k(s));  // k(T) with the current T
   if (!timing_on) return launch_apply<double>(d64, src, dst, s);
   if (timing_used == timing_events.size()) {
     hipEvent_t a, b;
@@ -33,6 +34,28 @@ int mpdg_heat_op::vmult_cells(double *dst, const double *src, hipStream_t s) con
   return rc;
 }
 
+int mpdg_heat_op::refresh_metric_k(hipStream_t s) const {
+  const bool needs = (precision == MPDG_FP64 ? d64.metric : (const double *)d32.metric) &&
+                     (d64.kappa_mode == KAPPA_QP || d64.kappa_mode == KAPPA_LAW);
+  if (!needs) return MPDG_OK;
+  const int n1 = d64.degree + 1;
+  const size_t nq = (size_t)d64.n_cells * n1 * n1 * n1;
+  if (precision == MPDG_FP64) {
+    MPDG_TRY(kq.resize(nq));
+    MPDG_TRY(metric64_k.resize(nq * 7));
+    MPDG_TRY(launch_kappa_qp<double>(d64, kq.get(), s));
+    MPDG_TRY(launch_scale_metric<double>(d64, kq.get(), metric64_k.get(), s));
+    d64.metric_k = metric64_k.get();
+  } else {
+    MPDG_TRY(kq32.resize(nq));
+    MPDG_TRY(metric32_k.resize(nq * 7));
+    MPDG_TRY(launch_kappa_qp<float>(d32, kq32.get(), s));
+    MPDG_TRY(launch_scale_metric<float>(d32, kq32.get(), metric32_k.get(), s));
+    d32.metric_k = metric32_k.get();
+  }
+  return MPDG_OK;
+}
+
 // HeatDiffuseMultiPhaseOperator::vmult: cell loop + boundary loop, dst zeroed
 // (heat_diffuse_operator.cpp:181-197)
 int mpdg_heat_op::vmult(double *dst, const double *src, hipStream_t s) const {
@@ -43,11 +66,13 @@ int mpdg_heat_op::vmult(double *dst, const double *src, hipStream_t s) const {
 
 int mpdg_heat_op::vmult_f32(float *dst, const float *src, hipStream_t s) const {
   if (precision != MPDG_FP32) return MPDG_ERR_INVALID;
+  if (d32.metric && d32.kappa_mode == KAPPA_LAW) MPDG_TRY(refresh_metric_k(s));
   return launch_apply<float>(d32, src, dst, s);
 }
 
 int mpdg_heat_op::diagonal64(double *diag, hipStream_t s) const {
   const int n1 = d64.degree + 1;
+  MPDG_TRY(refresh_metric_k(s));
   MPDG_TRY(kq.resize((size_t)d64.n_cells * n1 * n1 * n1));
   MPDG_TRY(launch_diagonal<double>(d64, diag, kq.get(), s));
   if (d64.robin_mask) MPDG_TRY(launch_boundary(d64, 1, nullptr, diag, nullptr, T_robin, s));
@@ -61,9 +86,10 @@ int mpdg_heat_op::inverse_diagonal(double *inv, hipStream_t s) const {
 }
 
 // create_rhs (heat_diffuse_operator.cpp:785-806, cell part :395-529, faces :721-781):
-//   dst += -[(c_m (T - T_old) - q) N + k(T) grad T . grad N] + boundary terms
-// evaluated as  dst += M (c_m T_old + q) - J_cells(T) T + boundary residual,
-// with J_cells the cell part of the tangent at T (the same quadrature).
+//   dst += -[(c_m (T - T_old) - q) N + k(T) grad T . grad N] + boundary terms.
+// As in the reference the difference T - T_old is formed before the mass is applied
+// (:493-497), which keeps the residual free of the cancellation between c_m M T and
+// c_m M T_old:  dst -= M_{c_m} (T - T_old - q/c_m) + K_{k(T)} T, then the boundary residual.
 int mpdg_heat_op::create_rhs(double *dst, const double *T, const double *T_old, const double *q,
                              hipStream_t s) const {
   if (precision != MPDG_FP64) return MPDG_ERR_INVALID;
@@ -71,20 +97,31 @@ int mpdg_heat_op::create_rhs(double *dst, const double *T, const double *T_old, 
   const long n = d64.n_dofs;
   MPDG_TRY(tmp1.resize(n));
   MPDG_TRY(tmp2.resize(n));
-  // tmp2 = c_m T_old + q ; tmp1 = M tmp2
-  MPDG_TRY(launch_convert<double, double>(T_old, tmp2.get(), n, s));
-  MPDG_TRY(launch_axpby(tmp2.get(), 0.0, tmp2.get(), d64.c_mass, n, s));
-  if (q) MPDG_TRY(launch_axpby(tmp2.get(), 1.0, q, 1.0, n, s));
+  // tmp2 = T - T_old - q / c_m ; tmp1 = c_m M tmp2
+  MPDG_TRY(launch_convert<double, double>(T, tmp2.get(), n, s));
+  MPDG_TRY(launch_axpby(tmp2.get(), -1.0, T_old, 1.0, n, s));
+  if (q) MPDG_TRY(launch_axpby(tmp2.get(), -1.0 / d64.c_mass, q, 1.0, n, s));
   OperatorData<double> mass = d64;
-  mass.c_mass = 1.0;
   mass.kappa_mode = KAPPA_CONST;
   mass.kappa = 0.0;
   mass.robin_mask = 0;
   MPDG_TRY(launch_apply<double>(mass, tmp2.get(), tmp1.get(), s));
-  MPDG_TRY(launch_axpby(dst, 1.0, tmp1.get(), 1.0, n, s));
-  // tmp1 = J_cells(T) T, conductivity law evaluated at T
+  MPDG_TRY(launch_axpby(dst, -1.0, tmp1.get(), 1.0, n, s));
+  // tmp1 = K_{k(T)} T, conductivity law evaluated at T
   OperatorData<double> jt = d64;
-  if (jt.kappa_mode == KAPPA_LAW) jt.T = T;
+  jt.c_mass = 0.0;
+  if (jt.kappa_mode == KAPPA_LAW) {
+    jt.T = T;
+    if (jt.metric) {
+      const int n1 = d64.degree + 1;
+      const size_t nq = (size_t)d64.n_cells * n1 * n1 * n1;
+      MPDG_TRY(kq.resize(nq));
+      MPDG_TRY(metric64_k.resize(nq * 7));
+      MPDG_TRY(launch_kappa_qp<double>(jt, kq.get(), s));
+      MPDG_TRY(launch_scale_metric<double>(jt, kq.get(), metric64_k.get(), s));
+      jt.metric_k = metric64_k.get();
+    }
+  }
   MPDG_TRY(launch_apply<double>(jt, T, tmp1.get(), s));
   MPDG_TRY(launch_axpby(dst, -1.0, tmp1.get(), 1.0, n, s));
   // boundary residuals and surface flux
@@ -266,7 +303,7 @@ struct Level {
   DeviceArray<T> t1, t2;
   std::vector<double> cheb_c1, cheb_c2;
   DeviceArray<double> coords;  // mapping node coordinates of this level (mapped meshes)
-  DeviceArray<T> metric;
+  DeviceArray<T> metric, metric_k;
 };
 
 template <typename T>
@@ -449,6 +486,16 @@ struct GMGPrecond : mpdg_precond {
         MPDG_TRY(launch_convert<double, T>(f.T, top.Tlev.get(), f.n_dofs, s));
         top.od.T = top.Tlev.get();
       }
+    }
+    // mapped mesh with per-qp / k(T) conductivity: fold it into the level metric
+    if (top.od.metric && (top.od.kappa_mode == KAPPA_QP || top.od.kappa_mode == KAPPA_LAW)) {
+      const int n1 = top.od.degree + 1;
+      const size_t nq = (size_t)top.od.n_cells * n1 * n1 * n1;
+      MPDG_TRY(top.kq_scratch.resize(nq));
+      MPDG_TRY(top.metric_k.resize(nq * 7));
+      MPDG_TRY(launch_kappa_qp<T>(top.od, top.kq_scratch.get(), s));
+      MPDG_TRY(launch_scale_metric<T>(top.od, top.kq_scratch.get(), top.metric_k.get(), s));
+      top.od.metric_k = top.metric_k.get();
     }
     // quadrature-weighted cell averages of the fine conductivity
     if (lev.size() > 1) MPDG_TRY(launch_op_cell_average(f, kcell64.get(), dscratch.get(), s));
@@ -857,7 +904,8 @@ int mpdg_heat_op_set_conductivity_qp(mpdg_heat_op *op, const void *k) {
   op->d64.kappa_mode = op->d32.kappa_mode = KAPPA_QP;
   op->d64.kappa_arr = (const double *)k;
   op->d32.kappa_arr = (const float *)k;
-  return MPDG_OK;
+  MPDG_TRY(op->refresh_metric_k(nullptr));
+  return hipDeviceSynchronize() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 int mpdg_heat_op_set_conductivity_cell(mpdg_heat_op *op, const void *k) {
   if (!op || !k) return MPDG_ERR_INVALID;
@@ -902,6 +950,7 @@ int mpdg_heat_op_set_mapping(mpdg_heat_op *op, const double *coords) {
     MPDG_TRY(launch_metric<float>(op->d32, coords, op->metric32.get(), nullptr));
     op->d32.metric = op->metric32.get();
   }
+  MPDG_TRY(op->refresh_metric_k(nullptr));
   return hipDeviceSynchronize() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 

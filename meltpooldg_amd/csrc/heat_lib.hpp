@@ -77,12 +77,13 @@ struct mpdg_heat_op {
   mpdg_heat_desc desc{};
   int precision = MPDG_FP64;
   double dt = 0.0;
-  mpdg::OperatorData<double> d64;
-  mpdg::OperatorData<float> d32;
+  mutable mpdg::OperatorData<double> d64;
+  mutable mpdg::OperatorData<float> d32;
   double flux[6] = {0, 0, 0, 0, 0, 0};
   const double *T_robin = nullptr;
-  mutable mpdg::DeviceArray<double> kq, tmp1, tmp2, scal, tables, metric64;
+  mutable mpdg::DeviceArray<double> kq, tmp1, tmp2, scal, tables, metric64, metric64_k;
   mpdg::DeviceArray<float> metric32;
+  mutable mpdg::DeviceArray<float> metric32_k, kq32;
   const double *coords = nullptr;
   mutable mpdg::ReductionScratch red;
   // optional HIP-event timing of the cell-apply kernel (bench / profiling)
@@ -96,6 +97,8 @@ struct mpdg_heat_op {
     }
   }
 
+  // mapped meshes with per-qp / k(T) conductivity: fold the conductivity into metric_k
+  int refresh_metric_k(hipStream_t s) const;
   long n_dofs() const { return d64.n_dofs; }
   long n_cells() const { return d64.n_cells; }
   int vmult(double *dst, const double *src, hipStream_t s) const;

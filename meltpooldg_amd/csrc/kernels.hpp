@@ -36,6 +36,8 @@ struct OperatorData {
   const double *T_robin = nullptr;  // temperature for the radiation linearisation (FP64)
   // mapped geometry: per quadrature point [cell][7][q]: JxW, then G = JxW J^-1 J^-T as (00, 01, 11, 02, 12, 22)
   const Number *metric = nullptr;
+  // metric with the conductivity folded in (kappa_q G), used for per-qp / k(T) conductivity on mapped meshes
+  const Number *metric_k = nullptr;
   const double *coords = nullptr;   // mapping node coordinates (X block, Y block, Z block), FP64
   void set_mesh(int k, const int *ncells, const double *ext) {
     degree = k;
@@ -108,6 +110,20 @@ struct FluxNodes {
   double nodes[kMaxN1];
   double p[6];
 };
+// The operator the kernels see: on mapped meshes with per-qp or k(T) conductivity the
+// conductivity sits in metric_k and the kernels run with a constant unit conductivity.
+template <typename Number>
+OperatorData<Number> kernel_view(const OperatorData<Number> &op) {
+  OperatorData<Number> v = op;
+  if (op.metric && (op.kappa_mode == KAPPA_QP || op.kappa_mode == KAPPA_LAW)) {
+    v.metric = op.metric_k;
+    v.kappa_mode = KAPPA_CONST;
+    v.kappa = 1.0;
+  }
+  return v;
+}
+template <typename Number>
+int launch_scale_metric(const OperatorData<Number> &op, const Number *kq, Number *metric_k, hipStream_t s);
 template <typename Number>
 int launch_metric(const OperatorData<Number> &op, const double *coords, Number *metric, hipStream_t s);
 int launch_surface_flux_field(const OperatorData<double> &op, const double *p6, double *out, hipStream_t s);
