@@ -34,7 +34,9 @@ def parse():
     ap.add_argument("--degree", type=int, default=3)
     ap.add_argument("--solve", type=int, default=1)
     ap.add_argument("--cpu", type=int, default=1)
-    ap.add_argument("--cpu-cells", type=int, default=40)
+    ap.add_argument("--cpu-cells", type=int, default=64)
+    ap.add_argument("--copy", type=int, default=0,
+                    help="also run 3 device copies of the src vector (traffic calibration for rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -66,7 +68,7 @@ def cpu_baseline(k, N, threads):
         op.vmult(x, threads=threads)
         reps += 1
         el = time.perf_counter() - t0
-        if (reps >= 3 and el > 8.0) or reps >= 50:
+        if (reps >= 3 and el > 10.0) or reps >= 400:
             break
     return op.n_dofs * reps / el / 1e9, reps, el, op.n_dofs
 
@@ -155,6 +157,10 @@ def main():
                      "kernel": "heat_apply_kernel<4,double,KAPPA_QP>", "kernel_avg_ms": round(kern_avg, 5),
                      "algorithmic_bytes": bytes_alg},
     }
+    if a.copy:
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
     del kq
     if a.solve and k == 3:
         try:

@@ -178,6 +178,12 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
    source it reads (the caller refreshes the field before each step) */
 typedef struct mpdg_learned mpdg_learned;
 int mpdg_heat_operation_set_learned(mpdg_heat_operation *h, mpdg_learned *net, const double *q_field);
+/* record, from the next step on, the network's residual input of every step into the device
+   DoF vector z (z = D^-1 (b - J T_lin), the residual of the linear extrapolation
+   T_lin = T_n + (dt/dt_old)(T_n - T_{n-1}) scaled by the inverse operator diagonal; in
+   MPDG_STEP_NEWTON b - J T_lin is the Newton residual at T_lin); NULL stops recording.
+   Training pairs for mpdg_learned_train_step come from a run with this set. */
+int mpdg_heat_operation_record_guess_feature(mpdg_heat_operation *h, double *z);
 /* statistics of the last step: 1 if the learned guess was replaced by T_n, initial and final
    relative residuals ||b - J T||/||b|| of the step's linear solve (MPDG_STEP_LINEAR) */
 int mpdg_heat_operation_last_step_info(const mpdg_heat_operation *h, int *fell_back, double *rel_res0,
@@ -189,7 +195,7 @@ const double *mpdg_heat_operation_old_solution(const mpdg_heat_operation *h);
 /* nodal field of the z+ surface flux of create_rhs (zero off the z+ face), for the network input */
 int mpdg_heat_op_surface_flux_field(const mpdg_heat_op *op, double *out, void *stream);
 
-/* --- learned initial guess: 3D CNN (3 -> 16 -> 16 -> 16 -> 1 channels, 3x3x3 taps, ReLU) on the
+/* --- learned initial guess: 3D CNN (4 -> 16 -> 16 -> 16 -> 1 channels, 3x3x3 taps, ReLU) on the
        DoF grid, BF16 MFMA kernels, FP32 master weights, Adam. Enters the predictor slot of
        include/meltpooldg/linear_algebra/predictor.hpp:118-160 as a new predictor type. --- */
 int mpdg_learned_create(const int *dims, double qscale, unsigned long long seed, mpdg_learned **net);
@@ -198,15 +204,20 @@ int mpdg_learned_n_params(const mpdg_learned *net);
 /* weights as one FP32 host array: per layer W[co][ci][tap] then b[co] */
 int mpdg_learned_get_weights(const mpdg_learned *net, float *host);
 int mpdg_learned_set_weights(mpdg_learned *net, const float *host);
-/* out = T_n + s (c0 + y(T_{n-1}, T_n, q)) (device FP64 DoF vectors; q may be NULL) */
+/* out = (2 T_n - T_{n-1}) + sz y(T_{n-1}, T_n, q, z), sz = rms(z): linear extrapolation (equal
+   steps) plus the learned correction. Device FP64 DoF vectors; z is the residual input recorded
+   by mpdg_heat_operation_record_guess_feature; q and z may be NULL (channel zero, sz = rms of
+   T_n - T_{n-1}) */
 int mpdg_learned_predict(mpdg_learned *net, const double *T_nm1, const double *T_n, const double *q,
-                         double *out, void *stream);
-/* one Adam step on the sample (T_{n-1}, T_n, q) -> T_{n+1}; loss = mean squared error before the step */
+                         const double *z, double *out, void *stream);
+/* one Adam step on the sample (T_{n-1}, T_n, q, z) -> T_{n+1}; loss = mean squared error of
+   y against (T_{n+1} - (2 T_n - T_{n-1}))/sz before the step */
 int mpdg_learned_train_step(mpdg_learned *net, const double *T_nm1, const double *T_n, const double *q,
-                            const double *T_np1, double lr, double *loss, void *stream);
+                            const double *z, const double *T_np1, double lr, double *loss, void *stream);
 /* gradients of the loss (host FP32, weight layout) without an update */
 int mpdg_learned_gradients(mpdg_learned *net, const double *T_nm1, const double *T_n, const double *q,
-                           const double *T_np1, float *host_grad, double *loss, void *stream);
+                           const double *z, const double *T_np1, float *host_grad, double *loss,
+                           void *stream);
 /* layer activations of the last forward pass as bf16 bit patterns [voxel][16] (layer 0 = input) */
 int mpdg_learned_activation(const mpdg_learned *net, int layer, unsigned short *host);
 

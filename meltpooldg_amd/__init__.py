@@ -106,6 +106,7 @@ _SIGNATURES = {
     "mpdg_heat_operation_solution": ([_V], _V),
     "mpdg_heat_operation_old_solution": ([_V], _V),
     "mpdg_heat_operation_set_learned": ([_V, _V, _V], _I),
+    "mpdg_heat_operation_record_guess_feature": ([_V, _V], _I),
     "mpdg_heat_operation_last_step_info": ([_V, C.POINTER(_I), C.POINTER(_D), C.POINTER(_D)], _I),
     "mpdg_heat_op_surface_flux_field": ([_V, _V, _V], _I),
     "mpdg_learned_create": ([C.POINTER(_I), _D, C.c_ulonglong, C.POINTER(_V)], _I),
@@ -113,9 +114,9 @@ _SIGNATURES = {
     "mpdg_learned_n_params": ([_V], _I),
     "mpdg_learned_get_weights": ([_V, _V], _I),
     "mpdg_learned_set_weights": ([_V, _V], _I),
-    "mpdg_learned_predict": ([_V, _V, _V, _V, _V, _V], _I),
-    "mpdg_learned_train_step": ([_V, _V, _V, _V, _V, _D, C.POINTER(_D), _V], _I),
-    "mpdg_learned_gradients": ([_V, _V, _V, _V, _V, _V, C.POINTER(_D), _V], _I),
+    "mpdg_learned_predict": ([_V, _V, _V, _V, _V, _V, _V], _I),
+    "mpdg_learned_train_step": ([_V, _V, _V, _V, _V, _V, _D, C.POINTER(_D), _V], _I),
+    "mpdg_learned_gradients": ([_V, _V, _V, _V, _V, _V, _V, C.POINTER(_D), _V], _I),
     "mpdg_learned_activation": ([_V, _I, _V], _I),
 }
 
@@ -413,6 +414,12 @@ class HeatDiffuseOperation:
         if learned is not None:
             _check(self.lib.mpdg_heat_operation_set_learned(h, learned.h, _ptr(q_field)), "set_learned")
 
+    def record_guess_feature(self, z):
+        """From the next step on, write each step's network residual input D^-1 (b - J T_lin)
+        into the device vector z (None stops); training pairs need it."""
+        self.z_record = z
+        _check(self.lib.mpdg_heat_operation_record_guess_feature(self.h, _ptr(z)), "record_guess_feature")
+
     def last_step_info(self):
         """(learned guess replaced by T_n, initial and final ||b - J T||/||b|| of the step)"""
         fb, r0, r = C.c_int(), C.c_double(), C.c_double()
@@ -454,7 +461,8 @@ class HeatDiffuseOperation:
 
 
 class LearnedPredictor:
-    """Learned initial guess: 3D CNN on the DoF grid predicting T_{n+1} from (T_{n-1}, T_n, q).
+    """Learned initial guess: 3D CNN on the DoF grid predicting T_{n+1} from (T_{n-1}, T_n, q) and
+    z = D^-1 (b - J T_lin), the scaled residual of the linear extrapolation for the step.
 
     A new predictor type for the slot of include/meltpooldg/linear_algebra/predictor.hpp:118-160;
     training (Adam, FP32 masters) and inference run in the library's BF16 MFMA kernels.
@@ -478,21 +486,22 @@ class LearnedPredictor:
         except Exception:
             pass
 
-    def predict(self, T_nm1, T_n, q, out):
-        _check(self.lib.mpdg_learned_predict(self.h, _ptr(T_nm1), _ptr(T_n), _ptr(q), _ptr(out), _stream()),
-               "learned predict")
+    def predict(self, T_nm1, T_n, q, z, out):
+        """out = (2 T_n - T_{n-1}) + rms(z) y (device FP64 vectors; q, z may be None)."""
+        _check(self.lib.mpdg_learned_predict(self.h, _ptr(T_nm1), _ptr(T_n), _ptr(q), _ptr(z), _ptr(out),
+                                             _stream()), "learned predict")
 
-    def train_step(self, T_nm1, T_n, q, T_np1, lr):
+    def train_step(self, T_nm1, T_n, q, z, T_np1, lr):
         loss = C.c_double()
-        _check(self.lib.mpdg_learned_train_step(self.h, _ptr(T_nm1), _ptr(T_n), _ptr(q), _ptr(T_np1), lr,
+        _check(self.lib.mpdg_learned_train_step(self.h, _ptr(T_nm1), _ptr(T_n), _ptr(q), _ptr(z), _ptr(T_np1), lr,
                                                 C.byref(loss), _stream()), "learned train_step")
         return loss.value
 
-    def gradients(self, T_nm1, T_n, q, T_np1):
+    def gradients(self, T_nm1, T_n, q, z, T_np1):
         import numpy as np
         g = np.zeros(self.n_params, dtype=np.float32)
         loss = C.c_double()
-        _check(self.lib.mpdg_learned_gradients(self.h, _ptr(T_nm1), _ptr(T_n), _ptr(q), _ptr(T_np1),
+        _check(self.lib.mpdg_learned_gradients(self.h, _ptr(T_nm1), _ptr(T_n), _ptr(q), _ptr(z), _ptr(T_np1),
                                                g.ctypes.data_as(C.c_void_p), C.byref(loss), _stream()),
                "learned gradients")
         return loss.value, g

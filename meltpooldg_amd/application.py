@@ -74,21 +74,26 @@ class LaserScan:
 
 
 def generate_trajectory(degree, cells, n_steps, **kw):
-    """Run n_steps and keep every T_n (T_0 = 0 included) and the source field of each step."""
+    """Run n_steps and keep every T_n (T_0 = 0 included), the source field of each step and the
+    network's residual input of each step: qs[n], zs[n] belong to the step T_n -> T_{n+1}."""
     case = LaserScan(degree, cells, **kw)
+    z = case.torch.zeros_like(case.T0)
+    case.heat.record_guess_feature(z)
     Ts = [case.solution()]
-    qs = []
+    qs, zs = [], []
     stats = []
     for _ in range(n_steps):
         its, el, info = case.step()
         Ts.append(case.solution())
         qs.append(case.q.clone())
+        zs.append(z.clone())
         stats.append((its, el, info))
-    return case, Ts, qs, stats
+    case.heat.record_guess_feature(None)
+    return case, Ts, qs, zs, stats
 
 
-def train(net, Ts, qs, epochs=4, lr=1e-3, first=2, seed=0, log=None):
-    """Adam over the solver's own pairs ((T_{n-1}, T_n, q_{n+1}) -> T_{n+1}), n >= first."""
+def train(net, Ts, qs, zs, epochs=4, lr=1e-3, first=2, seed=0, log=None):
+    """Adam over the solver's own pairs ((T_{n-1}, T_n, q_{n+1}, z_n) -> T_{n+1}), n >= first."""
     import random
     rng = random.Random(seed)
     idx = list(range(first, len(Ts) - 1))
@@ -97,7 +102,7 @@ def train(net, Ts, qs, epochs=4, lr=1e-3, first=2, seed=0, log=None):
         rng.shuffle(idx)
         tot = 0.0
         for n in idx:
-            tot += net.train_step(Ts[n - 1], Ts[n], qs[n], Ts[n + 1], lr)
+            tot += net.train_step(Ts[n - 1], Ts[n], qs[n], zs[n], Ts[n + 1], lr)
         losses.append(tot / len(idx))
         if log:
             log("epoch %d loss %.4e" % (e, losses[-1]))

@@ -334,6 +334,16 @@ int launch_add(T *x, const T *y, long n, hipStream_t s) {
 template int launch_add<float>(float *, const float *, long, hipStream_t);
 template int launch_add<double>(double *, const double *, long, hipStream_t);
 
+// x *= y (pointwise)
+__global__ __launch_bounds__(256) void mul_kernel(double *x, const double *y, long n) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) x[i] *= y[i];
+}
+int launch_mul(double *x, const double *y, long n, hipStream_t s) {
+  hipLaunchKernelGGL(mul_kernel, dim3(std::max<long>(1, std::min<long>((n + 255) / 256, 8192))), dim3(256), 0, s, x,
+                     y, n);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
 // 1D ELL matrix applied along direction `dir` of a lexicographic 3D array:
 // out(i with i_dir = row) = sum_e val[row][e] in(i with i_dir = col[row][e])
 template <typename T>

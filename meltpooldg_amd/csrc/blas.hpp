@@ -44,6 +44,7 @@ int launch_chebyshev_step(T *x, T *d, const T *b, const T *Ax, const T *inv, lon
                           bool first, hipStream_t s);
 template <typename T> int launch_residual(T *r, const T *b, const T *Ax, long n, hipStream_t s);
 template <typename T> int launch_add(T *x, const T *y, long n, hipStream_t s);
+int launch_mul(double *x, const double *y, long n, hipStream_t s);
 template <typename T>
 int launch_ell_dir(const T *in, T *out, const int *odims, const int *idims, int dir, int width, const int *col,
                    const T *val, hipStream_t s);

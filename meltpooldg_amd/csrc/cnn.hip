@@ -202,37 +202,34 @@ __global__ void adam_kernel(float *__restrict__ w, float *__restrict__ m, float 
 }
 
 // ---- features and targets (FP64 fields -> bf16 channels) ----
-// channels: 0 (Tn - Tnm1)/s, 1 q*qscale/s, 2 (Tn - tmean)/tscale
+// stats = {s, mean T_n, tscale, sz}: s = rms(T_n - T_{n-1}), sz = rms(z) (s when z is absent)
+// channels: 0 (Tn - Tnm1)/s, 1 q*qscale/s, 2 (Tn - tmean)/tscale, 3 z/sz
 __global__ void features_kernel(const double *__restrict__ Tnm1, const double *__restrict__ Tn,
-                                const double *__restrict__ q, long n, const double *__restrict__ stats, double qscale,
-                                unsigned short *__restrict__ x) {
-  const double s = stats[0], tmean = stats[1], ts = stats[2];
+                                const double *__restrict__ q, const double *__restrict__ z, long n,
+                                const double *__restrict__ stats, double qscale, unsigned short *__restrict__ x) {
+  const double s = stats[0], tmean = stats[1], ts = stats[2], sz = stats[3];
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-    const float c0 = (float)((Tn[i] - Tnm1[i]) / s);
-    const float c1 = q ? (float)(q[i] * qscale / s) : 0.f;
-    const float c2 = (float)((Tn[i] - tmean) / ts);
-    unsigned short o[16] = {0};
-    o[0] = f32_to_bf16(c0);
-    o[1] = f32_to_bf16(c1);
-    o[2] = f32_to_bf16(c2);
+    const unsigned c0 = f32_to_bf16((float)((Tn[i] - Tnm1[i]) / s));
+    const unsigned c1 = q ? f32_to_bf16((float)(q[i] * qscale / s)) : 0u;
+    const unsigned c2 = f32_to_bf16((float)((Tn[i] - tmean) / ts));
+    const unsigned c3 = z ? f32_to_bf16((float)(z[i] / sz)) : 0u;
     uint4 *dst = reinterpret_cast<uint4 *>(x + i * 16);
-    dst[0] = make_uint4(o[0] | (o[1] << 16), o[2], 0, 0);
+    dst[0] = make_uint4(c0 | (c1 << 16), c2 | (c3 << 16), 0, 0);
     dst[1] = make_uint4(0, 0, 0, 0);
   }
 }
 
-// loss gradient on channel 0: target c* = (Tnp1 - Tn)/s - c0, pred = y[v][0];
-// g = 2 (pred - c*) / n * gain; partial sums of the squared error
+// loss gradient on channel 0: target y* = (Tnp1 - (2 Tn - Tnm1))/sz, pred = y[v][0];
+// g = 2 (pred - y*) / n * gain; partial sums of the squared error
 __global__ __launch_bounds__(256) void loss_kernel(const unsigned short *__restrict__ y, const double *__restrict__ Tnm1,
                                                    const double *__restrict__ Tn, const double *__restrict__ Tnp1,
                                                    long n, const double *__restrict__ stats, float gain,
                                                    unsigned short *__restrict__ g, double *__restrict__ part) {
   __shared__ double sh[256];
-  const double s = stats[0];
+  const double sz = stats[3];
   double acc = 0.0;
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-    const double c0 = (Tn[i] - Tnm1[i]) / s;
-    const double tgt = (Tnp1[i] - Tn[i]) / s - c0;
+    const double tgt = (Tnp1[i] - (2.0 * Tn[i] - Tnm1[i])) / sz;
     const double e = (double)bf16_to_f32(y[i * 16]) - tgt;
     acc += e * e;
     uint4 *dst = reinterpret_cast<uint4 *>(g + i * 16);
@@ -248,51 +245,56 @@ __global__ __launch_bounds__(256) void loss_kernel(const unsigned short *__restr
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// guess = Tn + s (c0 + y[v][0])
+// guess = (2 Tn - Tnm1) + sz y[v][0]
 __global__ void predict_kernel(const unsigned short *__restrict__ y, const double *__restrict__ Tnm1,
                                const double *__restrict__ Tn, long n, const double *__restrict__ stats,
                                double *__restrict__ out) {
-  const double s = stats[0];
-  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-    const double c0 = (Tn[i] - Tnm1[i]) / s;
-    out[i] = Tn[i] + s * (c0 + (double)bf16_to_f32(y[i * 16]));
-  }
+  const double sz = stats[3];
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+    out[i] = (2.0 * Tn[i] - Tnm1[i]) + sz * (double)bf16_to_f32(y[i * 16]);
 }
 
-// statistics of a sample: s = rms(Tn - Tnm1) (+ tiny), mean(Tn), rms(Tn - mean) (+ tiny)
+// statistics of a sample: sums of (Tn - Tnm1)^2, Tn, Tn^2, z^2
 __global__ __launch_bounds__(256) void stats_partial_kernel(const double *__restrict__ Tnm1, const double *__restrict__ Tn,
-                                                            long n, double *__restrict__ part) {
-  __shared__ double sh[3][256];
-  double a = 0, b = 0, c = 0;
+                                                            const double *__restrict__ z, long n,
+                                                            double *__restrict__ part) {
+  __shared__ double sh[4][256];
+  double a = 0, b = 0, c = 0, e = 0;
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
     const double d = Tn[i] - Tnm1[i];
     a += d * d;
     b += Tn[i];
     c += Tn[i] * Tn[i];
+    if (z) e += z[i] * z[i];
   }
   sh[0][threadIdx.x] = a;
   sh[1][threadIdx.x] = b;
   sh[2][threadIdx.x] = c;
+  sh[3][threadIdx.x] = e;
   __syncthreads();
   for (int k = 128; k > 0; k >>= 1) {
     if (threadIdx.x < k)
-      for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + k];
+      for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + k];
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int q = 0; q < 3; ++q) part[blockIdx.x * 3 + q] = sh[q][0];
+    for (int q = 0; q < 4; ++q) part[blockIdx.x * 4 + q] = sh[q][0];
 }
-__global__ void stats_final_kernel(const double *__restrict__ part, int nb, long n, double *__restrict__ stats) {
-  double a = 0, b = 0, c = 0;
+// s = rms(Tn - Tnm1) (+ tiny), mean(Tn), rms(Tn - mean) (+ tiny), sz = rms(z) (+ tiny) or s without z
+__global__ void stats_final_kernel(const double *__restrict__ part, int nb, long n, int has_z,
+                                   double *__restrict__ stats) {
+  double a = 0, b = 0, c = 0, e = 0;
   for (int i = 0; i < nb; ++i) {
-    a += part[3 * i];
-    b += part[3 * i + 1];
-    c += part[3 * i + 2];
+    a += part[4 * i];
+    b += part[4 * i + 1];
+    c += part[4 * i + 2];
+    e += part[4 * i + 3];
   }
   const double mean = b / n;
   stats[0] = sqrt(a / n) + 1e-30;
   stats[1] = mean;
   stats[2] = sqrt(fmax(c / n - mean * mean, 0.0)) + 1e-12 + 1e-6 * fabs(mean);
+  stats[3] = has_z ? sqrt(e / n) + 1e-30 : stats[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -331,17 +333,18 @@ int cnn_adam(float *w, float *m, float *v, const float *dW, const float *db, flo
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
-int cnn_stats(const double *Tnm1, const double *Tn, long n, double *part, double *stats, hipStream_t s) {
+int cnn_stats(const double *Tnm1, const double *Tn, const double *z, long n, double *part, double *stats,
+              hipStream_t s) {
   const int nb = (int)std::min<long>(1024, (n + 255) / 256);
-  hipLaunchKernelGGL(stats_partial_kernel, dim3(nb), dim3(256), 0, s, Tnm1, Tn, n, part);
-  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(1), 0, s, part, nb, n, stats);
+  hipLaunchKernelGGL(stats_partial_kernel, dim3(nb), dim3(256), 0, s, Tnm1, Tn, z, n, part);
+  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(1), 0, s, part, nb, n, z != nullptr ? 1 : 0, stats);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
-int cnn_features(const double *Tnm1, const double *Tn, const double *q, long n, const double *stats, double qscale,
-                 unsigned short *x, hipStream_t s) {
+int cnn_features(const double *Tnm1, const double *Tn, const double *q, const double *z, long n,
+                 const double *stats, double qscale, unsigned short *x, hipStream_t s) {
   hipLaunchKernelGGL(features_kernel, dim3((int)std::min<long>(8192, (n + 255) / 256)), dim3(256), 0, s, Tnm1, Tn, q,
-                     n, stats, qscale, x);
+                     z, n, stats, qscale, x);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 

@@ -12,9 +12,10 @@ int cnn_wgrad(const unsigned short *g, const unsigned short *in, const int *dims
 int cnn_pack(const float *master, int cin, int cout, unsigned short *W, unsigned short *Wt, hipStream_t s);
 int cnn_adam(float *w, float *m, float *v, const float *dW, const float *db, float *b, float *mb, float *vb, int cin,
              int cout, float lr, float b1, float b2, float eps, int step, float gscale, hipStream_t s);
-int cnn_stats(const double *Tnm1, const double *Tn, long n, double *part, double *stats, hipStream_t s);
-int cnn_features(const double *Tnm1, const double *Tn, const double *q, long n, const double *stats, double qscale,
-                 unsigned short *x, hipStream_t s);
+int cnn_stats(const double *Tnm1, const double *Tn, const double *z, long n, double *part, double *stats,
+              hipStream_t s);
+int cnn_features(const double *Tnm1, const double *Tn, const double *q, const double *z, long n,
+                 const double *stats, double qscale, unsigned short *x, hipStream_t s);
 int cnn_loss(const unsigned short *y, const double *Tnm1, const double *Tn, const double *Tnp1, long n,
              const double *stats, float gain, unsigned short *g, double *part, int nblocks, hipStream_t s);
 int cnn_predict(const unsigned short *y, const double *Tnm1, const double *Tn, long n, const double *stats,

@@ -43,7 +43,8 @@ template <int N1> struct ApplyGeom {
   static constexpr int TBUF = 4 * CPW * CS;           // transpose buffers of the 4 wavefronts
   static constexpr int LDS = TBUF > ACC ? TBUF : ACC;
   static constexpr int CARRY = AY * AZ;
-  static constexpr int WPE = N1 <= 3 ? 4 : (N1 == 4 ? 3 : 2);  // wavefronts per SIMD the register budget targets
+  // x rows of the DoF brick written per pass of the write-out loop (lane = x in the row)
+  static constexpr int ROWS_PER_PASS = 256 / AX;
 };
 
 // ---- register-level 1D contractions on an N1 x N1 array a[row][col] ----
@@ -163,8 +164,15 @@ __device__ __forceinline__ Number law_kappa(int law, const Number *p, Number T) 
   return p[0] + (p[1] - p[0]) * (Number(3) - Number(2) * fs) * fs * fs;
 }
 
+// wavefronts per SIMD the register allocation targets (FP64 Q3: 2, the rest: the compiler's choice)
+template <int N1, typename Number> struct Occupancy { static constexpr int W = 1; };
+template <> struct Occupancy<4, double> { static constexpr int W = 2; };
+template <> struct Occupancy<4, float> { static constexpr int W = 3; };
+template <> struct Occupancy<5, float> { static constexpr int W = 2; };
+
 template <int N1, typename Number, int KMODE, bool MAPPED>
-__global__ __launch_bounds__(256) void heat_apply_kernel(const ApplyArgs<Number, N1> P,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Occupancy<N1, Number>::W)))
+void heat_apply_kernel(const ApplyArgs<Number, N1> P,
                                                          const Number *__restrict__ src,
                                                          Number *__restrict__ dst) {
   using G = ApplyGeom<N1>;
@@ -256,7 +264,9 @@ __global__ __launch_bounds__(256) void heat_apply_kernel(const ApplyArgs<Number,
     const Number wy = P.w[l];
     Number kc = P.kappa;
     if (KMODE == KAPPA_CELL) kc = valid ? P.kappa_arr[cell] : Number(0);
-    {
+    Number e[N1][N1];
+    if (MAPPED) {
+      // all three reference gradients at a point are needed together
       Number gx[N1][N1], gz[N1][N1];
       sweep_row<N1, false>(P.D, v, gz);
       sweep_col<N1, false>(P.D, v, gx);
@@ -269,30 +279,71 @@ __global__ __launch_bounds__(256) void heat_apply_kernel(const ApplyArgs<Number,
           Number kap = kc;
           if (KMODE == KAPPA_QP) kap = valid ? P.kappa_arr[cell * NQ + q] : Number(0);
           if (KMODE == KAPPA_LAW) kap = kq[qz][qx];
-          if (MAPPED) {
-            // flux in reference coordinates: kappa G grad_xi u, G = JxW J^-1 J^-T
-            const Number *M = P.metric + (valid ? cell : 0) * 7 * NQ + q;
-            const Number jxw = valid ? M[0] : Number(0);
-            const Number g00 = M[NQ], g01 = M[2 * NQ], g11 = M[3 * NQ], g02 = M[4 * NQ], g12 = M[5 * NQ],
-                         g22 = M[6 * NQ];
-            const Number a = gx[qz][qx], b = gy[qz][qx], c = gz[qz][qx];
-            v[qz][qx] *= P.cm * jxw;
-            gx[qz][qx] = kap * (g00 * a + g01 * b + g02 * c);
-            gy[qz][qx] = kap * (g01 * a + g11 * b + g12 * c);
-            gz[qz][qx] = kap * (g02 * a + g12 * b + g22 * c);
-          } else {
-            const Number W = P.w[qx] * wy * P.w[qz];
-            const Number kw = kap * W;
-            v[qz][qx] *= P.cm * W;
-            gx[qz][qx] *= kw * P.ks[0];
-            gy[qz][qx] *= kw * P.ks[1];
-            gz[qz][qx] *= kw * P.ks[2];
-          }
+          // flux in reference coordinates: kappa G grad_xi u, G = JxW J^-1 J^-T
+          const Number *M = P.metric + (valid ? cell : 0) * 7 * NQ + q;
+          const Number jxw = valid ? M[0] : Number(0);
+          const Number g00 = M[NQ], g01 = M[2 * NQ], g11 = M[3 * NQ], g02 = M[4 * NQ], g12 = M[5 * NQ],
+                       g22 = M[6 * NQ];
+          const Number a = gx[qz][qx], b = gy[qz][qx], c = gz[qz][qx];
+          e[qz][qx] = P.cm * jxw * v[qz][qx];
+          gx[qz][qx] = kap * (g00 * a + g01 * b + g02 * c);
+          gy[qz][qx] = kap * (g01 * a + g11 * b + g12 * c);
+          gz[qz][qx] = kap * (g02 * a + g12 * b + g22 * c);
         }
-      sweep_row_add<N1, true>(P.D, gz, v);
-      sweep_col_add<N1, true>(P.D, gx, v);
+      sweep_row_add<N1, true>(P.D, gz, e);
+      sweep_col_add<N1, true>(P.D, gx, e);
+    } else {
+      // x direction row by row (with the mass term and the y flux), then z column by column,
+      // so that only v, e and gy are live as full arrays
+      constexpr int NQ = N1 * N1 * N1;
+#pragma unroll
+      for (int qz = 0; qz < N1; ++qz) {
+        Number f[N1];
+#pragma unroll
+        for (int qx = 0; qx < N1; ++qx) {
+          const Number W = P.w[qx] * wy * P.w[qz];
+          Number kap = kc;
+          if (KMODE == KAPPA_QP) kap = valid ? P.kappa_arr[cell * NQ + qx + N1 * (l + N1 * qz)] : Number(0);
+          if (KMODE == KAPPA_LAW) kap = kq[qz][qx];
+          const Number kw = kap * W;
+          e[qz][qx] = P.cm * W * v[qz][qx];
+          gy[qz][qx] *= kw * P.ks[1];
+          Number g = 0;
+#pragma unroll
+          for (int p = 0; p < N1; ++p) g = fma(P.D[qx][p], v[qz][p], g);
+          f[qx] = kw * P.ks[0] * g;
+        }
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+          Number acc = e[qz][i];
+#pragma unroll
+          for (int qx = 0; qx < N1; ++qx) acc = fma(P.D[qx][i], f[qx], acc);
+          e[qz][i] = acc;
+        }
+      }
+#pragma unroll
+      for (int qx = 0; qx < N1; ++qx) {
+        Number f[N1];
+#pragma unroll
+        for (int qz = 0; qz < N1; ++qz) {
+          const Number W = P.w[qx] * wy * P.w[qz];
+          Number kap = kc;
+          if (KMODE == KAPPA_QP) kap = valid ? P.kappa_arr[cell * NQ + qx + N1 * (l + N1 * qz)] : Number(0);
+          if (KMODE == KAPPA_LAW) kap = kq[qz][qx];
+          Number g = 0;
+#pragma unroll
+          for (int p = 0; p < N1; ++p) g = fma(P.D[qz][p], v[p][qx], g);
+          f[qz] = kap * W * P.ks[2] * g;
+        }
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+          Number acc = e[i][qx];
+#pragma unroll
+          for (int qz = 0; qz < N1; ++qz) acc = fma(P.D[qz][i], f[qz], acc);
+          e[i][qx] = acc;
+        }
+      }
     }
-    Number (&e)[N1][N1] = v;
     // ---- back to the plane layout: e' = Sz^T e, ey = Sz^T (kW gy) ----
     sweep_row<N1, true>(P.S, e, u);
     transpose_yz<N1>(slot, l, lane_on, u, t);
@@ -321,47 +372,60 @@ __global__ __launch_bounds__(256) void heat_apply_kernel(const ApplyArgs<Number,
     }
     __syncthreads();
     const bool last = chunk == nchunks - 1;
-    for (int idx = tid; idx < G::ACC; idx += 256) {
-      const int x = idx % G::AX, y = (idx / G::AX) % G::AY, z = idx / (G::AX * G::AY);
-      if (x >= DX || y >= DY || z >= DZ) continue;
-      const Number val = lds[idx];
-      if (x == DX - 1 && !last) {
-        carry[y + G::AY * z] = val;
-        continue;
-      }
-      const int ix = X0 + x, iy = Y0 + y, iz = Z0 + z;
-      const long g = ix + sy * iy + sz * iz;
-      const bool face = (y == 0 || y == DY - 1 || z == 0 || z == DZ - 1);
-      bool con = false;
-      if (P.dmask)
-        con = ((P.dmask & 1) && ix == 0) || ((P.dmask & 2) && ix == P.nd[0] - 1) ||
-              ((P.dmask & 4) && iy == 0) || ((P.dmask & 8) && iy == P.nd[1] - 1) ||
-              ((P.dmask & 16) && iz == 0) || ((P.dmask & 32) && iz == P.nd[2] - 1);
-      if (face) {
-        if (!con) atomicAdd(dst + g, val);
-      } else {
-        dst[g] = con ? Number(0) : val;
+    // write-out by x rows of the brick: lane wx of row group wr (coalesced along x)
+    const int wr = tid / G::AX, wx = tid - wr * G::AX;
+    if (wr < G::ROWS_PER_PASS && wx < DX) {
+      const int ix = X0 + wx;
+      for (int r = wr; r < G::AY * G::AZ; r += G::ROWS_PER_PASS) {
+        const int y = r % G::AY, z = r / G::AY;
+        if (y >= DY || z >= DZ) continue;
+        const Number val = lds[wx + G::AX * r];
+        if (wx == DX - 1 && !last) {
+          carry[r] = val;
+          continue;
+        }
+        const int iy = Y0 + y, iz = Z0 + z;
+        const long g = ix + sy * iy + sz * iz;
+        const bool face = (y == 0 || y == DY - 1 || z == 0 || z == DZ - 1);
+        bool con = false;
+        if (P.dmask)
+          con = ((P.dmask & 1) && ix == 0) || ((P.dmask & 2) && ix == P.nd[0] - 1) ||
+                ((P.dmask & 4) && iy == 0) || ((P.dmask & 8) && iy == P.nd[1] - 1) ||
+                ((P.dmask & 16) && iz == 0) || ((P.dmask & 32) && iz == P.nd[2] - 1);
+        if (face) {
+          if (!con) atomicAdd(dst + g, val);
+        } else {
+          dst[g] = con ? Number(0) : val;
+        }
       }
     }
     __syncthreads();
   }
 }
 
-// Zero the DoF rows shared between neighbouring columns (y or z on a column face).
+// Zero the DoF rows shared between neighbouring columns: the x rows on the
+// y-face planes (iy a multiple of py, and the last plane) and on the z-face
+// planes. One workgroup per row; rows on both a y and a z face are zeroed twice.
 template <typename Number>
-__global__ __launch_bounds__(256) void zero_face_rows_kernel(Number *__restrict__ dst, int nd0, int nd1,
-                                                             int nd2, int py, int pz) {
-  const long nrows = (long)nd1 * nd2;
-  for (long r = blockIdx.y; r < nrows; r += gridDim.y) {
-    const int iy = r % nd1, iz = r / nd1;
-    const bool face = (iy % py == 0) || (iy == nd1 - 1) || (iz % pz == 0) || (iz == nd2 - 1);
-    if (!face) continue;
-    for (int ix = blockIdx.x * 256 + threadIdx.x; ix < nd0; ix += gridDim.x * 256) dst[ix + r * nd0] = 0;
+__global__ __launch_bounds__(128) void zero_face_rows_kernel(Number *__restrict__ dst, int nd0, int nd1,
+                                                             int nd2, int py, int pz, int nfy, int nfz) {
+  long b = blockIdx.x;
+  int iy, iz;
+  if (b < (long)nfy * nd2) {
+    iy = min((int)(b % nfy) * py, nd1 - 1);
+    iz = (int)(b / nfy);
+  } else {
+    b -= (long)nfy * nd2;
+    iz = min((int)(b % nfz) * pz, nd2 - 1);
+    iy = (int)(b / nfz);
   }
+  Number *row = dst + (long)nd0 * (iy + (long)nd1 * iz);
+  for (int ix = threadIdx.x; ix < nd0; ix += 128) row[ix] = 0;
 }
 
 template <int N1, typename Number>
-int launch_apply_n1(const OperatorData<Number> &op_in, const Number *src, Number *dst, hipStream_t stream) {
+int launch_apply_n1(const OperatorData<Number> &op_in, const Number *src, Number *dst, hipStream_t stream,
+                    hipEvent_t kernel_start) {
   using G = ApplyGeom<N1>;
   const OperatorData<Number> op = kernel_view(op_in);
   if (op.metric == nullptr && op_in.metric != nullptr) return MPDG_ERR_INVALID;  // metric_k not computed
@@ -372,11 +436,14 @@ int launch_apply_n1(const OperatorData<Number> &op_in, const Number *src, Number
   const int ncols = A.ncol[0] * A.ncol[1];
   A.xcd_cols = (ncols % 8 == 0 && ncols >= 64) ? ncols / 8 : 0;
   {
-    const long nrows = (long)op.nd[1] * op.nd[2];
-    dim3 grid((op.nd[0] + 255) / 256, (unsigned)std::min<long>(nrows, 65535));
-    hipLaunchKernelGGL(zero_face_rows_kernel<Number>, grid, dim3(256), 0, stream, dst, op.nd[0],
-                       op.nd[1], op.nd[2], (N1 - 1) * G::BY, (N1 - 1) * G::BZ);
+    // face planes j*p for j = 0 .. ceil((nd-1)/p)-1, then the last plane nd-1
+    const int py = (N1 - 1) * G::BY, pz = (N1 - 1) * G::BZ;
+    const int nfy = (op.nd[1] - 1 + py - 1) / py + 1, nfz = (op.nd[2] - 1 + pz - 1) / pz + 1;
+    const long nrows = (long)nfy * op.nd[2] + (long)nfz * op.nd[1];
+    hipLaunchKernelGGL(zero_face_rows_kernel<Number>, dim3((unsigned)nrows), dim3(128), 0, stream, dst,
+                       op.nd[0], op.nd[1], op.nd[2], py, pz, nfy, nfz);
   }
+  if (kernel_start) hipEventRecord(kernel_start, stream);
   if (op.metric) {
     switch (op.kappa_mode) {
       case KAPPA_CONST: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CONST, true>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
@@ -396,17 +463,18 @@ int launch_apply_n1(const OperatorData<Number> &op_in, const Number *src, Number
 }
 
 template <typename Number>
-int launch_apply(const OperatorData<Number> &op, const Number *src, Number *dst, hipStream_t stream) {
+int launch_apply(const OperatorData<Number> &op, const Number *src, Number *dst, hipStream_t stream,
+                 hipEvent_t kernel_start) {
   switch (op.degree) {
-    case 1: return launch_apply_n1<2, Number>(op, src, dst, stream);
-    case 2: return launch_apply_n1<3, Number>(op, src, dst, stream);
-    case 3: return launch_apply_n1<4, Number>(op, src, dst, stream);
-    case 4: return launch_apply_n1<5, Number>(op, src, dst, stream);
+    case 1: return launch_apply_n1<2, Number>(op, src, dst, stream, kernel_start);
+    case 2: return launch_apply_n1<3, Number>(op, src, dst, stream, kernel_start);
+    case 3: return launch_apply_n1<4, Number>(op, src, dst, stream, kernel_start);
+    case 4: return launch_apply_n1<5, Number>(op, src, dst, stream, kernel_start);
     default: return MPDG_ERR_INVALID;
   }
 }
 
-template int launch_apply<double>(const OperatorData<double> &, const double *, double *, hipStream_t);
-template int launch_apply<float>(const OperatorData<float> &, const float *, float *, hipStream_t);
+template int launch_apply<double>(const OperatorData<double> &, const double *, double *, hipStream_t, hipEvent_t);
+template int launch_apply<float>(const OperatorData<float> &, const float *, float *, hipStream_t, hipEvent_t);
 
 }  // namespace mpdg

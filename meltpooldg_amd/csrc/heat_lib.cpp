@@ -28,8 +28,7 @@ int mpdg_heat_op::vmult_cells(double *dst, const double *src, hipStream_t s) con
     timing_events.push_back({a, b});
   }
   auto &ev = timing_events[timing_used++];
-  hipEventRecord(ev.first, s);
-  const int rc = launch_apply<double>(d64, src, dst, s);
+  const int rc = launch_apply<double>(d64, src, dst, s, ev.first);
   hipEventRecord(ev.second, s);
   return rc;
 }
@@ -851,6 +850,8 @@ struct mpdg_heat_operation {
   DeviceArray<double> scal, bvec, zero, alt;
   mpdg_learned *net = nullptr;
   const double *q_field = nullptr;
+  double *z_record = nullptr;        // caller's vector for the residual input of each step
+  DeviceArray<double> zbuf, invd;    // residual input and inverse diagonal (learned predictor)
   int fell_back = 0;
   double rel_res0 = 0.0, rel_res = 0.0;
 };
@@ -1119,6 +1120,45 @@ int mpdg_heat_operation_set_initial(mpdg_heat_operation *h, const double *T0, vo
   return hipStreamSynchronize(S(stream)) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
+// Residual input of the learned guess for the predicted T (the linear extrapolation):
+// z = D^-1 (b - J T) with b given (linear step), or z = D^-1 r_Newton(T, T_old) (b == nullptr).
+static int guess_feature(mpdg_heat_operation *h, const double *b, const double *T, const double *T_old, double *z,
+                         hipStream_t s) {
+  mpdg_heat_op *op = h->op;
+  const long n = op->n_dofs();
+  MPDG_TRY(h->invd.resize(n));
+  MPDG_TRY(op->inverse_diagonal(h->invd.get(), s));
+  if (b) {
+    MPDG_TRY(op->vmult(z, T, s));
+    MPDG_TRY(launch_axpby(z, 1.0, b, -1.0, n, s));
+  } else {
+    MPDG_TRY(launch_fill<double>(z, 0.0, n, s));
+    MPDG_TRY(op->create_rhs(z, T, T_old, nullptr, s));
+  }
+  return launch_mul(z, h->invd.get(), n, s);
+}
+
+// Learned predictor after the linear extrapolation T (= T_lin) is in place: records the
+// residual input and, with equal steps and a network, replaces T by the network's guess.
+static int learned_guess(mpdg_heat_operation *h, double dt, const double *b, double *T, const double *T_old,
+                         hipStream_t s) {
+  const bool learned = h->prm.predictor == MPDG_PREDICTOR_LEARNED;
+  if (!learned && !h->z_record) return MPDG_OK;
+  double *z = h->z_record;
+  if (!z) {
+    MPDG_TRY(h->zbuf.resize(h->op->n_dofs()));
+    z = h->zbuf.get();
+  }
+  MPDG_TRY(guess_feature(h, b, T, T_old, z, s));
+  // the network extrapolates with equal steps (as in its training trajectories)
+  const bool equal_steps = h->dt_old > 0.0 && std::fabs(dt - h->dt_old) <= 1e-12 * dt;
+  if (learned && equal_steps) {
+    const double *Tnm1 = h->buf[h->spare].get();  // T_{n-1} until the next step overwrites it
+    MPDG_TRY(mpdg_learned_predict(h->net, Tnm1, T_old, h->q_field, z, T, s));
+  }
+  return MPDG_OK;
+}
+
 // One linear implicit step with the coefficient frozen at T_n:
 // J(T_n) T_{n+1} = b,  b = M (c_m T_n + q) + surface flux, from the predicted T.
 static int linear_step(mpdg_heat_operation *h, double dt, double *T, const double *T_old, int *lin_iters,
@@ -1135,6 +1175,7 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
   }
   MPDG_TRY(launch_fill<double>(h->bvec.get(), 0.0, n, s));
   MPDG_TRY(op->create_rhs(h->bvec.get(), h->zero.get(), T_old, nullptr, s));
+  MPDG_TRY(learned_guess(h, dt, h->bvec.get(), T, T_old, s));
   if (h->prm.predictor == MPDG_PREDICTOR_LEARNED && h->prm.guess_check) {
     // residuals of the learned guess (T) and of the previous solution (T_old)
     MPDG_TRY(op->vmult(h->alt.get(), T, s));
@@ -1182,6 +1223,9 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
   switch (h->prm.predictor) {
     case MPDG_PREDICTOR_NONE: MPDG_TRY(launch_convert<double, double>(Tn, pred, n, s)); break;
     case MPDG_PREDICTOR_ZERO: MPDG_TRY(launch_fill<double>(pred, 0.0, n, s)); break;
+    case MPDG_PREDICTOR_LEARNED:
+      if (!h->net) return MPDG_ERR_INVALID;
+      [[fallthrough]];  // the network corrects the linear extrapolation once b is known
     case MPDG_PREDICTOR_LINEAR: {
       // compute_linear_predictor (source/linear_algebra/predictor_linear.cpp:76-97)
       if (std::fabs(h->dt_old) < 1e-12) MPDG_TRY(launch_convert<double, double>(Tn, pred, n, s));
@@ -1190,11 +1234,6 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
         MPDG_TRY(launch_convert<double, double>(Tn, pred, n, s));
         MPDG_TRY(launch_axpby(pred, -f, Tnm1, 1.0 + f, n, s));
       }
-      break;
-    }
-    case MPDG_PREDICTOR_LEARNED: {
-      if (!h->net) return MPDG_ERR_INVALID;
-      MPDG_TRY(mpdg_learned_predict(h->net, Tnm1, Tn, h->q_field, pred, s));
       break;
     }
     default: return MPDG_ERR_INVALID;
@@ -1210,6 +1249,7 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
   if (op->d64.robin_mask) op->T_robin = T;
   h->fell_back = 0;
   if (h->prm.step_mode == MPDG_STEP_LINEAR) return linear_step(h, dt, T, T_old, lin_iters, nl_iters, s);
+  MPDG_TRY(learned_guess(h, dt, nullptr, T, T_old, s));
   // preconditioner.update() at the predicted state (heat_diffuse_operation.cpp:276)
   if (h->pc) MPDG_TRY(h->pc->update(s));
   // NewtonRaphsonSolver::solve (newton_raphson_solver.cpp:24-66)
@@ -1252,6 +1292,11 @@ int mpdg_heat_operation_set_learned(mpdg_heat_operation *h, mpdg_learned *net, c
   if (!h) return MPDG_ERR_INVALID;
   h->net = net;
   h->q_field = q_field;
+  return MPDG_OK;
+}
+int mpdg_heat_operation_record_guess_feature(mpdg_heat_operation *h, double *z) {
+  if (!h) return MPDG_ERR_INVALID;
+  h->z_record = z;
   return MPDG_OK;
 }
 int mpdg_heat_operation_last_step_info(const mpdg_heat_operation *h, int *fell_back, double *r0, double *r) {

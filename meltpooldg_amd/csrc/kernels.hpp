@@ -96,9 +96,11 @@ void fill_apply_args(const OperatorData<Number> &op, ApplyArgs<Number, N1> &A) {
   A.ncol[0] = A.ncol[1] = 1;
 }
 
-// heat_apply.hip
+// heat_apply.hip; `kernel_start`, when given, is recorded on the stream right before the
+// cell kernel (after the face-row pre-pass) so that an event pair times that kernel alone
 template <typename Number>
-int launch_apply(const OperatorData<Number> &op, const Number *src, Number *dst, hipStream_t stream);
+int launch_apply(const OperatorData<Number> &op, const Number *src, Number *dst, hipStream_t stream,
+                 hipEvent_t kernel_start = nullptr);
 
 // heat_aux.hip: diagonal, boundary faces, residual, norms, conductivity
 template <typename Number>

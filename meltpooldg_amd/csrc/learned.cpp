@@ -1,14 +1,18 @@
 // Learned initial guess for the implicit heat step: a 4-layer 3D convolutional
-// network (3 -> 16 -> 16 -> 16 -> 1 channels, 3x3x3 taps, ReLU) on the nodal
+// network (4 -> 16 -> 16 -> 16 -> 1 channels, 3x3x3 taps, ReLU) on the nodal
 // grid predicts T_{n+1} from (T_{n-1}, T_n, heat source). It plugs into the
 // reference's predictor slot (include/meltpooldg/linear_algebra/predictor.hpp:118-160)
 // as a new predictor type; training and inference run in the BF16 MFMA kernels
 // of cnn.hip with FP32 master weights and Adam.
 //
 // Features (per sample, s = rms(T_n - T_{n-1})): c0 = (T_n - T_{n-1})/s,
-// c1 = q * qscale / s, c2 = (T_n - mean T_n)/rms. The network output y is the
-// normalised change of the increment: guess = T_n + s (c0 + y), i.e. linear
-// extrapolation plus a learned correction.
+// c1 = q * qscale / s, c2 = (T_n - mean T_n)/rms, c3 = z / sz, where z is the
+// residual of the linear extrapolation T_lin = 2 T_n - T_{n-1} for the step,
+// scaled by the inverse operator diagonal (z = D^-1 (b - J T_lin)), and
+// sz = rms(z). The network output y is the correction of the extrapolation in
+// units of sz: guess = T_lin + sz y. The exact correction is J^-1 (b - J T_lin),
+// a short-range smoothing of z, which the network learns from the solver's own
+// (T_{n-1}, T_n, q_{n+1}, z_n) -> T_{n+1} pairs.
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -21,7 +25,7 @@ using namespace mpdg;
 
 namespace {
 constexpr int kLayers = 4;
-constexpr int kCin[kLayers] = {3, 16, 16, 16};
+constexpr int kCin[kLayers] = {4, 16, 16, 16};
 constexpr int kCout[kLayers] = {16, 16, 16, 1};
 constexpr int kWgradBlocks = 256;
 inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
@@ -82,7 +86,7 @@ struct mpdg_learned {
     MPDG_TRY(g1.resize((size_t)n * 16));
     MPDG_TRY(part.resize((size_t)kWgradBlocks * (27 * 256 + 16)));
     MPDG_TRY(stats.resize(4));
-    MPDG_TRY(spart.resize(3 * 1024));
+    MPDG_TRY(spart.resize(4 * 1024));
     MPDG_TRY(lpart.resize(1024));
     return pack(nullptr);
   }
@@ -90,9 +94,9 @@ struct mpdg_learned {
     for (int l = 0; l < kLayers; ++l) MPDG_TRY(cnn_pack(w[l].get(), kCin[l], kCout[l], W[l].get(), Wt[l].get(), s));
     return MPDG_OK;
   }
-  int forward(const double *Tnm1, const double *Tn, const double *q, hipStream_t s) {
-    MPDG_TRY(cnn_stats(Tnm1, Tn, n, spart.get(), stats.get(), s));
-    MPDG_TRY(cnn_features(Tnm1, Tn, q, n, stats.get(), qscale, act[0].get(), s));
+  int forward(const double *Tnm1, const double *Tn, const double *q, const double *z, hipStream_t s) {
+    MPDG_TRY(cnn_stats(Tnm1, Tn, z, n, spart.get(), stats.get(), s));
+    MPDG_TRY(cnn_features(Tnm1, Tn, q, z, n, stats.get(), qscale, act[0].get(), s));
     for (int l = 0; l < kLayers; ++l)
       MPDG_TRY(cnn_conv(act[l].get(), act[l + 1].get(), W[l].get(), b[l].get(), dims, l == kLayers - 1 ? 0 : 1,
                         nullptr, s));
@@ -185,19 +189,19 @@ int mpdg_learned_set_weights(mpdg_learned *h, const float *host) {
   return hipDeviceSynchronize() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
-int mpdg_learned_predict(mpdg_learned *h, const double *Tnm1, const double *Tn, const double *q, double *out,
-                         void *stream) {
+int mpdg_learned_predict(mpdg_learned *h, const double *Tnm1, const double *Tn, const double *q, const double *z,
+                         double *out, void *stream) {
   if (!h || !Tnm1 || !Tn || !out) return MPDG_ERR_INVALID;
-  MPDG_TRY(h->forward(Tnm1, Tn, q, S(stream)));
+  MPDG_TRY(h->forward(Tnm1, Tn, q, z, S(stream)));
   return cnn_predict(h->act[kLayers].get(), Tnm1, Tn, h->n, h->stats.get(), out, S(stream));
 }
 
 // gradients (FP32, same layout as the weights) and the loss of one sample, no update
-int mpdg_learned_gradients(mpdg_learned *h, const double *Tnm1, const double *Tn, const double *q,
+int mpdg_learned_gradients(mpdg_learned *h, const double *Tnm1, const double *Tn, const double *q, const double *z,
                            const double *Tnp1, float *host_grad, double *loss, void *stream) {
   if (!h || !Tnm1 || !Tn || !Tnp1 || !host_grad) return MPDG_ERR_INVALID;
   hipStream_t s = S(stream);
-  MPDG_TRY(h->forward(Tnm1, Tn, q, s));
+  MPDG_TRY(h->forward(Tnm1, Tn, q, z, s));
   MPDG_TRY(h->backward(Tnm1, Tn, Tnp1, loss, s));
   size_t o = 0;
   for (int l = 0; l < kLayers; ++l) {
@@ -214,13 +218,13 @@ int mpdg_learned_gradients(mpdg_learned *h, const double *Tnm1, const double *Tn
   return MPDG_OK;
 }
 
-// one Adam step on one sample (T_{n-1}, T_n, q) -> T_{n+1}; loss = mean squared error
+// one Adam step on one sample (T_{n-1}, T_n, q, z) -> T_{n+1}; loss = mean squared error
 // of the normalised correction before the update
-int mpdg_learned_train_step(mpdg_learned *h, const double *Tnm1, const double *Tn, const double *q,
+int mpdg_learned_train_step(mpdg_learned *h, const double *Tnm1, const double *Tn, const double *q, const double *z,
                             const double *Tnp1, double lr, double *loss, void *stream) {
   if (!h || !Tnm1 || !Tn || !Tnp1) return MPDG_ERR_INVALID;
   hipStream_t s = S(stream);
-  MPDG_TRY(h->forward(Tnm1, Tn, q, s));
+  MPDG_TRY(h->forward(Tnm1, Tn, q, z, s));
   MPDG_TRY(h->backward(Tnm1, Tn, Tnp1, loss, s));
   h->step += 1;
   for (int l = 0; l < kLayers; ++l)

@@ -1,10 +1,11 @@
 """CPU oracle of the learned initial guess network -- TEST INFRASTRUCTURE ONLY.
 
 numpy restatement of the network the HIP library trains (meltpooldg_amd/csrc/cnn.hip,
-learned.cpp): features from (T_{n-1}, T_n, q), four 3x3x3 convolutions with zero
-padding (3 -> 16 -> 16 -> 16 -> 1 channels, ReLU on the first three), output =
-normalised correction of the linear extrapolation, loss = mean squared error,
-and its exact backward pass. With bf16=True the features, weights, activations
+learned.cpp): features from (T_{n-1}, T_n, q, z) with z the inverse-diagonal
+scaled residual of the linear extrapolation, four 3x3x3 convolutions with zero
+padding (4 -> 16 -> 16 -> 16 -> 1 channels, ReLU on the first three), output =
+correction of the linear extrapolation in units of rms(z), loss = mean squared
+error, and its exact backward pass. With bf16=True the features, weights, activations
 and back-propagated gradients are rounded to bfloat16 (round to nearest even)
 at the same places as the GPU kernels, so the two agree up to the FP32
 accumulation order; with bf16=False everything is FP64.
@@ -14,7 +15,7 @@ Weights use the library's layout: per layer W[co][ci][tap] then b[co], tap =
 """
 import numpy as np
 
-CIN = (3, 16, 16, 16)
+CIN = (4, 16, 16, 16)
 COUT = (16, 16, 16, 1)
 
 
@@ -64,23 +65,26 @@ def conv(x, W, b):
     return out
 
 
-def features(Tnm1, Tn, q, qscale):
+def features(Tnm1, Tn, q, z, qscale):
+    """Channels (c0..c3) and the output scale sz (= rms(z), or s = rms(T_n - T_{n-1}) without z)."""
     d = Tn - Tnm1
     n = Tn.size
     s = np.sqrt(np.sum(d * d) / n) + 1e-30
     mean = np.sum(Tn) / n
     ts = np.sqrt(max(np.sum(Tn * Tn) / n - mean * mean, 0.0)) + 1e-12 + 1e-6 * abs(mean)
-    c = np.zeros((3,) + Tn.shape)
+    sz = (np.sqrt(np.sum(z * z) / n) + 1e-30) if z is not None else s
+    c = np.zeros((4,) + Tn.shape)
     c[0] = d / s
     c[1] = (q * qscale / s) if q is not None else 0.0
     c[2] = (Tn - mean) / ts
-    return c, s
+    c[3] = (z / sz) if z is not None else 0.0
+    return c, sz
 
 
-def forward(flat, Tnm1, Tn, q, qscale=1.0, bf=True):
+def forward(flat, Tnm1, Tn, q, z, qscale=1.0, bf=True):
     rnd = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
     layers = [(rnd(W), np.asarray(b, dtype=np.float32).astype(np.float64)) for W, b in unpack(flat)]
-    x, s = features(Tnm1, Tn, q, qscale)
+    x, s = features(Tnm1, Tn, q, z, qscale)
     acts = [rnd(x)]
     for l, (W, b) in enumerate(layers):
         y = conv(acts[-1], W, b)
@@ -90,17 +94,16 @@ def forward(flat, Tnm1, Tn, q, qscale=1.0, bf=True):
     return acts, s, layers
 
 
-def predict(flat, Tnm1, Tn, q, qscale=1.0, bf=True):
-    acts, s, _ = forward(flat, Tnm1, Tn, q, qscale, bf)
-    c0 = (Tn - Tnm1) / s
-    return Tn + s * (c0 + acts[-1][0])
+def predict(flat, Tnm1, Tn, q, z, qscale=1.0, bf=True):
+    """Linear extrapolation plus the output in units of sz."""
+    acts, sz, _ = forward(flat, Tnm1, Tn, q, z, qscale, bf)
+    return (2.0 * Tn - Tnm1) + sz * acts[-1][0]
 
 
-def loss_and_gradients(flat, Tnm1, Tn, q, Tnp1, qscale=1.0, bf=True):
+def loss_and_gradients(flat, Tnm1, Tn, q, z, Tnp1, qscale=1.0, bf=True):
     rnd = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
-    acts, s, layers = forward(flat, Tnm1, Tn, q, qscale, bf)
-    c0 = (Tn - Tnm1) / s
-    tgt = (Tnp1 - Tn) / s - c0
+    acts, sz, layers = forward(flat, Tnm1, Tn, q, z, qscale, bf)
+    tgt = (Tnp1 - (2.0 * Tn - Tnm1)) / sz
     e = acts[-1][0] - tgt
     n = Tn.size
     loss = float(np.sum(e * e) / n)

@@ -13,7 +13,9 @@ def sample(shape=(5, 6, 7), seed=0):
     Tn = Tnm1 + 0.1 * np.exp(-((x - 0.4) ** 2 + (y - 0.5) ** 2) / 0.05) + 0.01 * rng.standard_normal(shape)
     Tnp1 = Tn + 0.12 * np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2) / 0.05)
     q = np.where(np.arange(shape[0])[:, None, None] == shape[0] - 1, np.exp(-((x - 0.5) ** 2) / 0.02), 0.0)
-    return Tnm1, Tn, q, Tnp1
+    # stands in for the scaled residual of the linear extrapolation: a smooth bump plus noise
+    zres = 0.03 * np.exp(-((x - 0.45) ** 2 + (y - 0.5) ** 2) / 0.03) + 0.002 * rng.standard_normal(shape)
+    return Tnm1, Tn, q, zres, Tnp1
 
 
 def init(seed=1):
@@ -54,9 +56,9 @@ def test_conv_matches_direct_loop():
 
 
 def test_backward_matches_finite_differences():
-    Tnm1, Tn, q, Tnp1 = sample()
+    Tnm1, Tn, q, z, Tnp1 = sample()
     w = init()
-    loss, g = N.loss_and_gradients(w, Tnm1, Tn, q, Tnp1, bf=False)
+    loss, g = N.loss_and_gradients(w, Tnm1, Tn, q, z, Tnp1, bf=False)
     rng = np.random.default_rng(3)
     idx = rng.choice(w.size, 40, replace=False)
     h = 1e-6
@@ -64,17 +66,36 @@ def test_backward_matches_finite_differences():
         wp, wm = w.copy(), w.copy()
         wp[i] += h
         wm[i] -= h
-        fp = N.loss_and_gradients(wp, Tnm1, Tn, q, Tnp1, bf=False)[0]
-        fm = N.loss_and_gradients(wm, Tnm1, Tn, q, Tnp1, bf=False)[0]
+        fp = N.loss_and_gradients(wp, Tnm1, Tn, q, z, Tnp1, bf=False)[0]
+        fm = N.loss_and_gradients(wm, Tnm1, Tn, q, z, Tnp1, bf=False)[0]
         fd = (fp - fm) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g).max())
 
 
 def test_prediction_is_linear_extrapolation_plus_output():
-    Tnm1, Tn, q, _ = sample()
+    Tnm1, Tn, q, z, _ = sample()
     w = init()
     w[-1] = 0.0  # last bias
     nlast = N.COUT[3] * N.CIN[3] * 27
     w[-1 - nlast:-1] = 0.0  # last weights: output 0
-    p = N.predict(w, Tnm1, Tn, q, bf=False)
+    p = N.predict(w, Tnm1, Tn, q, z, bf=False)
     assert np.allclose(p, 2 * Tn - Tnm1, atol=1e-12)
+    # a bias b on the output adds b rms(z) everywhere
+    w[-1] = 0.5
+    p = N.predict(w, Tnm1, Tn, q, z, bf=False)
+    assert np.allclose(p - (2 * Tn - Tnm1), 0.5 * np.sqrt(np.mean(z * z)), atol=1e-12)
+
+
+def test_features_channels():
+    """Each channel is its normalised input; without z the output scale is rms(T_n - T_{n-1})."""
+    Tnm1, Tn, q, z, _ = sample()
+    c, sz = N.features(Tnm1, Tn, q, z, 0.5)
+    d = Tn - Tnm1
+    s = np.sqrt(np.mean(d * d))
+    assert np.isclose(sz, np.sqrt(np.mean(z * z)))
+    assert np.allclose(c[0] * s, d) and np.allclose(c[1] * s, 0.5 * q) and np.allclose(c[3] * sz, z)
+    # c2 is T_n standardised; its scale carries a 1e-6 |mean| guard against constant fields
+    assert abs(np.mean(c[2])) < 1e-12
+    assert np.isclose(np.std(c[2]) * (np.std(Tn) + 1e-12 + 1e-6 * abs(np.mean(Tn))), np.std(Tn), rtol=1e-12)
+    c, s2 = N.features(Tnm1, Tn, None, None, 0.5)
+    assert np.isclose(s2, s) and not c[1].any() and not c[3].any()

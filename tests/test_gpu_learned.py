@@ -19,14 +19,14 @@ def to_dev(torch, a):
 def test_forward_matches_oracle(cuda, shape):
     torch = cuda
     import meltpooldg_amd as M
-    Tnm1, Tn, q, _ = sample(shape)
+    Tnm1, Tn, q, z, _ = sample(shape)
     net = M.LearnedPredictor((shape[2], shape[1], shape[0]), qscale=0.7)
     w = init().astype(np.float32)
     net.set_weights(w)
     out = torch.empty(Tn.size, dtype=torch.float64, device="cuda")
-    net.predict(to_dev(torch, Tnm1), to_dev(torch, Tn), to_dev(torch, q), out)
+    net.predict(to_dev(torch, Tnm1), to_dev(torch, Tn), to_dev(torch, q), to_dev(torch, z), out)
     torch.cuda.synchronize()
-    ref = N.predict(w, Tnm1, Tn, q, qscale=0.7, bf=True).ravel()
+    ref = N.predict(w, Tnm1, Tn, q, z, qscale=0.7, bf=True).ravel()
     lin = (2 * Tn - Tnm1).ravel()
     # compare the learned correction (prediction minus linear extrapolation)
     got, exp = out.cpu().numpy() - lin, ref - lin
@@ -37,12 +37,12 @@ def test_gradients_match_oracle(cuda):
     torch = cuda
     import meltpooldg_amd as M
     shape = (6, 7, 18)
-    Tnm1, Tn, q, Tnp1 = sample(shape, seed=2)
+    Tnm1, Tn, q, z, Tnp1 = sample(shape, seed=2)
     net = M.LearnedPredictor((shape[2], shape[1], shape[0]), qscale=0.5)
     w = init(3).astype(np.float32)
     net.set_weights(w)
-    loss, g = net.gradients(to_dev(torch, Tnm1), to_dev(torch, Tn), to_dev(torch, q), to_dev(torch, Tnp1))
-    ref_loss, ref = N.loss_and_gradients(w, Tnm1, Tn, q, Tnp1, qscale=0.5, bf=True)
+    loss, g = net.gradients(*[to_dev(torch, a) for a in (Tnm1, Tn, q, z, Tnp1)])
+    ref_loss, ref = N.loss_and_gradients(w, Tnm1, Tn, q, z, Tnp1, qscale=0.5, bf=True)
     assert abs(loss - ref_loss) <= 2e-2 * ref_loss
     o = 0
     for ci, co in zip(N.CIN, N.COUT):
@@ -57,9 +57,9 @@ def test_training_reduces_loss(cuda):
     torch = cuda
     import meltpooldg_amd as M
     shape = (8, 8, 16)
-    Tnm1, Tn, q, Tnp1 = sample(shape, seed=4)
+    Tnm1, Tn, q, z, Tnp1 = sample(shape, seed=4)
     net = M.LearnedPredictor((shape[2], shape[1], shape[0]), qscale=0.5, seed=7)
-    d = [to_dev(torch, a) for a in (Tnm1, Tn, q, Tnp1)]
+    d = [to_dev(torch, a) for a in (Tnm1, Tn, q, z, Tnp1)]
     first = net.train_step(*d, 1e-3)
     for _ in range(60):
         last = net.train_step(*d, 1e-3)
@@ -67,24 +67,24 @@ def test_training_reduces_loss(cuda):
 
 
 def test_trained_network_beats_the_predictors_without_it(cuda):
-    """Q2 on 16^3 cells, laser scan: train on steps 2..40, evaluate steps 41..60 from the
-    same trajectory state. The learned guess starts CG closer to the solution than both
-    the previous step (predictor none) and linear extrapolation, and needs no more
-    iterations."""
+    """Q2 on 16^3 cells, laser scan: train on the pairs of steps 2..40, evaluate steps 41..60
+    from the same trajectory state. The learned guess starts CG closer to the solution than
+    linear extrapolation (the untrained network's guess) and than the previous step (the
+    fallback), and needs no more iterations."""
     torch = cuda
     import meltpooldg_amd as M
     from meltpooldg_amd.application import LaserScan, generate_trajectory, train
     k, cells, dt = 2, 16, 1e-3
     h = 1.0 / cells
-    _, Ts, qs, _ = generate_trajectory(k, cells, 60, dt=dt, precond="gmg32", predictor=M.PREDICTOR_LINEAR)
+    _, Ts, qs, zs, _ = generate_trajectory(k, cells, 60, dt=dt, precond="gmg32", predictor=M.PREDICTOR_LINEAR)
     dims = (k * cells + 1,) * 3
     net = M.LearnedPredictor(dims, qscale=dt / h, seed=11)
-    train(net, Ts[:41], qs[:40], epochs=15, lr=2e-3)
+    losses = train(net, Ts[:41], qs[:40], zs[:40], epochs=20, lr=2e-3)
     res = {}
     for name, pred in (("none", M.PREDICTOR_NONE), ("linear", M.PREDICTOR_LINEAR), ("learned", M.PREDICTOR_LEARNED)):
         case = LaserScan(k, cells, dt=dt, predictor=pred, learned=net if pred == M.PREDICTOR_LEARNED else None)
         # restart from the trajectory at step 39; one replayed step rebuilds the history (T_39, T_40)
-        r0s, its = [], []
+        r0s, its, fbs = [], [], []
         case.t = 39 * dt
         case.heat.set_initial_condition(Ts[39])
         case.step()
@@ -92,6 +92,10 @@ def test_trained_network_beats_the_predictors_without_it(cuda):
             it, _, (fb, r0, r) = case.step()
             r0s.append(r0)
             its.append(it)
-        res[name] = (float(np.mean(np.log10(r0s))), float(np.mean(its)))
-    assert res["learned"][0] < res["linear"][0] < res["none"][0]
+            fbs.append(fb)
+        res[name] = (float(np.mean(np.log10(r0s))), float(np.mean(its)), int(sum(fbs)))
+    print("training loss", losses[0], losses[-1], "log10 r0 / iterations / fallbacks", res)
+    assert losses[-1] < 0.5 * losses[0]
+    assert res["learned"][2] == 0
+    assert res["learned"][0] < res["linear"][0] - 0.3 and res["linear"][0] < res["none"][0]
     assert res["learned"][1] <= res["linear"][1]
