@@ -66,6 +66,7 @@ int mpdg_heat_op_reset_time_increment(mpdg_heat_op *op, double dt);
 /* conductivity data (Material::compute_parameters, heat_diffuse_operator.cpp:1172-1210);
    arrays are device memory of the operator's precision and stay owned by the caller */
 int mpdg_heat_op_set_conductivity_constant(mpdg_heat_op *op, double k);
+/* k_qp: [cell][qz][qy][qx] per quadrature point, 16-byte aligned (MPDG_ERR_INVALID otherwise) */
 int mpdg_heat_op_set_conductivity_qp(mpdg_heat_op *op, const void *k_qp);
 int mpdg_heat_op_set_conductivity_cell(mpdg_heat_op *op, const void *k_cell);
 int mpdg_heat_op_set_conductivity_law(mpdg_heat_op *op, int law, const double *params4,

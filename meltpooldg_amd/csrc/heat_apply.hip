@@ -164,6 +164,54 @@ __device__ __forceinline__ Number law_kappa(int law, const Number *p, Number T) 
   return p[0] + (p[1] - p[0]) * (Number(3) - Number(2) * fs) * fs * fs;
 }
 
+// Gather the z-plane l of a cell (read_dof_values with homogeneous Dirichlet rows zeroed).
+template <int N1, typename Number>
+__device__ __forceinline__ void gather_plane(const Number *__restrict__ src, long base, bool valid, int dmask,
+                                             const int *nd, int ix0, int iy0, int iz, long sy,
+                                             Number (&u)[N1][N1]) {
+#pragma unroll
+  for (int j = 0; j < N1; ++j)
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      Number v = 0;
+      if (valid) {
+        v = src[base + i + sy * j];
+        if (dmask) {
+          const int ix = ix0 + i, iy = iy0 + j;
+          if (((dmask & 1) && ix == 0) || ((dmask & 2) && ix == nd[0] - 1) || ((dmask & 4) && iy == 0) ||
+              ((dmask & 8) && iy == nd[1] - 1) || ((dmask & 16) && iz == 0) || ((dmask & 32) && iz == nd[2] - 1))
+            v = 0;
+        }
+      }
+      u[j][i] = v;
+    }
+}
+
+// Per-quadrature-point conductivity of a cell at qy = l in the y-plane layout: kq[qz][qx] =
+// kappa[cell][qz][l][qx], i.e. N1 contiguous values per qz (16-byte loads where they fit).
+template <int N1, typename Number>
+__device__ __forceinline__ void load_kappa_qp(const Number *__restrict__ kappa, long cell, int l, bool valid,
+                                              Number (&kq)[N1][N1]) {
+  constexpr int NQ = N1 * N1 * N1;
+  const Number *p = kappa + (valid ? cell : 0) * NQ + N1 * l;
+  constexpr bool vec = sizeof(Number) == 8 && N1 % 2 == 0;  // 2 doubles per load, 16-byte aligned rows
+#pragma unroll
+  for (int qz = 0; qz < N1; ++qz) {
+    if constexpr (vec) {
+      const double2 *r = reinterpret_cast<const double2 *>(p + N1 * N1 * qz);
+#pragma unroll
+      for (int h = 0; h < N1 / 2; ++h) {
+        const double2 v = r[h];
+        kq[qz][2 * h] = valid ? v.x : Number(0);
+        kq[qz][2 * h + 1] = valid ? v.y : Number(0);
+      }
+    } else {
+#pragma unroll
+      for (int qx = 0; qx < N1; ++qx) kq[qz][qx] = valid ? p[N1 * N1 * qz + qx] : Number(0);
+    }
+  }
+}
+
 // wavefronts per SIMD the register allocation targets (FP64 Q3: 2, the rest: the compiler's choice)
 template <int N1, typename Number> struct Occupancy { static constexpr int W = 1; };
 template <> struct Occupancy<4, double> { static constexpr int W = 2; };
@@ -199,34 +247,30 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
   const long sy = P.nd[0], sz = (long)P.nd[0] * P.nd[1];
   const int nchunks = (P.nc[0] + G::BX - 1) / G::BX;
 
+  // the src plane of the next chunk is loaded while the current one is scattered and written
+  Number nxt[N1][N1];
+  {
+    const int cx = lx;
+    const bool valid = lane_on && cx < P.nc[0] && cy < P.nc[1] && cz < P.nc[2];
+    const long base = (long)K * cx + sy * (K * cy) + sz * (K * cz + l);
+    gather_plane<N1>(src, base, valid, P.dmask, P.nd, K * cx, K * cy, K * cz + l, sy, nxt);
+  }
+
   for (int chunk = 0; chunk < nchunks; ++chunk) {
     const int cx = chunk * G::BX + lx;
     const bool valid = lane_on && cx < P.nc[0] && cy < P.nc[1] && cz < P.nc[2];
     const long cell = (long)cx + (long)P.nc[0] * (cy + (long)P.nc[1] * cz);
-
-    // ---- gather the z-plane l of the cell (read_dof_values with homogeneous DBC) ----
-    Number u[N1][N1];
     const long base = (long)K * cx + sy * (K * cy) + sz * (K * cz + l);
+
+    Number u[N1][N1];
 #pragma unroll
     for (int j = 0; j < N1; ++j)
 #pragma unroll
-      for (int i = 0; i < N1; ++i) {
-        Number v = 0;
-        if (valid) {
-          v = src[base + i + sy * j];
-          if (P.dmask) {
-            const int ix = K * cx + i, iy = K * cy + j, iz = K * cz + l;
-            if (((P.dmask & 1) && ix == 0) || ((P.dmask & 2) && ix == P.nd[0] - 1) ||
-                ((P.dmask & 4) && iy == 0) || ((P.dmask & 8) && iy == P.nd[1] - 1) ||
-                ((P.dmask & 16) && iz == 0) || ((P.dmask & 32) && iz == P.nd[2] - 1))
-              v = 0;
-          }
-        }
-        u[j][i] = v;
-      }
+      for (int i = 0; i < N1; ++i) u[j][i] = nxt[j][i];
 
-    // ---- conductivity law: T at the quadrature points in the y-plane layout ----
+    // ---- conductivity at the quadrature points in the y-plane layout ----
     Number kq[N1][N1];
+    if (KMODE == KAPPA_QP) load_kappa_qp<N1>(P.kappa_arr, cell, l, valid, kq);
     if (KMODE == KAPPA_LAW) {
       Number T0[N1][N1], w1[N1][N1], t1[N1][N1];
 #pragma unroll
@@ -277,8 +321,7 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
           constexpr int NQ = N1 * N1 * N1;
           const int q = qx + N1 * (l + N1 * qz);
           Number kap = kc;
-          if (KMODE == KAPPA_QP) kap = valid ? P.kappa_arr[cell * NQ + q] : Number(0);
-          if (KMODE == KAPPA_LAW) kap = kq[qz][qx];
+          if (KMODE == KAPPA_QP || KMODE == KAPPA_LAW) kap = kq[qz][qx];
           // flux in reference coordinates: kappa G grad_xi u, G = JxW J^-1 J^-T
           const Number *M = P.metric + (valid ? cell : 0) * 7 * NQ + q;
           const Number jxw = valid ? M[0] : Number(0);
@@ -295,7 +338,6 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
     } else {
       // x direction row by row (with the mass term and the y flux), then z column by column,
       // so that only v, e and gy are live as full arrays
-      constexpr int NQ = N1 * N1 * N1;
 #pragma unroll
       for (int qz = 0; qz < N1; ++qz) {
         Number f[N1];
@@ -303,8 +345,7 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
         for (int qx = 0; qx < N1; ++qx) {
           const Number W = P.w[qx] * wy * P.w[qz];
           Number kap = kc;
-          if (KMODE == KAPPA_QP) kap = valid ? P.kappa_arr[cell * NQ + qx + N1 * (l + N1 * qz)] : Number(0);
-          if (KMODE == KAPPA_LAW) kap = kq[qz][qx];
+          if (KMODE == KAPPA_QP || KMODE == KAPPA_LAW) kap = kq[qz][qx];
           const Number kw = kap * W;
           e[qz][qx] = P.cm * W * v[qz][qx];
           gy[qz][qx] *= kw * P.ks[1];
@@ -328,8 +369,7 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
         for (int qz = 0; qz < N1; ++qz) {
           const Number W = P.w[qx] * wy * P.w[qz];
           Number kap = kc;
-          if (KMODE == KAPPA_QP) kap = valid ? P.kappa_arr[cell * NQ + qx + N1 * (l + N1 * qz)] : Number(0);
-          if (KMODE == KAPPA_LAW) kap = kq[qz][qx];
+          if (KMODE == KAPPA_QP || KMODE == KAPPA_LAW) kap = kq[qz][qx];
           Number g = 0;
 #pragma unroll
           for (int p = 0; p < N1; ++p) g = fma(P.D[qz][p], v[p][qx], g);
@@ -353,6 +393,11 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
     sweep_row_add<N1, true>(P.D, b, t);
     sweep_row<N1, true>(P.S, t, b);
     sweep_col<N1, true>(P.S, b, u);
+    if (chunk + 1 < nchunks) {
+      const int cxn = cx + G::BX;
+      const bool vn = lane_on && cxn < P.nc[0] && cy < P.nc[1] && cz < P.nc[2];
+      gather_plane<N1>(src, base + (long)K * G::BX, vn, P.dmask, P.nd, K * cxn, K * cy, K * cz + l, sy, nxt);
+    }
 
     // ---- scatter: sum the chunk's cells in the LDS DoF brick ----
     __syncthreads();

@@ -2,6 +2,7 @@
 #include "heat_lib.hpp"
 
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <new>
 #include <type_traits>
@@ -902,6 +903,7 @@ int mpdg_heat_op_set_conductivity_constant(mpdg_heat_op *op, double k) {
 }
 int mpdg_heat_op_set_conductivity_qp(mpdg_heat_op *op, const void *k) {
   if (!op || !k) return MPDG_ERR_INVALID;
+  if (reinterpret_cast<uintptr_t>(k) % 16 != 0) return MPDG_ERR_INVALID;  // read with 16-byte loads
   op->d64.kappa_mode = op->d32.kappa_mode = KAPPA_QP;
   op->d64.kappa_arr = (const double *)k;
   op->d32.kappa_arr = (const float *)k;

@@ -13,8 +13,9 @@ def sample(shape=(5, 6, 7), seed=0):
     Tn = Tnm1 + 0.1 * np.exp(-((x - 0.4) ** 2 + (y - 0.5) ** 2) / 0.05) + 0.01 * rng.standard_normal(shape)
     Tnp1 = Tn + 0.12 * np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2) / 0.05)
     q = np.where(np.arange(shape[0])[:, None, None] == shape[0] - 1, np.exp(-((x - 0.5) ** 2) / 0.02), 0.0)
-    # stands in for the scaled residual of the linear extrapolation: a smooth bump plus noise
-    zres = 0.03 * np.exp(-((x - 0.45) ** 2 + (y - 0.5) ** 2) / 0.03) + 0.002 * rng.standard_normal(shape)
+    # stands in for the scaled residual of the linear extrapolation, which is close to the
+    # correction T_{n+1} - (2 T_n - T_{n-1}) for a mass-dominated operator: that plus noise
+    zres = 0.8 * (Tnp1 - (2 * Tn - Tnm1)) + 0.002 * rng.standard_normal(shape)
     return Tnm1, Tn, q, zres, Tnp1
 
 
