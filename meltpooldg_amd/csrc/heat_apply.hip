@@ -41,7 +41,13 @@ template <int N1> struct ApplyGeom {
   static constexpr int AX = K * BX + 1, AY = K * BY + 1, AZ = K * BZ + 1;
   static constexpr int ACC = AX * AY * AZ;            // DoF brick of a chunk
   static constexpr int TBUF = 4 * CPW * CS;           // transpose buffers of the 4 wavefronts
-  static constexpr int LDS = TBUF > ACC ? TBUF : ACC;
+  // the DoF brick is summed in FP64 at every precision (LDS FP64 adds are fast on CDNA4,
+  // FP32 LDS atomics are not); LDS in doubles: max(transpose buffers, brick)
+  template <typename Number>
+  static constexpr int lds_doubles() {
+    constexpr int tb = (TBUF * (int)sizeof(Number) + 7) / 8;
+    return tb > ACC ? tb : ACC;
+  }
   static constexpr int CARRY = AY * AZ;
   // x rows of the DoF brick written per pass of the write-out loop (lane = x in the row)
   static constexpr int ROWS_PER_PASS = 256 / AX;
@@ -49,8 +55,8 @@ template <int N1> struct ApplyGeom {
 
 // ---- register-level 1D contractions on an N1 x N1 array a[row][col] ----
 // along columns (second index): out[r][q] = sum_i M[q][i] in[r][i]
-template <int N1, bool TR, typename Number>
-__device__ __forceinline__ void sweep_col(const Number (&M)[N1][N1], const Number (&in)[N1][N1],
+template <int N1, bool TR, typename Number, typename Mat>
+__device__ __forceinline__ void sweep_col(const Mat &M, const Number (&in)[N1][N1],
                                           Number (&out)[N1][N1]) {
 #pragma unroll
   for (int r = 0; r < N1; ++r)
@@ -58,13 +64,13 @@ __device__ __forceinline__ void sweep_col(const Number (&M)[N1][N1], const Numbe
     for (int q = 0; q < N1; ++q) {
       Number acc = 0;
 #pragma unroll
-      for (int i = 0; i < N1; ++i) acc = fma(TR ? M[i][q] : M[q][i], in[r][i], acc);
+      for (int i = 0; i < N1; ++i) acc = fma(TR ? M(i, q) : M(q, i), in[r][i], acc);
       out[r][q] = acc;
     }
 }
 // along rows (first index): out[q][c] = sum_i M[q][i] in[i][c]
-template <int N1, bool TR, typename Number>
-__device__ __forceinline__ void sweep_row(const Number (&M)[N1][N1], const Number (&in)[N1][N1],
+template <int N1, bool TR, typename Number, typename Mat>
+__device__ __forceinline__ void sweep_row(const Mat &M, const Number (&in)[N1][N1],
                                           Number (&out)[N1][N1]) {
 #pragma unroll
   for (int q = 0; q < N1; ++q)
@@ -72,12 +78,12 @@ __device__ __forceinline__ void sweep_row(const Number (&M)[N1][N1], const Numbe
     for (int c = 0; c < N1; ++c) {
       Number acc = 0;
 #pragma unroll
-      for (int i = 0; i < N1; ++i) acc = fma(TR ? M[i][q] : M[q][i], in[i][c], acc);
+      for (int i = 0; i < N1; ++i) acc = fma(TR ? M(i, q) : M(q, i), in[i][c], acc);
       out[q][c] = acc;
     }
 }
-template <int N1, bool TR, typename Number>
-__device__ __forceinline__ void sweep_row_add(const Number (&M)[N1][N1], const Number (&in)[N1][N1],
+template <int N1, bool TR, typename Number, typename Mat>
+__device__ __forceinline__ void sweep_row_add(const Mat &M, const Number (&in)[N1][N1],
                                               Number (&out)[N1][N1]) {
 #pragma unroll
   for (int q = 0; q < N1; ++q)
@@ -85,12 +91,12 @@ __device__ __forceinline__ void sweep_row_add(const Number (&M)[N1][N1], const N
     for (int c = 0; c < N1; ++c) {
       Number acc = out[q][c];
 #pragma unroll
-      for (int i = 0; i < N1; ++i) acc = fma(TR ? M[i][q] : M[q][i], in[i][c], acc);
+      for (int i = 0; i < N1; ++i) acc = fma(TR ? M(i, q) : M(q, i), in[i][c], acc);
       out[q][c] = acc;
     }
 }
-template <int N1, bool TR, typename Number>
-__device__ __forceinline__ void sweep_col_add(const Number (&M)[N1][N1], const Number (&in)[N1][N1],
+template <int N1, bool TR, typename Number, typename Mat>
+__device__ __forceinline__ void sweep_col_add(const Mat &M, const Number (&in)[N1][N1],
                                               Number (&out)[N1][N1]) {
 #pragma unroll
   for (int r = 0; r < N1; ++r)
@@ -98,7 +104,7 @@ __device__ __forceinline__ void sweep_col_add(const Number (&M)[N1][N1], const N
     for (int q = 0; q < N1; ++q) {
       Number acc = out[r][q];
 #pragma unroll
-      for (int i = 0; i < N1; ++i) acc = fma(TR ? M[i][q] : M[q][i], in[r][i], acc);
+      for (int i = 0; i < N1; ++i) acc = fma(TR ? M(i, q) : M(q, i), in[r][i], acc);
       out[r][q] = acc;
     }
 }
@@ -225,8 +231,10 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
                                                          Number *__restrict__ dst) {
   using G = ApplyGeom<N1>;
   constexpr int K = G::K;
-  __shared__ Number lds[G::LDS];
-  __shared__ Number carry[G::CARRY];
+  __shared__ double lds_raw[G::template lds_doubles<Number>()];
+  __shared__ double carry[G::CARRY];
+  Number *const lds = reinterpret_cast<Number *>(lds_raw);  // transpose slots
+  double *const acc = lds_raw;                               // DoF brick of the chunk
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -351,14 +359,14 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
           gy[qz][qx] *= kw * P.ks[1];
           Number g = 0;
 #pragma unroll
-          for (int p = 0; p < N1; ++p) g = fma(P.D[qx][p], v[qz][p], g);
+          for (int p = 0; p < N1; ++p) g = fma(P.D(qx, p), v[qz][p], g);
           f[qx] = kw * P.ks[0] * g;
         }
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
           Number acc = e[qz][i];
 #pragma unroll
-          for (int qx = 0; qx < N1; ++qx) acc = fma(P.D[qx][i], f[qx], acc);
+          for (int qx = 0; qx < N1; ++qx) acc = fma(P.D(qx, i), f[qx], acc);
           e[qz][i] = acc;
         }
       }
@@ -372,14 +380,14 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
           if (KMODE == KAPPA_QP || KMODE == KAPPA_LAW) kap = kq[qz][qx];
           Number g = 0;
 #pragma unroll
-          for (int p = 0; p < N1; ++p) g = fma(P.D[qz][p], v[p][qx], g);
+          for (int p = 0; p < N1; ++p) g = fma(P.D(qz, p), v[p][qx], g);
           f[qz] = kap * W * P.ks[2] * g;
         }
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
           Number acc = e[i][qx];
 #pragma unroll
-          for (int qz = 0; qz < N1; ++qz) acc = fma(P.D[qz][i], f[qz], acc);
+          for (int qz = 0; qz < N1; ++qz) acc = fma(P.D(qz, i), f[qz], acc);
           e[i][qx] = acc;
         }
       }
@@ -405,7 +413,7 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
     const int DX = K * min(G::BX, P.nc[0] - chunk * G::BX) + 1;
     for (int idx = tid; idx < G::ACC; idx += 256) {
       const int x = idx % G::AX, yz = idx / G::AX;
-      lds[idx] = (x == 0 && chunk > 0) ? carry[yz] : Number(0);
+      acc[idx] = (x == 0 && chunk > 0) ? carry[yz] : 0.0;
     }
     __syncthreads();
     if (valid) {
@@ -413,7 +421,7 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
       for (int j = 0; j < N1; ++j)
 #pragma unroll
         for (int i = 0; i < N1; ++i)
-          atomicAdd(&lds[(K * lx + i) + G::AX * ((K * ly + j) + G::AY * (K * lz + l))], u[j][i]);
+          atomicAdd(&acc[(K * lx + i) + G::AX * ((K * ly + j) + G::AY * (K * lz + l))], (double)u[j][i]);
     }
     __syncthreads();
     const bool last = chunk == nchunks - 1;
@@ -424,11 +432,12 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
       for (int r = wr; r < G::AY * G::AZ; r += G::ROWS_PER_PASS) {
         const int y = r % G::AY, z = r / G::AY;
         if (y >= DY || z >= DZ) continue;
-        const Number val = lds[wx + G::AX * r];
+        const double sum = acc[wx + G::AX * r];
         if (wx == DX - 1 && !last) {
-          carry[r] = val;
+          carry[r] = sum;
           continue;
         }
+        const Number val = (Number)sum;
         const int iy = Y0 + y, iz = Z0 + z;
         const long g = ix + sy * iy + sz * iz;
         const bool face = (y == 0 || y == DY - 1 || z == 0 || z == DZ - 1);

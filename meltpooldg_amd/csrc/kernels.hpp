@@ -52,10 +52,27 @@ struct OperatorData {
   }
 };
 
+// A 1D N1 x N1 table stored by its symmetry about the cell centre (Gauss-Lobatto nodes and
+// Gauss points are symmetric): T[N1-1-q][N1-1-i] = T[q][i] (ANTI = false, shape values) or
+// -T[q][i] (ANTI = true, derivatives); only rows q < (N1+1)/2 are kept. Halving the tables
+// halves the scalar registers the kernels hold them in. Indices are compile-time constants
+// after unrolling, so the mirrored half costs at most a sign modifier.
+template <typename Number, int N1, bool ANTI>
+struct Half1D {
+  Number h[(N1 + 1) / 2][N1];
+  __host__ __device__ __forceinline__ Number operator()(int q, int i) const {
+    return (2 * q < N1) ? h[q][i] : (ANTI ? -h[N1 - 1 - q][N1 - 1 - i] : h[N1 - 1 - q][N1 - 1 - i]);
+  }
+  void set(const double (&full)[kMaxN1][kMaxN1]) {
+    for (int q = 0; 2 * q < N1; ++q)
+      for (int i = 0; i < N1; ++i) h[q][i] = (Number)full[q][i];
+  }
+};
+
 template <typename Number, int N1>
 struct ApplyArgs {
-  Number S[N1][N1];  // shape values at the Gauss points
-  Number D[N1][N1];  // collocation derivative on the Gauss points
+  Half1D<Number, N1, false> S;  // shape values at the Gauss points
+  Half1D<Number, N1, true> D;   // collocation derivative on the Gauss points
   Number w[N1];      // Gauss weights
   int nc[3], nd[3], ncol[2];
   int dmask, xcd_cols, law;
@@ -71,13 +88,9 @@ struct ApplyArgs {
 template <typename Number, int N1>
 void fill_apply_args(const OperatorData<Number> &op, ApplyArgs<Number, N1> &A) {
   FE1D fe(N1 - 1);
-  for (int a = 0; a < N1; ++a) {
-    A.w[a] = (Number)fe.qw[a];
-    for (int b = 0; b < N1; ++b) {
-      A.S[a][b] = (Number)fe.S[a][b];
-      A.D[a][b] = (Number)fe.D[a][b];
-    }
-  }
+  for (int a = 0; a < N1; ++a) A.w[a] = (Number)fe.qw[a];
+  A.S.set(fe.S);
+  A.D.set(fe.D);
   for (int d = 0; d < 3; ++d) {
     A.nc[d] = op.nc[d];
     A.nd[d] = op.nd[d];
