@@ -199,27 +199,36 @@ int mpdg_heat_op_surface_flux_field(const mpdg_heat_op *op, double *out, void *s
 /* --- learned initial guess: 3D CNN (4 -> 16 -> 16 -> 16 -> 1 channels, 3x3x3 taps, ReLU) on the
        DoF grid, BF16 MFMA kernels, FP32 master weights, Adam. Enters the predictor slot of
        include/meltpooldg/linear_algebra/predictor.hpp:118-160 as a new predictor type. --- */
-int mpdg_learned_create(const int *dims, double qscale, unsigned long long seed, mpdg_learned **net);
+/* network for the DoF grid of Q_degree on n_cells[3] cells; it works on the DoF grid of
+   Q_degree on n_cells / coarsening cells (coarsening 1, 2, 4 or 8, dividing n_cells): fields are
+   carried there by full weighting and the correction back by the FE prolongation. qscale
+   multiplies the heat-source channel (e.g. dt / (rho c_p h)). */
+int mpdg_learned_create(const int *n_cells, int degree, int coarsening, double qscale,
+                        unsigned long long seed, mpdg_learned **net);
+/* DoF grid of the fields it takes and its working grid (either pointer may be NULL) */
+int mpdg_learned_grid(const mpdg_learned *net, int *dof_dims, int *net_dims);
 int mpdg_learned_destroy(mpdg_learned *net);
 int mpdg_learned_n_params(const mpdg_learned *net);
 /* weights as one FP32 host array: per layer W[co][ci][tap] then b[co] */
 int mpdg_learned_get_weights(const mpdg_learned *net, float *host);
 int mpdg_learned_set_weights(mpdg_learned *net, const float *host);
-/* out = (2 T_n - T_{n-1}) + sz y(T_{n-1}, T_n, q, z), sz = rms(z): linear extrapolation (equal
-   steps) plus the learned correction. Device FP64 DoF vectors; z is the residual input recorded
-   by mpdg_heat_operation_record_guess_feature; q and z may be NULL (channel zero, sz = rms of
-   T_n - T_{n-1}) */
+/* out = (2 T_n - T_{n-1}) + P (sz y(T_{n-1}, T_n, q, z)), sz = rms(z) on the network grid:
+   linear extrapolation (equal steps) plus the learned correction. Device FP64 DoF vectors; z is
+   the residual input recorded by mpdg_heat_operation_record_guess_feature; q and z may be NULL
+   (channel zero; without z, sz = rms(T_n - T_{n-1})) */
 int mpdg_learned_predict(mpdg_learned *net, const double *T_nm1, const double *T_n, const double *q,
                          const double *z, double *out, void *stream);
 /* one Adam step on the sample (T_{n-1}, T_n, q, z) -> T_{n+1}; loss = mean squared error of
-   y against (T_{n+1} - (2 T_n - T_{n-1}))/sz before the step */
+   y against W(T_{n+1} - (2 T_n - T_{n-1}))/sz before the step (W = full weighting to the
+   network grid) */
 int mpdg_learned_train_step(mpdg_learned *net, const double *T_nm1, const double *T_n, const double *q,
                             const double *z, const double *T_np1, double lr, double *loss, void *stream);
 /* gradients of the loss (host FP32, weight layout) without an update */
 int mpdg_learned_gradients(mpdg_learned *net, const double *T_nm1, const double *T_n, const double *q,
                            const double *z, const double *T_np1, float *host_grad, double *loss,
                            void *stream);
-/* layer activations of the last forward pass as bf16 bit patterns [voxel][16] (layer 0 = input) */
+/* layer activations of the last forward pass as bf16 bit patterns [voxel][16] on the network grid
+   (layer 0 = input) */
 int mpdg_learned_activation(const mpdg_learned *net, int layer, unsigned short *host);
 
 #ifdef __cplusplus

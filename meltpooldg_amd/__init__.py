@@ -109,7 +109,8 @@ _SIGNATURES = {
     "mpdg_heat_operation_record_guess_feature": ([_V, _V], _I),
     "mpdg_heat_operation_last_step_info": ([_V, C.POINTER(_I), C.POINTER(_D), C.POINTER(_D)], _I),
     "mpdg_heat_op_surface_flux_field": ([_V, _V, _V], _I),
-    "mpdg_learned_create": ([C.POINTER(_I), _D, C.c_ulonglong, C.POINTER(_V)], _I),
+    "mpdg_learned_create": ([C.POINTER(_I), _I, _I, _D, C.c_ulonglong, C.POINTER(_V)], _I),
+    "mpdg_learned_grid": ([_V, C.POINTER(_I), C.POINTER(_I)], _I),
     "mpdg_learned_destroy": ([_V], _I),
     "mpdg_learned_n_params": ([_V], _I),
     "mpdg_learned_get_weights": ([_V, _V], _I),
@@ -461,21 +462,26 @@ class HeatDiffuseOperation:
 
 
 class LearnedPredictor:
-    """Learned initial guess: 3D CNN on the DoF grid predicting T_{n+1} from (T_{n-1}, T_n, q) and
+    """Learned initial guess: 3D CNN predicting T_{n+1} from (T_{n-1}, T_n, q) and
     z = D^-1 (b - J T_lin), the scaled residual of the linear extrapolation for the step.
 
+    Fields are DoF vectors of Q_degree on n_cells cells; the network works on the DoF grid of
+    Q_degree on n_cells / coarsening cells (full weighting there, FE prolongation back).
     A new predictor type for the slot of include/meltpooldg/linear_algebra/predictor.hpp:118-160;
     training (Adam, FP32 masters) and inference run in the library's BF16 MFMA kernels.
     """
 
-    def __init__(self, dims, qscale=1.0, seed=1234):
+    def __init__(self, n_cells, degree=1, coarsening=1, qscale=1.0, seed=1234):
         self.lib = load_library()
         _torch()
         h = C.c_void_p()
-        d = (C.c_int * 3)(*dims)
-        _check(self.lib.mpdg_learned_create(d, qscale, seed, C.byref(h)), "mpdg_learned_create")
+        nc = (C.c_int * 3)(*n_cells)
+        _check(self.lib.mpdg_learned_create(nc, degree, coarsening, qscale, seed, C.byref(h)),
+               "mpdg_learned_create")
         self.h = h
-        self.dims = tuple(dims)
+        fd, nd = (C.c_int * 3)(), (C.c_int * 3)()
+        _check(self.lib.mpdg_learned_grid(h, fd, nd), "mpdg_learned_grid")
+        self.dof_dims, self.dims = tuple(fd), tuple(nd)
         self.n_params = self.lib.mpdg_learned_n_params(h)
 
     def __del__(self):

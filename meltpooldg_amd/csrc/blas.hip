@@ -377,17 +377,34 @@ template int launch_ell_dir<float>(const float *, float *, const int *, const in
 template int launch_ell_dir<double>(const double *, double *, const int *, const int *, int, int, const int *,
                                     const double *, hipStream_t);
 
-// y = A x for a dense column-major FP64 matrix (coarse-grid solve with the inverse)
-__global__ __launch_bounds__(256) void dense_gemv_kernel(const double *__restrict__ A, const double *__restrict__ x,
-                                                         double *__restrict__ y, int n) {
+// Coarse-grid solve with the stored inverse (dense, column-major, FP64):
+// y_i = sum_j A[j n + i] x_j in two deterministic stages: segment k of the j range
+// (kGemvSeg segments) gives part[k][i], then the segments are summed in order.
+constexpr int kGemvSeg = 32;
+__global__ __launch_bounds__(256) void dense_gemv_part_kernel(const double *__restrict__ A,
+                                                              const double *__restrict__ x,
+                                                              double *__restrict__ part, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int k = blockIdx.y;
+  if (i >= n) return;
+  const int len = (n + kGemvSeg - 1) / kGemvSeg;
+  const int j0 = k * len, j1 = min(n, j0 + len);
+  double acc = 0.0;
+  for (int j = j0; j < j1; ++j) acc += A[(long)j * n + i] * x[j];
+  part[(long)k * n + i] = acc;
+}
+__global__ __launch_bounds__(256) void dense_gemv_sum_kernel(const double *__restrict__ part, double *__restrict__ y,
+                                                             int n) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   double acc = 0.0;
-  for (int j = 0; j < n; ++j) acc += A[(long)j * n + i] * x[j];
+  for (int k = 0; k < kGemvSeg; ++k) acc += part[(long)k * n + i];
   y[i] = acc;
 }
-int launch_dense_gemv(const double *A, const double *x, double *y, int n, hipStream_t s) {
-  hipLaunchKernelGGL(dense_gemv_kernel, dim3((n + 255) / 256), dim3(256), 0, s, A, x, y, n);
+long dense_gemv_scratch(int n) { return (long)kGemvSeg * n; }
+int launch_dense_gemv(const double *A, const double *x, double *y, double *part, int n, hipStream_t s) {
+  hipLaunchKernelGGL(dense_gemv_part_kernel, dim3((n + 255) / 256, kGemvSeg), dim3(256), 0, s, A, x, part, n);
+  hipLaunchKernelGGL(dense_gemv_sum_kernel, dim3((n + 255) / 256), dim3(256), 0, s, part, y, n);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 

@@ -245,13 +245,18 @@ __global__ __launch_bounds__(256) void loss_kernel(const unsigned short *__restr
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// guess = (2 Tn - Tnm1) + sz y[v][0]
-__global__ void predict_kernel(const unsigned short *__restrict__ y, const double *__restrict__ Tnm1,
-                               const double *__restrict__ Tn, long n, const double *__restrict__ stats,
-                               double *__restrict__ out) {
+// correction of the linear extrapolation on the network grid: corr = sz y[v][0]
+__global__ void correction_kernel(const unsigned short *__restrict__ y, long n, const double *__restrict__ stats,
+                                  double *__restrict__ corr) {
   const double sz = stats[3];
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
-    out[i] = (2.0 * Tn[i] - Tnm1[i]) + sz * (double)bf16_to_f32(y[i * 16]);
+    corr[i] = sz * (double)bf16_to_f32(y[i * 16]);
+}
+// guess = (2 Tn - Tnm1) + corr on the DoF grid
+__global__ void extrapolate_add_kernel(const double *__restrict__ Tnm1, const double *__restrict__ Tn,
+                                       const double *__restrict__ corr, long n, double *__restrict__ out) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+    out[i] = (2.0 * Tn[i] - Tnm1[i]) + corr[i];
 }
 
 // statistics of a sample: sums of (Tn - Tnm1)^2, Tn, Tn^2, z^2
@@ -354,10 +359,15 @@ int cnn_loss(const unsigned short *y, const double *Tnm1, const double *Tn, cons
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
-int cnn_predict(const unsigned short *y, const double *Tnm1, const double *Tn, long n, const double *stats,
-                double *out, hipStream_t s) {
-  hipLaunchKernelGGL(predict_kernel, dim3((int)std::min<long>(8192, (n + 255) / 256)), dim3(256), 0, s, y, Tnm1, Tn,
-                     n, stats, out);
+int cnn_correction(const unsigned short *y, long n, const double *stats, double *corr, hipStream_t s) {
+  hipLaunchKernelGGL(correction_kernel, dim3((int)std::min<long>(8192, (n + 255) / 256)), dim3(256), 0, s, y, n, stats,
+                     corr);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+int cnn_extrapolate_add(const double *Tnm1, const double *Tn, const double *corr, long n, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(extrapolate_add_kernel, dim3((int)std::min<long>(8192, (n + 255) / 256)), dim3(256), 0, s, Tnm1,
+                     Tn, corr, n, out);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 

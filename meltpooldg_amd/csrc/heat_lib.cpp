@@ -221,49 +221,6 @@ double tridiag_max_eig(const std::vector<double> &d, const std::vector<double> &
   return mx;
 }
 
-// 1D transfer between two structured levels, from (kc, ncc) to (kf, ncf), ncf in {ncc, 2 ncc}:
-// prolongation rows are fine nodes holding the coarse Lagrange basis at the node;
-// rows of fine nodes on Dirichlet ends and columns of coarse nodes on Dirichlet ends are dropped.
-struct Transfer1DHost {
-  int nf = 0, nc = 0, pw = 0, rw = 0;
-  std::vector<int> pcol, rcol;
-  std::vector<double> pval, rval;
-  Transfer1DHost(int kc, int ncc, int kf, int ncf, bool dir_lo, bool dir_hi) {
-    FE1D fc(kc), ff(kf);
-    nf = kf * ncf + 1;
-    nc = kc * ncc + 1;
-    pw = kc + 1;
-    pcol.assign((size_t)nf * pw, -1);
-    pval.assign((size_t)nf * pw, 0.0);
-    std::vector<std::vector<std::pair<int, double>>> cols(nc);
-    for (int g = 0; g < nf; ++g) {
-      const int cf = std::min(g / kf, ncf - 1), i = g - kf * cf;
-      const int cc = (int)((long)cf * ncc / ncf);
-      const double x = (cf + ff.nodes[i]) * (double)ncc / (double)ncf - cc;
-      double v[kMaxN1];
-      FE1D::lagrange(kc + 1, fc.nodes, x, v, nullptr);
-      const bool drop_row = (dir_lo && g == 0) || (dir_hi && g == nf - 1);
-      for (int j = 0; j < pw; ++j) {
-        const int c = kc * cc + j;
-        const bool drop_col = (dir_lo && c == 0) || (dir_hi && c == nc - 1);
-        if (drop_row || drop_col) continue;
-        pcol[(size_t)g * pw + j] = c;
-        pval[(size_t)g * pw + j] = v[j];
-        cols[c].push_back({g, v[j]});
-      }
-    }
-    rw = 1;
-    for (auto &c : cols) rw = std::max<int>(rw, (int)c.size());
-    rcol.assign((size_t)nc * rw, -1);
-    rval.assign((size_t)nc * rw, 0.0);
-    for (int c = 0; c < nc; ++c)
-      for (size_t e = 0; e < cols[c].size(); ++e) {
-        rcol[(size_t)c * rw + e] = cols[c][e].first;
-        rval[(size_t)c * rw + e] = cols[c][e].second;
-      }
-  }
-};
-
 // 1D evaluation of a fine Q_kf function at the nodes of (kc, ncc): carries the mapping
 // node coordinates from a level to the next coarser one.
 struct Interp1DHost {
@@ -334,7 +291,7 @@ template <typename T>
 struct GMGPrecond : mpdg_precond {
   mpdg_gmg_params prm{};
   std::vector<std::unique_ptr<Level<T>>> lev;  // 0 = coarsest
-  DeviceArray<double> coarse_inv, cvec_in, cvec_out, dscratch, eig_sc, eig_hist, kcell64;
+  DeviceArray<double> coarse_inv, cvec_in, cvec_out, cvec_part, dscratch, eig_sc, eig_hist, kcell64;
   ReductionScratch red;
   int nc_coarse_dofs = 0;
 
@@ -408,6 +365,7 @@ struct GMGPrecond : mpdg_precond {
     MPDG_TRY(coarse_inv.resize((size_t)nc_coarse_dofs * nc_coarse_dofs));
     MPDG_TRY(cvec_in.resize(nc_coarse_dofs));
     MPDG_TRY(cvec_out.resize(nc_coarse_dofs));
+    MPDG_TRY(cvec_part.resize(dense_gemv_scratch((int)nc_coarse_dofs)));
     MPDG_TRY(dscratch.resize(kMaxN1 * kMaxN1 + kMaxN1));
     MPDG_TRY(kcell64.resize(f.n_cells));
     return MPDG_OK;
@@ -735,7 +693,7 @@ struct GMGPrecond : mpdg_precond {
     Level<T> &L = *lev[l];
     if (l == 0) {
       MPDG_TRY(launch_convert<T, double>(L.b.get(), cvec_in.get(), L.od.n_dofs, s));
-      MPDG_TRY(launch_dense_gemv(coarse_inv.get(), cvec_in.get(), cvec_out.get(), (int)L.od.n_dofs, s));
+      MPDG_TRY(launch_dense_gemv(coarse_inv.get(), cvec_in.get(), cvec_out.get(), cvec_part.get(), (int)L.od.n_dofs, s));
       return launch_convert<double, T>(cvec_out.get(), L.x.get(), L.od.n_dofs, s);
     }
     MPDG_TRY(smooth(L, true, s));
@@ -808,7 +766,9 @@ int krylov_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &p
   }
   // The done flag makes every kernel of a finished solve a no-op, so a batch of
   // iterations can be enqueued before the host reads the state back.
-  const int batch = n > 4000000 ? 2 : 8;
+  // large systems: one iteration per host check (an iteration far outweighs the check; a
+  // batch would run whole iterations after convergence); small ones: batches hide launch latency
+  const int batch = n > 2000000 ? 1 : 8;
   int it = 0;
   while (true) {
     for (int k = 0; k < batch; ++k) {
@@ -853,6 +813,7 @@ struct mpdg_heat_operation {
   const double *q_field = nullptr;
   double *z_record = nullptr;        // caller's vector for the residual input of each step
   DeviceArray<double> zbuf, invd;    // residual input and inverse diagonal (learned predictor)
+  double invd_dt = 0.0;              // time step the cached inverse diagonal was computed for
   int fell_back = 0;
   double rel_res0 = 0.0, rel_res = 0.0;
 };
@@ -1128,8 +1089,13 @@ static int guess_feature(mpdg_heat_operation *h, const double *b, const double *
                          hipStream_t s) {
   mpdg_heat_op *op = h->op;
   const long n = op->n_dofs();
-  MPDG_TRY(h->invd.resize(n));
-  MPDG_TRY(op->inverse_diagonal(h->invd.get(), s));
+  // the inverse diagonal is kept while dt is unchanged and the conductivity does not follow T
+  const bool t_dependent = op->d64.kappa_mode == KAPPA_LAW || op->d64.robin_mask != 0;
+  if (h->invd.n != (size_t)n || t_dependent || h->invd_dt != op->dt) {
+    MPDG_TRY(h->invd.resize(n));
+    MPDG_TRY(op->inverse_diagonal(h->invd.get(), s));
+    h->invd_dt = t_dependent ? 0.0 : op->dt;
+  }
   if (b) {
     MPDG_TRY(op->vmult(z, T, s));
     MPDG_TRY(launch_axpby(z, 1.0, b, -1.0, n, s));
@@ -1292,6 +1258,12 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
 
 int mpdg_heat_operation_set_learned(mpdg_heat_operation *h, mpdg_learned *net, const double *q_field) {
   if (!h) return MPDG_ERR_INVALID;
+  if (net) {
+    int fd[3];
+    MPDG_TRY(mpdg_learned_grid(net, fd, nullptr));
+    for (int d = 0; d < 3; ++d)
+      if (fd[d] != h->op->d64.nd[d]) return MPDG_ERR_INVALID;  // network made for another DoF grid
+  }
   h->net = net;
   h->q_field = q_field;
   return MPDG_OK;

@@ -5,10 +5,13 @@
 // HeatDiffuseOperation (see the citations on each class).
 #pragma once
 
+#include <algorithm>
 #include <memory>
+#include <utility>
 #include <vector>
 
 #include "blas.hpp"
+#include "fe1d.hpp"
 
 namespace mpdg {
 
@@ -66,6 +69,49 @@ struct ReductionScratch {
     const int rc__ = (__VA_ARGS__); \
     if (rc__ != MPDG_OK) return rc__; \
   } while (0)
+
+// 1D transfer between (kc, ncc) and (kf, ncf), ncf a multiple of ncc: prolongation P (fine
+// node g <- coarse basis functions at g, ELL width kc+1) and its transpose R (rows = coarse
+// nodes); Dirichlet end points drop their rows and columns.
+struct Transfer1DHost {
+  int nf = 0, nc = 0, pw = 0, rw = 0;
+  std::vector<int> pcol, rcol;
+  std::vector<double> pval, rval;
+  Transfer1DHost(int kc, int ncc, int kf, int ncf, bool dir_lo, bool dir_hi) {
+    FE1D fc(kc), ff(kf);
+    nf = kf * ncf + 1;
+    nc = kc * ncc + 1;
+    pw = kc + 1;
+    pcol.assign((size_t)nf * pw, -1);
+    pval.assign((size_t)nf * pw, 0.0);
+    std::vector<std::vector<std::pair<int, double>>> cols(nc);
+    for (int g = 0; g < nf; ++g) {
+      const int cf = std::min(g / kf, ncf - 1), i = g - kf * cf;
+      const int cc = (int)((long)cf * ncc / ncf);
+      const double x = (cf + ff.nodes[i]) * (double)ncc / (double)ncf - cc;
+      double v[kMaxN1];
+      FE1D::lagrange(kc + 1, fc.nodes, x, v, nullptr);
+      const bool drop_row = (dir_lo && g == 0) || (dir_hi && g == nf - 1);
+      for (int j = 0; j < pw; ++j) {
+        const int c = kc * cc + j;
+        const bool drop_col = (dir_lo && c == 0) || (dir_hi && c == nc - 1);
+        if (drop_row || drop_col) continue;
+        pcol[(size_t)g * pw + j] = c;
+        pval[(size_t)g * pw + j] = v[j];
+        cols[c].push_back({g, v[j]});
+      }
+    }
+    rw = 1;
+    for (auto &c : cols) rw = std::max<int>(rw, (int)c.size());
+    rcol.assign((size_t)nc * rw, -1);
+    rval.assign((size_t)nc * rw, 0.0);
+    for (int c = 0; c < nc; ++c)
+      for (size_t e = 0; e < cols[c].size(); ++e) {
+        rcol[(size_t)c * rw + e] = cols[c][e].first;
+        rval[(size_t)c * rw + e] = cols[c][e].second;
+      }
+  }
+};
 
 }  // namespace mpdg
 

@@ -10,9 +10,18 @@
 // residual of the linear extrapolation T_lin = 2 T_n - T_{n-1} for the step,
 // scaled by the inverse operator diagonal (z = D^-1 (b - J T_lin)), and
 // sz = rms(z). The network output y is the correction of the extrapolation in
-// units of sz: guess = T_lin + sz y. The exact correction is J^-1 (b - J T_lin),
-// a short-range smoothing of z, which the network learns from the solver's own
-// (T_{n-1}, T_n, q_{n+1}, z_n) -> T_{n+1} pairs.
+// units of sz: guess = T_lin + sz y. The exact correction is J^-1 (b - J T_lin);
+// where the operator is mass-dominated (dt k / (rho c_p h^2) of order one or
+// less) that is a short-range smoothing of z, which the network learns from the
+// solver's own (T_{n-1}, T_n, q_{n+1}, z_n) -> T_{n+1} pairs.
+//
+// Working grid: the network runs on the DoF grid of Q_k on (n_cells / rho) cells, rho in
+// {1, 2, 4, 8}. The fields are carried there by full weighting W = diag(P^T 1)^-1 P^T and
+// the correction back by the FE prolongation P (Q_k on n_cells/rho -> Q_k on n_cells), both
+// applied direction by direction. The correction of the extrapolation is smooth on the
+// diffusion length sqrt(k dt / rho c_p); when that spans more nodes than the 9^3 receptive
+// field (stiff steps, dt k / h^2 >> 1) a coarser working grid lets the network see it, at
+// rho^3 less work.
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -45,18 +54,32 @@ struct Rng {
 }  // namespace
 
 struct mpdg_learned {
-  int dims[3] = {0, 0, 0};
-  long n = 0;
+  int fdims[3] = {0, 0, 0};  // DoF grid of the fields passed in
+  int dims[3] = {0, 0, 0};   // network grid
+  int rho = 1, degree = 1;
+  long nf = 0, n = 0;
   double qscale = 1.0;
   int step = 0;
   DeviceArray<float> w[kLayers], b[kLayers], mw[kLayers], vw[kLayers], mb[kLayers], vb[kLayers];
   DeviceArray<unsigned short> W[kLayers], Wt[kLayers], act[kLayers + 1], g0, g1;
   DeviceArray<float> dW[kLayers], db[kLayers], part;
   DeviceArray<double> stats, spart, lpart;
+  // rho > 1: 1D full weighting (coarse <- fine) and prolongation (fine <- coarse) per direction,
+  // the fields on the network grid and the intermediate arrays of the direction passes
+  DeviceArray<int> wcol[3], pcol[3];
+  DeviceArray<double> wval[3], pval[3];
+  int ww[3] = {0, 0, 0}, pw[3] = {0, 0, 0};
+  DeviceArray<double> cf[5], tmpa, tmpb, corr, corr_f;
 
-  int init(const int *d, double qs, unsigned long long seed) {
-    for (int i = 0; i < 3; ++i) dims[i] = d[i];
-    n = (long)d[0] * d[1] * d[2];
+  int init(const int *cells, int k, int coarsening, double qs, unsigned long long seed) {
+    degree = k;
+    rho = coarsening;
+    for (int i = 0; i < 3; ++i) {
+      fdims[i] = k * cells[i] + 1;
+      dims[i] = k * (cells[i] / rho) + 1;
+    }
+    nf = (long)fdims[0] * fdims[1] * fdims[2];
+    n = (long)dims[0] * dims[1] * dims[2];
     qscale = qs;
     Rng rng{seed};
     for (int l = 0; l < kLayers; ++l) {
@@ -88,24 +111,75 @@ struct mpdg_learned {
     MPDG_TRY(stats.resize(4));
     MPDG_TRY(spart.resize(4 * 1024));
     MPDG_TRY(lpart.resize(1024));
+    MPDG_TRY(corr.resize(n));
+    if (rho > 1) {
+      for (int d = 0; d < 3; ++d) {
+        Transfer1DHost t(k, cells[d] / rho, k, cells[d], false, false);
+        // full weighting: rows of P^T scaled to sum 1
+        std::vector<double> rv = t.rval;
+        for (int c = 0; c < t.nc; ++c) {
+          double sum = 0.0;
+          for (int e = 0; e < t.rw; ++e) sum += rv[(size_t)c * t.rw + e];
+          for (int e = 0; e < t.rw; ++e) rv[(size_t)c * t.rw + e] /= sum;
+        }
+        ww[d] = t.rw;
+        pw[d] = t.pw;
+        MPDG_TRY(wcol[d].resize(t.rcol.size()));
+        MPDG_TRY(wval[d].resize(rv.size()));
+        MPDG_TRY(pcol[d].resize(t.pcol.size()));
+        MPDG_TRY(pval[d].resize(t.pval.size()));
+        hipMemcpy(wcol[d].get(), t.rcol.data(), t.rcol.size() * sizeof(int), hipMemcpyHostToDevice);
+        hipMemcpy(wval[d].get(), rv.data(), rv.size() * sizeof(double), hipMemcpyHostToDevice);
+        hipMemcpy(pcol[d].get(), t.pcol.data(), t.pcol.size() * sizeof(int), hipMemcpyHostToDevice);
+        hipMemcpy(pval[d].get(), t.pval.data(), t.pval.size() * sizeof(double), hipMemcpyHostToDevice);
+      }
+      for (auto &a : cf) MPDG_TRY(a.resize(n));
+      const long a1 = std::max((long)dims[0] * fdims[1] * fdims[2], (long)fdims[0] * fdims[1] * dims[2]);
+      const long a2 = std::max((long)dims[0] * dims[1] * fdims[2], (long)fdims[0] * dims[1] * dims[2]);
+      MPDG_TRY(tmpa.resize(a1));
+      MPDG_TRY(tmpb.resize(a2));
+      MPDG_TRY(corr_f.resize(nf));
+    }
     return pack(nullptr);
   }
   int pack(hipStream_t s) {
     for (int l = 0; l < kLayers; ++l) MPDG_TRY(cnn_pack(w[l].get(), kCin[l], kCout[l], W[l].get(), Wt[l].get(), s));
     return MPDG_OK;
   }
+  // a DoF-grid field on the network grid (the field itself when rho = 1, nullptr stays nullptr)
+  const double *to_net(const double *f, int slot, hipStream_t s, int *rc) {
+    if (!f || rho == 1) return f;
+    const int o0[3] = {dims[0], fdims[1], fdims[2]}, o1[3] = {dims[0], dims[1], fdims[2]};
+    *rc = launch_ell_dir<double>(f, tmpa.get(), o0, fdims, 0, ww[0], wcol[0].get(), wval[0].get(), s);
+    if (!*rc) *rc = launch_ell_dir<double>(tmpa.get(), tmpb.get(), o1, o0, 1, ww[1], wcol[1].get(), wval[1].get(), s);
+    if (!*rc) *rc = launch_ell_dir<double>(tmpb.get(), cf[slot].get(), dims, o1, 2, ww[2], wcol[2].get(), wval[2].get(), s);
+    return cf[slot].get();
+  }
   int forward(const double *Tnm1, const double *Tn, const double *q, const double *z, hipStream_t s) {
-    MPDG_TRY(cnn_stats(Tnm1, Tn, z, n, spart.get(), stats.get(), s));
-    MPDG_TRY(cnn_features(Tnm1, Tn, q, z, n, stats.get(), qscale, act[0].get(), s));
+    int rc = MPDG_OK;
+    const double *a = to_net(Tnm1, 0, s, &rc);
+    MPDG_TRY(rc);
+    const double *t = to_net(Tn, 1, s, &rc);
+    MPDG_TRY(rc);
+    const double *qq = to_net(q, 2, s, &rc);
+    MPDG_TRY(rc);
+    const double *zz = to_net(z, 3, s, &rc);
+    MPDG_TRY(rc);
+    MPDG_TRY(cnn_stats(a, t, zz, n, spart.get(), stats.get(), s));
+    MPDG_TRY(cnn_features(a, t, qq, zz, n, stats.get(), qscale, act[0].get(), s));
     for (int l = 0; l < kLayers; ++l)
       MPDG_TRY(cnn_conv(act[l].get(), act[l + 1].get(), W[l].get(), b[l].get(), dims, l == kLayers - 1 ? 0 : 1,
                         nullptr, s));
     return MPDG_OK;
   }
-  // loss and gradients of all layers (dW, db); returns the mean squared error
+  // loss and gradients of all layers (dW, db) after forward(); returns the mean squared error
   int backward(const double *Tnm1, const double *Tn, const double *Tnp1, double *loss, hipStream_t s) {
     const int nb = 512;
-    MPDG_TRY(cnn_loss(act[kLayers].get(), Tnm1, Tn, Tnp1, n, stats.get(), 1.0f, g0.get(), lpart.get(), nb, s));
+    int rc = MPDG_OK;
+    const double *a = rho == 1 ? Tnm1 : cf[0].get(), *t = rho == 1 ? Tn : cf[1].get();
+    const double *tp = to_net(Tnp1, 4, s, &rc);
+    MPDG_TRY(rc);
+    MPDG_TRY(cnn_loss(act[kLayers].get(), a, t, tp, n, stats.get(), 1.0f, g0.get(), lpart.get(), nb, s));
     unsigned short *g = g0.get(), *gn = g1.get();
     for (int l = kLayers - 1; l >= 0; --l) {
       MPDG_TRY(cnn_wgrad(g, act[l].get(), dims, part.get(), kWgradBlocks, dW[l].get(), db[l].get(), s));
@@ -126,15 +200,32 @@ struct mpdg_learned {
     }
     return MPDG_OK;
   }
+  // out = (2 Tn - Tnm1) + P corr after forward()
+  int predict(const double *Tnm1, const double *Tn, double *out, hipStream_t s) {
+    MPDG_TRY(cnn_correction(act[kLayers].get(), n, stats.get(), corr.get(), s));
+    const double *c = corr.get();
+    if (rho > 1) {
+      const int o0[3] = {fdims[0], dims[1], dims[2]}, o1[3] = {fdims[0], fdims[1], dims[2]};
+      MPDG_TRY(launch_ell_dir<double>(corr.get(), tmpb.get(), o0, dims, 0, pw[0], pcol[0].get(), pval[0].get(), s));
+      MPDG_TRY(launch_ell_dir<double>(tmpb.get(), tmpa.get(), o1, o0, 1, pw[1], pcol[1].get(), pval[1].get(), s));
+      MPDG_TRY(launch_ell_dir<double>(tmpa.get(), corr_f.get(), fdims, o1, 2, pw[2], pcol[2].get(), pval[2].get(), s));
+      c = corr_f.get();
+    }
+    return cnn_extrapolate_add(Tnm1, Tn, c, nf, out, s);
+  }
 };
 
 extern "C" {
 
-int mpdg_learned_create(const int *dims, double qscale, unsigned long long seed, mpdg_learned **out) {
-  if (!dims || !out || dims[0] < 1 || dims[1] < 1 || dims[2] < 1) return MPDG_ERR_INVALID;
+int mpdg_learned_create(const int *n_cells, int degree, int coarsening, double qscale, unsigned long long seed,
+                        mpdg_learned **out) {
+  if (!n_cells || !out || degree < 1 || degree > 4) return MPDG_ERR_INVALID;
+  if (coarsening != 1 && coarsening != 2 && coarsening != 4 && coarsening != 8) return MPDG_ERR_INVALID;
+  for (int d = 0; d < 3; ++d)
+    if (n_cells[d] < coarsening || n_cells[d] % coarsening != 0) return MPDG_ERR_INVALID;
   auto *h = new (std::nothrow) mpdg_learned;
   if (!h) return MPDG_ERR_NOMEM;
-  const int rc = h->init(dims, qscale, seed);
+  const int rc = h->init(n_cells, degree, coarsening, qscale, seed);
   if (rc) {
     delete h;
     return rc;
@@ -193,7 +284,7 @@ int mpdg_learned_predict(mpdg_learned *h, const double *Tnm1, const double *Tn, 
                          double *out, void *stream) {
   if (!h || !Tnm1 || !Tn || !out) return MPDG_ERR_INVALID;
   MPDG_TRY(h->forward(Tnm1, Tn, q, z, S(stream)));
-  return cnn_predict(h->act[kLayers].get(), Tnm1, Tn, h->n, h->stats.get(), out, S(stream));
+  return h->predict(Tnm1, Tn, out, S(stream));
 }
 
 // gradients (FP32, same layout as the weights) and the loss of one sample, no update
@@ -233,6 +324,16 @@ int mpdg_learned_train_step(mpdg_learned *h, const double *Tnm1, const double *T
                       1.0f, s));
   MPDG_TRY(h->pack(s));
   if (loss && !std::isfinite(*loss)) return MPDG_ERR_NONFINITE;
+  return MPDG_OK;
+}
+
+// DoF grid of the fields and the network's working grid
+int mpdg_learned_grid(const mpdg_learned *h, int *dof_dims, int *net_dims) {
+  if (!h) return MPDG_ERR_INVALID;
+  for (int d = 0; d < 3; ++d) {
+    if (dof_dims) dof_dims[d] = h->fdims[d];
+    if (net_dims) net_dims[d] = h->dims[d];
+  }
   return MPDG_OK;
 }
 

@@ -4,19 +4,76 @@ numpy restatement of the network the HIP library trains (meltpooldg_amd/csrc/cnn
 learned.cpp): features from (T_{n-1}, T_n, q, z) with z the inverse-diagonal
 scaled residual of the linear extrapolation, four 3x3x3 convolutions with zero
 padding (4 -> 16 -> 16 -> 16 -> 1 channels, ReLU on the first three), output =
-correction of the linear extrapolation in units of rms(z), loss = mean squared
-error, and its exact backward pass. With bf16=True the features, weights, activations
+correction of the linear extrapolation in units of sz = rms(z), loss = mean
+squared error, and its exact backward pass. With bf16=True the features, weights, activations
 and back-propagated gradients are rounded to bfloat16 (round to nearest even)
 at the same places as the GPU kernels, so the two agree up to the FP32
 accumulation order; with bf16=False everything is FP64.
 
 Weights use the library's layout: per layer W[co][ci][tap] then b[co], tap =
 (dx+1) + 3 (dy+1) + 9 (dz+1). Grids are (nz, ny, nx) arrays of the DoF grid.
+
+With a coarsening rho > 1 (`transfers`), the network runs on the DoF grid of Q_k on
+n_cells / rho cells: inputs go there by full weighting W = diag(P^T 1)^-1 P^T, the
+correction comes back by the FE prolongation P (coarse Lagrange basis on Gauss-Lobatto
+nodes evaluated at the fine nodes), both per direction.
 """
 import numpy as np
 
 CIN = (4, 16, 16, 16)
 COUT = (16, 16, 16, 1)
+
+
+def gl_nodes(k):
+    """Gauss-Lobatto nodes of degree k on [0, 1]: the end points and the roots of P_k'."""
+    inner = np.sort(np.polynomial.legendre.Legendre.basis(k).deriv().roots().real) if k > 1 else np.zeros(0)
+    return np.concatenate([[0.0], 0.5 * (1.0 + inner), [1.0]])
+
+
+def lagrange(nodes, x):
+    """Values of the Lagrange basis on `nodes` at x."""
+    v = np.ones(len(nodes))
+    for i in range(len(nodes)):
+        for j in range(len(nodes)):
+            if j != i:
+                v[i] *= (x - nodes[j]) / (nodes[i] - nodes[j])
+    return v
+
+
+def transfer_1d(k, n_cells, rho):
+    """(P, W) along one direction: P (fine nodes x coarse nodes) prolongs Q_k on n_cells/rho
+    cells to Q_k on n_cells cells; W = P^T with rows scaled to sum 1 (full weighting)."""
+    nodes = gl_nodes(k)
+    ncc = n_cells // rho
+    nf, nc = k * n_cells + 1, k * ncc + 1
+    P = np.zeros((nf, nc))
+    for g in range(nf):
+        cf = min(g // k, n_cells - 1)
+        i = g - k * cf
+        cc = cf * ncc // n_cells
+        xi = (cf + nodes[i]) * ncc / n_cells - cc
+        P[g, k * cc:k * cc + k + 1] = lagrange(nodes, xi)
+    W = P.T / P.T.sum(axis=1, keepdims=True)
+    return P, W
+
+
+def transfers(k, n_cells, rho):
+    """Per-direction (P, W) for a (nx, ny, nz)-cell mesh, or None for rho = 1."""
+    if rho == 1:
+        return None
+    return [transfer_1d(k, n, rho) for n in n_cells]
+
+
+def along(f, mats):
+    """Apply (Mx, My, Mz), each (out, in), to a (nz, ny, nx) array."""
+    Mx, My, Mz = mats
+    f = np.einsum("ai,zyi->zya", Mx, f)
+    f = np.einsum("bj,zjx->zbx", My, f)
+    return np.einsum("ck,kyx->cyx", Mz, f)
+
+
+def _net(f, tr):
+    return f if (tr is None or f is None) else along(f, [t[1] for t in tr])
 
 
 def bf16(a):
@@ -81,10 +138,10 @@ def features(Tnm1, Tn, q, z, qscale):
     return c, sz
 
 
-def forward(flat, Tnm1, Tn, q, z, qscale=1.0, bf=True):
+def forward(flat, Tnm1, Tn, q, z, qscale=1.0, bf=True, tr=None):
     rnd = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
     layers = [(rnd(W), np.asarray(b, dtype=np.float32).astype(np.float64)) for W, b in unpack(flat)]
-    x, s = features(Tnm1, Tn, q, z, qscale)
+    x, s = features(_net(Tnm1, tr), _net(Tn, tr), _net(q, tr), _net(z, tr), qscale)
     acts = [rnd(x)]
     for l, (W, b) in enumerate(layers):
         y = conv(acts[-1], W, b)
@@ -94,18 +151,22 @@ def forward(flat, Tnm1, Tn, q, z, qscale=1.0, bf=True):
     return acts, s, layers
 
 
-def predict(flat, Tnm1, Tn, q, z, qscale=1.0, bf=True):
-    """Linear extrapolation plus the output in units of sz."""
-    acts, sz, _ = forward(flat, Tnm1, Tn, q, z, qscale, bf)
-    return (2.0 * Tn - Tnm1) + sz * acts[-1][0]
+def predict(flat, Tnm1, Tn, q, z, qscale=1.0, bf=True, tr=None):
+    """Linear extrapolation plus the output in units of sz (prolonged from the network grid)."""
+    acts, sz, _ = forward(flat, Tnm1, Tn, q, z, qscale, bf, tr)
+    corr = sz * acts[-1][0]
+    if tr is not None:
+        corr = along(corr, [t[0] for t in tr])
+    return (2.0 * Tn - Tnm1) + corr
 
 
-def loss_and_gradients(flat, Tnm1, Tn, q, z, Tnp1, qscale=1.0, bf=True):
+def loss_and_gradients(flat, Tnm1, Tn, q, z, Tnp1, qscale=1.0, bf=True, tr=None):
     rnd = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
-    acts, sz, layers = forward(flat, Tnm1, Tn, q, z, qscale, bf)
-    tgt = (Tnp1 - (2.0 * Tn - Tnm1)) / sz
+    acts, sz, layers = forward(flat, Tnm1, Tn, q, z, qscale, bf, tr)
+    a, t, tp = _net(Tnm1, tr), _net(Tn, tr), _net(Tnp1, tr)
+    tgt = (tp - (2.0 * t - a)) / sz
     e = acts[-1][0] - tgt
-    n = Tn.size
+    n = t.size
     loss = float(np.sum(e * e) / n)
     g = np.zeros_like(acts[-1])
     g[0] = 2.0 * e / n

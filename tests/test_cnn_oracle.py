@@ -100,3 +100,39 @@ def test_features_channels():
     assert np.isclose(np.std(c[2]) * (np.std(Tn) + 1e-12 + 1e-6 * abs(np.mean(Tn))), np.std(Tn), rtol=1e-12)
     c, s2 = N.features(Tnm1, Tn, None, None, 0.5)
     assert np.isclose(s2, s) and not c[1].any() and not c[3].any()
+
+
+def test_prolongation_is_exact_for_coarse_polynomials():
+    """P carries the coarse nodal values of a polynomial of degree <= k to its fine nodal values."""
+    for k in (1, 2, 3):
+        for rho in (2, 4):
+            P, W = N.transfer_1d(k, 8, rho)
+            nodes = N.gl_nodes(k)
+            xf = np.array([(min(g // k, 7) + nodes[g - k * min(g // k, 7)]) / 8 for g in range(8 * k + 1)])
+            ncc = 8 // rho
+            xc = np.array([(min(g // k, ncc - 1) + nodes[g - k * min(g // k, ncc - 1)]) / ncc
+                           for g in range(ncc * k + 1)])
+            for p in (lambda x: 1.0 + 0 * x, lambda x: 2 * x - 1, lambda x: (x - 0.3) ** k):
+                assert np.allclose(P @ p(xc), p(xf), atol=1e-13)
+            assert np.allclose(W @ np.ones(len(xf)), 1.0, atol=1e-14)
+            assert np.allclose(W.sum(axis=1), 1.0)
+
+
+def test_backward_with_coarsening_matches_finite_differences():
+    """Q2 on 4^3 cells, network on 2^3 cells (5^3 nodes)."""
+    Tnm1, Tn, q, z, Tnp1 = sample((9, 9, 9), seed=5)
+    tr = N.transfers(2, (4, 4, 4), 2)
+    w = init(2)
+    loss, g = N.loss_and_gradients(w, Tnm1, Tn, q, z, Tnp1, bf=False, tr=tr)
+    rng = np.random.default_rng(4)
+    h = 1e-6
+    for i in rng.choice(w.size, 25, replace=False):
+        wp, wm = w.copy(), w.copy()
+        wp[i] += h
+        wm[i] -= h
+        fd = (N.loss_and_gradients(wp, Tnm1, Tn, q, z, Tnp1, bf=False, tr=tr)[0] -
+              N.loss_and_gradients(wm, Tnm1, Tn, q, z, Tnp1, bf=False, tr=tr)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g).max())
+    # with a zero network the guess is the linear extrapolation on the fine grid
+    w0 = np.zeros_like(w)
+    assert np.allclose(N.predict(w0, Tnm1, Tn, q, z, bf=False, tr=tr), 2 * Tn - Tnm1, atol=1e-12)

@@ -15,34 +15,41 @@ def to_dev(torch, a):
     return torch.tensor(np.ascontiguousarray(a).ravel(), dtype=torch.float64, device="cuda")
 
 
-@pytest.mark.parametrize("shape", [(5, 6, 7), (4, 9, 21)])
-def test_forward_matches_oracle(cuda, shape):
+def make_net(M, shape, degree=1, rho=1, **kw):
+    """Network for a (nz, ny, nx) DoF grid of Q_degree."""
+    cells = tuple((n - 1) // degree for n in (shape[2], shape[1], shape[0]))
+    return M.LearnedPredictor(cells, degree=degree, coarsening=rho, **kw), N.transfers(degree, cells, rho)
+
+
+@pytest.mark.parametrize("shape,degree,rho", [((5, 6, 7), 1, 1), ((4, 9, 21), 1, 1), ((17, 13, 9), 2, 2),
+                                              ((13, 25, 13), 3, 4)])
+def test_forward_matches_oracle(cuda, shape, degree, rho):
     torch = cuda
     import meltpooldg_amd as M
     Tnm1, Tn, q, z, _ = sample(shape)
-    net = M.LearnedPredictor((shape[2], shape[1], shape[0]), qscale=0.7)
+    net, tr = make_net(M, shape, degree, rho, qscale=0.7)
     w = init().astype(np.float32)
     net.set_weights(w)
     out = torch.empty(Tn.size, dtype=torch.float64, device="cuda")
     net.predict(to_dev(torch, Tnm1), to_dev(torch, Tn), to_dev(torch, q), to_dev(torch, z), out)
     torch.cuda.synchronize()
-    ref = N.predict(w, Tnm1, Tn, q, z, qscale=0.7, bf=True).ravel()
+    ref = N.predict(w, Tnm1, Tn, q, z, qscale=0.7, bf=True, tr=tr).ravel()
     lin = (2 * Tn - Tnm1).ravel()
     # compare the learned correction (prediction minus linear extrapolation)
     got, exp = out.cpu().numpy() - lin, ref - lin
     assert np.linalg.norm(got - exp) <= 2e-2 * np.linalg.norm(exp)
 
 
-def test_gradients_match_oracle(cuda):
+@pytest.mark.parametrize("shape,degree,rho", [((6, 7, 18), 1, 1), ((17, 13, 9), 2, 2)])
+def test_gradients_match_oracle(cuda, shape, degree, rho):
     torch = cuda
     import meltpooldg_amd as M
-    shape = (6, 7, 18)
     Tnm1, Tn, q, z, Tnp1 = sample(shape, seed=2)
-    net = M.LearnedPredictor((shape[2], shape[1], shape[0]), qscale=0.5)
+    net, tr = make_net(M, shape, degree, rho, qscale=0.5)
     w = init(3).astype(np.float32)
     net.set_weights(w)
     loss, g = net.gradients(*[to_dev(torch, a) for a in (Tnm1, Tn, q, z, Tnp1)])
-    ref_loss, ref = N.loss_and_gradients(w, Tnm1, Tn, q, z, Tnp1, qscale=0.5, bf=True)
+    ref_loss, ref = N.loss_and_gradients(w, Tnm1, Tn, q, z, Tnp1, qscale=0.5, bf=True, tr=tr)
     assert abs(loss - ref_loss) <= 2e-2 * ref_loss
     o = 0
     for ci, co in zip(N.CIN, N.COUT):
@@ -58,7 +65,7 @@ def test_training_reduces_loss(cuda):
     import meltpooldg_amd as M
     shape = (8, 8, 16)
     Tnm1, Tn, q, z, Tnp1 = sample(shape, seed=4)
-    net = M.LearnedPredictor((shape[2], shape[1], shape[0]), qscale=0.5, seed=7)
+    net, _ = make_net(M, shape, qscale=0.5, seed=7)
     d = [to_dev(torch, a) for a in (Tnm1, Tn, q, z, Tnp1)]
     first = net.train_step(*d, 1e-3)
     for _ in range(60):
@@ -77,8 +84,7 @@ def test_trained_network_beats_the_predictors_without_it(cuda):
     k, cells, dt = 2, 16, 1e-3
     h = 1.0 / cells
     _, Ts, qs, zs, _ = generate_trajectory(k, cells, 60, dt=dt, precond="gmg32", predictor=M.PREDICTOR_LINEAR)
-    dims = (k * cells + 1,) * 3
-    net = M.LearnedPredictor(dims, qscale=dt / h, seed=11)
+    net = M.LearnedPredictor((cells,) * 3, degree=k, qscale=dt / h, seed=11)
     losses = train(net, Ts[:41], qs[:40], zs[:40], epochs=20, lr=2e-3)
     res = {}
     for name, pred in (("none", M.PREDICTOR_NONE), ("linear", M.PREDICTOR_LINEAR), ("learned", M.PREDICTOR_LEARNED)):
