@@ -344,38 +344,80 @@ int launch_mul(double *x, const double *y, long n, hipStream_t s) {
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
-// 1D ELL matrix applied along direction `dir` of a lexicographic 3D array:
-// out(i with i_dir = row) = sum_e val[row][e] in(i with i_dir = col[row][e])
+// 1D ELL matrix applied along direction DIR of a lexicographic 3D array:
+// out(i with i_DIR = row) (+)= sum_e val[row][e] (in - in2)(i with i_DIR = col[row][e]).
+// in2 (may be null) fuses a residual b - Ax into the first pass of a restriction; add fuses
+// x += P e into the last pass of a prolongation.
+// DIR 0: one workgroup per (y, z) line; the input line is staged in LDS and every thread
+//        forms outputs of the line from it.
+// DIR 1, 2: one thread per x and one workgroup row per (y, z) output; the ELL row is uniform
+//        over the workgroup (scalar loads) and the reads are coalesced along x.
+constexpr int kEllLine = 1024;  // longest x line staged in LDS (DoF grids up to 1024 nodes in x)
 template <typename T>
-__global__ __launch_bounds__(256) void ell_dir_kernel(const T *__restrict__ in, T *__restrict__ out, int o0, int o1,
-                                                      int o2, int i0, int i1, int dir, int width,
-                                                      const int *__restrict__ col, const T *__restrict__ val) {
-  const long n = (long)o0 * o1 * o2;
-  for (long g = (long)blockIdx.x * 256 + threadIdx.x; g < n; g += (long)gridDim.x * 256) {
-    const int x = g % o0, y = (g / o0) % o1, z = g / ((long)o0 * o1);
-    const int row = dir == 0 ? x : (dir == 1 ? y : z);
+__global__ __launch_bounds__(256) void ell_dir0_kernel(const T *__restrict__ in, const T *__restrict__ in2,
+                                                       T *__restrict__ out, int o0, int i0, int width,
+                                                       const int *__restrict__ col, const T *__restrict__ val,
+                                                       int add) {
+  __shared__ T line[kEllLine];
+  const long lin = (long)i0 * (blockIdx.x + (long)gridDim.x * blockIdx.y);
+  const long lout = (long)o0 * (blockIdx.x + (long)gridDim.x * blockIdx.y);
+  for (int x = threadIdx.x; x < i0; x += 256) line[x] = in2 ? in[lin + x] - in2[lin + x] : in[lin + x];
+  __syncthreads();
+  for (int x = threadIdx.x; x < o0; x += 256) {
     T acc = 0;
     for (int e = 0; e < width; ++e) {
-      const int c = col[row * width + e];
-      if (c < 0) continue;
-      const int xx = dir == 0 ? c : x, yy = dir == 1 ? c : y, zz = dir == 2 ? c : z;
-      acc += val[row * width + e] * in[xx + (long)i0 * (yy + (long)i1 * zz)];
+      const int c = col[x * width + e];
+      if (c >= 0) acc += val[x * width + e] * line[c];
     }
-    out[g] = acc;
+    out[lout + x] = add ? out[lout + x] + acc : acc;
   }
+}
+template <typename T, int DIR>
+__global__ __launch_bounds__(256) void ell_dir12_kernel(const T *__restrict__ in, const T *__restrict__ in2,
+                                                        T *__restrict__ out, int o0, int o1, int i0, int i1,
+                                                        int width, const int *__restrict__ col,
+                                                        const T *__restrict__ val, int add) {
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  if (x >= o0) return;
+  const int row = DIR == 1 ? y : z;
+  // input offset of (x, y, z) with the DIR index set to 0, and the stride along DIR
+  const long base = DIR == 1 ? x + (long)i0 * i1 * z : x + (long)i0 * y;
+  const long stride = DIR == 1 ? (long)i0 : (long)i0 * i1;
+  T acc = 0;
+  for (int e = 0; e < width; ++e) {
+    const int c = col[row * width + e];
+    if (c < 0) continue;
+    const long src = base + stride * c;
+    const T v = in2 ? in[src] - in2[src] : in[src];
+    acc += val[row * width + e] * v;
+  }
+  const long g = x + (long)o0 * (y + (long)o1 * z);
+  out[g] = add ? out[g] + acc : acc;
 }
 template <typename T>
 int launch_ell_dir(const T *in, T *out, const int *odims, const int *idims, int dir, int width, const int *col,
-                   const T *val, hipStream_t s) {
-  const long n = (long)odims[0] * odims[1] * odims[2];
-  hipLaunchKernelGGL(ell_dir_kernel<T>, dim3(std::min<long>((n + 255) / 256, 16384)), dim3(256), 0, s, in, out,
-                     odims[0], odims[1], odims[2], idims[0], idims[1], dir, width, col, val);
+                   const T *val, hipStream_t s, const T *in2, bool add) {
+  if (dir == 0) {
+    if (idims[0] > kEllLine) return MPDG_ERR_INVALID;
+    const dim3 grid(odims[1], odims[2]);
+    hipLaunchKernelGGL(ell_dir0_kernel<T>, grid, dim3(256), 0, s, in, in2, out, odims[0], idims[0], width, col, val,
+                       add ? 1 : 0);
+  } else {
+    const dim3 grid((odims[0] + 255) / 256, odims[1], odims[2]);
+    if (dir == 1)
+      hipLaunchKernelGGL((ell_dir12_kernel<T, 1>), grid, dim3(256), 0, s, in, in2, out, odims[0], odims[1], idims[0],
+                         idims[1], width, col, val, add ? 1 : 0);
+    else
+      hipLaunchKernelGGL((ell_dir12_kernel<T, 2>), grid, dim3(256), 0, s, in, in2, out, odims[0], odims[1], idims[0],
+                         idims[1], width, col, val, add ? 1 : 0);
+  }
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 template int launch_ell_dir<float>(const float *, float *, const int *, const int *, int, int, const int *,
-                                   const float *, hipStream_t);
+                                   const float *, hipStream_t, const float *, bool);
 template int launch_ell_dir<double>(const double *, double *, const int *, const int *, int, int, const int *,
-                                    const double *, hipStream_t);
+                                    const double *, hipStream_t, const double *, bool);
 
 // Coarse-grid solve with the stored inverse (dense, column-major, FP64):
 // y_i = sum_j A[j n + i] x_j in two deterministic stages: segment k of the j range

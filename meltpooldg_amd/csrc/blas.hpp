@@ -47,7 +47,7 @@ template <typename T> int launch_add(T *x, const T *y, long n, hipStream_t s);
 int launch_mul(double *x, const double *y, long n, hipStream_t s);
 template <typename T>
 int launch_ell_dir(const T *in, T *out, const int *odims, const int *idims, int dir, int width, const int *col,
-                   const T *val, hipStream_t s);
+                   const T *val, hipStream_t s, const T *in2 = nullptr, bool add = false);
 // y = A^T x for a dense column-major n x n matrix (coarse-grid inverse); part: dense_gemv_scratch(n) doubles
 long dense_gemv_scratch(int n);
 int launch_dense_gemv(const double *A, const double *x, double *y, double *part, int n, hipStream_t s);

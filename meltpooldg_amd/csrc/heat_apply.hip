@@ -221,7 +221,7 @@ __device__ __forceinline__ void load_kappa_qp(const Number *__restrict__ kappa, 
 // wavefronts per SIMD the register allocation targets (FP64 Q3: 2, the rest: the compiler's choice)
 template <int N1, typename Number> struct Occupancy { static constexpr int W = 1; };
 template <> struct Occupancy<4, double> { static constexpr int W = 2; };
-template <> struct Occupancy<4, float> { static constexpr int W = 3; };
+template <> struct Occupancy<4, float> { static constexpr int W = 4; };
 template <> struct Occupancy<5, float> { static constexpr int W = 2; };
 
 template <int N1, typename Number, int KMODE, bool MAPPED>

@@ -654,19 +654,20 @@ struct GMGPrecond : mpdg_precond {
   }
 
   int prolongate_add(Level<T> &F, const Level<T> &C, hipStream_t s) {
-    // C.x (coarse) -> F.Ax (fine) via x, y, z passes, then F.x += F.Ax
+    // F.x += P C.x via x, y, z passes, the last one adding into F.x
     int o[3] = {F.od.nd[0], C.od.nd[1], C.od.nd[2]}, i[3] = {C.od.nd[0], C.od.nd[1], C.od.nd[2]};
     MPDG_TRY(launch_ell_dir<T>(C.x.get(), F.t1.get(), o, i, 0, F.pw[0], F.pcol[0].get(), F.pval[0].get(), s));
     int o2[3] = {F.od.nd[0], F.od.nd[1], C.od.nd[2]};
     MPDG_TRY(launch_ell_dir<T>(F.t1.get(), F.t2.get(), o2, o, 1, F.pw[1], F.pcol[1].get(), F.pval[1].get(), s));
     int o3[3] = {F.od.nd[0], F.od.nd[1], F.od.nd[2]};
-    MPDG_TRY(launch_ell_dir<T>(F.t2.get(), F.Ax.get(), o3, o2, 2, F.pw[2], F.pcol[2].get(), F.pval[2].get(), s));
-    return launch_add<T>(F.x.get(), F.Ax.get(), F.od.n_dofs, s);
+    return launch_ell_dir<T>(F.t2.get(), F.x.get(), o3, o2, 2, F.pw[2], F.pcol[2].get(), F.pval[2].get(), s,
+                             nullptr, true);
   }
   int restrict_residual(const Level<T> &F, Level<T> &C, hipStream_t s) {
-    // F.r (fine) -> C.b (coarse) via z, y, x passes
+    // C.b = R (F.b - F.Ax) via z, y, x passes, the residual formed in the first
     int i3[3] = {F.od.nd[0], F.od.nd[1], F.od.nd[2]}, o2[3] = {F.od.nd[0], F.od.nd[1], C.od.nd[2]};
-    MPDG_TRY(launch_ell_dir<T>(F.r.get(), F.t2.get(), o2, i3, 2, F.rw[2], F.rcol[2].get(), F.rval[2].get(), s));
+    MPDG_TRY(launch_ell_dir<T>(F.b.get(), F.t2.get(), o2, i3, 2, F.rw[2], F.rcol[2].get(), F.rval[2].get(), s,
+                               F.Ax.get()));
     int o1[3] = {F.od.nd[0], C.od.nd[1], C.od.nd[2]};
     MPDG_TRY(launch_ell_dir<T>(F.t2.get(), F.t1.get(), o1, o2, 1, F.rw[1], F.rcol[1].get(), F.rval[1].get(), s));
     int o0[3] = {C.od.nd[0], C.od.nd[1], C.od.nd[2]};
@@ -698,7 +699,6 @@ struct GMGPrecond : mpdg_precond {
     }
     MPDG_TRY(smooth(L, true, s));
     MPDG_TRY(level_apply<T>(L.od, L.x.get(), L.Ax.get(), s));
-    MPDG_TRY(launch_residual<T>(L.r.get(), L.b.get(), L.Ax.get(), L.od.n_dofs, s));
     Level<T> &C = *lev[l - 1];
     MPDG_TRY(restrict_residual(L, C, s));
     MPDG_TRY(vcycle(l - 1, s));
