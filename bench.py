@@ -11,7 +11,13 @@ quadrature point for k) over the kernel time measured with HIP events on the
 launch stream; cpu_baseline = the CPU oracle's apply (same algorithm, OpenMP)
 on a bounded sample. With --solve (default) it also times one heat-step
 solve to ||b - J x|| <= 1e-10 ||b|| with FP64 CG + FP64 GMG and with
-flexible CG + FP32 GMG.
+flexible CG + FP32 GMG. With --scan (default) it times the implicit steps of a
+laser scan on the same mesh (Gaussian flux on the top face moving at speed 1,
+sigma 0.05, peak 1e3, k = 1, dt = 1e-3, T0 = 0): steps 1..N_train run with
+linear extrapolation while the network's training pairs are recorded, the
+network is trained on the GPU, then steps N_train+1..N_train+N_eval are replayed
+from the same state with each solver/predictor pair, each step timed from the
+predictor to ||b - J T|| <= 1e-10 ||b||.
 """
 import argparse
 import json
@@ -23,6 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the apply kernel at Q3 128^3 (per-qp k), from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE) scaled by a device copy of known size; the summary with the commands
+# is profiles/heat_apply_q3_128_fp64_hbm_traffic.md. Counters need their own profiler runs, so
+# the bench reports the profiled figure for that workload and null for others.
+PROFILED_TRAFFIC = {(3, 128): 2.774e9}
 
 
 def parse():
@@ -35,6 +46,11 @@ def parse():
     ap.add_argument("--solve", type=int, default=1)
     ap.add_argument("--cpu", type=int, default=1)
     ap.add_argument("--cpu-cells", type=int, default=64)
+    ap.add_argument("--scan", type=int, default=1)
+    ap.add_argument("--scan-train", type=int, default=30)
+    ap.add_argument("--scan-eval", type=int, default=10)
+    ap.add_argument("--scan-epochs", type=int, default=4)
+    ap.add_argument("--scan-coarsening", type=int, default=4)
     ap.add_argument("--copy", type=int, default=0,
                     help="also run 3 device copies of the src vector (traffic calibration for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -105,6 +121,54 @@ def solve_timings(torch, M, op, reps=2):
     return out
 
 
+def scan_timings(torch, M, k, N, n_train, n_eval, epochs, rho):
+    """Time-to-solve per step of a laser scan (see the module docstring)."""
+    import numpy as np
+    from meltpooldg_amd.application import LaserScan, generate_trajectory, train
+    dt = 1e-3
+    t0 = time.perf_counter()
+    _, Ts, qs, zs, _ = generate_trajectory(k, N, n_train, dt=dt, precond="gmg32", predictor=M.PREDICTOR_LINEAR)
+    torch.cuda.synchronize()
+    t_traj = time.perf_counter() - t0
+    net = M.LearnedPredictor((N, N, N), degree=k, coarsening=rho, qscale=dt * N, seed=11)
+    t0 = time.perf_counter()
+    losses = train(net, Ts, qs, zs, epochs=epochs, lr=2e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del qs, zs
+    out = {"workload": "laser scan, Q%d %d^3 cells, k=1, dt=1e-3, steps %d-%d, ||b-JT|| <= 1e-10 ||b||, coarse Q2 "
+                       "4^3, Chebyshev 3" % (k, N, n_train + 1, n_train + n_eval),
+           "training": {"pairs": n_train - 2, "epochs": epochs, "coarsening": rho, "seconds": round(t_train, 2),
+                        "loss_first_epoch": losses[0], "loss_last_epoch": losses[-1],
+                        "trajectory_seconds": round(t_traj, 2)}}
+    for name, pc, pred in (("fp64_cg_gmg64_prev_step", "gmg64", M.PREDICTOR_NONE),
+                           ("fp64_cg_gmg64_linear", "gmg64", M.PREDICTOR_LINEAR),
+                           ("fcg_gmg32_linear", "gmg32", M.PREDICTOR_LINEAR),
+                           ("fcg_gmg32_learned", "gmg32", M.PREDICTOR_LEARNED)):
+        case = LaserScan(k, N, dt=dt, precond=pc, predictor=pred,
+                         learned=net if pred == M.PREDICTOR_LEARNED else None)
+        case.t = (n_train - 1) * dt
+        case.heat.set_initial_condition(Ts[n_train - 1])
+        case.step()  # rebuilds the history (T_{n-1}, T_n) from the recorded state
+        ms, its, r0s, rs, fbs = [], [], [], [], []
+        for _ in range(n_eval):
+            it, el, (fb, r0, r) = case.step()
+            ms.append(el * 1e3)
+            its.append(it)
+            r0s.append(r0)
+            rs.append(r)
+            fbs.append(fb)
+        out[name] = {"ms_per_step": round(float(np.mean(ms)), 3), "iterations": float(np.mean(its)),
+                     "log10_initial_rel_residual": round(float(np.mean(np.log10(r0s))), 3),
+                     "fallbacks": int(sum(fbs)), "final_rel_residual_max": float(max(rs))}
+        del case
+        torch.cuda.empty_cache()
+    base = out["fp64_cg_gmg64_prev_step"]["ms_per_step"]
+    out["speedup_vs_fp64_prev_step"] = {n: round(base / out[n]["ms_per_step"], 3)
+                                        for n in ("fp64_cg_gmg64_linear", "fcg_gmg32_linear", "fcg_gmg32_learned")}
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -153,7 +217,7 @@ def main():
                                "point = k0(1+0.5 tanh((T-1)/0.1)), T~U[0,2]" % (k, N, op.n_dofs, k + 1),
                    "degree": k, "cells": N, "n_dofs": op.n_dofs, "n_q_points": op.n_q_points},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": PROFILED_TRAFFIC.get((k, N)),
                      "kernel": "heat_apply_kernel<4,double,KAPPA_QP>", "kernel_avg_ms": round(kern_avg, 5),
                      "algorithmic_bytes": bytes_alg},
     }
@@ -167,6 +231,12 @@ def main():
             line["solve"] = solve_timings(torch, M, op)
         except Exception as e:  # report, do not hide the apply line
             line["solve"] = {"error": str(e)}
+    if a.scan and k == 3:
+        try:
+            line["time_to_solve"] = scan_timings(torch, M, k, N, a.scan_train, a.scan_eval, a.scan_epochs,
+                                                 a.scan_coarsening)
+        except Exception as e:  # report, do not hide the apply line
+            line["time_to_solve"] = {"error": str(e)}
     if a.cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         threads = max(1, min(threads, os.cpu_count() or 1))

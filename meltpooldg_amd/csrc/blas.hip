@@ -350,9 +350,10 @@ int launch_mul(double *x, const double *y, long n, hipStream_t s) {
 // x += P e into the last pass of a prolongation.
 // DIR 0: one workgroup per (y, z) line; the input line is staged in LDS and every thread
 //        forms outputs of the line from it.
-// DIR 1, 2: one thread per x and one workgroup row per (y, z) output; the ELL row is uniform
+// DIR 1, 2: one thread per x walking kEllWalk output rows along DIR; the ELL row is uniform
 //        over the workgroup (scalar loads) and the reads are coalesced along x.
 constexpr int kEllLine = 1024;  // longest x line staged in LDS (DoF grids up to 1024 nodes in x)
+constexpr int kEllWalk = 8;     // output rows per thread along y or z
 template <typename T>
 __global__ __launch_bounds__(256) void ell_dir0_kernel(const T *__restrict__ in, const T *__restrict__ in2,
                                                        T *__restrict__ out, int o0, int i0, int width,
@@ -374,26 +375,32 @@ __global__ __launch_bounds__(256) void ell_dir0_kernel(const T *__restrict__ in,
 }
 template <typename T, int DIR>
 __global__ __launch_bounds__(256) void ell_dir12_kernel(const T *__restrict__ in, const T *__restrict__ in2,
-                                                        T *__restrict__ out, int o0, int o1, int i0, int i1,
+                                                        T *__restrict__ out, int o0, int o1, int o2, int i0, int i1,
                                                         int width, const int *__restrict__ col,
                                                         const T *__restrict__ val, int add) {
+  // thread: one x and kEllWalk consecutive output rows along DIR, so that the input rows
+  // shared by neighbouring outputs are re-read by the same wavefront (L1) right away
   const int x = blockIdx.x * 256 + threadIdx.x;
-  const int y = blockIdx.y, z = blockIdx.z;
+  const int w = blockIdx.y;  // the other (non-DIR) index
+  const int r0 = blockIdx.z * kEllWalk;
+  const int nrows = DIR == 1 ? o1 : o2;
   if (x >= o0) return;
-  const int row = DIR == 1 ? y : z;
-  // input offset of (x, y, z) with the DIR index set to 0, and the stride along DIR
-  const long base = DIR == 1 ? x + (long)i0 * i1 * z : x + (long)i0 * y;
+  const long base = DIR == 1 ? x + (long)i0 * i1 * w : x + (long)i0 * w;
   const long stride = DIR == 1 ? (long)i0 : (long)i0 * i1;
-  T acc = 0;
-  for (int e = 0; e < width; ++e) {
-    const int c = col[row * width + e];
-    if (c < 0) continue;
-    const long src = base + stride * c;
-    const T v = in2 ? in[src] - in2[src] : in[src];
-    acc += val[row * width + e] * v;
+  const long obase = DIR == 1 ? x + (long)o0 * o1 * w : x + (long)o0 * w;
+  const long ostride = DIR == 1 ? (long)o0 : (long)o0 * o1;
+  for (int row = r0; row < min(nrows, r0 + kEllWalk); ++row) {
+    T acc = 0;
+    for (int e = 0; e < width; ++e) {
+      const int c = col[row * width + e];
+      if (c < 0) continue;
+      const long src = base + stride * c;
+      const T v = in2 ? in[src] - in2[src] : in[src];
+      acc += val[row * width + e] * v;
+    }
+    const long g = obase + ostride * row;
+    out[g] = add ? out[g] + acc : acc;
   }
-  const long g = x + (long)o0 * (y + (long)o1 * z);
-  out[g] = add ? out[g] + acc : acc;
 }
 template <typename T>
 int launch_ell_dir(const T *in, T *out, const int *odims, const int *idims, int dir, int width, const int *col,
@@ -403,14 +410,14 @@ int launch_ell_dir(const T *in, T *out, const int *odims, const int *idims, int 
     const dim3 grid(odims[1], odims[2]);
     hipLaunchKernelGGL(ell_dir0_kernel<T>, grid, dim3(256), 0, s, in, in2, out, odims[0], idims[0], width, col, val,
                        add ? 1 : 0);
+  } else if (dir == 1) {
+    const dim3 grid((odims[0] + 255) / 256, odims[2], (odims[1] + kEllWalk - 1) / kEllWalk);
+    hipLaunchKernelGGL((ell_dir12_kernel<T, 1>), grid, dim3(256), 0, s, in, in2, out, odims[0], odims[1], odims[2],
+                       idims[0], idims[1], width, col, val, add ? 1 : 0);
   } else {
-    const dim3 grid((odims[0] + 255) / 256, odims[1], odims[2]);
-    if (dir == 1)
-      hipLaunchKernelGGL((ell_dir12_kernel<T, 1>), grid, dim3(256), 0, s, in, in2, out, odims[0], odims[1], idims[0],
-                         idims[1], width, col, val, add ? 1 : 0);
-    else
-      hipLaunchKernelGGL((ell_dir12_kernel<T, 2>), grid, dim3(256), 0, s, in, in2, out, odims[0], odims[1], idims[0],
-                         idims[1], width, col, val, add ? 1 : 0);
+    const dim3 grid((odims[0] + 255) / 256, odims[1], (odims[2] + kEllWalk - 1) / kEllWalk);
+    hipLaunchKernelGGL((ell_dir12_kernel<T, 2>), grid, dim3(256), 0, s, in, in2, out, odims[0], odims[1], odims[2],
+                       idims[0], idims[1], width, col, val, add ? 1 : 0);
   }
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
