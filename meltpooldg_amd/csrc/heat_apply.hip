@@ -171,26 +171,35 @@ __device__ __forceinline__ Number law_kappa(int law, const Number *p, Number T) 
 }
 
 // Gather the z-plane l of a cell (read_dof_values with homogeneous Dirichlet rows zeroed).
+// Lanes without a cell read cell 0's plane (in bounds) and zero it afterwards, so the loads
+// carry no per-element branches.
 template <int N1, typename Number>
 __device__ __forceinline__ void gather_plane(const Number *__restrict__ src, long base, bool valid, int dmask,
                                              const int *nd, int ix0, int iy0, int iz, long sy,
                                              Number (&u)[N1][N1]) {
+  const Number *p = src + (valid ? base : 0);
 #pragma unroll
   for (int j = 0; j < N1; ++j)
 #pragma unroll
-    for (int i = 0; i < N1; ++i) {
-      Number v = 0;
-      if (valid) {
-        v = src[base + i + sy * j];
-        if (dmask) {
-          const int ix = ix0 + i, iy = iy0 + j;
-          if (((dmask & 1) && ix == 0) || ((dmask & 2) && ix == nd[0] - 1) || ((dmask & 4) && iy == 0) ||
-              ((dmask & 8) && iy == nd[1] - 1) || ((dmask & 16) && iz == 0) || ((dmask & 32) && iz == nd[2] - 1))
-            v = 0;
-        }
+    for (int i = 0; i < N1; ++i) u[j][i] = p[i + sy * j];
+  if (!valid) {
+#pragma unroll
+    for (int j = 0; j < N1; ++j)
+#pragma unroll
+      for (int i = 0; i < N1; ++i) u[j][i] = 0;
+  }
+  if (dmask) {
+    const bool zlo = (dmask & 16) && iz == 0, zhi = (dmask & 32) && iz == nd[2] - 1;
+#pragma unroll
+    for (int j = 0; j < N1; ++j)
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        const int ix = ix0 + i, iy = iy0 + j;
+        if (zlo || zhi || ((dmask & 1) && ix == 0) || ((dmask & 2) && ix == nd[0] - 1) ||
+            ((dmask & 4) && iy == 0) || ((dmask & 8) && iy == nd[1] - 1))
+          u[j][i] = 0;
       }
-      u[j][i] = v;
-    }
+  }
 }
 
 // Per-quadrature-point conductivity of a cell at qy = l in the y-plane layout: kq[qz][qx] =
@@ -208,13 +217,19 @@ __device__ __forceinline__ void load_kappa_qp(const Number *__restrict__ kappa, 
 #pragma unroll
       for (int h = 0; h < N1 / 2; ++h) {
         const double2 v = r[h];
-        kq[qz][2 * h] = valid ? v.x : Number(0);
-        kq[qz][2 * h + 1] = valid ? v.y : Number(0);
+        kq[qz][2 * h] = v.x;
+        kq[qz][2 * h + 1] = v.y;
       }
     } else {
 #pragma unroll
-      for (int qx = 0; qx < N1; ++qx) kq[qz][qx] = valid ? p[N1 * N1 * qz + qx] : Number(0);
+      for (int qx = 0; qx < N1; ++qx) kq[qz][qx] = p[N1 * N1 * qz + qx];
     }
+  }
+  if (!valid) {
+#pragma unroll
+    for (int qz = 0; qz < N1; ++qz)
+#pragma unroll
+      for (int qx = 0; qx < N1; ++qx) kq[qz][qx] = 0;
   }
 }
 
