@@ -233,14 +233,17 @@ __device__ __forceinline__ void load_kappa_qp(const Number *__restrict__ kappa, 
   }
 }
 
-// wavefronts per SIMD the register allocation targets (FP64 Q3: 2, the rest: the compiler's choice)
-template <int N1, typename Number> struct Occupancy { static constexpr int W = 1; };
-template <> struct Occupancy<4, double> { static constexpr int W = 2; };
-template <> struct Occupancy<4, float> { static constexpr int W = 4; };
-template <> struct Occupancy<5, float> { static constexpr int W = 2; };
+// wavefronts per SIMD the register allocation targets (1 = the compiler's choice). Mapped
+// cells keep all three flux components and the metric live at the quadrature points and get
+// the full register file rather than spill.
+template <int N1, typename Number, bool MAPPED> struct Occupancy { static constexpr int W = 1; };
+template <> struct Occupancy<4, double, false> { static constexpr int W = 2; };
+template <> struct Occupancy<4, float, false> { static constexpr int W = 4; };
+template <> struct Occupancy<4, float, true> { static constexpr int W = 2; };
+template <> struct Occupancy<5, float, false> { static constexpr int W = 2; };
 
 template <int N1, typename Number, int KMODE, bool MAPPED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Occupancy<N1, Number>::W)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Occupancy<N1, Number, MAPPED>::W)))
 void heat_apply_kernel(const ApplyArgs<Number, N1> P,
                                                          const Number *__restrict__ src,
                                                          Number *__restrict__ dst) {
@@ -329,7 +332,7 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
     //      val = c_m u and flux = kappa grad u at every point, then the z and x parts of the
     //      integration right away: e = c W v + Dz^T(kW gz) + Dx^T(kW gx), in place of v ----
     const Number wy = P.w[l];
-    Number kc = P.kappa;
+    Number kc = valid ? P.kappa : Number(0);  // per lane: keeps the constant case on the cell case's schedule
     if (KMODE == KAPPA_CELL) kc = valid ? P.kappa_arr[cell] : Number(0);
     Number e[N1][N1];
     if (MAPPED) {
@@ -359,59 +362,80 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
       sweep_row_add<N1, true>(P.D, gz, e);
       sweep_col_add<N1, true>(P.D, gx, e);
     } else {
-      // x direction row by row (with the mass term and the y flux), then z column by column,
-      // so that only v, e and gy are live as full arrays
+      // Weights and conductivity folded once into kw (the registers of kq); then the z
+      // direction column by column initialises e, and the x direction row by row adds the
+      // x flux and the mass term and scales the y flux. v dies row by row in the second
+      // pass, and no weight products stay live between the passes (the mass term's weight
+      // is formed in a different order on purpose).
+      // per-point weight (times the per-point conductivity), in the registers of kq; a
+      // cell-wise conductivity goes into the direction factors instead
+      Number kx = P.ks[0], ky = P.ks[1], kz = P.ks[2];
+      if (KMODE == KAPPA_CONST || KMODE == KAPPA_CELL) {
+        kx *= kc;
+        ky *= kc;
+        kz *= kc;
+      }
 #pragma unroll
-      for (int qz = 0; qz < N1; ++qz) {
-        Number f[N1];
+      for (int qz = 0; qz < N1; ++qz)
 #pragma unroll
         for (int qx = 0; qx < N1; ++qx) {
-          const Number W = P.w[qx] * wy * P.w[qz];
-          Number kap = kc;
-          if (KMODE == KAPPA_QP || KMODE == KAPPA_LAW) kap = kq[qz][qx];
-          const Number kw = kap * W;
-          e[qz][qx] = P.cm * W * v[qz][qx];
-          gy[qz][qx] *= kw * P.ks[1];
-          Number g = 0;
-#pragma unroll
-          for (int p = 0; p < N1; ++p) g = fma(P.D(qx, p), v[qz][p], g);
-          f[qx] = kw * P.ks[0] * g;
+          const Number W = (wy * P.w[qz]) * P.w[qx];
+          kq[qz][qx] = (KMODE == KAPPA_QP || KMODE == KAPPA_LAW) ? kq[qz][qx] * W : W;
         }
-#pragma unroll
-        for (int i = 0; i < N1; ++i) {
-          Number acc = e[qz][i];
-#pragma unroll
-          for (int qx = 0; qx < N1; ++qx) acc = fma(P.D(qx, i), f[qx], acc);
-          e[qz][i] = acc;
-        }
-      }
+      auto kw = [&](int qz, int qx) -> Number { return kq[qz][qx]; };
 #pragma unroll
       for (int qx = 0; qx < N1; ++qx) {
         Number f[N1];
 #pragma unroll
         for (int qz = 0; qz < N1; ++qz) {
-          const Number W = P.w[qx] * wy * P.w[qz];
-          Number kap = kc;
-          if (KMODE == KAPPA_QP || KMODE == KAPPA_LAW) kap = kq[qz][qx];
           Number g = 0;
 #pragma unroll
           for (int p = 0; p < N1; ++p) g = fma(P.D(qz, p), v[p][qx], g);
-          f[qz] = kap * W * P.ks[2] * g;
+          f[qz] = kw(qz, qx) * kz * g;
         }
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
-          Number acc = e[i][qx];
+          Number acc = 0;
 #pragma unroll
           for (int qz = 0; qz < N1; ++qz) acc = fma(P.D(qz, i), f[qz], acc);
           e[i][qx] = acc;
         }
       }
+      // y flux: scale, then take it back to the plane layout right away (ey = Sz^T (kW gy))
+#pragma unroll
+      for (int qz = 0; qz < N1; ++qz)
+#pragma unroll
+        for (int qx = 0; qx < N1; ++qx) gy[qz][qx] *= kw(qz, qx) * ky;
+      sweep_row<N1, true>(P.S, gy, u);
+      transpose_yz<N1>(slot, l, lane_on, u, b);
+      const Number cmwy = P.cm * wy;
+#pragma unroll
+      for (int qz = 0; qz < N1; ++qz) {
+        Number f[N1];
+#pragma unroll
+        for (int qx = 0; qx < N1; ++qx) {
+          Number g = 0;
+#pragma unroll
+          for (int p = 0; p < N1; ++p) g = fma(P.D(qx, p), v[qz][p], g);
+          f[qx] = kw(qz, qx) * kx * g;
+        }
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+          Number acc = fma((cmwy * P.w[i]) * P.w[qz], v[qz][i], e[qz][i]);
+#pragma unroll
+          for (int qx = 0; qx < N1; ++qx) acc = fma(P.D(qx, i), f[qx], acc);
+          e[qz][i] = acc;
+        }
+      }
     }
-    // ---- back to the plane layout: e' = Sz^T e, ey = Sz^T (kW gy) ----
+    // ---- back to the plane layout: e' = Sz^T e, ey = Sz^T (kW gy) (the latter already in b
+    //      for affine cells) ----
     sweep_row<N1, true>(P.S, e, u);
     transpose_yz<N1>(slot, l, lane_on, u, t);
-    sweep_row<N1, true>(P.S, gy, u);
-    transpose_yz<N1>(slot, l, lane_on, u, b);
+    if (MAPPED) {
+      sweep_row<N1, true>(P.S, gy, u);
+      transpose_yz<N1>(slot, l, lane_on, u, b);
+    }
     // ---- plane layout: out = Sx^T Sy^T (e' + Dy^T ey) ----
     sweep_row_add<N1, true>(P.D, b, t);
     sweep_row<N1, true>(P.S, t, b);
