@@ -137,7 +137,8 @@ def scan_timings(torch, M, k, N, n_train, n_eval, epochs, rho):
     t_train = time.perf_counter() - t0
     del qs, zs
     out = {"workload": "laser scan, Q%d %d^3 cells, k=1, dt=1e-3, steps %d-%d, ||b-JT|| <= 1e-10 ||b||, coarse Q2 "
-                       "4^3, Chebyshev 3" % (k, N, n_train + 1, n_train + n_eval),
+                       "4^3; FP64 GMG: Chebyshev 3 pre / 3 post (CG), FP32 GMG: Chebyshev 1 pre / 3 post "
+                       "(flexible CG)" % (k, N, n_train + 1, n_train + n_eval),
            "training": {"pairs": n_train - 2, "epochs": epochs, "coarsening": rho, "seconds": round(t_train, 2),
                         "loss_first_epoch": losses[0], "loss_last_epoch": losses[-1],
                         "trajectory_seconds": round(t_traj, 2)}}
@@ -146,7 +147,8 @@ def scan_timings(torch, M, k, N, n_train, n_eval, epochs, rho):
                            ("fcg_gmg32_linear", "gmg32", M.PREDICTOR_LINEAR),
                            ("fcg_gmg32_learned", "gmg32", M.PREDICTOR_LEARNED)):
         case = LaserScan(k, N, dt=dt, precond=pc, predictor=pred,
-                         learned=net if pred == M.PREDICTOR_LEARNED else None)
+                         learned=net if pred == M.PREDICTOR_LEARNED else None,
+                         pre_smoother_degree=1 if pc == "gmg32" else 0)
         case.t = (n_train - 1) * dt
         case.heat.set_initial_condition(Ts[n_train - 1])
         case.step()  # rebuilds the history (T_{n-1}, T_n) from the recorded state

@@ -119,6 +119,8 @@ typedef struct mpdg_gmg_params {
   int smoother_degree;    /* Chebyshev degree */
   double smoothing_range; /* Chebyshev smoothing range */
   int eig_cg_n_iterations;/* CG steps of the eigenvalue estimate */
+  int pre_smoother_degree;/* Chebyshev degree before the coarse correction (0: smoother_degree);
+                             the first step from a zero guess needs no operator apply */
 } mpdg_gmg_params;
 
 int mpdg_precond_create(mpdg_heat_op *op, int type, const mpdg_gmg_params *gmg, mpdg_precond **p);

@@ -18,7 +18,7 @@ import meltpooldg_amd as M
 class LaserScan:
     def __init__(self, degree, cells, dt=1e-3, rho_cp=1.0, k0=1.0, law=None, peak=1e3, sigma=0.05, speed=1.0,
                  x0=0.3, y0=0.5, precond="gmg32", predictor=M.PREDICTOR_NONE, learned=None, rel_tol=1e-10,
-                 n_cells_coarse=4, coarse_degree=2, guess_check=True, smoother_degree=3):
+                 n_cells_coarse=4, coarse_degree=2, guess_check=True, smoother_degree=3, pre_smoother_degree=0):
         import torch
         self.torch = torch
         self.dt, self.peak, self.sigma, self.speed, self.x0, self.y0 = dt, peak, sigma, speed, x0, y0
@@ -33,11 +33,13 @@ class LaserScan:
             self.op.set_conductivity_constant(k0)
         if precond == "gmg32":
             self.pc = M.GMGPreconditioner(self.op, precision=M.FP32, n_cells_coarse=n_cells_coarse,
-                                          coarse_degree=coarse_degree, smoother_degree=smoother_degree)
+                                          coarse_degree=coarse_degree, smoother_degree=smoother_degree,
+                                          pre_smoother_degree=pre_smoother_degree)
             solver = M.SOLVER_FCG
         elif precond == "gmg64":
             self.pc = M.GMGPreconditioner(self.op, precision=M.FP64, n_cells_coarse=n_cells_coarse,
-                                          coarse_degree=coarse_degree, smoother_degree=smoother_degree)
+                                          coarse_degree=coarse_degree, smoother_degree=smoother_degree,
+                                          pre_smoother_degree=pre_smoother_degree)
             solver = M.SOLVER_CG
         else:
             self.pc = M.JacobiPreconditioner(self.op)

@@ -676,7 +676,8 @@ struct GMGPrecond : mpdg_precond {
 
   int smooth(Level<T> &L, bool x_zero, hipStream_t s) {
     const long n = L.od.n_dofs;
-    for (int k = 0; k < prm.smoother_degree; ++k) {
+    const int degree = (x_zero && prm.pre_smoother_degree > 0) ? prm.pre_smoother_degree : prm.smoother_degree;
+    for (int k = 0; k < degree; ++k) {
       const bool first = (k == 0);
       if (first && x_zero) {
         MPDG_TRY(launch_chebyshev_step<T>(L.x.get(), L.d.get(), L.b.get(), L.Ax.get(), L.inv.get(), n, 0.0,
@@ -983,7 +984,8 @@ int mpdg_precond_create(mpdg_heat_op *op, int type, const mpdg_gmg_params *gmg, 
   if (type == MPDG_PRECOND_IDENTITY) p = new (std::nothrow) IdentityPrecond;
   else if (type == MPDG_PRECOND_DIAGONAL) p = new (std::nothrow) DiagonalPrecond;
   else if (type == MPDG_PRECOND_GMG) {
-    if (!gmg || gmg->smoother_degree < 1 || gmg->smoothing_range <= 1.0 || gmg->eig_cg_n_iterations < 1)
+    if (!gmg || gmg->smoother_degree < 1 || gmg->smoothing_range <= 1.0 || gmg->eig_cg_n_iterations < 1 ||
+        gmg->pre_smoother_degree < 0 || gmg->pre_smoother_degree > gmg->smoother_degree)
       return MPDG_ERR_INVALID;
     if (gmg->precision == MPDG_FP32) {
       auto *g = new (std::nothrow) GMGPrecond<float>;
