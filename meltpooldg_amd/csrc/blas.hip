@@ -348,56 +348,117 @@ int launch_mul(double *x, const double *y, long n, hipStream_t s) {
 // out(i with i_DIR = row) (+)= sum_e val[row][e] (in - in2)(i with i_DIR = col[row][e]).
 // in2 (may be null) fuses a residual b - Ax into the first pass of a restriction; add fuses
 // x += P e into the last pass of a prolongation.
-// DIR 0: one workgroup per (y, z) line; the input line is staged in LDS and every thread
-//        forms outputs of the line from it.
-// DIR 1, 2: one thread per x walking kEllWalk output rows along DIR; the ELL row is uniform
-//        over the workgroup (scalar loads) and the reads are coalesced along x.
+// DIR 0: one wavefront per (y, z) line; the input line is staged in LDS and the wavefront
+//        forms the outputs of the line from it.
+// DIR 1, 2: each thread walks a few output rows along DIR; the ELL rows are uniform over the
+//        workgroup (scalar loads). The input rows those outputs use (a contiguous range that
+//        fits the LDS stage for the multigrid transfers) are read once into a per-thread LDS
+//        column and the sums are formed from there, so every input element is fetched about
+//        once per workgroup instead of once per ELL entry that uses it.
 constexpr int kEllLine = 1024;  // longest x line staged in LDS (DoF grids up to 1024 nodes in x)
-constexpr int kEllWalk = 8;     // output rows per thread along y or z
-template <typename T>
+template <typename T> struct EllStage { static constexpr int rows = 32; };  // 32 KB of LDS either way
+template <> struct EllStage<double> { static constexpr int rows = 16; };
+template <typename T, int MAXW>
 __global__ __launch_bounds__(256) void ell_dir0_kernel(const T *__restrict__ in, const T *__restrict__ in2,
-                                                       T *__restrict__ out, int o0, int i0, int width,
+                                                       T *__restrict__ out, int o0, int i0, long nlines, int width,
                                                        const int *__restrict__ col, const T *__restrict__ val,
                                                        int add) {
-  __shared__ T line[kEllLine];
-  const long lin = (long)i0 * (blockIdx.x + (long)gridDim.x * blockIdx.y);
-  const long lout = (long)o0 * (blockIdx.x + (long)gridDim.x * blockIdx.y);
-  for (int x = threadIdx.x; x < i0; x += 256) line[x] = in2 ? in[lin + x] - in2[lin + x] : in[lin + x];
-  __syncthreads();
-  for (int x = threadIdx.x; x < o0; x += 256) {
+  __shared__ T lines[4][kEllLine];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long line_id = (long)blockIdx.x * 4 + wave;
+  if (line_id >= nlines) return;  // whole wavefront; only wavefront-local LDS use below
+  T *line = lines[wave];
+  const long lin = (long)i0 * line_id, lout = (long)o0 * line_id;
+  for (int x = lane; x < i0; x += 64) line[x] = in2 ? in[lin + x] - in2[lin + x] : in[lin + x];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int x = lane; x < o0; x += 64) {
     T acc = 0;
-    for (int e = 0; e < width; ++e) {
-      const int c = col[x * width + e];
-      if (c >= 0) acc += val[x * width + e] * line[c];
+#pragma unroll
+    for (int e = 0; e < MAXW; ++e) {
+      if (e < width) {
+        const int c = col[x * width + e];
+        if (c >= 0) acc += val[x * width + e] * line[c];
+      }
     }
     out[lout + x] = add ? out[lout + x] + acc : acc;
   }
 }
-template <typename T, int DIR>
+// DIR 1: thread index = x, blockIdx.y = z; DIR 2: thread index = position in the x-y plane
+// (consecutive threads read consecutive addresses of a plane). Each thread walks `walk`
+// output rows along DIR.
+template <typename T, int DIR, int MAXW>
 __global__ __launch_bounds__(256) void ell_dir12_kernel(const T *__restrict__ in, const T *__restrict__ in2,
                                                         T *__restrict__ out, int o0, int o1, int o2, int i0, int i1,
                                                         int width, const int *__restrict__ col,
-                                                        const T *__restrict__ val, int add) {
-  // thread: one x and kEllWalk consecutive output rows along DIR, so that the input rows
-  // shared by neighbouring outputs are re-read by the same wavefront (L1) right away
-  const int x = blockIdx.x * 256 + threadIdx.x;
-  const int w = blockIdx.y;  // the other (non-DIR) index
-  const int r0 = blockIdx.z * kEllWalk;
+                                                        const T *__restrict__ val, int add, int walk) {
+  constexpr int SR = EllStage<T>::rows;
+  __shared__ T stage[SR][256];
+  const int tx = threadIdx.x;
+  const long t = (long)blockIdx.x * 256 + tx;
+  const int r0 = blockIdx.z * walk;
   const int nrows = DIR == 1 ? o1 : o2;
-  if (x >= o0) return;
-  const long base = DIR == 1 ? x + (long)i0 * i1 * w : x + (long)i0 * w;
-  const long stride = DIR == 1 ? (long)i0 : (long)i0 * i1;
-  const long obase = DIR == 1 ? x + (long)o0 * o1 * w : x + (long)o0 * w;
-  const long ostride = DIR == 1 ? (long)o0 : (long)o0 * o1;
-  for (int row = r0; row < min(nrows, r0 + kEllWalk); ++row) {
-    T acc = 0;
+  const int r1 = min(nrows, r0 + walk);
+  // input rows used by the output rows r0 .. r1-1 (uniform over the workgroup)
+  int cmin = 0x7fffffff, cmax = -1;
+  for (int row = r0; row < r1; ++row)
     for (int e = 0; e < width; ++e) {
       const int c = col[row * width + e];
-      if (c < 0) continue;
-      const long src = base + stride * c;
-      const T v = in2 ? in[src] - in2[src] : in[src];
-      acc += val[row * width + e] * v;
+      if (c >= 0) {
+        cmin = min(cmin, c);
+        cmax = max(cmax, c);
+      }
     }
+  long base, stride, obase, ostride;
+  if (DIR == 1) {
+    if (t >= o0) return;  // no workgroup barrier below: each thread uses only its own LDS column
+    const long z = blockIdx.y;
+    base = t + (long)i0 * i1 * z;
+    stride = i0;
+    obase = t + (long)o0 * o1 * z;
+    ostride = o0;
+  } else {
+    const long plane = (long)o0 * o1;
+    if (t >= plane) return;
+    base = t;
+    stride = (long)i0 * i1;
+    obase = t;
+    ostride = plane;
+  }
+  const bool staged = cmax >= 0 && cmax - cmin < SR;
+  if (staged) {
+    for (int j = 0; j <= cmax - cmin; ++j) {
+      const long src = base + stride * (cmin + j);
+      stage[j][tx] = in2 ? in[src] - in2[src] : in[src];
+    }
+  }
+  for (int row = r0; row < r1; ++row) {
+    // the row's ELL entries first (uniform), then all reads, then the sum
+    int cs[MAXW];
+    T vs[MAXW], xs[MAXW];
+#pragma unroll
+    for (int e = 0; e < MAXW; ++e) {
+      cs[e] = e < width ? col[row * width + e] : -1;
+      vs[e] = e < width ? val[row * width + e] : T(0);
+    }
+#pragma unroll
+    for (int e = 0; e < MAXW; ++e) {
+      const int c = cs[e];
+      T v = 0;
+      if (c >= 0) {
+        if (staged) {
+          v = stage[c - cmin][tx];
+        } else {
+          const long src = base + stride * c;
+          v = in2 ? in[src] - in2[src] : in[src];
+        }
+      }
+      xs[e] = v;
+    }
+    T acc = 0;
+#pragma unroll
+    for (int e = 0; e < MAXW; ++e) acc += vs[e] * xs[e];
     const long g = obase + ostride * row;
     out[g] = add ? out[g] + acc : acc;
   }
@@ -405,22 +466,246 @@ __global__ __launch_bounds__(256) void ell_dir12_kernel(const T *__restrict__ in
 template <typename T>
 int launch_ell_dir(const T *in, T *out, const int *odims, const int *idims, int dir, int width, const int *col,
                    const T *val, hipStream_t s, const T *in2, bool add) {
+  if (width > 32) return MPDG_ERR_INVALID;
   if (dir == 0) {
     if (idims[0] > kEllLine) return MPDG_ERR_INVALID;
-    const dim3 grid(odims[1], odims[2]);
-    hipLaunchKernelGGL(ell_dir0_kernel<T>, grid, dim3(256), 0, s, in, in2, out, odims[0], idims[0], width, col, val,
-                       add ? 1 : 0);
-  } else if (dir == 1) {
-    const dim3 grid((odims[0] + 255) / 256, odims[2], (odims[1] + kEllWalk - 1) / kEllWalk);
-    hipLaunchKernelGGL((ell_dir12_kernel<T, 1>), grid, dim3(256), 0, s, in, in2, out, odims[0], odims[1], odims[2],
-                       idims[0], idims[1], width, col, val, add ? 1 : 0);
-  } else {
-    const dim3 grid((odims[0] + 255) / 256, odims[1], (odims[2] + kEllWalk - 1) / kEllWalk);
-    hipLaunchKernelGGL((ell_dir12_kernel<T, 2>), grid, dim3(256), 0, s, in, in2, out, odims[0], odims[1], odims[2],
-                       idims[0], idims[1], width, col, val, add ? 1 : 0);
+    const long nlines = (long)odims[1] * odims[2];
+    const dim3 grid((unsigned)((nlines + 3) / 4));
+#define MPDG_ELL0(W)                                                                                             \
+  hipLaunchKernelGGL((ell_dir0_kernel<T, W>), grid, dim3(256), 0, s, in, in2, out, odims[0], idims[0], nlines,  \
+                     width, col, val, add ? 1 : 0)
+    if (width <= 4) MPDG_ELL0(4);
+    else if (width <= 8) MPDG_ELL0(8);
+    else if (width <= 16) MPDG_ELL0(16);
+    else MPDG_ELL0(32);
+#undef MPDG_ELL0
+    return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
   }
+  // output rows per thread: as many as keep the input rows they use within the LDS stage
+  // (a row of a 2:1 restriction spans about 2 walk + width input rows, of a prolongation
+  // about walk / 2 + width)
+  int walk = 8;
+  while (walk > 1 && 2 * walk + width > EllStage<T>::rows) walk /= 2;
+  dim3 grid;
+  if (dir == 1) grid = dim3((odims[0] + 255) / 256, odims[2], (odims[1] + walk - 1) / walk);
+  else grid = dim3((unsigned)(((long)odims[0] * odims[1] + 255) / 256), 1, (odims[2] + walk - 1) / walk);
+#define MPDG_ELL12(D, W)                                                                                        \
+  hipLaunchKernelGGL((ell_dir12_kernel<T, D, W>), grid, dim3(256), 0, s, in, in2, out, odims[0], odims[1],      \
+                     odims[2], idims[0], idims[1], width, col, val, add ? 1 : 0, walk)
+  if (dir == 1) {
+    if (width <= 4) MPDG_ELL12(1, 4);
+    else if (width <= 8) MPDG_ELL12(1, 8);
+    else if (width <= 16) MPDG_ELL12(1, 16);
+    else MPDG_ELL12(1, 32);
+  } else {
+    if (width <= 4) MPDG_ELL12(2, 4);
+    else if (width <= 8) MPDG_ELL12(2, 8);
+    else if (width <= 16) MPDG_ELL12(2, 16);
+    else MPDG_ELL12(2, 32);
+  }
+#undef MPDG_ELL12
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
+// ---------------------------------------------------------------------------
+// Structured 2:1 h-transfers of Q_K along one direction (the multigrid's h levels).
+// P of one coarse cell, tab[m][j] = coarse basis j at fine node m of the cell (m = 0..2K);
+// every coarse cell has the same table, so the kernels need no index arrays: prolongation
+// reads the K+1 coarse values of a cell once and forms its 2K fine values, restriction
+// (P^T) reads the 2K fine values of a cell once and forms its K coarse values, carrying the
+// shared vertex's partial sum to the next cell. lo / hi: homogeneous Dirichlet end (the end
+// value is dropped on input and output, as the ELL form drops its rows and columns).
+// DIR 1, 2: one thread per line along DIR, kHSeg coarse cells per thread (restriction
+// recomputes the carry of the cell before its segment). DIR 0: one wavefront per x line,
+// the line staged in LDS.
+// ---------------------------------------------------------------------------
+template <typename T, int K> struct HTab { T v[2 * K + 1][K + 1]; };
+constexpr int kHSeg = 4;
+
+// element i of line (t, by): base + i * stride, base = t + by * bstride
+template <typename T, int K>
+__global__ __launch_bounds__(256) void hprolong_kernel(const T *__restrict__ in, T *__restrict__ out, long nl,
+                                                       long ibstride, long obstride, long istride, long ostride,
+                                                       int ncc, HTab<T, K> P, int lo, int hi, int add) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= nl) return;
+  const long ib = t + ibstride * blockIdx.y, ob = t + obstride * blockIdx.y;
+  const int nfine = 2 * K * ncc + 1, ncoarse = K * ncc + 1;
+  const int c0 = blockIdx.z * kHSeg, c1 = min(ncc, c0 + kHSeg);
+  for (int cc = c0; cc < c1; ++cc) {
+    T c[K + 1];
+#pragma unroll
+    for (int j = 0; j <= K; ++j) {
+      const int n = K * cc + j;
+      c[j] = ((lo && n == 0) || (hi && n == ncoarse - 1)) ? T(0) : in[ib + istride * n];
+    }
+#pragma unroll
+    for (int m = 0; m < 2 * K; ++m) {
+      const int g = 2 * K * cc + m;
+      T a = 0;
+#pragma unroll
+      for (int j = 0; j <= K; ++j) a += P.v[m][j] * c[j];
+      if (lo && g == 0) a = 0;
+      const long o = ob + ostride * g;
+      out[o] = add ? out[o] + a : a;
+    }
+    if (cc == ncc - 1) {
+      const long o = ob + ostride * (nfine - 1);
+      const T a = hi ? T(0) : c[K];
+      out[o] = add ? out[o] + a : a;
+    }
+  }
+}
+template <typename T, int K>
+__global__ __launch_bounds__(256) void hrestrict_kernel(const T *__restrict__ in, const T *__restrict__ in2,
+                                                        T *__restrict__ out, long nl, long ibstride, long obstride,
+                                                        long istride, long ostride, int ncc, HTab<T, K> P, int lo,
+                                                        int hi) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= nl) return;
+  const long ib = t + ibstride * blockIdx.y, ob = t + obstride * blockIdx.y;
+  const int nfine = 2 * K * ncc + 1, ncoarse = K * ncc + 1;
+  const int c0 = blockIdx.z * kHSeg, c1 = min(ncc, c0 + kHSeg);
+  auto F = [&](int g) -> T {
+    if ((lo && g == 0) || (hi && g == nfine - 1)) return T(0);
+    const long a = ib + istride * g;
+    return in2 ? in[a] - in2[a] : in[a];
+  };
+  T carry = 0;
+  if (c0 > 0) {
+#pragma unroll
+    for (int m = 1; m < 2 * K; ++m) carry += P.v[m][K] * F(2 * K * (c0 - 1) + m);
+  }
+  for (int cc = c0; cc < c1; ++cc) {
+    T f[2 * K];
+#pragma unroll
+    for (int m = 0; m < 2 * K; ++m) f[m] = F(2 * K * cc + m);
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const int n = K * cc + j;
+      T a = j == 0 ? carry : T(0);
+#pragma unroll
+      for (int m = 0; m < 2 * K; ++m) a += P.v[m][j] * f[m];
+      out[ob + ostride * n] = (lo && n == 0) ? T(0) : a;
+    }
+    carry = 0;
+#pragma unroll
+    for (int m = 1; m < 2 * K; ++m) carry += P.v[m][K] * f[m];
+    if (cc == ncc - 1) out[ob + ostride * (ncoarse - 1)] = hi ? T(0) : carry + F(nfine - 1);
+  }
+}
+// x lines: one wavefront per line, the input line in LDS, the table read from global memory
+template <typename T, int K, bool RESTRICT>
+__global__ __launch_bounds__(256) void htransfer_x_kernel(const T *__restrict__ in, const T *__restrict__ in2,
+                                                          T *__restrict__ out, long nlines, int ncc,
+                                                          const T *__restrict__ tab, int lo, int hi, int add) {
+  __shared__ T lines[4][kEllLine];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long line_id = (long)blockIdx.x * 4 + wave;
+  if (line_id >= nlines) return;  // whole wavefront; only wavefront-local LDS use below
+  const int nfine = 2 * K * ncc + 1, ncoarse = K * ncc + 1;
+  const int ni = RESTRICT ? nfine : ncoarse, no = RESTRICT ? ncoarse : nfine;
+  T *line = lines[wave];
+  const long lin = (long)ni * line_id, lout = (long)no * line_id;
+  for (int x = lane; x < ni; x += 64) {
+    const bool drop = (lo && x == 0) || (hi && x == ni - 1);
+    line[x] = drop ? T(0) : (in2 ? in[lin + x] - in2[lin + x] : in[lin + x]);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  constexpr int W = K + 1;
+  for (int x = lane; x < no; x += 64) {
+    T a = 0;
+    if (!RESTRICT) {
+      // fine node x: cell cc, row m (the last node is the last cell's right vertex)
+      const int cc = min(x / (2 * K), ncc - 1), m = x - 2 * K * cc;
+#pragma unroll
+      for (int j = 0; j <= K; ++j) a += tab[m * W + j] * line[K * cc + j];
+      if ((lo && x == 0) || (hi && x == no - 1)) a = 0;
+      out[lout + x] = add ? out[lout + x] + a : a;
+    } else {
+      // coarse node x = K cc + j: this cell's rows 0..2K-1, the previous cell's for a vertex,
+      // and the last fine node for the last vertex
+      const int cc = min(x / K, ncc - 1), j = x - K * cc;
+      if (j < K) {
+#pragma unroll
+        for (int m = 0; m < 2 * K; ++m) a += tab[m * W + j] * line[2 * K * cc + m];
+      }
+      const int cp = j == K ? cc : cc - 1;  // cell whose right vertex x is
+      if ((j == 0 && cc > 0) || j == K) {
+#pragma unroll
+        for (int m = 1; m < 2 * K; ++m) a += tab[m * W + K] * line[2 * K * cp + m];
+      }
+      if (j == K) a += line[nfine - 1];
+      if ((lo && x == 0) || (hi && x == no - 1)) a = 0;
+      out[lout + x] = a;
+    }
+  }
+}
+template <typename T, int K>
+static int htransfer_k(bool restrict_mode, int dir, const T *in, const T *in2, T *out, const int *idims,
+                       const int *odims, const double *tab_host, const T *tab_dev, int lo, int hi, bool add,
+                       hipStream_t s) {
+  HTab<T, K> P;
+  for (int m = 0; m <= 2 * K; ++m)
+    for (int j = 0; j <= K; ++j) P.v[m][j] = (T)tab_host[m * (K + 1) + j];
+  const int ncoarse = restrict_mode ? odims[dir] : idims[dir];
+  if ((ncoarse - 1) % K != 0) return MPDG_ERR_INVALID;
+  const int ncc = (ncoarse - 1) / K;
+  if ((restrict_mode ? idims[dir] : odims[dir]) != 2 * K * ncc + 1) return MPDG_ERR_INVALID;
+  if (dir == 0) {
+    if (idims[0] > kEllLine) return MPDG_ERR_INVALID;
+    const long nlines = (long)odims[1] * odims[2];
+    const dim3 grid((unsigned)((nlines + 3) / 4));
+    if (restrict_mode)
+      hipLaunchKernelGGL((htransfer_x_kernel<T, K, true>), grid, dim3(256), 0, s, in, in2, out, nlines, ncc, tab_dev,
+                         lo, hi, 0);
+    else
+      hipLaunchKernelGGL((htransfer_x_kernel<T, K, false>), grid, dim3(256), 0, s, in, in2, out, nlines, ncc,
+                         tab_dev, lo, hi, add ? 1 : 0);
+    return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+  }
+  long nl, ibs, obs, is, os;
+  unsigned gy;
+  if (dir == 2) {  // lines over the x-y plane (unchanged by a z pass)
+    nl = (long)odims[0] * odims[1];
+    ibs = obs = 0;
+    is = os = nl;
+    gy = 1;
+  } else {  // lines over x, one block row per z
+    nl = odims[0];
+    ibs = (long)idims[0] * idims[1];
+    obs = (long)odims[0] * odims[1];
+    is = idims[0];
+    os = odims[0];
+    gy = (unsigned)odims[2];
+  }
+  const dim3 grid((unsigned)((nl + 255) / 256), gy, (ncc + kHSeg - 1) / kHSeg);
+  if (restrict_mode)
+    hipLaunchKernelGGL((hrestrict_kernel<T, K>), grid, dim3(256), 0, s, in, in2, out, nl, ibs, obs, is, os, ncc, P, lo,
+                       hi);
+  else
+    hipLaunchKernelGGL((hprolong_kernel<T, K>), grid, dim3(256), 0, s, in, out, nl, ibs, obs, is, os, ncc, P, lo, hi,
+                       add ? 1 : 0);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+template <typename T>
+int launch_htransfer(bool restrict_mode, int dir, const T *in, const T *in2, T *out, const int *idims,
+                     const int *odims, int k, const double *tab_host, const T *tab_dev, int lo, int hi, bool add,
+                     hipStream_t s) {
+  switch (k) {
+    case 1: return htransfer_k<T, 1>(restrict_mode, dir, in, in2, out, idims, odims, tab_host, tab_dev, lo, hi, add, s);
+    case 2: return htransfer_k<T, 2>(restrict_mode, dir, in, in2, out, idims, odims, tab_host, tab_dev, lo, hi, add, s);
+    case 3: return htransfer_k<T, 3>(restrict_mode, dir, in, in2, out, idims, odims, tab_host, tab_dev, lo, hi, add, s);
+    case 4: return htransfer_k<T, 4>(restrict_mode, dir, in, in2, out, idims, odims, tab_host, tab_dev, lo, hi, add, s);
+    default: return MPDG_ERR_INVALID;
+  }
+}
+template int launch_htransfer<float>(bool, int, const float *, const float *, float *, const int *, const int *, int,
+                                     const double *, const float *, int, int, bool, hipStream_t);
+template int launch_htransfer<double>(bool, int, const double *, const double *, double *, const int *, const int *,
+                                      int, const double *, const double *, int, int, bool, hipStream_t);
+
 template int launch_ell_dir<float>(const float *, float *, const int *, const int *, int, int, const int *,
                                    const float *, hipStream_t, const float *, bool);
 template int launch_ell_dir<double>(const double *, double *, const int *, const int *, int, int, const int *,

@@ -48,6 +48,13 @@ int launch_mul(double *x, const double *y, long n, hipStream_t s);
 template <typename T>
 int launch_ell_dir(const T *in, T *out, const int *odims, const int *idims, int dir, int width, const int *col,
                    const T *val, hipStream_t s, const T *in2 = nullptr, bool add = false);
+// structured 2:1 h-transfer of Q_k along `dir` (see blas.hip): restrict_mode false: out (+)= P in
+// (fine <- coarse, accumulating when add); true: out = P^T (in - in2) (in2 may be null).
+// tab_host / tab_dev: the (2k+1) x (k+1) table of one coarse cell; lo / hi: Dirichlet ends
+template <typename T>
+int launch_htransfer(bool restrict_mode, int dir, const T *in, const T *in2, T *out, const int *idims,
+                     const int *odims, int k, const double *tab_host, const T *tab_dev, int lo, int hi, bool add,
+                     hipStream_t s);
 // y = A^T x for a dense column-major n x n matrix (coarse-grid inverse); part: dense_gemv_scratch(n) doubles
 long dense_gemv_scratch(int n);
 int launch_dense_gemv(const double *A, const double *x, double *y, double *part, int n, hipStream_t s);

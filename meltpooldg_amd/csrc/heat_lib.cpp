@@ -257,6 +257,11 @@ struct Level {
   DeviceArray<int> pcol[3], rcol[3];
   DeviceArray<T> pval[3], rval[3];
   int pw[3] = {0, 0, 0}, rw[3] = {0, 0, 0};
+  // 2:1 h-transfer of the same degree: the table of one coarse cell (structured kernels)
+  bool hstruct = false;
+  std::vector<double> htab;
+  DeviceArray<T> htab_dev;
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   DeviceArray<T> t1, t2;
   std::vector<double> cheb_c1, cheb_c2;
   DeviceArray<double> coords;  // mapping node coordinates of this level (mapped meshes)
@@ -351,6 +356,21 @@ struct GMGPrecond : mpdg_precond {
         MPDG_TRY(upload_int(L.rcol[d], tr.rcol));
         MPDG_TRY(upload<T>(L.pval[d], tr.pval));
         MPDG_TRY(upload<T>(L.rval[d], tr.rval));
+        L.lo[d] = (f.dmask >> (2 * d)) & 1;
+        L.hi[d] = (f.dmask >> (2 * d + 1)) & 1;
+      }
+      L.hstruct = C.od.degree == L.od.degree && L.od.nd[0] <= 1024;
+      for (int d = 0; d < 3; ++d) L.hstruct = L.hstruct && L.od.nc[d] == 2 * C.od.nc[d];
+      if (L.hstruct) {
+        const int k = L.od.degree;
+        Transfer1DHost cell(k, 1, k, 2, false, false);  // one coarse cell: rows m = 0..2k, cols j = 0..k
+        L.htab.assign((size_t)(2 * k + 1) * (k + 1), 0.0);
+        for (int m = 0; m <= 2 * k; ++m)
+          for (int e = 0; e < cell.pw; ++e) {
+            const int j = cell.pcol[(size_t)m * cell.pw + e];
+            if (j >= 0) L.htab[(size_t)m * (k + 1) + j] = cell.pval[(size_t)m * cell.pw + e];
+          }
+        MPDG_TRY(upload<T>(L.htab_dev, L.htab));
       }
       t1 = (long)L.od.nd[0] * C.od.nd[1] * C.od.nd[2];
       t2 = (long)L.od.nd[0] * L.od.nd[1] * C.od.nd[2];
@@ -656,6 +676,16 @@ struct GMGPrecond : mpdg_precond {
   int prolongate_add(Level<T> &F, const Level<T> &C, hipStream_t s) {
     // F.x += P C.x via x, y, z passes, the last one adding into F.x
     int o[3] = {F.od.nd[0], C.od.nd[1], C.od.nd[2]}, i[3] = {C.od.nd[0], C.od.nd[1], C.od.nd[2]};
+    if (F.hstruct) {
+      int o2[3] = {F.od.nd[0], F.od.nd[1], C.od.nd[2]}, o3[3] = {F.od.nd[0], F.od.nd[1], F.od.nd[2]};
+      const int k = F.od.degree;
+      MPDG_TRY(launch_htransfer<T>(false, 0, C.x.get(), nullptr, F.t1.get(), i, o, k, F.htab.data(),
+                                   F.htab_dev.get(), F.lo[0], F.hi[0], false, s));
+      MPDG_TRY(launch_htransfer<T>(false, 1, F.t1.get(), nullptr, F.t2.get(), o, o2, k, F.htab.data(),
+                                   F.htab_dev.get(), F.lo[1], F.hi[1], false, s));
+      return launch_htransfer<T>(false, 2, F.t2.get(), nullptr, F.x.get(), o2, o3, k, F.htab.data(),
+                                 F.htab_dev.get(), F.lo[2], F.hi[2], true, s);
+    }
     MPDG_TRY(launch_ell_dir<T>(C.x.get(), F.t1.get(), o, i, 0, F.pw[0], F.pcol[0].get(), F.pval[0].get(), s));
     int o2[3] = {F.od.nd[0], F.od.nd[1], C.od.nd[2]};
     MPDG_TRY(launch_ell_dir<T>(F.t1.get(), F.t2.get(), o2, o, 1, F.pw[1], F.pcol[1].get(), F.pval[1].get(), s));
@@ -666,6 +696,16 @@ struct GMGPrecond : mpdg_precond {
   int restrict_residual(const Level<T> &F, Level<T> &C, hipStream_t s) {
     // C.b = R (F.b - F.Ax) via z, y, x passes, the residual formed in the first
     int i3[3] = {F.od.nd[0], F.od.nd[1], F.od.nd[2]}, o2[3] = {F.od.nd[0], F.od.nd[1], C.od.nd[2]};
+    if (F.hstruct) {
+      int o1[3] = {F.od.nd[0], C.od.nd[1], C.od.nd[2]}, o0[3] = {C.od.nd[0], C.od.nd[1], C.od.nd[2]};
+      const int k = F.od.degree;
+      MPDG_TRY(launch_htransfer<T>(true, 2, F.b.get(), F.Ax.get(), F.t2.get(), i3, o2, k, F.htab.data(),
+                                   F.htab_dev.get(), F.lo[2], F.hi[2], false, s));
+      MPDG_TRY(launch_htransfer<T>(true, 1, F.t2.get(), nullptr, F.t1.get(), o2, o1, k, F.htab.data(),
+                                   F.htab_dev.get(), F.lo[1], F.hi[1], false, s));
+      return launch_htransfer<T>(true, 0, F.t1.get(), nullptr, C.b.get(), o1, o0, k, F.htab.data(),
+                                 F.htab_dev.get(), F.lo[0], F.hi[0], false, s);
+    }
     MPDG_TRY(launch_ell_dir<T>(F.b.get(), F.t2.get(), o2, i3, 2, F.rw[2], F.rcol[2].get(), F.rval[2].get(), s,
                                F.Ax.get()));
     int o1[3] = {F.od.nd[0], C.od.nd[1], C.od.nd[2]};
