@@ -166,6 +166,9 @@ typedef struct mpdg_heat_operation_params {
   double field_correction_tolerance;
   int step_mode;          /* MPDG_STEP_NEWTON or MPDG_STEP_LINEAR */
   int guess_check;        /* learned predictor: fall back to T_n when its residual is not smaller */
+  int guess_smoothing;    /* learned predictor (with guess_check, linear step mode): smoothing
+                             steps on J T = b applied to an accepted guess (multigrid: Chebyshev
+                             steps of its finest level; 0: none) */
 } mpdg_heat_operation_params;
 int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p,
                                const mpdg_heat_operation_params *params,
@@ -207,6 +210,10 @@ int mpdg_heat_op_surface_flux_field(const mpdg_heat_op *op, double *out, void *s
    multiplies the heat-source channel (e.g. dt / (rho c_p h)). */
 int mpdg_learned_create(const int *n_cells, int degree, int coarsening, double qscale,
                         unsigned long long seed, mpdg_learned **net);
+/* output scale: the correction is oscale * rms(z) * y; the first train step sets it from its
+   sample (target rms 1) unless set before. Part of a checkpoint next to the weights. */
+int mpdg_learned_get_output_scale(const mpdg_learned *net, double *oscale);
+int mpdg_learned_set_output_scale(mpdg_learned *net, double oscale);
 /* DoF grid of the fields it takes and its working grid (either pointer may be NULL) */
 int mpdg_learned_grid(const mpdg_learned *net, int *dof_dims, int *net_dims);
 int mpdg_learned_destroy(mpdg_learned *net);
@@ -214,15 +221,15 @@ int mpdg_learned_n_params(const mpdg_learned *net);
 /* weights as one FP32 host array: per layer W[co][ci][tap] then b[co] */
 int mpdg_learned_get_weights(const mpdg_learned *net, float *host);
 int mpdg_learned_set_weights(mpdg_learned *net, const float *host);
-/* out = (2 T_n - T_{n-1}) + P (sz y(T_{n-1}, T_n, q, z)), sz = rms(z) on the network grid:
+/* out = (2 T_n - T_{n-1}) + P (oscale sz y(T_{n-1}, T_n, q, z)), sz = rms(z) on the network grid:
    linear extrapolation (equal steps) plus the learned correction. Device FP64 DoF vectors; z is
    the residual input recorded by mpdg_heat_operation_record_guess_feature; q and z may be NULL
    (channel zero; without z, sz = rms(T_n - T_{n-1})) */
 int mpdg_learned_predict(mpdg_learned *net, const double *T_nm1, const double *T_n, const double *q,
                          const double *z, double *out, void *stream);
 /* one Adam step on the sample (T_{n-1}, T_n, q, z) -> T_{n+1}; loss = mean squared error of
-   y against W(T_{n+1} - (2 T_n - T_{n-1}))/sz before the step (W = full weighting to the
-   network grid) */
+   y against W(T_{n+1} - (2 T_n - T_{n-1}))/(oscale sz) before the step (W = full weighting to
+   the network grid) */
 int mpdg_learned_train_step(mpdg_learned *net, const double *T_nm1, const double *T_n, const double *q,
                             const double *z, const double *T_np1, double lr, double *loss, void *stream);
 /* gradients of the loss (host FP32, weight layout) without an update */

@@ -64,7 +64,7 @@ class SolverParams(C.Structure):
 class OperationParams(C.Structure):
     _fields_ = [("predictor", C.c_int), ("linear", SolverParams), ("max_nonlinear_iterations", C.c_int),
                 ("residual_tolerance", C.c_double), ("field_correction_tolerance", C.c_double),
-                ("step_mode", C.c_int), ("guess_check", C.c_int)]
+                ("step_mode", C.c_int), ("guess_check", C.c_int), ("guess_smoothing", C.c_int)]
 
 
 STEP_NEWTON, STEP_LINEAR = 0, 1
@@ -111,6 +111,8 @@ _SIGNATURES = {
     "mpdg_heat_op_surface_flux_field": ([_V, _V, _V], _I),
     "mpdg_learned_create": ([C.POINTER(_I), _I, _I, _D, C.c_ulonglong, C.POINTER(_V)], _I),
     "mpdg_learned_grid": ([_V, C.POINTER(_I), C.POINTER(_I)], _I),
+    "mpdg_learned_get_output_scale": ([_V, C.POINTER(_D)], _I),
+    "mpdg_learned_set_output_scale": ([_V, _D], _I),
     "mpdg_learned_destroy": ([_V], _I),
     "mpdg_learned_n_params": ([_V], _I),
     "mpdg_learned_get_weights": ([_V, _V], _I),
@@ -394,7 +396,7 @@ class HeatDiffuseOperation:
     """Implicit heat time step: predictor, preconditioner update, Newton-Raphson with CG."""
 
     def __init__(self, op, preconditioner=None, predictor=PREDICTOR_LINEAR, linear=None, nlsolve=None,
-                 step_mode=STEP_NEWTON, learned=None, q_field=None, guess_check=True):
+                 step_mode=STEP_NEWTON, learned=None, q_field=None, guess_check=True, guess_smoothing=0):
         self.op = op
         self.lib = op.lib
         self.pc = preconditioner
@@ -409,6 +411,7 @@ class HeatDiffuseOperation:
         p.field_correction_tolerance = nlsolve.field_correction_tolerance
         p.step_mode = step_mode
         p.guess_check = 1 if guess_check else 0
+        p.guess_smoothing = guess_smoothing
         h = C.c_void_p()
         _check(self.lib.mpdg_heat_operation_create(op.h, preconditioner.h if preconditioner else None,
                                                    C.byref(p), C.byref(h)), "heat_operation_create")
@@ -525,13 +528,29 @@ class LearnedPredictor:
         assert w.size == self.n_params
         _check(self.lib.mpdg_learned_set_weights(self.h, w.ctypes.data_as(C.c_void_p)), "set_weights")
 
+    @property
+    def output_scale(self):
+        v = C.c_double()
+        _check(self.lib.mpdg_learned_get_output_scale(self.h, C.byref(v)), "get_output_scale")
+        return v.value
+
+    @output_scale.setter
+    def output_scale(self, v):
+        _check(self.lib.mpdg_learned_set_output_scale(self.h, float(v)), "set_output_scale")
+
     def save(self, path):
-        """Checkpoint: the FP32 weights as a raw little-endian binary."""
-        self.get_weights().tofile(path)
+        """Checkpoint: the FP32 weights then the FP64 output scale, raw little-endian."""
+        import numpy as np
+        with open(path, "wb") as f:
+            f.write(self.get_weights().tobytes())
+            f.write(np.float64(self.output_scale).tobytes())
 
     def load(self, path):
         import numpy as np
-        self.set_weights(np.fromfile(path, dtype=np.float32))
+        raw = np.fromfile(path, dtype=np.uint8)
+        nw = 4 * self.n_params
+        self.set_weights(raw[:nw].view(np.float32))
+        self.output_scale = float(raw[nw:nw + 8].view(np.float64)[0])
 
 
 def _copy_device(dst, src_ptr, n):

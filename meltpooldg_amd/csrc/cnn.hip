@@ -219,17 +219,22 @@ __global__ void features_kernel(const double *__restrict__ Tnm1, const double *_
   }
 }
 
-// loss gradient on channel 0: target y* = (Tnp1 - (2 Tn - Tnm1))/sz, pred = y[v][0];
-// g = 2 (pred - y*) / n * gain; partial sums of the squared error
+// loss gradient on channel 0: target y* = (Tnp1 - (2 Tn - Tnm1))/(sz oscale), pred = y[v][0];
+// g = 2 (pred - y*) / n * gain; partial sums of the squared error. With y = null: partial
+// sums of y*^2 only (the output scale's calibration).
 __global__ __launch_bounds__(256) void loss_kernel(const unsigned short *__restrict__ y, const double *__restrict__ Tnm1,
                                                    const double *__restrict__ Tn, const double *__restrict__ Tnp1,
-                                                   long n, const double *__restrict__ stats, float gain,
+                                                   long n, const double *__restrict__ stats, double oscale, float gain,
                                                    unsigned short *__restrict__ g, double *__restrict__ part) {
   __shared__ double sh[256];
-  const double sz = stats[3];
+  const double sz = stats[3] * oscale;
   double acc = 0.0;
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
     const double tgt = (Tnp1[i] - (2.0 * Tn[i] - Tnm1[i])) / sz;
+    if (!y) {
+      acc += tgt * tgt;
+      continue;
+    }
     const double e = (double)bf16_to_f32(y[i * 16]) - tgt;
     acc += e * e;
     uint4 *dst = reinterpret_cast<uint4 *>(g + i * 16);
@@ -245,10 +250,10 @@ __global__ __launch_bounds__(256) void loss_kernel(const unsigned short *__restr
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// correction of the linear extrapolation on the network grid: corr = sz y[v][0]
+// correction of the linear extrapolation on the network grid: corr = sz oscale y[v][0]
 __global__ void correction_kernel(const unsigned short *__restrict__ y, long n, const double *__restrict__ stats,
-                                  double *__restrict__ corr) {
-  const double sz = stats[3];
+                                  double oscale, double *__restrict__ corr) {
+  const double sz = stats[3] * oscale;
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
     corr[i] = sz * (double)bf16_to_f32(y[i * 16]);
 }
@@ -354,14 +359,15 @@ int cnn_features(const double *Tnm1, const double *Tn, const double *q, const do
 }
 
 int cnn_loss(const unsigned short *y, const double *Tnm1, const double *Tn, const double *Tnp1, long n,
-             const double *stats, float gain, unsigned short *g, double *part, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL(loss_kernel, dim3(nblocks), dim3(256), 0, s, y, Tnm1, Tn, Tnp1, n, stats, gain, g, part);
+             const double *stats, double oscale, float gain, unsigned short *g, double *part, int nblocks,
+             hipStream_t s) {
+  hipLaunchKernelGGL(loss_kernel, dim3(nblocks), dim3(256), 0, s, y, Tnm1, Tn, Tnp1, n, stats, oscale, gain, g, part);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
-int cnn_correction(const unsigned short *y, long n, const double *stats, double *corr, hipStream_t s) {
+int cnn_correction(const unsigned short *y, long n, const double *stats, double oscale, double *corr, hipStream_t s) {
   hipLaunchKernelGGL(correction_kernel, dim3((int)std::min<long>(8192, (n + 255) / 256)), dim3(256), 0, s, y, n, stats,
-                     corr);
+                     oscale, corr);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 

@@ -714,9 +714,10 @@ struct GMGPrecond : mpdg_precond {
     return launch_ell_dir<T>(F.t1.get(), C.b.get(), o0, o1, 0, F.rw[0], F.rcol[0].get(), F.rval[0].get(), s);
   }
 
-  int smooth(Level<T> &L, bool x_zero, hipStream_t s) {
+  int smooth(Level<T> &L, bool x_zero, hipStream_t s, int degree_override = 0) {
     const long n = L.od.n_dofs;
-    const int degree = (x_zero && prm.pre_smoother_degree > 0) ? prm.pre_smoother_degree : prm.smoother_degree;
+    int degree = (x_zero && prm.pre_smoother_degree > 0) ? prm.pre_smoother_degree : prm.smoother_degree;
+    if (degree_override > 0) degree = std::min(degree_override, prm.smoother_degree);
     for (int k = 0; k < degree; ++k) {
       const bool first = (k == 0);
       if (first && x_zero) {
@@ -752,6 +753,13 @@ struct GMGPrecond : mpdg_precond {
     const long n = top.od.n_dofs;
     MPDG_TRY(launch_convert<double, T>(src, top.b.get(), n, s));
     MPDG_TRY(vcycle((int)lev.size() - 1, s));
+    return launch_convert<T, double>(top.x.get(), dst, n, s);
+  }
+  int smooth(double *dst, const double *src, int degree, hipStream_t s) override {
+    Level<T> &top = *lev.back();
+    const long n = top.od.n_dofs;
+    MPDG_TRY(launch_convert<double, T>(src, top.b.get(), n, s));
+    MPDG_TRY(smooth(top, true, s, degree));
     return launch_convert<T, double>(top.x.get(), dst, n, s);
   }
 
@@ -1187,13 +1195,13 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
   MPDG_TRY(op->create_rhs(h->bvec.get(), h->zero.get(), T_old, nullptr, s));
   MPDG_TRY(learned_guess(h, dt, h->bvec.get(), T, T_old, s));
   if (h->prm.predictor == MPDG_PREDICTOR_LEARNED && h->prm.guess_check) {
-    // residuals of the learned guess (T) and of the previous solution (T_old)
-    MPDG_TRY(op->vmult(h->alt.get(), T, s));
-    MPDG_TRY(launch_axpby(h->alt.get(), 1.0, h->bvec.get(), -1.0, n, s));
-    MPDG_TRY(launch_dot<double, double>(h->alt.get(), h->alt.get(), n, h->red.red(), h->scal.get(), s));
+    // residuals of the previous solution (T_old) and of the learned guess (T, kept in alt)
     MPDG_TRY(op->vmult(h->alt.get(), T_old, s));
     MPDG_TRY(launch_axpby(h->alt.get(), 1.0, h->bvec.get(), -1.0, n, s));
     MPDG_TRY(launch_dot<double, double>(h->alt.get(), h->alt.get(), n, h->red.red(), h->scal.get() + 1, s));
+    MPDG_TRY(op->vmult(h->alt.get(), T, s));
+    MPDG_TRY(launch_axpby(h->alt.get(), 1.0, h->bvec.get(), -1.0, n, s));
+    MPDG_TRY(launch_dot<double, double>(h->alt.get(), h->alt.get(), n, h->red.red(), h->scal.get(), s));
     double rr[2];
     if (hipMemcpyAsync(rr, h->scal.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
       return MPDG_ERR_DEVICE;
@@ -1201,6 +1209,11 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
     if (!(rr[0] == rr[0]) || !(rr[0] < rr[1])) {
       MPDG_TRY(launch_convert<double, double>(T_old, T, n, s));
       h->fell_back = 1;
+    } else if (h->prm.guess_smoothing > 0 && h->pc) {
+      // the network's correction is prolonged from its working grid; smoothing its error
+      // removes the high-frequency part that the residual (and CG) weighs most
+      MPDG_TRY(h->pc->smooth(h->du.get(), h->alt.get(), h->prm.guess_smoothing, s));
+      MPDG_TRY(launch_axpby(T, 1.0, h->du.get(), 1.0, n, s));
     }
   }
   mpdg_solver_params lp = h->prm.linear;

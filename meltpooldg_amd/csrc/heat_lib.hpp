@@ -167,6 +167,12 @@ struct mpdg_precond {
   virtual int update(hipStream_t s) = 0;
   virtual int vmult(double *dst, const double *src, hipStream_t s) = 0;
   virtual const double *inverse_diagonal() const { return nullptr; }
+  // dst = an approximate solution of J dst = src by `degree` smoothing steps from zero on the
+  // finest level (multigrid: its Chebyshev smoother; others: the preconditioner itself)
+  virtual int smooth(double *dst, const double *src, int degree, hipStream_t s) {
+    (void)degree;
+    return vmult(dst, src, s);
+  }
   virtual int n_levels() const { return 0; }
   virtual int level_info(int, int *, int *, double *) const { return MPDG_ERR_INVALID; }
 };

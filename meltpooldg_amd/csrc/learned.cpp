@@ -59,6 +59,11 @@ struct mpdg_learned {
   int rho = 1, degree = 1;
   long nf = 0, n = 0;
   double qscale = 1.0;
+  // output scale: the correction is oscale * rms(z) * y. Set from the first training sample
+  // (so that its target has rms 1) unless given; where the step is stiff the correction is
+  // much larger than the diagonal-scaled residual and an O(1) output could not reach it.
+  double oscale = 1.0;
+  bool oscale_set = false;
   int step = 0;
   DeviceArray<float> w[kLayers], b[kLayers], mw[kLayers], vw[kLayers], mb[kLayers], vb[kLayers];
   DeviceArray<unsigned short> W[kLayers], Wt[kLayers], act[kLayers + 1], g0, g1;
@@ -165,11 +170,32 @@ struct mpdg_learned {
     MPDG_TRY(rc);
     const double *zz = to_net(z, 3, s, &rc);
     MPDG_TRY(rc);
+    last_Tnm1 = a;
+    last_Tn = t;
     MPDG_TRY(cnn_stats(a, t, zz, n, spart.get(), stats.get(), s));
     MPDG_TRY(cnn_features(a, t, qq, zz, n, stats.get(), qscale, act[0].get(), s));
     for (int l = 0; l < kLayers; ++l)
       MPDG_TRY(cnn_conv(act[l].get(), act[l + 1].get(), W[l].get(), b[l].get(), dims, l == kLayers - 1 ? 0 : 1,
                         nullptr, s));
+    return MPDG_OK;
+  }
+  // rms of the target (T_{n+1} - T_lin) / rms(z) of a sample after forward()
+  int target_rms(const double *Tnp1, double *rms, hipStream_t s) {
+    int rc = MPDG_OK;
+    const double *tp = to_net(Tnp1, 4, s, &rc);
+    MPDG_TRY(rc);
+    return target_rms_net(tp, rms, 512, s);
+  }
+  const double *last_Tnm1 = nullptr, *last_Tn = nullptr;  // network-grid inputs of the last forward()
+  int target_rms_net(const double *tp, double *rms, int nb, hipStream_t s) {
+    MPDG_TRY(cnn_loss(nullptr, last_Tnm1, last_Tn, tp, n, stats.get(), 1.0, 1.0f, nullptr, lpart.get(), nb, s));
+    std::vector<double> h(nb);
+    if (hipMemcpyAsync(h.data(), lpart.get(), nb * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
+      return MPDG_ERR_DEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+    double sum = 0.0;
+    for (double v : h) sum += v;
+    *rms = std::sqrt(sum / (double)n);
     return MPDG_OK;
   }
   // loss and gradients of all layers (dW, db) after forward(); returns the mean squared error
@@ -179,7 +205,7 @@ struct mpdg_learned {
     const double *a = rho == 1 ? Tnm1 : cf[0].get(), *t = rho == 1 ? Tn : cf[1].get();
     const double *tp = to_net(Tnp1, 4, s, &rc);
     MPDG_TRY(rc);
-    MPDG_TRY(cnn_loss(act[kLayers].get(), a, t, tp, n, stats.get(), 1.0f, g0.get(), lpart.get(), nb, s));
+    MPDG_TRY(cnn_loss(act[kLayers].get(), a, t, tp, n, stats.get(), oscale, 1.0f, g0.get(), lpart.get(), nb, s));
     unsigned short *g = g0.get(), *gn = g1.get();
     for (int l = kLayers - 1; l >= 0; --l) {
       MPDG_TRY(cnn_wgrad(g, act[l].get(), dims, part.get(), kWgradBlocks, dW[l].get(), db[l].get(), s));
@@ -202,7 +228,7 @@ struct mpdg_learned {
   }
   // out = (2 Tn - Tnm1) + P corr after forward()
   int predict(const double *Tnm1, const double *Tn, double *out, hipStream_t s) {
-    MPDG_TRY(cnn_correction(act[kLayers].get(), n, stats.get(), corr.get(), s));
+    MPDG_TRY(cnn_correction(act[kLayers].get(), n, stats.get(), oscale, corr.get(), s));
     const double *c = corr.get();
     if (rho > 1) {
       const int o0[3] = {fdims[0], dims[1], dims[2]}, o1[3] = {fdims[0], fdims[1], dims[2]};
@@ -316,6 +342,12 @@ int mpdg_learned_train_step(mpdg_learned *h, const double *Tnm1, const double *T
   if (!h || !Tnm1 || !Tn || !Tnp1) return MPDG_ERR_INVALID;
   hipStream_t s = S(stream);
   MPDG_TRY(h->forward(Tnm1, Tn, q, z, s));
+  if (!h->oscale_set) {
+    double rms = 0.0;
+    MPDG_TRY(h->target_rms(Tnp1, &rms, s));
+    h->oscale = (std::isfinite(rms) && rms > 1e-12) ? rms : 1.0;
+    h->oscale_set = true;
+  }
   MPDG_TRY(h->backward(Tnm1, Tn, Tnp1, loss, s));
   h->step += 1;
   for (int l = 0; l < kLayers; ++l)
@@ -324,6 +356,19 @@ int mpdg_learned_train_step(mpdg_learned *h, const double *Tnm1, const double *T
                       1.0f, s));
   MPDG_TRY(h->pack(s));
   if (loss && !std::isfinite(*loss)) return MPDG_ERR_NONFINITE;
+  return MPDG_OK;
+}
+
+// output scale of the correction (see mpdg_learned::oscale)
+int mpdg_learned_get_output_scale(const mpdg_learned *h, double *oscale) {
+  if (!h || !oscale) return MPDG_ERR_INVALID;
+  *oscale = h->oscale;
+  return MPDG_OK;
+}
+int mpdg_learned_set_output_scale(mpdg_learned *h, double oscale) {
+  if (!h || !(oscale > 0.0) || !std::isfinite(oscale)) return MPDG_ERR_INVALID;
+  h->oscale = oscale;
+  h->oscale_set = true;
   return MPDG_OK;
 }
 

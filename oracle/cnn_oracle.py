@@ -151,20 +151,20 @@ def forward(flat, Tnm1, Tn, q, z, qscale=1.0, bf=True, tr=None):
     return acts, s, layers
 
 
-def predict(flat, Tnm1, Tn, q, z, qscale=1.0, bf=True, tr=None):
-    """Linear extrapolation plus the output in units of sz (prolonged from the network grid)."""
+def predict(flat, Tnm1, Tn, q, z, qscale=1.0, bf=True, tr=None, oscale=1.0):
+    """Linear extrapolation plus the output in units of oscale sz (prolonged from the network grid)."""
     acts, sz, _ = forward(flat, Tnm1, Tn, q, z, qscale, bf, tr)
-    corr = sz * acts[-1][0]
+    corr = oscale * sz * acts[-1][0]
     if tr is not None:
         corr = along(corr, [t[0] for t in tr])
     return (2.0 * Tn - Tnm1) + corr
 
 
-def loss_and_gradients(flat, Tnm1, Tn, q, z, Tnp1, qscale=1.0, bf=True, tr=None):
+def loss_and_gradients(flat, Tnm1, Tn, q, z, Tnp1, qscale=1.0, bf=True, tr=None, oscale=1.0):
     rnd = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
     acts, sz, layers = forward(flat, Tnm1, Tn, q, z, qscale, bf, tr)
     a, t, tp = _net(Tnm1, tr), _net(Tn, tr), _net(Tnp1, tr)
-    tgt = (tp - (2.0 * t - a)) / sz
+    tgt = (tp - (2.0 * t - a)) / (oscale * sz)
     e = acts[-1][0] - tgt
     n = t.size
     loss = float(np.sum(e * e) / n)

@@ -73,6 +73,29 @@ def test_training_reduces_loss(cuda):
     assert last < 0.2 * first
 
 
+def test_output_scale_is_set_by_the_first_sample(cuda):
+    """The first train step sets the output scale to the target's rms in units of rms(z), and
+    predictions use it as the oracle does."""
+    torch = cuda
+    import meltpooldg_amd as M
+    shape = (9, 13, 17)
+    Tnm1, Tn, q, z, Tnp1 = sample(shape, seed=6)
+    net, tr = make_net(M, shape, degree=2, rho=2, qscale=0.5, seed=3)
+    net.train_step(*[to_dev(torch, a) for a in (Tnm1, Tn, q, z, Tnp1)], 1e-3)
+    _, sz = N.features(*[N._net(a, tr) for a in (Tnm1, Tn, q, z)], 0.5)
+    a, t, tp = (N._net(x, tr) for x in (Tnm1, Tn, Tnp1))
+    expect = np.sqrt(np.mean(((tp - (2 * t - a)) / sz) ** 2))
+    assert abs(net.output_scale - expect) <= 1e-9 * expect
+    w = net.get_weights()
+    out = torch.empty(Tn.size, dtype=torch.float64, device="cuda")
+    net.predict(to_dev(torch, Tnm1), to_dev(torch, Tn), to_dev(torch, q), to_dev(torch, z), out)
+    torch.cuda.synchronize()
+    ref = N.predict(w, Tnm1, Tn, q, z, qscale=0.5, bf=True, tr=tr, oscale=net.output_scale).ravel()
+    lin = (2 * Tn - Tnm1).ravel()
+    got, exp = out.cpu().numpy() - lin, ref - lin
+    assert np.linalg.norm(got - exp) <= 2e-2 * np.linalg.norm(exp)
+
+
 def test_trained_network_beats_the_predictors_without_it(cuda):
     """Q2 on 16^3 cells, laser scan: train on the pairs of steps 2..40, evaluate steps 41..60
     from the same trajectory state. The learned guess starts CG closer to the solution than
@@ -87,8 +110,10 @@ def test_trained_network_beats_the_predictors_without_it(cuda):
     net = M.LearnedPredictor((cells,) * 3, degree=k, qscale=dt / h, seed=11)
     losses = train(net, Ts[:41], qs[:40], zs[:40], epochs=20, lr=2e-3)
     res = {}
-    for name, pred in (("none", M.PREDICTOR_NONE), ("linear", M.PREDICTOR_LINEAR), ("learned", M.PREDICTOR_LEARNED)):
-        case = LaserScan(k, cells, dt=dt, predictor=pred, learned=net if pred == M.PREDICTOR_LEARNED else None)
+    for name, pred, sm in (("none", M.PREDICTOR_NONE, 0), ("linear", M.PREDICTOR_LINEAR, 0),
+                           ("learned", M.PREDICTOR_LEARNED, 0), ("learned_smoothed", M.PREDICTOR_LEARNED, 2)):
+        case = LaserScan(k, cells, dt=dt, predictor=pred, learned=net if pred == M.PREDICTOR_LEARNED else None,
+                         guess_smoothing=sm)
         # restart from the trajectory at step 39; one replayed step rebuilds the history (T_39, T_40)
         r0s, its, fbs = [], [], []
         case.t = 39 * dt
@@ -105,3 +130,5 @@ def test_trained_network_beats_the_predictors_without_it(cuda):
     assert res["learned"][2] == 0
     assert res["learned"][0] < res["linear"][0] - 0.3 and res["linear"][0] < res["none"][0]
     assert res["learned"][1] <= res["linear"][1]
+    # two Chebyshev steps of the finest multigrid level on an accepted guess lower its residual
+    assert res["learned_smoothed"][2] == 0 and res["learned_smoothed"][0] < res["learned"][0]
