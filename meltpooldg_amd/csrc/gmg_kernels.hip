@@ -1,97 +1,250 @@
-// Setup kernels of the multigrid preconditioner: conductivity cell averages
-// straight from an operator's data, and an in-place Gauss-Jordan inverse of the
-// (small, symmetric positive definite) coarse-grid matrix.
+// Setup kernels of the multigrid preconditioner: the dense coarse-grid matrix
+// assembled on the device from element matrices, and its in-place Gauss-Jordan
+// inverse (the matrix is symmetric positive definite: no pivoting), blocked so
+// that most of the work is a rank-32 update with the panels staged in LDS.
 #include "blas.hpp"
 
 namespace mpdg {
 
-// quadrature-weighted cell average of the conductivity of an FP64 operator
-__global__ __launch_bounds__(256) void op_cell_average_kernel(int n1, int nc0, int nc1, long ncells, int nd0, int nd1,
-                                                              int mode, double kappa, const double *__restrict__ karr,
-                                                              int law, double p0, double p1, double p2, double p3,
-                                                              const double *__restrict__ T, double S00, double *Sg,
-                                                              double *wg, double *__restrict__ kc) {
-  (void)S00;
-  const long c = (long)blockIdx.x * 256 + threadIdx.x;
-  if (c >= ncells) return;
-  const int nq = n1 * n1 * n1, K = n1 - 1;
-  double s = 0.0;
-  if (mode == KAPPA_CONST) s = kappa;
-  else if (mode == KAPPA_CELL) s = karr[c];
-  else if (mode == KAPPA_QP) {
-    for (int q = 0; q < nq; ++q) s += wg[q % n1] * wg[(q / n1) % n1] * wg[q / (n1 * n1)] * karr[c * nq + q];
-  } else {
-    const int cx = c % nc0, cy = (c / nc0) % nc1, cz = c / ((long)nc0 * nc1);
-    for (int q = 0; q < nq; ++q) {
-      const int qx = q % n1, qy = (q / n1) % n1, qz = q / (n1 * n1);
-      double Tq = 0.0;
-      for (int l = 0; l < n1; ++l)
-        for (int j = 0; j < n1; ++j) {
-          const long row = (long)K * cx + (long)nd0 * ((K * cy + j) + (long)nd1 * (K * cz + l));
-          double r = 0.0;
-          for (int i = 0; i < n1; ++i) r += Sg[qx * kMaxN1 + i] * T[row + i];
-          Tq += Sg[qy * kMaxN1 + j] * Sg[qz * kMaxN1 + l] * r;
+struct CoarseTables {
+  double S[kMaxN1][kMaxN1], G[kMaxN1][kMaxN1], w[kMaxN1];
+};
+
+__device__ __forceinline__ bool constrained(long g, int nd0, int nd1, int nd2, int m) {
+  const int ix = g % nd0, iy = (g / nd0) % nd1, iz = g / ((long)nd0 * nd1);
+  return ((m & 1) && ix == 0) || ((m & 2) && ix == nd0 - 1) || ((m & 4) && iy == 0) ||
+         ((m & 8) && iy == nd1 - 1) || ((m & 16) && iz == 0) || ((m & 32) && iz == nd2 - 1);
+}
+
+// One thread per (cell, I, J) entry of the element matrix
+//   A_e[I][J] = sum_q m_q phi_I phi_J + k_cell grad phi_I . G_q grad phi_J,
+// Cartesian cells: m_q = c_m vol W_q, G_q = diag(vol/h_d^2) W_q; mapped cells: m_q = c_m JxW_q,
+// G_q = JxW_q B B^T. Summed into the dense row-major matrix A; rows and columns of
+// constrained DoFs are left out (their identity diagonal is set separately).
+template <typename T>
+__global__ __launch_bounds__(256) void coarse_matrix_kernel(CoarseTables t, int n1, int nc0, int nc1, long ncells,
+                                                            int nd0, int nd1, int nd2, int dmask, double cm,
+                                                            double ks0, double ks1, double ks2, double c_mass,
+                                                            const double *__restrict__ kc, double kconst,
+                                                            const T *__restrict__ metric, double *__restrict__ A) {
+  const int nloc = n1 * n1 * n1, K = n1 - 1;
+  const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+  if (gid >= ncells * nloc * nloc) return;
+  const long cell = gid / ((long)nloc * nloc);
+  const int e = (int)(gid - cell * nloc * nloc), I = e / nloc, J = e - I * nloc;
+  const int cx = cell % nc0, cy = (cell / nc0) % nc1, cz = cell / ((long)nc0 * nc1);
+  const int i0 = I % n1, i1 = (I / n1) % n1, i2 = I / (n1 * n1);
+  const int j0 = J % n1, j1 = (J / n1) % n1, j2 = J / (n1 * n1);
+  const long gI = (long)(K * cx + i0) + (long)nd0 * ((K * cy + i1) + (long)nd1 * (K * cz + i2));
+  const long gJ = (long)(K * cx + j0) + (long)nd0 * ((K * cy + j1) + (long)nd1 * (K * cz + j2));
+  if (dmask && (constrained(gI, nd0, nd1, nd2, dmask) || constrained(gJ, nd0, nd1, nd2, dmask))) return;
+  const double kap = kc ? kc[cell] : kconst;
+  double acc = 0.0;
+  for (int qz = 0; qz < n1; ++qz)
+    for (int qy = 0; qy < n1; ++qy)
+      for (int qx = 0; qx < n1; ++qx) {
+        const double pI = t.S[qx][i0] * t.S[qy][i1] * t.S[qz][i2], pJ = t.S[qx][j0] * t.S[qy][j1] * t.S[qz][j2];
+        const double a[3] = {t.G[qx][i0] * t.S[qy][i1] * t.S[qz][i2], t.S[qx][i0] * t.G[qy][i1] * t.S[qz][i2],
+                             t.S[qx][i0] * t.S[qy][i1] * t.G[qz][i2]};
+        const double b[3] = {t.G[qx][j0] * t.S[qy][j1] * t.S[qz][j2], t.S[qx][j0] * t.G[qy][j1] * t.S[qz][j2],
+                             t.S[qx][j0] * t.S[qy][j1] * t.G[qz][j2]};
+        if (metric) {
+          const int q = qx + n1 * (qy + n1 * qz);
+          const T *M = metric + cell * 7 * nloc + q;
+          const double g00 = M[nloc], g01 = M[2 * nloc], g11 = M[3 * nloc], g02 = M[4 * nloc], g12 = M[5 * nloc],
+                       g22 = M[6 * nloc];
+          const double gg = a[0] * (g00 * b[0] + g01 * b[1] + g02 * b[2]) +
+                            a[1] * (g01 * b[0] + g11 * b[1] + g12 * b[2]) +
+                            a[2] * (g02 * b[0] + g12 * b[1] + g22 * b[2]);
+          acc += c_mass * (double)M[0] * pI * pJ + kap * gg;
+        } else {
+          const double W = t.w[qx] * t.w[qy] * t.w[qz];
+          acc += W * (cm * pI * pJ + kap * (ks0 * a[0] * b[0] + ks1 * a[1] * b[1] + ks2 * a[2] * b[2]));
         }
-      double k;
-      if (law == LAW_TANH) k = p0 * (1.0 + p1 * tanh((Tq - p2) / p3));
-      else {
-        double fs = (p3 - Tq) / (p3 - p2);
-        fs = fs < 0.0 ? 0.0 : (fs > 1.0 ? 1.0 : fs);
-        k = p0 + (p1 - p0) * (3.0 - 2.0 * fs) * fs * fs;
       }
-      s += wg[qx] * wg[qy] * wg[qz] * k;
+  atomicAdd(A + gI * (nd0 * (long)nd1 * nd2) + gJ, acc);
+}
+
+__global__ __launch_bounds__(256) void coarse_dirichlet_kernel(double *A, long n, int nd0, int nd1, int nd2,
+                                                               int dmask) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n && constrained(i, nd0, nd1, nd2, dmask)) A[i * n + i] = 1.0;
+}
+
+template <typename T>
+int launch_coarse_matrix(const OperatorData<T> &od, double c_mass, const double *kc, double kconst, double *A,
+                         hipStream_t s) {
+  FE1D fe(od.degree);
+  CoarseTables t{};
+  for (int a = 0; a < fe.n1; ++a) {
+    t.w[a] = fe.qw[a];
+    for (int b = 0; b < fe.n1; ++b) {
+      t.S[a][b] = fe.S[a][b];
+      t.G[a][b] = fe.G[a][b];
     }
   }
-  kc[c] = s;
-}
-
-int launch_op_cell_average(const OperatorData<double> &op, double *kc, double *table_scratch, hipStream_t stream) {
-  FE1D fe(op.degree);
-  double host[kMaxN1 * kMaxN1 + kMaxN1] = {};
-  for (int a = 0; a < fe.n1; ++a) {
-    host[kMaxN1 * kMaxN1 + a] = fe.qw[a];
-    for (int b = 0; b < fe.n1; ++b) host[a * kMaxN1 + b] = fe.S[a][b];
-  }
-  hipMemcpyAsync(table_scratch, host, sizeof(host), hipMemcpyHostToDevice, stream);
-  hipLaunchKernelGGL(op_cell_average_kernel, dim3((op.n_cells + 255) / 256), dim3(256), 0, stream, op.degree + 1,
-                     op.nc[0], op.nc[1], op.n_cells, op.nd[0], op.nd[1], op.kappa_mode, op.kappa, op.kappa_arr, op.law,
-                     op.law_p[0], op.law_p[1], op.law_p[2], op.law_p[3], op.T, 0.0, table_scratch,
-                     table_scratch + kMaxN1 * kMaxN1, kc);
+  const long n = od.n_dofs, nloc = (long)fe.n1 * fe.n1 * fe.n1, work = od.n_cells * nloc * nloc;
+  const double vol = od.h[0] * od.h[1] * od.h[2];
+  const int rc = launch_fill<double>(A, 0.0, n * n, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(coarse_matrix_kernel<T>, dim3((work + 255) / 256), dim3(256), 0, s, t, fe.n1, od.nc[0],
+                     od.nc[1], od.n_cells, od.nd[0], od.nd[1], od.nd[2], od.dmask, c_mass * vol,
+                     vol / (od.h[0] * od.h[0]), vol / (od.h[1] * od.h[1]), vol / (od.h[2] * od.h[2]), c_mass, kc,
+                     kconst, od.metric, A);
+  if (od.dmask)
+    hipLaunchKernelGGL(coarse_dirichlet_kernel, dim3((n + 255) / 256), dim3(256), 0, s, A, n, od.nd[0], od.nd[1],
+                       od.nd[2], od.dmask);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
-// In-place Gauss-Jordan inversion without pivoting (SPD matrix), step k in three kernels.
-__global__ void gj_row_kernel(double *A, int n, int k) {
+// Blocked in-place Gauss-Jordan inversion. For the diagonal block [k0, k0 + nb) of the current
+// matrix [[A11, A12], [A21, A22]] one step writes
+//   A11 <- A11^-1,  A12 <- A11^-1 A12,  A22 <- A22 - A21 A11^-1 A12,  A21 <- -A21 A11^-1
+// (the scalar step with 1x1 blocks is the textbook in-place Gauss-Jordan); after the last block
+// the array holds A^-1. Four launches per block: diagonal block, row panel, rank-nb update of
+// everything outside the block's rows and columns (which reads the old column panel and the new
+// row panel), column panel.
+constexpr int kGJB = 32;    // block width
+constexpr int kGJTile = 64;  // update tile (64 x 64 per workgroup, 4 x 4 per thread)
+
+__global__ __launch_bounds__(256) void gjb_diag_kernel(double *A, int n, int k0, int nb) {
+  __shared__ double D[kGJB][kGJB + 1];
+  for (int e = threadIdx.x; e < nb * nb; e += 256) {
+    const int i = e / nb, j = e - i * nb;
+    D[i][j] = A[(long)(k0 + i) * n + k0 + j];
+  }
+  __syncthreads();
+  constexpr int kPer = kGJB * kGJB / 256;
+  for (int k = 0; k < nb; ++k) {
+    const double piv = 1.0 / D[k][k];
+    double v[kPer];
+#pragma unroll
+    for (int m = 0; m < kPer; ++m) {
+      const int e = threadIdx.x + 256 * m, i = e / kGJB, j = e - i * kGJB;
+      double x = 0.0;
+      if (i < nb && j < nb) {
+        if (i == k && j == k) x = piv;
+        else if (i == k) x = D[k][j] * piv;
+        else if (j == k) x = -D[i][k] * piv;
+        else x = D[i][j] - D[i][k] * D[k][j] * piv;
+      }
+      v[m] = x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kPer; ++m) {
+      const int e = threadIdx.x + 256 * m, i = e / kGJB, j = e - i * kGJB;
+      if (i < nb && j < nb) D[i][j] = v[m];
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < nb * nb; e += 256) {
+    const int i = e / nb, j = e - i * nb;
+    A[(long)(k0 + i) * n + k0 + j] = D[i][j];
+  }
+}
+
+// A12 <- A11^-1 A12, one thread per column outside the block
+__global__ __launch_bounds__(256) void gjb_row_kernel(double *A, int n, int k0, int nb) {
+  __shared__ double D[kGJB][kGJB + 1];
+  for (int e = threadIdx.x; e < nb * nb; e += 256) {
+    const int i = e / nb, j = e - i * nb;
+    D[i][j] = A[(long)(k0 + i) * n + k0 + j];
+  }
+  __syncthreads();
   const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= n || j == k) return;
-  A[(long)k * n + j] /= A[(long)k * n + k];
+  if (j >= n || (j >= k0 && j < k0 + nb)) return;
+  double c[kGJB];
+#pragma unroll
+  for (int r = 0; r < kGJB; ++r) c[r] = r < nb ? A[(long)(k0 + r) * n + j] : 0.0;
+#pragma unroll
+  for (int r = 0; r < kGJB; ++r) {
+    if (r >= nb) break;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < kGJB; ++q) s += (q < nb ? D[r][q] : 0.0) * c[q];
+    A[(long)(k0 + r) * n + j] = s;
+  }
 }
-__global__ void gj_update_kernel(double *A, int n, int k) {
-  const long g = (long)blockIdx.x * 256 + threadIdx.x;
-  if (g >= (long)n * n) return;
-  const int i = g / n, j = g % n;
-  if (i == k || j == k) return;
-  A[(long)i * n + j] -= A[(long)i * n + k] * A[(long)k * n + j];
+
+// A22 <- A22 - A21 A12 for rows and columns outside the block (A12 already updated)
+__global__ __launch_bounds__(256) void gjb_update_kernel(double *A, int n, int k0, int nb) {
+  __shared__ double Ct[kGJB][kGJTile + 1], Rt[kGJB][kGJTile + 1];
+  const int i0 = blockIdx.y * kGJTile, j0 = blockIdx.x * kGJTile;
+  for (int e = threadIdx.x; e < kGJTile * kGJB; e += 256) {
+    const int ii = e / kGJB, c = e - ii * kGJB;
+    Ct[c][ii] = (i0 + ii < n && c < nb) ? A[(long)(i0 + ii) * n + k0 + c] : 0.0;
+  }
+  for (int e = threadIdx.x; e < kGJTile * kGJB; e += 256) {
+    const int c = e / kGJTile, jj = e - c * kGJTile;
+    Rt[c][jj] = (j0 + jj < n && c < nb) ? A[(long)(k0 + c) * n + j0 + jj] : 0.0;
+  }
+  __syncthreads();
+  const int ty = threadIdx.x / 16, tx = threadIdx.x - ty * 16;
+  double acc[4][4] = {};
+#pragma unroll 8
+  for (int c = 0; c < kGJB; ++c) {
+    double a[4], b[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      a[m] = Ct[c][ty + 16 * m];
+      b[m] = Rt[c][tx + 16 * m];
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int p = 0; p < 4; ++p) acc[m][p] = fma(a[m], b[p], acc[m][p]);
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int i = i0 + ty + 16 * m;
+    if (i >= n || (i >= k0 && i < k0 + nb)) continue;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int j = j0 + tx + 16 * p;
+      if (j < n && !(j >= k0 && j < k0 + nb)) A[(long)i * n + j] -= acc[m][p];
+    }
+  }
 }
-__global__ void gj_col_kernel(double *A, int n, int k) {
+
+// A21 <- -A21 A11^-1, one thread per row outside the block
+__global__ __launch_bounds__(256) void gjb_col_kernel(double *A, int n, int k0, int nb) {
+  __shared__ double D[kGJB][kGJB + 1];
+  for (int e = threadIdx.x; e < nb * nb; e += 256) {
+    const int i = e / nb, j = e - i * nb;
+    D[i][j] = A[(long)(k0 + i) * n + k0 + j];
+  }
+  __syncthreads();
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const double piv = 1.0 / A[(long)k * n + k];
-  if (i == k) return;
-  A[(long)i * n + k] = -A[(long)i * n + k] * piv;
+  if (i >= n || (i >= k0 && i < k0 + nb)) return;
+  double r[kGJB];
+#pragma unroll
+  for (int q = 0; q < kGJB; ++q) r[q] = q < nb ? A[(long)i * n + k0 + q] : 0.0;
+#pragma unroll
+  for (int c = 0; c < kGJB; ++c) {
+    if (c >= nb) break;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < kGJB; ++q) s += r[q] * (q < nb ? D[q][c] : 0.0);
+    A[(long)i * n + k0 + c] = -s;
+  }
 }
-__global__ void gj_pivot_kernel(double *A, int n, int k) { A[(long)k * n + k] = 1.0 / A[(long)k * n + k]; }
 
 int launch_gauss_jordan_inverse(double *A, int n, hipStream_t s) {
-  const int b1 = (n + 255) / 256;
-  const long b2 = ((long)n * n + 255) / 256;
-  for (int k = 0; k < n; ++k) {
-    hipLaunchKernelGGL(gj_row_kernel, dim3(b1), dim3(256), 0, s, A, n, k);
-    hipLaunchKernelGGL(gj_update_kernel, dim3(b2), dim3(256), 0, s, A, n, k);
-    hipLaunchKernelGGL(gj_col_kernel, dim3(b1), dim3(256), 0, s, A, n, k);
-    hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(1), 0, s, A, n, k);
+  const int b1 = (n + 255) / 256, bt = (n + kGJTile - 1) / kGJTile;
+  for (int k0 = 0; k0 < n; k0 += kGJB) {
+    const int nb = std::min(kGJB, n - k0);
+    hipLaunchKernelGGL(gjb_diag_kernel, dim3(1), dim3(256), 0, s, A, n, k0, nb);
+    hipLaunchKernelGGL(gjb_row_kernel, dim3(b1), dim3(256), 0, s, A, n, k0, nb);
+    hipLaunchKernelGGL(gjb_update_kernel, dim3(bt, bt), dim3(256), 0, s, A, n, k0, nb);
+    hipLaunchKernelGGL(gjb_col_kernel, dim3(b1), dim3(256), 0, s, A, n, k0, nb);
   }
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
+
+template int launch_coarse_matrix<double>(const OperatorData<double> &, double, const double *, double, double *,
+                                          hipStream_t);
+template int launch_coarse_matrix<float>(const OperatorData<float> &, double, const double *, double, double *,
+                                         hipStream_t);
 
 }  // namespace mpdg

@@ -2,8 +2,10 @@
 // points, cell averages for the multigrid levels, and the boundary-face terms.
 //
 // These run once per time step or once per Newton iteration, not once per
-// Krylov iteration, so they favour simple indexing over tiling.
-#include "kernels.hpp"
+// Krylov iteration. The per-cell tensor work (k(T) at the quadrature points, the
+// diagonal, cell averages) is sum-factorized through LDS; the rest favours simple
+// indexing over tiling.
+#include "blas.hpp"
 
 namespace mpdg {
 
@@ -36,88 +38,282 @@ __device__ __forceinline__ Number law_value(int law, const Number *p, Number T) 
   return p[0] + (p[1] - p[0]) * (Number(3) - Number(2) * fs) * fs * fs;
 }
 
-// conductivity at every quadrature point of every cell (for KAPPA_LAW: k(T_q))
+// Cell-local tensor kernels. A 256-thread workgroup holds kCellsPerBlock<N1> cells with N1^2
+// threads per cell, one per line of the cell's N1^3 tensor; a contraction with a 1D matrix
+// along each direction is one sweep in registers and one transpose through LDS, so a cell
+// costs 3 N1^4 operations instead of the N1^6 of an element-by-element sum.
+template <int N1>
+constexpr int kCellsPerBlock = 256 / (N1 * N1);
+
+template <int N1, typename Number>
+__device__ __forceinline__ void cell_of(long cell, int nc0, int nc1, int *c) {
+  c[0] = (int)(cell % nc0);
+  c[1] = (int)((cell / nc0) % nc1);
+  c[2] = (int)(cell / ((long)nc0 * nc1));
+}
+
+// k(T) at the quadrature points of every cell: T interpolated by sum factorization (S along x,
+// y, z), the law evaluated per point. kq != nullptr: k per point, cell-major [cell][qz][qy][qx];
+// kc != nullptr: the quadrature-weighted cell average sum_q w_q k_q instead.
+template <typename Number, int N1>
+__global__ __launch_bounds__(256) void kappa_law_kernel(AuxTables<Number> t, int nc0, int nc1, long ncells, int nd0,
+                                                        int nd1, int law, Number p0, Number p1, Number p2, Number p3,
+                                                        const Number *__restrict__ T, Number *__restrict__ kq,
+                                                        Number *__restrict__ kc) {
+  constexpr int L2 = N1 * N1, NQ = N1 * L2, CPB = kCellsPerBlock<N1>, K = N1 - 1;
+  __shared__ Number buf[2][CPB * NQ];
+  const int lc = threadIdx.x / L2, r = threadIdx.x - lc * L2, a = r / N1, b = r - a * N1;
+  const long cell = (long)blockIdx.x * CPB + lc;
+  const bool in_block = lc < CPB, active = in_block && cell < ncells;
+  Number *B0 = buf[0] + (in_block ? lc : 0) * NQ, *B1 = buf[1] + (in_block ? lc : 0) * NQ;
+  Number u[N1];
+  // x: thread (l = a, j = b) interpolates its row of DoFs
+  if (active) {
+    int c[3];
+    cell_of<N1, Number>(cell, nc0, nc1, c);
+    const long row = (long)K * c[0] + (long)nd0 * ((K * c[1] + b) + (long)nd1 * (K * c[2] + a));
+#pragma unroll
+    for (int i = 0; i < N1; ++i) u[i] = T[row + i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < N1; ++i) u[i] = Number(0);
+  }
+  if (in_block) {
+#pragma unroll
+    for (int q = 0; q < N1; ++q) {
+      Number s = Number(0);
+#pragma unroll
+      for (int i = 0; i < N1; ++i) s += t.S[q][i] * u[i];
+      B0[r * N1 + q] = s;  // [l][j][qx]
+    }
+  }
+  __syncthreads();
+  // y: thread (l = a, qx = b)
+  if (in_block) {
+#pragma unroll
+    for (int j = 0; j < N1; ++j) u[j] = B0[(a * N1 + j) * N1 + b];
+#pragma unroll
+    for (int q = 0; q < N1; ++q) {
+      Number s = Number(0);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) s += t.S[q][j] * u[j];
+      B1[(a * N1 + q) * N1 + b] = s;  // [l][qy][qx]
+    }
+  }
+  __syncthreads();
+  // z: thread (qy = a, qx = b), then the law
+  const Number p[4] = {p0, p1, p2, p3};
+  Number part = Number(0);
+  if (in_block) {
+#pragma unroll
+    for (int l = 0; l < N1; ++l) u[l] = B1[(l * N1 + a) * N1 + b];
+#pragma unroll
+    for (int q = 0; q < N1; ++q) {
+      Number s = Number(0);
+#pragma unroll
+      for (int l = 0; l < N1; ++l) s += t.S[q][l] * u[l];
+      const Number k = law_value<Number>(law, p, s);
+      if (kq && active) kq[cell * NQ + q * L2 + r] = k;
+      part += t.w[q] * k;
+    }
+  }
+  if (!kc) return;  // uniform across the grid: no thread waits below
+  // cell average: the L2 partial sums of a cell through LDS (B0 is free again)
+  if (in_block) B0[r] = part * t.w[a] * t.w[b];
+  __syncthreads();
+  if (active && r == 0) {
+    Number s = Number(0);
+    for (int i = 0; i < L2; ++i) s += B0[i];
+    kc[cell] = s;
+  }
+}
+
+// quadrature-weighted cell averages of per-point values, cell-major [cell][q]
+template <typename Number, int N1>
+__global__ __launch_bounds__(256) void cell_average_qp_kernel(AuxTables<Number> t, long ncells,
+                                                              const Number *__restrict__ kq, Number *__restrict__ kc) {
+  constexpr int L2 = N1 * N1, NQ = N1 * L2, CPB = kCellsPerBlock<N1>;
+  __shared__ Number buf[CPB * L2];
+  const int lc = threadIdx.x / L2, r = threadIdx.x - lc * L2, a = r / N1, b = r - a * N1;
+  const long cell = (long)blockIdx.x * CPB + lc;
+  const bool in_block = lc < CPB, active = in_block && cell < ncells;
+  Number part = Number(0);
+  if (active)
+#pragma unroll
+    for (int q = 0; q < N1; ++q) part += t.w[q] * kq[cell * NQ + q * L2 + r];
+  if (in_block) buf[lc * L2 + r] = part * t.w[a] * t.w[b];
+  __syncthreads();
+  if (active && r == 0) {
+    Number s = Number(0);
+    for (int i = 0; i < L2; ++i) s += buf[lc * L2 + i];
+    kc[cell] = s;
+  }
+}
+
+// conductivity at every quadrature point of every cell, non-law modes
 template <typename Number>
-__global__ __launch_bounds__(256) void kappa_qp_kernel(AuxTables<Number> t, int n1, int nc0, int nc1,
-                                                       long ncells, int nd0, int nd1, int mode, Number kappa,
-                                                       const Number *__restrict__ karr, int law, Number p0,
-                                                       Number p1, Number p2, Number p3,
-                                                       const Number *__restrict__ T, Number *__restrict__ kq) {
-  const int nq = n1 * n1 * n1;
+__global__ __launch_bounds__(256) void kappa_fill_kernel(long nq, long n, int mode, Number kappa,
+                                                         const Number *__restrict__ karr, Number *__restrict__ kq) {
   const long gid = (long)blockIdx.x * 256 + threadIdx.x;
-  if (gid >= ncells * nq) return;
-  const long cell = gid / nq;
-  const int q = gid - cell * nq;
-  const int qx = q % n1, qy = (q / n1) % n1, qz = q / (n1 * n1);
+  if (gid >= n) return;
   Number k = kappa;
   if (mode == KAPPA_QP) k = karr[gid];
-  else if (mode == KAPPA_CELL) k = karr[cell];
-  else if (mode == KAPPA_LAW) {
-    const int cx = cell % nc0, cy = (cell / nc0) % nc1, cz = cell / ((long)nc0 * nc1);
-    const int K = n1 - 1;
-    Number Tq = 0;
-    for (int l = 0; l < n1; ++l)
-      for (int j = 0; j < n1; ++j) {
-        Number r = 0;
-        const long row = (long)K * cx + (long)nd0 * ((K * cy + j) + (long)nd1 * (K * cz + l));
-        for (int i = 0; i < n1; ++i) r += t.S[qx][i] * T[row + i];
-        Tq += t.S[qy][j] * t.S[qz][l] * r;
-      }
-    const Number p[4] = {p0, p1, p2, p3};
-    k = law_value<Number>(law, p, Tq);
-  }
+  else if (mode == KAPPA_CELL) k = karr[gid / nq];
   kq[gid] = k;
+}
+
+template <typename Number, int N1>
+static void launch_kappa_law_n1(const OperatorData<Number> &op, Number *kq, Number *kc, hipStream_t stream) {
+  constexpr int CPB = kCellsPerBlock<N1>;
+  hipLaunchKernelGGL((kappa_law_kernel<Number, N1>), dim3((op.n_cells + CPB - 1) / CPB), dim3(256), 0, stream,
+                     make_aux_tables<Number>(op.degree), op.nc[0], op.nc[1], op.n_cells, op.nd[0], op.nd[1], op.law,
+                     (Number)op.law_p[0], (Number)op.law_p[1], (Number)op.law_p[2], (Number)op.law_p[3], op.T, kq, kc);
+}
+template <typename Number>
+static int launch_kappa_law(const OperatorData<Number> &op, Number *kq, Number *kc, hipStream_t stream) {
+  switch (op.degree) {
+    case 1: launch_kappa_law_n1<Number, 2>(op, kq, kc, stream); break;
+    case 2: launch_kappa_law_n1<Number, 3>(op, kq, kc, stream); break;
+    case 3: launch_kappa_law_n1<Number, 4>(op, kq, kc, stream); break;
+    case 4: launch_kappa_law_n1<Number, 5>(op, kq, kc, stream); break;
+    default: return MPDG_ERR_INVALID;
+  }
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
 template <typename Number>
 int launch_kappa_qp(const OperatorData<Number> &op, Number *kq, hipStream_t stream) {
+  if (op.kappa_mode == KAPPA_LAW) return launch_kappa_law<Number>(op, kq, nullptr, stream);
   const int n1 = op.degree + 1;
-  const long n = op.n_cells * n1 * n1 * n1;
-  hipLaunchKernelGGL(kappa_qp_kernel<Number>, dim3((n + 255) / 256), dim3(256), 0, stream,
-                     make_aux_tables<Number>(op.degree), n1, op.nc[0], op.nc[1], op.n_cells, op.nd[0],
-                     op.nd[1], op.kappa_mode, (Number)op.kappa, op.kappa_arr, op.law, (Number)op.law_p[0],
-                     (Number)op.law_p[1], (Number)op.law_p[2], (Number)op.law_p[3], op.T, kq);
+  const long nq = (long)n1 * n1 * n1, n = op.n_cells * nq;
+  hipLaunchKernelGGL(kappa_fill_kernel<Number>, dim3((n + 255) / 256), dim3(256), 0, stream, nq, n, op.kappa_mode,
+                     (Number)op.kappa, op.kappa_arr, kq);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
-// diag_I += sum_q W_q (c_m vol phi_I^2 + k_q sum_d vol/h_d^2 (d_d phi_I)^2)
 template <typename Number>
-__global__ __launch_bounds__(256) void diagonal_kernel(AuxTables<Number> t, int n1, int nc0, int nc1,
-                                                       long ncells, int nd0, int nd1, Number cm, Number ks0,
-                                                       Number ks1, Number ks2, const Number *__restrict__ kq,
-                                                       const Number *__restrict__ metric,
-                                                       Number *__restrict__ diag) {
-  const int nloc = n1 * n1 * n1;
-  const long gid = (long)blockIdx.x * 256 + threadIdx.x;
-  if (gid >= ncells * nloc) return;
-  const long cell = gid / nloc;
-  const int a = gid - cell * nloc;
-  const int i = a % n1, j = (a / n1) % n1, l = a / (n1 * n1);
-  const int cx = cell % nc0, cy = (cell / nc0) % nc1, cz = cell / ((long)nc0 * nc1);
-  const int K = n1 - 1;
-  Number acc = 0;
-  for (int qz = 0; qz < n1; ++qz)
-    for (int qy = 0; qy < n1; ++qy)
-      for (int qx = 0; qx < n1; ++qx) {
-        const Number W = t.w[qx] * t.w[qy] * t.w[qz];
-        const Number sx = t.S[qx][i], sy = t.S[qy][j], sz = t.S[qz][l];
-        const Number gx = t.G[qx][i], gy = t.G[qy][j], gz = t.G[qz][l];
-        const int q = qx + n1 * (qy + n1 * qz);
-        const Number k = kq[cell * nloc + q];
-        const Number m = sx * sy * sz;
-        if (metric) {
-          const Number *M = metric + cell * 7 * nloc + q;
-          const Number d0 = gx * sy * sz, d1 = sx * gy * sz, d2 = sx * sy * gz;
-          acc += cm * M[0] * m * m +
-                 k * (M[nloc] * d0 * d0 + M[3 * nloc] * d1 * d1 + M[6 * nloc] * d2 * d2 +
-                      Number(2) * (M[2 * nloc] * d0 * d1 + M[4 * nloc] * d0 * d2 + M[5 * nloc] * d1 * d2));
-        } else {
-          acc += W * (cm * m * m + k * (ks0 * gx * gx * sy * sy * sz * sz + ks1 * sx * sx * gy * gy * sz * sz +
-                                        ks2 * sx * sx * sy * sy * gz * gz));
+int launch_cell_average(const OperatorData<Number> &op, Number *kc, hipStream_t stream) {
+  switch (op.kappa_mode) {
+    case KAPPA_CONST: return launch_fill<Number>(kc, (Number)op.kappa, op.n_cells, stream);
+    case KAPPA_CELL:
+      return hipMemcpyAsync(kc, op.kappa_arr, sizeof(Number) * op.n_cells, hipMemcpyDeviceToDevice, stream) ==
+                     hipSuccess
+                 ? MPDG_OK
+                 : MPDG_ERR_DEVICE;
+    case KAPPA_LAW: return launch_kappa_law<Number>(op, nullptr, kc, stream);
+    default: break;
+  }
+  const AuxTables<Number> t = make_aux_tables<Number>(op.degree);
+  switch (op.degree) {
+#define MPDG_CELL_AVG(N1)                                                                                       \
+  case N1 - 1:                                                                                                  \
+    hipLaunchKernelGGL((cell_average_qp_kernel<Number, N1>), dim3((op.n_cells + kCellsPerBlock<N1> - 1) /      \
+                                                                  kCellsPerBlock<N1>),                          \
+                       dim3(256), 0, stream, t, op.n_cells, op.kappa_arr, kc);                                 \
+    break;
+    MPDG_CELL_AVG(2)
+    MPDG_CELL_AVG(3)
+    MPDG_CELL_AVG(4)
+    MPDG_CELL_AVG(5)
+#undef MPDG_CELL_AVG
+    default: return MPDG_ERR_INVALID;
+  }
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+// Diagonal of the cell matrices, summed into the global vector:
+//   diag_I = sum_q c_q^(0) phi_I^2 + sum_(a,b) c_q^(ab) d_a phi_I d_b phi_I.
+// Every term is a tensor product of 1D factors S^2, G^2 or G S, so each is three sum-factorized
+// sweeps of its per-point coefficient. Cartesian cells: c^(0) = c_m vol W, c^(aa) = k vol/h_a^2 W;
+// mapped cells: c^(0) = c_m JxW, c^(ab) = k (JxW B B^T)_ab (cross terms twice).
+template <typename Number>
+__device__ __forceinline__ Number diag_factor(const AuxTables<Number> &t, int code, int q, int i) {
+  const Number s = t.S[q][i], g = t.G[q][i];
+  return code == 0 ? s * s : (code == 1 ? g * g : g * s);
+}
+
+template <typename Number, int N1, bool MAPPED>
+__global__ __launch_bounds__(256) void diagonal_kernel(AuxTables<Number> t, int nc0, int nc1, long ncells, int nd0,
+                                                       int nd1, Number cm, Number ks0, Number ks1, Number ks2,
+                                                       int kmode, Number kappa, const Number *__restrict__ karr,
+                                                       const Number *__restrict__ metric, Number *__restrict__ diag) {
+  constexpr int L2 = N1 * N1, NQ = N1 * L2, CPB = kCellsPerBlock<N1>, K = N1 - 1, NT = MAPPED ? 7 : 4;
+  // term: metric component, 1D factor codes along x, y, z (0: S^2, 1: G^2, 2: G S)
+  constexpr int comp[7] = {0, 1, 3, 6, 2, 4, 5};
+  constexpr int fx[7] = {0, 1, 0, 0, 2, 2, 0}, fy[7] = {0, 0, 1, 0, 2, 0, 2}, fz[7] = {0, 0, 0, 1, 0, 2, 2};
+  __shared__ Number buf[2][CPB * NQ];
+  const int lc = threadIdx.x / L2, r = threadIdx.x - lc * L2, a = r / N1, b = r - a * N1;
+  const long cell = (long)blockIdx.x * CPB + lc;
+  const bool in_block = lc < CPB, active = in_block && cell < ncells;
+  Number *B0 = buf[0] + (in_block ? lc : 0) * NQ, *B1 = buf[1] + (in_block ? lc : 0) * NQ;
+  const Number ks[3] = {ks0, ks1, ks2};
+  // conductivity along this thread's qz line: constant, per point, or per cell
+  Number kk[N1];
+#pragma unroll
+  for (int qz = 0; qz < N1; ++qz)
+    kk[qz] = !active ? Number(0) : (kmode == 0 ? kappa : (kmode == 1 ? karr[cell * NQ + qz * L2 + r] : karr[cell]));
+  Number d[N1], u[N1];
+#pragma unroll
+  for (int i = 0; i < N1; ++i) d[i] = Number(0);
+#pragma unroll
+  for (int term = 0; term < NT; ++term) {
+    // z: thread (qy = a, qx = b) forms the coefficient along qz
+    if (in_block) {
+#pragma unroll
+      for (int qz = 0; qz < N1; ++qz) {
+        Number c = Number(0);
+        if (active) {
+          if (MAPPED) {
+            const Number m = metric[(cell * 7 + comp[term]) * NQ + qz * L2 + r];
+            c = term == 0 ? cm * m : (term >= 4 ? Number(2) : Number(1)) * kk[qz] * m;
+          } else {
+            const Number W = t.w[qz] * t.w[a] * t.w[b];
+            c = term == 0 ? cm * W : ks[term - 1] * kk[qz] * W;
+          }
         }
+        u[qz] = c;
       }
-  const long g = (long)(K * cx + i) + (long)nd0 * ((K * cy + j) + (long)nd1 * (K * cz + l));
-  atomicAdd(diag + g, acc);
+#pragma unroll
+      for (int l = 0; l < N1; ++l) {
+        Number s = Number(0);
+#pragma unroll
+        for (int qz = 0; qz < N1; ++qz) s += diag_factor(t, fz[term], qz, l) * u[qz];
+        B0[(l * N1 + a) * N1 + b] = s;  // [l][qy][qx]
+      }
+    }
+    __syncthreads();
+    // y: thread (l = a, qx = b)
+    if (in_block) {
+#pragma unroll
+      for (int qy = 0; qy < N1; ++qy) u[qy] = B0[(a * N1 + qy) * N1 + b];
+#pragma unroll
+      for (int j = 0; j < N1; ++j) {
+        Number s = Number(0);
+#pragma unroll
+        for (int qy = 0; qy < N1; ++qy) s += diag_factor(t, fy[term], qy, j) * u[qy];
+        B1[(a * N1 + j) * N1 + b] = s;  // [l][j][qx]
+      }
+    }
+    __syncthreads();
+    // x: thread (l = a, j = b) accumulates its row of DoFs
+    if (in_block) {
+#pragma unroll
+      for (int qx = 0; qx < N1; ++qx) u[qx] = B1[r * N1 + qx];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        Number s = Number(0);
+#pragma unroll
+        for (int qx = 0; qx < N1; ++qx) s += diag_factor(t, fx[term], qx, i) * u[qx];
+        d[i] += s;
+      }
+    }
+  }
+  if (!active) return;
+  int c[3];
+  cell_of<N1, Number>(cell, nc0, nc1, c);
+  const long row = (long)K * c[0] + (long)nd0 * ((K * c[1] + b) + (long)nd1 * (K * c[2] + a));
+#pragma unroll
+  for (int i = 0; i < N1; ++i) atomicAdd(diag + row + i, d[i]);
 }
 
 template <typename Number>
@@ -254,41 +450,52 @@ int launch_boundary(const OperatorData<double> &op, int mode, const double *flux
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
+template <typename Number, int N1>
+static void launch_diagonal_n1(const OperatorData<Number> &op, int kmode, const Number *karr, Number *diag,
+                               hipStream_t stream) {
+  constexpr int CPB = kCellsPerBlock<N1>;
+  const double vol = op.metric ? 1.0 : op.h[0] * op.h[1] * op.h[2];
+  const dim3 grid((op.n_cells + CPB - 1) / CPB);
+  const AuxTables<Number> t = make_aux_tables<Number>(op.degree);
+  const Number cm = (Number)(op.c_mass * vol), ks0 = (Number)(vol / (op.h[0] * op.h[0])),
+               ks1 = (Number)(vol / (op.h[1] * op.h[1])), ks2 = (Number)(vol / (op.h[2] * op.h[2]));
+  if (op.metric)
+    hipLaunchKernelGGL((diagonal_kernel<Number, N1, true>), grid, dim3(256), 0, stream, t, op.nc[0], op.nc[1],
+                       op.n_cells, op.nd[0], op.nd[1], cm, ks0, ks1, ks2, kmode, (Number)op.kappa, karr, op.metric,
+                       diag);
+  else
+    hipLaunchKernelGGL((diagonal_kernel<Number, N1, false>), grid, dim3(256), 0, stream, t, op.nc[0], op.nc[1],
+                       op.n_cells, op.nd[0], op.nd[1], cm, ks0, ks1, ks2, kmode, (Number)op.kappa, karr, op.metric,
+                       diag);
+}
+
+// kq: scratch of n_cells (k+1)^3 values, used for a conductivity law
 template <typename Number>
 int launch_diagonal(const OperatorData<Number> &op_in, Number *diag, Number *kq, hipStream_t stream) {
   const OperatorData<Number> op = kernel_view(op_in);
   if (op.metric == nullptr && op_in.metric != nullptr) return MPDG_ERR_INVALID;
-  const int n1 = op.degree + 1;
-  int rc = launch_kappa_qp<Number>(op, kq, stream);
-  if (rc) return rc;
-  hipMemsetAsync(diag, 0, sizeof(Number) * op.n_dofs, stream);
-  const double vol = op.metric ? 1.0 : op.h[0] * op.h[1] * op.h[2];
-  const long n = op.n_cells * n1 * n1 * n1;
-  hipLaunchKernelGGL(diagonal_kernel<Number>, dim3((n + 255) / 256), dim3(256), 0, stream,
-                     make_aux_tables<Number>(op.degree), n1, op.nc[0], op.nc[1], op.n_cells, op.nd[0], op.nd[1],
-                     (Number)(op.c_mass * vol), (Number)(vol / (op.h[0] * op.h[0])),
-                     (Number)(vol / (op.h[1] * op.h[1])), (Number)(vol / (op.h[2] * op.h[2])), kq, op.metric, diag);
+  int kmode = 0;
+  const Number *karr = nullptr;
+  if (op.kappa_mode == KAPPA_QP || op.kappa_mode == KAPPA_CELL) {
+    kmode = op.kappa_mode == KAPPA_QP ? 1 : 2;
+    karr = op.kappa_arr;
+  } else if (op.kappa_mode == KAPPA_LAW) {
+    const int rc = launch_kappa_law<Number>(op, kq, nullptr, stream);
+    if (rc) return rc;
+    kmode = 1;
+    karr = kq;
+  }
+  if (hipMemsetAsync(diag, 0, sizeof(Number) * op.n_dofs, stream) != hipSuccess) return MPDG_ERR_DEVICE;
+  switch (op.degree) {
+    case 1: launch_diagonal_n1<Number, 2>(op, kmode, karr, diag, stream); break;
+    case 2: launch_diagonal_n1<Number, 3>(op, kmode, karr, diag, stream); break;
+    case 3: launch_diagonal_n1<Number, 4>(op, kmode, karr, diag, stream); break;
+    case 4: launch_diagonal_n1<Number, 5>(op, kmode, karr, diag, stream); break;
+    default: return MPDG_ERR_INVALID;
+  }
   if (op.dmask)
     hipLaunchKernelGGL(diag_dirichlet_kernel<Number>, dim3((op.n_dofs + 255) / 256), dim3(256), 0, stream, diag,
                        op.nd[0], op.nd[1], op.nd[2], op.dmask);
-  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
-}
-
-// quadrature-weighted cell averages of the conductivity
-template <typename Number>
-__global__ __launch_bounds__(256) void cell_average_kernel(AuxTables<Number> t, int n1, long ncells,
-                                                           const Number *__restrict__ kq, Number *__restrict__ kc) {
-  const long c = (long)blockIdx.x * 256 + threadIdx.x;
-  if (c >= ncells) return;
-  const int nq = n1 * n1 * n1;
-  Number s = 0;
-  for (int q = 0; q < nq; ++q) s += t.w[q % n1] * t.w[(q / n1) % n1] * t.w[q / (n1 * n1)] * kq[c * nq + q];
-  kc[c] = s;
-}
-template <typename Number>
-int launch_cell_average(const OperatorData<Number> &op, const Number *kq, Number *kc, hipStream_t stream) {
-  hipLaunchKernelGGL(cell_average_kernel<Number>, dim3((op.n_cells + 255) / 256), dim3(256), 0, stream,
-                     make_aux_tables<Number>(op.degree), op.degree + 1, op.n_cells, kq, kc);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
@@ -429,8 +636,8 @@ template int launch_kappa_qp<double>(const OperatorData<double> &, double *, hip
 template int launch_kappa_qp<float>(const OperatorData<float> &, float *, hipStream_t);
 template int launch_diagonal<double>(const OperatorData<double> &, double *, double *, hipStream_t);
 template int launch_diagonal<float>(const OperatorData<float> &, float *, float *, hipStream_t);
-template int launch_cell_average<double>(const OperatorData<double> &, const double *, double *, hipStream_t);
-template int launch_cell_average<float>(const OperatorData<float> &, const float *, float *, hipStream_t);
+template int launch_cell_average<double>(const OperatorData<double> &, double *, hipStream_t);
+template int launch_cell_average<float>(const OperatorData<float> &, float *, hipStream_t);
 template int launch_coarsen_cells<double>(const int *, const double *, double *, hipStream_t);
 template int launch_coarsen_cells<float>(const int *, const float *, float *, hipStream_t);
 

@@ -296,7 +296,7 @@ template <typename T>
 struct GMGPrecond : mpdg_precond {
   mpdg_gmg_params prm{};
   std::vector<std::unique_ptr<Level<T>>> lev;  // 0 = coarsest
-  DeviceArray<double> coarse_inv, cvec_in, cvec_out, cvec_part, dscratch, eig_sc, eig_hist, kcell64;
+  DeviceArray<double> coarse_inv, cvec_in, cvec_out, cvec_part, eig_sc, eig_hist, kcell64, kcell_pp[2];
   ReductionScratch red;
   int nc_coarse_dofs = 0;
 
@@ -386,8 +386,8 @@ struct GMGPrecond : mpdg_precond {
     MPDG_TRY(cvec_in.resize(nc_coarse_dofs));
     MPDG_TRY(cvec_out.resize(nc_coarse_dofs));
     MPDG_TRY(cvec_part.resize(dense_gemv_scratch((int)nc_coarse_dofs)));
-    MPDG_TRY(dscratch.resize(kMaxN1 * kMaxN1 + kMaxN1));
     MPDG_TRY(kcell64.resize(f.n_cells));
+    for (auto &k : kcell_pp) MPDG_TRY(k.resize(std::max<long>(1, f.n_cells / 8)));
     return MPDG_OK;
   }
 
@@ -475,38 +475,26 @@ struct GMGPrecond : mpdg_precond {
       MPDG_TRY(launch_scale_metric<T>(top.od, top.kq_scratch.get(), top.metric_k.get(), s));
       top.od.metric_k = top.metric_k.get();
     }
-    // quadrature-weighted cell averages of the fine conductivity
-    if (lev.size() > 1) MPDG_TRY(launch_op_cell_average(f, kcell64.get(), dscratch.get(), s));
-    // the coarsened averages level by level (FP64 on the host is plenty for these sizes)
+    // quadrature-weighted cell averages of the fine conductivity, coarsened level by level
     if (lev.size() > 1) {
-      std::vector<double> kc((size_t)f.n_cells);
-      if (hipMemcpyAsync(kc.data(), kcell64.get(), kc.size() * sizeof(double), hipMemcpyDeviceToHost, s) !=
-          hipSuccess)
-        return MPDG_ERR_DEVICE;
-      if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
-      int nc[3] = {f.nc[0], f.nc[1], f.nc[2]};
+      MPDG_TRY(launch_cell_average<double>(f, kcell64.get(), s));
+      const double *kf = kcell64.get();
+      int nc[3] = {f.nc[0], f.nc[1], f.nc[2]}, pp = 0;
       for (int l = (int)lev.size() - 2; l >= 0; --l) {
         Level<T> &L = *lev[l];
         if (L.od.nc[0] != nc[0]) {
-          std::vector<double> k2((size_t)L.od.n_cells);
-          for (int z = 0; z < L.od.nc[2]; ++z)
-            for (int y = 0; y < L.od.nc[1]; ++y)
-              for (int x = 0; x < L.od.nc[0]; ++x) {
-                double sum = 0.0;
-                for (int c = 0; c < 8; ++c)
-                  sum += kc[(2 * x + (c & 1)) + (size_t)nc[0] * ((2 * y + ((c >> 1) & 1)) + (size_t)nc[1] * (2 * z + (c >> 2)))];
-                k2[x + (size_t)L.od.nc[0] * (y + (size_t)L.od.nc[1] * z)] = 0.125 * sum;
-              }
-          kc.swap(k2);
+          double *kc = kcell_pp[pp].get();
+          pp ^= 1;
+          MPDG_TRY(launch_coarsen_cells<double>(nc, kf, kc, s));
+          kf = kc;
           for (int d = 0; d < 3; ++d) nc[d] = L.od.nc[d];
         }
-        MPDG_TRY(upload<T>(L.kcell, kc));
+        MPDG_TRY(launch_convert<double, T>(kf, L.kcell.get(), L.od.n_cells, s));
         L.od.kappa_arr = L.kcell.get();
-        if (l == 0) MPDG_TRY(assemble_coarse(kc, s));
+        if (l == 0) MPDG_TRY(assemble_coarse(kf, s));
       }
     } else {
-      std::vector<double> kc;
-      MPDG_TRY(assemble_coarse(kc, s));
+      MPDG_TRY(assemble_coarse(nullptr, s));
     }
     // inverse diagonals and eigenvalue estimates
     for (size_t l = 1; l < lev.size(); ++l) {
@@ -578,99 +566,19 @@ struct GMGPrecond : mpdg_precond {
     }
   }
 
-  // dense coarse matrix from element matrices (FP64), Dirichlet rows/columns identity, inverted on the device
-  int assemble_coarse(const std::vector<double> &kcell, hipStream_t s) {
+  // dense coarse matrix from element matrices (FP64) with the cell-averaged conductivities kc
+  // (device), Dirichlet rows/columns identity, inverted on the device
+  int assemble_coarse(const double *kc, hipStream_t s) {
     const OperatorData<T> &od = lev[0]->od;
     const auto &f = op->d64;
-    const int k = od.degree, n1 = k + 1, nloc = n1 * n1 * n1;
-    const long n = od.n_dofs;
-    FE1D fe(k);
-    std::vector<double> A((size_t)n * n, 0.0), Ae((size_t)nloc * nloc);
-    const double vol = od.h[0] * od.h[1] * od.h[2];
-    const double cm = f.c_mass * vol, ks[3] = {vol / (od.h[0] * od.h[0]), vol / (od.h[1] * od.h[1]),
-                                                vol / (od.h[2] * od.h[2])};
-    std::vector<double> kc = kcell;
-    if (kc.empty()) {
+    if (!kc && f.kappa_mode != KAPPA_CONST) {
       // single-level hierarchy: the fine operator itself is the coarse operator
-      kc.assign((size_t)od.n_cells, f.kappa);
-      if (f.kappa_mode != KAPPA_CONST) {
-        MPDG_TRY(launch_op_cell_average(f, kcell64.get(), dscratch.get(), s));
-        if (hipMemcpyAsync(kc.data(), kcell64.get(), kc.size() * sizeof(double), hipMemcpyDeviceToHost, s) !=
-            hipSuccess)
-          return MPDG_ERR_DEVICE;
-        if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
-      }
+      MPDG_TRY(launch_cell_average<double>(f, kcell64.get(), s));
+      kc = kcell64.get();
     }
-    std::vector<double> met;
-    if (od.metric) {
-      std::vector<T> mt((size_t)od.n_cells * 7 * nloc);
-      if (hipMemcpy(mt.data(), od.metric, mt.size() * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess)
-        return MPDG_ERR_DEVICE;
-      met.assign(mt.begin(), mt.end());
-    }
-    for (int cz = 0; cz < od.nc[2]; ++cz)
-      for (int cy = 0; cy < od.nc[1]; ++cy)
-        for (int cx = 0; cx < od.nc[0]; ++cx) {
-          const size_t cell = cx + (size_t)od.nc[0] * (cy + (size_t)od.nc[1] * cz);
-          const double kap = kc[cell];
-          std::fill(Ae.begin(), Ae.end(), 0.0);
-          for (int qz = 0; qz < n1; ++qz)
-            for (int qy = 0; qy < n1; ++qy)
-              for (int qx = 0; qx < n1; ++qx) {
-                const double W = fe.qw[qx] * fe.qw[qy] * fe.qw[qz];
-                const int q = qx + n1 * (qy + n1 * qz);
-                double Gm[3][3] = {{ks[0] * W, 0, 0}, {0, ks[1] * W, 0}, {0, 0, ks[2] * W}};
-                double mw = cm * W;
-                if (od.metric) {
-                  const double *M = met.data() + cell * 7 * nloc + q;
-                  mw = f.c_mass * M[0];
-                  Gm[0][0] = M[nloc];
-                  Gm[0][1] = Gm[1][0] = M[2 * nloc];
-                  Gm[1][1] = M[3 * nloc];
-                  Gm[0][2] = Gm[2][0] = M[4 * nloc];
-                  Gm[1][2] = Gm[2][1] = M[5 * nloc];
-                  Gm[2][2] = M[6 * nloc];
-                }
-                for (int I = 0; I < nloc; ++I) {
-                  const int i0 = I % n1, i1 = (I / n1) % n1, i2 = I / (n1 * n1);
-                  const double pI = fe.S[qx][i0] * fe.S[qy][i1] * fe.S[qz][i2];
-                  const double gI[3] = {fe.G[qx][i0] * fe.S[qy][i1] * fe.S[qz][i2],
-                                        fe.S[qx][i0] * fe.G[qy][i1] * fe.S[qz][i2],
-                                        fe.S[qx][i0] * fe.S[qy][i1] * fe.G[qz][i2]};
-                  for (int J = 0; J < nloc; ++J) {
-                    const int j0 = J % n1, j1 = (J / n1) % n1, j2 = J / (n1 * n1);
-                    const double pJ = fe.S[qx][j0] * fe.S[qy][j1] * fe.S[qz][j2];
-                    const double gJ[3] = {fe.G[qx][j0] * fe.S[qy][j1] * fe.S[qz][j2],
-                                          fe.S[qx][j0] * fe.G[qy][j1] * fe.S[qz][j2],
-                                          fe.S[qx][j0] * fe.S[qy][j1] * fe.G[qz][j2]};
-                    double gg = 0.0;
-                    for (int a = 0; a < 3; ++a)
-                      for (int b = 0; b < 3; ++b) gg += gI[a] * Gm[a][b] * gJ[b];
-                    Ae[(size_t)I * nloc + J] += mw * pI * pJ + kap * gg;
-                  }
-                }
-              }
-          std::vector<long> idx(nloc);
-          for (int I = 0; I < nloc; ++I)
-            idx[I] = (long)(k * cx + I % n1) + (long)od.nd[0] * ((k * cy + (I / n1) % n1) + (long)od.nd[1] * (k * cz + I / (n1 * n1)));
-          for (int I = 0; I < nloc; ++I)
-            for (int J = 0; J < nloc; ++J) A[(size_t)idx[I] * n + idx[J]] += Ae[(size_t)I * nloc + J];
-        }
-    if (od.dmask)
-      for (long i = 0; i < n; ++i) {
-        const int ix = i % od.nd[0], iy = (i / od.nd[0]) % od.nd[1], iz = i / ((long)od.nd[0] * od.nd[1]);
-        const int m = od.dmask;
-        const bool con = ((m & 1) && ix == 0) || ((m & 2) && ix == od.nd[0] - 1) || ((m & 4) && iy == 0) ||
-                         ((m & 8) && iy == od.nd[1] - 1) || ((m & 16) && iz == 0) || ((m & 32) && iz == od.nd[2] - 1);
-        if (!con) continue;
-        for (long j = 0; j < n; ++j) A[(size_t)i * n + j] = A[(size_t)j * n + i] = 0.0;
-        A[(size_t)i * n + i] = 1.0;
-      }
-    if (hipMemcpyAsync(coarse_inv.get(), A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
-      return MPDG_ERR_DEVICE;
-    MPDG_TRY(launch_gauss_jordan_inverse(coarse_inv.get(), (int)n, s));
+    MPDG_TRY(launch_coarse_matrix<T>(od, f.c_mass, kc, f.kappa, coarse_inv.get(), s));
     // the inverse of a symmetric matrix is symmetric: row-major == column-major for the gemv
-    return hipStreamSynchronize(s) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+    return launch_gauss_jordan_inverse(coarse_inv.get(), (int)od.n_dofs, s);
   }
 
   int prolongate_add(Level<T> &F, const Level<T> &C, hipStream_t s) {
@@ -863,6 +771,8 @@ struct mpdg_heat_operation {
   double *z_record = nullptr;        // caller's vector for the residual input of each step
   DeviceArray<double> zbuf, invd;    // residual input and inverse diagonal (learned predictor)
   double invd_dt = 0.0;              // time step the cached inverse diagonal was computed for
+  DeviceArray<double> kq_step;       // k(T_n) per quadrature point, frozen for a linear step
+  bool law_frozen = false;
   int fell_back = 0;
   double rel_res0 = 0.0, rel_res = 0.0;
 };
@@ -1140,7 +1050,7 @@ static int guess_feature(mpdg_heat_operation *h, const double *b, const double *
   mpdg_heat_op *op = h->op;
   const long n = op->n_dofs();
   // the inverse diagonal is kept while dt is unchanged and the conductivity does not follow T
-  const bool t_dependent = op->d64.kappa_mode == KAPPA_LAW || op->d64.robin_mask != 0;
+  const bool t_dependent = op->d64.kappa_mode == KAPPA_LAW || h->law_frozen || op->d64.robin_mask != 0;
   if (h->invd.n != (size_t)n || t_dependent || h->invd_dt != op->dt) {
     MPDG_TRY(h->invd.resize(n));
     MPDG_TRY(op->inverse_diagonal(h->invd.get(), s));
@@ -1177,6 +1087,35 @@ static int learned_guess(mpdg_heat_operation *h, double dt, const double *b, dou
   return MPDG_OK;
 }
 
+// A conductivity law k(T) frozen at T_n for one linear step: evaluated once per quadrature
+// point and applied as per-point data, so the step's many applies read k instead of
+// interpolating T and evaluating the law at every point of every apply. Restores the law on
+// destruction (every return path of the step).
+struct FrozenLaw {
+  mpdg_heat_operation *h;
+  int mode = -1;
+  const double *arr = nullptr;
+  FrozenLaw(mpdg_heat_operation *h_) : h(h_) {}
+  int freeze(hipStream_t s) {
+    OperatorData<double> &od = h->op->d64;
+    if (od.kappa_mode != KAPPA_LAW || od.metric) return MPDG_OK;  // mapped cells fold k into the metric
+    MPDG_TRY(h->kq_step.resize((size_t)od.n_cells * (od.degree + 1) * (od.degree + 1) * (od.degree + 1)));
+    MPDG_TRY(launch_kappa_qp<double>(od, h->kq_step.get(), s));
+    mode = od.kappa_mode;
+    arr = od.kappa_arr;
+    od.kappa_mode = KAPPA_QP;
+    od.kappa_arr = h->kq_step.get();
+    h->law_frozen = true;
+    return MPDG_OK;
+  }
+  ~FrozenLaw() {
+    if (mode < 0) return;
+    h->op->d64.kappa_mode = mode;
+    h->op->d64.kappa_arr = arr;
+    h->law_frozen = false;
+  }
+};
+
 // One linear implicit step with the coefficient frozen at T_n:
 // J(T_n) T_{n+1} = b,  b = M (c_m T_n + q) + surface flux, from the predicted T.
 static int linear_step(mpdg_heat_operation *h, double dt, double *T, const double *T_old, int *lin_iters,
@@ -1187,6 +1126,8 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
   if (op->d64.kappa_mode == KAPPA_LAW) op->d64.T = T_old;
   if (op->d64.robin_mask) op->T_robin = T_old;
   const bool coeff_changes = op->d64.kappa_mode == KAPPA_LAW;
+  FrozenLaw frozen(h);
+  MPDG_TRY(frozen.freeze(s));
   if (h->pc && (coeff_changes || dt != h->dt_precond)) {
     MPDG_TRY(h->pc->update(s));
     h->dt_precond = dt;

@@ -15,7 +15,11 @@
 
 namespace mpdg {
 
-int launch_op_cell_average(const OperatorData<double> &op, double *kc, double *table_scratch, hipStream_t stream);
+// dense coarse-grid matrix (row-major, n_dofs^2) of the level `od` with cell conductivities kc
+// (nullptr: kconst everywhere); constrained DoFs get identity rows and columns
+template <typename T>
+int launch_coarse_matrix(const OperatorData<T> &od, double c_mass, const double *kc, double kconst, double *A,
+                         hipStream_t s);
 int launch_gauss_jordan_inverse(double *A, int n, hipStream_t s);
 
 // device buffer owned by the library

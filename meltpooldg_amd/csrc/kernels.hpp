@@ -144,8 +144,9 @@ int launch_metric(const OperatorData<Number> &op, const double *coords, Number *
 int launch_surface_flux_field(const OperatorData<double> &op, const double *p6, double *out, hipStream_t s);
 template <typename Number>
 int launch_kappa_qp(const OperatorData<Number> &op, Number *kq, hipStream_t stream);
+// quadrature-weighted cell averages sum_q w_q k_q of the conductivity (any mode)
 template <typename Number>
-int launch_cell_average(const OperatorData<Number> &op, const Number *kq, Number *kcell, hipStream_t stream);
+int launch_cell_average(const OperatorData<Number> &op, Number *kcell, hipStream_t stream);
 template <typename Number>
 int launch_coarsen_cells(const int *nc_fine, const Number *kf, Number *kc, hipStream_t stream);
 

@@ -130,3 +130,37 @@ def test_heat_steps_golden_norms(cuda, case):
         heat.solve(dt)
         norms.append(op.l2_norm(heat.solution()))
     assert ["%.5e" % v for v in norms] == ["%.5e" % v for v in GOLDEN[case]]
+
+
+@pytest.mark.parametrize("precond", ["gmg32", "gmg64"])
+def test_linear_step_with_conductivity_law(cuda, precond):
+    """Linear laser-scan steps with k(T) frozen at T_n: each step solves
+    (c_m M + K_{k(T_n)}) T_{n+1} = c_m M T_n + flux, checked with an independent operator that
+    evaluates the law at every quadrature point inside the apply."""
+    torch = cuda
+    import meltpooldg_amd as M
+    from meltpooldg_amd.application import LaserScan
+    from tests.helpers import tanh_law_params
+    k, cells, dt = 3, 6, 2e-3
+    law = (M.LAW_TANH, tanh_law_params())
+    case = LaserScan(k, cells, dt=dt, law=law, precond=precond, predictor=M.PREDICTOR_LINEAR, rel_tol=1e-11,
+                     n_cells_coarse=3)
+    X = O.Operator(k, (cells,) * 3, (1.0, 1.0, 1.0)).node_coords()
+    T0 = 1.0 + 0.4 * np.sin(2 * np.pi * X[0]) * np.cos(np.pi * X[1]) + 0.1 * X[2]
+    case.heat.set_initial_condition(torch.tensor(T0, device="cuda"))
+    ref = M.HeatDiffuseMultiPhaseOperator(k, (cells,) * 3, (1.0, 1.0, 1.0), rho_cp=1.0)
+    ref.reset_time_increment(dt)
+    for _ in range(3):
+        case.step()
+        T_old, T = case.heat.old_solution().clone(), case.solution().clone()
+        ref.set_conductivity_law(law[0], law[1], T_old)
+        ref.set_surface_flux(case.peak, case.sigma, case.x0, case.y0, case.speed, case.t)
+        b = torch.zeros_like(T)
+        ref.create_rhs(b, torch.zeros_like(T), T_old)
+        r = torch.empty_like(T)
+        ref.vmult(r, T)
+        torch.cuda.synchronize()
+        assert float(torch.linalg.norm(b - r)) <= 1e-9 * float(torch.linalg.norm(b))
+    # the conductivity varies over the field, so a constant-k solve would not pass the check
+    kq = np.array([O.kappa_law(law[0], law[1], t)[0] for t in T_old.cpu().numpy()[::7]])
+    assert kq.max() - kq.min() > 0.3
