@@ -191,6 +191,31 @@ inline void kappa_law(int law, const double *p, double T, double &k, double &dk)
   }
 }
 
+// Material of the full tangent: constant (liquid values) or the liquid/solid mushy zone
+// (material.templates.hpp:296-380): every property is the cubic blend of its liquid and
+// solid values in the solid fraction fs = clamp((T_l - T)/(T_l - T_s)) (:546-559, :765-778),
+// its T-derivative -(1/(T_l - T_s)) d blend/d fs, zero outside the mushy interval (:585-598);
+// rho c_p = rho * c_p, d(rho c_p)/dT = d c_p/dT rho + d rho/dT c_p
+// (heat_diffuse_operator.cpp:1185-1193).
+inline void material(const orc_tangent &tg, double T, double &rc, double &k, double &drc, double &dk) {
+  if (tg.transition == 0) {
+    rc = tg.rho[0] * tg.cp[0];
+    k = tg.k[0];
+    drc = dk = 0.0;
+    return;
+  }
+  const double inv = 1.0 / (tg.T_liquidus - tg.T_solidus);
+  const double fs = std::min(1.0, std::max(0.0, (tg.T_liquidus - T) * inv));
+  const bool pure = fs == 0.0 || fs == 1.0;
+  auto blend = [&](const double *x) { return fs == 0.0 ? x[0] : (fs == 1.0 ? x[1] : x[0] + (x[1] - x[0]) * (3.0 * fs * fs - 2.0 * fs * fs * fs)); };
+  auto dblend = [&](const double *x) { return pure ? 0.0 : -inv * (x[1] - x[0]) * (6.0 * fs - 6.0 * fs * fs); };
+  const double rho = blend(tg.rho), cp = blend(tg.cp);
+  k = blend(tg.k);
+  dk = dblend(tg.k);
+  rc = rho * cp;
+  drc = dblend(tg.cp) * rho + dblend(tg.rho) * cp;
+}
+
 // ---------------------------------------------------------------------------
 // Sum factorisation on one cell (scalar), n = n1 points per direction.
 // Array layout a[x + n*(y + n*z)].
@@ -318,8 +343,69 @@ void cell_kappa(const orc_op *op, const FE1D &fe, const Mesh &m, int cx, int cy,
 // Local tangent cell operation (heat_diffuse_operator.cpp:957-1086, for the
 // terms active in the benchmark configurations: val = (rho cp/dt) u,
 // flux = k grad u, lines :1034-1036, :1076-1079).
+// T (values, physical gradients) and T_old (values) at the quadrature points of one cell
+void cell_temperature(const orc_op *op, const FE1D &fe, const Mesh &m, int cx, int cy, int cz,
+                      const CellGeom &geo, double *tq, double gT[3][512], double *toq) {
+  const int n = fe.n, nq = n * n * n;
+  double u[512], uo[512], tmp[512], g[3][512];
+  for (int l = 0; l < n; ++l)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const long d = m.cell_dof(cx, cy, cz, i, j, l);
+        u[i + n * (j + n * l)] = op->T[d];
+        uo[i + n * (j + n * l)] = op->tg.T_old ? op->tg.T_old[d] : op->T[d];
+      }
+  eval_values(fe, u, tq, tmp);
+  eval_gradients(fe, tq, g[0], g[1], g[2]);
+  eval_values(fe, uo, toq, tmp);
+  for (int q = 0; q < nq; ++q) {
+    const double(*B)[3] = geo.Jinv[q];
+    for (int b = 0; b < 3; ++b) gT[b][q] = B[0][b] * g[0][q] + B[1][b] * g[1][q] + B[2][b] * g[2][q];
+  }
+}
+
+// Full tangent (heat_diffuse_operator.cpp:1028-1056):
+//   val  = (rho_cp/dt) u + rho_cp v.grad u
+//          [+ (d rho_cp/dT)(T - T_old)/dt u + (d rho_cp/dT)(grad T . v) u]   (solidification)
+//   flux = k grad u  [+ (dk/dT) grad T u]                                    (solidification)
+void local_cell_apply_tangent(const orc_op *op, const FE1D &fe, const Mesh &m, int cx, int cy, int cz,
+                              const double *u, double *out, CellGeom &geo) {
+  const int n = fe.n, nq = n * n * n;
+  const orc_tangent &tg = op->tg;
+  double v[512], tmp[512], g[3][512], tq[512], toq[512], gT[3][512];
+  cell_geometry(fe, m, op->coords, cx, cy, cz, geo);
+  cell_temperature(op, fe, m, cx, cy, cz, geo, tq, gT, toq);
+  eval_values(fe, u, v, tmp);
+  eval_gradients(fe, v, g[0], g[1], g[2]);
+  for (int q = 0; q < nq; ++q) {
+    const double(*B)[3] = geo.Jinv[q];
+    double gp[3], flux[3];
+    for (int b = 0; b < 3; ++b) gp[b] = B[0][b] * g[0][q] + B[1][b] * g[1][q] + B[2][b] * g[2][q];
+    double rc, k, drc, dk;
+    material(tg, tq[q], rc, k, drc, dk);
+    const double *w = tg.velocity;
+    double val = rc / tg.dt * v[q] + rc * (w[0] * gp[0] + w[1] * gp[1] + w[2] * gp[2]);
+    for (int b = 0; b < 3; ++b) flux[b] = k * gp[b];
+    if (tg.do_solidification) {
+      val += drc / tg.dt * (tq[q] - toq[q]) * v[q];
+      val += drc * (gT[0][q] * w[0] + gT[1][q] * w[1] + gT[2][q] * w[2]) * v[q];
+      for (int b = 0; b < 3; ++b) flux[b] += dk * gT[b][q] * v[q];
+    }
+    for (int a = 0; a < 3; ++a)
+      g[a][q] = geo.JxW[q] * (B[a][0] * flux[0] + B[a][1] * flux[1] + B[a][2] * flux[2]);
+    v[q] = val * geo.JxW[q];
+  }
+  sweep<true, true>(n, 0, fe.D, g[0], v);
+  sweep<true, true>(n, 1, fe.D, g[1], v);
+  sweep<true, true>(n, 2, fe.D, g[2], v);
+  sweep<true, false>(n, 0, fe.S, v, tmp);
+  sweep<true, false>(n, 1, fe.S, tmp, v);
+  sweep<true, false>(n, 2, fe.S, v, out);
+}
+
 void local_cell_apply(const orc_op *op, const FE1D &fe, const Mesh &m, int cx, int cy, int cz,
                       const double *u, double *out, CellGeom &geo) {
+  if (op->tg.on) return local_cell_apply_tangent(op, fe, m, cx, cy, cz, u, out, geo);
   const int n = fe.n, nq = n * n * n;
   double v[512], tmp[512], g[3][512], kq[512];
   cell_geometry(fe, m, op->coords, cx, cy, cz, geo);
@@ -684,7 +770,7 @@ void orc_vmult(const orc_op *op, double *dst, const double *src, int n_threads) 
   FE1D fe(op->degree);
   const int n = fe.n;
   std::fill(dst, dst + m.ndofs, 0.0);
-  if (!op->coords && op->degree >= 1 && op->degree <= 4) {
+  if (!op->coords && !op->tg.on && op->degree >= 1 && op->degree <= 4) {
     switch (op->degree) {
       case 1: vmult_fast<2>(op, fe, m, dst, src, n_threads); break;
       case 2: vmult_fast<3>(op, fe, m, dst, src, n_threads); break;
@@ -825,6 +911,17 @@ void orc_diagonal(const orc_op *op, double *diag) {
     static thread_local CellGeom geo;
     double kq[512];
     cell_geometry(fe, m, op->coords, cx, cy, cz, geo);
+    if (op->tg.on) {
+      // full tangent: diagonal entries of the local operator, one unit vector at a time
+      double u[512], out[512];
+      for (int a = 0; a < nd; ++a) {
+        std::fill(u, u + nd, 0.0);
+        u[a] = 1.0;
+        local_cell_apply_tangent(op, fe, m, cx, cy, cz, u, out, geo);
+        diag[m.cell_dof(cx, cy, cz, a % n, (a / n) % n, a / (n * n))] += out[a];
+      }
+      return;
+    }
     cell_kappa(op, fe, m, cx, cy, cz, kq);
     for (int l = 0; l < n; ++l)
       for (int j = 0; j < n; ++j)
@@ -849,7 +946,6 @@ void orc_diagonal(const orc_op *op, double *diag) {
           diag[m.cell_dof(cx, cy, cz, i, j, l)] += acc;
         }
   });
-  (void)nd;
   if (op->robin_mask) {
     for_each_boundary_face(m, op->robin_mask, [&](int f, int cx, int cy, int cz) {
       double Tq[64], Tv[64];
@@ -918,7 +1014,7 @@ void orc_create_rhs(const orc_op *op, double *dst, const double *T, const double
           qs[a] = q_vol ? q_vol[idx[a]] : 0.0;
         }
     cell_geometry(fe, m, op->coords, cx, cy, cz, geo);
-    cell_kappa(&opT, fe, m, cx, cy, cz, kq);
+    if (!op->tg.on) cell_kappa(&opT, fe, m, cx, cy, cz, kq);
     eval_values(fe, t, v, tmp);
     eval_gradients(fe, v, g[0], g[1], g[2]);
     eval_values(fe, to, vo, tmp);
@@ -927,9 +1023,19 @@ void orc_create_rhs(const orc_op *op, double *dst, const double *T, const double
       const double(*B)[3] = geo.Jinv[q];
       double gp[3];
       for (int b = 0; b < 3; ++b) gp[b] = B[0][b] * g[0][q] + B[1][b] * g[1][q] + B[2][b] * g[2][q];
-      const double s = -kq[q] * geo.JxW[q];
+      double val, kk;
+      if (op->tg.on) {
+        // residual with the material at T and the prescribed velocity (:486-505)
+        double rc, drc, dk;
+        material(op->tg, v[q], rc, kk, drc, dk);
+        const double *w = op->tg.velocity;
+        val = rc / op->tg.dt * (v[q] - vo[q]) - vq[q] + rc * (gp[0] * w[0] + gp[1] * w[1] + gp[2] * w[2]);
+      } else {
+        kk = kq[q];
+        val = op->c_mass * (v[q] - vo[q]) - vq[q];
+      }
+      const double s = -kk * geo.JxW[q];
       for (int a = 0; a < 3; ++a) g[a][q] = s * (B[a][0] * gp[0] + B[a][1] * gp[1] + B[a][2] * gp[2]);
-      const double val = op->c_mass * (v[q] - vo[q]) - vq[q];
       v[q] = -val * geo.JxW[q];
     }
     sweep<true, true>(n, 0, fe.D, g[0], v);
@@ -996,6 +1102,10 @@ double orc_l2_norm(const orc_op *op, const double *u) {
       }
   delete geo;
   return std::sqrt(sum);
+}
+
+void orc_material(const orc_tangent *tg, double T, double *out) {
+  material(*tg, T, out[0], out[1], out[2], out[3]);
 }
 
 void orc_kappa_law(int law, const double *p, double T, double *out) {

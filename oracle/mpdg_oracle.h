@@ -33,6 +33,21 @@ extern "C" {
 #define ORC_LAW_TANH 1   /* k0*(1 + a*tanh((T-T0)/w)); p = {k0, a, T0, w} */
 #define ORC_LAW_MUSHY 2  /* liquid_solid mushy zone; p = {k_liquid, k_solid, T_solidus, T_liquidus} */
 
+/* Full Newton tangent of the heat residual with a temperature-dependent material and a
+   prescribed velocity (heat_diffuse_operator.cpp:1028-1056 tangent, :467-527 residual).
+   transition 0: constant material (liquid values); 1: liquid/solid mushy zone between the
+   solidus and liquidus temperatures (material.templates.hpp:296-380, :546-598, :765-778). */
+typedef struct orc_tangent {
+  int on;                 /* 0: the operator without these terms (c_mass, kappa_mode) */
+  double dt;
+  int transition;
+  double k[2], cp[2], rho[2];  /* liquid, solid */
+  double T_solidus, T_liquidus;
+  int do_solidification;  /* the d/dT terms of the tangent */
+  double velocity[3];
+  const double *T_old;
+} orc_tangent;
+
 typedef struct orc_op {
   int degree;            /* k */
   int nc[3];             /* cells per direction */
@@ -55,6 +70,7 @@ typedef struct orc_op {
   double T_inf_rad;      /* radiation ambient temperature */
   int laser_on;          /* Gaussian surface flux on face 5 (z+) in create_rhs */
   double laser_p[6];     /* peak, sigma, x0, y0, vx, t : q = peak*exp(-((x-x0-vx*t)^2+(y-y0)^2)/(2 sigma^2)) */
+  orc_tangent tg;        /* tg.on: vmult / diagonal / create_rhs use the full tangent at T */
 } orc_op;
 
 /* 1D tables: nodes[n1], qp[n1], qw[n1], S[q*n1+i] = phi_i(qp_q), G = phi_i'(qp_q),
@@ -93,6 +109,16 @@ void orc_interpolate_qp(const orc_op *op, const double *u, double *u_qp);
 int orc_cg(const orc_op *op, const double *b, double *x, double rel_tol, double abs_tol,
            int max_it, int precond, void *gmg, int criterion, double *res_out, int n_threads);
 
+/* Restarted GMRES with right preconditioning (deal.II SolverGMRES with
+   right_preconditioning = true, include/meltpooldg/linear_algebra/linear_solver.hpp:112-118):
+   at most `restart` basis vectors per cycle, modified Gram-Schmidt, Givens rotations.
+   precond: 0 identity, 1 inverse diagonal; criterion as orc_cg. */
+int orc_gmres(const orc_op *op, const double *b, double *x, double rel_tol, double abs_tol, int max_it,
+              int restart, int precond, int criterion, double *res_out, int n_threads);
+
+/* material at a scalar temperature: out = {rho_cp, k, d rho_cp/dT, dk/dT} */
+void orc_material(const orc_tangent *tg, double T, double *out);
+
 /* Geometric multigrid (FP64): h-coarsening by 2 down to nc_coarse cells in x
    (all directions scaled alike), optional p-coarsening to coarse_degree on the
    coarsest mesh, Chebyshev(degree cheb_degree, smoothing range, eig_iters CG
@@ -124,6 +150,10 @@ double orc_estimate_lambda_max(const orc_op *op, const double *inv_diag, int eig
 int orc_heat_run(orc_op *op, double rho_cp, double dt, int n_steps, int predictor,
                  double lin_rel_tol, double nl_res_tol, double nl_corr_tol, int nl_max_it,
                  double *T, double *norms, int *lin_iters, int n_threads);
+/* The same with op->tg.on (full tangent, GMRES + inverse diagonal): tg.dt is the step. */
+int orc_heat_run_tangent(orc_op *op, int n_steps, int predictor, double lin_rel_tol, double nl_res_tol,
+                         double nl_corr_tol, int nl_max_it, double *T, double *norms, int *lin_iters,
+                         int n_threads);
 
 #ifdef __cplusplus
 }

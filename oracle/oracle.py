@@ -20,6 +20,22 @@ LAW_TANH, LAW_MUSHY = 1, 2
 _dp = C.POINTER(C.c_double)
 
 
+class OrcTangent(C.Structure):
+    _fields_ = [
+        ("on", C.c_int),
+        ("dt", C.c_double),
+        ("transition", C.c_int),
+        ("k", C.c_double * 2),
+        ("cp", C.c_double * 2),
+        ("rho", C.c_double * 2),
+        ("T_solidus", C.c_double),
+        ("T_liquidus", C.c_double),
+        ("do_solidification", C.c_int),
+        ("velocity", C.c_double * 3),
+        ("T_old", _dp),
+    ]
+
+
 class OrcOp(C.Structure):
     _fields_ = [
         ("degree", C.c_int),
@@ -43,6 +59,7 @@ class OrcOp(C.Structure):
         ("T_inf_rad", C.c_double),
         ("laser_on", C.c_int),
         ("laser_p", C.c_double * 6),
+        ("tg", OrcTangent),
     ]
 
 
@@ -94,6 +111,13 @@ def lib():
                                    C.c_double, C.c_double, C.c_int, _dp, _dp,
                                    C.POINTER(C.c_int), C.c_int]
         L.orc_heat_run.restype = C.c_int
+        L.orc_gmres.argtypes = [P, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, _dp,
+                                C.c_int]
+        L.orc_gmres.restype = C.c_int
+        L.orc_material.argtypes = [C.POINTER(OrcTangent), C.c_double, _dp]
+        L.orc_heat_run_tangent.argtypes = [P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _dp,
+                                           _dp, C.POINTER(C.c_int), C.c_int]
+        L.orc_heat_run_tangent.restype = C.c_int
         _lib = L
     return _lib
 
@@ -215,6 +239,42 @@ class Operator:
         lib().orc_interpolate_qp(self.ref, ptr(np.ascontiguousarray(u, dtype=np.float64)), ptr(out))
         return out
 
+    def set_tangent(self, dt, material=None, velocity=(0.0, 0.0, 0.0), do_solidification=False, T_old=None):
+        """Switch to the full Newton tangent. material: None (constant: rho_cp = c_mass*dt, k = kappa)
+        or dict(k=(liquid, solid), cp=(..), rho=(..), T_solidus=.., T_liquidus=..) for the mushy zone."""
+        t = self.s.tg
+        t.on = 1
+        t.dt = dt
+        if material is None:
+            t.transition = 0
+            t.k[:] = [self.s.kappa, self.s.kappa]
+            t.cp[:] = [self.s.c_mass * dt] * 2
+            t.rho[:] = [1.0, 1.0]
+        else:
+            t.transition = 1
+            t.k[:] = list(material["k"])
+            t.cp[:] = list(material["cp"])
+            t.rho[:] = list(material["rho"])
+            t.T_solidus, t.T_liquidus = material["T_solidus"], material["T_liquidus"]
+        t.do_solidification = 1 if do_solidification else 0
+        t.velocity[:] = list(velocity)
+        if T_old is not None:
+            self._keep["T_old"] = np.ascontiguousarray(T_old, dtype=np.float64)
+            t.T_old = ptr(self._keep["T_old"])
+
+    def material(self, T):
+        out = np.zeros(4)
+        lib().orc_material(C.byref(self.s.tg), float(T), ptr(out))
+        return out
+
+    def gmres(self, b, x=None, rel_tol=1e-10, abs_tol=1e-20, max_it=10000, restart=28, precond=1, criterion=1,
+              threads=os.cpu_count()):
+        x = np.zeros(self.n_dofs) if x is None else np.array(x, dtype=np.float64)
+        res = C.c_double(0)
+        it = lib().orc_gmres(self.ref, ptr(np.ascontiguousarray(b)), ptr(x), rel_tol, abs_tol, max_it, restart,
+                             precond, criterion, C.byref(res), threads)
+        return x, it, res.value
+
     def cg(self, b, x=None, rel_tol=1e-10, abs_tol=1e-20, max_it=10000, precond=1, gmg=None,
            criterion=1, threads=os.cpu_count()):
         x = np.zeros(self.n_dofs) if x is None else np.array(x, dtype=np.float64)
@@ -282,4 +342,15 @@ def heat_run(op, rho_cp, dt, n_steps, T0, predictor=2, lin_rel_tol=1e-12, nl_res
     its = (C.c_int * n_steps)()
     rc = lib().orc_heat_run(op.ref, rho_cp, dt, n_steps, predictor, lin_rel_tol, nl_res_tol,
                             nl_corr_tol, nl_max_it, ptr(T), ptr(norms), its, threads)
+    return rc, T, norms, list(its)
+
+
+def heat_run_tangent(op, n_steps, T0, predictor=2, lin_rel_tol=1e-12, nl_res_tol=1e-9, nl_corr_tol=1e-10,
+                     nl_max_it=10, threads=os.cpu_count()):
+    """Time loop with the full tangent (op.set_tangent first): Newton + GMRES + inverse diagonal."""
+    T = np.array(T0, dtype=np.float64)
+    norms = np.zeros(n_steps)
+    its = (C.c_int * n_steps)()
+    rc = lib().orc_heat_run_tangent(op.ref, n_steps, predictor, lin_rel_tol, nl_res_tol, nl_corr_tol, nl_max_it,
+                                    ptr(T), ptr(norms), its, threads)
     return rc, T, norms, list(its)

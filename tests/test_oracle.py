@@ -298,3 +298,109 @@ def test_planted_mistake_is_caught():
     T0 = op.node_coords()[0].copy()
     _, _, norms, _ = O.heat_run(op, 1.0, 1e-4, 5, T0, predictor=2, nl_res_tol=5e-9)
     assert ["%.5e" % v for v in norms] != ["%.5e" % v for v in GOLDEN["plain"]]
+
+
+# ------------------------------------------ full Newton tangent (solidification, velocity)
+MUSHY = dict(k=(55.563, 17.0), cp=(460.0, 700.0), rho=(7850.0, 7850.0), T_solidus=1966.0, T_liquidus=1974.0)
+TANGENT_GOLDEN = {
+    # unidirectional_heat_transfer_with_velocity.mpirun=4.output
+    "velocity": [1.00000e+02, 1.00000e+02, 9.99999e+01, 9.99999e+01, 9.99999e+01],
+    # unidirectional_heat_transfer_with_velocity_and_solidification.mpirun=4.output
+    "velocity_solidification": [1.97002e+02, 1.97003e+02, 1.97004e+02, 1.97005e+02, 1.97006e+02],
+}
+
+
+def tangent_case(case):
+    """The reference's 2D Q1 32x32 setups on [0,0.1]^2 (cases/unidirectional_heat_transfer.cpp:194-260 and
+    the two .json files), run as a one-cell-thick 3D mesh. Returns (op, T0, Newton residual tolerance)."""
+    nc, L = (32, 32, 1), (0.1, 0.1, 1.0)
+    if case == "velocity":
+        # gas 7850 x 460, k 55.563; convection 25 and radiation 0.98 to 300 K on x+; T0 = 1000; v = -0.01
+        dt, rho_cp = 1e-3, 7850.0 * 460.0
+        op = O.Operator(1, nc, L, c_mass=rho_cp / dt, kappa=55.563, robin_mask=2, alpha=25.0, emissivity=0.98,
+                        T_inf_conv=300.0, T_inf_rad=300.0)
+        op.set_tangent(dt, velocity=(-0.01, 0.0, 0.0))
+        return op, np.full(op.n_dofs, 1000.0), 1e-6
+    op = O.Operator(1, nc, L)
+    op.set_tangent(0.01, material=MUSHY, velocity=(-0.005, 0.0, 0.0), do_solidification=True)
+    return op, 1960.0 + 20.0 * op.node_coords()[0] / 0.1, 1e-7
+
+
+@pytest.mark.parametrize("case", sorted(TANGENT_GOLDEN))
+def test_tangent_golden_norm_sequences(case):
+    op, T0, tol = tangent_case(case)
+    rc, _, norms, _ = O.heat_run_tangent(op, 5, T0, nl_res_tol=tol, threads=4)
+    assert rc == 0
+    assert ["%.5e" % v for v in norms] == ["%.5e" % v for v in TANGENT_GOLDEN[case]]
+
+
+def test_tangent_planted_mistake_is_caught():
+    """Without the velocity the solidification sequence changes in the printed digits."""
+    op, T0, tol = tangent_case("velocity_solidification")
+    op.s.tg.velocity[0] = 0.0
+    _, _, norms, _ = O.heat_run_tangent(op, 5, T0, nl_res_tol=tol, threads=4)
+    assert ["%.5e" % v for v in norms] != ["%.5e" % v for v in TANGENT_GOLDEN["velocity_solidification"]]
+
+
+def test_material_blend_closed_form():
+    """rho c_p and k of the mushy zone: cubic blends of the phase values, derivatives by central differences,
+    pure-phase values outside [T_s, T_l]; k agrees with the conductivity law pinned by material.output."""
+    op = O.Operator(1, (1, 1, 1))
+    op.set_tangent(1.0, material=MUSHY)
+    for T in (1967.0, 1970.0, 1973.5):
+        rc, k, drc, dk = op.material(T)
+        fs = (1974.0 - T) / 8.0
+        blend = lambda l, s: l + (s - l) * (3 * fs ** 2 - 2 * fs ** 3)
+        assert abs(k - blend(55.563, 17.0)) < 1e-12 and abs(rc - 7850.0 * blend(460.0, 700.0)) < 1e-6
+        assert abs(k - O.kappa_law(O.LAW_MUSHY, (55.563, 17.0, 1966.0, 1974.0), T)[0]) < 1e-12
+        h = 1e-5
+        fd = (op.material(T + h) - op.material(T - h)) / (2 * h)
+        assert abs(drc - fd[0]) <= 1e-6 * abs(drc) and abs(dk - fd[1]) <= 1e-6 * abs(dk)
+    assert tuple(op.material(1960.0)) == (7850.0 * 700.0, 17.0, 0.0, 0.0)
+    assert tuple(op.material(1980.0)) == (7850.0 * 460.0, 55.563, 0.0, 0.0)
+
+
+@pytest.mark.parametrize("solid", [True, False])
+def test_tangent_is_the_derivative_of_the_residual(solid):
+    """J(T) u = -d/de create_rhs(T + e u) (central differences): the tangent carries every term of the
+    residual's derivative; without the d/dT terms (solid = False) the check must fail."""
+    rng = np.random.default_rng(5)
+    op = O.Operator(2, (3, 3, 2), (0.3, 0.3, 0.2))
+    X = op.node_coords()
+    T = 1970.0 + 2.0 * np.sin(5 * X[0]) * np.cos(4 * X[1]) + 0.5 * X[2]  # inside the mushy interval
+    T_old = T - 0.3 * np.cos(3 * X[0])
+    op.set_tangent(0.01, material=MUSHY, velocity=(-0.02, 0.01, 0.005), do_solidification=solid, T_old=T_old)
+    op.set_T(T)
+    u = rng.uniform(-1, 1, op.n_dofs)
+    Ju = op.vmult(u, threads=1)
+    eps = 1e-4
+
+    def res(Tx):
+        return op.create_rhs(np.zeros(op.n_dofs), np.ascontiguousarray(Tx), T_old)
+
+    fd = -(res(T + eps * u) - res(T - eps * u)) / (2 * eps)
+    err = np.linalg.norm(Ju - fd) / np.linalg.norm(fd)
+    if solid:
+        assert err < 1e-7
+    else:
+        assert err > 1e-3
+
+
+def test_gmres_matches_direct_solve():
+    """Non-symmetric tangent (velocity, solidification terms) on a small mesh: GMRES with and without the
+    inverse diagonal against numpy's dense solve; restarted (restart 5) and full cycles agree."""
+    op = O.Operator(2, (3, 2, 2), (0.3, 0.2, 0.2))
+    X = op.node_coords()
+    T = 1970.0 + 3.0 * np.sin(4 * X[0]) + X[1]
+    op.set_tangent(0.01, material=MUSHY, velocity=(-0.5, 0.2, 0.0), do_solidification=True, T_old=T - 0.5)
+    op.set_T(T)
+    A = np.stack([op.vmult(e, threads=1) for e in np.eye(op.n_dofs)], axis=1)
+    assert np.linalg.norm(A - A.T) > 1e-3 * np.linalg.norm(A)  # really non-symmetric
+    assert np.allclose(np.diag(A), op.diagonal(), rtol=1e-12, atol=0.0)
+    b = np.random.default_rng(2).uniform(-1, 1, op.n_dofs)
+    xs = np.linalg.solve(A, b)
+    for precond in (0, 1):
+        for restart in (5, 200):
+            x, it, res = op.gmres(b, rel_tol=1e-12, precond=precond, restart=restart, threads=1)
+            assert it > 0 and res <= 1e-12 * np.linalg.norm(b)
+            assert np.linalg.norm(x - xs) <= 1e-8 * np.linalg.norm(xs)
