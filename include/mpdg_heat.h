@@ -84,6 +84,36 @@ int mpdg_heat_op_set_mapping(mpdg_heat_op *op, const double *coords);
    p = {peak, sigma, x0, y0, vx, t}; peak = 0 switches it off */
 int mpdg_heat_op_set_surface_flux(mpdg_heat_op *op, const double *p6);
 
+/* Full Newton tangent with a temperature-dependent material and a prescribed constant
+   velocity (tangent_local_cell_operation with do_solidification and a velocity field,
+   heat_diffuse_operator.cpp:1028-1056; residual :467-527), Cartesian cells. While it is set,
+   vmult applies the tangent at the temperatures last bound by mpdg_heat_op_bind_temperature,
+   create_rhs evaluates the residual with this material and velocity, and the diagonal is the
+   tangent's; the constant rho_cp and conductivity settings are not used. The tangent is
+   non-symmetric: solve with MPDG_SOLVER_GMRES (the reference's default heat solver,
+   source/heat/heat_data.cpp:11-12) and the diagonal preconditioner; the multigrid
+   preconditioner and the FP32 apply refuse it (MPDG_ERR_INVALID). */
+#define MPDG_TRANSITION_NONE 0  /* constant material: the liquid values */
+#define MPDG_TRANSITION_MUSHY 1 /* liquid/solid cubic blend in the solid fraction
+                                   clamp((liquidus - T)/(liquidus - solidus)), rho_cp = rho*c_p
+                                   (include/meltpooldg/core/material.templates.hpp:296-380,
+                                   :546-598, :765-778) */
+typedef struct mpdg_heat_material {
+  int transition;
+  double conductivity[2];  /* liquid, solid */
+  double specific_heat[2]; /* liquid, solid */
+  double density[2];       /* liquid, solid */
+  double solidus, liquidus;
+} mpdg_heat_material;
+/* do_solidification: add the d/dT terms of the material to the tangent (:1044-1056) */
+int mpdg_heat_op_set_tangent(mpdg_heat_op *op, const mpdg_heat_material *mat, int do_solidification,
+                             const double *velocity3);
+int mpdg_heat_op_clear_tangent(mpdg_heat_op *op);
+/* evaluates the tangent's per-quadrature-point data at T (and T_old for the solidification
+   term; both device FP64 DoF vectors, read now) -- the reference reads them inside every
+   tangent apply (do_reinit_cells, :1001-1015) */
+int mpdg_heat_op_bind_temperature(mpdg_heat_op *op, const double *T, const double *T_old, void *stream);
+
 /* vmult (heat_diffuse_operator.cpp:181-197): dst = J src, dst is overwritten */
 int mpdg_heat_op_vmult(const mpdg_heat_op *op, double *dst, const double *src, void *stream);
 int mpdg_heat_op_vmult_f32(const mpdg_heat_op *op, float *dst, const float *src, void *stream);
@@ -134,6 +164,7 @@ int mpdg_gmg_level_info(const mpdg_precond *p, int level, int *degree, int *n_ce
 /* --- LinearSolver::solve (include/meltpooldg/linear_algebra/linear_solver.hpp:21-134) --- */
 #define MPDG_SOLVER_CG 0
 #define MPDG_SOLVER_FCG 1 /* flexible CG (Polak-Ribiere beta) for variable preconditioners */
+#define MPDG_SOLVER_GMRES 2 /* restarted GMRES, right preconditioning (linear_solver.hpp:112-118) */
 #define MPDG_CRIT_REDUCTION 0 /* ReductionControl: ||r|| <= max(abs_tol, rel_tol*||r0||) */
 #define MPDG_CRIT_RHS 1       /* ||b - J x|| <= rel_tol*||b|| */
 typedef struct mpdg_solver_params {
@@ -142,6 +173,8 @@ typedef struct mpdg_solver_params {
   double rel_tolerance;
   double abs_tolerance;
   int criterion;
+  int gmres_restart; /* GMRES basis vectors per cycle, 1..31 (0: 28, deal.II's default
+                        max_n_tmp_vectors 30 minus its two work vectors) */
 } mpdg_solver_params;
 /* x holds the initial guess on entry; iterations = last_step(); res = final ||r|| */
 int mpdg_linear_solve(mpdg_heat_op *op, mpdg_precond *p, const mpdg_solver_params *params,

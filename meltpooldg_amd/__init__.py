@@ -30,7 +30,8 @@ FP64, FP32 = 0, 1
 KAPPA_CONST, KAPPA_QP, KAPPA_CELL, KAPPA_LAW = 0, 1, 2, 3
 LAW_TANH, LAW_MUSHY = 1, 2
 PRECOND_IDENTITY, PRECOND_DIAGONAL, PRECOND_GMG = 0, 1, 2
-SOLVER_CG, SOLVER_FCG = 0, 1
+SOLVER_CG, SOLVER_FCG, SOLVER_GMRES = 0, 1, 2
+TRANSITION_NONE, TRANSITION_MUSHY = 0, 1
 CRIT_REDUCTION, CRIT_RHS = 0, 1
 PREDICTOR_NONE, PREDICTOR_ZERO, PREDICTOR_LINEAR, PREDICTOR_LEARNED = 0, 1, 2, 3
 
@@ -58,7 +59,13 @@ class GMGParams(C.Structure):
 
 class SolverParams(C.Structure):
     _fields_ = [("solver_type", C.c_int), ("max_iterations", C.c_int), ("rel_tolerance", C.c_double),
-                ("abs_tolerance", C.c_double), ("criterion", C.c_int)]
+                ("abs_tolerance", C.c_double), ("criterion", C.c_int), ("gmres_restart", C.c_int)]
+
+
+class Material(C.Structure):
+    """mpdg_heat_material: liquid/solid values (index 0 liquid, 1 solid) and the mushy interval."""
+    _fields_ = [("transition", C.c_int), ("conductivity", C.c_double * 2), ("specific_heat", C.c_double * 2),
+                ("density", C.c_double * 2), ("solidus", C.c_double), ("liquidus", C.c_double)]
 
 
 class OperationParams(C.Structure):
@@ -73,6 +80,9 @@ STEP_NEWTON, STEP_LINEAR = 0, 1
 _V, _I, _D, _L = C.c_void_p, C.c_int, C.c_double, C.c_long
 _SIGNATURES = {
     "mpdg_heat_op_create": ([C.POINTER(HeatDesc), C.POINTER(_V)], _I),
+    "mpdg_heat_op_set_tangent": ([_V, C.POINTER(Material), _I, C.POINTER(_D)], _I),
+    "mpdg_heat_op_clear_tangent": ([_V], _I),
+    "mpdg_heat_op_bind_temperature": ([_V, _V, _V, _V], _I),
     "mpdg_heat_op_destroy": ([_V], _I),
     "mpdg_heat_op_n_dofs": ([_V], _L),
     "mpdg_heat_op_n_cells": ([_V], _L),
@@ -246,6 +256,30 @@ class HeatDiffuseMultiPhaseOperator:
         _check(self.lib.mpdg_heat_op_set_robin(self.h, face_mask, alpha, T_inf_conv, emissivity, T_inf_rad,
                                                _ptr(T)), "set_robin")
 
+    def set_tangent(self, material=None, velocity=(0.0, 0.0, 0.0), do_solidification=False):
+        """Full Newton tangent (heat_diffuse_operator.cpp:1028-1056) with a prescribed velocity.
+        material: dict(conductivity=(l, s), specific_heat=(l, s), density=(l, s), solidus=.., liquidus=..)
+        for the liquid/solid mushy zone, or dict(conductivity=k, specific_heat=c, density=rho) constant."""
+        m = Material()
+        mat = material or {}
+        def pair(v):
+            return (v, v) if not isinstance(v, (tuple, list)) else tuple(v)
+        m.transition = TRANSITION_MUSHY if "solidus" in mat else TRANSITION_NONE
+        m.conductivity[:] = pair(mat.get("conductivity", 1.0))
+        m.specific_heat[:] = pair(mat.get("specific_heat", 1.0))
+        m.density[:] = pair(mat.get("density", 1.0))
+        m.solidus, m.liquidus = mat.get("solidus", 0.0), mat.get("liquidus", 1.0)
+        v = (C.c_double * 3)(*velocity)
+        _check(self.lib.mpdg_heat_op_set_tangent(self.h, C.byref(m), 1 if do_solidification else 0, v), "set_tangent")
+
+    def clear_tangent(self):
+        _check(self.lib.mpdg_heat_op_clear_tangent(self.h), "clear_tangent")
+
+    def bind_temperature(self, T, T_old=None):
+        """Evaluate the tangent at T (and T_old) -- call after T changes, before vmult / diagonal."""
+        _check(self.lib.mpdg_heat_op_bind_temperature(self.h, _ptr(T), _ptr(T_old) if T_old is not None else None,
+                                                      _stream()), "bind_temperature")
+
     def set_mapping(self, coords):
         """Mapped mesh: node coordinates (device FP64, X block, Y block, Z block)."""
         assert coords.numel() == 3 * self.n_dofs
@@ -362,11 +396,13 @@ class LinearSolverData:
     rel_tolerance: float = 1e-12
     abs_tolerance: float = 1e-20
     criterion: int = CRIT_REDUCTION
+    gmres_restart: int = 0
 
     def c(self):
         p = SolverParams()
         p.solver_type, p.max_iterations = self.solver_type, self.max_iterations
         p.rel_tolerance, p.abs_tolerance, p.criterion = self.rel_tolerance, self.abs_tolerance, self.criterion
+        p.gmres_restart = self.gmres_restart
         return p
 
 

@@ -46,6 +46,77 @@ int launch_axpby(double *y, double a, const double *x, double b, long n, hipStre
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
+constexpr int kMdotBlocks = 512;
+long mdot_scratch() { return (long)kMdotBlocks * kMdotMax; }
+
+// partial sums of J dot products against w: one pass over w, J accumulators per thread
+__global__ __launch_bounds__(256) void mdot_partial_kernel(const double *__restrict__ V, long n, int J,
+                                                           const double *__restrict__ w, double *__restrict__ part) {
+  double acc[kMdotMax];
+#pragma unroll
+  for (int i = 0; i < kMdotMax; ++i) acc[i] = 0.0;
+  for (long k = (long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long)gridDim.x * 256) {
+    const double wk = w[k];
+#pragma unroll
+    for (int i = 0; i < kMdotMax; ++i)
+      if (i < J) acc[i] = fma(V[i * n + k], wk, acc[i]);
+  }
+  __shared__ double red[4][kMdotMax];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < kMdotMax; ++i) {
+    if (i >= J) break;
+    double v = acc[i];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) red[wave][i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < J)
+    part[blockIdx.x * kMdotMax + threadIdx.x] =
+        (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+__global__ void mdot_final_kernel(const double *__restrict__ part, int nb, int J, double *__restrict__ h,
+                                  int accumulate) {
+  const int i = threadIdx.x;
+  if (i >= J) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[b * kMdotMax + i];
+  h[i] = accumulate ? h[i] + s : s;
+}
+int launch_mdot(const double *V, long n, int J, const double *w, double *part, double *h, bool accumulate,
+                hipStream_t s) {
+  if (J < 1 || J > kMdotMax) return MPDG_ERR_INVALID;
+  const int nb = (int)std::max<long>(1, std::min<long>(kMdotBlocks, (n + 255) / 256));
+  hipLaunchKernelGGL(mdot_partial_kernel, dim3(nb), dim3(256), 0, s, V, n, J, w, part);
+  hipLaunchKernelGGL(mdot_final_kernel, dim3(1), dim3(kMdotMax), 0, s, part, nb, J, h, accumulate ? 1 : 0);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+__global__ __launch_bounds__(256) void maxpy_kernel(double *__restrict__ w, const double *__restrict__ V, long n,
+                                                    int J, const double *__restrict__ c, double sign) {
+  for (long k = (long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long)gridDim.x * 256) {
+    double s = 0.0;
+    for (int i = 0; i < J; ++i) s = fma(c[i], V[i * n + k], s);
+    w[k] += sign * s;
+  }
+}
+int launch_maxpy(double *w, const double *V, long n, int J, const double *c, double sign, hipStream_t s) {
+  if (J < 1) return MPDG_OK;
+  const long nb = std::min<long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(maxpy_kernel, dim3(std::max<long>(nb, 1)), dim3(256), 0, s, w, V, n, J, c, sign);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+__global__ __launch_bounds__(256) void scale_kernel(double *__restrict__ y, const double *__restrict__ x, double a,
+                                                    long n) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) y[i] = a * x[i];
+}
+int launch_scale(double *y, const double *x, double a, long n, hipStream_t s) {
+  const long nb = std::min<long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(scale_kernel, dim3(std::max<long>(nb, 1)), dim3(256), 0, s, y, x, a, n);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void invert_diag_kernel(T *d, long n) {
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {

@@ -39,6 +39,16 @@ int cg_update(double *x, double *r, const double *p, const double *q, double *z,
 int cg_beta(const double *r, const double *z, long n, const Reduction &red, CGState *st, hipStream_t s);
 int cg_direction(double *p, const double *z, long n, const CGState *st, hipStream_t s);
 
+// GMRES building blocks over a basis V of J rows of length n (row i at V + i n), J <= kMdotMax:
+// h[i] (+)= V_i . w (deterministic two-stage reduction; part: mdot_scratch() doubles),
+// w += sign * sum_i c_i V_i with c in device memory, y = a x.
+constexpr int kMdotMax = 32;
+long mdot_scratch();
+int launch_mdot(const double *V, long n, int J, const double *w, double *part, double *h, bool accumulate,
+                hipStream_t s);
+int launch_maxpy(double *w, const double *V, long n, int J, const double *c, double sign, hipStream_t s);
+int launch_scale(double *y, const double *x, double a, long n, hipStream_t s);
+
 template <typename T>
 int launch_chebyshev_step(T *x, T *d, const T *b, const T *Ax, const T *inv, long n, double c1, double c2,
                           bool first, hipStream_t s);

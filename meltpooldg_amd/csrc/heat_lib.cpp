@@ -1,5 +1,6 @@
 // Host side of the MI355X heat operator library and its C ABI (include/mpdg_heat.h).
 #include "heat_lib.hpp"
+#include "tangent.hpp"
 
 #include <cmath>
 #include <cstdint>
@@ -22,6 +23,7 @@ inline int dev_ok() { return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ER
 int mpdg_heat_op::vmult_cells(double *dst, const double *src, hipStream_t s) const {
   if (precision != MPDG_FP64) return MPDG_ERR_INVALID;
   if (d64.metric && d64.kappa_mode == KAPPA_LAW) MPDG_TRY(refresh_metric_k(s));  // k(T) with the current T
+  if (tangent) return td_bound ? launch_tangent_apply(d64, td.get(), src, dst, s) : MPDG_ERR_INVALID;
   if (!timing_on) return launch_apply<double>(d64, src, dst, s);
   if (timing_used == timing_events.size()) {
     hipEvent_t a, b;
@@ -65,12 +67,18 @@ int mpdg_heat_op::vmult(double *dst, const double *src, hipStream_t s) const {
 }
 
 int mpdg_heat_op::vmult_f32(float *dst, const float *src, hipStream_t s) const {
-  if (precision != MPDG_FP32) return MPDG_ERR_INVALID;
+  if (precision != MPDG_FP32 || tangent) return MPDG_ERR_INVALID;
   if (d32.metric && d32.kappa_mode == KAPPA_LAW) MPDG_TRY(refresh_metric_k(s));
   return launch_apply<float>(d32, src, dst, s);
 }
 
 int mpdg_heat_op::diagonal64(double *diag, hipStream_t s) const {
+  if (tangent) {
+    if (!td_bound) return MPDG_ERR_INVALID;
+    MPDG_TRY(launch_tangent_diagonal(d64, td.get(), diag, s));
+    if (d64.robin_mask) MPDG_TRY(launch_boundary(d64, 1, nullptr, diag, nullptr, T_robin, s));
+    return MPDG_OK;
+  }
   const int n1 = d64.degree + 1;
   MPDG_TRY(refresh_metric_k(s));
   MPDG_TRY(kq.resize((size_t)d64.n_cells * n1 * n1 * n1));
@@ -94,6 +102,11 @@ int mpdg_heat_op::create_rhs(double *dst, const double *T, const double *T_old, 
                              hipStream_t s) const {
   if (precision != MPDG_FP64) return MPDG_ERR_INVALID;
   if (dt == 0.0) return MPDG_ERR_ZERO_TIME_INCREMENT;
+  if (tangent) {
+    // residual with the tangent's material and velocity (heat_diffuse_operator.cpp:467-527)
+    MPDG_TRY(launch_tangent_residual(d64, dt, mat, vel, T, T_old, q, dst, s));
+    return launch_boundary(d64, 2, flux, dst, nullptr, T, s);
+  }
   const long n = d64.n_dofs;
   MPDG_TRY(tmp1.resize(n));
   MPDG_TRY(tmp2.resize(n));
@@ -439,6 +452,7 @@ struct GMGPrecond : mpdg_precond {
 
   // level data that depends on the fine operator's coefficient and time increment
   int update(hipStream_t s) override {
+    if (op->tangent) return MPDG_ERR_INVALID;
     const auto &f = op->d64;
     Level<T> &top = *lev.back();
     for (auto &L : lev) L->od.c_mass = f.c_mass;
@@ -688,10 +702,27 @@ struct GMGPrecond : mpdg_precond {
 // ===========================================================================
 namespace {
 
+struct GmresWork {
+  DeviceArray<double> V, w, z, r, hd, part;
+  int m = 0;
+  int init(long n, int basis) {
+    if (m != basis || V.n != (size_t)(basis + 1) * n) {
+      MPDG_TRY(V.resize((size_t)(basis + 1) * n));
+      m = basis;
+    }
+    MPDG_TRY(w.resize(n));
+    MPDG_TRY(z.resize(n));
+    MPDG_TRY(r.resize(n));
+    MPDG_TRY(hd.resize(3 * kMdotMax));
+    return part.resize(mdot_scratch());
+  }
+};
+
 struct KrylovWork {
-  DeviceArray<double> r, z, p, q;
+  DeviceArray<double> r, z, p, q, scal;
   DeviceArray<CGState> st;
   ReductionScratch red;
+  GmresWork gmres;
   int init(long n) {
     MPDG_TRY(r.resize(n));
     MPDG_TRY(z.resize(n));
@@ -702,10 +733,114 @@ struct KrylovWork {
   }
 };
 
+// SolverGMRES with right preconditioning (linear_solver.hpp:112-118): restarted Arnoldi with
+// classical Gram-Schmidt applied twice (fused multi-dot / multi-axpy kernels, one host read of
+// the Hessenberg column per iteration), Givens rotations and the small triangular solve on
+// the host; x += M^-1 V y at the end of each cycle. Stops when the rotated residual estimate
+// meets the tolerance (after which the true residual is recomputed) or after max_iterations.
+int gmres_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &prm, const double *b, double *x,
+                GmresWork &g, ReductionScratch &red, double *scal, int *iterations, double *res_out,
+                hipStream_t s) {
+  const long n = op->n_dofs();
+  const int m = prm.gmres_restart > 0 ? prm.gmres_restart : 28;
+  if (m >= kMdotMax) return MPDG_ERR_INVALID;
+  MPDG_TRY(g.init(n, m));
+  double *V = g.V.get();
+  auto precondition = [&](double *dst, const double *src) -> int {
+    if (pc) return pc->vmult(dst, src, s);
+    return launch_convert<double, double>(src, dst, n, s);
+  };
+  auto norm = [&](const double *v, double &out) -> int {
+    MPDG_TRY(launch_dot<double, double>(v, v, n, red.red(), scal, s));
+    double h2 = 0.0;
+    if (hipMemcpyAsync(&h2, scal, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) return MPDG_ERR_DEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+    out = std::sqrt(std::max(h2, 0.0));
+    return (h2 == h2) ? MPDG_OK : MPDG_ERR_NONFINITE;
+  };
+  // r = b - A x
+  auto residual = [&](double &out) -> int {
+    MPDG_TRY(op->vmult(g.w.get(), x, s));
+    MPDG_TRY(launch_convert<double, double>(b, g.r.get(), n, s));
+    MPDG_TRY(launch_axpby(g.r.get(), -1.0, g.w.get(), 1.0, n, s));
+    return norm(g.r.get(), out);
+  };
+  double res = 0.0, bnorm = 0.0;
+  MPDG_TRY(residual(res));
+  if (prm.criterion == MPDG_CRIT_RHS) MPDG_TRY(norm(b, bnorm));
+  const double tol = prm.criterion == MPDG_CRIT_RHS ? prm.rel_tolerance * bnorm
+                                                    : std::max(prm.abs_tolerance, prm.rel_tolerance * res);
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m), hh(2 * kMdotMax + 1);
+  auto h = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
+  int it = 0;
+  while (res > tol && it < prm.max_iterations) {
+    MPDG_TRY(launch_scale(V, g.r.get(), 1.0 / res, n, s));
+    std::fill(gv.begin(), gv.end(), 0.0);
+    gv[0] = res;
+    int k = 0;
+    while (k < m && it < prm.max_iterations) {
+      ++it;
+      double *vk1 = V + (size_t)(k + 1) * n;
+      MPDG_TRY(precondition(g.z.get(), V + (size_t)k * n));
+      MPDG_TRY(op->vmult(g.w.get(), g.z.get(), s));
+      double *h1 = g.hd.get(), *h2 = g.hd.get() + kMdotMax;
+      MPDG_TRY(launch_mdot(V, n, k + 1, g.w.get(), g.part.get(), h1, false, s));
+      MPDG_TRY(launch_maxpy(g.w.get(), V, n, k + 1, h1, -1.0, s));
+      MPDG_TRY(launch_mdot(V, n, k + 1, g.w.get(), g.part.get(), h2, false, s));
+      MPDG_TRY(launch_maxpy(g.w.get(), V, n, k + 1, h2, -1.0, s));
+      MPDG_TRY(launch_dot<double, double>(g.w.get(), g.w.get(), n, red.red(), g.hd.get() + 2 * kMdotMax, s));
+      if (hipMemcpyAsync(hh.data(), g.hd.get(), (2 * kMdotMax + 1) * sizeof(double), hipMemcpyDeviceToHost, s) !=
+          hipSuccess)
+        return MPDG_ERR_DEVICE;
+      if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+      for (int i = 0; i <= k; ++i) h(i, k) = hh[i] + hh[kMdotMax + i];
+      const double hn = std::sqrt(std::max(hh[2 * kMdotMax], 0.0));
+      if (!(hn == hn)) return MPDG_ERR_NONFINITE;
+      h(k + 1, k) = hn;
+      if (hn > 0.0) MPDG_TRY(launch_scale(vk1, g.w.get(), 1.0 / hn, n, s));
+      for (int i = 0; i < k; ++i) {  // earlier rotations on the new column
+        const double a = h(i, k), c = h(i + 1, k);
+        h(i, k) = cs[i] * a + sn[i] * c;
+        h(i + 1, k) = -sn[i] * a + cs[i] * c;
+      }
+      const double a = h(k, k), c = h(k + 1, k), d = std::sqrt(a * a + c * c);
+      if (!(d == d)) return MPDG_ERR_NONFINITE;
+      cs[k] = d > 0.0 ? a / d : 1.0;
+      sn[k] = d > 0.0 ? c / d : 0.0;
+      h(k, k) = d;
+      h(k + 1, k) = 0.0;
+      gv[k + 1] = -sn[k] * gv[k];
+      gv[k] = cs[k] * gv[k];
+      ++k;
+      if (std::fabs(gv[k]) <= tol || hn == 0.0) break;
+    }
+    // x += M^-1 V y, H y = g
+    for (int i = k - 1; i >= 0; --i) {
+      double t = gv[i];
+      for (int j = i + 1; j < k; ++j) t -= h(i, j) * y[j];
+      y[i] = h(i, i) != 0.0 ? t / h(i, i) : 0.0;
+    }
+    if (hipMemcpyAsync(g.hd.get(), y.data(), k * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
+      return MPDG_ERR_DEVICE;
+    MPDG_TRY(launch_fill<double>(g.w.get(), 0.0, n, s));
+    MPDG_TRY(launch_maxpy(g.w.get(), V, n, k, g.hd.get(), 1.0, s));
+    MPDG_TRY(precondition(g.z.get(), g.w.get()));
+    MPDG_TRY(launch_axpby(x, 1.0, g.z.get(), 1.0, n, s));
+    MPDG_TRY(residual(res));
+  }
+  *iterations = it;
+  *res_out = res;
+  return res <= tol ? MPDG_OK : MPDG_ERR_NO_CONVERGENCE;
+}
+
 int krylov_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &prm, const double *b, double *x,
                  KrylovWork &w, int *iterations, double *res, hipStream_t s) {
   const long n = op->n_dofs();
   MPDG_TRY(w.init(n));
+  if (prm.solver_type == MPDG_SOLVER_GMRES) {
+    MPDG_TRY(w.scal.resize(1));
+    return gmres_solve(op, pc, prm, b, x, w.gmres, w.red, w.scal.get(), iterations, res, s);
+  }
   const bool external = pc && pc->type == MPDG_PRECOND_GMG;
   const double *inv = (pc && pc->type == MPDG_PRECOND_DIAGONAL) ? pc->inverse_diagonal() : nullptr;
   const bool jacobi = !external;
@@ -883,6 +1018,41 @@ int mpdg_heat_op_set_surface_flux(mpdg_heat_op *op, const double *p) {
   return MPDG_OK;
 }
 
+int mpdg_heat_op_set_tangent(mpdg_heat_op *op, const mpdg_heat_material *mat, int do_solidification,
+                             const double *velocity3) {
+  if (!op || !mat || !velocity3) return MPDG_ERR_INVALID;
+  if (op->precision != MPDG_FP64 || op->coords) return MPDG_ERR_INVALID;
+  if (mat->transition != MPDG_TRANSITION_NONE && mat->transition != MPDG_TRANSITION_MUSHY) return MPDG_ERR_INVALID;
+  if (mat->transition == MPDG_TRANSITION_MUSHY && !(mat->liquidus > mat->solidus)) return MPDG_ERR_INVALID;
+  for (int i = 0; i < 3; ++i)
+    if (!std::isfinite(velocity3[i])) return MPDG_ERR_INVALID;
+  op->mat = *mat;
+  op->solid = do_solidification ? 1 : 0;
+  for (int i = 0; i < 3; ++i) op->vel[i] = velocity3[i];
+  const int n1 = op->d64.degree + 1;
+  MPDG_TRY(op->td.resize((size_t)8 * op->d64.n_cells * n1 * n1 * n1));
+  op->tangent = true;
+  op->td_bound = false;
+  return MPDG_OK;
+}
+
+int mpdg_heat_op_clear_tangent(mpdg_heat_op *op) {
+  if (!op) return MPDG_ERR_INVALID;
+  op->tangent = false;
+  op->td_bound = false;
+  op->td.release();
+  return MPDG_OK;
+}
+
+int mpdg_heat_op_bind_temperature(mpdg_heat_op *op, const double *T, const double *T_old, void *stream) {
+  if (!op || !T || !op->tangent) return MPDG_ERR_INVALID;
+  if (op->dt == 0.0) return MPDG_ERR_ZERO_TIME_INCREMENT;
+  MPDG_TRY(launch_tangent_data(op->d64, op->dt, op->mat, op->solid, op->vel, T, T_old ? T_old : T, op->td.get(),
+                               S(stream)));
+  op->td_bound = true;
+  return MPDG_OK;
+}
+
 int mpdg_heat_op_set_timing(mpdg_heat_op *op, int enable) {
   if (!op) return MPDG_ERR_INVALID;
   op->timing_on = enable != 0;
@@ -942,6 +1112,7 @@ int mpdg_precond_create(mpdg_heat_op *op, int type, const mpdg_gmg_params *gmg, 
   if (type == MPDG_PRECOND_IDENTITY) p = new (std::nothrow) IdentityPrecond;
   else if (type == MPDG_PRECOND_DIAGONAL) p = new (std::nothrow) DiagonalPrecond;
   else if (type == MPDG_PRECOND_GMG) {
+    if (op->tangent) return MPDG_ERR_INVALID;  // the level operators are the symmetric ones
     if (!gmg || gmg->smoother_degree < 1 || gmg->smoothing_range <= 1.0 || gmg->eig_cg_n_iterations < 1 ||
         gmg->pre_smoother_degree < 0 || gmg->pre_smoother_degree > gmg->smoother_degree)
       return MPDG_ERR_INVALID;
@@ -994,10 +1165,16 @@ int mpdg_gmg_level_info(const mpdg_precond *p, int level, int *degree, int *nc, 
   return p->level_info(level, degree, nc, lambda);
 }
 
+static bool valid_solver(const mpdg_solver_params &p) {
+  if (p.solver_type != MPDG_SOLVER_CG && p.solver_type != MPDG_SOLVER_FCG && p.solver_type != MPDG_SOLVER_GMRES)
+    return false;
+  return p.gmres_restart >= 0 && p.gmres_restart < kMdotMax;
+}
+
 int mpdg_linear_solve(mpdg_heat_op *op, mpdg_precond *p, const mpdg_solver_params *prm, const double *b, double *x,
                       int *iterations, double *res, void *stream) {
   if (!op || !prm || !b || !x || !iterations || !res || op->precision != MPDG_FP64) return MPDG_ERR_INVALID;
-  if (prm->solver_type != MPDG_SOLVER_CG && prm->solver_type != MPDG_SOLVER_FCG) return MPDG_ERR_INVALID;
+  if (!valid_solver(*prm)) return MPDG_ERR_INVALID;
   KrylovWork w;
   return krylov_solve(op, p, *prm, b, x, w, iterations, res, S(stream));
 }
@@ -1007,6 +1184,7 @@ int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p, const mpdg_hea
   if (!op || !prm || !out || op->precision != MPDG_FP64) return MPDG_ERR_INVALID;
   if (prm->predictor < MPDG_PREDICTOR_NONE || prm->predictor > MPDG_PREDICTOR_LEARNED) return MPDG_ERR_INVALID;
   if (prm->step_mode != MPDG_STEP_NEWTON && prm->step_mode != MPDG_STEP_LINEAR) return MPDG_ERR_INVALID;
+  if (!valid_solver(prm->linear)) return MPDG_ERR_INVALID;
   auto *h = new (std::nothrow) mpdg_heat_operation;
   if (!h) return MPDG_ERR_NOMEM;
   h->op = op;
@@ -1122,7 +1300,7 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
                        int *nl_iters, hipStream_t s) {
   mpdg_heat_op *op = h->op;
   const long n = op->n_dofs();
-  if (op->d64.emissivity > 0.0) return MPDG_ERR_INVALID;
+  if (op->d64.emissivity > 0.0 || op->tangent) return MPDG_ERR_INVALID;
   if (op->d64.kappa_mode == KAPPA_LAW) op->d64.T = T_old;
   if (op->d64.robin_mask) op->T_robin = T_old;
   const bool coeff_changes = op->d64.kappa_mode == KAPPA_LAW;
@@ -1214,6 +1392,8 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
   h->fell_back = 0;
   if (h->prm.step_mode == MPDG_STEP_LINEAR) return linear_step(h, dt, T, T_old, lin_iters, nl_iters, s);
   MPDG_TRY(learned_guess(h, dt, nullptr, T, T_old, s));
+  // the full tangent is evaluated at the current iterate (and T_old)
+  if (op->tangent) MPDG_TRY(mpdg_heat_op_bind_temperature(op, T, T_old, s));
   // preconditioner.update() at the predicted state (heat_diffuse_operation.cpp:276)
   if (h->pc) MPDG_TRY(h->pc->update(s));
   // NewtonRaphsonSolver::solve (newton_raphson_solver.cpp:24-66)
@@ -1221,6 +1401,7 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
   bool converged = false;
   const int max_it = std::max(1, h->prm.max_nonlinear_iterations);
   for (it = 0; it < max_it; ++it) {
+    if (op->tangent && it > 0) MPDG_TRY(mpdg_heat_op_bind_temperature(op, T, T_old, s));
     MPDG_TRY(launch_fill<double>(h->rhs.get(), 0.0, n, s));
     MPDG_TRY(op->create_rhs(h->rhs.get(), T, T_old, nullptr, s));
     MPDG_TRY(launch_fill<double>(h->du.get(), 0.0, n, s));

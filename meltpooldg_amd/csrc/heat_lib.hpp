@@ -136,6 +136,13 @@ struct mpdg_heat_op {
   mutable mpdg::DeviceArray<float> metric32_k, kq32;
   const double *coords = nullptr;
   mutable mpdg::ReductionScratch red;
+  // full Newton tangent (mpdg_heat_op_set_tangent): material, velocity, per-qp data at the bound T
+  bool tangent = false;
+  mpdg_heat_material mat{};
+  int solid = 0;
+  double vel[3] = {0, 0, 0};
+  mpdg::DeviceArray<double> td;
+  bool td_bound = false;
   // optional HIP-event timing of the cell-apply kernel (bench / profiling)
   mutable std::vector<std::pair<hipEvent_t, hipEvent_t>> timing_events;
   mutable size_t timing_used = 0;

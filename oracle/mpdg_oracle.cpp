@@ -17,6 +17,11 @@
 //   conductivity laws      pinned: unit_tests/material.output (mushy zone) and
 //                          the closed form of the tanh law.
 //   CG                     pinned: scipy.sparse.linalg.cg on the brute-force matrix.
+//   full Newton tangent    pinned: central differences of the residual (tangent = its
+//   (solidification,       derivative), the golden ||T||_L2 sequences of
+//   velocity), material    unidirectional_heat_transfer_with_velocity{,_and_solidification}
+//                          .mpirun=4.output, the cubic blend's closed form.
+//   GMRES                  pinned: numpy's dense solve of the assembled tangent.
 //   GMG pieces             pinned: transfer exactness on polynomials, two-grid
 //                          contraction, Chebyshev polynomial identity; the V-cycle
 //                          as a whole is "parity unpinned" against the reference

@@ -1,0 +1,116 @@
+"""GPU parity of the full Newton tangent (temperature-dependent liquid/solid material and a
+prescribed velocity, heat_diffuse_operator.cpp:1028-1056): apply, diagonal and residual against
+the oracle, GMRES against the oracle's solution, and the reference's golden norm sequences of the
+velocity and velocity + solidification cases run through HeatDiffuseOperation with GMRES."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.test_oracle import MUSHY, TANGENT_GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+GPU_MUSHY = dict(conductivity=MUSHY["k"], specific_heat=MUSHY["cp"], density=MUSHY["rho"],
+                 solidus=MUSHY["T_solidus"], liquidus=MUSHY["T_liquidus"])
+DT = 0.01
+
+
+def pair(torch, M, degree, nc, L, dmask, solid, vel, seed=0):
+    """Oracle and GPU operators with the tangent at a temperature field that crosses the mushy interval."""
+    rng = np.random.default_rng(seed)
+    orc = O.Operator(degree, nc, L, dirichlet_mask=dmask)
+    X = orc.node_coords()
+    T = 1970.0 + 6.0 * np.sin(7 * X[0] + 1.0) * np.cos(5 * X[1]) + 2.0 * X[2] + 0.1 * rng.standard_normal(orc.n_dofs)
+    T_old = T - 0.4 * np.cos(4 * X[0]) - 0.05 * rng.standard_normal(orc.n_dofs)
+    orc.set_tangent(DT, material=MUSHY, velocity=vel, do_solidification=solid, T_old=T_old)
+    orc.set_T(T)
+    op = M.HeatDiffuseMultiPhaseOperator(degree, nc, L, dirichlet_faces=dmask)
+    op.reset_time_increment(DT)
+    op.set_tangent(GPU_MUSHY, velocity=vel, do_solidification=solid)
+    Td, Tod = (torch.tensor(a, device="cuda") for a in (T, T_old))
+    op.bind_temperature(Td, Tod)
+    return orc, op, T, T_old, Td, Tod
+
+
+CASES = [(1, (5, 4, 3), (0.5, 0.4, 0.3), 0, True, (-0.3, 0.1, 0.05)),
+         (2, (3, 4, 2), (0.3, 0.4, 0.2), 0b000011, True, (0.2, -0.4, 0.1)),
+         (3, (3, 2, 2), (0.3, 0.2, 0.2), 0, False, (-0.5, 0.0, 0.0)),
+         (4, (2, 2, 3), (0.2, 0.2, 0.3), 0b110000, True, (0.0, 0.3, -0.2)),
+         (2, (7, 5, 6), (0.7, 0.5, 0.6), 0b001100, True, (0.1, 0.1, 0.1))]
+
+
+@pytest.mark.parametrize("degree,nc,L,dmask,solid,vel", CASES)
+def test_tangent_vmult_matches_oracle(cuda, degree, nc, L, dmask, solid, vel):
+    torch = cuda
+    import meltpooldg_amd as M
+    orc, op, *_ = pair(torch, M, degree, nc, L, dmask, solid, vel)
+    src = np.random.default_rng(1).uniform(-1, 1, orc.n_dofs)
+    dst = torch.empty(orc.n_dofs, dtype=torch.float64, device="cuda")
+    op.vmult(dst, torch.tensor(src, device="cuda"))
+    ref = orc.vmult(src, threads=1)
+    assert np.linalg.norm(dst.cpu().numpy() - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("degree,nc,L,dmask,solid,vel", CASES[:4])
+def test_tangent_diagonal_and_residual_match_oracle(cuda, degree, nc, L, dmask, solid, vel):
+    torch = cuda
+    import meltpooldg_amd as M
+    orc, op, T, T_old, Td, Tod = pair(torch, M, degree, nc, L, dmask, solid, vel, seed=3)
+    d = torch.empty(orc.n_dofs, dtype=torch.float64, device="cuda")
+    op.compute_diagonal(d)
+    ref = orc.diagonal()
+    assert np.linalg.norm(d.cpu().numpy() - ref) <= 1e-12 * np.linalg.norm(ref)
+    q = np.random.default_rng(4).uniform(0, 100, orc.n_dofs)
+    r = torch.full((orc.n_dofs,), 0.5, dtype=torch.float64, device="cuda")  # create_rhs accumulates
+    op.create_rhs(r, Td, Tod, torch.tensor(q, device="cuda"))
+    ref = orc.create_rhs(np.full(orc.n_dofs, 0.5), T, T_old, q)
+    assert np.linalg.norm(r.cpu().numpy() - ref) <= 1e-11 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("restart", [0, 5])
+def test_gmres_matches_oracle(cuda, restart):
+    """J x = b with the non-symmetric tangent: GMRES + inverse diagonal on the GPU and in the oracle reach
+    the same solution; the iteration counts agree within 2 (same Krylov spaces, different rounding)."""
+    torch = cuda
+    import meltpooldg_amd as M
+    orc, op, *_ = pair(torch, M, 2, (6, 5, 4), (0.6, 0.5, 0.4), 0, True, (-0.8, 0.3, 0.1), seed=5)
+    b = np.random.default_rng(6).uniform(-1, 1, orc.n_dofs)
+    xr, itr, _ = orc.gmres(b, rel_tol=1e-12, precond=1, restart=restart or 28, criterion=1, threads=1)
+    pc = M.JacobiPreconditioner(op)
+    pc.update()
+    x = torch.zeros(orc.n_dofs, dtype=torch.float64, device="cuda")
+    it = M.LinearSolver.solve(op, x, torch.tensor(b, device="cuda"),
+                              M.LinearSolverData(solver_type=M.SOLVER_GMRES, rel_tolerance=1e-12,
+                                                 criterion=M.CRIT_RHS, gmres_restart=restart), pc)
+    assert itr > 0 and abs(it - itr) <= 2
+    assert np.linalg.norm(x.cpu().numpy() - xr) <= 1e-9 * np.linalg.norm(xr)
+
+
+@pytest.mark.parametrize("case", sorted(TANGENT_GOLDEN))
+def test_tangent_golden_norm_sequences(cuda, case):
+    """2D Q1 32x32 on [0,0.1]^2 as a one-cell-thick 3D mesh; Newton with GMRES + Jacobi."""
+    torch = cuda
+    import meltpooldg_amd as M
+    nc, L = (32, 32, 1), (0.1, 0.1, 1.0)
+    op = M.HeatDiffuseMultiPhaseOperator(1, nc, L)
+    orc = O.Operator(1, nc, L)
+    if case == "velocity":
+        dt, tol = 1e-3, 1e-6
+        op.set_tangent(dict(conductivity=55.563, specific_heat=460.0, density=7850.0), velocity=(-0.01, 0.0, 0.0))
+        T0 = np.full(orc.n_dofs, 1000.0)
+        placeholder = torch.tensor(T0, device="cuda")
+        op.set_robin(2, alpha=25.0, T_inf_conv=300.0, emissivity=0.98, T_inf_rad=300.0, T=placeholder)
+    else:
+        dt, tol = 0.01, 1e-7
+        op.set_tangent(GPU_MUSHY, velocity=(-0.005, 0.0, 0.0), do_solidification=True)
+        T0 = 1960.0 + 20.0 * orc.node_coords()[0] / 0.1
+    pc = M.JacobiPreconditioner(op)
+    heat = M.HeatDiffuseOperation(op, pc, predictor=M.PREDICTOR_LINEAR,
+                                  linear=M.LinearSolverData(solver_type=M.SOLVER_GMRES, rel_tolerance=1e-12),
+                                  nlsolve=M.NonlinearSolverData(residual_tolerance=tol))
+    heat.set_initial_condition(torch.tensor(T0, device="cuda"))
+    norms = []
+    for _ in range(5):
+        heat.solve(dt)
+        norms.append(op.l2_norm(heat.solution()))
+    assert ["%.5e" % v for v in norms] == ["%.5e" % v for v in TANGENT_GOLDEN[case]]
