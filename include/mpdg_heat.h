@@ -234,6 +234,26 @@ const double *mpdg_heat_operation_old_solution(const mpdg_heat_operation *h);
 /* nodal field of the z+ surface flux of create_rhs (zero off the z+ face), for the network input */
 int mpdg_heat_op_surface_flux_field(const mpdg_heat_op *op, double *out, void *stream);
 
+/* --- data formats on either side of the path --- */
+/* Paraview output as Postprocessor::do_paraview_output writes it
+   (source/post_processing/postprocessor.cpp:195-270, DataOut::build_patches(mapping, k)): every
+   cell subdivided into k^3 equal hexahedra whose vertices carry the FE fields' values (shared
+   between neighbouring cells), base64 binary VTU. fields: n_fields host arrays of n_dofs
+   lexicographic values, e.g. "temperature", "temperature_old", "heat_source"
+   (heat_diffuse_operation.cpp:365-405). */
+int mpdg_io_write_vtu(const char *filename, int degree, const int *n_cells, const double *extent, int n_fields,
+                      const char *const *names, const double *const *fields);
+/* .pvd record relating the VTU files to times (DataOutBase::write_pvd_record) */
+int mpdg_io_write_pvd(const char *filename, int n, const double *times, const char *const *vtu_names);
+/* DoF numbering of a DoFHandler on the same box: points (host, n x 3) are its support points
+   (DoFTools::map_dofs_to_support_points); lex_index[i] (host) = lexicographic node of DoF i.
+   MPDG_ERR_INVALID when a point is off the node grid or two DoFs share a node. */
+int mpdg_io_dof_permutation(int degree, const int *n_cells, const double *extent, const double *points, long n,
+                            long *lex_index);
+/* device vectors: to_lex: dst[lex_index[i]] = src[i]; else dst[i] = src[lex_index[i]]
+   (lex_index: a device copy of mpdg_io_dof_permutation's output; src != dst) */
+int mpdg_io_permute(const long *lex_index, const double *src, double *dst, long n, int to_lex, void *stream);
+
 /* --- learned initial guess: 3D CNN (4 -> 16 -> 16 -> 16 -> 1 channels, 3x3x3 taps, ReLU) on the
        DoF grid, BF16 MFMA kernels, FP32 master weights, Adam. Enters the predictor slot of
        include/meltpooldg/linear_algebra/predictor.hpp:118-160 as a new predictor type. --- */

@@ -81,6 +81,11 @@ _V, _I, _D, _L = C.c_void_p, C.c_int, C.c_double, C.c_long
 _SIGNATURES = {
     "mpdg_heat_op_create": ([C.POINTER(HeatDesc), C.POINTER(_V)], _I),
     "mpdg_heat_op_set_tangent": ([_V, C.POINTER(Material), _I, C.POINTER(_D)], _I),
+    "mpdg_io_write_vtu": ([C.c_char_p, _I, C.POINTER(_I), C.POINTER(_D), _I, C.POINTER(C.c_char_p),
+                           C.POINTER(C.POINTER(_D))], _I),
+    "mpdg_io_write_pvd": ([C.c_char_p, _I, C.POINTER(_D), C.POINTER(C.c_char_p)], _I),
+    "mpdg_io_dof_permutation": ([_I, C.POINTER(_I), C.POINTER(_D), C.POINTER(_D), _L, C.POINTER(_L)], _I),
+    "mpdg_io_permute": ([_V, _V, _V, _L, _I, _V], _I),
     "mpdg_heat_op_clear_tangent": ([_V], _I),
     "mpdg_heat_op_bind_temperature": ([_V, _V, _V, _V], _I),
     "mpdg_heat_op_destroy": ([_V], _I),
@@ -598,3 +603,71 @@ def _copy_device(dst, src_ptr, n):
     if rc != 0:
         raise MpdgError(-5, "hipMemcpy")
     torch.cuda.synchronize()
+
+
+# ---------------------------------------------------------------------------
+# Data formats: Paraview output (Postprocessor::do_paraview_output,
+# source/post_processing/postprocessor.cpp:195-270) and the DoF numbering of a
+# DoFHandler vs. the lexicographic node grid.
+# ---------------------------------------------------------------------------
+def write_vtu(filename, degree, n_cells, extent, fields):
+    """fields: {name: lexicographic DoF values (numpy array or tensor, copied to the host)}."""
+    import numpy as np
+    lib = load_library()
+    names = list(fields)
+    arrays = []
+    for nm in names:
+        v = fields[nm]
+        if hasattr(v, "detach"):
+            v = v.detach().cpu().numpy()
+        arrays.append(np.ascontiguousarray(v, dtype=np.float64))
+    c_names = (C.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
+    c_fields = (C.POINTER(_D) * max(1, len(arrays)))(*[a.ctypes.data_as(C.POINTER(_D)) for a in arrays])
+    _check(lib.mpdg_io_write_vtu(str(filename).encode(), degree, (_I * 3)(*n_cells), (_D * 3)(*extent), len(names),
+                                 c_names, c_fields), "write_vtu")
+
+
+def write_pvd(filename, times, vtu_names):
+    lib = load_library()
+    n = len(times)
+    _check(lib.mpdg_io_write_pvd(str(filename).encode(), n, (_D * max(1, n))(*times),
+                                 (C.c_char_p * max(1, n))(*[str(v).encode() for v in vtu_names])), "write_pvd")
+
+
+class ParaviewOutput:
+    """One .vtu per output step and a .pvd record (the reference's paraview output: file names
+    <filename>_<step with n_digits>.vtu, fields temperature / temperature_old / heat_source)."""
+
+    def __init__(self, directory, op, filename="solution", n_digits=4):
+        import os
+        os.makedirs(directory, exist_ok=True)
+        self.dir, self.op, self.filename, self.n_digits = directory, op, filename, n_digits
+        self.records = []
+
+    def write(self, step, time, fields):
+        import os
+        name = "%s_%0*d.vtu" % (self.filename, self.n_digits, step)
+        write_vtu(os.path.join(self.dir, name), self.op.degree, self.op.n_cells, self.op.extent, fields)
+        if not self.records or self.records[-1][1] != name:
+            self.records.append((time, name))
+        write_pvd(os.path.join(self.dir, self.filename + ".pvd"), [t for t, _ in self.records],
+                  [n for _, n in self.records])
+        return name
+
+
+def dof_permutation(degree, n_cells, extent, points):
+    """lex_index[i] = lexicographic node of DoF i, from a DoFHandler's support points (n x 3)."""
+    import numpy as np
+    lib = load_library()
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(len(pts), dtype=np.int64)
+    _check(lib.mpdg_io_dof_permutation(degree, (_I * 3)(*n_cells), (_D * 3)(*extent), pts.ctypes.data_as(C.POINTER(_D)),
+                                       len(pts), out.ctypes.data_as(C.POINTER(_L))), "dof_permutation")
+    return out
+
+
+def permute(lex_index, src, dst, to_lex=True):
+    """Device vectors: to_lex: dst[lex_index[i]] = src[i]; else dst[i] = src[lex_index[i]]."""
+    lib = load_library()
+    _check(lib.mpdg_io_permute(_ptr(lex_index, _torch().int64), _ptr(src), _ptr(dst), src.numel(),
+                               1 if to_lex else 0, _stream()), "permute")

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmpdg_heat.so")
 SOURCES = ["heat_apply.hip", "heat_aux.hip", "blas.hip", "gmg_kernels.hip", "tangent.hip", "cnn.hip", "heat_lib.cpp",
-           "learned.cpp"]
+           "learned.cpp", "io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function",
