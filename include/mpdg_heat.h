@@ -188,6 +188,8 @@ typedef struct mpdg_heat_operation mpdg_heat_operation;
 #define MPDG_PREDICTOR_ZERO 1
 #define MPDG_PREDICTOR_LINEAR 2
 #define MPDG_PREDICTOR_LEARNED 3
+#define MPDG_PREDICTOR_EXTRAPOLATION 4 /* polynomial extrapolation of order extrapolation_order
+                                          (1: linear_extrapolation) through the last solutions */
 #define MPDG_STEP_NEWTON 0 /* Newton-Raphson as the reference (>= 2 linear solves per step) */
 #define MPDG_STEP_LINEAR 1 /* one linear solve J(T_n) T_{n+1} = b per step, coefficient frozen at T_n,
                               stopped at ||b - J T|| <= rel_tolerance ||b|| (no radiation faces) */
@@ -202,6 +204,8 @@ typedef struct mpdg_heat_operation_params {
   int guess_smoothing;    /* learned predictor (with guess_check, linear step mode): smoothing
                              steps on J T = b applied to an accepted guess (multigrid: Chebyshev
                              steps of its finest level; 0: none) */
+  int extrapolation_order;/* MPDG_PREDICTOR_EXTRAPOLATION: 1..4 (the order drops to what the
+                             history holds: after set_initial the first steps use fewer points) */
 } mpdg_heat_operation_params;
 int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p,
                                const mpdg_heat_operation_params *params,

@@ -33,7 +33,7 @@ PRECOND_IDENTITY, PRECOND_DIAGONAL, PRECOND_GMG = 0, 1, 2
 SOLVER_CG, SOLVER_FCG, SOLVER_GMRES = 0, 1, 2
 TRANSITION_NONE, TRANSITION_MUSHY = 0, 1
 CRIT_REDUCTION, CRIT_RHS = 0, 1
-PREDICTOR_NONE, PREDICTOR_ZERO, PREDICTOR_LINEAR, PREDICTOR_LEARNED = 0, 1, 2, 3
+PREDICTOR_NONE, PREDICTOR_ZERO, PREDICTOR_LINEAR, PREDICTOR_LEARNED, PREDICTOR_EXTRAPOLATION = 0, 1, 2, 3, 4
 
 
 class MpdgError(RuntimeError):
@@ -71,7 +71,8 @@ class Material(C.Structure):
 class OperationParams(C.Structure):
     _fields_ = [("predictor", C.c_int), ("linear", SolverParams), ("max_nonlinear_iterations", C.c_int),
                 ("residual_tolerance", C.c_double), ("field_correction_tolerance", C.c_double),
-                ("step_mode", C.c_int), ("guess_check", C.c_int), ("guess_smoothing", C.c_int)]
+                ("step_mode", C.c_int), ("guess_check", C.c_int), ("guess_smoothing", C.c_int),
+                ("extrapolation_order", C.c_int)]
 
 
 STEP_NEWTON, STEP_LINEAR = 0, 1
@@ -437,7 +438,8 @@ class HeatDiffuseOperation:
     """Implicit heat time step: predictor, preconditioner update, Newton-Raphson with CG."""
 
     def __init__(self, op, preconditioner=None, predictor=PREDICTOR_LINEAR, linear=None, nlsolve=None,
-                 step_mode=STEP_NEWTON, learned=None, q_field=None, guess_check=True, guess_smoothing=0):
+                 step_mode=STEP_NEWTON, learned=None, q_field=None, guess_check=True, guess_smoothing=0,
+                 extrapolation_order=2):
         self.op = op
         self.lib = op.lib
         self.pc = preconditioner
@@ -453,6 +455,7 @@ class HeatDiffuseOperation:
         p.step_mode = step_mode
         p.guess_check = 1 if guess_check else 0
         p.guess_smoothing = guess_smoothing
+        p.extrapolation_order = extrapolation_order
         h = C.c_void_p()
         _check(self.lib.mpdg_heat_operation_create(op.h, preconditioner.h if preconditioner else None,
                                                    C.byref(p), C.byref(h)), "heat_operation_create")

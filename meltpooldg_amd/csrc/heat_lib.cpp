@@ -894,9 +894,14 @@ struct mpdg_heat_operation {
   mpdg_heat_op *op = nullptr;
   mpdg_precond *pc = nullptr;
   mpdg_heat_operation_params prm{};
-  DeviceArray<double> buf[3], rhs, du;
+  static constexpr int kMaxOlder = 3;  // solutions kept before T_{n-1} (extrapolation order <= 4)
+  DeviceArray<double> buf[3 + kMaxOlder], rhs, du;
   int cur = 0, old = 1, spare = 2;
+  int older[kMaxOlder] = {3, 4, 5};  // buffers of T_{n-2}, T_{n-3}, ...
+  int depth = 0;                     // older buffers in use (extrapolation order - 1)
+  int n_older = 0;                   // of those, how many hold past solutions
   double dt_old = 0.0;
+  double dt_hist[kMaxOlder + 1] = {};  // dt_hist[j] = t_{n-j} - t_{n-j-1} (dt_hist[0] = dt_old)
   double dt_precond = 0.0;
   KrylovWork work;
   ReductionScratch red;
@@ -1182,9 +1187,12 @@ int mpdg_linear_solve(mpdg_heat_op *op, mpdg_precond *p, const mpdg_solver_param
 int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p, const mpdg_heat_operation_params *prm,
                                mpdg_heat_operation **out) {
   if (!op || !prm || !out || op->precision != MPDG_FP64) return MPDG_ERR_INVALID;
-  if (prm->predictor < MPDG_PREDICTOR_NONE || prm->predictor > MPDG_PREDICTOR_LEARNED) return MPDG_ERR_INVALID;
+  if (prm->predictor < MPDG_PREDICTOR_NONE || prm->predictor > MPDG_PREDICTOR_EXTRAPOLATION) return MPDG_ERR_INVALID;
   if (prm->step_mode != MPDG_STEP_NEWTON && prm->step_mode != MPDG_STEP_LINEAR) return MPDG_ERR_INVALID;
   if (!valid_solver(prm->linear)) return MPDG_ERR_INVALID;
+  if (prm->predictor == MPDG_PREDICTOR_EXTRAPOLATION &&
+      (prm->extrapolation_order < 1 || prm->extrapolation_order > mpdg_heat_operation::kMaxOlder + 1))
+    return MPDG_ERR_INVALID;
   auto *h = new (std::nothrow) mpdg_heat_operation;
   if (!h) return MPDG_ERR_NOMEM;
   h->op = op;
@@ -1192,7 +1200,8 @@ int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p, const mpdg_hea
   h->prm = *prm;
   const long n = op->n_dofs();
   int rc = MPDG_OK;
-  for (auto &b : h->buf) rc = rc ? rc : b.resize(n);
+  h->depth = prm->predictor == MPDG_PREDICTOR_EXTRAPOLATION ? prm->extrapolation_order - 1 : 0;
+  for (int b = 0; b < 3 + h->depth; ++b) rc = rc ? rc : h->buf[b].resize(n);
   rc = rc ? rc : h->rhs.resize(n);
   rc = rc ? rc : h->du.resize(n);
   rc = rc ? rc : h->red.init();
@@ -1218,6 +1227,7 @@ int mpdg_heat_operation_set_initial(mpdg_heat_operation *h, const double *T0, vo
   MPDG_TRY(launch_convert<double, double>(T0, h->buf[h->cur].get(), n, S(stream)));
   MPDG_TRY(launch_convert<double, double>(T0, h->buf[h->old].get(), n, S(stream)));
   h->dt_old = 0.0;
+  h->n_older = 0;
   return hipStreamSynchronize(S(stream)) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
@@ -1259,10 +1269,16 @@ static int learned_guess(mpdg_heat_operation *h, double dt, const double *b, dou
   // the network extrapolates with equal steps (as in its training trajectories)
   const bool equal_steps = h->dt_old > 0.0 && std::fabs(dt - h->dt_old) <= 1e-12 * dt;
   if (learned && equal_steps) {
-    const double *Tnm1 = h->buf[h->spare].get();  // T_{n-1} until the next step overwrites it
+    const double *Tnm1 = h->buf[h->depth > 0 ? h->older[0] : h->spare].get();  // T_{n-1} of this step
     MPDG_TRY(mpdg_learned_predict(h->net, Tnm1, T_old, h->q_field, z, T, s));
   }
   return MPDG_OK;
+}
+
+static void commit_time_increment(mpdg_heat_operation *h, double dt) {
+  for (int j = mpdg_heat_operation::kMaxOlder; j > 0; --j) h->dt_hist[j] = h->dt_hist[j - 1];
+  h->dt_hist[0] = dt;
+  h->dt_old = dt;
 }
 
 // A conductivity law k(T) frozen at T_n for one linear step: evaluated once per quadrature
@@ -1346,10 +1362,40 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
   if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
   h->rel_res0 = std::sqrt(st.rr0 / st.bb);
   h->rel_res = std::sqrt(st.rr / st.bb);
-  h->dt_old = dt;
+  commit_time_increment(h, dt);
   if (lin_iters) *lin_iters = li;
   if (nl_iters) *nl_iters = 1;
   return rc;
+}
+
+// Polynomial extrapolation of order p through T_n, T_{n-1}, ..., T_{n-p} at their times to
+// t_{n+1} = t_n + dt (Lagrange weights; p = 1 is compute_linear_predictor,
+// source/linear_algebra/predictor_linear.cpp:76-97). The order drops to what the history holds.
+static int extrapolate(mpdg_heat_operation *h, double dt, double *pred, hipStream_t s) {
+  const long n = h->op->n_dofs();
+  const double *pts[mpdg_heat_operation::kMaxOlder + 2];
+  double tau[mpdg_heat_operation::kMaxOlder + 2];
+  int m = 1;
+  pts[0] = h->buf[h->cur].get();
+  tau[0] = 0.0;
+  if (h->dt_old > 0.0) {
+    pts[m] = h->buf[h->old].get();
+    tau[m] = -h->dt_hist[0];
+    ++m;
+    for (int j = 0; j < h->n_older && m <= h->prm.extrapolation_order; ++j, ++m) {
+      pts[m] = h->buf[h->older[j]].get();
+      tau[m] = tau[m - 1] - h->dt_hist[j + 1];
+    }
+  }
+  m = std::min(m, h->prm.extrapolation_order + 1);
+  for (int j = 0; j < m; ++j) {
+    double w = 1.0;
+    for (int i = 0; i < m; ++i)
+      if (i != j) w *= (dt - tau[i]) / (tau[j] - tau[i]);
+    if (j == 0) MPDG_TRY(launch_scale(pred, pts[0], w, n, s));
+    else MPDG_TRY(launch_axpby(pred, w, pts[j], 1.0, n, s));
+  }
+  return MPDG_OK;
 }
 
 // HeatDiffuseOperation::init_time_advance + solve (heat_diffuse_operation.cpp:215-289)
@@ -1378,10 +1424,18 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
       }
       break;
     }
+    case MPDG_PREDICTOR_EXTRAPOLATION: MPDG_TRY(extrapolate(h, dt, pred, s)); break;
     default: return MPDG_ERR_INVALID;
   }
-  // commit_old_solutions + swap: old <- T_n, cur <- prediction
-  const int new_old = h->cur, new_cur = h->spare, new_spare = h->old;
+  // commit_old_solutions + swap: old <- T_n, cur <- prediction (older <- T_{n-1}, ...)
+  int new_spare = h->old;
+  if (h->depth > 0) {
+    new_spare = h->older[h->depth - 1];
+    for (int j = h->depth - 1; j > 0; --j) h->older[j] = h->older[j - 1];
+    h->older[0] = h->old;
+    h->n_older = h->dt_old > 0.0 ? std::min(h->n_older + 1, h->depth) : 0;
+  }
+  const int new_old = h->cur, new_cur = h->spare;
   h->old = new_old;
   h->cur = new_cur;
   h->spare = new_spare;
@@ -1426,7 +1480,7 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
     }
     MPDG_TRY(launch_axpby(T, 1.0, h->du.get(), 1.0, n, s));
   }
-  h->dt_old = dt;
+  commit_time_increment(h, dt);
   if (lin_iters) *lin_iters = acc;
   if (nl_iters) *nl_iters = it + 1;
   if (!converged) return MPDG_ERR_NO_CONVERGENCE;  // ExcNewtonDidNotConverge

@@ -164,3 +164,34 @@ def test_linear_step_with_conductivity_law(cuda, precond):
     # the conductivity varies over the field, so a constant-k solve would not pass the check
     kq = np.array([O.kappa_law(law[0], law[1], t)[0] for t in T_old.cpu().numpy()[::7]])
     assert kq.max() - kq.min() > 0.3
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_extrapolation_predictor(cuda, order):
+    """The step's initial residual is that of the Lagrange extrapolation (order 2, 3) of the recorded
+    solutions to the new time, with variable step sizes."""
+    torch = cuda
+    import meltpooldg_amd as M
+    from meltpooldg_amd.application import LaserScan
+    case = LaserScan(2, 8, dt=1e-3, precond="gmg64", predictor=M.PREDICTOR_EXTRAPOLATION, extrapolation_order=order,
+                     n_cells_coarse=2)
+    Ts, ts = [case.solution()], [0.0]
+    for dt in (1e-3, 1e-3, 1.5e-3, 1e-3, 0.8e-3):
+        case.dt = dt
+        case.step()
+        Ts.append(case.solution())
+        ts.append(case.t)
+    dt = 1.2e-3
+    t_new = ts[-1] + dt
+    pts, tau = Ts[-1 - order:][::-1], [t - ts[-1] for t in ts[-1 - order:][::-1]]
+    w = [np.prod([(dt - tau[i]) / (tau[j] - tau[i]) for i in range(order + 1) if i != j]) for j in range(order + 1)]
+    guess = sum(wj * T for wj, T in zip(w, pts))
+    case.dt = dt
+    _, _, (_, r0, _) = case.step()
+    assert abs(case.t - t_new) < 1e-15
+    b = torch.zeros_like(guess)
+    case.op.create_rhs(b, torch.zeros_like(guess), Ts[-1])
+    Jg = torch.empty_like(guess)
+    case.op.vmult(Jg, guess)
+    expect = float(torch.linalg.norm(b - Jg) / torch.linalg.norm(b))
+    assert abs(r0 - expect) <= 1e-6 * expect
