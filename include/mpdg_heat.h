@@ -190,6 +190,9 @@ typedef struct mpdg_heat_operation mpdg_heat_operation;
 #define MPDG_PREDICTOR_LEARNED 3
 #define MPDG_PREDICTOR_EXTRAPOLATION 4 /* polynomial extrapolation of order extrapolation_order
                                           (1: linear_extrapolation) through the last solutions */
+#define MPDG_PREDICTOR_LSQ 5 /* least_squares_projection (predictor.hpp:19-86): the combination of
+                                the last n_old_solution_vectors solutions whose residual
+                                ||b - J x|| is smallest, b = create_rhs(T = 0, T_old = T_n) */
 #define MPDG_STEP_NEWTON 0 /* Newton-Raphson as the reference (>= 2 linear solves per step) */
 #define MPDG_STEP_LINEAR 1 /* one linear solve J(T_n) T_{n+1} = b per step, coefficient frozen at T_n,
                               stopped at ||b - J T|| <= rel_tolerance ||b|| (no radiation faces) */
@@ -204,8 +207,10 @@ typedef struct mpdg_heat_operation_params {
   int guess_smoothing;    /* learned predictor (with guess_check, linear step mode): smoothing
                              steps on J T = b applied to an accepted guess (multigrid: Chebyshev
                              steps of its finest level; 0: none) */
-  int extrapolation_order;/* MPDG_PREDICTOR_EXTRAPOLATION: 1..4 (the order drops to what the
+  int extrapolation_order;/* MPDG_PREDICTOR_EXTRAPOLATION: 1..6 (the order drops to what the
                              history holds: after set_initial the first steps use fewer points) */
+  int n_old_solution_vectors; /* MPDG_PREDICTOR_LSQ: 1..7 solutions (predictor_data.hpp:27-28;
+                                 until the history holds them, the previous solution is used) */
 } mpdg_heat_operation_params;
 int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p,
                                const mpdg_heat_operation_params *params,

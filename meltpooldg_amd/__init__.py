@@ -33,7 +33,8 @@ PRECOND_IDENTITY, PRECOND_DIAGONAL, PRECOND_GMG = 0, 1, 2
 SOLVER_CG, SOLVER_FCG, SOLVER_GMRES = 0, 1, 2
 TRANSITION_NONE, TRANSITION_MUSHY = 0, 1
 CRIT_REDUCTION, CRIT_RHS = 0, 1
-PREDICTOR_NONE, PREDICTOR_ZERO, PREDICTOR_LINEAR, PREDICTOR_LEARNED, PREDICTOR_EXTRAPOLATION = 0, 1, 2, 3, 4
+PREDICTOR_NONE, PREDICTOR_ZERO, PREDICTOR_LINEAR, PREDICTOR_LEARNED, PREDICTOR_EXTRAPOLATION, PREDICTOR_LSQ = \
+    0, 1, 2, 3, 4, 5
 
 
 class MpdgError(RuntimeError):
@@ -72,7 +73,7 @@ class OperationParams(C.Structure):
     _fields_ = [("predictor", C.c_int), ("linear", SolverParams), ("max_nonlinear_iterations", C.c_int),
                 ("residual_tolerance", C.c_double), ("field_correction_tolerance", C.c_double),
                 ("step_mode", C.c_int), ("guess_check", C.c_int), ("guess_smoothing", C.c_int),
-                ("extrapolation_order", C.c_int)]
+                ("extrapolation_order", C.c_int), ("n_old_solution_vectors", C.c_int)]
 
 
 STEP_NEWTON, STEP_LINEAR = 0, 1
@@ -439,7 +440,7 @@ class HeatDiffuseOperation:
 
     def __init__(self, op, preconditioner=None, predictor=PREDICTOR_LINEAR, linear=None, nlsolve=None,
                  step_mode=STEP_NEWTON, learned=None, q_field=None, guess_check=True, guess_smoothing=0,
-                 extrapolation_order=2):
+                 extrapolation_order=2, n_old_solution_vectors=2):
         self.op = op
         self.lib = op.lib
         self.pc = preconditioner
@@ -456,6 +457,7 @@ class HeatDiffuseOperation:
         p.guess_check = 1 if guess_check else 0
         p.guess_smoothing = guess_smoothing
         p.extrapolation_order = extrapolation_order
+        p.n_old_solution_vectors = n_old_solution_vectors
         h = C.c_void_p()
         _check(self.lib.mpdg_heat_operation_create(op.h, preconditioner.h if preconditioner else None,
                                                    C.byref(p), C.byref(h)), "heat_operation_create")

@@ -117,6 +117,40 @@ int launch_scale(double *y, const double *x, double a, long n, hipStream_t s) {
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
+struct CombineArgs {
+  const double *x[kCombineMax];
+  double w[kCombineMax];
+};
+template <int M>
+__global__ __launch_bounds__(256) void combine_kernel(double *__restrict__ y, CombineArgs a, long n) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) s = fma(a.w[j], a.x[j][i], s);
+    y[i] = s;
+  }
+}
+int launch_combine(double *y, const double *const *x, const double *w, int m, long n, hipStream_t s) {
+  if (m < 1 || m > kCombineMax) return MPDG_ERR_INVALID;
+  CombineArgs a{};
+  for (int j = 0; j < m; ++j) {
+    a.x[j] = x[j];
+    a.w[j] = w[j];
+  }
+  const dim3 g(std::max<long>(1, std::min<long>((n + 255) / 256, 8192)));
+  switch (m) {
+    case 1: hipLaunchKernelGGL(combine_kernel<1>, g, dim3(256), 0, s, y, a, n); break;
+    case 2: hipLaunchKernelGGL(combine_kernel<2>, g, dim3(256), 0, s, y, a, n); break;
+    case 3: hipLaunchKernelGGL(combine_kernel<3>, g, dim3(256), 0, s, y, a, n); break;
+    case 4: hipLaunchKernelGGL(combine_kernel<4>, g, dim3(256), 0, s, y, a, n); break;
+    case 5: hipLaunchKernelGGL(combine_kernel<5>, g, dim3(256), 0, s, y, a, n); break;
+    case 6: hipLaunchKernelGGL(combine_kernel<6>, g, dim3(256), 0, s, y, a, n); break;
+    case 7: hipLaunchKernelGGL(combine_kernel<7>, g, dim3(256), 0, s, y, a, n); break;
+    default: hipLaunchKernelGGL(combine_kernel<8>, g, dim3(256), 0, s, y, a, n); break;
+  }
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void invert_diag_kernel(T *d, long n) {
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {

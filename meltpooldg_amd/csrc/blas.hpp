@@ -48,6 +48,9 @@ int launch_mdot(const double *V, long n, int J, const double *w, double *part, d
                 hipStream_t s);
 int launch_maxpy(double *w, const double *V, long n, int J, const double *c, double sign, hipStream_t s);
 int launch_scale(double *y, const double *x, double a, long n, hipStream_t s);
+// y = sum_j w_j x_j over m <= kCombineMax vectors in one pass (y distinct from the x_j)
+constexpr int kCombineMax = 8;
+int launch_combine(double *y, const double *const *x, const double *w, int m, long n, hipStream_t s);
 
 template <typename T>
 int launch_chebyshev_step(T *x, T *d, const T *b, const T *Ax, const T *inv, long n, double c1, double c2,

@@ -98,8 +98,8 @@ int mpdg_heat_op::inverse_diagonal(double *inv, hipStream_t s) const {
 // As in the reference the difference T - T_old is formed before the mass is applied
 // (:493-497), which keeps the residual free of the cancellation between c_m M T and
 // c_m M T_old:  dst -= M_{c_m} (T - T_old - q/c_m) + K_{k(T)} T, then the boundary residual.
-int mpdg_heat_op::create_rhs(double *dst, const double *T, const double *T_old, const double *q,
-                             hipStream_t s) const {
+int mpdg_heat_op::create_rhs(double *dst, const double *T, const double *T_old, const double *q, hipStream_t s,
+                             bool T_zero) const {
   if (precision != MPDG_FP64) return MPDG_ERR_INVALID;
   if (dt == 0.0) return MPDG_ERR_ZERO_TIME_INCREMENT;
   if (tangent) {
@@ -110,6 +110,22 @@ int mpdg_heat_op::create_rhs(double *dst, const double *T, const double *T_old, 
   const long n = d64.n_dofs;
   MPDG_TRY(tmp1.resize(n));
   MPDG_TRY(tmp2.resize(n));
+  if (T_zero && !tangent) {
+    // dst += c_m M (T_old + q / c_m) + boundary terms at T = 0 (K T vanishes)
+    OperatorData<double> mass = d64;
+    mass.kappa_mode = KAPPA_CONST;
+    mass.kappa = 0.0;
+    mass.robin_mask = 0;
+    const double *src = T_old;
+    if (q) {
+      MPDG_TRY(launch_scale(tmp2.get(), T_old, 1.0, n, s));
+      MPDG_TRY(launch_axpby(tmp2.get(), 1.0 / d64.c_mass, q, 1.0, n, s));
+      src = tmp2.get();
+    }
+    MPDG_TRY(launch_apply<double>(mass, src, tmp1.get(), s));
+    MPDG_TRY(launch_axpby(dst, 1.0, tmp1.get(), 1.0, n, s));
+    return launch_boundary(d64, 2, flux, dst, nullptr, T, s);
+  }
   // tmp2 = T - T_old - q / c_m ; tmp1 = c_m M tmp2
   MPDG_TRY(launch_convert<double, double>(T, tmp2.get(), n, s));
   MPDG_TRY(launch_axpby(tmp2.get(), -1.0, T_old, 1.0, n, s));
@@ -894,10 +910,11 @@ struct mpdg_heat_operation {
   mpdg_heat_op *op = nullptr;
   mpdg_precond *pc = nullptr;
   mpdg_heat_operation_params prm{};
-  static constexpr int kMaxOlder = 3;  // solutions kept before T_{n-1} (extrapolation order <= 4)
+  static constexpr int kMaxOlder = 5;  // solutions kept before T_{n-1} (extrapolation order <= 6,
+                                       // least-squares projection over <= 7 solutions)
   DeviceArray<double> buf[3 + kMaxOlder], rhs, du;
   int cur = 0, old = 1, spare = 2;
-  int older[kMaxOlder] = {3, 4, 5};  // buffers of T_{n-2}, T_{n-3}, ...
+  int older[kMaxOlder] = {3, 4, 5, 6, 7};  // buffers of T_{n-2}, T_{n-3}, ...
   int depth = 0;                     // older buffers in use (extrapolation order - 1)
   int n_older = 0;                   // of those, how many hold past solutions
   double dt_old = 0.0;
@@ -912,6 +929,7 @@ struct mpdg_heat_operation {
   DeviceArray<double> zbuf, invd;    // residual input and inverse diagonal (learned predictor)
   double invd_dt = 0.0;              // time step the cached inverse diagonal was computed for
   DeviceArray<double> kq_step;       // k(T_n) per quadrature point, frozen for a linear step
+  DeviceArray<double> lsq_x, lsq_q, lsq_part, lsq_h;  // least-squares projection: bases and scalars
   bool law_frozen = false;
   int fell_back = 0;
   double rel_res0 = 0.0, rel_res = 0.0;
@@ -1187,7 +1205,10 @@ int mpdg_linear_solve(mpdg_heat_op *op, mpdg_precond *p, const mpdg_solver_param
 int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p, const mpdg_heat_operation_params *prm,
                                mpdg_heat_operation **out) {
   if (!op || !prm || !out || op->precision != MPDG_FP64) return MPDG_ERR_INVALID;
-  if (prm->predictor < MPDG_PREDICTOR_NONE || prm->predictor > MPDG_PREDICTOR_EXTRAPOLATION) return MPDG_ERR_INVALID;
+  if (prm->predictor < MPDG_PREDICTOR_NONE || prm->predictor > MPDG_PREDICTOR_LSQ) return MPDG_ERR_INVALID;
+  if (prm->predictor == MPDG_PREDICTOR_LSQ &&
+      (prm->n_old_solution_vectors < 1 || prm->n_old_solution_vectors > mpdg_heat_operation::kMaxOlder + 2))
+    return MPDG_ERR_INVALID;
   if (prm->step_mode != MPDG_STEP_NEWTON && prm->step_mode != MPDG_STEP_LINEAR) return MPDG_ERR_INVALID;
   if (!valid_solver(prm->linear)) return MPDG_ERR_INVALID;
   if (prm->predictor == MPDG_PREDICTOR_EXTRAPOLATION &&
@@ -1200,7 +1221,9 @@ int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p, const mpdg_hea
   h->prm = *prm;
   const long n = op->n_dofs();
   int rc = MPDG_OK;
-  h->depth = prm->predictor == MPDG_PREDICTOR_EXTRAPOLATION ? prm->extrapolation_order - 1 : 0;
+  h->depth = prm->predictor == MPDG_PREDICTOR_EXTRAPOLATION ? prm->extrapolation_order - 1
+             : prm->predictor == MPDG_PREDICTOR_LSQ          ? std::max(0, prm->n_old_solution_vectors - 2)
+                                                             : 0;
   for (int b = 0; b < 3 + h->depth; ++b) rc = rc ? rc : h->buf[b].resize(n);
   rc = rc ? rc : h->rhs.resize(n);
   rc = rc ? rc : h->du.resize(n);
@@ -1327,7 +1350,7 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
     h->dt_precond = dt;
   }
   MPDG_TRY(launch_fill<double>(h->bvec.get(), 0.0, n, s));
-  MPDG_TRY(op->create_rhs(h->bvec.get(), h->zero.get(), T_old, nullptr, s));
+  MPDG_TRY(op->create_rhs(h->bvec.get(), h->zero.get(), T_old, nullptr, s, true));
   MPDG_TRY(learned_guess(h, dt, h->bvec.get(), T, T_old, s));
   if (h->prm.predictor == MPDG_PREDICTOR_LEARNED && h->prm.guess_check) {
     // residuals of the previous solution (T_old) and of the learned guess (T, kept in alt)
@@ -1388,14 +1411,79 @@ static int extrapolate(mpdg_heat_operation *h, double dt, double *pred, hipStrea
     }
   }
   m = std::min(m, h->prm.extrapolation_order + 1);
+  double w[mpdg_heat_operation::kMaxOlder + 2];
   for (int j = 0; j < m; ++j) {
-    double w = 1.0;
+    w[j] = 1.0;
     for (int i = 0; i < m; ++i)
-      if (i != j) w *= (dt - tau[i]) / (tau[j] - tau[i]);
-    if (j == 0) MPDG_TRY(launch_scale(pred, pts[0], w, n, s));
-    else MPDG_TRY(launch_axpby(pred, w, pts[j], 1.0, n, s));
+      if (i != j) w[j] *= (dt - tau[i]) / (tau[j] - tau[i]);
   }
-  return MPDG_OK;
+  return launch_combine(pred, pts, w, m, n, s);
+}
+
+// LeastSquaresProjection::vmult (predictor.hpp:36-80): x = sum_s c_s T_{n-s} (s < L) minimising
+// ||b - J x||. The reference orthogonalises the images J T_{n-s} by modified Gram-Schmidt; here
+// the basis is first changed to backward differences (T_n, T_n - T_{n-1}, second difference,
+// ...), whose images differ by orders of magnitude instead of being nearly parallel, and the
+// images are orthonormalised by classical Gram-Schmidt applied twice (fused multi-dot /
+// multi-axpy) with the same operations on the solution basis; a direction whose image
+// vanishes against the previous ones is dropped (the reference stops there, :58-59).
+static int lsq_project(mpdg_heat_operation *h, const double *b, double *pred, hipStream_t s) {
+  mpdg_heat_op *op = h->op;
+  const long n = op->n_dofs();
+  const double *pts[mpdg_heat_operation::kMaxOlder + 2];
+  int L = 0;
+  pts[L++] = h->buf[h->cur].get();
+  if (h->dt_old > 0.0) {
+    pts[L++] = h->buf[h->old].get();
+    for (int j = 0; j < h->n_older; ++j) pts[L++] = h->buf[h->older[j]].get();
+  }
+  if (L < h->prm.n_old_solution_vectors) return launch_convert<double, double>(pts[0], pred, n, s);
+  L = h->prm.n_old_solution_vectors;
+  MPDG_TRY(h->lsq_x.resize((size_t)L * n));
+  MPDG_TRY(h->lsq_q.resize((size_t)L * n));
+  MPDG_TRY(h->lsq_part.resize(mdot_scratch()));
+  MPDG_TRY(h->lsq_h.resize(3 * kMdotMax));
+  double *X = h->lsq_x.get(), *Q = h->lsq_q.get(), *hd = h->lsq_h.get();
+  // backward differences: D_j = sum_i (-1)^i C(j, i) T_{n-i}
+  for (int j = 0; j < L; ++j) {
+    double *D = X + (size_t)j * n;
+    double c = 1.0;
+    for (int i = 0; i <= j; ++i) {
+      if (i == 0) MPDG_TRY(launch_scale(D, pts[0], 1.0, n, s));
+      else MPDG_TRY(launch_axpby(D, c, pts[i], 1.0, n, s));
+      c = -c * (double)(j - i) / (double)(i + 1);
+    }
+  }
+  int m = 0;  // accepted directions
+  for (int j = 0; j < L; ++j) {
+    double *xj = X + (size_t)m * n, *qj = Q + (size_t)m * n;
+    if (m != j) MPDG_TRY(launch_scale(xj, X + (size_t)j * n, 1.0, n, s));
+    MPDG_TRY(op->vmult(qj, xj, s));
+    double nrm0 = 0.0, hh[2 * kMdotMax + 1];
+    MPDG_TRY(launch_dot<double, double>(qj, qj, n, h->red.red(), hd + 2 * kMdotMax, s));
+    for (int pass = 0; pass < 2 && m > 0; ++pass) {
+      MPDG_TRY(launch_mdot(Q, n, m, qj, h->lsq_part.get(), hd + pass * kMdotMax, false, s));
+      MPDG_TRY(launch_maxpy(qj, Q, n, m, hd + pass * kMdotMax, -1.0, s));
+      MPDG_TRY(launch_maxpy(xj, X, n, m, hd + pass * kMdotMax, -1.0, s));
+    }
+    if (hipMemcpyAsync(&nrm0, hd + 2 * kMdotMax, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
+      return MPDG_ERR_DEVICE;
+    MPDG_TRY(launch_dot<double, double>(qj, qj, n, h->red.red(), hd + 2 * kMdotMax, s));
+    if (hipMemcpyAsync(hh, hd + 2 * kMdotMax, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
+      return MPDG_ERR_DEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+    const double nrm = std::sqrt(std::max(hh[0], 0.0));
+    if (!(nrm == nrm)) return MPDG_ERR_NONFINITE;
+    if (!(nrm > 1e-12 * std::sqrt(std::max(nrm0, 0.0)))) continue;  // dependent on the earlier ones
+    MPDG_TRY(launch_axpby(qj, 0.0, qj, 1.0 / nrm, n, s));
+    MPDG_TRY(launch_axpby(xj, 0.0, xj, 1.0 / nrm, n, s));
+    ++m;
+  }
+  if (m == 0) return launch_convert<double, double>(pts[0], pred, n, s);
+  // x = sum_j (q_j . b) x_j
+  MPDG_TRY(launch_mdot(Q, n, m, b, h->lsq_part.get(), hd, false, s));
+  MPDG_TRY(launch_fill<double>(pred, 0.0, n, s));
+  return launch_maxpy(pred, X, n, m, hd, 1.0, s);
 }
 
 // HeatDiffuseOperation::init_time_advance + solve (heat_diffuse_operation.cpp:215-289)
@@ -1425,6 +1513,13 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
       break;
     }
     case MPDG_PREDICTOR_EXTRAPOLATION: MPDG_TRY(extrapolate(h, dt, pred, s)); break;
+    case MPDG_PREDICTOR_LSQ: {
+      // predictor rhs (heat_diffuse_operation.cpp:233-244): here the step's b with T_old = T_n
+      MPDG_TRY(launch_fill<double>(h->bvec.get(), 0.0, n, s));
+      MPDG_TRY(op->create_rhs(h->bvec.get(), h->zero.get(), Tn, nullptr, s, true));
+      MPDG_TRY(lsq_project(h, h->bvec.get(), pred, s));
+      break;
+    }
     default: return MPDG_ERR_INVALID;
   }
   // commit_old_solutions + swap: old <- T_n, cur <- prediction (older <- T_{n-1}, ...)

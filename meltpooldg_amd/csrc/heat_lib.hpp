@@ -163,7 +163,9 @@ struct mpdg_heat_op {
   int vmult_f32(float *dst, const float *src, hipStream_t s) const;
   int diagonal64(double *diag, hipStream_t s) const;
   int inverse_diagonal(double *inv, hipStream_t s) const;
-  int create_rhs(double *dst, const double *T, const double *T_old, const double *q, hipStream_t s) const;
+  // T_zero: T is the zero vector (the right-hand side of a linear step), so K T is not applied
+  int create_rhs(double *dst, const double *T, const double *T_old, const double *q, hipStream_t s,
+                 bool T_zero = false) const;
   int l2_norm(const double *u, double *norm, hipStream_t s) const;
 };
 

@@ -195,3 +195,34 @@ def test_extrapolation_predictor(cuda, order):
     case.op.vmult(Jg, guess)
     expect = float(torch.linalg.norm(b - Jg) / torch.linalg.norm(b))
     assert abs(r0 - expect) <= 1e-6 * expect
+
+
+def test_least_squares_projection_predictor(cuda):
+    """least_squares_projection over the last 4 solutions: the step starts from the combination with the
+    smallest residual ||b - J x|| (numpy's dense least squares on the same vectors), which is no worse than
+    cubic extrapolation through the same solutions."""
+    torch = cuda
+    import meltpooldg_amd as M
+    from meltpooldg_amd.application import LaserScan
+    case = LaserScan(2, 8, dt=1e-3, precond="gmg64", predictor=M.PREDICTOR_LSQ, n_old_solution_vectors=4,
+                     n_cells_coarse=2)
+    Ts = [case.solution()]
+    for _ in range(6):
+        case.step()
+        Ts.append(case.solution())
+    _, _, (_, r0, _) = case.step()
+    b = torch.zeros_like(Ts[-1])
+    case.op.create_rhs(b, torch.zeros_like(b), Ts[-1])
+    hist = Ts[-4:][::-1]
+    A = []
+    for T in hist:
+        a = torch.empty_like(T)
+        case.op.vmult(a, T)
+        A.append(a.cpu().numpy())
+    A, bn = np.stack(A, 1), b.cpu().numpy()
+    c, *_ = np.linalg.lstsq(A, bn, rcond=None)
+    r_min = np.linalg.norm(bn - A @ c) / np.linalg.norm(bn)
+    cubic = np.array([4.0, -6.0, 4.0, -1.0])
+    r_cubic = np.linalg.norm(bn - A @ cubic) / np.linalg.norm(bn)
+    assert r0 <= r_cubic * (1 + 1e-6)
+    assert abs(r0 - r_min) <= 1e-3 * r_min
