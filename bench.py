@@ -51,7 +51,9 @@ def parse():
     ap.add_argument("--scan-eval", type=int, default=10)
     ap.add_argument("--scan-epochs", type=int, default=4)
     ap.add_argument("--scan-coarsening", type=int, default=4)
-    ap.add_argument("--copy", type=int, default=0,
+    ap.add_argument("--scan-variants", default="all",
+                    help="comma-separated solver/predictor rows of time_to_solve (default: all)")
+    ap.add_argument("--copy", type=int, default=0,  # kept for old command lines; the copy rate is always measured
                     help="also run 3 device copies of the src vector (traffic calibration for rocprofv3 --pmc)")
     return ap.parse_args()
 
@@ -121,7 +123,7 @@ def solve_timings(torch, M, op, reps=2):
     return out
 
 
-def scan_timings(torch, M, k, N, n_train, n_eval, epochs, rho):
+def scan_timings(torch, M, k, N, n_train, n_eval, epochs, rho, only="all"):
     """Time-to-solve per step of a laser scan (see the module docstring)."""
     import numpy as np
     from meltpooldg_amd.application import LaserScan, generate_trajectory, train
@@ -130,11 +132,14 @@ def scan_timings(torch, M, k, N, n_train, n_eval, epochs, rho):
     _, Ts, qs, zs, _ = generate_trajectory(k, N, n_train, dt=dt, precond="gmg32", predictor=M.PREDICTOR_LINEAR)
     torch.cuda.synchronize()
     t_traj = time.perf_counter() - t0
-    net = M.LearnedPredictor((N, N, N), degree=k, coarsening=rho, qscale=dt * N, seed=11)
-    t0 = time.perf_counter()
-    losses = train(net, Ts, qs, zs, epochs=epochs, lr=2e-3)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
+    wanted = None if only == "all" else set(only.split(","))
+    net, losses, t_train = None, [None], 0.0
+    if wanted is None or "fcg_gmg32_learned" in wanted:
+        net = M.LearnedPredictor((N, N, N), degree=k, coarsening=rho, qscale=dt * N, seed=11)
+        t0 = time.perf_counter()
+        losses = train(net, Ts, qs, zs, epochs=epochs, lr=2e-3)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
     del qs, zs
     out = {"workload": "laser scan, Q%d %d^3 cells, k=1, dt=1e-3, steps %d-%d, ||b-JT|| <= 1e-10 ||b||, coarse Q2 "
                        "4^3; FP64 GMG: Chebyshev 3 pre / 3 post (CG), FP32 GMG: Chebyshev 1 pre / 3 post "
@@ -149,6 +154,7 @@ def scan_timings(torch, M, k, N, n_train, n_eval, epochs, rho):
                 ("fcg_gmg32_extrapolation3", "gmg32", M.PREDICTOR_EXTRAPOLATION, 3),
                 ("fcg_gmg32_extrapolation6", "gmg32", M.PREDICTOR_EXTRAPOLATION, 6),
                 ("fcg_gmg32_learned", "gmg32", M.PREDICTOR_LEARNED, 1))
+    variants = tuple(v for v in variants if wanted is None or v[0] in wanted or v[0] == "fp64_cg_gmg64_prev_step")
     for name, pc, pred, order in variants:
         case = LaserScan(k, N, dt=dt, precond=pc, predictor=pred,
                          learned=net if pred == M.PREDICTOR_LEARNED else None,
@@ -228,10 +234,19 @@ def main():
                      "kernel": "heat_apply_kernel<4,double,KAPPA_QP>", "kernel_avg_ms": round(kern_avg, 5),
                      "algorithmic_bytes": bytes_alg},
     }
-    if a.copy:
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
+    # measured copy bandwidth on the same vectors (device-to-device copy, read + write), the
+    # attainable HBM rate next to the 8 TB/s spec
+    for _ in range(3):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_ms = e0.elapsed_time(e1) / 20
+    line["hbm_copy"] = {"GBps": round(2 * 8 * op.n_dofs / (copy_ms * 1e-3) / 1e9, 1), "bytes_per_copy": 16 * op.n_dofs,
+                        "note": "torch copy of one FP64 DoF vector, read + write"}
     del kq
     if a.solve and k == 3:
         try:
@@ -241,7 +256,7 @@ def main():
     if a.scan and k == 3:
         try:
             line["time_to_solve"] = scan_timings(torch, M, k, N, a.scan_train, a.scan_eval, a.scan_epochs,
-                                                 a.scan_coarsening)
+                                                 a.scan_coarsening, a.scan_variants)
         except Exception as e:  # report, do not hide the apply line
             line["time_to_solve"] = {"error": str(e)}
     if a.cpu:

@@ -42,11 +42,11 @@ template <int N1> struct ApplyGeom {
   static constexpr int ACC = AX * AY * AZ;            // DoF brick of a chunk
   static constexpr int TBUF = 4 * CPW * CS;           // transpose buffers of the 4 wavefronts
   // the DoF brick is summed in FP64 at every precision (LDS FP64 adds are fast on CDNA4,
-  // FP32 LDS atomics are not); LDS in doubles: max(transpose buffers, brick)
+  // FP32 LDS atomics are not) and has its own LDS, apart from the transpose buffers, so that
+  // only the scatter needs workgroup barriers
   template <typename Number>
-  static constexpr int lds_doubles() {
-    constexpr int tb = (TBUF * (int)sizeof(Number) + 7) / 8;
-    return tb > ACC ? tb : ACC;
+  static constexpr int tbuf_doubles() {
+    return (TBUF * (int)sizeof(Number) + 7) / 8;
   }
   static constexpr int CARRY = AY * AZ;
   // x rows of the DoF brick written per pass of the write-out loop (lane = x in the row)
@@ -249,10 +249,10 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
                                                          Number *__restrict__ dst) {
   using G = ApplyGeom<N1>;
   constexpr int K = G::K;
-  __shared__ double lds_raw[G::template lds_doubles<Number>()];
-  __shared__ double carry[G::CARRY];
+  __shared__ double lds_raw[G::template tbuf_doubles<Number>()];
+  __shared__ double acc[G::ACC];     // DoF brick of the chunk
+  __shared__ double carry[G::CARRY]; // its last x plane, carried to the next chunk
   Number *const lds = reinterpret_cast<Number *>(lds_raw);  // transpose slots
-  double *const acc = lds_raw;                               // DoF brick of the chunk
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -272,6 +272,9 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
   const int DZ = K * min(G::BZ, P.nc[2] - col_z * G::BZ) + 1;
   const long sy = P.nd[0], sz = (long)P.nd[0] * P.nd[1];
   const int nchunks = (P.nc[0] + G::BX - 1) / G::BX;
+  // the brick starts at zero; the write-out of every chunk zeroes what it reads
+  for (int idx = threadIdx.x; idx < G::ACC; idx += 256) acc[idx] = 0.0;
+  for (int idx = threadIdx.x; idx < G::CARRY; idx += 256) carry[idx] = 0.0;
 
   // the src plane of the next chunk is loaded while the current one is scattered and written
   Number nxt[N1][N1];
@@ -446,15 +449,13 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
       gather_plane<N1>(src, base + (long)K * G::BX, vn, P.dmask, P.nd, K * cxn, K * cy, K * cz + l, sy, nxt);
     }
 
-    // ---- scatter: sum the chunk's cells in the LDS DoF brick ----
+    // ---- scatter: sum the chunk's cells in the LDS DoF brick (zero, after the previous
+    //      chunk's write-out, which the barrier orders before these adds) ----
     __syncthreads();
     const int X0 = K * G::BX * chunk;
     const int DX = K * min(G::BX, P.nc[0] - chunk * G::BX) + 1;
-    for (int idx = tid; idx < G::ACC; idx += 256) {
-      const int x = idx % G::AX, yz = idx / G::AX;
-      acc[idx] = (x == 0 && chunk > 0) ? carry[yz] : 0.0;
-    }
-    __syncthreads();
+    if (chunk > 0)
+      for (int idx = tid; idx < G::CARRY; idx += 256) atomicAdd(&acc[G::AX * idx], carry[idx]);
     if (valid) {
 #pragma unroll
       for (int j = 0; j < N1; ++j)
@@ -472,6 +473,7 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
         const int y = r % G::AY, z = r / G::AY;
         if (y >= DY || z >= DZ) continue;
         const double sum = acc[wx + G::AX * r];
+        acc[wx + G::AX * r] = 0.0;
         if (wx == DX - 1 && !last) {
           carry[r] = sum;
           continue;
@@ -492,7 +494,6 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
         }
       }
     }
-    __syncthreads();
   }
 }
 
