@@ -6,7 +6,7 @@
 // :957-1086) run through deal.II's MatrixFree::loop + FEEvaluation.
 //
 // Work decomposition
-//   * A workgroup (256 threads, 4 wavefronts) owns a column of BY x BZ cells in
+//   * A workgroup (NW wavefronts, 4 by default) owns a column of BY x BZ cells in
 //     y-z and walks it along x in chunks of BX cells. Each cell is handled by
 //     N1 = k+1 consecutive lanes of one wavefront; lane l of a cell holds the
 //     z-plane l of the cell's DoFs (N1 x N1 values in registers).
@@ -24,23 +24,29 @@
 
 namespace mpdg {
 
+// cells of a chunk along x, y, z and wavefronts per workgroup (NW * (64 / N1) cells per chunk).
+// Q3: of the 64-cell shapes 4x4x4, 8x4x2, 16x2x2, 8x2x4, 16x4x1, 8x8x1 (and 32-cell ones with two
+// wavefronts), 8x4x2 is fastest at 128^3 (FP64 per-qp 0.645 vs 0.694 ms for 4x4x4): half as many
+// chunks, hence scatter barriers, per column, and longer coalesced x rows in the write-out.
 template <int N1> struct ChunkShape;
-template <> struct ChunkShape<2> { static constexpr int BX = 8, BY = 4, BZ = 4; };
-template <> struct ChunkShape<3> { static constexpr int BX = 7, BY = 4, BZ = 3; };
-template <> struct ChunkShape<4> { static constexpr int BX = 4, BY = 4, BZ = 4; };
-template <> struct ChunkShape<5> { static constexpr int BX = 4, BY = 4, BZ = 3; };
+template <> struct ChunkShape<2> { static constexpr int BX = 8, BY = 4, BZ = 4, NW = 4; };
+template <> struct ChunkShape<3> { static constexpr int BX = 7, BY = 4, BZ = 3, NW = 4; };
+template <> struct ChunkShape<4> { static constexpr int BX = 8, BY = 4, BZ = 2, NW = 4; };
+template <> struct ChunkShape<5> { static constexpr int BX = 4, BY = 4, BZ = 3, NW = 4; };
 
 template <int N1> struct ApplyGeom {
   static constexpr int K = N1 - 1;
   static constexpr int CPW = 64 / N1;                 // cells per wavefront
-  static constexpr int CPB = 4 * CPW;                 // cells per chunk
+  static constexpr int NW = ChunkShape<N1>::NW;       // wavefronts per workgroup
+  static constexpr int NT = 64 * NW;                  // threads per workgroup
+  static constexpr int CPB = NW * CPW;                // cells per chunk
   static constexpr int BX = ChunkShape<N1>::BX, BY = ChunkShape<N1>::BY, BZ = ChunkShape<N1>::BZ;
   static_assert(BX * BY * BZ == CPB, "chunk shape must match the cells per workgroup");
   static constexpr int SQY = N1 * N1 + 1;             // padded qy stride of a cell's transpose slot
   static constexpr int CS = N1 * SQY;                 // transpose slot per cell
   static constexpr int AX = K * BX + 1, AY = K * BY + 1, AZ = K * BZ + 1;
   static constexpr int ACC = AX * AY * AZ;            // DoF brick of a chunk
-  static constexpr int TBUF = 4 * CPW * CS;           // transpose buffers of the 4 wavefronts
+  static constexpr int TBUF = NW * CPW * CS;          // transpose buffers of the wavefronts
   // the DoF brick is summed in FP64 at every precision (LDS FP64 adds are fast on CDNA4,
   // FP32 LDS atomics are not) and has its own LDS, apart from the transpose buffers, so that
   // only the scatter needs workgroup barriers
@@ -50,7 +56,7 @@ template <int N1> struct ApplyGeom {
   }
   static constexpr int CARRY = AY * AZ;
   // x rows of the DoF brick written per pass of the write-out loop (lane = x in the row)
-  static constexpr int ROWS_PER_PASS = 256 / AX;
+  static constexpr int ROWS_PER_PASS = NT / AX;
 };
 
 // ---- register-level 1D contractions on an N1 x N1 array a[row][col] ----
@@ -238,12 +244,12 @@ __device__ __forceinline__ void load_kappa_qp(const Number *__restrict__ kappa, 
 // the full register file rather than spill.
 template <int N1, typename Number, bool MAPPED> struct Occupancy { static constexpr int W = 1; };
 template <> struct Occupancy<4, double, false> { static constexpr int W = 2; };
-template <> struct Occupancy<4, float, false> { static constexpr int W = 4; };
+template <> struct Occupancy<4, float, false> { static constexpr int W = 3; };
 template <> struct Occupancy<4, float, true> { static constexpr int W = 2; };
 template <> struct Occupancy<5, float, false> { static constexpr int W = 2; };
 
 template <int N1, typename Number, int KMODE, bool MAPPED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Occupancy<N1, Number, MAPPED>::W)))
+__global__ __launch_bounds__(ApplyGeom<N1>::NT) __attribute__((amdgpu_waves_per_eu(Occupancy<N1, Number, MAPPED>::W)))
 void heat_apply_kernel(const ApplyArgs<Number, N1> P,
                                                          const Number *__restrict__ src,
                                                          Number *__restrict__ dst) {
@@ -273,8 +279,8 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
   const long sy = P.nd[0], sz = (long)P.nd[0] * P.nd[1];
   const int nchunks = (P.nc[0] + G::BX - 1) / G::BX;
   // the brick starts at zero; the write-out of every chunk zeroes what it reads
-  for (int idx = threadIdx.x; idx < G::ACC; idx += 256) acc[idx] = 0.0;
-  for (int idx = threadIdx.x; idx < G::CARRY; idx += 256) carry[idx] = 0.0;
+  for (int idx = threadIdx.x; idx < G::ACC; idx += G::NT) acc[idx] = 0.0;
+  for (int idx = threadIdx.x; idx < G::CARRY; idx += G::NT) carry[idx] = 0.0;
 
   // the src plane of the next chunk is loaded while the current one is scattered and written
   Number nxt[N1][N1];
@@ -455,7 +461,7 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
     const int X0 = K * G::BX * chunk;
     const int DX = K * min(G::BX, P.nc[0] - chunk * G::BX) + 1;
     if (chunk > 0)
-      for (int idx = tid; idx < G::CARRY; idx += 256) atomicAdd(&acc[G::AX * idx], carry[idx]);
+      for (int idx = tid; idx < G::CARRY; idx += G::NT) atomicAdd(&acc[G::AX * idx], carry[idx]);
     if (valid) {
 #pragma unroll
       for (int j = 0; j < N1; ++j)
@@ -540,16 +546,16 @@ int launch_apply_n1(const OperatorData<Number> &op_in, const Number *src, Number
   if (kernel_start) hipEventRecord(kernel_start, stream);
   if (op.metric) {
     switch (op.kappa_mode) {
-      case KAPPA_CONST: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CONST, true>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
-      case KAPPA_CELL: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CELL, true>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
+      case KAPPA_CONST: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CONST, true>), dim3(ncols), dim3(G::NT), 0, stream, A, src, dst); break;
+      case KAPPA_CELL: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CELL, true>), dim3(ncols), dim3(G::NT), 0, stream, A, src, dst); break;
       default: return MPDG_ERR_INVALID;  // per-qp conductivity on mapped meshes runs through metric_k
     }
   } else {
     switch (op.kappa_mode) {
-      case KAPPA_CONST: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CONST, false>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
-      case KAPPA_QP: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_QP, false>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
-      case KAPPA_CELL: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CELL, false>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
-      case KAPPA_LAW: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_LAW, false>), dim3(ncols), dim3(256), 0, stream, A, src, dst); break;
+      case KAPPA_CONST: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CONST, false>), dim3(ncols), dim3(G::NT), 0, stream, A, src, dst); break;
+      case KAPPA_QP: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_QP, false>), dim3(ncols), dim3(G::NT), 0, stream, A, src, dst); break;
+      case KAPPA_CELL: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CELL, false>), dim3(ncols), dim3(G::NT), 0, stream, A, src, dst); break;
+      case KAPPA_LAW: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_LAW, false>), dim3(ncols), dim3(G::NT), 0, stream, A, src, dst); break;
       default: return MPDG_ERR_INVALID;
     }
   }
