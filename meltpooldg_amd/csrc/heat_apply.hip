@@ -24,23 +24,28 @@
 
 namespace mpdg {
 
-// cells of a chunk along x, y, z and wavefronts per workgroup (NW * (64 / N1) cells per chunk).
-// Q3: of the 64-cell shapes 4x4x4, 8x4x2, 16x2x2, 8x2x4, 16x4x1, 8x8x1 (and 32-cell ones with two
-// wavefronts), 8x4x2 is fastest at 128^3 (FP64 per-qp 0.645 vs 0.694 ms for 4x4x4): half as many
-// chunks, hence scatter barriers, per column, and longer coalesced x rows in the write-out.
-template <int N1> struct ChunkShape;
-template <> struct ChunkShape<2> { static constexpr int BX = 8, BY = 4, BZ = 4, NW = 4; };
-template <> struct ChunkShape<3> { static constexpr int BX = 7, BY = 4, BZ = 3, NW = 4; };
-template <> struct ChunkShape<4> { static constexpr int BX = 8, BY = 4, BZ = 2, NW = 4; };
-template <> struct ChunkShape<5> { static constexpr int BX = 4, BY = 4, BZ = 3, NW = 4; };
+// cells of a chunk along x, y, z and wavefronts per workgroup (NW * (64 / N1) cells per chunk),
+// chosen by measurement of the apply on large meshes (Q2 64^3, Q3 128^3, Q4 96^3). Longer chunks
+// halve the scatter barriers per column and lengthen the coalesced x rows of the write-out; flat
+// columns (BZ = 1) add face rows on every z plane; two- and eight-wavefront workgroups were
+// slower everywhere. Q3 (FP64 per-qp): 4x4x4 0.694 ms, 16x2x2 0.652, 8x4x2 0.645, 8x2x4 0.667,
+// 16x4x1 0.878, 8x8x1 0.945. Q2: 14x3x2 beats 7x4x3 by 5 % (FP64) and 25 % (FP32). Q4: FP64
+// keeps 4x4x3 (the longer shapes are 1-4 % slower), FP32 gains 11 % with 12x2x2.
+template <int N1, typename Number> struct ChunkShape;
+template <typename Number> struct ChunkShape<2, Number> { static constexpr int BX = 8, BY = 4, BZ = 4, NW = 4; };
+template <typename Number> struct ChunkShape<3, Number> { static constexpr int BX = 14, BY = 3, BZ = 2, NW = 4; };
+template <typename Number> struct ChunkShape<4, Number> { static constexpr int BX = 8, BY = 4, BZ = 2, NW = 4; };
+template <> struct ChunkShape<5, double> { static constexpr int BX = 4, BY = 4, BZ = 3, NW = 4; };
+template <> struct ChunkShape<5, float> { static constexpr int BX = 12, BY = 2, BZ = 2, NW = 4; };
 
-template <int N1> struct ApplyGeom {
+template <int N1, typename Number = double> struct ApplyGeom {
   static constexpr int K = N1 - 1;
   static constexpr int CPW = 64 / N1;                 // cells per wavefront
-  static constexpr int NW = ChunkShape<N1>::NW;       // wavefronts per workgroup
+  static constexpr int NW = ChunkShape<N1, Number>::NW;  // wavefronts per workgroup
   static constexpr int NT = 64 * NW;                  // threads per workgroup
   static constexpr int CPB = NW * CPW;                // cells per chunk
-  static constexpr int BX = ChunkShape<N1>::BX, BY = ChunkShape<N1>::BY, BZ = ChunkShape<N1>::BZ;
+  static constexpr int BX = ChunkShape<N1, Number>::BX, BY = ChunkShape<N1, Number>::BY,
+                       BZ = ChunkShape<N1, Number>::BZ;
   static_assert(BX * BY * BZ == CPB, "chunk shape must match the cells per workgroup");
   static constexpr int SQY = N1 * N1 + 1;             // padded qy stride of a cell's transpose slot
   static constexpr int CS = N1 * SQY;                 // transpose slot per cell
@@ -50,10 +55,7 @@ template <int N1> struct ApplyGeom {
   // the DoF brick is summed in FP64 at every precision (LDS FP64 adds are fast on CDNA4,
   // FP32 LDS atomics are not) and has its own LDS, apart from the transpose buffers, so that
   // only the scatter needs workgroup barriers
-  template <typename Number>
-  static constexpr int tbuf_doubles() {
-    return (TBUF * (int)sizeof(Number) + 7) / 8;
-  }
+  static constexpr int TBUF_DOUBLES = (TBUF * (int)sizeof(Number) + 7) / 8;
   static constexpr int CARRY = AY * AZ;
   // x rows of the DoF brick written per pass of the write-out loop (lane = x in the row)
   static constexpr int ROWS_PER_PASS = NT / AX;
@@ -249,13 +251,13 @@ template <> struct Occupancy<4, float, true> { static constexpr int W = 2; };
 template <> struct Occupancy<5, float, false> { static constexpr int W = 2; };
 
 template <int N1, typename Number, int KMODE, bool MAPPED>
-__global__ __launch_bounds__(ApplyGeom<N1>::NT) __attribute__((amdgpu_waves_per_eu(Occupancy<N1, Number, MAPPED>::W)))
+__global__ __launch_bounds__((ApplyGeom<N1, Number>::NT)) __attribute__((amdgpu_waves_per_eu(Occupancy<N1, Number, MAPPED>::W)))
 void heat_apply_kernel(const ApplyArgs<Number, N1> P,
                                                          const Number *__restrict__ src,
                                                          Number *__restrict__ dst) {
-  using G = ApplyGeom<N1>;
+  using G = ApplyGeom<N1, Number>;
   constexpr int K = G::K;
-  __shared__ double lds_raw[G::template tbuf_doubles<Number>()];
+  __shared__ double lds_raw[G::TBUF_DOUBLES];
   __shared__ double acc[G::ACC];     // DoF brick of the chunk
   __shared__ double carry[G::CARRY]; // its last x plane, carried to the next chunk
   Number *const lds = reinterpret_cast<Number *>(lds_raw);  // transpose slots
@@ -526,7 +528,7 @@ __global__ __launch_bounds__(128) void zero_face_rows_kernel(Number *__restrict_
 template <int N1, typename Number>
 int launch_apply_n1(const OperatorData<Number> &op_in, const Number *src, Number *dst, hipStream_t stream,
                     hipEvent_t kernel_start) {
-  using G = ApplyGeom<N1>;
+  using G = ApplyGeom<N1, Number>;
   const OperatorData<Number> op = kernel_view(op_in);
   if (op.metric == nullptr && op_in.metric != nullptr) return MPDG_ERR_INVALID;  // metric_k not computed
   ApplyArgs<Number, N1> A;
