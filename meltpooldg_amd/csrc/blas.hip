@@ -308,11 +308,12 @@ template <bool JACOBI>
 __global__ __launch_bounds__(256) void cg_init_kernel(const double *b, double *r, const double *q, double *z,
                                                       double *p, const double *inv, long n, double *partials,
                                                       unsigned *counter, CGState *st, double rel_tol,
-                                                      double abs_tol, int criterion) {
+                                                      double abs_tol, int criterion, float *r32) {
   double v[3] = {0.0, 0.0, 0.0};
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
     const double ri = b[i] - q[i];
     r[i] = ri;
+    if (r32) r32[i] = (float)ri;
     v[0] += ri * ri;
     v[2] += b[i] * b[i];
     if (JACOBI) {
@@ -351,13 +352,89 @@ __global__ __launch_bounds__(256) void cg_rz0_kernel(const double *r, const doub
 }
 
 int cg_init(const double *b, double *r, const double *q, double *z, double *p, const double *inv, bool jacobi,
-            long n, const Reduction &red, CGState *st, double rel_tol, double abs_tol, int criterion, hipStream_t s) {
+            long n, const Reduction &red, CGState *st, double rel_tol, double abs_tol, int criterion, hipStream_t s,
+            float *r32) {
   if (jacobi)
     hipLaunchKernelGGL(cg_init_kernel<true>, dim3(red_blocks(n)), dim3(256), 0, s, b, r, q, z, p, inv, n,
-                       red.partials, red.counter, st, rel_tol, abs_tol, criterion);
+                       red.partials, red.counter, st, rel_tol, abs_tol, criterion, (float *)nullptr);
   else
     hipLaunchKernelGGL(cg_init_kernel<false>, dim3(red_blocks(n)), dim3(256), 0, s, b, r, q, z, p, inv, n,
-                       red.partials, red.counter, st, rel_tol, abs_tol, criterion);
+                       red.partials, red.counter, st, rel_tol, abs_tol, criterion, r32);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+// ---- the same iteration with an FP32 preconditioner working in place: r is also written in
+//      FP32 (the preconditioner's input) and z is read in FP32 (its output), which spares the
+//      two conversion passes per iteration and reads half the bytes of z ----
+__global__ __launch_bounds__(256) void fcg32_start_kernel(const double *r, const float *z, double *p, long n,
+                                                          double *partials, unsigned *counter, CGState *st) {
+  if (st->done) return;
+  double v[1] = {0.0};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const double zi = z[i];
+    p[i] = zi;
+    v[0] += r[i] * zi;
+  }
+  if (reduce_to_last_block<1>(v, partials, counter) && threadIdx.x == 0) st->rz = v[0];
+}
+__global__ __launch_bounds__(256) void fcg32_update_kernel(double *x, double *r, const double *p, const double *q,
+                                                           const float *z, float *r32, long n, double *partials,
+                                                           unsigned *counter, CGState *st) {
+  if (st->done) return;
+  const double alpha = st->alpha;
+  double v[2] = {0.0, 0.0};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    r32[i] = (float)ri;
+    v[0] += ri * ri;
+    v[1] += ri * (double)z[i];  // r_new . z_old for the flexible beta
+  }
+  if (reduce_to_last_block<2>(v, partials, counter) && threadIdx.x == 0) {
+    st->rr = v[0];
+    st->iters += 1;
+    const bool conv = v[0] <= st->tol2;
+    const bool bad = !(v[0] == v[0]) || !(fabs(v[0]) <= 1.7e308);
+    st->rzold_new = v[1];
+    if (conv || bad || st->iters >= st->max_it) st->done = bad ? 2 : 1;
+  }
+}
+__global__ __launch_bounds__(256) void fcg32_beta_kernel(const double *r, const float *z, long n, double *partials,
+                                                         unsigned *counter, CGState *st) {
+  if (st->done) return;
+  double v[1] = {0.0};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) v[0] += r[i] * (double)z[i];
+  if (reduce_to_last_block<1>(v, partials, counter) && threadIdx.x == 0) {
+    const double num = st->flexible ? v[0] - st->rzold_new : v[0];
+    st->beta = num / st->rz;
+    st->rz = v[0];
+  }
+}
+__global__ void fcg32_direction_kernel(double *p, const float *z, long n, const CGState *st) {
+  if (st->done) return;
+  const double beta = st->beta;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = (double)z[i] + beta * p[i];
+}
+int fcg32_start(const double *r, const float *z, double *p, long n, const Reduction &red, CGState *st,
+                hipStream_t s) {
+  hipLaunchKernelGGL(fcg32_start_kernel, dim3(red_blocks(n)), dim3(256), 0, s, r, z, p, n, red.partials, red.counter,
+                     st);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+int fcg32_update(double *x, double *r, const double *p, const double *q, const float *z, float *r32, long n,
+                 const Reduction &red, CGState *st, hipStream_t s) {
+  hipLaunchKernelGGL(fcg32_update_kernel, dim3(red_blocks(n)), dim3(256), 0, s, x, r, p, q, z, r32, n, red.partials,
+                     red.counter, st);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+int fcg32_beta(const double *r, const float *z, long n, const Reduction &red, CGState *st, hipStream_t s) {
+  hipLaunchKernelGGL(fcg32_beta_kernel, dim3(red_blocks(n)), dim3(256), 0, s, r, z, n, red.partials, red.counter, st);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+int fcg32_direction(double *p, const float *z, long n, const CGState *st, hipStream_t s) {
+  hipLaunchKernelGGL(fcg32_direction_kernel, dim3(std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, s, p, z, n,
+                     st);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 int cg_after_external_precond_init(const double *r, const double *z, double *p, long n, const Reduction &red,

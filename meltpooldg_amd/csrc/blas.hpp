@@ -30,7 +30,15 @@ template <typename TA, typename TB>
 int launch_dot(const TA *a, const TB *b, long n, const Reduction &r, double *out, hipStream_t s);
 
 int cg_init(const double *b, double *r, const double *q, double *z, double *p, const double *inv, bool jacobi,
-            long n, const Reduction &red, CGState *st, double rel_tol, double abs_tol, int criterion, hipStream_t s);
+            long n, const Reduction &red, CGState *st, double rel_tol, double abs_tol, int criterion, hipStream_t s,
+            float *r32 = nullptr);
+// CG / flexible CG with an FP32 preconditioner run in place: r32 its input, z its output
+int fcg32_start(const double *r, const float *z, double *p, long n, const Reduction &red, CGState *st,
+                hipStream_t s);
+int fcg32_update(double *x, double *r, const double *p, const double *q, const float *z, float *r32, long n,
+                 const Reduction &red, CGState *st, hipStream_t s);
+int fcg32_beta(const double *r, const float *z, long n, const Reduction &red, CGState *st, hipStream_t s);
+int fcg32_direction(double *p, const float *z, long n, const CGState *st, hipStream_t s);
 int cg_after_external_precond_init(const double *r, const double *z, double *p, long n, const Reduction &red,
                                    CGState *st, hipStream_t s);
 int cg_alpha(const double *p, const double *q, long n, const Reduction &red, CGState *st, hipStream_t s);

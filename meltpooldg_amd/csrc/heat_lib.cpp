@@ -693,6 +693,18 @@ struct GMGPrecond : mpdg_precond {
     MPDG_TRY(vcycle((int)lev.size() - 1, s));
     return launch_convert<T, double>(top.x.get(), dst, n, s);
   }
+  float *f32_input() override {
+    if constexpr (std::is_same<T, float>::value) return lev.back()->b.get();
+    return nullptr;
+  }
+  const float *f32_output() const override {
+    if constexpr (std::is_same<T, float>::value) return lev.back()->x.get();
+    return nullptr;
+  }
+  int vmult_f32_in_place(hipStream_t s) override {
+    if constexpr (!std::is_same<T, float>::value) return MPDG_ERR_INVALID;
+    return vcycle((int)lev.size() - 1, s);
+  }
   int smooth(double *dst, const double *src, int degree, hipStream_t s) override {
     Level<T> &top = *lev.back();
     const long n = top.od.n_dofs;
@@ -849,6 +861,43 @@ int gmres_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &pr
   return res <= tol ? MPDG_OK : MPDG_ERR_NO_CONVERGENCE;
 }
 
+// (flexible) CG with an FP32 preconditioner run in place: the iteration of krylov_solve below
+// with r also written in FP32 (the preconditioner's input) and z read in FP32 (its output)
+int fcg32_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &prm, const double *b, double *x,
+                KrylovWork &w, int *iterations, double *res, float *r32, const float *z32, hipStream_t s) {
+  const long n = op->n_dofs();
+  CGState h{};
+  h.max_it = prm.max_iterations;
+  h.flexible = prm.solver_type == MPDG_SOLVER_FCG ? 1 : 0;
+  if (hipMemcpyAsync(w.st.get(), &h, sizeof(CGState), hipMemcpyHostToDevice, s) != hipSuccess) return MPDG_ERR_DEVICE;
+  MPDG_TRY(op->vmult(w.q.get(), x, s));
+  MPDG_TRY(cg_init(b, w.r.get(), w.q.get(), nullptr, nullptr, nullptr, false, n, w.red.red(), w.st.get(),
+                   prm.rel_tolerance, prm.abs_tolerance, prm.criterion, s, r32));
+  MPDG_TRY(pc->vmult_f32_in_place(s));
+  MPDG_TRY(fcg32_start(w.r.get(), z32, w.p.get(), n, w.red.red(), w.st.get(), s));
+  const int batch = n > 2000000 ? 1 : 8;
+  int it = 0;
+  while (true) {
+    for (int k = 0; k < batch; ++k) {
+      MPDG_TRY(op->vmult(w.q.get(), w.p.get(), s));
+      MPDG_TRY(cg_alpha(w.p.get(), w.q.get(), n, w.red.red(), w.st.get(), s));
+      MPDG_TRY(fcg32_update(x, w.r.get(), w.p.get(), w.q.get(), z32, r32, n, w.red.red(), w.st.get(), s));
+      MPDG_TRY(pc->vmult_f32_in_place(s));
+      MPDG_TRY(fcg32_beta(w.r.get(), z32, n, w.red.red(), w.st.get(), s));
+      MPDG_TRY(fcg32_direction(w.p.get(), z32, n, w.st.get(), s));
+    }
+    it += batch;
+    if (hipMemcpyAsync(&h, w.st.get(), sizeof(CGState), hipMemcpyDeviceToHost, s) != hipSuccess) return MPDG_ERR_DEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+    if (h.done || it > prm.max_iterations + batch) break;
+  }
+  *iterations = h.iters;
+  *res = std::sqrt(std::max(h.rr, 0.0));
+  if (h.done == 2) return MPDG_ERR_NONFINITE;
+  if (h.rr > h.tol2) return MPDG_ERR_NO_CONVERGENCE;
+  return MPDG_OK;
+}
+
 int krylov_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &prm, const double *b, double *x,
                  KrylovWork &w, int *iterations, double *res, hipStream_t s) {
   const long n = op->n_dofs();
@@ -864,6 +913,10 @@ int krylov_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &p
   h.max_it = prm.max_iterations;
   h.flexible = prm.solver_type == MPDG_SOLVER_FCG ? 1 : 0;
   if (hipMemcpyAsync(w.st.get(), &h, sizeof(CGState), hipMemcpyHostToDevice, s) != hipSuccess) return MPDG_ERR_DEVICE;
+  // FP32 multigrid: its input and output are used in place, in FP32
+  float *r32 = external ? pc->f32_input() : nullptr;
+  const float *z32 = external ? pc->f32_output() : nullptr;
+  if (r32 && z32) return fcg32_solve(op, pc, prm, b, x, w, iterations, res, r32, z32, s);
   // r = b - A x
   MPDG_TRY(op->vmult(w.q.get(), x, s));
   MPDG_TRY(cg_init(b, w.r.get(), w.q.get(), w.z.get(), w.p.get(), inv, jacobi, n, w.red.red(), w.st.get(),

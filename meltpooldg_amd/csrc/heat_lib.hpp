@@ -186,6 +186,11 @@ struct mpdg_precond {
     (void)degree;
     return vmult(dst, src, s);
   }
+  // an FP32 preconditioner that runs in place: the Krylov solver writes its input and reads its
+  // output in FP32 (nullptr: not available)
+  virtual float *f32_input() { return nullptr; }
+  virtual const float *f32_output() const { return nullptr; }
+  virtual int vmult_f32_in_place(hipStream_t) { return MPDG_ERR_INVALID; }
   virtual int n_levels() const { return 0; }
   virtual int level_info(int, int *, int *, double *) const { return MPDG_ERR_INVALID; }
 };
