@@ -211,6 +211,9 @@ typedef struct mpdg_heat_operation_params {
                              history holds: after set_initial the first steps use fewer points) */
   int n_old_solution_vectors; /* MPDG_PREDICTOR_LSQ: 1..7 solutions (predictor_data.hpp:27-28;
                                  until the history holds them, the previous solution is used) */
+  int preconditioner_update_interval; /* linear step mode with a conductivity law: the
+                                 preconditioner is rebuilt every this many steps (0 or 1: every
+                                 step, as the reference; a change of dt always rebuilds it) */
 } mpdg_heat_operation_params;
 int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p,
                                const mpdg_heat_operation_params *params,

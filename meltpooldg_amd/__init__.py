@@ -73,7 +73,8 @@ class OperationParams(C.Structure):
     _fields_ = [("predictor", C.c_int), ("linear", SolverParams), ("max_nonlinear_iterations", C.c_int),
                 ("residual_tolerance", C.c_double), ("field_correction_tolerance", C.c_double),
                 ("step_mode", C.c_int), ("guess_check", C.c_int), ("guess_smoothing", C.c_int),
-                ("extrapolation_order", C.c_int), ("n_old_solution_vectors", C.c_int)]
+                ("extrapolation_order", C.c_int), ("n_old_solution_vectors", C.c_int),
+                ("preconditioner_update_interval", C.c_int)]
 
 
 STEP_NEWTON, STEP_LINEAR = 0, 1
@@ -150,6 +151,10 @@ def load_library(path=LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise FileNotFoundError("libmpdg_heat.so not built: run __graft_entry__.build() (%s)" % path)
+    # torch first: its HIP runtime (soname libamdhip64.so.7) is then the one the library binds to.
+    # Loaded the other way round, the library pulls in the system runtime and torch later loads
+    # its own copy, and device work of the two does not mix (the first sync reports an error).
+    import torch  # noqa: F401
     L = C.CDLL(path)
     for name, (args, res) in _SIGNATURES.items():
         fn = getattr(L, name)
@@ -440,7 +445,7 @@ class HeatDiffuseOperation:
 
     def __init__(self, op, preconditioner=None, predictor=PREDICTOR_LINEAR, linear=None, nlsolve=None,
                  step_mode=STEP_NEWTON, learned=None, q_field=None, guess_check=True, guess_smoothing=0,
-                 extrapolation_order=2, n_old_solution_vectors=2):
+                 extrapolation_order=2, n_old_solution_vectors=2, preconditioner_update_interval=1):
         self.op = op
         self.lib = op.lib
         self.pc = preconditioner
@@ -458,6 +463,7 @@ class HeatDiffuseOperation:
         p.guess_smoothing = guess_smoothing
         p.extrapolation_order = extrapolation_order
         p.n_old_solution_vectors = n_old_solution_vectors
+        p.preconditioner_update_interval = preconditioner_update_interval
         h = C.c_void_p()
         _check(self.lib.mpdg_heat_operation_create(op.h, preconditioner.h if preconditioner else None,
                                                    C.byref(p), C.byref(h)), "heat_operation_create")

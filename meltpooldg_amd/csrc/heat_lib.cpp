@@ -985,6 +985,7 @@ struct mpdg_heat_operation {
   DeviceArray<double> lsq_x, lsq_q, lsq_part, lsq_h;  // least-squares projection: bases and scalars
   bool law_frozen = false;
   int fell_back = 0;
+  int steps_since_update = 0;  // linear steps since the preconditioner was rebuilt
   double rel_res0 = 0.0, rel_res = 0.0;
 };
 
@@ -1398,9 +1399,15 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
   const bool coeff_changes = op->d64.kappa_mode == KAPPA_LAW;
   FrozenLaw frozen(h);
   MPDG_TRY(frozen.freeze(s));
-  if (h->pc && (coeff_changes || dt != h->dt_precond)) {
+  // preconditioner.update() (heat_diffuse_operation.cpp:275); with a conductivity law it may be
+  // kept for a few steps (any fixed SPD preconditioner keeps CG exact; the FP64 test decides)
+  const int every = std::max(1, h->prm.preconditioner_update_interval);
+  if (h->pc && (dt != h->dt_precond || (coeff_changes && h->steps_since_update + 1 >= every))) {
     MPDG_TRY(h->pc->update(s));
     h->dt_precond = dt;
+    h->steps_since_update = 0;
+  } else {
+    ++h->steps_since_update;
   }
   MPDG_TRY(launch_fill<double>(h->bvec.get(), 0.0, n, s));
   MPDG_TRY(op->create_rhs(h->bvec.get(), h->zero.get(), T_old, nullptr, s, true));

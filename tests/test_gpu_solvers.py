@@ -226,3 +226,22 @@ def test_least_squares_projection_predictor(cuda):
     r_cubic = np.linalg.norm(bn - A @ cubic) / np.linalg.norm(bn)
     assert r0 <= r_cubic * (1 + 1e-6)
     assert abs(r0 - r_min) <= 1e-3 * r_min
+
+
+def test_preconditioner_update_interval(cuda):
+    """Linear steps with a conductivity law: rebuilding the multigrid only every third step still
+    solves every step to the FP64 tolerance, so the trajectory equals the every-step one."""
+    torch = cuda
+    import meltpooldg_amd as M
+    from meltpooldg_amd.application import LaserScan
+    from tests.helpers import tanh_law_params
+    out = []
+    for every in (1, 3):
+        case = LaserScan(2, 8, dt=2e-3, law=(M.LAW_TANH, tanh_law_params()), precond="gmg32",
+                         predictor=M.PREDICTOR_EXTRAPOLATION, extrapolation_order=3, rel_tol=1e-12, n_cells_coarse=2,
+                         preconditioner_update_interval=every)
+        case.heat.set_initial_condition(torch.full_like(case.T0, 0.95))
+        for _ in range(7):
+            case.step()
+        out.append(case.solution())
+    assert float(torch.linalg.norm(out[0] - out[1]) / torch.linalg.norm(out[0])) < 1e-9
