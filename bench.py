@@ -185,7 +185,8 @@ def scan_timings(torch, M, k, N, n_train, n_eval, epochs, rho, only="all"):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported (see DESIGN.md)" % a.gpus,
+        print("bench.py: measures one GPU; --gpus %d is not supported (the multi-GPU path is a design, not built)"
+              % a.gpus,
               file=sys.stderr)
         sys.exit(2)
     import torch
