@@ -23,7 +23,7 @@ inline int dev_ok() { return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ER
 int mpdg_heat_op::vmult_cells(double *dst, const double *src, hipStream_t s) const {
   if (precision != MPDG_FP64) return MPDG_ERR_INVALID;
   if (d64.metric && d64.kappa_mode == KAPPA_LAW) MPDG_TRY(refresh_metric_k(s));  // k(T) with the current T
-  if (tangent) return td_bound ? launch_tangent_apply(d64, td.get(), src, dst, s) : MPDG_ERR_INVALID;
+  if (tangent) return td_bound ? launch_tangent_apply(d64, td.get(), vel, solid, src, dst, s) : MPDG_ERR_INVALID;
   if (!timing_on) return launch_apply<double>(d64, src, dst, s);
   if (timing_used == timing_events.size()) {
     hipEvent_t a, b;
@@ -75,7 +75,7 @@ int mpdg_heat_op::vmult_f32(float *dst, const float *src, hipStream_t s) const {
 int mpdg_heat_op::diagonal64(double *diag, hipStream_t s) const {
   if (tangent) {
     if (!td_bound) return MPDG_ERR_INVALID;
-    MPDG_TRY(launch_tangent_diagonal(d64, td.get(), diag, s));
+    MPDG_TRY(launch_tangent_diagonal(d64, td.get(), vel, solid, diag, s));
     if (d64.robin_mask) MPDG_TRY(launch_boundary(d64, 1, nullptr, diag, nullptr, T_robin, s));
     return MPDG_OK;
   }
@@ -1107,7 +1107,7 @@ int mpdg_heat_op_set_tangent(mpdg_heat_op *op, const mpdg_heat_material *mat, in
   op->solid = do_solidification ? 1 : 0;
   for (int i = 0; i < 3; ++i) op->vel[i] = velocity3[i];
   const int n1 = op->d64.degree + 1;
-  MPDG_TRY(op->td.resize((size_t)8 * op->d64.n_cells * n1 * n1 * n1));
+  MPDG_TRY(op->td.resize((size_t)kTangentCoeffs * op->d64.n_cells * n1 * n1 * n1));
   op->tangent = true;
   op->td_bound = false;
   return MPDG_OK;

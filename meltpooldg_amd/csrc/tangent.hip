@@ -5,15 +5,19 @@
 //
 // Per quadrature point the tangent is val = a u + c . grad u, flux = k grad u + b u. The
 // coefficients depend on T and T_old only, so they are evaluated once per Newton iteration
-// (tangent_data_kernel) into 8 values per point, with the quadrature weight, the cell volume
-// and the 1/h factors folded in:
+// (tangent_data_kernel) into kTangentCoeffs values per point, with the quadrature weight, the
+// cell volume and the 1/h factors folded in:
 //   t0 = JxW (rho_cp/dt [+ d rho_cp/dT ((T - T_old)/dt + grad T . v)])
-//   t1..t3 = JxW rho_cp v_d / h_d                       (value slot, times d u / d xi_d)
-//   t4 = JxW k                                           (times 1/h_d^2 d u / d xi_d)
-//   t5..t7 = JxW dk/dT (grad T)_d / h_d  [solidification] (gradient slot, times u)
-// The apply, residual and diagonal kernels are cell-local sum factorization with one thread
-// per line of the cell's N1^3 tensor (N1^2 threads per cell) and LDS transposes between the
-// directions; contributions are summed into the global vector with FP64 atomics.
+//   t1 = JxW k                                           (times 1/h_d^2 d u / d xi_d)
+//   t2 = JxW rho_cp           [velocity]                 (times sum_d v_d/h_d d u / d xi_d)
+//   t3..t5 = JxW dk/dT (grad T)_d / h_d  [solidification] (gradient slot, times u)
+// Coefficients a problem does not have (no velocity, no solidification) are neither written
+// nor read. The apply, residual and diagonal kernels are cell-local sum factorization with one
+// thread per line of the cell's N1^3 tensor (N1^2 threads per cell) and LDS transposes between
+// the directions. When a block's cells lie in one x-row of cells (the usual case), the block
+// loads and stores its DoFs as N1^2 lines along x with consecutive lanes on consecutive
+// addresses, sums the shared x-faces in LDS, stores the DoFs only it touches and adds the rest
+// with FP64 atomics; otherwise every cell adds its rows with atomics.
 #include "blas.hpp"
 #include "tangent.hpp"
 
@@ -46,6 +50,8 @@ struct Geometry {
   int nd[3];
   int dmask;
   double h[3], ih2[3], vol, dt;
+  double cv[3];     // v_d / h_d
+  int conv, solid;  // coefficient t2 / t3..t5 present
 };
 
 __device__ __forceinline__ bool is_constrained(int m, const int *nd, int ix, int iy, int iz) {
@@ -53,32 +59,52 @@ __device__ __forceinline__ bool is_constrained(int m, const int *nd, int ix, int
          ((m & 8) && iy == nd[1] - 1) || ((m & 16) && iz == 0) || ((m & 32) && iz == nd[2] - 1);
 }
 
-// The thread's place: cell `cell` (valid when `active`), line (a, b) of the cell's tensor.
+// Blocks walk the x-rows of cells: block = (row of cells, segment of kCells cells along x; the
+// last segment of a row may be shorter). The block's DoFs form N1^2 lines (a, b) along x of
+// n_cells_in_block K + 1 DoFs each.
+template <int N1>
+constexpr int kLine = kCells<N1> * (N1 - 1) + 1;
+
+inline long tangent_blocks(int n1, const int *nc) {
+  const int cpb = 256 / (n1 * n1);
+  return (long)nc[1] * nc[2] * ((nc[0] + cpb - 1) / cpb);
+}
+
+// The thread's place: cell `cell` (valid when `active`), line (a, b) of the cell's tensor;
+// block-uniform: ncb cells in this block, first DoF (ix0, iy0, iz0) of its lines.
 template <int N1>
 struct Place {
-  int lc, r, a, b, cx, cy, cz;
+  int lc, r, a, b, ncb, ix0, iy0, iz0;
   long cell;
   bool in_block, active;
   __device__ Place(const Geometry &g) {
-    constexpr int L2 = N1 * N1;
+    constexpr int L2 = N1 * N1, K = N1 - 1, CPB = kCells<N1>;
     lc = threadIdx.x / L2;
     r = threadIdx.x - lc * L2;
     a = r / N1;
     b = r - a * N1;
-    cell = (long)blockIdx.x * kCells<N1> + lc;
-    in_block = lc < kCells<N1>;
-    active = in_block && cell < g.ncells;
-    cx = cy = cz = 0;
-    if (active) {
-      cx = (int)(cell % g.nc0);
-      cy = (int)((cell / g.nc0) % g.nc1);
-      cz = (int)(cell / ((long)g.nc0 * g.nc1));
-    }
+    const int nseg = (g.nc0 + CPB - 1) / CPB;
+    const long row = blockIdx.x / nseg;
+    const int x0 = (int)(blockIdx.x - row * nseg) * CPB;
+    ncb = min(CPB, g.nc0 - x0);
+    ix0 = K * x0;
+    iy0 = K * (int)(row % g.nc1);
+    iz0 = K * (int)(row / g.nc1);
+    in_block = lc < CPB;
+    active = lc < ncb;
+    cell = row * g.nc0 + x0 + lc;
   }
-  // first DoF of this thread's row (l = a, j = b) of the cell
-  __device__ long row(const Geometry &g) const {
-    constexpr int K = N1 - 1;
-    return (long)K * cx + (long)g.nd[0] * ((K * cy + b) + (long)g.nd[1] * (K * cz + a));
+  // DoF indices of entry e = r' kLine + x of the block's lines (r' = a' N1 + b': z-line a',
+  // y-line b')
+  __device__ void at(const Geometry &g, int e, int &la, int &lb, int &x, int &ix, int &iy, int &iz, long &gi) const {
+    const int rr = e / kLine<N1>;
+    x = e - rr * kLine<N1>;
+    la = rr / N1;
+    lb = rr - la * N1;
+    ix = ix0 + x;
+    iy = iy0 + lb;
+    iz = iz0 + la;
+    gi = ix + (long)g.nd[0] * (iy + (long)g.nd[1] * iz);
   }
 };
 
@@ -228,33 +254,58 @@ __device__ void integrate(const Tables1D &t, const Place<N1> &p, const double *v
   }
 }
 
-// the thread's DoF row of a global vector (constrained entries read as 0 when `zero_con`)
+// The thread's DoF row (l = a, j = b) of a global vector in u (constrained entries read as 0
+// when `zero_con`; 0 for threads without a cell): the block loads its lines into `line`
+// (kCells N1^3 LDS values; any array: starts with a barrier) with consecutive lanes on
+// consecutive addresses, then each thread takes its row.
 template <int N1>
-__device__ void gather_row(const Geometry &g, const Place<N1> &p, const double *src, bool zero_con, double *u) {
-  constexpr int K = N1 - 1;
-  if (!p.active) {
-#pragma unroll
-    for (int i = 0; i < N1; ++i) u[i] = 0.0;
-    return;
+__device__ void gather(const Geometry &g, const Place<N1> &p, const double *src, bool zero_con, double *line,
+                       double *u) {
+  constexpr int K = N1 - 1, NE = N1 * N1 * kLine<N1>;
+  static_assert(NE <= kCells<N1> * N1 * N1 * N1, "block lines exceed the LDS array");
+  const int len = p.ncb * K + 1;
+  __syncthreads();
+  for (int e = threadIdx.x; e < NE; e += blockDim.x) {
+    int la, lb, x, ix, iy, iz;
+    long gi;
+    p.at(g, e, la, lb, x, ix, iy, iz, gi);
+    if (x < len) line[e] = (zero_con && g.dmask && is_constrained(g.dmask, g.nd, ix, iy, iz)) ? 0.0 : src[gi];
   }
-  const long row = p.row(g);
-  const int iy = K * p.cy + p.b, iz = K * p.cz + p.a;
+  __syncthreads();
 #pragma unroll
-  for (int i = 0; i < N1; ++i) {
-    const bool con = zero_con && g.dmask && is_constrained(g.dmask, g.nd, K * p.cx + i, iy, iz);
-    u[i] = con ? 0.0 : src[row + i];
-  }
+  for (int i = 0; i < N1; ++i) u[i] = p.active ? line[p.r * kLine<N1> + p.lc * K + i] : 0.0;
 }
 
-template <int N1>
-__device__ void scatter_row(const Geometry &g, const Place<N1> &p, double *dst, const double *out) {
-  constexpr int K = N1 - 1;
-  if (!p.active) return;
-  const long row = p.row(g);
-  const int iy = K * p.cy + p.b, iz = K * p.cz + p.a;
+// dst += the threads' DoF rows (l = a, j = b) in out: the rows are summed into the block's
+// lines in LDS (shared x-faces of neighbouring cells added there), then written with
+// consecutive lanes on consecutive addresses. With ZEROED (dst cleared before the launch) the
+// DoFs that only this block touches -- y, z interior to the cells, x interior to the block --
+// are stored, the others added with atomics. Constrained rows are left alone.
+template <int N1, bool ZEROED>
+__device__ void scatter(const Geometry &g, const Place<N1> &p, double *dst, const double *out, double *line) {
+  constexpr int K = N1 - 1, NE = N1 * N1 * kLine<N1>;
+  const int len = p.ncb * K + 1;
+  double *my = line + (p.active ? p.r * kLine<N1> + p.lc * K : 0);
+  __syncthreads();
+  if (p.active) {
 #pragma unroll
-  for (int i = 0; i < N1; ++i)
-    if (!(g.dmask && is_constrained(g.dmask, g.nd, K * p.cx + i, iy, iz))) atomicAdd(dst + row + i, out[i]);
+    for (int i = 0; i < K; ++i) my[i] = out[i];
+    if (p.lc == p.ncb - 1) my[K] = out[K];
+  }
+  __syncthreads();
+  if (p.active && p.lc < p.ncb - 1) my[K] += out[K];  // first DoF of the next cell's row
+  __syncthreads();
+  for (int e = threadIdx.x; e < NE; e += blockDim.x) {
+    int la, lb, x, ix, iy, iz;
+    long gi;
+    p.at(g, e, la, lb, x, ix, iy, iz, gi);
+    if (x >= len || (g.dmask && is_constrained(g.dmask, g.nd, ix, iy, iz))) continue;
+    const bool own = ZEROED && la > 0 && la < K && lb > 0 && lb < K && x > 0 && x < len - 1;
+    if (own)
+      dst[gi] = line[e];
+    else
+      atomicAdd(dst + gi, line[e]);
+  }
 }
 
 // material of the tangent (the oracle's `material`, material.templates.hpp:296-380, :546-598, :765-778)
@@ -293,10 +344,10 @@ __global__ __launch_bounds__(256) void tangent_data_kernel(Tables1D t, TangentAr
   const Place<N1> p(A.g);
   const int o = (p.in_block ? p.lc : 0) * NQ;
   double u[N1], tq[N1], gx[N1], gy[N1], gz[N1], to[N1];
-  gather_row<N1>(A.g, p, T, false, u);
+  gather<N1>(A.g, p, T, false, lds[4], u);
   evaluate<N1, true>(t, p, u, lds[0] + o, lds[1] + o, lds[2] + o, lds[3] + o, lds[4] + o, tq, gx, gy, gz);
-  gather_row<N1>(A.g, p, T_old, false, u);
-  evaluate<N1, false>(t, p, u, lds[0] + o, lds[1] + o, lds[2] + o, lds[3] + o, lds[4] + o, to, nullptr, nullptr,
+  gather<N1>(A.g, p, T_old, false, lds[0], u);
+  evaluate<N1, false>(t, p, u, lds[1] + o, lds[2] + o, lds[3] + o, lds[4] + o, lds[0] + o, to, nullptr, nullptr,
                       nullptr);
   if (!p.active) return;
   const Geometry &g = A.g;
@@ -308,11 +359,12 @@ __global__ __launch_bounds__(256) void tangent_data_kernel(Tables1D t, TangentAr
     const double gT[3] = {gx[qz] / g.h[0], gy[qz] / g.h[1], gz[qz] / g.h[2]};
     double m = rc / g.dt;
     if (A.solid) m += drc * ((tq[qz] - to[qz]) / g.dt + gT[0] * A.vel[0] + gT[1] * A.vel[1] + gT[2] * A.vel[2]);
-    double *c = td + p.cell * 8 * NQ + qz * N1 * N1 + p.r;
+    double *c = td + p.cell * kTangentCoeffs * NQ + qz * N1 * N1 + p.r;
     c[0] = JxW * m;
-    for (int d = 0; d < 3; ++d) c[(1 + d) * NQ] = JxW * rc * A.vel[d] / g.h[d];
-    c[4 * NQ] = JxW * k;
-    for (int d = 0; d < 3; ++d) c[(5 + d) * NQ] = A.solid ? JxW * dk * gT[d] / g.h[d] : 0.0;
+    c[NQ] = JxW * k;
+    if (g.conv) c[2 * NQ] = JxW * rc;
+    if (g.solid)
+      for (int d = 0; d < 3; ++d) c[(3 + d) * NQ] = JxW * dk * gT[d] / g.h[d];
   }
 }
 
@@ -324,21 +376,28 @@ __global__ __launch_bounds__(256) void tangent_apply_kernel(Tables1D t, Geometry
   const Place<N1> p(g);
   const int o = (p.in_block ? p.lc : 0) * NQ;
   double u[N1], v[N1], gx[N1], gy[N1], gz[N1];
-  gather_row<N1>(g, p, src, true, u);
+  gather<N1>(g, p, src, true, lds[4], u);
   evaluate<N1, true>(t, p, u, lds[0] + o, lds[1] + o, lds[2] + o, lds[3] + o, lds[4] + o, v, gx, gy, gz);
   if (p.active) {
 #pragma unroll
     for (int qz = 0; qz < N1; ++qz) {
-      const double *c = td + p.cell * 8 * NQ + qz * N1 * N1 + p.r;
-      const double uq = v[qz], ax = gx[qz], ay = gy[qz], az = gz[qz];
-      v[qz] = c[0] * uq + c[NQ] * ax + c[2 * NQ] * ay + c[3 * NQ] * az;
-      gx[qz] = c[4 * NQ] * g.ih2[0] * ax + c[5 * NQ] * uq;
-      gy[qz] = c[4 * NQ] * g.ih2[1] * ay + c[6 * NQ] * uq;
-      gz[qz] = c[4 * NQ] * g.ih2[2] * az + c[7 * NQ] * uq;
+      const double *c = td + p.cell * kTangentCoeffs * NQ + qz * N1 * N1 + p.r;
+      const double uq = v[qz], ax = gx[qz], ay = gy[qz], az = gz[qz], k = c[NQ];
+      double val = c[0] * uq;
+      if (g.conv) val += c[2 * NQ] * (g.cv[0] * ax + g.cv[1] * ay + g.cv[2] * az);
+      v[qz] = val;
+      gx[qz] = k * g.ih2[0] * ax;
+      gy[qz] = k * g.ih2[1] * ay;
+      gz[qz] = k * g.ih2[2] * az;
+      if (g.solid) {
+        gx[qz] += c[3 * NQ] * uq;
+        gy[qz] += c[4 * NQ] * uq;
+        gz[qz] += c[5 * NQ] * uq;
+      }
     }
   }
   integrate<N1>(t, p, v, gx, gy, gz, lds[0] + o, lds[1] + o, lds[2] + o, lds[3] + o, lds[4] + o, u);
-  scatter_row<N1>(g, p, dst, u);
+  scatter<N1, true>(g, p, dst, u, lds[0]);
 }
 
 // dst += -R(T): -[rho_cp/dt (T - T_old) - q + rho_cp v . grad T] N - k grad T . grad N
@@ -352,16 +411,16 @@ __global__ __launch_bounds__(256) void tangent_residual_kernel(Tables1D t, Tange
   const Geometry &g = A.g;
   const int o = (p.in_block ? p.lc : 0) * NQ;
   double u[N1], tq[N1], gx[N1], gy[N1], gz[N1], to[N1], qq[N1];
-  gather_row<N1>(g, p, T, false, u);
+  gather<N1>(g, p, T, false, lds[4], u);
   evaluate<N1, true>(t, p, u, lds[0] + o, lds[1] + o, lds[2] + o, lds[3] + o, lds[4] + o, tq, gx, gy, gz);
-  gather_row<N1>(g, p, T_old, false, u);
-  evaluate<N1, false>(t, p, u, lds[0] + o, lds[1] + o, lds[2] + o, lds[3] + o, lds[4] + o, to, nullptr, nullptr,
+  gather<N1>(g, p, T_old, false, lds[0], u);
+  evaluate<N1, false>(t, p, u, lds[1] + o, lds[2] + o, lds[3] + o, lds[4] + o, lds[0] + o, to, nullptr, nullptr,
                       nullptr);
 #pragma unroll
   for (int q = 0; q < N1; ++q) qq[q] = 0.0;
   if (qv) {
-    gather_row<N1>(g, p, qv, false, u);
-    evaluate<N1, false>(t, p, u, lds[0] + o, lds[1] + o, lds[2] + o, lds[3] + o, lds[4] + o, qq, nullptr, nullptr,
+    gather<N1>(g, p, qv, false, lds[0], u);
+    evaluate<N1, false>(t, p, u, lds[1] + o, lds[2] + o, lds[3] + o, lds[4] + o, lds[0] + o, qq, nullptr, nullptr,
                         nullptr);
   }
   if (p.in_block) {
@@ -380,7 +439,7 @@ __global__ __launch_bounds__(256) void tangent_residual_kernel(Tables1D t, Tange
     }
   }
   integrate<N1>(t, p, tq, gx, gy, gz, lds[0] + o, lds[1] + o, lds[2] + o, lds[3] + o, lds[4] + o, u);
-  scatter_row<N1>(g, p, dst, u);
+  scatter<N1, false>(g, p, dst, u, lds[0]);
 }
 
 __device__ __forceinline__ double factor(const Tables1D &t, int code, int q, int i) {
@@ -388,8 +447,8 @@ __device__ __forceinline__ double factor(const Tables1D &t, int code, int q, int
   return code == 0 ? s * s : (code == 1 ? d * d : d * s);
 }
 
-// diag_I = sum_q t0 phi^2 + sum_d (t_{1+d} + t_{5+d}) phi d_d phi + t4/h_d^2 (d_d phi)^2: seven
-// tensor-product terms with 1D factors S^2 (0), G^2 (1), G S (2), three sweeps each
+// diag_I = sum_q t0 phi^2 + sum_d (t2 v_d/h_d + t_{3+d}) phi d_d phi + t1/h_d^2 (d_d phi)^2:
+// seven tensor-product terms with 1D factors S^2 (0), G^2 (1), G S (2), three sweeps each
 template <int N1>
 __global__ __launch_bounds__(256) void tangent_diagonal_kernel(Tables1D t, Geometry g, const double *__restrict__ td,
                                                                double *__restrict__ diag) {
@@ -408,10 +467,15 @@ __global__ __launch_bounds__(256) void tangent_diagonal_kernel(Tables1D t, Geome
       for (int qz = 0; qz < N1; ++qz) {
         double c = 0.0;
         if (p.active) {
-          const double *cc = td + p.cell * 8 * NQ + qz * N1 * N1 + p.r;
-          if (term == 0) c = cc[0];
-          else if (term <= 3) c = cc[term * NQ] + cc[(term + 4) * NQ];
-          else c = cc[4 * NQ] * g.ih2[term - 4];
+          const double *cc = td + p.cell * kTangentCoeffs * NQ + qz * N1 * N1 + p.r;
+          if (term == 0) {
+            c = cc[0];
+          } else if (term <= 3) {
+            if (g.conv) c += cc[2 * NQ] * g.cv[term - 1];
+            if (g.solid) c += cc[(term + 2) * NQ];
+          } else {
+            c = cc[NQ] * g.ih2[term - 4];
+          }
         }
         u[qz] = c;
       }
@@ -448,10 +512,7 @@ __global__ __launch_bounds__(256) void tangent_diagonal_kernel(Tables1D t, Geome
       }
     }
   }
-  if (!p.active) return;
-  const long row = p.row(g);
-#pragma unroll
-  for (int i = 0; i < N1; ++i) atomicAdd(diag + row + i, d[i]);
+  scatter<N1, true>(g, p, diag, d, lds[0]);
 }
 
 __global__ __launch_bounds__(256) void unit_constrained_kernel(double *v, Geometry g, long n) {
@@ -461,8 +522,13 @@ __global__ __launch_bounds__(256) void unit_constrained_kernel(double *v, Geomet
   if (is_constrained(g.dmask, g.nd, ix, iy, iz)) v[i] = 1.0;
 }
 
-Geometry geometry(const OperatorData<double> &op, double dt) {
+Geometry geometry(const OperatorData<double> &op, double dt, const double *vel = nullptr, int solid = 0) {
   Geometry g{};
+  for (int d = 0; d < 3; ++d) {
+    g.cv[d] = vel ? vel[d] / op.h[d] : 0.0;
+    if (g.cv[d] != 0.0) g.conv = 1;
+  }
+  g.solid = solid ? 1 : 0;
   g.nc0 = op.nc[0];
   g.nc1 = op.nc[1];
   g.ncells = op.n_cells;
@@ -480,16 +546,16 @@ Geometry geometry(const OperatorData<double> &op, double dt) {
 int finish(hipStream_t) { return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE; }
 
 #define MPDG_TANGENT_DISPATCH(KERNEL, ...)                                                                       \
-  switch (op.degree) {                                                                                          \
-    case 1: hipLaunchKernelGGL(KERNEL<2>, dim3((op.n_cells + kCells<2> - 1) / kCells<2>), dim3(256), 0, s,      \
-                               __VA_ARGS__); break;                                                             \
-    case 2: hipLaunchKernelGGL(KERNEL<3>, dim3((op.n_cells + kCells<3> - 1) / kCells<3>), dim3(256), 0, s,      \
-                               __VA_ARGS__); break;                                                             \
-    case 3: hipLaunchKernelGGL(KERNEL<4>, dim3((op.n_cells + kCells<4> - 1) / kCells<4>), dim3(256), 0, s,      \
-                               __VA_ARGS__); break;                                                             \
-    case 4: hipLaunchKernelGGL(KERNEL<5>, dim3((op.n_cells + kCells<5> - 1) / kCells<5>), dim3(256), 0, s,      \
-                               __VA_ARGS__); break;                                                             \
-    default: return MPDG_ERR_INVALID;                                                                           \
+  switch (op.degree) {                                                                                           \
+    case 1: hipLaunchKernelGGL(KERNEL<2>, dim3(tangent_blocks(2, op.nc)), dim3(256), 0, s,                       \
+                               __VA_ARGS__); break;                                                              \
+    case 2: hipLaunchKernelGGL(KERNEL<3>, dim3(tangent_blocks(3, op.nc)), dim3(256), 0, s,                       \
+                               __VA_ARGS__); break;                                                              \
+    case 3: hipLaunchKernelGGL(KERNEL<4>, dim3(tangent_blocks(4, op.nc)), dim3(256), 0, s,                       \
+                               __VA_ARGS__); break;                                                              \
+    case 4: hipLaunchKernelGGL(KERNEL<5>, dim3(tangent_blocks(5, op.nc)), dim3(256), 0, s,                       \
+                               __VA_ARGS__); break;                                                              \
+    default: return MPDG_ERR_INVALID;                                                                            \
   }
 
 }  // namespace
@@ -497,7 +563,7 @@ int finish(hipStream_t) { return hipGetLastError() == hipSuccess ? MPDG_OK : MPD
 int launch_tangent_data(const OperatorData<double> &op, double dt, const mpdg_heat_material &mat, int solid,
                         const double *vel, const double *T, const double *T_old, double *td, hipStream_t s) {
   TangentArgs A{};
-  A.g = geometry(op, dt);
+  A.g = geometry(op, dt, vel, solid);
   A.mat = mat;
   for (int d = 0; d < 3; ++d) A.vel[d] = vel[d];
   A.solid = solid;
@@ -506,9 +572,9 @@ int launch_tangent_data(const OperatorData<double> &op, double dt, const mpdg_he
   return finish(s);
 }
 
-int launch_tangent_apply(const OperatorData<double> &op, const double *td, const double *src, double *dst,
-                         hipStream_t s) {
-  const Geometry g = geometry(op, 1.0);
+int launch_tangent_apply(const OperatorData<double> &op, const double *td, const double *vel, int solid,
+                         const double *src, double *dst, hipStream_t s) {
+  const Geometry g = geometry(op, 1.0, vel, solid);
   const Tables1D t = make_tables(op.degree);
   if (hipMemsetAsync(dst, 0, sizeof(double) * op.n_dofs, s) != hipSuccess) return MPDG_ERR_DEVICE;
   MPDG_TANGENT_DISPATCH(tangent_apply_kernel, t, g, td, src, dst)
@@ -527,8 +593,9 @@ int launch_tangent_residual(const OperatorData<double> &op, double dt, const mpd
   return finish(s);
 }
 
-int launch_tangent_diagonal(const OperatorData<double> &op, const double *td, double *diag, hipStream_t s) {
-  const Geometry g = geometry(op, 1.0);
+int launch_tangent_diagonal(const OperatorData<double> &op, const double *td, const double *vel, int solid,
+                            double *diag, hipStream_t s) {
+  const Geometry g = geometry(op, 1.0, vel, solid);
   const Tables1D t = make_tables(op.degree);
   if (hipMemsetAsync(diag, 0, sizeof(double) * op.n_dofs, s) != hipSuccess) return MPDG_ERR_DEVICE;
   MPDG_TANGENT_DISPATCH(tangent_diagonal_kernel, t, g, td, diag)

@@ -5,17 +5,22 @@
 
 namespace mpdg {
 
-// td: 8 n_cells (k+1)^3 values, [cell][coefficient][q]
+// coefficients per quadrature point in the tangent data
+constexpr int kTangentCoeffs = 6;
+
+// td: kTangentCoeffs n_cells (k+1)^3 values, [cell][coefficient][q]
 int launch_tangent_data(const OperatorData<double> &op, double dt, const mpdg_heat_material &mat, int solid,
                         const double *vel, const double *T, const double *T_old, double *td, hipStream_t s);
 // dst = J src (dst zeroed first; constrained rows stay 0, constrained src entries read as 0)
-int launch_tangent_apply(const OperatorData<double> &op, const double *td, const double *src, double *dst,
-                         hipStream_t s);
+// (vel, solid: as given to launch_tangent_data)
+int launch_tangent_apply(const OperatorData<double> &op, const double *td, const double *vel, int solid,
+                         const double *src, double *dst, hipStream_t s);
 // dst += -R(T) (cell part; q may be null)
 int launch_tangent_residual(const OperatorData<double> &op, double dt, const mpdg_heat_material &mat,
                             const double *vel, const double *T, const double *T_old, const double *q, double *dst,
                             hipStream_t s);
 // diag = diagonal of the cell part of J (constrained rows: 1)
-int launch_tangent_diagonal(const OperatorData<double> &op, const double *td, double *diag, hipStream_t s);
+int launch_tangent_diagonal(const OperatorData<double> &op, const double *td, const double *vel, int solid,
+                            double *diag, hipStream_t s);
 
 }  // namespace mpdg

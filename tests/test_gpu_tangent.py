@@ -36,7 +36,13 @@ CASES = [(1, (5, 4, 3), (0.5, 0.4, 0.3), 0, True, (-0.3, 0.1, 0.05)),
          (2, (3, 4, 2), (0.3, 0.4, 0.2), 0b000011, True, (0.2, -0.4, 0.1)),
          (3, (3, 2, 2), (0.3, 0.2, 0.2), 0, False, (-0.5, 0.0, 0.0)),
          (4, (2, 2, 3), (0.2, 0.2, 0.3), 0b110000, True, (0.0, 0.3, -0.2)),
-         (2, (7, 5, 6), (0.7, 0.5, 0.6), 0b001100, True, (0.1, 0.1, 0.1))]
+         (2, (7, 5, 6), (0.7, 0.5, 0.6), 0b001100, True, (0.1, 0.1, 0.1)),
+         # rows of cells longer than one block (several segments, the last one partial);
+         # without velocity and/or without solidification
+         (3, (20, 2, 2), (2.0, 0.2, 0.2), 0b000001, True, (0.0, 0.0, 0.0)),
+         (1, (70, 2, 1), (0.7, 0.2, 0.1), 0, False, (0.0, 0.0, 0.0)),
+         (4, (13, 2, 2), (0.65, 0.1, 0.1), 0b000010, True, (0.2, 0.0, -0.1)),
+         (2, (30, 2, 1), (0.6, 0.04, 0.02), 0, False, (0.0, 0.1, 0.0))]
 
 
 @pytest.mark.parametrize("degree,nc,L,dmask,solid,vel", CASES)
@@ -51,7 +57,7 @@ def test_tangent_vmult_matches_oracle(cuda, degree, nc, L, dmask, solid, vel):
     assert np.linalg.norm(dst.cpu().numpy() - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
-@pytest.mark.parametrize("degree,nc,L,dmask,solid,vel", CASES[:4])
+@pytest.mark.parametrize("degree,nc,L,dmask,solid,vel", CASES[:4] + CASES[5:])
 def test_tangent_diagonal_and_residual_match_oracle(cuda, degree, nc, L, dmask, solid, vel):
     torch = cuda
     import meltpooldg_amd as M
