@@ -137,3 +137,64 @@ def test_robin_tangent_and_rhs(cuda):
     gop.create_rhs(dst, Td, torch.tensor(T_old, device="cuda"))
     torch.cuda.synchronize()
     assert rel_err(dst.cpu().numpy(), ref) < 1e-11
+
+
+# Meshes with 64 columns of chunks (a multiple of the 8 XCDs): the launch then gives each XCD
+# a contiguous range of columns, a path the smaller meshes above do not take.
+XCD_CASES = [(1, (8, 32, 32), "const", 0, 0), (2, (14, 24, 16), "qp", 0, 0), (3, (8, 32, 16), "qp", 0b000101, 0),
+             (4, (4, 32, 24), "law", 0, 0), (3, (8, 32, 16), "cell", 0, 1), (4, (12, 16, 16), "qp", 0, 1)]
+
+
+@pytest.mark.parametrize("degree,nc,mode,dmask,precision", XCD_CASES)
+def test_vmult_xcd_column_launch_matches_oracle(cuda, degree, nc, mode, dmask, precision):
+    torch = cuda
+    orc, gop, _ = make_pair(torch, degree, nc, L=(0.5, 1.0, 0.8), mode=mode, dirichlet=dmask, seed=11,
+                            precision=precision)
+    x = np.random.default_rng(12).uniform(-1.0, 1.0, orc.n_dofs)
+    ref = orc.vmult(x)
+    dtype = torch.float64 if precision == 0 else torch.float32
+    xd = torch.tensor(x, dtype=dtype, device="cuda")
+    yd = torch.full_like(xd, 3.0)
+    gop.vmult(yd, xd)
+    torch.cuda.synchronize()
+    assert rel_err(yd.cpu().numpy(), ref) < (1e-12 if precision == 0 else 1e-5)
+
+
+@pytest.mark.parametrize("degree,N,mode", [(3, 128, "qp"), (4, 96, "law")])
+def test_vmult_full_size_properties(cuda, degree, N, mode):
+    """The benchmark sizes (Q3 on 128^3 cells with per-point conductivity, Q4 on 96^3 cells with
+    k(T) from T DoFs; 57,066,625 DoFs each), checked by properties the oracle need not run:
+    sum_i (J 1)_i = c_m |Omega| (the mass matrix integrates the constant exactly, the stiffness
+    part annihilates constants), symmetry x.(J z) = z.(J x), and linearity."""
+    torch = cuda
+    import meltpooldg_amd as M
+    dt, rho_cp, L = 1e-3, 2.5, (1.0, 0.8, 1.2)
+    op = M.HeatDiffuseMultiPhaseOperator(degree, (N, N, N), L, rho_cp=rho_cp)
+    op.reset_time_increment(dt)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(21)
+    n = op.n_dofs
+    assert n == (degree * N + 1) ** 3 == 57066625
+    if mode == "qp":
+        T = torch.rand(op.n_q_points, generator=g, device="cuda", dtype=torch.float64) * 2.0
+        k = 1.0 + 0.5 * torch.tanh((T - 1.0) / 0.1)
+        del T
+        op.set_conductivity_qp(k)
+    else:
+        T = torch.rand(n, generator=g, device="cuda", dtype=torch.float64) * 2.0
+        op.set_conductivity_law(M.LAW_TANH, (1.0, 0.5, 1.0, 0.1), T)
+    y = torch.empty(n, dtype=torch.float64, device="cuda")
+    op.vmult(y, torch.ones(n, dtype=torch.float64, device="cuda"))
+    c_m, vol = rho_cp / dt, L[0] * L[1] * L[2]
+    assert abs(float(y.sum()) - c_m * vol) <= 1e-10 * c_m * vol
+    x = torch.rand(n, generator=g, device="cuda", dtype=torch.float64) - 0.5
+    z = torch.rand(n, generator=g, device="cuda", dtype=torch.float64) - 0.5
+    jx, jz = torch.empty_like(x), torch.empty_like(x)
+    op.vmult(jx, x)
+    op.vmult(jz, z)
+    a, b = float(torch.dot(z, jx)), float(torch.dot(x, jz))
+    assert abs(a - b) <= 1e-12 * float(torch.linalg.norm(x)) * float(torch.linalg.norm(jz))
+    assert float(torch.dot(x, jx)) > 0.0
+    op.vmult(y, 2.0 * x - 3.0 * z)
+    ref = 2.0 * jx - 3.0 * jz
+    assert float(torch.linalg.norm(y - ref)) <= 1e-13 * float(torch.linalg.norm(ref))
