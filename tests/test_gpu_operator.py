@@ -57,7 +57,8 @@ def test_vmult_mapped(cuda, degree, nc, mode, dmask):
     assert rel_err(d.cpu().numpy(), orc.inverse_diagonal()) < 1e-12
 
 
-@pytest.mark.parametrize("degree,nc,mode", [(2, (6, 6, 6), "qp"), (3, (5, 4, 7), "law"), (4, (4, 4, 4), "const")])
+@pytest.mark.parametrize("degree,nc,mode", [(2, (6, 6, 6), "qp"), (3, (5, 4, 7), "law"), (4, (4, 4, 4), "const"),
+                                             (3, (7, 6, 5), "const"), (3, (9, 5, 4), "cell")])
 def test_vmult_fp32(cuda, degree, nc, mode):
     torch = cuda
     orc, gop, _ = make_pair(torch, degree, nc, mode=mode, precision=1, seed=3)
@@ -142,7 +143,8 @@ def test_robin_tangent_and_rhs(cuda):
 # Meshes with 64 columns of chunks (a multiple of the 8 XCDs): the launch then gives each XCD
 # a contiguous range of columns, a path the smaller meshes above do not take.
 XCD_CASES = [(1, (8, 32, 32), "const", 0, 0), (2, (14, 24, 16), "qp", 0, 0), (3, (8, 32, 16), "qp", 0b000101, 0),
-             (4, (4, 32, 24), "law", 0, 0), (3, (8, 32, 16), "cell", 0, 1), (4, (12, 16, 16), "qp", 0, 1)]
+             (4, (4, 32, 24), "law", 0, 0), (3, (8, 32, 16), "cell", 0, 1), (3, (8, 32, 16), "const", 0b100000, 1),
+             (4, (12, 16, 16), "qp", 0, 1)]
 
 
 @pytest.mark.parametrize("degree,nc,mode,dmask,precision", XCD_CASES)
