@@ -49,46 +49,50 @@ int launch_axpby(double *y, double a, const double *x, double b, long n, hipStre
 constexpr int kMdotBlocks = 512;
 long mdot_scratch() { return (long)kMdotBlocks * kMdotMax; }
 
-// partial sums of J dot products against w: one pass over w, J accumulators per thread
-__global__ __launch_bounds__(256) void mdot_partial_kernel(const double *__restrict__ V, long n, int J,
+// partial sums of the J dot products against w: block (i, b) sums row i of V times w over the
+// b-th share of the index range; blockIdx.x = i runs fastest, so the rows' blocks over one
+// range of w are dispatched together and share its reads in the caches. Fixed summation order
+// (grid-stride, two interleaved accumulators, then a wavefront and block tree).
+__global__ __launch_bounds__(256) void mdot_partial_kernel(const double *__restrict__ V, long n,
                                                            const double *__restrict__ w, double *__restrict__ part) {
-  double acc[kMdotMax];
-#pragma unroll
-  for (int i = 0; i < kMdotMax; ++i) acc[i] = 0.0;
-  for (long k = (long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long)gridDim.x * 256) {
-    const double wk = w[k];
-#pragma unroll
-    for (int i = 0; i < kMdotMax; ++i)
-      if (i < J) acc[i] = fma(V[i * n + k], wk, acc[i]);
+  const int i = blockIdx.x;
+  const double *__restrict__ v = V + (long)i * n;
+  const long stride = (long)gridDim.y * 256;
+  long k = (long)blockIdx.y * 256 + threadIdx.x;
+  double a0 = 0.0, a1 = 0.0;
+  for (; k + stride < n; k += 2 * stride) {
+    a0 = fma(v[k], w[k], a0);
+    a1 = fma(v[k + stride], w[k + stride], a1);
   }
-  __shared__ double red[4][kMdotMax];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int i = 0; i < kMdotMax; ++i) {
-    if (i >= J) break;
-    double v = acc[i];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if (lane == 0) red[wave][i] = v;
-  }
+  if (k < n) a0 = fma(v[k], w[k], a0);
+  double t = a0 + a1;
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
   __syncthreads();
-  if (threadIdx.x < J)
-    part[blockIdx.x * kMdotMax + threadIdx.x] =
-        (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+  if (threadIdx.x == 0) part[(long)i * kMdotBlocks + blockIdx.y] = (red[0] + red[1]) + (red[2] + red[3]);
 }
-__global__ void mdot_final_kernel(const double *__restrict__ part, int nb, int J, double *__restrict__ h,
-                                  int accumulate) {
-  const int i = threadIdx.x;
-  if (i >= J) return;
-  double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[b * kMdotMax + i];
-  h[i] = accumulate ? h[i] + s : s;
+// h[i] (+)= sum over the nb partials of row i: one block per row
+__global__ __launch_bounds__(256) void mdot_final_kernel(const double *__restrict__ part, int nb,
+                                                         double *__restrict__ h, int accumulate) {
+  const int i = blockIdx.x;
+  double t = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) t += part[(long)i * kMdotBlocks + b];
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double sum = (red[0] + red[1]) + (red[2] + red[3]);
+    h[i] = accumulate ? h[i] + sum : sum;
+  }
 }
 int launch_mdot(const double *V, long n, int J, const double *w, double *part, double *h, bool accumulate,
                 hipStream_t s) {
   if (J < 1 || J > kMdotMax) return MPDG_ERR_INVALID;
   const int nb = (int)std::max<long>(1, std::min<long>(kMdotBlocks, (n + 255) / 256));
-  hipLaunchKernelGGL(mdot_partial_kernel, dim3(nb), dim3(256), 0, s, V, n, J, w, part);
-  hipLaunchKernelGGL(mdot_final_kernel, dim3(1), dim3(kMdotMax), 0, s, part, nb, J, h, accumulate ? 1 : 0);
+  hipLaunchKernelGGL(mdot_partial_kernel, dim3(J, nb), dim3(256), 0, s, V, n, w, part);
+  hipLaunchKernelGGL(mdot_final_kernel, dim3(J), dim3(256), 0, s, part, nb, h, accumulate ? 1 : 0);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
