@@ -254,7 +254,7 @@ def main():
             line["solve"] = solve_timings(torch, M, op)
         except Exception as e:  # report, do not hide the apply line
             line["solve"] = {"error": str(e)}
-    if a.scan and k == 3:
+    if a.scan:
         try:
             line["time_to_solve"] = scan_timings(torch, M, k, N, a.scan_train, a.scan_eval, a.scan_epochs,
                                                  a.scan_coarsening, a.scan_variants)
