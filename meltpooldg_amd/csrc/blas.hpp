@@ -62,7 +62,7 @@ int launch_combine(double *y, const double *const *x, const double *w, int m, lo
 
 template <typename T>
 int launch_chebyshev_step(T *x, T *d, const T *b, const T *Ax, const T *inv, long n, double c1, double c2,
-                          bool first, hipStream_t s);
+                          bool zero_start, bool read_d, bool write_d, hipStream_t s);
 template <typename T> int launch_residual(T *r, const T *b, const T *Ax, long n, hipStream_t s);
 template <typename T> int launch_add(T *x, const T *y, long n, hipStream_t s);
 int launch_mul(double *x, const double *y, long n, hipStream_t s);

@@ -657,15 +657,15 @@ struct GMGPrecond : mpdg_precond {
     int degree = (x_zero && prm.pre_smoother_degree > 0) ? prm.pre_smoother_degree : prm.smoother_degree;
     if (degree_override > 0) degree = std::min(degree_override, prm.smoother_degree);
     for (int k = 0; k < degree; ++k) {
-      const bool first = (k == 0);
+      const bool first = (k == 0), last = (k == degree - 1);  // cheb_c1[0] = 0: the first step ignores d
       if (first && x_zero) {
         MPDG_TRY(launch_chebyshev_step<T>(L.x.get(), L.d.get(), L.b.get(), L.Ax.get(), L.inv.get(), n, 0.0,
-                                          L.cheb_c2[0], true, s));
+                                          L.cheb_c2[0], true, false, !last, s));
         continue;
       }
       MPDG_TRY(level_apply<T>(L.od, L.x.get(), L.Ax.get(), s));
       MPDG_TRY(launch_chebyshev_step<T>(L.x.get(), L.d.get(), L.b.get(), L.Ax.get(), L.inv.get(), n, L.cheb_c1[k],
-                                        L.cheb_c2[k], false, s));
+                                        L.cheb_c2[k], false, !first, !last, s));
     }
     return MPDG_OK;
   }
