@@ -29,12 +29,15 @@ namespace mpdg {
 // halve the scatter barriers per column and lengthen the coalesced x rows of the write-out; flat
 // columns (BZ = 1) add face rows on every z plane; two- and eight-wavefront workgroups were
 // slower everywhere. Q3 (FP64 per-qp): 4x4x4 0.694 ms, 16x2x2 0.652, 8x4x2 0.645, 8x2x4 0.667,
-// 16x4x1 0.878, 8x8x1 0.945. Q2: 14x3x2 beats 7x4x3 by 5 % (FP64) and 25 % (FP32). Q4: FP64
-// keeps 4x4x3 (the longer shapes are 1-4 % slower), FP32 gains 11 % with 12x2x2.
+// 16x4x1 0.878, 8x8x1 0.945; FP32 Q3 (4 wavefronts per SIMD with constant k): 16x2x2 is 1-1.5 %
+// faster than 8x4x2 for every conductivity mode, 8x2x4 equal, 4x4x4 9 % slower. Q2: 14x3x2 beats
+// 7x4x3 by 5 % (FP64) and 25 % (FP32). Q4: FP64 keeps 4x4x3 (the longer shapes are 1-4 % slower),
+// FP32 gains 11 % with 12x2x2.
 template <int N1, typename Number> struct ChunkShape;
 template <typename Number> struct ChunkShape<2, Number> { static constexpr int BX = 8, BY = 4, BZ = 4, NW = 4; };
 template <typename Number> struct ChunkShape<3, Number> { static constexpr int BX = 14, BY = 3, BZ = 2, NW = 4; };
-template <typename Number> struct ChunkShape<4, Number> { static constexpr int BX = 8, BY = 4, BZ = 2, NW = 4; };
+template <> struct ChunkShape<4, double> { static constexpr int BX = 8, BY = 4, BZ = 2, NW = 4; };
+template <> struct ChunkShape<4, float> { static constexpr int BX = 16, BY = 2, BZ = 2, NW = 4; };
 template <> struct ChunkShape<5, double> { static constexpr int BX = 4, BY = 4, BZ = 3, NW = 4; };
 template <> struct ChunkShape<5, float> { static constexpr int BX = 12, BY = 2, BZ = 2, NW = 4; };
 
