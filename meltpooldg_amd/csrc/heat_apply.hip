@@ -40,15 +40,21 @@ template <> struct ChunkShape<4, double> { static constexpr int BX = 8, BY = 4, 
 template <> struct ChunkShape<4, float> { static constexpr int BX = 16, BY = 2, BZ = 2, NW = 4; };
 template <> struct ChunkShape<5, double> { static constexpr int BX = 4, BY = 4, BZ = 3, NW = 4; };
 template <> struct ChunkShape<5, float> { static constexpr int BX = 12, BY = 2, BZ = 2, NW = 4; };
+// mapped cells (metric per point): FP64 Q3 runs 8 % faster with 16x2x2 than with the Cartesian
+// 8x4x2 (Q3 96^3, 1.02 vs 1.11 ms), where 8x4x2 stays 2 % ahead
+template <int N1, typename Number, bool MAPPED> struct ChunkShapeOf { using type = ChunkShape<N1, Number>; };
+template <> struct ChunkShapeOf<4, double, true> {
+  struct type { static constexpr int BX = 16, BY = 2, BZ = 2, NW = 4; };
+};
 
-template <int N1, typename Number = double> struct ApplyGeom {
+template <int N1, typename Number = double, bool MAPPED = false> struct ApplyGeom {
+  using Shape = typename ChunkShapeOf<N1, Number, MAPPED>::type;
   static constexpr int K = N1 - 1;
   static constexpr int CPW = 64 / N1;                 // cells per wavefront
-  static constexpr int NW = ChunkShape<N1, Number>::NW;  // wavefronts per workgroup
+  static constexpr int NW = Shape::NW;                // wavefronts per workgroup
   static constexpr int NT = 64 * NW;                  // threads per workgroup
   static constexpr int CPB = NW * CPW;                // cells per chunk
-  static constexpr int BX = ChunkShape<N1, Number>::BX, BY = ChunkShape<N1, Number>::BY,
-                       BZ = ChunkShape<N1, Number>::BZ;
+  static constexpr int BX = Shape::BX, BY = Shape::BY, BZ = Shape::BZ;
   static_assert(BX * BY * BZ == CPB, "chunk shape must match the cells per workgroup");
   static constexpr int SQY = N1 * N1 + 1;             // padded qy stride of a cell's transpose slot
   static constexpr int CS = N1 * SQY;                 // transpose slot per cell
@@ -257,11 +263,11 @@ template <int KM> struct Occupancy<4, float, true, KM> { static constexpr int W 
 template <int KM> struct Occupancy<5, float, false, KM> { static constexpr int W = 2; };
 
 template <int N1, typename Number, int KMODE, bool MAPPED>
-__global__ __launch_bounds__((ApplyGeom<N1, Number>::NT)) __attribute__((amdgpu_waves_per_eu(Occupancy<N1, Number, MAPPED, KMODE>::W)))
+__global__ __launch_bounds__((ApplyGeom<N1, Number, MAPPED>::NT)) __attribute__((amdgpu_waves_per_eu(Occupancy<N1, Number, MAPPED, KMODE>::W)))
 void heat_apply_kernel(const ApplyArgs<Number, N1> P,
                                                          const Number *__restrict__ src,
                                                          Number *__restrict__ dst) {
-  using G = ApplyGeom<N1, Number>;
+  using G = ApplyGeom<N1, Number, MAPPED>;
   constexpr int K = G::K;
   __shared__ double lds_raw[G::TBUF_DOUBLES];
   __shared__ double acc[G::ACC];     // DoF brick of the chunk
@@ -531,12 +537,10 @@ __global__ __launch_bounds__(128) void zero_face_rows_kernel(Number *__restrict_
   for (int ix = threadIdx.x; ix < nd0; ix += 128) row[ix] = 0;
 }
 
-template <int N1, typename Number>
-int launch_apply_n1(const OperatorData<Number> &op_in, const Number *src, Number *dst, hipStream_t stream,
-                    hipEvent_t kernel_start) {
-  using G = ApplyGeom<N1, Number>;
-  const OperatorData<Number> op = kernel_view(op_in);
-  if (op.metric == nullptr && op_in.metric != nullptr) return MPDG_ERR_INVALID;  // metric_k not computed
+template <int N1, typename Number, bool MAPPED>
+int launch_apply_shape(const OperatorData<Number> &op, const Number *src, Number *dst, hipStream_t stream,
+                       hipEvent_t kernel_start) {
+  using G = ApplyGeom<N1, Number, MAPPED>;
   ApplyArgs<Number, N1> A;
   fill_apply_args(op, A);
   A.ncol[0] = (op.nc[1] + G::BY - 1) / G::BY;
@@ -552,7 +556,7 @@ int launch_apply_n1(const OperatorData<Number> &op_in, const Number *src, Number
                        op.nd[0], op.nd[1], op.nd[2], py, pz, nfy, nfz);
   }
   if (kernel_start) hipEventRecord(kernel_start, stream);
-  if (op.metric) {
+  if constexpr (MAPPED) {
     switch (op.kappa_mode) {
       case KAPPA_CONST: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CONST, true>), dim3(ncols), dim3(G::NT), 0, stream, A, src, dst); break;
       case KAPPA_CELL: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CELL, true>), dim3(ncols), dim3(G::NT), 0, stream, A, src, dst); break;
@@ -568,6 +572,15 @@ int launch_apply_n1(const OperatorData<Number> &op_in, const Number *src, Number
     }
   }
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+template <int N1, typename Number>
+int launch_apply_n1(const OperatorData<Number> &op_in, const Number *src, Number *dst, hipStream_t stream,
+                    hipEvent_t kernel_start) {
+  const OperatorData<Number> op = kernel_view(op_in);
+  if (op.metric == nullptr && op_in.metric != nullptr) return MPDG_ERR_INVALID;  // metric_k not computed
+  if (op.metric) return launch_apply_shape<N1, Number, true>(op, src, dst, stream, kernel_start);
+  return launch_apply_shape<N1, Number, false>(op, src, dst, stream, kernel_start);
 }
 
 template <typename Number>

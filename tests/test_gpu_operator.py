@@ -200,3 +200,18 @@ def test_vmult_full_size_properties(cuda, degree, N, mode):
     op.vmult(y, 2.0 * x - 3.0 * z)
     ref = 2.0 * jx - 3.0 * jz
     assert float(torch.linalg.norm(y - ref)) <= 1e-13 * float(torch.linalg.norm(ref))
+
+
+@pytest.mark.parametrize("degree,nc,mode,precision", [(3, (8, 16, 16), "const", 0), (3, (12, 16, 16), "cell", 0),
+                                                      (3, (16, 16, 16), "const", 1)])
+def test_vmult_mapped_xcd_column_launch_matches_oracle(cuda, degree, nc, mode, precision):
+    """Mapped cells take their own chunk shape; meshes with 64 columns of it (XCD-remapped launch)."""
+    torch = cuda
+    orc, gop, _ = make_pair(torch, degree, nc, mode=mode, mapped=0.05, seed=13, precision=precision)
+    x = np.random.default_rng(14).uniform(-1.0, 1.0, orc.n_dofs)
+    ref = orc.vmult(x)
+    dtype = torch.float64 if precision == 0 else torch.float32
+    y = torch.full((orc.n_dofs,), 3.0, dtype=dtype, device="cuda")
+    gop.vmult(y, torch.tensor(x, dtype=dtype, device="cuda"))
+    torch.cuda.synchronize()
+    assert rel_err(y.cpu().numpy(), ref) < (1e-12 if precision == 0 else 1e-5)
