@@ -27,8 +27,9 @@ namespace mpdg {
 // cells of a chunk along x, y, z and wavefronts per workgroup (NW * (64 / N1) cells per chunk),
 // chosen by measurement of the apply on large meshes (Q2 64^3, Q3 128^3, Q4 96^3). Longer chunks
 // halve the scatter barriers per column and lengthen the coalesced x rows of the write-out; flat
-// columns (BZ = 1) add face rows on every z plane; two- and eight-wavefront workgroups were
-// slower everywhere. Q3 (FP64 per-qp): 4x4x4 0.694 ms, 16x2x2 0.652, 8x4x2 0.645, 8x2x4 0.667,
+// columns (BZ = 1) add face rows on every z plane; one-, two- and eight-wavefront workgroups were
+// slower everywhere (one wavefront, Q3 4x2x2: 0.70 vs 0.64 ms; its barriers are free, but its
+// 2x2-cell columns share 36 % of their rows with neighbours). Q3 (FP64 per-qp): 4x4x4 0.694 ms, 16x2x2 0.652, 8x4x2 0.645, 8x2x4 0.667,
 // 16x4x1 0.878, 8x8x1 0.945; FP32 Q3 (4 wavefronts per SIMD with constant k): 16x2x2 is 1-1.5 %
 // faster than 8x4x2 for every conductivity mode, 8x2x4 equal, 4x4x4 9 % slower. Q2: 14x3x2 beats
 // 7x4x3 by 5 % (FP64) and 25 % (FP32). Q4: FP64 keeps 4x4x3 (the longer shapes are 1-4 % slower),
