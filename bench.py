@@ -51,6 +51,11 @@ def parse():
     ap.add_argument("--scan-eval", type=int, default=10)
     ap.add_argument("--scan-epochs", type=int, default=4)
     ap.add_argument("--scan-coarsening", type=int, default=4)
+    ap.add_argument("--scan-law", type=int, default=0,
+                    help="scan with k = k0 (1 + 0.5 tanh((T - 1)/0.1)) from the T DoFs (frozen at T_n each "
+                         "step) and T0 = 0.95 instead of k = 1 and T0 = 0")
+    ap.add_argument("--scan-coarse-cells", type=int, default=4,
+                    help="cells per direction of the multigrid's coarse Q2 level in the scan")
     ap.add_argument("--scan-variants", default="all",
                     help="comma-separated solver/predictor rows of time_to_solve (default: all)")
     ap.add_argument("--copy", type=int, default=0,  # kept for old command lines; the copy rate is always measured
@@ -123,13 +128,17 @@ def solve_timings(torch, M, op, reps=2):
     return out
 
 
-def scan_timings(torch, M, k, N, n_train, n_eval, epochs, rho, only="all"):
+def scan_timings(torch, M, k, N, n_train, n_eval, epochs, rho, only="all", law=False, coarse=4):
     """Time-to-solve per step of a laser scan (see the module docstring)."""
     import numpy as np
     from meltpooldg_amd.application import LaserScan, generate_trajectory, train
     dt = 1e-3
+    kw = dict(n_cells_coarse=coarse)
+    if law:
+        kw["law"] = (M.LAW_TANH, (1.0, 0.5, 1.0, 0.1))
     t0 = time.perf_counter()
-    _, Ts, qs, zs, _ = generate_trajectory(k, N, n_train, dt=dt, precond="gmg32", predictor=M.PREDICTOR_LINEAR)
+    _, Ts, qs, zs, _ = generate_trajectory(k, N, n_train, T_init=0.95 if law else None, dt=dt, precond="gmg32",
+                                           predictor=M.PREDICTOR_LINEAR, **kw)
     torch.cuda.synchronize()
     t_traj = time.perf_counter() - t0
     wanted = None if only == "all" else set(only.split(","))
@@ -141,9 +150,10 @@ def scan_timings(torch, M, k, N, n_train, n_eval, epochs, rho, only="all"):
         torch.cuda.synchronize()
         t_train = time.perf_counter() - t0
     del qs, zs
-    out = {"workload": "laser scan, Q%d %d^3 cells, k=1, dt=1e-3, steps %d-%d, ||b-JT|| <= 1e-10 ||b||, coarse Q2 "
-                       "4^3; FP64 GMG: Chebyshev 3 pre / 3 post (CG), FP32 GMG: Chebyshev 1 pre / 3 post "
-                       "(flexible CG)" % (k, N, n_train + 1, n_train + n_eval),
+    out = {"workload": "laser scan, Q%d %d^3 cells, %s, dt=1e-3, steps %d-%d, ||b-JT|| <= 1e-10 ||b||, coarse Q2 "
+                       "%d^3; FP64 GMG: Chebyshev 3 pre / 3 post (CG), FP32 GMG: Chebyshev 1 pre / 3 post "
+                       "(flexible CG)" % (k, N, "k = 1 + 0.5 tanh((T - 1)/0.1) from T (frozen at T_n), T0 = 0.95"
+                                          if law else "k=1", n_train + 1, n_train + n_eval, coarse),
            "training": {"pairs": n_train - 2, "epochs": epochs, "coarsening": rho, "seconds": round(t_train, 2),
                         "loss_first_epoch": losses[0], "loss_last_epoch": losses[-1],
                         "trajectory_seconds": round(t_traj, 2)}}
@@ -158,7 +168,7 @@ def scan_timings(torch, M, k, N, n_train, n_eval, epochs, rho, only="all"):
     for name, pc, pred, order in variants:
         case = LaserScan(k, N, dt=dt, precond=pc, predictor=pred,
                          learned=net if pred == M.PREDICTOR_LEARNED else None,
-                         pre_smoother_degree=1 if pc == "gmg32" else 0, extrapolation_order=order)
+                         pre_smoother_degree=1 if pc == "gmg32" else 0, extrapolation_order=order, **kw)
         # restart from the recorded state; the replayed steps rebuild the history the predictor uses
         case.t = (n_train - order) * dt
         case.heat.set_initial_condition(Ts[n_train - order])
@@ -257,7 +267,8 @@ def main():
     if a.scan:
         try:
             line["time_to_solve"] = scan_timings(torch, M, k, N, a.scan_train, a.scan_eval, a.scan_epochs,
-                                                 a.scan_coarsening, a.scan_variants)
+                                                 a.scan_coarsening, a.scan_variants, law=bool(a.scan_law),
+                                                 coarse=a.scan_coarse_cells)
         except Exception as e:  # report, do not hide the apply line
             line["time_to_solve"] = {"error": str(e)}
     if a.cpu:

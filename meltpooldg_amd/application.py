@@ -80,10 +80,13 @@ class LaserScan:
         return self.heat.solution()
 
 
-def generate_trajectory(degree, cells, n_steps, **kw):
-    """Run n_steps and keep every T_n (T_0 = 0 included), the source field of each step and the
-    network's residual input of each step: qs[n], zs[n] belong to the step T_n -> T_{n+1}."""
+def generate_trajectory(degree, cells, n_steps, T_init=None, **kw):
+    """Run n_steps and keep every T_n (T_0 = 0, or the uniform T_init, included), the source field
+    of each step and the network's residual input of each step: qs[n], zs[n] belong to the step
+    T_n -> T_{n+1}."""
     case = LaserScan(degree, cells, **kw)
+    if T_init is not None:
+        case.heat.set_initial_condition(case.torch.full_like(case.T0, T_init))
     z = case.torch.zeros_like(case.T0)
     case.heat.record_guess_feature(z)
     Ts = [case.solution()]
