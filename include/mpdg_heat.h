@@ -81,7 +81,11 @@ int mpdg_heat_op_set_robin(mpdg_heat_op *op, int face_mask, double alpha, double
    non-affine cells (scratch_data.cpp:192-266); boundary-face terms need the affine box. */
 int mpdg_heat_op_set_mapping(mpdg_heat_op *op, const double *coords);
 /* Gaussian surface flux on face z+ used by create_rhs: q = peak*exp(-((x-x0-vx*t)^2+(y-y0)^2)/(2 sigma^2)),
-   p = {peak, sigma, x0, y0, vx, t}; peak = 0 switches it off */
+   p = {peak, sigma, x0, y0, vx, t}; peak = 0 switches it off. The reference's
+   GaussProjectionIntensityProfile (include/meltpooldg/heat/laser_intensity_profiles.hpp:98-165,
+   2P/(pi r^2) exp(-2 d^2/r^2)) is sigma = r/2, peak = 2P/(pi r^2). MPDG_ERR_INVALID for sigma <= 0
+   or when face z+ carries a Robin term (and set_robin refuses z+ while a flux is set): the
+   reference forbids Neumann and radiation/convection on one face (heat_diffuse_operator.cpp:739-743) */
 int mpdg_heat_op_set_surface_flux(mpdg_heat_op *op, const double *p6);
 
 /* Full Newton tangent with a temperature-dependent material and a prescribed constant
@@ -190,9 +194,15 @@ typedef struct mpdg_heat_operation mpdg_heat_operation;
 #define MPDG_PREDICTOR_LEARNED 3
 #define MPDG_PREDICTOR_EXTRAPOLATION 4 /* polynomial extrapolation of order extrapolation_order
                                           (1: linear_extrapolation) through the last solutions */
-#define MPDG_PREDICTOR_LSQ 5 /* least_squares_projection (predictor.hpp:19-86): the combination of
-                                the last n_old_solution_vectors solutions whose residual
-                                ||b - J x|| is smallest, b = create_rhs(T = 0, T_old = T_n) */
+#define MPDG_PREDICTOR_LSQ 5 /* least_squares_projection as the reference (predictor.hpp:19-86,
+                                heat_diffuse_operation.cpp:233-244): the combination of the last
+                                n_old_solution_vectors solutions T_n, T_{n-1}, ... minimising
+                                ||r - J x|| with r = create_rhs at T = T_old = T_n, J orthogonalised
+                                images by modified Gram-Schmidt; the previous solution until as many
+                                steps as solutions have run */
+#define MPDG_PREDICTOR_LSQ_SOLUTION 6 /* the same projection onto the step's own right-hand side
+                                b (create_rhs at T = 0, T_old = T_n), so that the combination is the
+                                best solution guess in that span (no counterpart in the reference) */
 #define MPDG_STEP_NEWTON 0 /* Newton-Raphson as the reference (>= 2 linear solves per step) */
 #define MPDG_STEP_LINEAR 1 /* one linear solve J(T_n) T_{n+1} = b per step, coefficient frozen at T_n,
                               stopped at ||b - J T|| <= rel_tolerance ||b|| (no radiation faces) */
@@ -209,7 +219,7 @@ typedef struct mpdg_heat_operation_params {
                              steps of its finest level; 0: none) */
   int extrapolation_order;/* MPDG_PREDICTOR_EXTRAPOLATION: 1..6 (the order drops to what the
                              history holds: after set_initial the first steps use fewer points) */
-  int n_old_solution_vectors; /* MPDG_PREDICTOR_LSQ: 1..7 solutions (predictor_data.hpp:27-28;
+  int n_old_solution_vectors; /* MPDG_PREDICTOR_LSQ(_SOLUTION): 1..7 solutions (predictor_data.hpp:27-28;
                                  until the history holds them, the previous solution is used) */
   int preconditioner_update_interval; /* linear step mode with a conductivity law: the
                                  preconditioner is rebuilt every this many steps (0 or 1: every

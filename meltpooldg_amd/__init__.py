@@ -35,6 +35,7 @@ TRANSITION_NONE, TRANSITION_MUSHY = 0, 1
 CRIT_REDUCTION, CRIT_RHS = 0, 1
 PREDICTOR_NONE, PREDICTOR_ZERO, PREDICTOR_LINEAR, PREDICTOR_LEARNED, PREDICTOR_EXTRAPOLATION, PREDICTOR_LSQ = \
     0, 1, 2, 3, 4, 5
+PREDICTOR_LSQ_SOLUTION = 6
 
 
 class MpdgError(RuntimeError):

@@ -431,6 +431,9 @@ int launch_boundary(const OperatorData<double> &op, int mode, const double *flux
   A.flux_on = (mode == 2 && flux && flux[0] != 0.0) ? 1 : 0;
   A.face_mask = op.robin_mask | (A.flux_on ? 32 : 0);
   if (!A.face_mask) return MPDG_OK;
+  // a Neumann flux and a radiation/convection term on the same face are refused, as the
+  // reference's rhs_boundary_loop does (heat_diffuse_operator.cpp:739-743)
+  if (A.flux_on && (op.robin_mask & 32)) return MPDG_ERR_INVALID;
   if (op.metric) return MPDG_ERR_INVALID;  // boundary faces are implemented for the affine box
   A.dmask = op.dmask;
   A.mode = mode;

@@ -986,6 +986,7 @@ struct mpdg_heat_operation {
   bool law_frozen = false;
   int fell_back = 0;
   int steps_since_update = 0;  // linear steps since the preconditioner was rebuilt
+  int n_calls = 0;             // predictor calls since set_initial (Predictor::n_calls, predictor.hpp:97)
   double rel_res0 = 0.0, rel_res = 0.0;
 };
 
@@ -1062,6 +1063,8 @@ int mpdg_heat_op_set_robin(mpdg_heat_op *op, int mask, double alpha, double T_in
                            double T_inf_rad, const double *T) {
   if (!op || op->precision != MPDG_FP64) return MPDG_ERR_INVALID;
   if (emissivity > 0.0 && !T) return MPDG_ERR_INVALID;
+  // Neumann flux and Robin terms on one face (z+) are refused (heat_diffuse_operator.cpp:739-743)
+  if ((mask & 32) && op->flux[0] != 0.0) return MPDG_ERR_INVALID;
   op->d64.robin_mask = mask & 63;
   op->d64.alpha = alpha;
   op->d64.T_inf_conv = T_inf_conv;
@@ -1091,6 +1094,7 @@ int mpdg_heat_op_set_mapping(mpdg_heat_op *op, const double *coords) {
 
 int mpdg_heat_op_set_surface_flux(mpdg_heat_op *op, const double *p) {
   if (!op || !p) return MPDG_ERR_INVALID;
+  if (p[0] != 0.0 && (p[1] <= 0.0 || (op->d64.robin_mask & 32))) return MPDG_ERR_INVALID;
   for (int i = 0; i < 6; ++i) op->flux[i] = p[i];
   return MPDG_OK;
 }
@@ -1259,8 +1263,9 @@ int mpdg_linear_solve(mpdg_heat_op *op, mpdg_precond *p, const mpdg_solver_param
 int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p, const mpdg_heat_operation_params *prm,
                                mpdg_heat_operation **out) {
   if (!op || !prm || !out || op->precision != MPDG_FP64) return MPDG_ERR_INVALID;
-  if (prm->predictor < MPDG_PREDICTOR_NONE || prm->predictor > MPDG_PREDICTOR_LSQ) return MPDG_ERR_INVALID;
-  if (prm->predictor == MPDG_PREDICTOR_LSQ &&
+  if (prm->predictor < MPDG_PREDICTOR_NONE || prm->predictor > MPDG_PREDICTOR_LSQ_SOLUTION) return MPDG_ERR_INVALID;
+  const bool lsq = prm->predictor == MPDG_PREDICTOR_LSQ || prm->predictor == MPDG_PREDICTOR_LSQ_SOLUTION;
+  if (lsq &&
       (prm->n_old_solution_vectors < 1 || prm->n_old_solution_vectors > mpdg_heat_operation::kMaxOlder + 2))
     return MPDG_ERR_INVALID;
   if (prm->step_mode != MPDG_STEP_NEWTON && prm->step_mode != MPDG_STEP_LINEAR) return MPDG_ERR_INVALID;
@@ -1276,7 +1281,7 @@ int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p, const mpdg_hea
   const long n = op->n_dofs();
   int rc = MPDG_OK;
   h->depth = prm->predictor == MPDG_PREDICTOR_EXTRAPOLATION ? prm->extrapolation_order - 1
-             : prm->predictor == MPDG_PREDICTOR_LSQ          ? std::max(0, prm->n_old_solution_vectors - 2)
+             : lsq                                           ? std::max(0, prm->n_old_solution_vectors - 2)
                                                              : 0;
   for (int b = 0; b < 3 + h->depth; ++b) rc = rc ? rc : h->buf[b].resize(n);
   rc = rc ? rc : h->rhs.resize(n);
@@ -1305,6 +1310,7 @@ int mpdg_heat_operation_set_initial(mpdg_heat_operation *h, const double *T0, vo
   MPDG_TRY(launch_convert<double, double>(T0, h->buf[h->old].get(), n, S(stream)));
   h->dt_old = 0.0;
   h->n_older = 0;
+  h->n_calls = 0;
   return hipStreamSynchronize(S(stream)) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
@@ -1480,8 +1486,9 @@ static int extrapolate(mpdg_heat_operation *h, double dt, double *pred, hipStrea
   return launch_combine(pred, pts, w, m, n, s);
 }
 
-// LeastSquaresProjection::vmult (predictor.hpp:36-80): x = sum_s c_s T_{n-s} (s < L) minimising
-// ||b - J x||. The reference orthogonalises the images J T_{n-s} by modified Gram-Schmidt; here
+// MPDG_PREDICTOR_LSQ_SOLUTION, a variant of LeastSquaresProjection::vmult (predictor.hpp:36-80)
+// with the step's right-hand side: x = sum_s c_s T_{n-s} (s < L) minimising ||b - J x||. The
+// reference orthogonalises the images J T_{n-s} by modified Gram-Schmidt; here
 // the basis is first changed to backward differences (T_n, T_n - T_{n-1}, second difference,
 // ...), whose images differ by orders of magnitude instead of being nearly parallel, and the
 // images are orthonormalised by classical Gram-Schmidt applied twice (fused multi-dot /
@@ -1546,6 +1553,56 @@ static int lsq_project(mpdg_heat_operation *h, const double *b, double *pred, hi
   return launch_maxpy(pred, X, n, m, hd, 1.0, s);
 }
 
+// LeastSquaresProjection::vmult as the reference computes it (predictor.hpp:36-80): images
+// v_s = J T_{n-s} orthogonalised in order by modified Gram-Schmidt without normalisation
+// (P(j, s) = v_s . v_j after the first j projections, P(s, s) = |v_s|^2), stopped at the first
+// image with P(s, s) < 1e-16 P(0, 0); coefficients by back substitution of
+// P(s, s) c_s = v_s . rhs - sum_{j > s} P(s, j) c_j; x = sum_s c_s T_{n-s}. Until the predictor
+// has been called n_old_solution_vectors times the previous solution is used (:147-151).
+static int lsq_reference(mpdg_heat_operation *h, const double *rhs, double *pred, hipStream_t s) {
+  mpdg_heat_op *op = h->op;
+  const long n = op->n_dofs();
+  const int L = h->prm.n_old_solution_vectors;
+  const double *pts[mpdg_heat_operation::kMaxOlder + 2];
+  int avail = 0;
+  pts[avail++] = h->buf[h->cur].get();
+  if (h->dt_old > 0.0) {
+    pts[avail++] = h->buf[h->old].get();
+    for (int j = 0; j < h->n_older; ++j) pts[avail++] = h->buf[h->older[j]].get();
+  }
+  if (h->n_calls < L || avail < L) return launch_convert<double, double>(pts[0], pred, n, s);
+  MPDG_TRY(h->lsq_q.resize((size_t)L * n));
+  MPDG_TRY(h->lsq_h.resize(4));
+  double *V = h->lsq_q.get(), *hd = h->lsq_h.get();
+  auto dot = [&](const double *a, const double *b, double *out) -> int {
+    MPDG_TRY(launch_dot<double, double>(a, b, n, h->red.red(), hd, s));
+    if (hipMemcpyAsync(out, hd, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) return MPDG_ERR_DEVICE;
+    return hipStreamSynchronize(s) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+  };
+  double P[mpdg_heat_operation::kMaxOlder + 2][mpdg_heat_operation::kMaxOlder + 2] = {};
+  int step = 0;
+  for (; step < L; ++step) {
+    double *vs = V + (size_t)step * n;
+    MPDG_TRY(op->vmult(vs, pts[step], s));
+    MPDG_TRY(dot(vs, V, &P[0][step]));
+    for (int j = 0; j < step; ++j) {
+      MPDG_TRY(launch_axpby(vs, -P[j][step] / P[j][j], V + (size_t)j * n, 1.0, n, s));
+      MPDG_TRY(dot(vs, V + (size_t)(j + 1) * n, &P[j + 1][step]));
+    }
+    if (!(P[step][step] == P[step][step])) return MPDG_ERR_NONFINITE;
+    if (P[step][step] < 1e-16 * P[0][0]) break;
+  }
+  if (step == 0) return launch_convert<double, double>(pts[0], pred, n, s);
+  double c[mpdg_heat_operation::kMaxOlder + 2];
+  for (int k = 0; k < step; ++k) MPDG_TRY(dot(V + (size_t)k * n, rhs, &c[k]));
+  for (int k = step - 1; k >= 0; --k) {
+    double sum = c[k];
+    for (int j = k + 1; j < step; ++j) sum -= c[j] * P[k][j];
+    c[k] = sum / P[k][k];
+  }
+  return launch_combine(pred, pts, c, step, n, s);
+}
+
 // HeatDiffuseOperation::init_time_advance + solve (heat_diffuse_operation.cpp:215-289)
 int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_iters, int *nl_iters, void *stream) {
   if (!h) return MPDG_ERR_INVALID;
@@ -1556,6 +1613,10 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
   // Predictor::vmult (predictor.hpp:118-160): pred from T_n (cur) and T_{n-1} (old)
   double *pred = h->buf[h->spare].get();
   const double *Tn = h->buf[h->cur].get(), *Tnm1 = h->buf[h->old].get();
+  // the predictor's operator and residual see the conductivity law and the radiation
+  // linearisation at T_n (the step rebinds them to its own iterate afterwards)
+  if (op->d64.kappa_mode == KAPPA_LAW) op->d64.T = Tn;
+  if (op->d64.robin_mask) op->T_robin = Tn;
   switch (h->prm.predictor) {
     case MPDG_PREDICTOR_NONE: MPDG_TRY(launch_convert<double, double>(Tn, pred, n, s)); break;
     case MPDG_PREDICTOR_ZERO: MPDG_TRY(launch_fill<double>(pred, 0.0, n, s)); break;
@@ -1574,7 +1635,15 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
     }
     case MPDG_PREDICTOR_EXTRAPOLATION: MPDG_TRY(extrapolate(h, dt, pred, s)); break;
     case MPDG_PREDICTOR_LSQ: {
-      // predictor rhs (heat_diffuse_operation.cpp:233-244): here the step's b with T_old = T_n
+      // predictor rhs (heat_diffuse_operation.cpp:233-244): user_rhs (zero) + create_rhs with the
+      // current solution as T and as T_old
+      MPDG_TRY(launch_fill<double>(h->bvec.get(), 0.0, n, s));
+      MPDG_TRY(op->create_rhs(h->bvec.get(), Tn, Tn, nullptr, s));
+      MPDG_TRY(lsq_reference(h, h->bvec.get(), pred, s));
+      break;
+    }
+    case MPDG_PREDICTOR_LSQ_SOLUTION: {
+      // the step's b with T_old = T_n, for which the projection is a solution guess
       MPDG_TRY(launch_fill<double>(h->bvec.get(), 0.0, n, s));
       MPDG_TRY(op->create_rhs(h->bvec.get(), h->zero.get(), Tn, nullptr, s, true));
       MPDG_TRY(lsq_project(h, h->bvec.get(), pred, s));
@@ -1590,6 +1659,7 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
     h->older[0] = h->old;
     h->n_older = h->dt_old > 0.0 ? std::min(h->n_older + 1, h->depth) : 0;
   }
+  ++h->n_calls;
   const int new_old = h->cur, new_cur = h->spare;
   h->old = new_old;
   h->cur = new_cur;

@@ -22,7 +22,7 @@ def make_net(M, shape, degree=1, rho=1, **kw):
 
 
 @pytest.mark.parametrize("shape,degree,rho", [((5, 6, 7), 1, 1), ((4, 9, 21), 1, 1), ((17, 13, 9), 2, 2),
-                                              ((13, 25, 13), 3, 4)])
+                                              ((13, 25, 13), 3, 4), ((9, 13, 17), 4, 1), ((9, 17, 17), 4, 2)])
 def test_forward_matches_oracle(cuda, shape, degree, rho):
     torch = cuda
     import meltpooldg_amd as M
@@ -40,7 +40,7 @@ def test_forward_matches_oracle(cuda, shape, degree, rho):
     assert np.linalg.norm(got - exp) <= 2e-2 * np.linalg.norm(exp)
 
 
-@pytest.mark.parametrize("shape,degree,rho", [((6, 7, 18), 1, 1), ((17, 13, 9), 2, 2)])
+@pytest.mark.parametrize("shape,degree,rho", [((6, 7, 18), 1, 1), ((17, 13, 9), 2, 2), ((9, 17, 17), 4, 2)])
 def test_gradients_match_oracle(cuda, shape, degree, rho):
     torch = cuda
     import meltpooldg_amd as M

@@ -215,3 +215,116 @@ def test_vmult_mapped_xcd_column_launch_matches_oracle(cuda, degree, nc, mode, p
     gop.vmult(y, torch.tensor(x, dtype=dtype, device="cuda"))
     torch.cuda.synchronize()
     assert rel_err(y.cpu().numpy(), ref) < (1e-12 if precision == 0 else 1e-5)
+
+
+# Meshes longer than one chunk along x that are no multiple of it, with ragged y/z columns: the
+# x-face carried in LDS between chunks, for every chunk shape (FP64 Q3 8x4x2, FP32 Q3 16x2x2,
+# mapped Q3 16x2x2, FP64 Q4 4x4x3, FP32 Q4 12x2x2, FP64 Q2 14x3x2).
+WIDE_CASES = [(3, (19, 5, 3), "qp", 0, None), (3, (37, 5, 3), "const", 1, None), (3, (37, 5, 3), "qp", 1, None),
+              (3, (37, 5, 3), "law", 1, None), (3, (35, 4, 5), "const", 0, 0.05), (3, (35, 4, 5), "qp", 0, 0.05),
+              (3, (35, 4, 5), "const", 1, 0.05), (4, (9, 5, 4), "qp", 0, None), (4, (27, 3, 5), "const", 1, None),
+              (4, (27, 3, 5), "law", 1, None), (2, (31, 4, 3), "cell", 0, None), (4, (9, 5, 4), "const", 0, 0.05)]
+
+
+@pytest.mark.parametrize("degree,nc,mode,precision,mapped", WIDE_CASES)
+def test_vmult_chunks_along_x_match_oracle(cuda, degree, nc, mode, precision, mapped):
+    torch = cuda
+    orc, gop, _ = make_pair(torch, degree, nc, L=(1.7, 0.6, 0.9), mode=mode, seed=17, precision=precision,
+                            mapped=mapped)
+    x = np.random.default_rng(18).uniform(-1.0, 1.0, orc.n_dofs)
+    ref = orc.vmult(x)
+    dtype = torch.float64 if precision == 0 else torch.float32
+    y = torch.full((orc.n_dofs,), 3.0, dtype=dtype, device="cuda")
+    gop.vmult(y, torch.tensor(x, dtype=dtype, device="cuda"))
+    torch.cuda.synchronize()
+    assert rel_err(y.cpu().numpy(), ref) < (1e-12 if precision == 0 else 1e-5)
+
+
+LASER = (1e3, 0.05, 0.57, 0.77, 1.0, 0.4)  # peak, sigma, x0, y0, vx, t: beam centre near the x+ edge
+
+
+@pytest.mark.parametrize("degree,nc,mode", [(2, (12, 10, 3), "const"), (3, (10, 8, 2), "law"), (1, (24, 20, 2), "qp")])
+def test_create_rhs_with_surface_flux_matches_oracle(cuda, degree, nc, mode):
+    """create_rhs with the Gaussian z+ flux (rhs_boundary_loop, heat_diffuse_operator.cpp:761-768) on
+    top of the cell residual, against the oracle's laser term (pinned by its closed-form moments)."""
+    torch = cuda
+    L = (1.0, 0.8, 0.5)
+    orc0, gop, data = make_pair(torch, degree, nc, L=L, mode=mode, seed=6)
+    kw = dict(c_mass=orc0.s.c_mass, laser=LASER)
+    if mode == "law":
+        kw.update(law=O.LAW_TANH, law_p=(1.0, 0.5, 1.0, 0.1), T=orc0._keep["T"])
+    elif mode == "qp":
+        kw.update(kappa_qp=orc0._keep["k"])
+    orc = O.Operator(degree, nc, L, **kw)
+    gop.set_surface_flux(*LASER)
+    rng = np.random.default_rng(12)
+    T = orc0._keep["T"] if mode == "law" else rng.uniform(0, 1, orc.n_dofs)
+    T_old = T + 0.01 * rng.standard_normal(orc.n_dofs)
+    ref = orc.create_rhs(np.zeros(orc.n_dofs), np.ascontiguousarray(T), T_old)
+    Td = data["T"] if mode == "law" else torch.tensor(T, device="cuda")
+    dst = torch.zeros(orc.n_dofs, dtype=torch.float64, device="cuda")
+    gop.create_rhs(dst, Td, torch.tensor(T_old, device="cuda"))
+    torch.cuda.synchronize()
+    assert rel_err(dst.cpu().numpy(), ref) < 1e-11
+    # the flux alone (T = T_old = 0) against the oracle
+    z = np.zeros(orc.n_dofs)
+    ref0 = orc.create_rhs(np.zeros(orc.n_dofs), z, z.copy()) if mode != "law" else None
+    if ref0 is not None:
+        d0 = torch.zeros_like(dst)
+        zd = torch.zeros_like(dst)
+        gop.create_rhs(d0, zd, zd)
+        torch.cuda.synchronize()
+        assert rel_err(d0.cpu().numpy(), ref0) < 1e-12
+
+
+def test_surface_flux_field_is_the_profile_at_the_nodes(cuda):
+    torch = cuda
+    import meltpooldg_amd as M
+    k, nc, L = 3, (8, 6, 4), (1.0, 0.8, 0.5)
+    op = M.HeatDiffuseMultiPhaseOperator(k, nc, L)
+    op.set_surface_flux(*LASER)
+    out = torch.full((op.n_dofs,), 7.0, dtype=torch.float64, device="cuda")
+    op.surface_flux_field(out)
+    torch.cuda.synchronize()
+    X = O.Operator(k, nc, L).node_coords()
+    peak, sig, x0, y0, vx, t = LASER
+    q = peak * np.exp(-((X[0] - x0 - vx * t) ** 2 + (X[1] - y0) ** 2) / (2 * sig ** 2))
+    q[X[2] < L[2] - 1e-12] = 0.0
+    assert rel_err(out.cpu().numpy(), q) < 1e-14
+
+
+def test_flux_and_robin_on_one_face_are_refused(cuda):
+    """The reference throws when a face has a Neumann flux and radiation/convection
+    (heat_diffuse_operator.cpp:739-743); both orders of setting are refused here."""
+    torch = cuda
+    import meltpooldg_amd as M
+    op = M.HeatDiffuseMultiPhaseOperator(1, (4, 4, 2))
+    op.set_robin(32, alpha=2.0, T_inf_conv=1.0)
+    with pytest.raises(M.MpdgError):
+        op.set_surface_flux(*LASER)
+    op.set_robin(16, alpha=2.0, T_inf_conv=1.0)  # z- only: the flux on z+ is allowed
+    op.set_surface_flux(*LASER)
+    with pytest.raises(M.MpdgError):
+        op.set_robin(32 | 2, alpha=2.0, T_inf_conv=1.0)
+    with pytest.raises(M.MpdgError):
+        op.set_surface_flux(1.0, 0.0, 0.5, 0.5, 0.0, 0.0)  # sigma <= 0
+
+
+@pytest.mark.parametrize("degree,nc,mode,precision", [(3, (8, 32, 16), "qp", 0), (2, (14, 24, 16), "const", 0),
+                                                      (4, (4, 32, 24), "law", 0), (3, (16, 16, 16), "qp", 1)])
+def test_vmult_is_deterministic(cuda, degree, nc, mode, precision):
+    """vmult on meshes of 64 and more columns, repeated: bitwise identical results (SURVEY's
+    run-to-run determinism requirement; shared DoF rows must not depend on the order in which
+    neighbouring columns finish)."""
+    torch = cuda
+    orc, gop, _ = make_pair(torch, degree, nc, mode=mode, seed=23, precision=precision)
+    dtype = torch.float64 if precision == 0 else torch.float32
+    x = torch.tensor(np.random.default_rng(24).uniform(-1.0, 1.0, orc.n_dofs), dtype=dtype, device="cuda")
+    outs = []
+    for _ in range(5):
+        y = torch.full_like(x, 3.0)
+        gop.vmult(y, x)
+        outs.append(y)
+    torch.cuda.synchronize()
+    for y in outs[1:]:
+        assert torch.equal(y, outs[0])

@@ -34,6 +34,8 @@ def test_cg_jacobi_q2_16(cuda):
     (3, (12, 12, 12), "law", 0, 2),
     (2, (8, 8, 8), "cell", 0b000011, 0),
     (3, (8, 8, 8), "qp", 0, -2),  # mapped mesh, coarse Q2
+    (4, (8, 8, 8), "qp", 0, 2),   # the Q4 hierarchy of the Q4 96^3 configuration, coarse Q2
+    (4, (8, 8, 8), "law", 0, 2),
 ])
 def test_gmg_vcycle_fp64_matches_oracle(cuda, degree, nc, mode, dmask, coarse_degree):
     torch = cuda
@@ -57,10 +59,15 @@ def test_gmg_vcycle_fp64_matches_oracle(cuda, degree, nc, mode, dmask, coarse_de
     assert rel_err(out.cpu().numpy(), ref) < 1e-9
 
 
-def test_gmg_vcycle_fp32(cuda):
+@pytest.mark.parametrize("degree,nc,mode,mapped", [(3, (8, 8, 8), "qp", None), (4, (8, 8, 8), "qp", None),
+                                                   (4, (8, 8, 8), "law", None), (3, (8, 8, 8), "const", 0.05),
+                                                   (3, (12, 12, 12), "qp", 0.05)])
+def test_gmg_vcycle_fp32(cuda, degree, nc, mode, mapped):
+    """FP32 level operators, smoothers and transfers (FP64 coarse solve) against the FP64 oracle
+    V-cycle: Cartesian Q3 and Q4 hierarchies and the mapped Q3 one, coarse Q2."""
     torch = cuda
     import meltpooldg_amd as M
-    orc, gop, _ = make_pair(torch, 3, (8, 8, 8), mode="qp", seed=4)
+    orc, gop, _ = make_pair(torch, degree, nc, mode=mode, seed=4, mapped=mapped)
     og = O.GMG(orc, nc_coarse=2, coarse_degree=2)
     pc = M.GMGPreconditioner(gop, precision=M.FP32, n_cells_coarse=2, coarse_degree=2)
     pc.update()
@@ -71,12 +78,17 @@ def test_gmg_vcycle_fp32(cuda):
     assert rel_err(out.cpu().numpy(), og.vcycle(b)) < 1e-4
 
 
-@pytest.mark.parametrize("prec,solver", [(0, 0), (1, 1)])
-def test_cg_gmg_solve(cuda, prec, solver):
-    """CG + GMG reaches 1e-10 relative residual in about the oracle's iteration count."""
+@pytest.mark.parametrize("prec,solver,degree,nc,mode,mapped", [
+    (0, 0, 3, (16, 16, 16), "qp", None), (1, 1, 3, (16, 16, 16), "qp", None),
+    (1, 1, 4, (12, 12, 12), "qp", None), (0, 0, 4, (12, 12, 12), "law", None),   # Q4 (the Q4 96^3 solve)
+    (1, 1, 3, (12, 12, 12), "const", 0.05), (1, 1, 3, (16, 16, 16), "qp", 0.05),  # mapped Q3 (Q3 96^3 mapped)
+])
+def test_cg_gmg_solve(cuda, prec, solver, degree, nc, mode, mapped):
+    """CG + GMG (FP64 levels) and flexible CG + FP32 GMG reach 1e-10 relative residual in about the
+    oracle's FP64 iteration count, with a coarse Q2 level."""
     torch = cuda
     import meltpooldg_amd as M
-    orc, gop, _ = make_pair(torch, 3, (16, 16, 16), mode="qp", seed=2)
+    orc, gop, _ = make_pair(torch, degree, nc, mode=mode, seed=2, mapped=mapped)
     og = O.GMG(orc, nc_coarse=2, coarse_degree=2)
     b = np.random.default_rng(4).uniform(-1, 1, orc.n_dofs)
     _, it_ref, _ = orc.cg(b, rel_tol=1e-10, precond=2, gmg=og, criterion=1)
@@ -197,15 +209,21 @@ def test_extrapolation_predictor(cuda, order):
     assert abs(r0 - expect) <= 1e-6 * expect
 
 
-def test_least_squares_projection_predictor(cuda):
-    """least_squares_projection over the last 4 solutions: the step starts from the combination with the
-    smallest residual ||b - J x|| (numpy's dense least squares on the same vectors), which is no worse than
-    cubic extrapolation through the same solutions."""
+@pytest.mark.parametrize("law", [False, True])
+def test_least_squares_projection_predictor(cuda, law):
+    """The step's own right-hand side projected on the last 4 solutions (PREDICTOR_LSQ_SOLUTION): the
+    step starts from the combination with the smallest residual ||b - J x|| (numpy's dense least squares
+    on the same vectors), which is no worse than cubic extrapolation through the same solutions. With a
+    conductivity law the projection uses the step's operator J(T_n)."""
     torch = cuda
     import meltpooldg_amd as M
     from meltpooldg_amd.application import LaserScan
-    case = LaserScan(2, 8, dt=1e-3, precond="gmg64", predictor=M.PREDICTOR_LSQ, n_old_solution_vectors=4,
-                     n_cells_coarse=2)
+    from tests.helpers import tanh_law_params
+    kw = dict(law=(M.LAW_TANH, tanh_law_params())) if law else {}
+    case = LaserScan(2, 8, dt=1e-3, precond="gmg64", predictor=M.PREDICTOR_LSQ_SOLUTION, n_old_solution_vectors=4,
+                     n_cells_coarse=2, **kw)
+    if law:
+        case.heat.set_initial_condition(torch.full_like(case.T0, 0.95))
     Ts = [case.solution()]
     for _ in range(6):
         case.step()
@@ -245,3 +263,75 @@ def test_preconditioner_update_interval(cuda):
             case.step()
         out.append(case.solution())
     assert float(torch.linalg.norm(out[0] - out[1]) / torch.linalg.norm(out[0])) < 1e-9
+
+
+def _mgs_projection(A, rhs):
+    """numpy restatement of LeastSquaresProjection::vmult (include/meltpooldg/linear_algebra/predictor.hpp:36-77):
+    modified Gram-Schmidt on the images without normalisation, stop at P(s,s) < 1e-16 P(0,0), back
+    substitution."""
+    n_old = A.shape[1]
+    V = [None] * n_old
+    P = np.zeros((n_old, n_old))
+    step = 0
+    while step < n_old:
+        v = A[:, step].copy()
+        P[0, step] = v @ (v if step == 0 else V[0])
+        for j in range(step):
+            v += -P[j, step] / P[j, j] * V[j]
+            P[j + 1, step] = v @ (v if j + 1 == step else V[j + 1])
+        V[step] = v
+        if P[step, step] < 1e-16 * P[0, 0]:
+            break
+        step += 1
+    c = np.array([V[s] @ rhs for s in range(step)])
+    for s in range(step - 1, -1, -1):
+        c[s] = (c[s] - sum(c[j] * P[s, j] for j in range(s + 1, step))) / P[s, s]
+    return c
+
+
+def test_least_squares_projection_reference_mode(cuda):
+    """least_squares_projection as the reference: right-hand side create_rhs at T = T_old = T_n
+    (heat_diffuse_operation.cpp:233-244), images of T_n, T_{n-1}, T_{n-2} orthogonalised by modified
+    Gram-Schmidt (the numpy restatement equals numpy's least squares, which fixes it), the previous
+    solution for the first three steps (predictor.hpp:147-151). The step's initial residual is that of
+    the restated guess."""
+    torch = cuda
+    import meltpooldg_amd as M
+    from meltpooldg_amd.application import LaserScan
+    L = 3
+    case = LaserScan(2, 8, dt=1e-3, precond="gmg64", predictor=M.PREDICTOR_LSQ, n_old_solution_vectors=L,
+                     n_cells_coarse=2)
+
+    def step_rhs_and_images(Tn, hist):
+        b = torch.zeros_like(Tn)
+        case.op.create_rhs(b, torch.zeros_like(Tn), Tn)
+        A = []
+        for T in hist:
+            a = torch.empty_like(T)
+            case.op.vmult(a, T)
+            A.append(a.cpu().numpy())
+        return b.cpu().numpy(), np.stack(A, 1)
+
+    Ts = [case.solution()]
+    for i in range(6):
+        _, _, (_, r0, _) = case.step()
+        if i < L:  # warm-up: the previous solution is the guess
+            bn, A = step_rhs_and_images(Ts[-1], [Ts[-1]])
+            expect = np.linalg.norm(bn - A[:, 0]) / np.linalg.norm(bn)
+            assert abs(r0 - expect) <= 1e-6 * expect
+        Ts.append(case.solution())
+    Tn = Ts[-1]
+    r = torch.zeros_like(Tn)
+    case.op.create_rhs(r, Tn, Tn)
+    hist = Ts[-L:][::-1]
+    bn, A = step_rhs_and_images(Tn, hist)
+    rn = r.cpu().numpy()
+    c = _mgs_projection(A, rn)
+    c_ls, *_ = np.linalg.lstsq(A, rn, rcond=None)
+    assert np.allclose(c, c_ls, rtol=1e-6, atol=1e-9 * np.abs(c_ls).max())
+    guess = sum(ci * T.cpu().numpy() for ci, T in zip(c, hist))
+    Jg = torch.empty_like(Tn)
+    case.op.vmult(Jg, torch.tensor(guess, device="cuda"))
+    expect = np.linalg.norm(bn - Jg.cpu().numpy()) / np.linalg.norm(bn)
+    _, _, (_, r0, _) = case.step()
+    assert abs(r0 - expect) <= 1e-6 * expect

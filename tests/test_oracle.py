@@ -404,3 +404,107 @@ def test_gmres_matches_direct_solve():
             x, it, res = op.gmres(b, rel_tol=1e-12, precond=precond, restart=restart, threads=1)
             assert it > 0 and res <= 1e-12 * np.linalg.norm(b)
             assert np.linalg.norm(x - xs) <= 1e-8 * np.linalg.norm(xs)
+
+
+# ------------------------------------------------- V-cycle composition (dense)
+def _dense(op):
+    n = op.n_dofs
+    return np.array([op.vmult_brute(e) for e in np.eye(n)]).T
+
+
+def _chebyshev_matrix(A, lam_max, deg=3, smoothing_range=20.0):
+    """C with x = C b for Chebyshev smoothing from x = 0: I - C A = T_deg((theta - D^-1 A)/delta) /
+    T_deg(theta/delta), evaluated through the eigen-decomposition of D^-1/2 A D^-1/2."""
+    s = 1.0 / np.sqrt(np.diag(A))
+    w, V = np.linalg.eigh(s[:, None] * A * s[None, :])
+    lmin = lam_max / smoothing_range
+    theta, delta = 0.5 * (lam_max + lmin), 0.5 * (lam_max - lmin)
+
+    def T(x):
+        return np.cos(deg * np.arccos(x)) if abs(x) <= 1 else np.cosh(deg * np.arccosh(abs(x))) * np.sign(x) ** deg
+    f = np.array([T((theta - x) / delta) / T(theta / delta) for x in w])
+    E = (s[:, None] * ((V * f) @ V.T)) / s[None, :]
+    return (np.eye(len(s)) - E) @ np.linalg.inv(A)
+
+
+def _n_nodes(k, nc):
+    return (k * nc[0] + 1) * (k * nc[1] + 1) * (k * nc[2] + 1)
+
+
+@pytest.mark.parametrize("k,nc,coarse_degree", [(1, (8, 8, 8), 0), (2, (4, 4, 4), 0), (3, (4, 4, 4), 2)])
+def test_vcycle_equals_dense_multigrid_composition(k, nc, coarse_degree):
+    """The oracle's V-cycle equals, to rounding, the recursive two-grid composition built from
+    assembled matrices: x = C b; x += P B_c P^T (b - A x); x += C (b - A x) with C the Chebyshev
+    smoother (closed form), P the prolongation assembled column by column, and B_c the next
+    coarser level's V-cycle (the exact inverse on the coarsest level). Dropping the
+    post-smoother changes the result by 15-30 %; the smoothers' lambda_max is the oracle's
+    20-step estimate, which is checked against the dense eigenvalue here too (a Ritz value within
+    2 % below it)."""
+    c_mass, kap = 10.0 if k < 3 else 50.0, 1.3
+    op = O.Operator(k, nc, c_mass=c_mass, kappa=kap)
+    g = O.GMG(op, nc_coarse=2, coarse_degree=coarse_degree, cheb_degree=3, smoothing_range=20.0, eig_iters=20)
+    levels, prev = [], None
+    for deg, ncl, lam in g.levels():
+        A = _dense(O.Operator(deg, ncl, c_mass=c_mass, kappa=kap))
+        P = None
+        if prev is not None:
+            n_c = _n_nodes(*prev)
+            P = np.array([O.prolongate(prev[0], prev[1], deg, ncl, e) for e in np.eye(n_c)]).T
+            exact = np.linalg.eigvals(A / np.diag(A)[:, None]).real.max()
+            # a Ritz value: a lower bound, close to the largest eigenvalue after 20 steps
+            assert (1 - 2e-2) * exact <= lam / 1.2 <= (1 + 1e-12) * exact
+        levels.append((A, lam, P))
+        prev = (deg, ncl)
+
+    def vcycle(l, b, post=True):
+        A, lam, P = levels[l]
+        if l == 0:
+            return np.linalg.solve(A, b)
+        C = _chebyshev_matrix(A, lam)
+        x = C @ b
+        x = x + P @ vcycle(l - 1, P.T @ (b - A @ x), post)
+        return x + C @ (b - A @ x) if post else x
+
+    b = np.random.default_rng(3).uniform(-1, 1, op.n_dofs)
+    ref = vcycle(len(levels) - 1, b)
+    got = g.vcycle(b)
+    assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+    # the check is sensitive to the post-smoother (a plausible slip)
+    assert np.linalg.norm(vcycle(len(levels) - 1, b, post=False) - ref) > 0.05 * np.linalg.norm(ref)
+
+
+# ----------------------------------------------------- laser surface flux
+def _gauss_integrals(c, sig, L):
+    """int_0^L g dx and int_0^L x g dx for g = exp(-(x-c)^2/(2 sig^2))."""
+    from math import erf, exp, pi, sqrt
+    s2 = sig * sqrt(2.0)
+    i0 = sig * sqrt(pi / 2.0) * (erf((L - c) / s2) - erf(-c / s2))
+    i1 = c * i0 + sig ** 2 * (exp(-c * c / (2 * sig ** 2)) - exp(-(L - c) ** 2 / (2 * sig ** 2)))
+    return i0, i1
+
+
+@pytest.mark.parametrize("k,nc", [(1, (64, 64, 1)), (2, (40, 30, 2)), (3, (32, 32, 1)), (4, (24, 24, 1))])
+def test_laser_flux_moments_closed_form(k, nc):
+    """Neumann flux of the Gauss profile on z+ (heat_diffuse_operator.cpp:761-768,
+    laser_intensity_profiles.hpp:134-139 with radius r = 2 sigma, peak = 2P/(pi r^2)):
+    with T = T_old = 0 only the flux is left in create_rhs, and since the Q_k basis reproduces
+    1, x and y, the sums of rhs_i, x_i rhs_i and y_i rhs_i are the integrals of q, x q and y q
+    over the top face, which have erf closed forms. The beam sits near the x+ edge, so a wrong
+    width (sqrt 2 sigma) or a wrong direction of travel (x0 - v t) changes the integral by
+    38-68 %; the tolerance is set by the face quadrature's error on the resolved Gaussian."""
+    L = (1.0, 0.8, 0.5)
+    peak, sig, x0, y0, vx, t = 1e3, 0.05, 0.57, 0.77, 1.0, 0.4
+    op = O.Operator(k, nc, L, c_mass=1e3, kappa=2.0, laser=(peak, sig, x0, y0, vx, t))
+    z = np.zeros(op.n_dofs)
+    r = op.create_rhs(np.zeros(op.n_dofs), z, z.copy())
+    X = op.node_coords()
+    ix0, ix1 = _gauss_integrals(x0 + vx * t, sig, L[0])
+    iy0, iy1 = _gauss_integrals(y0, sig, L[1])
+    tol = {1: 1e-5, 2: 1e-6, 3: 1e-8, 4: 1e-9}[k]
+    for got, exp in [(r.sum(), peak * ix0 * iy0), (X[0] @ r, peak * ix1 * iy0), (X[1] @ r, peak * ix0 * iy1)]:
+        assert abs(got - exp) <= tol * abs(exp)
+    # r vanishes off the top face
+    assert np.abs(r[X[2] < L[2] - 1e-12]).max() == 0.0
+    wrong_sigma = _gauss_integrals(x0 + vx * t, sig * np.sqrt(2), L[0])[0] * _gauss_integrals(y0, sig * np.sqrt(2), L[1])[0]
+    wrong_dir = _gauss_integrals(x0 - vx * t, sig, L[0])[0] * iy0
+    assert abs(peak * wrong_sigma / r.sum() - 1) > 0.3 and abs(peak * wrong_dir / r.sum() - 1) > 0.3
