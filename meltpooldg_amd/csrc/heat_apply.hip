@@ -20,6 +20,11 @@
 //     neighbouring columns) are added with global atomics, and the x-face
 //     between consecutive chunks stays in LDS and is carried to the next chunk,
 //     so no DoF row is split across x. A pre-pass zeroes the shared face rows.
+//   * Determinism: a face row gets at most two atomic adds onto zero, and
+//     0 + a + b does not depend on the order. Where four columns meet, the
+//     upper z-layer of columns adds into a separate edge buffer instead, which
+//     a small follow-up kernel adds into dst (and clears), so vmult is
+//     bitwise reproducible.
 #include "kernels.hpp"
 
 namespace mpdg {
@@ -509,7 +514,15 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
                 ((P.dmask & 4) && iy == 0) || ((P.dmask & 8) && iy == P.nd[1] - 1) ||
                 ((P.dmask & 16) && iz == 0) || ((P.dmask & 32) && iz == P.nd[2] - 1);
         if (face) {
-          if (!con) atomicAdd(dst + g, val);
+          // a row where four columns meet gets two contributions from each z-layer of columns; the
+          // upper layer's go to the edge buffer (summed in by edge_fixup_kernel), so every row
+          // receives at most two atomic adds onto zero, whose result does not depend on their order
+          if (!con) {
+            Number *tgt = dst + g;
+            if (z == 0 && col_z > 0 && ((y == 0 && col_y > 0) || (y == DY - 1 && col_y + 1 < P.ncol[0])))
+              tgt = P.edge + ((long)(col_z - 1) * (P.ncol[0] - 1) + (y == 0 ? col_y : col_y + 1) - 1) * P.nd[0] + ix;
+            atomicAdd(tgt, val);
+          }
         } else {
           dst[g] = con ? Number(0) : val;
         }
@@ -538,6 +551,41 @@ __global__ __launch_bounds__(128) void zero_face_rows_kernel(Number *__restrict_
   for (int ix = threadIdx.x; ix < nd0; ix += 128) row[ix] = 0;
 }
 
+// dst += the upper layer's contributions to the rows where four columns meet (interior y-face
+// plane fy and z-face plane fz of the column decomposition), and the edge buffer back to zero
+template <typename Number>
+__global__ __launch_bounds__(256) void edge_fixup_kernel(Number *__restrict__ dst, Number *__restrict__ edge, int nd0,
+                                                        long sy, long sz, int py, int pz, int nly, long n) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const long line = i / nd0;
+  const int ix = (int)(i - line * nd0);
+  const int fy = (int)(line % nly) + 1, fz = (int)(line / nly) + 1;
+  const Number v = edge[i];
+  edge[i] = Number(0);
+  dst[ix + sy * ((long)fy * py) + sz * ((long)fz * pz)] += v;
+}
+
+template <int N1, typename Number, bool MAPPED>
+size_t edge_elems_shape(const int *nc) {
+  using G = ApplyGeom<N1, Number, MAPPED>;
+  const long ny = (nc[1] + G::BY - 1) / G::BY, nz = (nc[2] + G::BZ - 1) / G::BZ;
+  return (size_t)((ny - 1) * (nz - 1) * ((long)(N1 - 1) * nc[0] + 1));
+}
+template <int N1, typename Number>
+size_t edge_elems_n1(const int *nc) {
+  return std::max(edge_elems_shape<N1, Number, false>(nc), edge_elems_shape<N1, Number, true>(nc));
+}
+size_t apply_edge_elems(int degree, const int *nc, bool fp32) {
+  switch (degree) {
+    case 1: return fp32 ? edge_elems_n1<2, float>(nc) : edge_elems_n1<2, double>(nc);
+    case 2: return fp32 ? edge_elems_n1<3, float>(nc) : edge_elems_n1<3, double>(nc);
+    case 3: return fp32 ? edge_elems_n1<4, float>(nc) : edge_elems_n1<4, double>(nc);
+    case 4: return fp32 ? edge_elems_n1<5, float>(nc) : edge_elems_n1<5, double>(nc);
+    default: return 0;
+  }
+}
+
 template <int N1, typename Number, bool MAPPED>
 int launch_apply_shape(const OperatorData<Number> &op, const Number *src, Number *dst, hipStream_t stream,
                        hipEvent_t kernel_start) {
@@ -548,6 +596,8 @@ int launch_apply_shape(const OperatorData<Number> &op, const Number *src, Number
   A.ncol[1] = (op.nc[2] + G::BZ - 1) / G::BZ;
   const int ncols = A.ncol[0] * A.ncol[1];
   A.xcd_cols = (ncols % 8 == 0 && ncols >= 64) ? ncols / 8 : 0;
+  const long n_edge = (long)(A.ncol[0] - 1) * (A.ncol[1] - 1) * op.nd[0];
+  if (n_edge > 0 && (op.edge == nullptr || (size_t)n_edge > op.edge_elems)) return MPDG_ERR_INVALID;
   {
     // face planes j*p for j = 0 .. ceil((nd-1)/p)-1, then the last plane nd-1
     const int py = (N1 - 1) * G::BY, pz = (N1 - 1) * G::BZ;
@@ -556,7 +606,7 @@ int launch_apply_shape(const OperatorData<Number> &op, const Number *src, Number
     hipLaunchKernelGGL(zero_face_rows_kernel<Number>, dim3((unsigned)nrows), dim3(128), 0, stream, dst,
                        op.nd[0], op.nd[1], op.nd[2], py, pz, nfy, nfz);
   }
-  if (kernel_start) hipEventRecord(kernel_start, stream);
+  if (kernel_start) (void)hipEventRecord(kernel_start, stream);
   if constexpr (MAPPED) {
     switch (op.kappa_mode) {
       case KAPPA_CONST: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CONST, true>), dim3(ncols), dim3(G::NT), 0, stream, A, src, dst); break;
@@ -572,6 +622,10 @@ int launch_apply_shape(const OperatorData<Number> &op, const Number *src, Number
       default: return MPDG_ERR_INVALID;
     }
   }
+  if (n_edge > 0)
+    hipLaunchKernelGGL(edge_fixup_kernel<Number>, dim3((unsigned)((n_edge + 255) / 256)), dim3(256), 0, stream, dst,
+                       op.edge, op.nd[0], (long)op.nd[0], (long)op.nd[0] * op.nd[1], (N1 - 1) * G::BY,
+                       (N1 - 1) * G::BZ, A.ncol[0] - 1, n_edge);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 

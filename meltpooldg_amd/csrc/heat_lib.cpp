@@ -20,8 +20,23 @@ inline int dev_ok() { return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ER
 // ===========================================================================
 // operator
 // ===========================================================================
+int mpdg_heat_op::ensure_edge() const {
+  const size_t n = apply_edge_elems(d64.degree, d64.nc, precision == MPDG_FP32);
+  if (precision == MPDG_FP64) {
+    MPDG_TRY(edge64.resize_zeroed(n));
+    d64.edge = edge64.get();
+    d64.edge_elems = n;
+  } else {
+    MPDG_TRY(edge32.resize_zeroed(n));
+    d32.edge = edge32.get();
+    d32.edge_elems = n;
+  }
+  return MPDG_OK;
+}
+
 int mpdg_heat_op::vmult_cells(double *dst, const double *src, hipStream_t s) const {
   if (precision != MPDG_FP64) return MPDG_ERR_INVALID;
+  MPDG_TRY(ensure_edge());
   if (d64.metric && d64.kappa_mode == KAPPA_LAW) MPDG_TRY(refresh_metric_k(s));  // k(T) with the current T
   if (tangent) return td_bound ? launch_tangent_apply(d64, td.get(), vel, solid, src, dst, s) : MPDG_ERR_INVALID;
   if (!timing_on) return launch_apply<double>(d64, src, dst, s);
@@ -68,6 +83,7 @@ int mpdg_heat_op::vmult(double *dst, const double *src, hipStream_t s) const {
 
 int mpdg_heat_op::vmult_f32(float *dst, const float *src, hipStream_t s) const {
   if (precision != MPDG_FP32 || tangent) return MPDG_ERR_INVALID;
+  MPDG_TRY(ensure_edge());
   if (d32.metric && d32.kappa_mode == KAPPA_LAW) MPDG_TRY(refresh_metric_k(s));
   return launch_apply<float>(d32, src, dst, s);
 }
@@ -110,6 +126,7 @@ int mpdg_heat_op::create_rhs(double *dst, const double *T, const double *T_old, 
   const long n = d64.n_dofs;
   MPDG_TRY(tmp1.resize(n));
   MPDG_TRY(tmp2.resize(n));
+  MPDG_TRY(ensure_edge());
   if (T_zero && !tangent) {
     // dst += c_m M (T_old + q / c_m) + boundary terms at T = 0 (K T vanishes)
     OperatorData<double> mass = d64;
@@ -165,6 +182,7 @@ int mpdg_heat_op::l2_norm(const double *u, double *norm, hipStream_t s) const {
   MPDG_TRY(tmp1.resize(n));
   MPDG_TRY(scal.resize(4));
   MPDG_TRY(red.init());
+  MPDG_TRY(ensure_edge());
   OperatorData<double> mass = d64;
   mass.c_mass = 1.0;
   mass.kappa_mode = KAPPA_CONST;
@@ -326,6 +344,7 @@ struct GMGPrecond : mpdg_precond {
   mpdg_gmg_params prm{};
   std::vector<std::unique_ptr<Level<T>>> lev;  // 0 = coarsest
   DeviceArray<double> coarse_inv, cvec_in, cvec_out, cvec_part, eig_sc, eig_hist, kcell64, kcell_pp[2];
+  DeviceArray<T> edge;  // apply scratch shared by the levels (one stream, zero between applies)
   ReductionScratch red;
   int nc_coarse_dofs = 0;
 
@@ -405,6 +424,13 @@ struct GMGPrecond : mpdg_precond {
       t2 = (long)L.od.nd[0] * L.od.nd[1] * C.od.nd[2];
       MPDG_TRY(L.t1.resize(t1));
       MPDG_TRY(L.t2.resize(t2));
+    }
+    size_t ne = 0;
+    for (auto &L : lev) ne = std::max(ne, apply_edge_elems(L->od.degree, L->od.nc, std::is_same<T, float>::value));
+    MPDG_TRY(edge.resize_zeroed(ne));
+    for (auto &L : lev) {
+      L->od.edge = edge.get();
+      L->od.edge_elems = ne;
     }
     if (op->coords) MPDG_TRY(build_geometry());
     MPDG_TRY(red.init());

@@ -22,6 +22,12 @@ int launch_coarse_matrix(const OperatorData<T> &od, double c_mass, const double 
                          hipStream_t s);
 int launch_gauss_jordan_inverse(double *A, int n, hipStream_t s);
 
+#define MPDG_TRY_INNER(...)         \
+  do {                              \
+    const int rc_i__ = (__VA_ARGS__); \
+    if (rc_i__ != MPDG_OK) return rc_i__; \
+  } while (0)
+
 // device buffer owned by the library
 template <typename T>
 struct DeviceArray {
@@ -45,6 +51,13 @@ struct DeviceArray {
       return MPDG_ERR_NOMEM;
     }
     n = count;
+    return MPDG_OK;
+  }
+  // resize and zero; keeps the contents when the size is unchanged
+  int resize_zeroed(size_t count) {
+    if (count == n && ptr) return MPDG_OK;
+    MPDG_TRY_INNER(resize(count));
+    if (count && hipMemset(ptr, 0, count * sizeof(T)) != hipSuccess) return MPDG_ERR_DEVICE;
     return MPDG_OK;
   }
   T *get() const { return ptr; }
@@ -136,6 +149,9 @@ struct mpdg_heat_op {
   mutable mpdg::DeviceArray<float> metric32_k, kq32;
   const double *coords = nullptr;
   mutable mpdg::ReductionScratch red;
+  // apply scratch for the rows where four columns meet (OperatorData::edge), zero between applies
+  mutable mpdg::DeviceArray<double> edge64;
+  mutable mpdg::DeviceArray<float> edge32;
   // full Newton tangent (mpdg_heat_op_set_tangent): material, velocity, per-qp data at the bound T
   bool tangent = false;
   mpdg_heat_material mat{};
@@ -156,6 +172,8 @@ struct mpdg_heat_op {
 
   // mapped meshes with per-qp / k(T) conductivity: fold the conductivity into metric_k
   int refresh_metric_k(hipStream_t s) const;
+  // allocates (zeroed) and binds the apply's edge scratch on first use
+  int ensure_edge() const;
   long n_dofs() const { return d64.n_dofs; }
   long n_cells() const { return d64.n_cells; }
   int vmult(double *dst, const double *src, hipStream_t s) const;

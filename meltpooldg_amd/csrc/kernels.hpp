@@ -39,6 +39,11 @@ struct OperatorData {
   // metric with the conductivity folded in (kappa_q G), used for per-qp / k(T) conductivity on mapped meshes
   const Number *metric_k = nullptr;
   const double *coords = nullptr;   // mapping node coordinates (X block, Y block, Z block), FP64
+  // apply scratch: contributions of the upper z-layer of columns to the DoF rows where four
+  // columns meet, summed into dst after the cell kernel so that vmult is bitwise reproducible
+  // (apply_edge_elems Number entries, all zero between applies)
+  Number *edge = nullptr;
+  size_t edge_elems = 0;
   void set_mesh(int k, const int *ncells, const double *ext) {
     degree = k;
     for (int d = 0; d < 3; ++d) {
@@ -83,6 +88,7 @@ struct ApplyArgs {
   const Number *kappa_arr;
   const Number *T;
   const Number *metric;
+  Number *edge;      // rows shared by four columns: the upper layer's contributions
 };
 
 template <typename Number, int N1>
@@ -107,6 +113,7 @@ void fill_apply_args(const OperatorData<Number> &op, ApplyArgs<Number, N1> &A) {
   for (int i = 0; i < 4; ++i) A.law_p[i] = (Number)op.law_p[i];
   A.xcd_cols = 0;
   A.ncol[0] = A.ncol[1] = 1;
+  A.edge = op.edge;
 }
 
 // heat_apply.hip; `kernel_start`, when given, is recorded on the stream right before the
@@ -114,6 +121,9 @@ void fill_apply_args(const OperatorData<Number> &op, ApplyArgs<Number, N1> &A) {
 template <typename Number>
 int launch_apply(const OperatorData<Number> &op, const Number *src, Number *dst, hipStream_t stream,
                  hipEvent_t kernel_start = nullptr);
+// entries of OperatorData::edge the apply of a mesh of degree k and n_cells cells needs (the
+// larger of its Cartesian and mapped chunk shapes); 0 when no DoF row is shared by four columns
+size_t apply_edge_elems(int degree, const int *n_cells, bool fp32);
 
 // heat_aux.hip: diagonal, boundary faces, residual, norms, conductivity
 template <typename Number>

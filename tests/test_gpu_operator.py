@@ -310,21 +310,26 @@ def test_flux_and_robin_on_one_face_are_refused(cuda):
         op.set_surface_flux(1.0, 0.0, 0.5, 0.5, 0.0, 0.0)  # sigma <= 0
 
 
-@pytest.mark.parametrize("degree,nc,mode,precision", [(3, (8, 32, 16), "qp", 0), (2, (14, 24, 16), "const", 0),
-                                                      (4, (4, 32, 24), "law", 0), (3, (16, 16, 16), "qp", 1)])
-def test_vmult_is_deterministic(cuda, degree, nc, mode, precision):
-    """vmult on meshes of 64 and more columns, repeated: bitwise identical results (SURVEY's
-    run-to-run determinism requirement; shared DoF rows must not depend on the order in which
-    neighbouring columns finish)."""
+@pytest.mark.parametrize("degree,nc,mode,precision,mapped,dmask", [
+    (3, (8, 32, 16), "qp", 0, None, 0), (2, (14, 24, 16), "const", 0, None, 0), (4, (4, 32, 24), "law", 0, None, 0),
+    (3, (16, 16, 16), "qp", 1, None, 0), (3, (12, 18, 10), "const", 0, 0.05, 0), (3, (9, 13, 11), "qp", 0, None, 0b110101),
+    (2, (10, 21, 9), "cell", 1, 0.05, 0)])
+def test_vmult_is_deterministic(cuda, degree, nc, mode, precision, mapped, dmask):
+    """vmult on meshes with many columns in y and z, repeated: bitwise identical results, and equal
+    to the oracle. DoF rows shared by neighbouring columns must not depend on the order in which
+    the columns finish; rows where four columns meet take the edge buffer and its fix-up pass."""
     torch = cuda
-    orc, gop, _ = make_pair(torch, degree, nc, mode=mode, seed=23, precision=precision)
+    orc, gop, _ = make_pair(torch, degree, nc, mode=mode, seed=23, precision=precision, mapped=mapped,
+                            dirichlet=dmask)
     dtype = torch.float64 if precision == 0 else torch.float32
-    x = torch.tensor(np.random.default_rng(24).uniform(-1.0, 1.0, orc.n_dofs), dtype=dtype, device="cuda")
+    x = np.random.default_rng(24).uniform(-1.0, 1.0, orc.n_dofs)
+    xd = torch.tensor(x, dtype=dtype, device="cuda")
     outs = []
     for _ in range(5):
-        y = torch.full_like(x, 3.0)
-        gop.vmult(y, x)
+        y = torch.full_like(xd, 3.0)
+        gop.vmult(y, xd)
         outs.append(y)
     torch.cuda.synchronize()
     for y in outs[1:]:
         assert torch.equal(y, outs[0])
+    assert rel_err(outs[0].cpu().numpy(), orc.vmult(x)) < (1e-12 if precision == 0 else 1e-5)

@@ -321,6 +321,8 @@ def test_least_squares_projection_reference_mode(cuda):
             assert abs(r0 - expect) <= 1e-6 * expect
         Ts.append(case.solution())
     Tn = Ts[-1]
+    # the laser at the next step's time, where the step evaluates the predictor's right-hand side
+    case.set_source(case.t + case.dt)
     r = torch.zeros_like(Tn)
     case.op.create_rhs(r, Tn, Tn)
     hist = Ts[-L:][::-1]
