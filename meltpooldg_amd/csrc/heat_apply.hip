@@ -39,18 +39,24 @@ namespace mpdg {
 // faster than 8x4x2 for every conductivity mode, 8x2x4 equal, 4x4x4 9 % slower. Q2: 14x3x2 beats
 // 7x4x3 by 5 % (FP64) and 25 % (FP32). Q4: FP64 keeps 4x4x3 (the longer shapes are 1-4 % slower),
 // FP32 gains 11 % with 12x2x2.
+// WX x WY x WZ: how the chunk is split into the NW wavefronts' boxes of cells (each box holds
+// the 64 / N1 cells of one wavefront and has its own sub-brick of DoF sums, see ApplyGeom).
 template <int N1, typename Number> struct ChunkShape;
-template <typename Number> struct ChunkShape<2, Number> { static constexpr int BX = 8, BY = 4, BZ = 4, NW = 4; };
-template <typename Number> struct ChunkShape<3, Number> { static constexpr int BX = 14, BY = 3, BZ = 2, NW = 4; };
-template <> struct ChunkShape<4, double> { static constexpr int BX = 8, BY = 4, BZ = 2, NW = 4; };
-template <> struct ChunkShape<4, float> { static constexpr int BX = 16, BY = 2, BZ = 2, NW = 4; };
-template <> struct ChunkShape<5, double> { static constexpr int BX = 4, BY = 4, BZ = 3, NW = 4; };
-template <> struct ChunkShape<5, float> { static constexpr int BX = 12, BY = 2, BZ = 2, NW = 4; };
+template <typename Number> struct ChunkShape<2, Number> {
+  static constexpr int BX = 8, BY = 4, BZ = 4, NW = 4, WX = 1, WY = 1, WZ = 4;
+};
+template <typename Number> struct ChunkShape<3, Number> {
+  static constexpr int BX = 14, BY = 3, BZ = 2, NW = 4, WX = 2, WY = 1, WZ = 2;
+};
+template <> struct ChunkShape<4, double> { static constexpr int BX = 8, BY = 4, BZ = 2, NW = 4, WX = 1, WY = 2, WZ = 2; };
+template <> struct ChunkShape<4, float> { static constexpr int BX = 16, BY = 2, BZ = 2, NW = 4, WX = 1, WY = 2, WZ = 2; };
+template <> struct ChunkShape<5, double> { static constexpr int BX = 4, BY = 4, BZ = 3, NW = 4, WX = 1, WY = 4, WZ = 1; };
+template <> struct ChunkShape<5, float> { static constexpr int BX = 12, BY = 2, BZ = 2, NW = 4, WX = 1, WY = 2, WZ = 2; };
 // mapped cells (metric per point): FP64 Q3 runs 8 % faster with 16x2x2 than with the Cartesian
 // 8x4x2 (Q3 96^3, 1.02 vs 1.11 ms), where 8x4x2 stays 2 % ahead
 template <int N1, typename Number, bool MAPPED> struct ChunkShapeOf { using type = ChunkShape<N1, Number>; };
 template <> struct ChunkShapeOf<4, double, true> {
-  struct type { static constexpr int BX = 16, BY = 2, BZ = 2, NW = 4; };
+  struct type { static constexpr int BX = 16, BY = 2, BZ = 2, NW = 4, WX = 1, WY = 2, WZ = 2; };
 };
 
 template <int N1, typename Number = double, bool MAPPED = false> struct ApplyGeom {
@@ -64,14 +70,26 @@ template <int N1, typename Number = double, bool MAPPED = false> struct ApplyGeo
   static_assert(BX * BY * BZ == CPB, "chunk shape must match the cells per workgroup");
   static constexpr int SQY = N1 * N1 + 1;             // padded qy stride of a cell's transpose slot
   static constexpr int CS = N1 * SQY;                 // transpose slot per cell
-  static constexpr int AX = K * BX + 1, AY = K * BY + 1, AZ = K * BZ + 1;
-  static constexpr int ACC = AX * AY * AZ;            // DoF brick of a chunk
+  static constexpr int AX = K * BX + 1, AY = K * BY + 1, AZ = K * BZ + 1;  // DoF brick of a chunk
+  // a wavefront's box of cells and its sub-brick of DoFs; neighbouring sub-bricks overlap in
+  // one plane, which the write-out sums in a fixed order
+  static constexpr int WX = Shape::WX, WY = Shape::WY, WZ = Shape::WZ;
+  static_assert(WX * WY * WZ == NW && BX % WX == 0 && BY % WY == 0 && BZ % WZ == 0, "wavefront grid");
+  static constexpr int EX = BX / WX, EY = BY / WY, EZ = BZ / WZ;
+  static_assert(EX * EY * EZ == CPW, "a wavefront's box holds its cells");
+  static constexpr int SX = K * EX + 1, SY = K * EY + 1, SZ = K * EZ + 1;
+  static constexpr int SUB = SX * SY * SZ;
+  static constexpr int ACC = NW * SUB;                // the sub-bricks of all wavefronts
   static constexpr int TBUF = NW * CPW * CS;          // transpose buffers of the wavefronts
-  // the DoF brick is summed in FP64 at every precision (LDS FP64 adds are fast on CDNA4,
-  // FP32 LDS atomics are not) and has its own LDS, apart from the transpose buffers, so that
+  // the DoF sums are kept in FP64 at every precision (LDS FP64 adds are fast on CDNA4, FP32
+  // LDS atomics are not) and have their own LDS, apart from the transpose buffers, so that
   // only the scatter needs workgroup barriers
   static constexpr int TBUF_DOUBLES = (TBUF * (int)sizeof(Number) + 7) / 8;
   static constexpr int CARRY = AY * AZ;
+  static constexpr int ROWS = AY * AZ;                 // x rows of the DoF brick
+  // LDS: transpose slots, sub-bricks, two carry planes, then per brick row its global offset
+  // (8 B) and its write-out code (4 B)
+  static constexpr int LDS_BYTES = 8 * (TBUF_DOUBLES + ACC + 2 * CARRY + ROWS) + 4 * ROWS;
   // x rows of the DoF brick written per pass of the write-out loop (lane = x in the row)
   static constexpr int ROWS_PER_PASS = NT / AX;
 };
@@ -181,6 +199,38 @@ __device__ __forceinline__ void transpose_yz(Number *slot, int s, bool on, const
   }
 }
 
+// write-out code of a brick row: low bits the row's y/z part of its sub-brick entry index
+// (of the last box that holds it), then flags
+constexpr int WO_PART = (1 << 16) - 1;
+constexpr int WO_Y2 = 1 << 16;    // on a plane two boxes share in y (the other entry is one box back)
+constexpr int WO_Z2 = 1 << 17;    // same in z
+constexpr int WO_FACE = 1 << 18;  // on the column's y/z faces: atomic add
+constexpr int WO_EDGE = 1 << 19;  // where four columns meet, upper layer: into the edge buffer
+constexpr int WO_CON = 1 << 20;   // on a Dirichlet y/z plane
+constexpr int WO_SKIP = 1 << 21;  // beyond the mesh (ragged column)
+
+// The wavefront boxes along one direction: W boxes of E DoF intervals each (E = k times the
+// cells of a box). Position p of the brick lies in box p / E at offset p % E, and a position
+// on a shared plane (offset 0, not the first box) also in the previous box at offset E.
+// Writes box * BOX_STRIDE + offset * STRIDE for each (lower box first) and returns their
+// number (1 or 2; always 1 when W = 1).
+template <int W, int E, int BOX_STRIDE, int STRIDE>
+__device__ __forceinline__ int brick_parts(int p, int (&part)[2]) {
+  if constexpr (W == 1) {
+    part[0] = p * STRIDE;
+    return 1;
+  } else {
+    const int b = min(p / E, W - 1), o = p - b * E;
+    if (o == 0 && b > 0) {
+      part[0] = (b - 1) * BOX_STRIDE + E * STRIDE;
+      part[1] = b * BOX_STRIDE;
+      return 2;
+    }
+    part[0] = b * BOX_STRIDE + o * STRIDE;
+    return 1;
+  }
+}
+
 template <typename Number>
 __device__ __forceinline__ Number law_kappa(int law, const Number *p, Number T) {
   if (law == LAW_TANH) {
@@ -275,17 +325,29 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
                                                          Number *__restrict__ dst) {
   using G = ApplyGeom<N1, Number, MAPPED>;
   constexpr int K = G::K;
-  __shared__ double lds_raw[G::TBUF_DOUBLES];
-  __shared__ double acc[G::ACC];     // DoF brick of the chunk
-  __shared__ double carry[G::CARRY]; // its last x plane, carried to the next chunk
+  // LDS (sized at launch, G::LDS_BYTES): the transpose slots, the wavefronts' sub-bricks and
+  // the last x plane of the chunk, carried to the next one (two buffers by chunk parity, since
+  // the write-out reads the previous chunk's carry while it writes its own). Sized at launch
+  // rather than declared: with the size known, the compiler derives the occupancy from the
+  // LDS and schedules for it (at Q3 FP64 it then spills 29 VGPRs; this way it needs 230).
+  extern __shared__ double dyn_lds[];
+  double *const lds_raw = dyn_lds;
+  double *const acc = dyn_lds + G::TBUF_DOUBLES;
+  double *const carry = acc + G::ACC;
+  long *const row_off = reinterpret_cast<long *>(carry + 2 * G::CARRY);
+  int *const row_code = reinterpret_cast<int *>(row_off + G::ROWS);
   Number *const lds = reinterpret_cast<Number *>(lds_raw);  // transpose slots
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int cw = lane / N1, l = lane - cw * N1;
-  const bool lane_on = cw < G::CPW;
-  const int lc = wave * G::CPW + cw;
-  const int lx = lc % G::BX, ly = (lc / G::BX) % G::BY, lz = lc / (G::BX * G::BY);
+  const bool lane_on = (64 % N1 == 0) || cw < G::CPW;  // every lane holds a cell plane when N1 divides 64
+  // the wavefront's box in the chunk, the cell's place in the box (cells beyond 64 / N1 lanes
+  // are off)
+  const int bx = wave % G::WX, by = (wave / G::WX) % G::WY, bz = wave / (G::WX * G::WY);
+  const int ex = cw % G::EX, ey = (cw / G::EX) % G::EY, ez = cw / (G::EX * G::EY);
+  const int lx = bx * G::EX + ex, ly = by * G::EY + ey, lz = bz * G::EZ + (lane_on ? ez : 0);
+  double *const sub = acc + wave * G::SUB;  // this wavefront's sub-brick
   Number *slot = lds + (wave * G::CPW + (lane_on ? cw : 0)) * G::CS;
 
   // column of this workgroup (consecutive XCDs get consecutive column ranges)
@@ -300,7 +362,40 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
   const int nchunks = (P.nc[0] + G::BX - 1) / G::BX;
   // the brick starts at zero; the write-out of every chunk zeroes what it reads
   for (int idx = threadIdx.x; idx < G::ACC; idx += G::NT) acc[idx] = 0.0;
-  for (int idx = threadIdx.x; idx < G::CARRY; idx += G::NT) carry[idx] = 0.0;
+  for (int idx = threadIdx.x; idx < 2 * G::CARRY; idx += G::NT) carry[idx] = 0.0;
+  // what the write-out needs of each x row (y, z) of the brick, the same for every chunk of
+  // the column: its global offset (in dst, or in the edge buffer for the upper layer's rows
+  // where four columns meet) and a code: y/z part of its sub-brick entry, whether it lies on a
+  // plane two boxes share in y / z, on the column's faces, on a Dirichlet y/z plane, beyond the
+  // mesh
+  for (int r = threadIdx.x; r < G::ROWS; r += G::NT) {
+    const int y = r % G::AY, z = r / G::AY;
+    int code = WO_SKIP;
+    long off = 0;
+    if (y < DY && z < DZ) {
+      int ye[2], ze[2];
+      const int nye = brick_parts<G::WY, K * G::EY, G::WX * G::SUB, G::SX>(y, ye);
+      const int nze = brick_parts<G::WZ, K * G::EZ, G::WX * G::WY * G::SUB, G::SX * G::SY>(z, ze);
+      code = ye[nye - 1] + ze[nze - 1];
+      if (nye == 2) code |= WO_Y2;
+      if (nze == 2) code |= WO_Z2;
+      const int iy = Y0 + y, iz = Z0 + z;
+      if (y == 0 || y == DY - 1 || z == 0 || z == DZ - 1) code |= WO_FACE;
+      if (((P.dmask & 4) && iy == 0) || ((P.dmask & 8) && iy == P.nd[1] - 1) || ((P.dmask & 16) && iz == 0) ||
+          ((P.dmask & 32) && iz == P.nd[2] - 1))
+        code |= WO_CON;
+      off = sy * iy + sz * iz;
+      // a row where four columns meet gets two contributions from each z-layer of columns; the
+      // upper layer's go to the edge buffer (added by edge_fixup_kernel), so every row receives
+      // at most two atomic adds onto zero, whose result does not depend on their order
+      if (z == 0 && col_z > 0 && ((y == 0 && col_y > 0) || (y == DY - 1 && col_y + 1 < P.ncol[0]))) {
+        code |= WO_EDGE;
+        off = ((long)(col_z - 1) * (P.ncol[0] - 1) + (y == 0 ? col_y : col_y + 1) - 1) * P.nd[0];
+      }
+    }
+    row_off[r] = off;
+    row_code[r] = code;
+  }
 
   // the src plane of the next chunk is loaded while the current one is scattered and written
   Number nxt[N1][N1];
@@ -477,52 +572,62 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
 
     // ---- scatter: sum the chunk's cells in the LDS DoF brick (zero, after the previous
     //      chunk's write-out, which the barrier orders before these adds) ----
+    //      sub-brick; one wavefront's LDS adds run in program order and no two lanes of one add
+    //      meet at one address, so every sum has a fixed order)
     __syncthreads();
     const int X0 = K * G::BX * chunk;
     const int DX = K * min(G::BX, P.nc[0] - chunk * G::BX) + 1;
-    if (chunk > 0)
-      for (int idx = tid; idx < G::CARRY; idx += G::NT) atomicAdd(&acc[G::AX * idx], carry[idx]);
     if (valid) {
 #pragma unroll
       for (int j = 0; j < N1; ++j)
 #pragma unroll
         for (int i = 0; i < N1; ++i)
-          atomicAdd(&acc[(K * lx + i) + G::AX * ((K * ly + j) + G::AY * (K * lz + l))], (double)u[j][i]);
+          atomicAdd(&sub[(K * ex + i) + G::SX * ((K * ey + j) + G::SY * (K * ez + l))], (double)u[j][i]);
     }
     __syncthreads();
     const bool last = chunk == nchunks - 1;
-    // write-out by x rows of the brick: lane wx of row group wr (coalesced along x)
+    // write-out by x rows of the brick: lane wx of row group wr (coalesced along x). A brick
+    // position lies in one sub-brick, or on a plane two neighbouring boxes share (then in two
+    // per direction); its entries are summed in a fixed order (lower boxes first) and cleared.
+    // The entry index splits into an x part (per lane) and the row's y/z part (row_code).
     const int wr = tid / G::AX, wx = tid - wr * G::AX;
     if (wr < G::ROWS_PER_PASS && wx < DX) {
       const int ix = X0 + wx;
-      for (int r = wr; r < G::AY * G::AZ; r += G::ROWS_PER_PASS) {
-        const int y = r % G::AY, z = r / G::AY;
-        if (y >= DY || z >= DZ) continue;
-        const double sum = acc[wx + G::AX * r];
-        acc[wx + G::AX * r] = 0.0;
-        if (wx == DX - 1 && !last) {
-          carry[r] = sum;
+      int xe[2];
+      const int nxe = brick_parts<G::WX, K * G::EX, G::SUB, 1>(wx, xe);
+      const bool con_x = ((P.dmask & 1) && ix == 0) || ((P.dmask & 2) && ix == P.nd[0] - 1);
+      const bool carry_out = wx == DX - 1 && !last, carry_in = wx == 0 && chunk > 0;
+      double *const carry_rd = carry + ((chunk + 1) & 1) * G::CARRY;
+      double *const carry_wr = carry + (chunk & 1) * G::CARRY;
+      for (int r = wr; r < G::ROWS; r += G::ROWS_PER_PASS) {
+        const int code = row_code[r];
+        if (code & WO_SKIP) continue;
+        const int yz = code & WO_PART;
+        // the previous box's entry in y / z: one box back, at the far end of its sub-brick
+        constexpr int DYP = -G::WX * G::SUB + K * G::EY * G::SX;
+        constexpr int DZP = -G::WX * G::WY * G::SUB + K * G::EZ * G::SX * G::SY;
+        const int y2 = (code & WO_Y2) ? 1 : 0, z2 = (code & WO_Z2) ? 1 : 0;
+        double sum = carry_in ? carry_rd[r] : 0.0;
+#pragma unroll
+        for (int c = 0; c < (G::WZ > 1 ? 2 : 1); ++c)
+#pragma unroll
+          for (int b = 0; b < (G::WY > 1 ? 2 : 1); ++b)
+#pragma unroll
+            for (int a = 0; a < (G::WX > 1 ? 2 : 1); ++a)
+              if (c <= z2 && b <= y2 && a < nxe) {
+                const int e = xe[a] + yz + (c < z2 ? DZP : 0) + (b < y2 ? DYP : 0);
+                sum += acc[e];
+                acc[e] = 0.0;
+              }
+        if (carry_out) {
+          carry_wr[r] = sum;
           continue;
         }
         const Number val = (Number)sum;
-        const int iy = Y0 + y, iz = Z0 + z;
-        const long g = ix + sy * iy + sz * iz;
-        const bool face = (y == 0 || y == DY - 1 || z == 0 || z == DZ - 1);
-        bool con = false;
-        if (P.dmask)
-          con = ((P.dmask & 1) && ix == 0) || ((P.dmask & 2) && ix == P.nd[0] - 1) ||
-                ((P.dmask & 4) && iy == 0) || ((P.dmask & 8) && iy == P.nd[1] - 1) ||
-                ((P.dmask & 16) && iz == 0) || ((P.dmask & 32) && iz == P.nd[2] - 1);
-        if (face) {
-          // a row where four columns meet gets two contributions from each z-layer of columns; the
-          // upper layer's go to the edge buffer (summed in by edge_fixup_kernel), so every row
-          // receives at most two atomic adds onto zero, whose result does not depend on their order
-          if (!con) {
-            Number *tgt = dst + g;
-            if (z == 0 && col_z > 0 && ((y == 0 && col_y > 0) || (y == DY - 1 && col_y + 1 < P.ncol[0])))
-              tgt = P.edge + ((long)(col_z - 1) * (P.ncol[0] - 1) + (y == 0 ? col_y : col_y + 1) - 1) * P.nd[0] + ix;
-            atomicAdd(tgt, val);
-          }
+        const bool con = con_x || (code & WO_CON);
+        const long g = row_off[r] + ix;
+        if (code & WO_FACE) {
+          if (!con) atomicAdd(((code & WO_EDGE) ? P.edge : dst) + g, val);
         } else {
           dst[g] = con ? Number(0) : val;
         }
@@ -573,18 +678,46 @@ size_t edge_elems_shape(const int *nc) {
   return (size_t)((ny - 1) * (nz - 1) * ((long)(N1 - 1) * nc[0] + 1));
 }
 template <int N1, typename Number>
-size_t edge_elems_n1(const int *nc) {
-  return std::max(edge_elems_shape<N1, Number, false>(nc), edge_elems_shape<N1, Number, true>(nc));
+size_t edge_elems_n1(const int *nc, int mapped) {
+  const size_t c = edge_elems_shape<N1, Number, false>(nc), m = edge_elems_shape<N1, Number, true>(nc);
+  return mapped < 0 ? std::max(c, m) : (mapped ? m : c);
 }
-size_t apply_edge_elems(int degree, const int *nc, bool fp32) {
+size_t apply_edge_elems(int degree, const int *nc, bool fp32, int mapped) {
   switch (degree) {
-    case 1: return fp32 ? edge_elems_n1<2, float>(nc) : edge_elems_n1<2, double>(nc);
-    case 2: return fp32 ? edge_elems_n1<3, float>(nc) : edge_elems_n1<3, double>(nc);
-    case 3: return fp32 ? edge_elems_n1<4, float>(nc) : edge_elems_n1<4, double>(nc);
-    case 4: return fp32 ? edge_elems_n1<5, float>(nc) : edge_elems_n1<5, double>(nc);
+    case 1: return fp32 ? edge_elems_n1<2, float>(nc, mapped) : edge_elems_n1<2, double>(nc, mapped);
+    case 2: return fp32 ? edge_elems_n1<3, float>(nc, mapped) : edge_elems_n1<3, double>(nc, mapped);
+    case 3: return fp32 ? edge_elems_n1<4, float>(nc, mapped) : edge_elems_n1<4, double>(nc, mapped);
+    case 4: return fp32 ? edge_elems_n1<5, float>(nc, mapped) : edge_elems_n1<5, double>(nc, mapped);
     default: return 0;
   }
 }
+
+// LDS beyond 64 KiB per workgroup has to be allowed per kernel before its first launch
+template <int N1, typename Number, int KMODE, bool MAPPED>
+int allow_apply_lds() {
+  using G = ApplyGeom<N1, Number, MAPPED>;
+  if (G::LDS_BYTES <= 65536) return MPDG_OK;
+  static const hipError_t rc = hipFuncSetAttribute(
+      reinterpret_cast<const void *>(&heat_apply_kernel<N1, Number, KMODE, MAPPED>),
+      hipFuncAttributeMaxDynamicSharedMemorySize, G::LDS_BYTES);
+  return rc == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+#define MPDG_APPLY_LDS_LIMIT(MAPPED_)                                                                \
+  do {                                                                                             \
+    int rc_ = MPDG_OK;                                                                             \
+    if constexpr (MAPPED_) {                                                                       \
+      rc_ = op.kappa_mode == KAPPA_CELL ? allow_apply_lds<N1, Number, KAPPA_CELL, true>()          \
+                                        : allow_apply_lds<N1, Number, KAPPA_CONST, true>();        \
+    } else {                                                                                       \
+      switch (op.kappa_mode) {                                                                     \
+        case KAPPA_QP: rc_ = allow_apply_lds<N1, Number, KAPPA_QP, false>(); break;                \
+        case KAPPA_CELL: rc_ = allow_apply_lds<N1, Number, KAPPA_CELL, false>(); break;            \
+        case KAPPA_LAW: rc_ = allow_apply_lds<N1, Number, KAPPA_LAW, false>(); break;              \
+        default: rc_ = allow_apply_lds<N1, Number, KAPPA_CONST, false>(); break;                   \
+      }                                                                                            \
+    }                                                                                              \
+    if (rc_ != MPDG_OK) return rc_;                                                                \
+  } while (0)
 
 template <int N1, typename Number, bool MAPPED>
 int launch_apply_shape(const OperatorData<Number> &op, const Number *src, Number *dst, hipStream_t stream,
@@ -607,18 +740,19 @@ int launch_apply_shape(const OperatorData<Number> &op, const Number *src, Number
                        op.nd[0], op.nd[1], op.nd[2], py, pz, nfy, nfz);
   }
   if (kernel_start) (void)hipEventRecord(kernel_start, stream);
+  MPDG_APPLY_LDS_LIMIT(MAPPED);
   if constexpr (MAPPED) {
     switch (op.kappa_mode) {
-      case KAPPA_CONST: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CONST, true>), dim3(ncols), dim3(G::NT), 0, stream, A, src, dst); break;
-      case KAPPA_CELL: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CELL, true>), dim3(ncols), dim3(G::NT), 0, stream, A, src, dst); break;
+      case KAPPA_CONST: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CONST, true>), dim3(ncols), dim3(G::NT), G::LDS_BYTES, stream, A, src, dst); break;
+      case KAPPA_CELL: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CELL, true>), dim3(ncols), dim3(G::NT), G::LDS_BYTES, stream, A, src, dst); break;
       default: return MPDG_ERR_INVALID;  // per-qp conductivity on mapped meshes runs through metric_k
     }
   } else {
     switch (op.kappa_mode) {
-      case KAPPA_CONST: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CONST, false>), dim3(ncols), dim3(G::NT), 0, stream, A, src, dst); break;
-      case KAPPA_QP: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_QP, false>), dim3(ncols), dim3(G::NT), 0, stream, A, src, dst); break;
-      case KAPPA_CELL: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CELL, false>), dim3(ncols), dim3(G::NT), 0, stream, A, src, dst); break;
-      case KAPPA_LAW: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_LAW, false>), dim3(ncols), dim3(G::NT), 0, stream, A, src, dst); break;
+      case KAPPA_CONST: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CONST, false>), dim3(ncols), dim3(G::NT), G::LDS_BYTES, stream, A, src, dst); break;
+      case KAPPA_QP: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_QP, false>), dim3(ncols), dim3(G::NT), G::LDS_BYTES, stream, A, src, dst); break;
+      case KAPPA_CELL: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_CELL, false>), dim3(ncols), dim3(G::NT), G::LDS_BYTES, stream, A, src, dst); break;
+      case KAPPA_LAW: hipLaunchKernelGGL((heat_apply_kernel<N1, Number, KAPPA_LAW, false>), dim3(ncols), dim3(G::NT), G::LDS_BYTES, stream, A, src, dst); break;
       default: return MPDG_ERR_INVALID;
     }
   }

@@ -121,9 +121,10 @@ void fill_apply_args(const OperatorData<Number> &op, ApplyArgs<Number, N1> &A) {
 template <typename Number>
 int launch_apply(const OperatorData<Number> &op, const Number *src, Number *dst, hipStream_t stream,
                  hipEvent_t kernel_start = nullptr);
-// entries of OperatorData::edge the apply of a mesh of degree k and n_cells cells needs (the
-// larger of its Cartesian and mapped chunk shapes); 0 when no DoF row is shared by four columns
-size_t apply_edge_elems(int degree, const int *n_cells, bool fp32);
+// entries of OperatorData::edge the apply of a mesh of degree k and n_cells cells needs with
+// the Cartesian (mapped = 0) or mapped (1) chunk shape, or the larger of the two (mapped = -1);
+// 0 when no DoF row is shared by four columns
+size_t apply_edge_elems(int degree, const int *n_cells, bool fp32, int mapped = -1);
 
 // heat_aux.hip: diagonal, boundary faces, residual, norms, conductivity
 template <typename Number>
