@@ -31,6 +31,7 @@ extern "C" {
 #define MPDG_ERR_NONFINITE (-4)        /* NaN/Inf met in a residual */
 #define MPDG_ERR_DEVICE (-5)           /* HIP runtime error */
 #define MPDG_ERR_ZERO_TIME_INCREMENT (-6) /* ExcZeroTimeIncrement (heat_diffuse_operator.cpp:788) */
+#define MPDG_ERR_TOO_LARGE (-7)        /* mesh beyond a fixed maximum (mpdg_heat_op_vmult_noalloc) */
 
 #define MPDG_FP64 0
 #define MPDG_FP32 1
@@ -121,6 +122,18 @@ int mpdg_heat_op_bind_temperature(mpdg_heat_op *op, const double *T, const doubl
 /* vmult (heat_diffuse_operator.cpp:181-197): dst = J src, dst is overwritten */
 int mpdg_heat_op_vmult(const mpdg_heat_op *op, double *dst, const double *src, void *stream);
 int mpdg_heat_op_vmult_f32(const mpdg_heat_op *op, float *dst, const float *src, void *stream);
+/* vmult that allocates nothing (no hipMalloc/hipFree, no allocating library call). No
+   counterpart in the reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid it). Same result as
+   mpdg_heat_op_vmult, bit for bit (same kernels). The apply's scratch -- the contributions to
+   the DoF rows where four columns of the launch meet -- is a fixed per-GPU array of 2^20
+   doubles (8 MiB) instead of the operator's lazily allocated buffer (entries needed: about
+   0.75 M for Q3 on 128^3 cells, 0.64 M for mapped Q3 on 96^3 cells, 0.27 M for Q4 on 96^3
+   cells); meshes that need more get MPDG_ERR_TOO_LARGE. FP64
+   operators only; a k(T) law on a mapped mesh needs one earlier mpdg_heat_op_vmult (which
+   sizes its rescaled metric), else MPDG_ERR_INVALID. All calls on one GPU share the array:
+   calls that overlap on one GPU race. */
+int mpdg_heat_op_vmult_noalloc(const mpdg_heat_op *op, double *dst, const double *src, void *stream);
 /* compute_inverse_diagonal_from_matrixfree (heat_diffuse_operator.cpp:273-284):
    inv = |d| > 1e-10 ? 1/d : 1, constrained rows 1 */
 int mpdg_heat_op_compute_inverse_diagonal(const mpdg_heat_op *op, double *inv_diag, void *stream);

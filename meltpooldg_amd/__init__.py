@@ -25,7 +25,8 @@ LIB_PATH = os.path.join(HERE, "libmpdg_heat.so")
 
 MPDG_OK = 0
 ERRORS = {-1: "invalid argument", -2: "out of device memory", -3: "no convergence",
-          -4: "non-finite value", -5: "HIP runtime error", -6: "zero time increment"}
+          -4: "non-finite value", -5: "HIP runtime error", -6: "zero time increment",
+          -7: "mesh beyond the fixed scratch maximum"}
 FP64, FP32 = 0, 1
 KAPPA_CONST, KAPPA_QP, KAPPA_CELL, KAPPA_LAW = 0, 1, 2, 3
 LAW_TANH, LAW_MUSHY = 1, 2
@@ -105,6 +106,7 @@ _SIGNATURES = {
     "mpdg_heat_op_set_mapping": ([_V, _V], _I),
     "mpdg_heat_op_vmult": ([_V, _V, _V, _V], _I),
     "mpdg_heat_op_vmult_f32": ([_V, _V, _V, _V], _I),
+    "mpdg_heat_op_vmult_noalloc": ([_V, _V, _V, _V], _I),
     "mpdg_heat_op_compute_inverse_diagonal": ([_V, _V, _V], _I),
     "mpdg_heat_op_compute_diagonal": ([_V, _V, _V], _I),
     "mpdg_heat_op_create_rhs": ([_V, _V, _V, _V, _V, _V], _I),

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <type_traits>
 
@@ -1186,6 +1187,70 @@ int mpdg_heat_op_vmult(const mpdg_heat_op *op, double *dst, const double *src, v
   if (!op || !dst || !src) return MPDG_ERR_INVALID;
   return op->vmult(dst, src, S(stream));
 }
+// ---------------------------------------------------------------------------
+// vmult without allocation (mpdg_heat_op_vmult_noalloc)
+// ---------------------------------------------------------------------------
+// The apply's only scratch is the edge buffer (rows where four columns meet). Here it lives
+// in a fixed device array of kNoAllocScratchDoubles doubles (8 MiB), one copy per GPU, held
+// until the process ends. Every call on a GPU uses that GPU's copy: calls that overlap on one
+// GPU (other streams, other threads) race.
+constexpr size_t kNoAllocScratchDoubles = size_t(1) << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// device address of the current device's copy of `scratch`, looked up once per device with
+// that device current (the array's name is a host-side stand-in, not a device address)
+int scratch_address(double **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return MPDG_ERR_DEVICE;
+  if (dev < 0 || dev >= kMaxScratchDevices) return MPDG_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return MPDG_ERR_DEVICE;
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return MPDG_OK;
+}
+}  // namespace
+
+int mpdg_heat_op_vmult_noalloc(const mpdg_heat_op *op, double *dst, const double *src, void *stream) {
+  if (!op || !dst || !src || op->precision != MPDG_FP64) return MPDG_ERR_INVALID;
+  hipStream_t s = S(stream);
+  if (op->tangent) {
+    // the full tangent's kernels use no scratch
+    if (!op->td_bound) return MPDG_ERR_INVALID;
+    MPDG_TRY(launch_tangent_apply(op->d64, op->td.get(), op->vel, op->solid, src, dst, s));
+  } else {
+    if (op->d64.metric && op->d64.kappa_mode == KAPPA_LAW) {
+      // k(T) on a mapped mesh rescales the metric into operator-owned arrays: only when they
+      // already exist at their size (an earlier vmult made them), so that nothing is allocated
+      const int n1 = op->d64.degree + 1;
+      const size_t nq = (size_t)op->d64.n_cells * n1 * n1 * n1;
+      if (op->kq.n != nq || op->metric64_k.n != nq * 7) return MPDG_ERR_INVALID;
+      MPDG_TRY(op->refresh_metric_k(s));
+    }
+    const size_t need = apply_edge_elems(op->d64.degree, op->d64.nc, false, op->d64.metric ? 1 : 0);
+    if (need > kNoAllocScratchDoubles) return MPDG_ERR_TOO_LARGE;
+    double *scr = nullptr;
+    MPDG_TRY(scratch_address(&scr));
+    OperatorData<double> od = op->d64;
+    od.edge = scr;
+    od.edge_elems = kNoAllocScratchDoubles;
+    // the previous call's contents stay in the array: clear the part this apply uses (a fill
+    // kernel, which a captured graph replays correctly)
+    if (need) MPDG_TRY(launch_fill<double>(scr, 0.0, (long)need, s));
+    MPDG_TRY(launch_apply<double>(od, src, dst, s));
+  }
+  if (op->d64.robin_mask) MPDG_TRY(launch_boundary(op->d64, 0, nullptr, dst, src, op->T_robin, s));
+  return MPDG_OK;
+}
+
 int mpdg_heat_op_vmult_f32(const mpdg_heat_op *op, float *dst, const float *src, void *stream) {
   if (!op || !dst || !src) return MPDG_ERR_INVALID;
   return op->vmult_f32(dst, src, S(stream));
