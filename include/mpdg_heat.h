@@ -226,11 +226,13 @@ typedef struct mpdg_heat_operation_params {
   double residual_tolerance;
   double field_correction_tolerance;
   int step_mode;          /* MPDG_STEP_NEWTON or MPDG_STEP_LINEAR */
-  int guess_check;        /* learned predictor: fall back to T_n when its residual is not smaller */
+  int guess_check;        /* learned predictor: fall back to its base guess (the extrapolation it
+                             corrects) when the learned guess's residual is not smaller */
   int guess_smoothing;    /* learned predictor (with guess_check, linear step mode): smoothing
                              steps on J T = b applied to an accepted guess (multigrid: Chebyshev
                              steps of its finest level; 0: none) */
-  int extrapolation_order;/* MPDG_PREDICTOR_EXTRAPOLATION: 1..6 (the order drops to what the
+  int extrapolation_order;/* MPDG_PREDICTOR_EXTRAPOLATION, and the base guess that
+                             MPDG_PREDICTOR_LEARNED corrects: 1..6 (the order drops to what the
                              history holds: after set_initial the first steps use fewer points) */
   int n_old_solution_vectors; /* MPDG_PREDICTOR_LSQ(_SOLUTION): 1..7 solutions (predictor_data.hpp:27-28;
                                  until the history holds them, the previous solution is used) */
@@ -252,12 +254,14 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
    source it reads (the caller refreshes the field before each step) */
 typedef struct mpdg_learned mpdg_learned;
 int mpdg_heat_operation_set_learned(mpdg_heat_operation *h, mpdg_learned *net, const double *q_field);
-/* record, from the next step on, the network's residual input of every step into the device
-   DoF vector z (z = D^-1 (b - J T_lin), the residual of the linear extrapolation
-   T_lin = T_n + (dt/dt_old)(T_n - T_{n-1}) scaled by the inverse operator diagonal; in
-   MPDG_STEP_NEWTON b - J T_lin is the Newton residual at T_lin); NULL stops recording.
-   Training pairs for mpdg_learned_train_step come from a run with this set. */
-int mpdg_heat_operation_record_guess_feature(mpdg_heat_operation *h, double *z);
+/* record, from the next step on, the network's inputs of every step: into the device DoF
+   vector z the residual of the predictor's guess T_b (the linear extrapolation
+   T_n + (dt/dt_old)(T_n - T_{n-1}), or the polynomial extrapolation of order
+   extrapolation_order) scaled by the inverse operator diagonal, z = D^-1 (b - J T_b) (in
+   MPDG_STEP_NEWTON b - J T_b is the Newton residual at T_b), and into base (may be NULL) T_b
+   itself; z = NULL stops recording. Training pairs for mpdg_learned_train_step come from a run
+   with this set. */
+int mpdg_heat_operation_record_guess_feature(mpdg_heat_operation *h, double *z, double *base);
 /* statistics of the last step: 1 if the learned guess was replaced by T_n, initial and final
    relative residuals ||b - J T||/||b|| of the step's linear solve (MPDG_STEP_LINEAR) */
 int mpdg_heat_operation_last_step_info(const mpdg_heat_operation *h, int *fell_back, double *rel_res0,
@@ -309,21 +313,23 @@ int mpdg_learned_n_params(const mpdg_learned *net);
 /* weights as one FP32 host array: per layer W[co][ci][tap] then b[co] */
 int mpdg_learned_get_weights(const mpdg_learned *net, float *host);
 int mpdg_learned_set_weights(mpdg_learned *net, const float *host);
-/* out = (2 T_n - T_{n-1}) + P (oscale sz y(T_{n-1}, T_n, q, z)), sz = rms(z) on the network grid:
-   linear extrapolation (equal steps) plus the learned correction. Device FP64 DoF vectors; z is
-   the residual input recorded by mpdg_heat_operation_record_guess_feature; q and z may be NULL
-   (channel zero; without z, sz = rms(T_n - T_{n-1})) */
+/* out = T_b + P (oscale sz y(T_{n-1}, T_n, q, z)), sz = rms(z) on the network grid: the base
+   guess T_b (base; NULL: the linear extrapolation 2 T_n - T_{n-1}, equal steps) plus the learned
+   correction. Device FP64 DoF vectors; z (the residual input of T_b) and base are recorded by
+   mpdg_heat_operation_record_guess_feature; q and z may be NULL (channel zero; without z,
+   sz = rms(T_n - T_{n-1})); out may be base */
 int mpdg_learned_predict(mpdg_learned *net, const double *T_nm1, const double *T_n, const double *q,
-                         const double *z, double *out, void *stream);
-/* one Adam step on the sample (T_{n-1}, T_n, q, z) -> T_{n+1}; loss = mean squared error of
-   y against W(T_{n+1} - (2 T_n - T_{n-1}))/(oscale sz) before the step (W = full weighting to
-   the network grid) */
+                         const double *z, const double *base, double *out, void *stream);
+/* one Adam step on the sample (T_{n-1}, T_n, q, z, T_b) -> T_{n+1}; loss = mean squared error of
+   y against W(T_{n+1} - T_b)/(oscale sz) before the step (W = full weighting to the network
+   grid; base NULL: T_b = 2 T_n - T_{n-1}) */
 int mpdg_learned_train_step(mpdg_learned *net, const double *T_nm1, const double *T_n, const double *q,
-                            const double *z, const double *T_np1, double lr, double *loss, void *stream);
+                            const double *z, const double *base, const double *T_np1, double lr, double *loss,
+                            void *stream);
 /* gradients of the loss (host FP32, weight layout) without an update */
 int mpdg_learned_gradients(mpdg_learned *net, const double *T_nm1, const double *T_n, const double *q,
-                           const double *z, const double *T_np1, float *host_grad, double *loss,
-                           void *stream);
+                           const double *z, const double *base, const double *T_np1, float *host_grad,
+                           double *loss, void *stream);
 /* layer activations of the last forward pass as bf16 bit patterns [voxel][16] on the network grid
    (layer 0 = input) */
 int mpdg_learned_activation(const mpdg_learned *net, int layer, unsigned short *host);

@@ -127,7 +127,7 @@ _SIGNATURES = {
     "mpdg_heat_operation_solution": ([_V], _V),
     "mpdg_heat_operation_old_solution": ([_V], _V),
     "mpdg_heat_operation_set_learned": ([_V, _V, _V], _I),
-    "mpdg_heat_operation_record_guess_feature": ([_V, _V], _I),
+    "mpdg_heat_operation_record_guess_feature": ([_V, _V, _V], _I),
     "mpdg_heat_operation_last_step_info": ([_V, C.POINTER(_I), C.POINTER(_D), C.POINTER(_D)], _I),
     "mpdg_heat_op_surface_flux_field": ([_V, _V, _V], _I),
     "mpdg_learned_create": ([C.POINTER(_I), _I, _I, _D, C.c_ulonglong, C.POINTER(_V)], _I),
@@ -138,9 +138,9 @@ _SIGNATURES = {
     "mpdg_learned_n_params": ([_V], _I),
     "mpdg_learned_get_weights": ([_V, _V], _I),
     "mpdg_learned_set_weights": ([_V, _V], _I),
-    "mpdg_learned_predict": ([_V, _V, _V, _V, _V, _V, _V], _I),
-    "mpdg_learned_train_step": ([_V, _V, _V, _V, _V, _V, _D, C.POINTER(_D), _V], _I),
-    "mpdg_learned_gradients": ([_V, _V, _V, _V, _V, _V, _V, C.POINTER(_D), _V], _I),
+    "mpdg_learned_predict": ([_V, _V, _V, _V, _V, _V, _V, _V], _I),
+    "mpdg_learned_train_step": ([_V, _V, _V, _V, _V, _V, _V, _D, C.POINTER(_D), _V], _I),
+    "mpdg_learned_gradients": ([_V, _V, _V, _V, _V, _V, _V, _V, C.POINTER(_D), _V], _I),
     "mpdg_learned_activation": ([_V, _I, _V], _I),
 }
 
@@ -474,14 +474,16 @@ class HeatDiffuseOperation:
         if learned is not None:
             _check(self.lib.mpdg_heat_operation_set_learned(h, learned.h, _ptr(q_field)), "set_learned")
 
-    def record_guess_feature(self, z):
-        """From the next step on, write each step's network residual input D^-1 (b - J T_lin)
-        into the device vector z (None stops); training pairs need it."""
-        self.z_record = z
-        _check(self.lib.mpdg_heat_operation_record_guess_feature(self.h, _ptr(z)), "record_guess_feature")
+    def record_guess_feature(self, z, base=None):
+        """From the next step on, write each step's network residual input D^-1 (b - J T_b) into
+        the device vector z and the base guess T_b (the predictor's extrapolation) into base
+        (None stops / skips); training pairs need them."""
+        self.z_record = (z, base)
+        _check(self.lib.mpdg_heat_operation_record_guess_feature(self.h, _ptr(z), _ptr(base)),
+               "record_guess_feature")
 
     def last_step_info(self):
-        """(learned guess replaced by T_n, initial and final ||b - J T||/||b|| of the step)"""
+        """(learned guess replaced by its base guess, initial and final ||b - J T||/||b|| of the step)"""
         fb, r0, r = C.c_int(), C.c_double(), C.c_double()
         _check(self.lib.mpdg_heat_operation_last_step_info(self.h, C.byref(fb), C.byref(r0), C.byref(r)), "info")
         return bool(fb.value), r0.value, r.value
@@ -521,8 +523,9 @@ class HeatDiffuseOperation:
 
 
 class LearnedPredictor:
-    """Learned initial guess: 3D CNN predicting T_{n+1} from (T_{n-1}, T_n, q) and
-    z = D^-1 (b - J T_lin), the scaled residual of the linear extrapolation for the step.
+    """Learned initial guess: 3D CNN predicting the correction of a base guess T_b (the linear
+    or a higher-order extrapolation of the history) for T_{n+1} from (T_{n-1}, T_n, q) and
+    z = D^-1 (b - J T_b), the scaled residual of the base guess for the step.
 
     Fields are DoF vectors of Q_degree on n_cells cells; the network works on the DoF grid of
     Q_degree on n_cells / coarsening cells (full weighting there, FE prolongation back).
@@ -551,24 +554,25 @@ class LearnedPredictor:
         except Exception:
             pass
 
-    def predict(self, T_nm1, T_n, q, z, out):
-        """out = (2 T_n - T_{n-1}) + rms(z) y (device FP64 vectors; q, z may be None)."""
-        _check(self.lib.mpdg_learned_predict(self.h, _ptr(T_nm1), _ptr(T_n), _ptr(q), _ptr(z), _ptr(out),
-                                             _stream()), "learned predict")
+    def predict(self, T_nm1, T_n, q, z, out, base=None):
+        """out = T_b + rms(z) y with T_b = base, or 2 T_n - T_{n-1} (device FP64 vectors; q, z,
+        base may be None)."""
+        _check(self.lib.mpdg_learned_predict(self.h, _ptr(T_nm1), _ptr(T_n), _ptr(q), _ptr(z), _ptr(base),
+                                             _ptr(out), _stream()), "learned predict")
 
-    def train_step(self, T_nm1, T_n, q, z, T_np1, lr):
+    def train_step(self, T_nm1, T_n, q, z, T_np1, lr, base=None):
         loss = C.c_double()
-        _check(self.lib.mpdg_learned_train_step(self.h, _ptr(T_nm1), _ptr(T_n), _ptr(q), _ptr(z), _ptr(T_np1), lr,
-                                                C.byref(loss), _stream()), "learned train_step")
+        _check(self.lib.mpdg_learned_train_step(self.h, _ptr(T_nm1), _ptr(T_n), _ptr(q), _ptr(z), _ptr(base),
+                                                _ptr(T_np1), lr, C.byref(loss), _stream()), "learned train_step")
         return loss.value
 
-    def gradients(self, T_nm1, T_n, q, z, T_np1):
+    def gradients(self, T_nm1, T_n, q, z, T_np1, base=None):
         import numpy as np
         g = np.zeros(self.n_params, dtype=np.float32)
         loss = C.c_double()
-        _check(self.lib.mpdg_learned_gradients(self.h, _ptr(T_nm1), _ptr(T_n), _ptr(q), _ptr(z), _ptr(T_np1),
-                                               g.ctypes.data_as(C.c_void_p), C.byref(loss), _stream()),
-               "learned gradients")
+        _check(self.lib.mpdg_learned_gradients(self.h, _ptr(T_nm1), _ptr(T_n), _ptr(q), _ptr(z), _ptr(base),
+                                               _ptr(T_np1), g.ctypes.data_as(C.c_void_p), C.byref(loss),
+                                               _stream()), "learned gradients")
         return loss.value, g
 
     def get_weights(self):

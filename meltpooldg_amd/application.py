@@ -80,30 +80,37 @@ class LaserScan:
         return self.heat.solution()
 
 
-def generate_trajectory(degree, cells, n_steps, T_init=None, **kw):
+def generate_trajectory(degree, cells, n_steps, T_init=None, with_base=False, **kw):
     """Run n_steps and keep every T_n (T_0 = 0, or the uniform T_init, included), the source field
     of each step and the network's residual input of each step: qs[n], zs[n] belong to the step
-    T_n -> T_{n+1}."""
+    T_n -> T_{n+1}. With with_base, also the base guess of each step (the predictor's
+    extrapolation, bs[n]) as a sixth return value."""
     case = LaserScan(degree, cells, **kw)
     if T_init is not None:
         case.heat.set_initial_condition(case.torch.full_like(case.T0, T_init))
     z = case.torch.zeros_like(case.T0)
-    case.heat.record_guess_feature(z)
+    base = case.torch.zeros_like(case.T0) if with_base else None
+    case.heat.record_guess_feature(z, base)
     Ts = [case.solution()]
-    qs, zs = [], []
+    qs, zs, bs = [], [], []
     stats = []
     for _ in range(n_steps):
         its, el, info = case.step()
         Ts.append(case.solution())
         qs.append(case.q.clone())
         zs.append(z.clone())
+        if with_base:
+            bs.append(base.clone())
         stats.append((its, el, info))
     case.heat.record_guess_feature(None)
+    if with_base:
+        return case, Ts, qs, zs, stats, bs
     return case, Ts, qs, zs, stats
 
 
-def train(net, Ts, qs, zs, epochs=4, lr=1e-3, first=2, seed=0, log=None):
-    """Adam over the solver's own pairs ((T_{n-1}, T_n, q_{n+1}, z_n) -> T_{n+1}), n >= first."""
+def train(net, Ts, qs, zs, epochs=4, lr=1e-3, first=2, seed=0, log=None, bs=None):
+    """Adam over the solver's own pairs ((T_{n-1}, T_n, q_{n+1}, z_n[, T_b,n]) -> T_{n+1}),
+    n >= first; bs (the base guesses) None: the linear extrapolation."""
     import random
     rng = random.Random(seed)
     idx = list(range(first, len(Ts) - 1))
@@ -112,7 +119,7 @@ def train(net, Ts, qs, zs, epochs=4, lr=1e-3, first=2, seed=0, log=None):
         rng.shuffle(idx)
         tot = 0.0
         for n in idx:
-            tot += net.train_step(Ts[n - 1], Ts[n], qs[n], zs[n], Ts[n + 1], lr)
+            tot += net.train_step(Ts[n - 1], Ts[n], qs[n], zs[n], Ts[n + 1], lr, base=None if bs is None else bs[n])
         losses.append(tot / len(idx))
         if log:
             log("epoch %d loss %.4e" % (e, losses[-1]))

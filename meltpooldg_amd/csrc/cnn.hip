@@ -223,14 +223,16 @@ __global__ void features_kernel(const double *__restrict__ Tnm1, const double *_
 // g = 2 (pred - y*) / n * gain; partial sums of the squared error. With y = null: partial
 // sums of y*^2 only (the output scale's calibration).
 __global__ __launch_bounds__(256) void loss_kernel(const unsigned short *__restrict__ y, const double *__restrict__ Tnm1,
-                                                   const double *__restrict__ Tn, const double *__restrict__ Tnp1,
-                                                   long n, const double *__restrict__ stats, double oscale, float gain,
+                                                   const double *__restrict__ Tn, const double *__restrict__ base,
+                                                   const double *__restrict__ Tnp1, long n,
+                                                   const double *__restrict__ stats, double oscale, float gain,
                                                    unsigned short *__restrict__ g, double *__restrict__ part) {
   __shared__ double sh[256];
   const double sz = stats[3] * oscale;
   double acc = 0.0;
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-    const double tgt = (Tnp1[i] - (2.0 * Tn[i] - Tnm1[i])) / sz;
+    const double b = base ? base[i] : 2.0 * Tn[i] - Tnm1[i];
+    const double tgt = (Tnp1[i] - b) / sz;
     if (!y) {
       acc += tgt * tgt;
       continue;
@@ -257,11 +259,12 @@ __global__ void correction_kernel(const unsigned short *__restrict__ y, long n, 
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
     corr[i] = sz * (double)bf16_to_f32(y[i * 16]);
 }
-// guess = (2 Tn - Tnm1) + corr on the DoF grid
+// guess = base + corr on the DoF grid, base = 2 Tn - Tnm1 when not given (out may be base)
 __global__ void extrapolate_add_kernel(const double *__restrict__ Tnm1, const double *__restrict__ Tn,
-                                       const double *__restrict__ corr, long n, double *__restrict__ out) {
+                                       const double *base, const double *__restrict__ corr, long n,
+                                       double *out) {
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
-    out[i] = (2.0 * Tn[i] - Tnm1[i]) + corr[i];
+    out[i] = (base ? base[i] : 2.0 * Tn[i] - Tnm1[i]) + corr[i];
 }
 
 // statistics of a sample: sums of (Tn - Tnm1)^2, Tn, Tn^2, z^2
@@ -358,10 +361,11 @@ int cnn_features(const double *Tnm1, const double *Tn, const double *q, const do
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
-int cnn_loss(const unsigned short *y, const double *Tnm1, const double *Tn, const double *Tnp1, long n,
-             const double *stats, double oscale, float gain, unsigned short *g, double *part, int nblocks,
+int cnn_loss(const unsigned short *y, const double *Tnm1, const double *Tn, const double *base, const double *Tnp1,
+             long n, const double *stats, double oscale, float gain, unsigned short *g, double *part, int nblocks,
              hipStream_t s) {
-  hipLaunchKernelGGL(loss_kernel, dim3(nblocks), dim3(256), 0, s, y, Tnm1, Tn, Tnp1, n, stats, oscale, gain, g, part);
+  hipLaunchKernelGGL(loss_kernel, dim3(nblocks), dim3(256), 0, s, y, Tnm1, Tn, base, Tnp1, n, stats, oscale, gain, g,
+                     part);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
@@ -371,9 +375,10 @@ int cnn_correction(const unsigned short *y, long n, const double *stats, double 
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
-int cnn_extrapolate_add(const double *Tnm1, const double *Tn, const double *corr, long n, double *out, hipStream_t s) {
+int cnn_extrapolate_add(const double *Tnm1, const double *Tn, const double *base, const double *corr, long n,
+                        double *out, hipStream_t s) {
   hipLaunchKernelGGL(extrapolate_add_kernel, dim3((int)std::min<long>(8192, (n + 255) / 256)), dim3(256), 0, s, Tnm1,
-                     Tn, corr, n, out);
+                     Tn, base, corr, n, out);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 

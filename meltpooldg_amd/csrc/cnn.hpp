@@ -16,11 +16,14 @@ int cnn_stats(const double *Tnm1, const double *Tn, const double *z, long n, dou
               hipStream_t s);
 int cnn_features(const double *Tnm1, const double *Tn, const double *q, const double *z, long n,
                  const double *stats, double qscale, unsigned short *x, hipStream_t s);
-int cnn_loss(const unsigned short *y, const double *Tnm1, const double *Tn, const double *Tnp1, long n,
-             const double *stats, double oscale, float gain, unsigned short *g, double *part, int nblocks,
+// base: the field the network corrects (nullptr: the linear extrapolation 2 Tn - Tnm1)
+int cnn_loss(const unsigned short *y, const double *Tnm1, const double *Tn, const double *base, const double *Tnp1,
+             long n, const double *stats, double oscale, float gain, unsigned short *g, double *part, int nblocks,
              hipStream_t s);
-// corr = rms(z) oscale y on the network grid; out = (2 Tn - Tnm1) + corr on the DoF grid
+// corr = rms(z) oscale y on the network grid; out = base + corr on the DoF grid (base nullptr:
+// 2 Tn - Tnm1)
 int cnn_correction(const unsigned short *y, long n, const double *stats, double oscale, double *corr, hipStream_t s);
-int cnn_extrapolate_add(const double *Tnm1, const double *Tn, const double *corr, long n, double *out, hipStream_t s);
+int cnn_extrapolate_add(const double *Tnm1, const double *Tn, const double *base, const double *corr, long n,
+                        double *out, hipStream_t s);
 
 }  // namespace mpdg

@@ -1006,7 +1006,10 @@ struct mpdg_heat_operation {
   mpdg_learned *net = nullptr;
   const double *q_field = nullptr;
   double *z_record = nullptr;        // caller's vector for the residual input of each step
+  double *base_record = nullptr;     // caller's vector for the base guess of each step
   DeviceArray<double> zbuf, invd;    // residual input and inverse diagonal (learned predictor)
+  DeviceArray<double> tbase;         // base guess the network corrected (the fallback)
+  bool learned_applied = false;      // this step's guess came from the network
   double invd_dt = 0.0;              // time step the cached inverse diagonal was computed for
   DeviceArray<double> kq_step;       // k(T_n) per quadrature point, frozen for a linear step
   DeviceArray<double> lsq_x, lsq_q, lsq_part, lsq_h;  // least-squares projection: bases and scalars
@@ -1361,8 +1364,9 @@ int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p, const mpdg_hea
     return MPDG_ERR_INVALID;
   if (prm->step_mode != MPDG_STEP_NEWTON && prm->step_mode != MPDG_STEP_LINEAR) return MPDG_ERR_INVALID;
   if (!valid_solver(prm->linear)) return MPDG_ERR_INVALID;
-  if (prm->predictor == MPDG_PREDICTOR_EXTRAPOLATION &&
-      (prm->extrapolation_order < 1 || prm->extrapolation_order > mpdg_heat_operation::kMaxOlder + 1))
+  if ((prm->predictor == MPDG_PREDICTOR_EXTRAPOLATION || prm->predictor == MPDG_PREDICTOR_LEARNED) &&
+      (prm->extrapolation_order < (prm->predictor == MPDG_PREDICTOR_LEARNED ? 0 : 1) ||
+       prm->extrapolation_order > mpdg_heat_operation::kMaxOlder + 1))
     return MPDG_ERR_INVALID;
   auto *h = new (std::nothrow) mpdg_heat_operation;
   if (!h) return MPDG_ERR_NOMEM;
@@ -1371,9 +1375,12 @@ int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p, const mpdg_hea
   h->prm = *prm;
   const long n = op->n_dofs();
   int rc = MPDG_OK;
-  h->depth = prm->predictor == MPDG_PREDICTOR_EXTRAPOLATION ? prm->extrapolation_order - 1
-             : lsq                                           ? std::max(0, prm->n_old_solution_vectors - 2)
-                                                             : 0;
+  // the learned predictor corrects the extrapolation of order extrapolation_order (0 or 1: linear)
+  if (prm->predictor == MPDG_PREDICTOR_LEARNED) h->prm.extrapolation_order = std::max(1, prm->extrapolation_order);
+  const bool extrap = prm->predictor == MPDG_PREDICTOR_EXTRAPOLATION || prm->predictor == MPDG_PREDICTOR_LEARNED;
+  h->depth = extrap ? h->prm.extrapolation_order - 1
+             : lsq  ? std::max(0, prm->n_old_solution_vectors - 2)
+                    : 0;
   for (int b = 0; b < 3 + h->depth; ++b) rc = rc ? rc : h->buf[b].resize(n);
   rc = rc ? rc : h->rhs.resize(n);
   rc = rc ? rc : h->du.resize(n);
@@ -1405,10 +1412,11 @@ int mpdg_heat_operation_set_initial(mpdg_heat_operation *h, const double *T0, vo
   return hipStreamSynchronize(S(stream)) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
-// Residual input of the learned guess for the predicted T (the linear extrapolation):
+// Residual input of the learned guess for the predicted T (the base guess, an extrapolation):
 // z = D^-1 (b - J T) with b given (linear step), or z = D^-1 r_Newton(T, T_old) (b == nullptr).
+// rr (device scalar, may be nullptr): |b - J T|^2 before the scaling.
 static int guess_feature(mpdg_heat_operation *h, const double *b, const double *T, const double *T_old, double *z,
-                         hipStream_t s) {
+                         double *rr, hipStream_t s) {
   mpdg_heat_op *op = h->op;
   const long n = op->n_dofs();
   // the inverse diagonal is kept while dt is unchanged and the conductivity does not follow T
@@ -1425,26 +1433,36 @@ static int guess_feature(mpdg_heat_operation *h, const double *b, const double *
     MPDG_TRY(launch_fill<double>(z, 0.0, n, s));
     MPDG_TRY(op->create_rhs(z, T, T_old, nullptr, s));
   }
+  if (rr) MPDG_TRY(launch_dot<double, double>(z, z, n, h->red.red(), rr, s));
   return launch_mul(z, h->invd.get(), n, s);
 }
 
-// Learned predictor after the linear extrapolation T (= T_lin) is in place: records the
-// residual input and, with equal steps and a network, replaces T by the network's guess.
+// Learned predictor after the base guess T (the extrapolation) is in place: records the
+// residual input (and the base) and, with equal steps and a network, replaces T by the
+// network's correction of it; the base stays in tbase as the fallback, and the base's squared
+// residual norm |b - J T|^2 in scal[1] for the guess check (linear step).
 static int learned_guess(mpdg_heat_operation *h, double dt, const double *b, double *T, const double *T_old,
                          hipStream_t s) {
   const bool learned = h->prm.predictor == MPDG_PREDICTOR_LEARNED;
+  h->learned_applied = false;
   if (!learned && !h->z_record) return MPDG_OK;
+  const long n = h->op->n_dofs();
   double *z = h->z_record;
   if (!z) {
-    MPDG_TRY(h->zbuf.resize(h->op->n_dofs()));
+    MPDG_TRY(h->zbuf.resize(n));
     z = h->zbuf.get();
   }
-  MPDG_TRY(guess_feature(h, b, T, T_old, z, s));
+  if (h->base_record) MPDG_TRY(launch_convert<double, double>(T, h->base_record, n, s));
+  const bool check = learned && b && h->prm.guess_check;
+  MPDG_TRY(guess_feature(h, b, T, T_old, z, check ? h->scal.get() + 1 : nullptr, s));
   // the network extrapolates with equal steps (as in its training trajectories)
   const bool equal_steps = h->dt_old > 0.0 && std::fabs(dt - h->dt_old) <= 1e-12 * dt;
   if (learned && equal_steps) {
     const double *Tnm1 = h->buf[h->depth > 0 ? h->older[0] : h->spare].get();  // T_{n-1} of this step
-    MPDG_TRY(mpdg_learned_predict(h->net, Tnm1, T_old, h->q_field, z, T, s));
+    MPDG_TRY(h->tbase.resize(n));
+    MPDG_TRY(launch_convert<double, double>(T, h->tbase.get(), n, s));
+    MPDG_TRY(mpdg_learned_predict(h->net, Tnm1, T_old, h->q_field, z, h->tbase.get(), T, s));
+    h->learned_applied = true;
   }
   return MPDG_OK;
 }
@@ -1509,11 +1527,9 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
   MPDG_TRY(launch_fill<double>(h->bvec.get(), 0.0, n, s));
   MPDG_TRY(op->create_rhs(h->bvec.get(), h->zero.get(), T_old, nullptr, s, true));
   MPDG_TRY(learned_guess(h, dt, h->bvec.get(), T, T_old, s));
-  if (h->prm.predictor == MPDG_PREDICTOR_LEARNED && h->prm.guess_check) {
-    // residuals of the previous solution (T_old) and of the learned guess (T, kept in alt)
-    MPDG_TRY(op->vmult(h->alt.get(), T_old, s));
-    MPDG_TRY(launch_axpby(h->alt.get(), 1.0, h->bvec.get(), -1.0, n, s));
-    MPDG_TRY(launch_dot<double, double>(h->alt.get(), h->alt.get(), n, h->red.red(), h->scal.get() + 1, s));
+  if (h->learned_applied && h->prm.guess_check) {
+    // the learned guess (T) against the base guess it corrects (its residual norm came with the
+    // network's input); the residual of T is kept in alt
     MPDG_TRY(op->vmult(h->alt.get(), T, s));
     MPDG_TRY(launch_axpby(h->alt.get(), 1.0, h->bvec.get(), -1.0, n, s));
     MPDG_TRY(launch_dot<double, double>(h->alt.get(), h->alt.get(), n, h->red.red(), h->scal.get(), s));
@@ -1522,7 +1538,7 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
       return MPDG_ERR_DEVICE;
     if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
     if (!(rr[0] == rr[0]) || !(rr[0] < rr[1])) {
-      MPDG_TRY(launch_convert<double, double>(T_old, T, n, s));
+      MPDG_TRY(launch_convert<double, double>(h->tbase.get(), T, n, s));
       h->fell_back = 1;
     } else if (h->prm.guess_smoothing > 0 && h->pc) {
       // the network's correction is prolonged from its working grid; smoothing its error
@@ -1712,8 +1728,13 @@ int mpdg_heat_operation_solve_step(mpdg_heat_operation *h, double dt, int *lin_i
     case MPDG_PREDICTOR_NONE: MPDG_TRY(launch_convert<double, double>(Tn, pred, n, s)); break;
     case MPDG_PREDICTOR_ZERO: MPDG_TRY(launch_fill<double>(pred, 0.0, n, s)); break;
     case MPDG_PREDICTOR_LEARNED:
+      // the network corrects the extrapolation of order extrapolation_order once b is known
       if (!h->net) return MPDG_ERR_INVALID;
-      [[fallthrough]];  // the network corrects the linear extrapolation once b is known
+      if (h->prm.extrapolation_order > 1) {
+        MPDG_TRY(extrapolate(h, dt, pred, s));
+        break;
+      }
+      [[fallthrough]];
     case MPDG_PREDICTOR_LINEAR: {
       // compute_linear_predictor (source/linear_algebra/predictor_linear.cpp:76-97)
       if (std::fabs(h->dt_old) < 1e-12) MPDG_TRY(launch_convert<double, double>(Tn, pred, n, s));
@@ -1815,9 +1836,10 @@ int mpdg_heat_operation_set_learned(mpdg_heat_operation *h, mpdg_learned *net, c
   h->q_field = q_field;
   return MPDG_OK;
 }
-int mpdg_heat_operation_record_guess_feature(mpdg_heat_operation *h, double *z) {
+int mpdg_heat_operation_record_guess_feature(mpdg_heat_operation *h, double *z, double *base) {
   if (!h) return MPDG_ERR_INVALID;
   h->z_record = z;
+  h->base_record = z ? base : nullptr;
   return MPDG_OK;
 }
 int mpdg_heat_operation_last_step_info(const mpdg_heat_operation *h, int *fell_back, double *r0, double *r) {

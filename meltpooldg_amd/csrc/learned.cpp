@@ -7,10 +7,11 @@
 //
 // Features (per sample, s = rms(T_n - T_{n-1})): c0 = (T_n - T_{n-1})/s,
 // c1 = q * qscale / s, c2 = (T_n - mean T_n)/rms, c3 = z / sz, where z is the
-// residual of the linear extrapolation T_lin = 2 T_n - T_{n-1} for the step,
-// scaled by the inverse operator diagonal (z = D^-1 (b - J T_lin)), and
-// sz = rms(z). The network output y is the correction of the extrapolation in
-// units of sz: guess = T_lin + sz y. The exact correction is J^-1 (b - J T_lin);
+// residual of the base guess T_b for the step (a polynomial extrapolation of the
+// history; by default the linear one, T_lin = 2 T_n - T_{n-1}), scaled by the inverse
+// operator diagonal (z = D^-1 (b - J T_b)), and sz = rms(z). The network output y is
+// the correction of the base guess in units of sz: guess = T_b + sz y. The exact
+// correction is J^-1 (b - J T_b);
 // where the operator is mass-dominated (dt k / (rho c_p h^2) of order one or
 // less) that is a short-range smoothing of z, which the network learns from the
 // solver's own (T_{n-1}, T_n, q_{n+1}, z_n) -> T_{n+1} pairs.
@@ -74,7 +75,7 @@ struct mpdg_learned {
   DeviceArray<int> wcol[3], pcol[3];
   DeviceArray<double> wval[3], pval[3];
   int ww[3] = {0, 0, 0}, pw[3] = {0, 0, 0};
-  DeviceArray<double> cf[5], tmpa, tmpb, corr, corr_f;
+  DeviceArray<double> cf[6], tmpa, tmpb, corr, corr_f;
 
   int init(const int *cells, int k, int coarsening, double qs, unsigned long long seed) {
     degree = k;
@@ -179,16 +180,18 @@ struct mpdg_learned {
                         nullptr, s));
     return MPDG_OK;
   }
-  // rms of the target (T_{n+1} - T_lin) / rms(z) of a sample after forward()
-  int target_rms(const double *Tnp1, double *rms, hipStream_t s) {
+  // rms of the target (T_{n+1} - T_b) / rms(z) of a sample after forward()
+  int target_rms(const double *base, const double *Tnp1, double *rms, hipStream_t s) {
     int rc = MPDG_OK;
     const double *tp = to_net(Tnp1, 4, s, &rc);
     MPDG_TRY(rc);
-    return target_rms_net(tp, rms, 512, s);
+    const double *bn = to_net(base, 5, s, &rc);
+    MPDG_TRY(rc);
+    return target_rms_net(bn, tp, rms, 512, s);
   }
   const double *last_Tnm1 = nullptr, *last_Tn = nullptr;  // network-grid inputs of the last forward()
-  int target_rms_net(const double *tp, double *rms, int nb, hipStream_t s) {
-    MPDG_TRY(cnn_loss(nullptr, last_Tnm1, last_Tn, tp, n, stats.get(), 1.0, 1.0f, nullptr, lpart.get(), nb, s));
+  int target_rms_net(const double *bn, const double *tp, double *rms, int nb, hipStream_t s) {
+    MPDG_TRY(cnn_loss(nullptr, last_Tnm1, last_Tn, bn, tp, n, stats.get(), 1.0, 1.0f, nullptr, lpart.get(), nb, s));
     std::vector<double> h(nb);
     if (hipMemcpyAsync(h.data(), lpart.get(), nb * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
       return MPDG_ERR_DEVICE;
@@ -199,13 +202,16 @@ struct mpdg_learned {
     return MPDG_OK;
   }
   // loss and gradients of all layers (dW, db) after forward(); returns the mean squared error
-  int backward(const double *Tnm1, const double *Tn, const double *Tnp1, double *loss, hipStream_t s) {
+  int backward(const double *Tnm1, const double *Tn, const double *base, const double *Tnp1, double *loss,
+               hipStream_t s) {
     const int nb = 512;
     int rc = MPDG_OK;
     const double *a = rho == 1 ? Tnm1 : cf[0].get(), *t = rho == 1 ? Tn : cf[1].get();
     const double *tp = to_net(Tnp1, 4, s, &rc);
     MPDG_TRY(rc);
-    MPDG_TRY(cnn_loss(act[kLayers].get(), a, t, tp, n, stats.get(), oscale, 1.0f, g0.get(), lpart.get(), nb, s));
+    const double *bn = to_net(base, 5, s, &rc);
+    MPDG_TRY(rc);
+    MPDG_TRY(cnn_loss(act[kLayers].get(), a, t, bn, tp, n, stats.get(), oscale, 1.0f, g0.get(), lpart.get(), nb, s));
     unsigned short *g = g0.get(), *gn = g1.get();
     for (int l = kLayers - 1; l >= 0; --l) {
       MPDG_TRY(cnn_wgrad(g, act[l].get(), dims, part.get(), kWgradBlocks, dW[l].get(), db[l].get(), s));
@@ -226,8 +232,9 @@ struct mpdg_learned {
     }
     return MPDG_OK;
   }
-  // out = (2 Tn - Tnm1) + P corr after forward()
-  int predict(const double *Tnm1, const double *Tn, double *out, hipStream_t s) {
+  // out = T_b + P corr after forward() (T_b = base, or 2 Tn - Tnm1 when base is nullptr; out
+  // may be base)
+  int predict(const double *Tnm1, const double *Tn, const double *base, double *out, hipStream_t s) {
     MPDG_TRY(cnn_correction(act[kLayers].get(), n, stats.get(), oscale, corr.get(), s));
     const double *c = corr.get();
     if (rho > 1) {
@@ -237,7 +244,7 @@ struct mpdg_learned {
       MPDG_TRY(launch_ell_dir<double>(tmpa.get(), corr_f.get(), fdims, o1, 2, pw[2], pcol[2].get(), pval[2].get(), s));
       c = corr_f.get();
     }
-    return cnn_extrapolate_add(Tnm1, Tn, c, nf, out, s);
+    return cnn_extrapolate_add(Tnm1, Tn, base, c, nf, out, s);
   }
 };
 
@@ -307,19 +314,19 @@ int mpdg_learned_set_weights(mpdg_learned *h, const float *host) {
 }
 
 int mpdg_learned_predict(mpdg_learned *h, const double *Tnm1, const double *Tn, const double *q, const double *z,
-                         double *out, void *stream) {
+                         const double *base, double *out, void *stream) {
   if (!h || !Tnm1 || !Tn || !out) return MPDG_ERR_INVALID;
   MPDG_TRY(h->forward(Tnm1, Tn, q, z, S(stream)));
-  return h->predict(Tnm1, Tn, out, S(stream));
+  return h->predict(Tnm1, Tn, base, out, S(stream));
 }
 
 // gradients (FP32, same layout as the weights) and the loss of one sample, no update
 int mpdg_learned_gradients(mpdg_learned *h, const double *Tnm1, const double *Tn, const double *q, const double *z,
-                           const double *Tnp1, float *host_grad, double *loss, void *stream) {
+                           const double *base, const double *Tnp1, float *host_grad, double *loss, void *stream) {
   if (!h || !Tnm1 || !Tn || !Tnp1 || !host_grad) return MPDG_ERR_INVALID;
   hipStream_t s = S(stream);
   MPDG_TRY(h->forward(Tnm1, Tn, q, z, s));
-  MPDG_TRY(h->backward(Tnm1, Tn, Tnp1, loss, s));
+  MPDG_TRY(h->backward(Tnm1, Tn, base, Tnp1, loss, s));
   size_t o = 0;
   for (int l = 0; l < kLayers; ++l) {
     std::vector<float> dw(27 * 256), db(16);
@@ -335,20 +342,20 @@ int mpdg_learned_gradients(mpdg_learned *h, const double *Tnm1, const double *Tn
   return MPDG_OK;
 }
 
-// one Adam step on one sample (T_{n-1}, T_n, q, z) -> T_{n+1}; loss = mean squared error
+// one Adam step on one sample (T_{n-1}, T_n, q, z, T_b) -> T_{n+1}; loss = mean squared error
 // of the normalised correction before the update
 int mpdg_learned_train_step(mpdg_learned *h, const double *Tnm1, const double *Tn, const double *q, const double *z,
-                            const double *Tnp1, double lr, double *loss, void *stream) {
+                            const double *base, const double *Tnp1, double lr, double *loss, void *stream) {
   if (!h || !Tnm1 || !Tn || !Tnp1) return MPDG_ERR_INVALID;
   hipStream_t s = S(stream);
   MPDG_TRY(h->forward(Tnm1, Tn, q, z, s));
   if (!h->oscale_set) {
     double rms = 0.0;
-    MPDG_TRY(h->target_rms(Tnp1, &rms, s));
+    MPDG_TRY(h->target_rms(base, Tnp1, &rms, s));
     h->oscale = (std::isfinite(rms) && rms > 1e-12) ? rms : 1.0;
     h->oscale_set = true;
   }
-  MPDG_TRY(h->backward(Tnm1, Tn, Tnp1, loss, s));
+  MPDG_TRY(h->backward(Tnm1, Tn, base, Tnp1, loss, s));
   h->step += 1;
   for (int l = 0; l < kLayers; ++l)
     MPDG_TRY(cnn_adam(h->w[l].get(), h->mw[l].get(), h->vw[l].get(), h->dW[l].get(), h->db[l].get(), h->b[l].get(),

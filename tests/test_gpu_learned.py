@@ -60,6 +60,36 @@ def test_gradients_match_oracle(cuda, shape, degree, rho):
         assert a @ b / (np.linalg.norm(a) * np.linalg.norm(b)) > 0.995
 
 
+def test_base_guess_is_what_the_network_corrects(cuda):
+    """With an explicit base guess T_b the prediction is T_b plus the same learned correction;
+    the linear extrapolation as T_b reproduces the default, and the train step's loss (taken
+    before its update) is that of the target T_{n+1} - T_b."""
+    torch = cuda
+    import meltpooldg_amd as M
+    shape = (9, 13, 17)
+    Tnm1, Tn, q, z, Tnp1 = sample(shape, seed=8)
+    net, _ = make_net(M, shape, degree=2, rho=2, qscale=0.5, seed=5)
+    d = [to_dev(torch, a) for a in (Tnm1, Tn, q, z, Tnp1)]
+    out0, out1, out2 = (torch.empty_like(d[1]) for _ in range(3))
+    lin = 2 * d[1] - d[0]
+    other = lin + 0.3 * torch.sin(torch.arange(lin.numel(), device="cuda", dtype=torch.float64))
+    net.predict(*d[:4], out0)
+    net.predict(*d[:4], out1, base=lin)
+    net.predict(*d[:4], out2, base=other)
+    torch.cuda.synchronize()
+    assert torch.allclose(out1, out0, rtol=0, atol=1e-12 * float(out0.abs().max()))
+    assert torch.allclose(out2 - other, out0 - lin, rtol=0, atol=1e-12 * float(out0.abs().max()))
+    w = net.get_weights()
+    l_default = net.train_step(*d, 1e-3)
+    net.set_weights(w)
+    l_lin = net.train_step(*d, 1e-3, base=lin)
+    net.set_weights(w)
+    # T_{n+1} - other differs from T_{n+1} - lin, so the loss differs
+    l_other = net.train_step(*d, 1e-3, base=other)
+    assert abs(l_lin - l_default) <= 1e-9 * l_default
+    assert abs(l_other - l_default) > 1e-3 * l_default
+
+
 def test_training_reduces_loss(cuda):
     torch = cuda
     import meltpooldg_amd as M
@@ -98,9 +128,9 @@ def test_output_scale_is_set_by_the_first_sample(cuda):
 
 def test_trained_network_beats_the_predictors_without_it(cuda):
     """Q2 on 16^3 cells, laser scan: train on the pairs of steps 2..40, evaluate steps 41..60
-    from the same trajectory state. The learned guess starts CG closer to the solution than
-    linear extrapolation (the untrained network's guess) and than the previous step (the
-    fallback), and needs no more iterations."""
+    from the same trajectory state. The learned guess (correcting the linear extrapolation)
+    starts CG closer to the solution than linear extrapolation (its base, and the untrained
+    network's guess) and than the previous step, and needs no more iterations."""
     torch = cuda
     import meltpooldg_amd as M
     from meltpooldg_amd.application import LaserScan, generate_trajectory, train
@@ -113,7 +143,7 @@ def test_trained_network_beats_the_predictors_without_it(cuda):
     for name, pred, sm in (("none", M.PREDICTOR_NONE, 0), ("linear", M.PREDICTOR_LINEAR, 0),
                            ("learned", M.PREDICTOR_LEARNED, 0), ("learned_smoothed", M.PREDICTOR_LEARNED, 2)):
         case = LaserScan(k, cells, dt=dt, predictor=pred, learned=net if pred == M.PREDICTOR_LEARNED else None,
-                         guess_smoothing=sm)
+                         guess_smoothing=sm, extrapolation_order=1)
         # restart from the trajectory at step 39; one replayed step rebuilds the history (T_39, T_40)
         r0s, its, fbs = [], [], []
         case.t = 39 * dt
@@ -132,3 +162,47 @@ def test_trained_network_beats_the_predictors_without_it(cuda):
     assert res["learned"][1] <= res["linear"][1]
     # two Chebyshev steps of the finest multigrid level on an accepted guess lower its residual
     assert res["learned_smoothed"][2] == 0 and res["learned_smoothed"][0] < res["learned"][0]
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_learned_guess_beats_its_base_predictor_q2_32(cuda, order):
+    """Q2 on 32^3 cells (274,625 DoFs), laser scan with FP32 multigrid: the network is trained
+    on the solver's own steps 2..80 to correct the extrapolation of the given order (its base
+    guess), then steps 81..100 are replayed from the same state with that extrapolation alone
+    and with the network on top. The learned guess starts at a smaller residual than its base
+    on average and in every step (the guess check never falls back), and needs fewer
+    iterations on average."""
+    torch = cuda
+    import meltpooldg_amd as M
+    from meltpooldg_amd.application import LaserScan, generate_trajectory, train
+    k, cells, dt = 2, 32, 1e-3
+    pred = M.PREDICTOR_LINEAR if order == 1 else M.PREDICTOR_EXTRAPOLATION
+    _, Ts, qs, zs, _, bs = generate_trajectory(k, cells, 100, dt=dt, precond="gmg32", predictor=pred,
+                                               extrapolation_order=order, pre_smoother_degree=1, with_base=True)
+    net = M.LearnedPredictor((cells,) * 3, degree=k, qscale=dt * cells, seed=11)
+    losses = train(net, Ts[:81], qs[:80], zs[:80], epochs=12, lr=2e-3, bs=bs[:80])
+    res = {}
+    for name, p in (("base", pred), ("learned", M.PREDICTOR_LEARNED)):
+        case = LaserScan(k, cells, dt=dt, predictor=p, learned=net if p == M.PREDICTOR_LEARNED else None,
+                         extrapolation_order=order, pre_smoother_degree=1)
+        # the replayed steps before 81 rebuild the history the extrapolation uses
+        case.t = (80 - order) * dt
+        case.heat.set_initial_condition(Ts[80 - order])
+        for _ in range(order):
+            case.step()
+        r0s, its, fbs = [], [], []
+        for _ in range(20):
+            it, _, (fb, r0, r) = case.step()
+            assert r <= 1.0001e-10
+            r0s.append(r0)
+            its.append(it)
+            fbs.append(fb)
+        res[name] = (np.log10(r0s), np.array(its), int(sum(fbs)))
+    lb, ib, _ = res["base"]
+    ll, il, fl = res["learned"]
+    print("order", order, "loss", losses[0], losses[-1], "log10 r0 base/learned %.2f / %.2f" % (lb.mean(), ll.mean()),
+          "iterations base/learned %.2f / %.2f" % (ib.mean(), il.mean()), "fallbacks", fl)
+    assert losses[-1] < 0.5 * losses[0]
+    assert fl == 0
+    assert ll.mean() < lb.mean() - 0.3 and np.all(ll < lb)
+    assert il.mean() < ib.mean()
