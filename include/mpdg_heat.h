@@ -229,8 +229,8 @@ typedef struct mpdg_heat_operation_params {
   int guess_check;        /* learned predictor: fall back to its base guess (the extrapolation it
                              corrects) when the learned guess's residual is not smaller */
   int guess_smoothing;    /* learned predictor (with guess_check, linear step mode): smoothing
-                             steps on J T = b applied to an accepted guess (multigrid: Chebyshev
-                             steps of its finest level; 0: none) */
+                             steps on J T = b applied to the network's guess before the check
+                             (multigrid: Chebyshev steps of its finest level; 0: none) */
   int extrapolation_order;/* MPDG_PREDICTOR_EXTRAPOLATION, and the base guess that
                              MPDG_PREDICTOR_LEARNED corrects: 1..6 (the order drops to what the
                              history holds: after set_initial the first steps use fewer points) */
@@ -272,6 +272,11 @@ const double *mpdg_heat_operation_old_solution(const mpdg_heat_operation *h);
 
 /* nodal field of the z+ surface flux of create_rhs (zero off the z+ face), for the network input */
 int mpdg_heat_op_surface_flux_field(const mpdg_heat_op *op, double *out, void *stream);
+
+/* Bandwidth probe for measurements (no counterpart in the reference): kind 0 copies src to dst,
+   1 reads src (dst receives 4096 partial sums), 2 writes dst; n doubles (even), 16-byte aligned
+   device vectors, 16-byte accesses. Gives the attainable HBM rates next to the device's spec. */
+int mpdg_stream_probe(int kind, double *dst, const double *src, long n, void *stream);
 
 /* --- data formats on either side of the path --- */
 /* Paraview output as Postprocessor::do_paraview_output writes it

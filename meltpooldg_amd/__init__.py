@@ -130,6 +130,7 @@ _SIGNATURES = {
     "mpdg_heat_operation_record_guess_feature": ([_V, _V, _V], _I),
     "mpdg_heat_operation_last_step_info": ([_V, C.POINTER(_I), C.POINTER(_D), C.POINTER(_D)], _I),
     "mpdg_heat_op_surface_flux_field": ([_V, _V, _V], _I),
+    "mpdg_stream_probe": ([_I, _V, _V, _L, _V], _I),
     "mpdg_learned_create": ([C.POINTER(_I), _I, _I, _D, C.c_ulonglong, C.POINTER(_V)], _I),
     "mpdg_learned_grid": ([_V, C.POINTER(_I), C.POINTER(_I)], _I),
     "mpdg_learned_get_output_scale": ([_V, C.POINTER(_D)], _I),
@@ -165,6 +166,14 @@ def load_library(path=LIB_PATH):
         fn.restype = res
     _lib = L
     return L
+
+
+def stream_probe(kind, dst, src=None):
+    """Bandwidth probe on device FP64 vectors: 0 copy src -> dst, 1 read src (dst >= 4096
+    entries receives partial sums), 2 write dst. Returns nothing; time it with events."""
+    L = load_library()
+    n = src.numel() if kind != 2 else dst.numel()
+    _check(L.mpdg_stream_probe(kind, _ptr(dst), _ptr(src), n, _stream()), "stream probe")
 
 
 def exported_symbols():

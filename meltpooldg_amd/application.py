@@ -38,11 +38,13 @@ class LaserScan:
                                           coarse_degree=coarse_degree, smoother_degree=smoother_degree,
                                           pre_smoother_degree=pre_smoother_degree)
             solver = M.SOLVER_FCG
-        elif precond == "gmg64":
+        elif precond in ("gmg64", "gmg64_fcg"):
+            # FP64 multigrid inside CG, or inside flexible CG (for non-symmetric cycles, e.g. fewer
+            # pre- than post-smoothing steps)
             self.pc = M.GMGPreconditioner(self.op, precision=M.FP64, n_cells_coarse=n_cells_coarse,
                                           coarse_degree=coarse_degree, smoother_degree=smoother_degree,
                                           pre_smoother_degree=pre_smoother_degree)
-            solver = M.SOLVER_CG
+            solver = M.SOLVER_CG if precond == "gmg64" else M.SOLVER_FCG
         else:
             self.pc = M.JacobiPreconditioner(self.op)
             solver = M.SOLVER_CG

@@ -1040,4 +1040,49 @@ template int launch_eig_step<float>(int, float *, float *, float *, const float 
 template int launch_eig_step<double>(int, double *, double *, double *, const double *, const double *, long,
                                      const Reduction &, double *, double *, int, hipStream_t);
 
+// ---- bandwidth probes (benchmark helpers): copy, read-only and write-only streams over FP64
+//      vectors, 16-byte accesses, grid-stride over n/2 pairs (n even) ----
+__global__ __launch_bounds__(256) void stream_copy_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                          long m) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < m; i += (long)gridDim.x * 256) dst[i] = src[i];
+}
+__global__ __launch_bounds__(256) void stream_read_kernel(const double2 *__restrict__ src, double *__restrict__ part,
+                                                          long m) {
+  __shared__ double sh[256];
+  double a = 0.0;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < m; i += (long)gridDim.x * 256) {
+    const double2 v = src[i];
+    a += v.x + v.y;
+  }
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ __launch_bounds__(256) void stream_write_kernel(double2 *__restrict__ dst, long m, double v) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < m; i += (long)gridDim.x * 256) dst[i] = make_double2(v, v);
+}
+int launch_stream(int kind, double *dst, const double *src, long n, hipStream_t s) {
+  const long m = n / 2;
+  const int blocks = 256 * 16;  // 16 workgroups per CU on 256 CUs
+  switch (kind) {
+    case 0:
+      hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const double2 *>(src),
+                         reinterpret_cast<double2 *>(dst), m);
+      break;
+    case 1:
+      hipLaunchKernelGGL(stream_read_kernel, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const double2 *>(src),
+                         dst, m);
+      break;
+    case 2:
+      hipLaunchKernelGGL(stream_write_kernel, dim3(blocks), dim3(256), 0, s, reinterpret_cast<double2 *>(dst), m, 1.0);
+      break;
+    default: return MPDG_ERR_INVALID;
+  }
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
 }  // namespace mpdg

@@ -85,4 +85,8 @@ template <typename T>
 int launch_eig_step(int phase, T *r, T *z, T *p, const T *q, const T *inv, long n, const Reduction &red, double *sc,
                     double *hist, int k, hipStream_t s);
 
+// bandwidth probe: kind 0 dst = src, 1 read src (4096 block partial sums into dst), 2 write
+// dst; n even, 16-byte aligned vectors
+int launch_stream(int kind, double *dst, const double *src, long n, hipStream_t s);
+
 }  // namespace mpdg

@@ -721,7 +721,7 @@ int allow_apply_lds() {
 
 template <int N1, typename Number, bool MAPPED>
 int launch_apply_shape(const OperatorData<Number> &op, const Number *src, Number *dst, hipStream_t stream,
-                       hipEvent_t kernel_start) {
+                       hipEvent_t kernel_start, hipEvent_t kernel_end) {
   using G = ApplyGeom<N1, Number, MAPPED>;
   ApplyArgs<Number, N1> A;
   fill_apply_args(op, A);
@@ -756,6 +756,7 @@ int launch_apply_shape(const OperatorData<Number> &op, const Number *src, Number
       default: return MPDG_ERR_INVALID;
     }
   }
+  if (kernel_end) (void)hipEventRecord(kernel_end, stream);
   if (n_edge > 0)
     hipLaunchKernelGGL(edge_fixup_kernel<Number>, dim3((unsigned)((n_edge + 255) / 256)), dim3(256), 0, stream, dst,
                        op.edge, op.nd[0], (long)op.nd[0], (long)op.nd[0] * op.nd[1], (N1 - 1) * G::BY,
@@ -765,26 +766,28 @@ int launch_apply_shape(const OperatorData<Number> &op, const Number *src, Number
 
 template <int N1, typename Number>
 int launch_apply_n1(const OperatorData<Number> &op_in, const Number *src, Number *dst, hipStream_t stream,
-                    hipEvent_t kernel_start) {
+                    hipEvent_t kernel_start, hipEvent_t kernel_end) {
   const OperatorData<Number> op = kernel_view(op_in);
   if (op.metric == nullptr && op_in.metric != nullptr) return MPDG_ERR_INVALID;  // metric_k not computed
-  if (op.metric) return launch_apply_shape<N1, Number, true>(op, src, dst, stream, kernel_start);
-  return launch_apply_shape<N1, Number, false>(op, src, dst, stream, kernel_start);
+  if (op.metric) return launch_apply_shape<N1, Number, true>(op, src, dst, stream, kernel_start, kernel_end);
+  return launch_apply_shape<N1, Number, false>(op, src, dst, stream, kernel_start, kernel_end);
 }
 
 template <typename Number>
 int launch_apply(const OperatorData<Number> &op, const Number *src, Number *dst, hipStream_t stream,
-                 hipEvent_t kernel_start) {
+                 hipEvent_t kernel_start, hipEvent_t kernel_end) {
   switch (op.degree) {
-    case 1: return launch_apply_n1<2, Number>(op, src, dst, stream, kernel_start);
-    case 2: return launch_apply_n1<3, Number>(op, src, dst, stream, kernel_start);
-    case 3: return launch_apply_n1<4, Number>(op, src, dst, stream, kernel_start);
-    case 4: return launch_apply_n1<5, Number>(op, src, dst, stream, kernel_start);
+    case 1: return launch_apply_n1<2, Number>(op, src, dst, stream, kernel_start, kernel_end);
+    case 2: return launch_apply_n1<3, Number>(op, src, dst, stream, kernel_start, kernel_end);
+    case 3: return launch_apply_n1<4, Number>(op, src, dst, stream, kernel_start, kernel_end);
+    case 4: return launch_apply_n1<5, Number>(op, src, dst, stream, kernel_start, kernel_end);
     default: return MPDG_ERR_INVALID;
   }
 }
 
-template int launch_apply<double>(const OperatorData<double> &, const double *, double *, hipStream_t, hipEvent_t);
-template int launch_apply<float>(const OperatorData<float> &, const float *, float *, hipStream_t, hipEvent_t);
+template int launch_apply<double>(const OperatorData<double> &, const double *, double *, hipStream_t, hipEvent_t,
+                                  hipEvent_t);
+template int launch_apply<float>(const OperatorData<float> &, const float *, float *, hipStream_t, hipEvent_t,
+                                 hipEvent_t);
 
 }  // namespace mpdg

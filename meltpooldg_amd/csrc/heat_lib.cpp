@@ -47,9 +47,7 @@ int mpdg_heat_op::vmult_cells(double *dst, const double *src, hipStream_t s) con
     timing_events.push_back({a, b});
   }
   auto &ev = timing_events[timing_used++];
-  const int rc = launch_apply<double>(d64, src, dst, s, ev.first);
-  hipEventRecord(ev.second, s);
-  return rc;
+  return launch_apply<double>(d64, src, dst, s, ev.first, ev.second);
 }
 
 int mpdg_heat_op::refresh_metric_k(hipStream_t s) const {
@@ -1529,9 +1527,17 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
   MPDG_TRY(learned_guess(h, dt, h->bvec.get(), T, T_old, s));
   if (h->learned_applied && h->prm.guess_check) {
     // the learned guess (T) against the base guess it corrects (its residual norm came with the
-    // network's input); the residual of T is kept in alt
+    // network's input); the residual of T is kept in alt. With guess_smoothing the network's
+    // guess is first smoothed (the correction is prolonged from its working grid and carries
+    // high-frequency error, which the residual and CG weigh most), then checked.
     MPDG_TRY(op->vmult(h->alt.get(), T, s));
     MPDG_TRY(launch_axpby(h->alt.get(), 1.0, h->bvec.get(), -1.0, n, s));
+    if (h->prm.guess_smoothing > 0 && h->pc) {
+      MPDG_TRY(h->pc->smooth(h->du.get(), h->alt.get(), h->prm.guess_smoothing, s));
+      MPDG_TRY(launch_axpby(T, 1.0, h->du.get(), 1.0, n, s));
+      MPDG_TRY(op->vmult(h->alt.get(), T, s));
+      MPDG_TRY(launch_axpby(h->alt.get(), 1.0, h->bvec.get(), -1.0, n, s));
+    }
     MPDG_TRY(launch_dot<double, double>(h->alt.get(), h->alt.get(), n, h->red.red(), h->scal.get(), s));
     double rr[2];
     if (hipMemcpyAsync(rr, h->scal.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
@@ -1540,11 +1546,6 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
     if (!(rr[0] == rr[0]) || !(rr[0] < rr[1])) {
       MPDG_TRY(launch_convert<double, double>(h->tbase.get(), T, n, s));
       h->fell_back = 1;
-    } else if (h->prm.guess_smoothing > 0 && h->pc) {
-      // the network's correction is prolonged from its working grid; smoothing its error
-      // removes the high-frequency part that the residual (and CG) weighs most
-      MPDG_TRY(h->pc->smooth(h->du.get(), h->alt.get(), h->prm.guess_smoothing, s));
-      MPDG_TRY(launch_axpby(T, 1.0, h->du.get(), 1.0, n, s));
     }
   }
   mpdg_solver_params lp = h->prm.linear;
@@ -1848,6 +1849,12 @@ int mpdg_heat_operation_last_step_info(const mpdg_heat_operation *h, int *fell_b
   if (r0) *r0 = h->rel_res0;
   if (r) *r = h->rel_res;
   return MPDG_OK;
+}
+int mpdg_stream_probe(int kind, double *dst, const double *src, long n, void *stream) {
+  if (!dst || n < 2 || n % 2 || (kind != 2 && !src)) return MPDG_ERR_INVALID;
+  if (reinterpret_cast<uintptr_t>(dst) % 16 || reinterpret_cast<uintptr_t>(src) % 16) return MPDG_ERR_INVALID;
+  if (kind == 1 && n < 4096) return MPDG_ERR_INVALID;  // dst holds the 4096 partial sums
+  return launch_stream(kind, dst, src, n, S(stream));
 }
 int mpdg_heat_op_surface_flux_field(const mpdg_heat_op *op, double *out, void *stream) {
   if (!op || !out) return MPDG_ERR_INVALID;

@@ -116,11 +116,12 @@ void fill_apply_args(const OperatorData<Number> &op, ApplyArgs<Number, N1> &A) {
   A.edge = op.edge;
 }
 
-// heat_apply.hip; `kernel_start`, when given, is recorded on the stream right before the
-// cell kernel (after the face-row pre-pass) so that an event pair times that kernel alone
+// heat_apply.hip; `kernel_start` and `kernel_end`, when given, are recorded on the stream
+// right before and after the cell kernel (between the face-row pre-pass and the edge fix-up)
+// so that the event pair times that kernel alone
 template <typename Number>
 int launch_apply(const OperatorData<Number> &op, const Number *src, Number *dst, hipStream_t stream,
-                 hipEvent_t kernel_start = nullptr);
+                 hipEvent_t kernel_start = nullptr, hipEvent_t kernel_end = nullptr);
 // entries of OperatorData::edge the apply of a mesh of degree k and n_cells cells needs with
 // the Cartesian (mapped = 0) or mapped (1) chunk shape, or the larger of the two (mapped = -1);
 // 0 when no DoF row is shared by four columns
