@@ -37,11 +37,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the apply kernel at Q3 128^3 (per-qp k), from two rocprofv3 --pmc runs
-# (FETCH_SIZE, WRITE_SIZE) scaled by a device copy of known size; the summary with the commands
-# is profiles/heat_apply_q3_128_fp64_hbm_traffic.md. Counters need their own profiler runs, so
-# the bench reports the profiled figure for that workload and null for others.
-PROFILED_TRAFFIC = {(3, 128): 2.499e+09}
+# HBM bytes per launch of the apply kernel (degree, cells, conductivity, mapped), from two
+# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by the library's streams of known size;
+# the summaries with the commands are under profiles/ (heat_apply_*). Counters need their own
+# profiler runs, so the bench reports the profiled figure for these workloads and null for others.
+PROFILED_TRAFFIC = {(3, 128, "qp", False): 2.506e+09, (4, 96, "qp", False): 2.360e+09,
+                    (4, 96, "law", False): 2.292e+09, (3, 96, "const", True): 3.759e+09}
 
 
 def parse():
@@ -51,6 +52,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--degree", type=int, default=3)
+    ap.add_argument("--kappa", default="qp", choices=("qp", "law", "const"),
+                    help="conductivity of the benchmark apply: per quadrature point, law from T, constant")
+    ap.add_argument("--mapped", type=int, default=0, help="apply on the curved box x + 0.05 sin(2 pi y) sin(2 pi z)")
     ap.add_argument("--solve", type=int, default=1)
     ap.add_argument("--cpu", type=int, default=1)
     ap.add_argument("--cpu-cells", type=int, default=64)
@@ -80,17 +84,32 @@ def parse():
     return ap.parse_args()
 
 
-def make_operator(torch, M, k, N, dt=1e-3):
-    n1 = k + 1
+def make_operator(torch, M, k, N, dt=1e-3, kappa="qp", mapped=False):
+    """The benchmark operator: k per quadrature point from T ~ U[0,2] (qp), the law evaluated from
+    a T DoF vector ~ U[0,2] inside the apply (law), or k = 1 (const); mapped: the curved box
+    x + 0.05 sin(2 pi y) sin(2 pi z) on all nodes. Returns the operator and the arrays it reads."""
     op = M.HeatDiffuseMultiPhaseOperator(k, (N, N, N), (1.0, 1.0, 1.0), rho_cp=1.0)
     op.reset_time_increment(dt)
     g = torch.Generator(device="cuda")
     g.manual_seed(2)
-    Tq = torch.rand(op.n_q_points, dtype=torch.float64, device="cuda", generator=g) * 2.0
-    kq = 1.0 * (1.0 + 0.5 * torch.tanh((Tq - 1.0) / 0.1))
-    del Tq
-    op.set_conductivity_qp(kq)
-    return op, kq
+    keep = []
+    if kappa == "qp":
+        Tq = torch.rand(op.n_q_points, dtype=torch.float64, device="cuda", generator=g) * 2.0
+        kq = 1.0 * (1.0 + 0.5 * torch.tanh((Tq - 1.0) / 0.1))
+        del Tq
+        op.set_conductivity_qp(kq)
+        keep.append(kq)
+    elif kappa == "law":
+        T = torch.rand(op.n_dofs, dtype=torch.float64, device="cuda", generator=g) * 2.0
+        op.set_conductivity_law(M.LAW_TANH, (1.0, 0.5, 1.0, 0.1), T)
+        keep.append(T)
+    else:
+        op.set_conductivity_constant(1.0)
+    if mapped:
+        X = M.node_coordinates(k, (N, N, N), mapping_amplitude=0.05)
+        op.set_mapping(X)
+        keep.append(X)
+    return op, keep
 
 
 def cpu_apply(k, N, threads, seconds=5.0):
@@ -397,7 +416,7 @@ def main():
     import torch
     import meltpooldg_amd as M
     k, N = a.degree, a.cells
-    op, kq = make_operator(torch, M, k, N)
+    op, kq = make_operator(torch, M, k, N, kappa=a.kappa, mapped=bool(a.mapped))
     g = torch.Generator(device="cuda")
     g.manual_seed(3)
     src = torch.rand(op.n_dofs, dtype=torch.float64, device="cuda", generator=g) * 2.0 - 1.0
@@ -417,7 +436,18 @@ def main():
     ms_step = el * 1e3 / a.steps
     kern_avg = kern_ms / max(nlaunch, 1)
     gdofs = op.n_dofs / (ms_step * 1e-3) / 1e9
-    bytes_alg = 16.0 * op.n_dofs + 8.0 * op.n_q_points
+    # algorithmic bytes per apply: src + dst, the conductivity data, the mapping nodes
+    bytes_alg = 16.0 * op.n_dofs + {"qp": 8.0 * op.n_q_points, "law": 8.0 * op.n_dofs, "const": 0.0}[a.kappa]
+    if a.mapped:
+        bytes_alg += 24.0 * op.n_dofs
+    kname = "heat_apply_kernel<%d,double,%s%s>" % (k + 1, {"qp": "KAPPA_QP", "law": "KAPPA_LAW",
+                                                          "const": "KAPPA_CONST"}[a.kappa],
+                                                   ",mapped" if a.mapped else "")
+    if a.mapped and a.kappa != "const":
+        kname = "heat_apply_kernel<%d,double,KAPPA_CONST,mapped> (conductivity folded into the metric)" % (k + 1)
+    kdesc = {"qp": "k per quadrature point = k0(1+0.5 tanh((T-1)/0.1)), T~U[0,2]",
+             "law": "k = k0(1+0.5 tanh((T-1)/0.1)) evaluated in the apply from a T DoF vector, T~U[0,2]",
+             "const": "k = 1"}[a.kappa] + (", mesh mapped by x + 0.05 sin(2 pi y) sin(2 pi z)" if a.mapped else "")
     achieved = bytes_alg / (kern_avg * 1e-3) / 1e9
     line = {
         "metric": "GDoF/s operator apply; time-to-solve per heat step at 1e-10 (learned+FP32 vs FP64)",
@@ -432,12 +462,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (torch.Generator seeds 2 and 3 on the GPU)",
-        "config": {"workload": "FP64 heat operator apply, Q%d hex, %d^3 cells, %d DoFs, QGauss(%d), k per quadrature "
-                               "point = k0(1+0.5 tanh((T-1)/0.1)), T~U[0,2]" % (k, N, op.n_dofs, k + 1),
+        "config": {"workload": "FP64 heat operator apply, Q%d hex, %d^3 cells, %d DoFs, QGauss(%d), %s"
+                               % (k, N, op.n_dofs, k + 1, kdesc),
                    "degree": k, "cells": N, "n_dofs": op.n_dofs, "n_q_points": op.n_q_points},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": PROFILED_TRAFFIC.get((k, N)),
-                     "kernel": "heat_apply_kernel<4,double,KAPPA_QP>", "kernel_avg_ms": round(kern_avg, 5),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": PROFILED_TRAFFIC.get((k, N, a.kappa, bool(a.mapped))),
+                     "kernel": kname, "kernel_avg_ms": round(kern_avg, 5),
                      "algorithmic_bytes": bytes_alg},
     }
     # the library's own streams over a DoF vector's size: the attainable HBM rates next to the

@@ -168,6 +168,30 @@ def load_library(path=LIB_PATH):
     return L
 
 
+def node_coordinates(degree, n_cells, extent=(1.0, 1.0, 1.0), mapping_amplitude=0.0):
+    """Coordinates of the Q_degree nodes (Gauss-Lobatto points in every cell, lexicographic) as one
+    device FP64 vector (X block, Y block, Z block), the layout set_mapping takes. With
+    mapping_amplitude a, every node is moved to x + a sin(2 pi y / L_y) sin(2 pi z / L_z) (on the
+    unit cube the curved box of the mapped benchmark mesh, x + a sin(2 pi y) sin(2 pi z))."""
+    import numpy as np
+    torch = _torch()
+    if degree == 1:
+        gl = np.array([-1.0, 1.0])
+    else:
+        P = np.polynomial.legendre.Legendre.basis(degree).deriv()
+        gl = np.concatenate([[-1.0], np.sort(P.roots().real), [1.0]])
+    gl = 0.5 * (gl + 1.0)
+    axes = []
+    for d in range(3):
+        h = extent[d] / n_cells[d]
+        pts = np.concatenate([c * h + gl[:-1] * h for c in range(n_cells[d])] + [np.array([extent[d]])])
+        axes.append(torch.tensor(pts, dtype=torch.float64, device="cuda"))
+    Z, Y, X = torch.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    if mapping_amplitude:
+        X = X + mapping_amplitude * torch.sin(2 * np.pi * Y / extent[1]) * torch.sin(2 * np.pi * Z / extent[2])
+    return torch.cat([X.reshape(-1), Y.reshape(-1), Z.reshape(-1)]).contiguous()
+
+
 def stream_probe(kind, dst, src=None):
     """Bandwidth probe on device FP64 vectors: 0 copy src -> dst, 1 read src (dst >= 4096
     entries receives partial sums), 2 write dst. Returns nothing; time it with events."""

@@ -47,6 +47,23 @@ def build(jobs=None):
     return LIB
 
 
+def build_abi_smoke():
+    """Compiles tests/cxx/abi_smoke.cpp, a C++ caller of include/mpdg_heat.h, against the built
+    library (the header compiled as C++ by a program that is not this package)."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cxx", "abi_smoke.cpp")
+    out = os.path.join(root, "tests", "cxx", "abi_smoke")
+    if not os.path.exists(src):
+        return None
+    cmd = [HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-x", "hip", src, "-o", out + ".tmp",
+           "-L" + HERE, "-lmpdg_heat", "-Wl,-rpath,$ORIGIN/../../meltpooldg_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("abi_smoke build failed:\n" + r.stderr[-4000:])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 if __name__ == "__main__":
     print(build())
     sys.exit(0)

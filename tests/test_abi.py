@@ -36,6 +36,17 @@ def test_library_exports_every_declared_symbol():
     assert set(M.exported_symbols()) <= set(declared_functions())
 
 
+@pytest.mark.parametrize("lang,compiler", [("c", "gcc"), ("c++", "g++")])
+def test_header_compiles_as_c_and_cxx(lang, compiler):
+    """include/mpdg_heat.h is a self-contained C header (and valid C++ with its extern "C" guard)."""
+    import subprocess
+    h = os.path.join(ROOT, "include", "mpdg_heat.h")
+    std = "-std=c99" if lang == "c" else "-std=c++17"
+    r = subprocess.run([compiler, "-fsyntax-only", "-Wall", "-Wextra", "-Werror", std, "-x", lang, h],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def test_python_mirror_fails_loudly_without_gpu():
     import meltpooldg_amd as M
     try:
