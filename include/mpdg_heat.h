@@ -79,7 +79,9 @@ int mpdg_heat_op_set_robin(mpdg_heat_op *op, int face_mask, double alpha, double
 /* mapped geometry: node coordinates of MappingQ(k) on the FE's support points (device FP64,
    3*n_dofs: X block, Y block, Z block; read when called and by multigrid set-up, kept by the
    caller). Stores the per-quadrature-point metric JxW and JxW J^-1 J^-T like MatrixFree does for
-   non-affine cells (scratch_data.cpp:192-266); boundary-face terms need the affine box. */
+   non-affine cells (scratch_data.cpp:192-266). Robin faces and the surface flux take their face
+   geometry (surface JxW, the flux's (x, y)) from the same coordinates, so the caller keeps them
+   alive while the operator is used. */
 int mpdg_heat_op_set_mapping(mpdg_heat_op *op, const double *coords);
 /* Gaussian surface flux on face z+ used by create_rhs: q = peak*exp(-((x-x0-vx*t)^2+(y-y0)^2)/(2 sigma^2)),
    p = {peak, sigma, x0, y0, vx, t}; peak = 0 switches it off. The reference's
@@ -113,6 +115,16 @@ typedef struct mpdg_heat_material {
 /* do_solidification: add the d/dT terms of the material to the tangent (:1044-1056) */
 int mpdg_heat_op_set_tangent(mpdg_heat_op *op, const mpdg_heat_material *mat, int do_solidification,
                              const double *velocity3);
+/* Velocity as a DoF field instead of the constant vector of mpdg_heat_op_set_tangent: the
+   reference's `velocity` vector of the operator (HeatDiffuseMultiPhaseOperator constructor,
+   heat_diffuse_operator.cpp:51-65, read with vel_eval at :449-452 and :975-978). velocity: device
+   FP64, 3 * n_dofs values (x block, y block, z block) on the temperature's Q_k nodes (the
+   reference's default, where the velocity's FE_Q degree is the heat degree), or NULL to go back
+   to the constant vector. The library keeps the pointer: the field is read by every
+   mpdg_heat_op_bind_temperature and create_rhs until it is replaced, so the caller keeps it alive
+   and may update its values between steps. Requires mpdg_heat_op_set_tangent (MPDG_ERR_INVALID
+   otherwise); clear_tangent drops it; the tangent must be bound again afterwards. */
+int mpdg_heat_op_set_velocity_field(mpdg_heat_op *op, const double *velocity);
 int mpdg_heat_op_clear_tangent(mpdg_heat_op *op);
 /* evaluates the tangent's per-quadrature-point data at T (and T_old for the solidification
    term; both device FP64 DoF vectors, read now) -- the reference reads them inside every
@@ -270,7 +282,8 @@ int mpdg_heat_operation_last_step_info(const mpdg_heat_operation *h, int *fell_b
 const double *mpdg_heat_operation_solution(const mpdg_heat_operation *h);
 const double *mpdg_heat_operation_old_solution(const mpdg_heat_operation *h);
 
-/* nodal field of the z+ surface flux of create_rhs (zero off the z+ face), for the network input */
+/* nodal field of the z+ surface flux of create_rhs (zero off the z+ face; on mapped meshes at the
+   nodes' mapped (x, y)), for the network input */
 int mpdg_heat_op_surface_flux_field(const mpdg_heat_op *op, double *out, void *stream);
 
 /* Bandwidth probe for measurements (no counterpart in the reference): kind 0 copies src to dst,

@@ -91,6 +91,7 @@ _SIGNATURES = {
     "mpdg_io_write_pvd": ([C.c_char_p, _I, C.POINTER(_D), C.POINTER(C.c_char_p)], _I),
     "mpdg_io_dof_permutation": ([_I, C.POINTER(_I), C.POINTER(_D), C.POINTER(_D), _L, C.POINTER(_L)], _I),
     "mpdg_io_permute": ([_V, _V, _V, _L, _I, _V], _I),
+    "mpdg_heat_op_set_velocity_field": ([_V, _V], _I),
     "mpdg_heat_op_clear_tangent": ([_V], _I),
     "mpdg_heat_op_bind_temperature": ([_V, _V, _V, _V], _I),
     "mpdg_heat_op_destroy": ([_V], _I),
@@ -320,7 +321,18 @@ class HeatDiffuseMultiPhaseOperator:
         v = (C.c_double * 3)(*velocity)
         _check(self.lib.mpdg_heat_op_set_tangent(self.h, C.byref(m), 1 if do_solidification else 0, v), "set_tangent")
 
+    def set_velocity_field(self, velocity):
+        """Velocity DoF field for the tangent (heat_diffuse_operator.cpp:975-978): a float64 GPU tensor of
+        3 n_dofs values (x, y, z blocks) on the temperature's nodes, or None for the constant vector of
+        set_tangent. Read by every bind_temperature and create_rhs; kept alive by this object."""
+        if velocity is not None and velocity.numel() != 3 * self.n_dofs:
+            raise ValueError("velocity field: expected 3 * n_dofs values")
+        self._keep["velocity"] = velocity
+        _check(self.lib.mpdg_heat_op_set_velocity_field(self.h, _ptr(velocity, _torch().float64)),
+               "set_velocity_field")
+
     def clear_tangent(self):
+        self._keep.pop("velocity", None)
         _check(self.lib.mpdg_heat_op_clear_tangent(self.h), "clear_tangent")
 
     def bind_temperature(self, T, T_old=None):

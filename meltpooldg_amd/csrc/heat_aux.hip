@@ -333,12 +333,17 @@ __global__ __launch_bounds__(256) void diag_dirichlet_kernel(Number *__restrict_
 // mode 1: its diagonal
 // mode 2: boundary residual -alpha (T - T_inf) - eps sigma (T^4 - T_inf^4) on Robin faces plus the
 //         Gaussian surface flux on z+ (heat_diffuse_operator.cpp:721-781)
+// On mapped meshes (coords set) the face geometry comes from the mapping's node coordinates on
+// the face: JxW_q = |dx/dxi_a x dx/dxi_b| w_qa w_qb and the laser sees the mapped (x, y) of the
+// point (FEFaceValues with MappingQ, which MatrixFree's face data hold).
 struct FaceArgs {
   AuxTables<double> t;
   double qp[kMaxN1];
   int n1, nc[3], nd[3], robin_mask, face_mask, dmask, mode, flux_on;
   double h[3], alpha, emissivity, T_inf_conv, T_inf_rad;
   double flux[6];  // peak, sigma, x0, y0, vx, t
+  const double *coords;  // null: the affine box
+  long n_dofs;
 };
 
 __global__ __launch_bounds__(256) void boundary_face_kernel(FaceArgs A, const double *__restrict__ src,
@@ -378,14 +383,29 @@ __global__ __launch_bounds__(256) void boundary_face_kernel(FaceArgs A, const do
     for (int qb = 0; qb < n1; ++qb)
       for (int qa = 0; qa < n1; ++qa) {
         double Tq = 0.0, uq = 0.0;
+        double dxa[3] = {0.0, 0.0, 0.0}, dxb[3] = {0.0, 0.0, 0.0}, xq[3] = {0.0, 0.0, 0.0};
         for (int xb = 0; xb < n1; ++xb)
           for (int xa = 0; xa < n1; ++xa) {
             const double phi = A.t.S[qa][xa] * A.t.S[qb][xb];
             const long g = dof(xa, xb);
             if (T) Tq += phi * T[g];
             if (A.mode == 0 && !constrained(g)) uq += phi * src[g];
+            if (A.coords) {
+              const double ga = A.t.G[qa][xa] * A.t.S[qb][xb], gb = A.t.S[qa][xa] * A.t.G[qb][xb];
+              for (int c = 0; c < 3; ++c) {
+                const double X = A.coords[c * A.n_dofs + g];
+                dxa[c] += ga * X;
+                dxb[c] += gb * X;
+                xq[c] += phi * X;
+              }
+            }
           }
-        const double JxW = A.h[ta] * A.h[tb] * A.t.w[qa] * A.t.w[qb];
+        double JxW = A.h[ta] * A.h[tb] * A.t.w[qa] * A.t.w[qb];
+        if (A.coords) {
+          const double c0 = dxa[1] * dxb[2] - dxa[2] * dxb[1], c1 = dxa[2] * dxb[0] - dxa[0] * dxb[2],
+                       c2 = dxa[0] * dxb[1] - dxa[1] * dxb[0];
+          JxW = sqrt(c0 * c0 + c1 * c1 + c2 * c2) * A.t.w[qa] * A.t.w[qb];
+        }
         const double phiI = A.t.S[qa][ia] * A.t.S[qb][ib];
         double val = 0.0;
         if (A.mode == 2) {
@@ -399,6 +419,8 @@ __global__ __launch_bounds__(256) void boundary_face_kernel(FaceArgs A, const do
             pos[d] = (c[d] + s) * A.h[d];
             pos[ta] = (c[ta] + A.qp[qa]) * A.h[ta];
             pos[tb] = (c[tb] + A.qp[qb]) * A.h[tb];
+            if (A.coords)
+              for (int e = 0; e < 3; ++e) pos[e] = xq[e];
             const double dx = pos[0] - A.flux[2] - A.flux[4] * A.flux[5], dy = pos[1] - A.flux[3];
             val += A.flux[0] * exp(-(dx * dx + dy * dy) / (2.0 * A.flux[1] * A.flux[1]));
           }
@@ -434,7 +456,8 @@ int launch_boundary(const OperatorData<double> &op, int mode, const double *flux
   // a Neumann flux and a radiation/convection term on the same face are refused, as the
   // reference's rhs_boundary_loop does (heat_diffuse_operator.cpp:739-743)
   if (A.flux_on && (op.robin_mask & 32)) return MPDG_ERR_INVALID;
-  if (op.metric) return MPDG_ERR_INVALID;  // boundary faces are implemented for the affine box
+  A.coords = op.coords;
+  A.n_dofs = op.n_dofs;
   A.dmask = op.dmask;
   A.mode = mode;
   A.alpha = op.alpha;
@@ -610,15 +633,21 @@ template int launch_metric<double>(const OperatorData<double> &, const double *,
 template int launch_metric<float>(const OperatorData<float> &, const double *, float *, hipStream_t);
 
 // nodal values of the z+ surface flux q = peak exp(-((x - x0 - vx t)^2 + (y - y0)^2)/(2 sigma^2)), 0 elsewhere
+// the flux at the nodes of face z+ (0 elsewhere); mapped meshes: at the nodes' mapped (x, y)
 __global__ __launch_bounds__(256) void surface_flux_field_kernel(double *__restrict__ out, int nd0, int nd1, int nd2,
-                                                                 int k, double h0, double h1, FluxNodes fn) {
+                                                                 int k, double h0, double h1, FluxNodes fn,
+                                                                 const double *__restrict__ coords) {
   const long n = (long)nd0 * nd1 * nd2;
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
     const int ix = i % nd0, iy = (i / nd0) % nd1, iz = i / ((long)nd0 * nd1);
     double v = 0.0;
     if (iz == nd2 - 1) {
       const int cx = min(ix / k, (nd0 - 1) / k - 1), cy = min(iy / k, (nd1 - 1) / k - 1);
-      const double x = (cx + fn.nodes[ix - k * cx]) * h0, y = (cy + fn.nodes[iy - k * cy]) * h1;
+      double x = (cx + fn.nodes[ix - k * cx]) * h0, y = (cy + fn.nodes[iy - k * cy]) * h1;
+      if (coords) {
+        x = coords[i];
+        y = coords[n + i];
+      }
       const double dx = x - fn.p[2] - fn.p[4] * fn.p[5], dy = y - fn.p[3];
       v = fn.p[0] * exp(-(dx * dx + dy * dy) / (2.0 * fn.p[1] * fn.p[1]));
     }
@@ -631,7 +660,7 @@ int launch_surface_flux_field(const OperatorData<double> &op, const double *p6, 
   for (int i = 0; i < fe.n1; ++i) fn.nodes[i] = fe.nodes[i];
   for (int i = 0; i < 6; ++i) fn.p[i] = p6[i];
   hipLaunchKernelGGL(surface_flux_field_kernel, dim3(std::min<long>((op.n_dofs + 255) / 256, 8192)), dim3(256), 0, s,
-                     out, op.nd[0], op.nd[1], op.nd[2], op.degree, op.h[0], op.h[1], fn);
+                     out, op.nd[0], op.nd[1], op.nd[2], op.degree, op.h[0], op.h[1], fn, op.coords);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 

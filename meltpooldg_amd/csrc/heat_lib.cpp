@@ -39,7 +39,10 @@ int mpdg_heat_op::vmult_cells(double *dst, const double *src, hipStream_t s) con
   if (precision != MPDG_FP64) return MPDG_ERR_INVALID;
   MPDG_TRY(ensure_edge());
   if (d64.metric && d64.kappa_mode == KAPPA_LAW) MPDG_TRY(refresh_metric_k(s));  // k(T) with the current T
-  if (tangent) return td_bound ? launch_tangent_apply(d64, td.get(), vel, solid, src, dst, s) : MPDG_ERR_INVALID;
+  if (tangent) {
+    if (!td_bound) return MPDG_ERR_INVALID;
+    return launch_tangent_apply(d64, td.get(), tangent_convects(vel, vfield), solid, src, dst, s);
+  }
   if (!timing_on) return launch_apply<double>(d64, src, dst, s);
   if (timing_used == timing_events.size()) {
     hipEvent_t a, b;
@@ -90,7 +93,7 @@ int mpdg_heat_op::vmult_f32(float *dst, const float *src, hipStream_t s) const {
 int mpdg_heat_op::diagonal64(double *diag, hipStream_t s) const {
   if (tangent) {
     if (!td_bound) return MPDG_ERR_INVALID;
-    MPDG_TRY(launch_tangent_diagonal(d64, td.get(), vel, solid, diag, s));
+    MPDG_TRY(launch_tangent_diagonal(d64, td.get(), tangent_convects(vel, vfield), solid, diag, s));
     if (d64.robin_mask) MPDG_TRY(launch_boundary(d64, 1, nullptr, diag, nullptr, T_robin, s));
     return MPDG_OK;
   }
@@ -119,7 +122,7 @@ int mpdg_heat_op::create_rhs(double *dst, const double *T, const double *T_old, 
   if (dt == 0.0) return MPDG_ERR_ZERO_TIME_INCREMENT;
   if (tangent) {
     // residual with the tangent's material and velocity (heat_diffuse_operator.cpp:467-527)
-    MPDG_TRY(launch_tangent_residual(d64, dt, mat, vel, T, T_old, q, dst, s));
+    MPDG_TRY(launch_tangent_residual(d64, dt, mat, vel, vfield, T, T_old, q, dst, s));
     return launch_boundary(d64, 2, flux, dst, nullptr, T, s);
   }
   const long n = d64.n_dofs;
@@ -1145,9 +1148,17 @@ int mpdg_heat_op_set_tangent(mpdg_heat_op *op, const mpdg_heat_material *mat, in
   return MPDG_OK;
 }
 
+int mpdg_heat_op_set_velocity_field(mpdg_heat_op *op, const double *velocity) {
+  if (!op || !op->tangent) return MPDG_ERR_INVALID;
+  op->vfield = velocity;
+  op->td_bound = false;
+  return MPDG_OK;
+}
+
 int mpdg_heat_op_clear_tangent(mpdg_heat_op *op) {
   if (!op) return MPDG_ERR_INVALID;
   op->tangent = false;
+  op->vfield = nullptr;
   op->td_bound = false;
   op->td.release();
   return MPDG_OK;
@@ -1156,7 +1167,7 @@ int mpdg_heat_op_clear_tangent(mpdg_heat_op *op) {
 int mpdg_heat_op_bind_temperature(mpdg_heat_op *op, const double *T, const double *T_old, void *stream) {
   if (!op || !T || !op->tangent) return MPDG_ERR_INVALID;
   if (op->dt == 0.0) return MPDG_ERR_ZERO_TIME_INCREMENT;
-  MPDG_TRY(launch_tangent_data(op->d64, op->dt, op->mat, op->solid, op->vel, T, T_old ? T_old : T, op->td.get(),
+  MPDG_TRY(launch_tangent_data(op->d64, op->dt, op->mat, op->solid, op->vel, op->vfield, T, T_old ? T_old : T, op->td.get(),
                                S(stream)));
   op->td_bound = true;
   return MPDG_OK;
@@ -1226,7 +1237,7 @@ int mpdg_heat_op_vmult_noalloc(const mpdg_heat_op *op, double *dst, const double
   if (op->tangent) {
     // the full tangent's kernels use no scratch
     if (!op->td_bound) return MPDG_ERR_INVALID;
-    MPDG_TRY(launch_tangent_apply(op->d64, op->td.get(), op->vel, op->solid, src, dst, s));
+    MPDG_TRY(launch_tangent_apply(op->d64, op->td.get(), tangent_convects(op->vel, op->vfield), op->solid, src, dst, s));
   } else {
     if (op->d64.metric && op->d64.kappa_mode == KAPPA_LAW) {
       // k(T) on a mapped mesh rescales the metric into operator-owned arrays: only when they

@@ -157,6 +157,7 @@ struct mpdg_heat_op {
   mpdg_heat_material mat{};
   int solid = 0;
   double vel[3] = {0, 0, 0};
+  const double *vfield = nullptr;  // velocity DoF field (mpdg_heat_op_set_velocity_field), caller-owned
   mpdg::DeviceArray<double> td;
   bool td_bound = false;
   // optional HIP-event timing of the cell-apply kernel (bench / profiling)

@@ -1,16 +1,17 @@
 // Full Newton tangent of the heat residual on Cartesian cells: temperature-dependent
-// material (liquid/solid mushy zone) and a prescribed velocity
-// (HeatDiffuseMultiPhaseOperator::tangent_local_cell_operation with do_solidification and a
-// velocity, source/heat/heat_diffuse_operator.cpp:1028-1056; residual :467-527).
+// material (liquid/solid mushy zone) and a prescribed velocity, constant or a velocity DoF field
+// on the temperature's nodes (HeatDiffuseMultiPhaseOperator::tangent_local_cell_operation with
+// do_solidification and a velocity, source/heat/heat_diffuse_operator.cpp:975-978, :1028-1056;
+// residual :449-452, :467-527).
 //
 // Per quadrature point the tangent is val = a u + c . grad u, flux = k grad u + b u. The
-// coefficients depend on T and T_old only, so they are evaluated once per Newton iteration
-// (tangent_data_kernel) into kTangentCoeffs values per point, with the quadrature weight, the
-// cell volume and the 1/h factors folded in:
+// coefficients depend on T, T_old and the velocity only, so they are evaluated once per Newton
+// iteration (tangent_data_kernel) into kTangentCoeffs values per point, with the quadrature
+// weight, the cell volume and the 1/h factors folded in (v = the velocity at the point):
 //   t0 = JxW (rho_cp/dt [+ d rho_cp/dT ((T - T_old)/dt + grad T . v)])
 //   t1 = JxW k                                           (times 1/h_d^2 d u / d xi_d)
-//   t2 = JxW rho_cp           [velocity]                 (times sum_d v_d/h_d d u / d xi_d)
-//   t3..t5 = JxW dk/dT (grad T)_d / h_d  [solidification] (gradient slot, times u)
+//   t2..t4 = JxW rho_cp v_d / h_d          [velocity]     (times d u / d xi_d)
+//   t5..t7 = JxW dk/dT (grad T)_d / h_d  [solidification] (gradient slot, times u)
 // Coefficients a problem does not have (no velocity, no solidification) are neither written
 // nor read. The apply, residual and diagonal kernels are cell-local sum factorization with one
 // thread per line of the cell's N1^3 tensor (N1^2 threads per cell) and LDS transposes between
@@ -50,8 +51,7 @@ struct Geometry {
   int nd[3];
   int dmask;
   double h[3], ih2[3], vol, dt;
-  double cv[3];     // v_d / h_d
-  int conv, solid;  // coefficient t2 / t3..t5 present
+  int conv, solid;  // coefficients t2..t4 / t5..t7 present
 };
 
 __device__ __forceinline__ bool is_constrained(int m, const int *nd, int ix, int iy, int iz) {
@@ -333,8 +333,31 @@ struct TangentArgs {
   Geometry g;
   mpdg_heat_material mat;
   double vel[3];
+  const double *vf;  // velocity DoF field (3 blocks of n_dofs) or null: vel
   int solid;
 };
+
+// velocity at the quadrature points (qz = 0..N1-1 of this thread's (qy, qx) column): the
+// constant vector, or the field's values (gathered into lds[0], evaluated through lds[1], lds[3])
+template <int N1>
+__device__ void velocity_at_points(const Tables1D &t, const TangentArgs &A, const Place<N1> &p, double (*lds)[kCells<N1> * N1 * N1 * N1],
+                                   int o, double (&w)[3][N1]) {
+  if (!A.vf) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < N1; ++q) w[c][q] = A.vel[c];
+    return;
+  }
+  const long n = (long)A.g.nd[0] * A.g.nd[1] * A.g.nd[2];
+  double u[N1];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    gather<N1>(A.g, p, A.vf + c * n, false, lds[0], u);
+    evaluate<N1, false>(t, p, u, lds[1] + o, lds[2] + o, lds[3] + o, lds[4] + o, lds[0] + o, w[c], nullptr, nullptr,
+                        nullptr);
+  }
+}
 
 template <int N1>
 __global__ __launch_bounds__(256) void tangent_data_kernel(Tables1D t, TangentArgs A, const double *__restrict__ T,
@@ -349,6 +372,8 @@ __global__ __launch_bounds__(256) void tangent_data_kernel(Tables1D t, TangentAr
   gather<N1>(A.g, p, T_old, false, lds[0], u);
   evaluate<N1, false>(t, p, u, lds[1] + o, lds[2] + o, lds[3] + o, lds[4] + o, lds[0] + o, to, nullptr, nullptr,
                       nullptr);
+  double w[3][N1];
+  velocity_at_points<N1>(t, A, p, lds, o, w);
   if (!p.active) return;
   const Geometry &g = A.g;
 #pragma unroll
@@ -358,13 +383,14 @@ __global__ __launch_bounds__(256) void tangent_data_kernel(Tables1D t, TangentAr
     material(A.mat, tq[qz], rc, k, drc, dk);
     const double gT[3] = {gx[qz] / g.h[0], gy[qz] / g.h[1], gz[qz] / g.h[2]};
     double m = rc / g.dt;
-    if (A.solid) m += drc * ((tq[qz] - to[qz]) / g.dt + gT[0] * A.vel[0] + gT[1] * A.vel[1] + gT[2] * A.vel[2]);
+    if (A.solid) m += drc * ((tq[qz] - to[qz]) / g.dt + gT[0] * w[0][qz] + gT[1] * w[1][qz] + gT[2] * w[2][qz]);
     double *c = td + p.cell * kTangentCoeffs * NQ + qz * N1 * N1 + p.r;
     c[0] = JxW * m;
     c[NQ] = JxW * k;
-    if (g.conv) c[2 * NQ] = JxW * rc;
+    if (g.conv)
+      for (int d = 0; d < 3; ++d) c[(2 + d) * NQ] = JxW * rc * w[d][qz] / g.h[d];
     if (g.solid)
-      for (int d = 0; d < 3; ++d) c[(3 + d) * NQ] = JxW * dk * gT[d] / g.h[d];
+      for (int d = 0; d < 3; ++d) c[(5 + d) * NQ] = JxW * dk * gT[d] / g.h[d];
   }
 }
 
@@ -384,15 +410,15 @@ __global__ __launch_bounds__(256) void tangent_apply_kernel(Tables1D t, Geometry
       const double *c = td + p.cell * kTangentCoeffs * NQ + qz * N1 * N1 + p.r;
       const double uq = v[qz], ax = gx[qz], ay = gy[qz], az = gz[qz], k = c[NQ];
       double val = c[0] * uq;
-      if (g.conv) val += c[2 * NQ] * (g.cv[0] * ax + g.cv[1] * ay + g.cv[2] * az);
+      if (g.conv) val += c[2 * NQ] * ax + c[3 * NQ] * ay + c[4 * NQ] * az;
       v[qz] = val;
       gx[qz] = k * g.ih2[0] * ax;
       gy[qz] = k * g.ih2[1] * ay;
       gz[qz] = k * g.ih2[2] * az;
       if (g.solid) {
-        gx[qz] += c[3 * NQ] * uq;
-        gy[qz] += c[4 * NQ] * uq;
-        gz[qz] += c[5 * NQ] * uq;
+        gx[qz] += c[5 * NQ] * uq;
+        gy[qz] += c[6 * NQ] * uq;
+        gz[qz] += c[7 * NQ] * uq;
       }
     }
   }
@@ -423,6 +449,8 @@ __global__ __launch_bounds__(256) void tangent_residual_kernel(Tables1D t, Tange
     evaluate<N1, false>(t, p, u, lds[1] + o, lds[2] + o, lds[3] + o, lds[4] + o, lds[0] + o, qq, nullptr, nullptr,
                         nullptr);
   }
+  double w[3][N1];
+  velocity_at_points<N1>(t, A, p, lds, o, w);
   if (p.in_block) {
 #pragma unroll
     for (int qz = 0; qz < N1; ++qz) {
@@ -431,7 +459,7 @@ __global__ __launch_bounds__(256) void tangent_residual_kernel(Tables1D t, Tange
       material(A.mat, tq[qz], rc, k, drc, dk);
       const double gT[3] = {gx[qz] / g.h[0], gy[qz] / g.h[1], gz[qz] / g.h[2]};
       const double val = rc / g.dt * (tq[qz] - to[qz]) - qq[qz] +
-                         rc * (gT[0] * A.vel[0] + gT[1] * A.vel[1] + gT[2] * A.vel[2]);
+                         rc * (gT[0] * w[0][qz] + gT[1] * w[1][qz] + gT[2] * w[2][qz]);
       tq[qz] = -JxW * val;
       gx[qz] = -JxW * k * gT[0] / g.h[0];
       gy[qz] = -JxW * k * gT[1] / g.h[1];
@@ -447,7 +475,7 @@ __device__ __forceinline__ double factor(const Tables1D &t, int code, int q, int
   return code == 0 ? s * s : (code == 1 ? d * d : d * s);
 }
 
-// diag_I = sum_q t0 phi^2 + sum_d (t2 v_d/h_d + t_{3+d}) phi d_d phi + t1/h_d^2 (d_d phi)^2:
+// diag_I = sum_q t0 phi^2 + sum_d (t_{2+d} + t_{5+d}) phi d_d phi + t1/h_d^2 (d_d phi)^2:
 // seven tensor-product terms with 1D factors S^2 (0), G^2 (1), G S (2), three sweeps each
 template <int N1>
 __global__ __launch_bounds__(256) void tangent_diagonal_kernel(Tables1D t, Geometry g, const double *__restrict__ td,
@@ -471,8 +499,8 @@ __global__ __launch_bounds__(256) void tangent_diagonal_kernel(Tables1D t, Geome
           if (term == 0) {
             c = cc[0];
           } else if (term <= 3) {
-            if (g.conv) c += cc[2 * NQ] * g.cv[term - 1];
-            if (g.solid) c += cc[(term + 2) * NQ];
+            if (g.conv) c += cc[(term + 1) * NQ];
+            if (g.solid) c += cc[(term + 4) * NQ];
           } else {
             c = cc[NQ] * g.ih2[term - 4];
           }
@@ -522,12 +550,9 @@ __global__ __launch_bounds__(256) void unit_constrained_kernel(double *v, Geomet
   if (is_constrained(g.dmask, g.nd, ix, iy, iz)) v[i] = 1.0;
 }
 
-Geometry geometry(const OperatorData<double> &op, double dt, const double *vel = nullptr, int solid = 0) {
+Geometry geometry(const OperatorData<double> &op, double dt, int conv = 0, int solid = 0) {
   Geometry g{};
-  for (int d = 0; d < 3; ++d) {
-    g.cv[d] = vel ? vel[d] / op.h[d] : 0.0;
-    if (g.cv[d] != 0.0) g.conv = 1;
-  }
+  g.conv = conv ? 1 : 0;
   g.solid = solid ? 1 : 0;
   g.nc0 = op.nc[0];
   g.nc1 = op.nc[1];
@@ -561,20 +586,22 @@ int finish(hipStream_t) { return hipGetLastError() == hipSuccess ? MPDG_OK : MPD
 }  // namespace
 
 int launch_tangent_data(const OperatorData<double> &op, double dt, const mpdg_heat_material &mat, int solid,
-                        const double *vel, const double *T, const double *T_old, double *td, hipStream_t s) {
+                        const double *vel, const double *vfield, const double *T, const double *T_old, double *td,
+                        hipStream_t s) {
   TangentArgs A{};
-  A.g = geometry(op, dt, vel, solid);
+  A.g = geometry(op, dt, tangent_convects(vel, vfield), solid);
   A.mat = mat;
   for (int d = 0; d < 3; ++d) A.vel[d] = vel[d];
+  A.vf = vfield;
   A.solid = solid;
   const Tables1D t = make_tables(op.degree);
   MPDG_TANGENT_DISPATCH(tangent_data_kernel, t, A, T, T_old, td)
   return finish(s);
 }
 
-int launch_tangent_apply(const OperatorData<double> &op, const double *td, const double *vel, int solid,
+int launch_tangent_apply(const OperatorData<double> &op, const double *td, int conv, int solid,
                          const double *src, double *dst, hipStream_t s) {
-  const Geometry g = geometry(op, 1.0, vel, solid);
+  const Geometry g = geometry(op, 1.0, conv, solid);
   const Tables1D t = make_tables(op.degree);
   if (hipMemsetAsync(dst, 0, sizeof(double) * op.n_dofs, s) != hipSuccess) return MPDG_ERR_DEVICE;
   MPDG_TANGENT_DISPATCH(tangent_apply_kernel, t, g, td, src, dst)
@@ -582,20 +609,21 @@ int launch_tangent_apply(const OperatorData<double> &op, const double *td, const
 }
 
 int launch_tangent_residual(const OperatorData<double> &op, double dt, const mpdg_heat_material &mat,
-                            const double *vel, const double *T, const double *T_old, const double *q, double *dst,
-                            hipStream_t s) {
+                            const double *vel, const double *vfield, const double *T, const double *T_old,
+                            const double *q, double *dst, hipStream_t s) {
   TangentArgs A{};
   A.g = geometry(op, dt);
   A.mat = mat;
   for (int d = 0; d < 3; ++d) A.vel[d] = vel[d];
+  A.vf = vfield;
   const Tables1D t = make_tables(op.degree);
   MPDG_TANGENT_DISPATCH(tangent_residual_kernel, t, A, T, T_old, q, dst)
   return finish(s);
 }
 
-int launch_tangent_diagonal(const OperatorData<double> &op, const double *td, const double *vel, int solid,
+int launch_tangent_diagonal(const OperatorData<double> &op, const double *td, int conv, int solid,
                             double *diag, hipStream_t s) {
-  const Geometry g = geometry(op, 1.0, vel, solid);
+  const Geometry g = geometry(op, 1.0, conv, solid);
   const Tables1D t = make_tables(op.degree);
   if (hipMemsetAsync(diag, 0, sizeof(double) * op.n_dofs, s) != hipSuccess) return MPDG_ERR_DEVICE;
   MPDG_TANGENT_DISPATCH(tangent_diagonal_kernel, t, g, td, diag)

@@ -369,6 +369,26 @@ void cell_temperature(const orc_op *op, const FE1D &fe, const Mesh &m, int cx, i
   }
 }
 
+// Velocity at the quadrature points of one cell: the constant vector, or the values of the
+// velocity DoF field (vel_eval.read_dof_values_plain + evaluate(values),
+// heat_diffuse_operator.cpp:449-452, :975-978)
+void cell_velocity(const orc_op *op, const FE1D &fe, const Mesh &m, int cx, int cy, int cz, double w[3][512]) {
+  const int n = fe.n, nq = n * n * n;
+  const double *vf = op->tg.velocity_field;
+  if (!vf) {
+    for (int c = 0; c < 3; ++c)
+      for (int q = 0; q < nq; ++q) w[c][q] = op->tg.velocity[c];
+    return;
+  }
+  double u[512], tmp[512];
+  for (int c = 0; c < 3; ++c) {
+    for (int l = 0; l < n; ++l)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) u[i + n * (j + n * l)] = vf[c * m.ndofs + m.cell_dof(cx, cy, cz, i, j, l)];
+    eval_values(fe, u, w[c], tmp);
+  }
+}
+
 // Full tangent (heat_diffuse_operator.cpp:1028-1056):
 //   val  = (rho_cp/dt) u + rho_cp v.grad u
 //          [+ (d rho_cp/dT)(T - T_old)/dt u + (d rho_cp/dT)(grad T . v) u]   (solidification)
@@ -377,9 +397,10 @@ void local_cell_apply_tangent(const orc_op *op, const FE1D &fe, const Mesh &m, i
                               const double *u, double *out, CellGeom &geo) {
   const int n = fe.n, nq = n * n * n;
   const orc_tangent &tg = op->tg;
-  double v[512], tmp[512], g[3][512], tq[512], toq[512], gT[3][512];
+  double v[512], tmp[512], g[3][512], tq[512], toq[512], gT[3][512], wq[3][512];
   cell_geometry(fe, m, op->coords, cx, cy, cz, geo);
   cell_temperature(op, fe, m, cx, cy, cz, geo, tq, gT, toq);
+  cell_velocity(op, fe, m, cx, cy, cz, wq);
   eval_values(fe, u, v, tmp);
   eval_gradients(fe, v, g[0], g[1], g[2]);
   for (int q = 0; q < nq; ++q) {
@@ -388,7 +409,7 @@ void local_cell_apply_tangent(const orc_op *op, const FE1D &fe, const Mesh &m, i
     for (int b = 0; b < 3; ++b) gp[b] = B[0][b] * g[0][q] + B[1][b] * g[1][q] + B[2][b] * g[2][q];
     double rc, k, drc, dk;
     material(tg, tq[q], rc, k, drc, dk);
-    const double *w = tg.velocity;
+    const double w[3] = {wq[0][q], wq[1][q], wq[2][q]};
     double val = rc / tg.dt * v[q] + rc * (w[0] * gp[0] + w[1] * gp[1] + w[2] * gp[2]);
     for (int b = 0; b < 3; ++b) flux[b] = k * gp[b];
     if (tg.do_solidification) {
@@ -1004,7 +1025,7 @@ void orc_create_rhs(const orc_op *op, double *dst, const double *T, const double
   colored_cell_loop(m, 1, [&](int cx, int cy, int cz) {
     static thread_local CellGeom geo;
     double t[512], to[512], qs[512], v[512], vo[512], vq[512], tmp[512], g[3][512], kq[512],
-        out[512];
+        out[512], wq[3][512];
     long idx[512];
     bool con[512];
     for (int l = 0; l < n; ++l)
@@ -1020,6 +1041,7 @@ void orc_create_rhs(const orc_op *op, double *dst, const double *T, const double
         }
     cell_geometry(fe, m, op->coords, cx, cy, cz, geo);
     if (!op->tg.on) cell_kappa(&opT, fe, m, cx, cy, cz, kq);
+    if (op->tg.on) cell_velocity(op, fe, m, cx, cy, cz, wq);
     eval_values(fe, t, v, tmp);
     eval_gradients(fe, v, g[0], g[1], g[2]);
     eval_values(fe, to, vo, tmp);
@@ -1033,7 +1055,7 @@ void orc_create_rhs(const orc_op *op, double *dst, const double *T, const double
         // residual with the material at T and the prescribed velocity (:486-505)
         double rc, drc, dk;
         material(op->tg, v[q], rc, kk, drc, dk);
-        const double *w = op->tg.velocity;
+        const double w[3] = {wq[0][q], wq[1][q], wq[2][q]};
         val = rc / op->tg.dt * (v[q] - vo[q]) - vq[q] + rc * (gp[0] * w[0] + gp[1] * w[1] + gp[2] * w[2]);
       } else {
         kk = kq[q];

@@ -46,6 +46,9 @@ typedef struct orc_tangent {
   int do_solidification;  /* the d/dT terms of the tangent */
   double velocity[3];
   const double *T_old;
+  /* NULL: the constant velocity above; else a velocity DoF field, 3*n_dofs values
+     (x block, y block, z block) on the temperature's Q_k nodes (heat_diffuse_operator.cpp:975-978) */
+  const double *velocity_field;
 } orc_tangent;
 
 typedef struct orc_op {

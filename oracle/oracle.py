@@ -33,6 +33,7 @@ class OrcTangent(C.Structure):
         ("do_solidification", C.c_int),
         ("velocity", C.c_double * 3),
         ("T_old", _dp),
+        ("velocity_field", _dp),
     ]
 
 
@@ -239,9 +240,11 @@ class Operator:
         lib().orc_interpolate_qp(self.ref, ptr(np.ascontiguousarray(u, dtype=np.float64)), ptr(out))
         return out
 
-    def set_tangent(self, dt, material=None, velocity=(0.0, 0.0, 0.0), do_solidification=False, T_old=None):
+    def set_tangent(self, dt, material=None, velocity=(0.0, 0.0, 0.0), do_solidification=False, T_old=None,
+                    velocity_field=None):
         """Switch to the full Newton tangent. material: None (constant: rho_cp = c_mass*dt, k = kappa)
-        or dict(k=(liquid, solid), cp=(..), rho=(..), T_solidus=.., T_liquidus=..) for the mushy zone."""
+        or dict(k=(liquid, solid), cp=(..), rho=(..), T_solidus=.., T_liquidus=..) for the mushy zone.
+        velocity_field: None (the constant `velocity`) or 3*n_dofs nodal values (x, y, z blocks)."""
         t = self.s.tg
         t.on = 1
         t.dt = dt
@@ -258,6 +261,12 @@ class Operator:
             t.T_solidus, t.T_liquidus = material["T_solidus"], material["T_liquidus"]
         t.do_solidification = 1 if do_solidification else 0
         t.velocity[:] = list(velocity)
+        if velocity_field is not None:
+            self._keep["velocity_field"] = np.ascontiguousarray(velocity_field, dtype=np.float64).reshape(-1)
+            assert self._keep["velocity_field"].size == 3 * self.n_dofs
+            t.velocity_field = ptr(self._keep["velocity_field"])
+        else:
+            t.velocity_field = None
         if T_old is not None:
             self._keep["T_old"] = np.ascontiguousarray(T_old, dtype=np.float64)
             t.T_old = ptr(self._keep["T_old"])

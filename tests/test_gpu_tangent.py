@@ -73,6 +73,63 @@ def test_tangent_diagonal_and_residual_match_oracle(cuda, degree, nc, L, dmask, 
     assert np.linalg.norm(r.cpu().numpy() - ref) <= 1e-11 * np.linalg.norm(ref)
 
 
+FIELD_CASES = [(1, (5, 4, 3), (0.5, 0.4, 0.3), 0, True),
+               (2, (3, 4, 2), (0.3, 0.4, 0.2), 0b000011, True),
+               (3, (20, 2, 2), (2.0, 0.2, 0.2), 0b000001, False),
+               (4, (13, 2, 2), (0.65, 0.1, 0.1), 0b110000, True)]
+
+
+@pytest.mark.parametrize("degree,nc,L,dmask,solid", FIELD_CASES)
+def test_tangent_velocity_field_matches_oracle(cuda, degree, nc, L, dmask, solid):
+    """Velocity DoF field (heat_diffuse_operator.cpp:975-978, :449-452): apply, diagonal and residual
+    against the oracle; None restores the constant vector given to set_tangent."""
+    torch = cuda
+    import meltpooldg_amd as M
+    vel = (0.1, -0.2, 0.05)
+    orc, op, T, T_old, Td, Tod = pair(torch, M, degree, nc, L, dmask, solid, vel, seed=7)
+    X = orc.node_coords()
+    vf = np.concatenate([-0.3 + 0.8 * X[1] * X[2], 0.2 * np.sin(9 * X[0]), 0.1 + 0.5 * X[0] - 0.3 * X[2] ** 2])
+    orc.set_tangent(DT, material=MUSHY, do_solidification=solid, T_old=T_old, velocity_field=vf)
+    vfd = torch.tensor(vf, device="cuda")
+    op.set_velocity_field(vfd)
+    op.bind_temperature(Td, Tod)
+    src = np.random.default_rng(8).uniform(-1, 1, orc.n_dofs)
+    dst = torch.empty(orc.n_dofs, dtype=torch.float64, device="cuda")
+    op.vmult(dst, torch.tensor(src, device="cuda"))
+    ref = orc.vmult(src, threads=1)
+    assert np.linalg.norm(dst.cpu().numpy() - ref) <= 1e-12 * np.linalg.norm(ref)
+    d = torch.empty(orc.n_dofs, dtype=torch.float64, device="cuda")
+    op.compute_diagonal(d)
+    ref = orc.diagonal()
+    assert np.linalg.norm(d.cpu().numpy() - ref) <= 1e-12 * np.linalg.norm(ref)
+    q = np.random.default_rng(9).uniform(0, 100, orc.n_dofs)
+    r = torch.full((orc.n_dofs,), 0.5, dtype=torch.float64, device="cuda")
+    op.create_rhs(r, Td, Tod, torch.tensor(q, device="cuda"))
+    ref = orc.create_rhs(np.full(orc.n_dofs, 0.5), T, T_old, q)
+    assert np.linalg.norm(r.cpu().numpy() - ref) <= 1e-11 * np.linalg.norm(ref)
+    # back to the constant vector: the tangent must be bound again first
+    op.set_velocity_field(None)
+    with pytest.raises(M.MpdgError):
+        op.vmult(dst, torch.tensor(src, device="cuda"))
+    op.bind_temperature(Td, Tod)
+    op.vmult(dst, torch.tensor(src, device="cuda"))
+    orc.set_tangent(DT, material=MUSHY, velocity=vel, do_solidification=solid, T_old=T_old)
+    ref = orc.vmult(src, threads=1)
+    assert np.linalg.norm(dst.cpu().numpy() - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_velocity_field_needs_the_tangent(cuda):
+    torch = cuda
+    import meltpooldg_amd as M
+    op = M.HeatDiffuseMultiPhaseOperator(2, (2, 2, 2))
+    with pytest.raises(M.MpdgError):
+        op.set_velocity_field(torch.zeros(3 * op.n_dofs, dtype=torch.float64, device="cuda"))
+    op.reset_time_increment(DT)
+    op.set_tangent(GPU_MUSHY)
+    with pytest.raises(ValueError):
+        op.set_velocity_field(torch.zeros(op.n_dofs, dtype=torch.float64, device="cuda"))
+
+
 @pytest.mark.parametrize("restart", [0, 5])
 def test_gmres_matches_oracle(cuda, restart):
     """J x = b with the non-symmetric tangent: GMRES + inverse diagonal on the GPU and in the oracle reach

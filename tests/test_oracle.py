@@ -386,6 +386,89 @@ def test_tangent_is_the_derivative_of_the_residual(solid):
         assert err > 1e-3
 
 
+def _box_integral(L, c0, cx, cy, cz):
+    """Integral of c0 + cx x + cy y + cz z over [0, Lx] x [0, Ly] x [0, Lz]."""
+    V = L[0] * L[1] * L[2]
+    return V * (c0 + cx * L[0] / 2 + cy * L[1] / 2 + cz * L[2] / 2)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_tangent_velocity_field_closed_forms(k):
+    """Velocity DoF field v = (1 + 2y, -0.5 + 3z, 0.25 + x), u = T = 1 + x + 2y + 3z (both reproduced by Q_k):
+    since sum_i phi_i = 1 and sum_i x_i phi_i = x, sum_i (J u)_i = int rc/dt u + rc v.grad u,
+    sum_i x_i (J u)_i = int (rc/dt u + rc v.grad u) x + k du/dx, and with T_old = T and no source
+    sum_i rhs_i = -int rc v.grad T. Components swapped, or the field ignored, change every sum."""
+    L = (0.3, 0.2, 0.25)
+    dt, cm, kap = 0.1, 3.0, 0.7
+    op = O.Operator(k, (2, 3, 2), L, c_mass=cm, kappa=kap)
+    x, y, z = op.node_coords()
+    vf = np.concatenate([1 + 2 * y, -0.5 + 3 * z, 0.25 + x])
+    u = 1 + x + 2 * y + 3 * z
+    op.set_T(u)
+    op.set_tangent(dt, velocity_field=vf)
+    rc = cm * dt
+    # v.grad u = 0.75 + 3x + 2y + 6z
+    mass = cm * _box_integral(L, 1, 1, 2, 3)
+    conv = rc * _box_integral(L, 0.75, 3, 2, 6)
+    Ju = op.vmult(u, threads=1)
+    assert abs(Ju.sum() - (mass + conv)) <= 1e-12 * abs(mass + conv)
+    # int f x for f = c0 + cx x + cy y + cz z
+    V = L[0] * L[1] * L[2]
+
+    def xmom(c0, cx, cy, cz):
+        return V * (c0 * L[0] / 2 + cx * L[0] ** 2 / 3 + cy * L[0] * L[1] / 4 + cz * L[0] * L[2] / 4)
+
+    xm = cm * xmom(1, 1, 2, 3) + rc * xmom(0.75, 3, 2, 6) + kap * 1.0 * V
+    assert abs(x @ Ju - xm) <= 1e-12 * abs(xm)
+    rhs = op.create_rhs(np.zeros(op.n_dofs), np.ascontiguousarray(u), np.ascontiguousarray(u))
+    assert abs(rhs.sum() + conv) <= 1e-12 * abs(conv)
+    # planted mistakes: the x and y components exchanged, the field dropped
+    for wrong in (np.concatenate([-0.5 + 3 * z, 1 + 2 * y, 0.25 + x]), np.zeros(3 * op.n_dofs)):
+        op.set_tangent(dt, velocity_field=wrong)
+        assert abs(op.vmult(u, threads=1).sum() - (mass + conv)) > 1e-3 * abs(conv)
+
+
+def test_tangent_constant_velocity_field_equals_constant_vector():
+    """A velocity field with the same value at every node gives the constant-vector tangent: apply,
+    diagonal and residual, with the mushy material and the solidification terms."""
+    op = O.Operator(2, (3, 2, 2), (0.3, 0.2, 0.2), dirichlet_mask=1)
+    X = op.node_coords()
+    T = 1970.0 + 3.0 * np.sin(4 * X[0]) + X[1]
+    T_old = T - 0.4 * np.cos(2 * X[2])
+    w = (-0.5, 0.2, 0.1)
+    out = []
+    for field in (None, np.concatenate([np.full(op.n_dofs, c) for c in w])):
+        op.set_tangent(0.01, material=MUSHY, velocity=w if field is None else (0.0, 0.0, 0.0),
+                       do_solidification=True, T_old=T_old, velocity_field=field)
+        op.set_T(T)
+        u = np.random.default_rng(3).uniform(-1, 1, op.n_dofs)
+        out.append((op.vmult(u, threads=1), op.diagonal(),
+                    op.create_rhs(np.zeros(op.n_dofs), np.ascontiguousarray(T), np.ascontiguousarray(T_old))))
+    for a, b in zip(out[0], out[1]):
+        assert np.linalg.norm(a - b) <= 1e-13 * np.linalg.norm(a)
+
+
+def test_tangent_with_velocity_field_is_the_derivative_of_the_residual():
+    """J(T) u = -d/de create_rhs(T + e u) with a non-uniform velocity field and the solidification terms."""
+    rng = np.random.default_rng(6)
+    op = O.Operator(2, (3, 3, 2), (0.3, 0.3, 0.2))
+    X = op.node_coords()
+    T = 1970.0 + 2.0 * np.sin(5 * X[0]) * np.cos(4 * X[1]) + 0.5 * X[2]
+    T_old = T - 0.3 * np.cos(3 * X[0])
+    vf = np.concatenate([-0.02 + 0.1 * X[1] ** 2, 0.01 * np.sin(6 * X[0]), 0.005 + 0.05 * X[0] * X[2]])
+    op.set_tangent(0.01, material=MUSHY, do_solidification=True, T_old=T_old, velocity_field=vf)
+    op.set_T(T)
+    u = rng.uniform(-1, 1, op.n_dofs)
+    Ju = op.vmult(u, threads=1)
+    eps = 1e-4
+
+    def res(Tx):
+        return op.create_rhs(np.zeros(op.n_dofs), np.ascontiguousarray(Tx), T_old)
+
+    fd = -(res(T + eps * u) - res(T - eps * u)) / (2 * eps)
+    assert np.linalg.norm(Ju - fd) <= 1e-7 * np.linalg.norm(fd)
+
+
 def test_gmres_matches_direct_solve():
     """Non-symmetric tangent (velocity, solidification terms) on a small mesh: GMRES with and without the
     inverse diagonal against numpy's dense solve; restarted (restart 5) and full cycles agree."""
@@ -481,6 +564,67 @@ def _gauss_integrals(c, sig, L):
     i0 = sig * sqrt(pi / 2.0) * (erf((L - c) / s2) - erf(-c / s2))
     i1 = c * i0 + sig ** 2 * (exp(-c * c / (2 * sig ** 2)) - exp(-(L - c) ** 2 / (2 * sig ** 2)))
     return i0, i1
+
+
+SHEAR = np.array([[1.1, 0.2, 0.1], [0.05, 0.9, 0.15], [0.1, -0.1, 1.2]])
+
+
+def _remap(op, fn):
+    """Replace the mapped operator's node coordinates (held by the operator) by fn(x, y, z) of the box's."""
+    X = op.node_coords()
+    op._keep["X"][:] = np.asarray(fn(*X)).reshape(-1)
+    return op._keep["X"].reshape(3, -1)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_mapped_robin_faces_give_the_mapped_areas(k):
+    """Robin faces on a mapped mesh (face JxW from the mapping, heat_diffuse_operator.cpp:1090-1129): under
+    an affine map x -> A x + b, which Q_k reproduces, sum_i (J 1)_i = c_m |det A| V + alpha sum_f |A e_a x A e_b|
+    L_a L_b; the box's face areas (the mapping ignored on the faces) miss it by >5 %."""
+    L = (0.5, 0.4, 0.3)
+    cm, alpha = 2.0, 3.5
+    op = O.Operator(k, (2, 3, 2), L, c_mass=cm, kappa=0.7, robin_mask=63, alpha=alpha, mapped_amp=0.0)
+    _remap(op, lambda x, y, z: SHEAR @ np.stack([x, y, z]) + np.array([[0.3], [-0.2], [0.1]]))
+    V = L[0] * L[1] * L[2]
+    E = SHEAR.T  # columns A e_d as rows
+    area = sum(2 * np.linalg.norm(np.cross(E[a], E[b])) * L[a] * L[b] for a, b in ((1, 2), (0, 2), (0, 1)))
+    exp = cm * abs(np.linalg.det(SHEAR)) * V + alpha * area
+    got = op.vmult(np.ones(op.n_dofs), threads=1).sum()
+    assert abs(got - exp) <= 1e-12 * exp
+    box = cm * abs(np.linalg.det(SHEAR)) * V + alpha * 2 * (L[1] * L[2] + L[0] * L[2] + L[0] * L[1])
+    assert abs(box - exp) > 0.05 * alpha * area
+    if k <= 2:  # the diagonal (faces included) is the diagonal of the applied matrix
+        A = np.stack([op.vmult(e, threads=1) for e in np.eye(op.n_dofs)], axis=1)
+        assert np.allclose(op.diagonal(), np.diag(A), rtol=1e-13, atol=0.0)
+
+
+@pytest.mark.parametrize("k,nc", [(1, (64, 64, 1)), (2, (40, 30, 2)), (3, (32, 32, 1)), (4, (24, 24, 1))])
+def test_mapped_laser_flux_integral(k, nc):
+    """Surface flux on a mapped top face: x -> x (1 + 0.1 y) (bilinear, reproduced by Q_k) stretches the
+    z+ face to x' <= L_x (1 + 0.1 y) with surface element 1 + 0.1 y; sum_i rhs_i = int q dA' =
+    int_0^Ly peak g(y) int_0^{Lx(1+0.1y)} g(x') dx' dy (erf inside, adaptive quadrature outside). Taking
+    the box's face (the mapping ignored) misses it by >3 %."""
+    from scipy.integrate import quad
+    from scipy.special import erf
+    L = (1.0, 0.8, 0.5)
+    peak, sig, x0, y0, vx, t = 1e3, 0.05, 0.57, 0.77, 1.0, 0.4
+    xc = x0 + vx * t
+    op = O.Operator(k, nc, L, c_mass=1e3, kappa=2.0, laser=(peak, sig, x0, y0, vx, t), mapped_amp=0.0)
+    Xm = _remap(op, lambda x, y, z: np.stack([x * (1 + 0.1 * y), y, z]))
+    z = np.zeros(op.n_dofs)
+    r = op.create_rhs(np.zeros(op.n_dofs), z, z.copy())
+    c = sig * np.sqrt(np.pi / 2)
+    ix = lambda U: c * (erf((U - xc) / (sig * np.sqrt(2))) - erf(-xc / (sig * np.sqrt(2))))
+    gy = lambda y: np.exp(-(y - y0) ** 2 / (2 * sig ** 2))
+    exp = peak * quad(lambda y: gy(y) * ix(L[0] * (1 + 0.1 * y)), 0.0, L[1], epsabs=0, epsrel=1e-13, limit=200)[0]
+    tol = {1: 1e-5, 2: 1e-6, 3: 1e-8, 4: 1e-9}[k]
+    assert abs(r.sum() - exp) <= tol * exp
+    # y moment: sum_i y_i rhs_i = int y q dA'
+    expy = peak * quad(lambda y: y * gy(y) * ix(L[0] * (1 + 0.1 * y)), 0.0, L[1], epsabs=0, epsrel=1e-13,
+                       limit=200)[0]
+    assert abs(Xm[1] @ r - expy) <= tol * expy
+    box = peak * ix(L[0]) * quad(gy, 0.0, L[1], epsabs=0, epsrel=1e-13)[0]
+    assert abs(box / exp - 1) > 0.03
 
 
 @pytest.mark.parametrize("k,nc", [(1, (64, 64, 1)), (2, (40, 30, 2)), (3, (32, 32, 1)), (4, (24, 24, 1))])
