@@ -140,6 +140,67 @@ def test_robin_tangent_and_rhs(cuda):
     assert rel_err(dst.cpu().numpy(), ref) < 1e-11
 
 
+MAPS = {
+    # affine: reproduced exactly, every face tilted
+    "shear": lambda x, y, z: np.stack([1.1 * x + 0.2 * y + 0.1 * z + 0.3, 0.05 * x + 0.9 * y + 0.15 * z,
+                                       0.1 * x - 0.1 * y + 1.2 * z]),
+    # curved faces x+- and a stretched top face
+    "curved": lambda x, y, z: np.stack([x * (1 + 0.1 * y) + 0.05 * np.sin(2 * np.pi * y / 0.8) * np.sin(
+        2 * np.pi * z / 0.5), y + 0.03 * np.sin(3 * x), z + 0.02 * x * y]),
+}
+
+
+@pytest.mark.parametrize("degree,nc,mapname", [(1, (10, 8, 3), "shear"), (2, (6, 5, 3), "curved"),
+                                               (3, (5, 4, 2), "curved"), (4, (3, 3, 2), "shear"),
+                                               (3, (20, 3, 2), "curved")])
+def test_mapped_robin_faces_and_flux_match_oracle(cuda, degree, nc, mapname):
+    """Robin faces (convection + radiation) on x-, x+, y-, y+, z- and the laser flux on z+ of a mapped
+    mesh: apply, inverse diagonal and create_rhs against the oracle (whose mapped face geometry is
+    pinned by the face areas of affine maps and the flux integral of a stretched face), and the nodal
+    flux field at the mapped nodes."""
+    torch = cuda
+    import meltpooldg_amd as M
+    L = (1.0, 0.8, 0.5)
+    rng = np.random.default_rng(31)
+    X = MAPS[mapname](*O.Operator(degree, nc, L).node_coords())
+    T = 1000.0 + 50.0 * rng.uniform(-1, 1, X.shape[1])
+    kw = dict(robin_mask=0b011111, alpha=25.0, emissivity=0.98, T_inf_conv=300.0, T_inf_rad=310.0)
+    orc = O.Operator(degree, nc, L, c_mass=3.6e8, kappa=55.563, T=T, laser=LASER, mapped_amp=0.0, **kw)
+    orc._keep["X"][:] = X.reshape(-1)
+    gop = M.HeatDiffuseMultiPhaseOperator(degree, nc, L, rho_cp=3.6e6)
+    gop.reset_time_increment(1e-2)
+    gop.set_conductivity_constant(55.563)
+    Xd = torch.tensor(X.reshape(-1), device="cuda")
+    gop.set_mapping(Xd)
+    Td = torch.tensor(T, device="cuda")
+    gop.set_robin(0b011111, alpha=25.0, T_inf_conv=300.0, emissivity=0.98, T_inf_rad=310.0, T=Td)
+    gop.set_surface_flux(*LASER)
+    x = rng.uniform(-1, 1, orc.n_dofs)
+    y = torch.empty(orc.n_dofs, dtype=torch.float64, device="cuda")
+    gop.vmult(y, torch.tensor(x, device="cuda"))
+    torch.cuda.synchronize()
+    assert rel_err(y.cpu().numpy(), orc.vmult(x)) < 1e-12
+    d = torch.empty_like(y)
+    gop.compute_inverse_diagonal_from_matrixfree(d)
+    torch.cuda.synchronize()
+    assert rel_err(d.cpu().numpy(), orc.inverse_diagonal()) < 1e-12
+    T_old = T - 1.0
+    ref = orc.create_rhs(np.zeros(orc.n_dofs), T, T_old)
+    dst = torch.zeros_like(y)
+    gop.create_rhs(dst, Td, torch.tensor(T_old, device="cuda"))
+    torch.cuda.synchronize()
+    assert rel_err(dst.cpu().numpy(), ref) < 1e-11
+    out = torch.full((orc.n_dofs,), 7.0, dtype=torch.float64, device="cuda")
+    gop.surface_flux_field(out)
+    torch.cuda.synchronize()
+    peak, sig, x0, y0, vx, t = LASER
+    q = peak * np.exp(-((X[0] - x0 - vx * t) ** 2 + (X[1] - y0) ** 2) / (2 * sig ** 2))
+    top = np.zeros(orc.n_dofs, dtype=bool)
+    top[-(nc[0] * degree + 1) * (nc[1] * degree + 1):] = True
+    q[~top] = 0.0
+    assert rel_err(out.cpu().numpy(), q) < 1e-14
+
+
 # Meshes with 64 columns of chunks (a multiple of the 8 XCDs): the launch then gives each XCD
 # a contiguous range of columns, a path the smaller meshes above do not take.
 XCD_CASES = [(1, (8, 32, 32), "const", 0, 0), (2, (14, 24, 16), "qp", 0, 0), (3, (8, 32, 16), "qp", 0b000101, 0),
