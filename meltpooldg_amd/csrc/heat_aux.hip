@@ -226,6 +226,9 @@ int launch_cell_average(const OperatorData<Number> &op, Number *kc, hipStream_t 
 // Every term is a tensor product of 1D factors S^2, G^2 or G S, so each is three sum-factorized
 // sweeps of its per-point coefficient. Cartesian cells: c^(0) = c_m vol W, c^(aa) = k vol/h_a^2 W;
 // mapped cells: c^(0) = c_m JxW, c^(ab) = k (JxW B B^T)_ab (cross terms twice).
+// diagonal_kernel stores each cell's N1^3 entries (cell-major, lexicographic inside the cell);
+// diag_gather_kernel then sums, for every DoF, the entries of the (up to 8) cells that hold it in
+// a fixed order: no atomics, and the result does not depend on the schedule.
 template <typename Number>
 __device__ __forceinline__ Number diag_factor(const AuxTables<Number> &t, int code, int q, int i) {
   const Number s = t.S[q][i], g = t.G[q][i];
@@ -236,7 +239,7 @@ template <typename Number, int N1, bool MAPPED>
 __global__ __launch_bounds__(256) void diagonal_kernel(AuxTables<Number> t, int nc0, int nc1, long ncells, int nd0,
                                                        int nd1, Number cm, Number ks0, Number ks1, Number ks2,
                                                        int kmode, Number kappa, const Number *__restrict__ karr,
-                                                       const Number *__restrict__ metric, Number *__restrict__ diag) {
+                                                       const Number *__restrict__ metric, Number *__restrict__ cellwise) {
   constexpr int L2 = N1 * N1, NQ = N1 * L2, CPB = kCellsPerBlock<N1>, K = N1 - 1, NT = MAPPED ? 7 : 4;
   // term: metric component, 1D factor codes along x, y, z (0: S^2, 1: G^2, 2: G S)
   constexpr int comp[7] = {0, 1, 3, 6, 2, 4, 5};
@@ -309,11 +312,40 @@ __global__ __launch_bounds__(256) void diagonal_kernel(AuxTables<Number> t, int 
     }
   }
   if (!active) return;
-  int c[3];
-  cell_of<N1, Number>(cell, nc0, nc1, c);
-  const long row = (long)K * c[0] + (long)nd0 * ((K * c[1] + b) + (long)nd1 * (K * c[2] + a));
+  Number *out = cellwise + cell * NQ + r * N1;
 #pragma unroll
-  for (int i = 0; i < N1; ++i) atomicAdd(diag + row + i, d[i]);
+  for (int i = 0; i < N1; ++i) out[i] = d[i];
+}
+
+// diag[I] = sum over the cells holding DoF I of their entry for I; along each direction a DoF on
+// a cell boundary belongs to the cells on both sides (lower cell first)
+template <typename Number, int N1>
+__global__ __launch_bounds__(256) void diag_gather_kernel(const Number *__restrict__ cellwise, int nc0, int nc1,
+                                                          int nc2, Number *__restrict__ diag) {
+  constexpr int K = N1 - 1, NQ = N1 * N1 * N1;
+  const int nd0 = K * nc0 + 1, nd1 = K * nc1 + 1, nd2 = K * nc2 + 1;
+  const long n = (long)nd0 * nd1 * nd2;
+  const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+  if (gid >= n) return;
+  const int ix = gid % nd0, iy = (gid / nd0) % nd1, iz = gid / ((long)nd0 * nd1);
+  // per direction: first cell, its local index, and whether the previous cell also holds the DoF
+  int c[3], l[3], two[3];
+  const int idx[3] = {ix, iy, iz}, nc[3] = {nc0, nc1, nc2};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    c[d] = min(idx[d] / K, nc[d] - 1);
+    l[d] = idx[d] - K * c[d];
+    two[d] = (l[d] == 0 && c[d] > 0) ? 1 : 0;
+  }
+  Number acc = Number(0);
+  for (int sz = two[2]; sz >= 0; --sz)
+    for (int sy = two[1]; sy >= 0; --sy)
+      for (int sx = two[0]; sx >= 0; --sx) {
+        const long cell = (c[0] - sx) + (long)nc0 * ((c[1] - sy) + (long)nc1 * (c[2] - sz));
+        const int loc = (sx ? K : l[0]) + N1 * ((sy ? K : l[1]) + N1 * (sz ? K : l[2]));
+        acc += cellwise[cell * NQ + loc];
+      }
+  diag[gid] = acc;
 }
 
 template <typename Number>
@@ -477,8 +509,8 @@ int launch_boundary(const OperatorData<double> &op, int mode, const double *flux
 }
 
 template <typename Number, int N1>
-static void launch_diagonal_n1(const OperatorData<Number> &op, int kmode, const Number *karr, Number *diag,
-                               hipStream_t stream) {
+static void launch_diagonal_n1(const OperatorData<Number> &op, int kmode, const Number *karr, Number *cellwise,
+                               Number *diag, hipStream_t stream) {
   constexpr int CPB = kCellsPerBlock<N1>;
   const double vol = op.metric ? 1.0 : op.h[0] * op.h[1] * op.h[2];
   const dim3 grid((op.n_cells + CPB - 1) / CPB);
@@ -488,16 +520,20 @@ static void launch_diagonal_n1(const OperatorData<Number> &op, int kmode, const 
   if (op.metric)
     hipLaunchKernelGGL((diagonal_kernel<Number, N1, true>), grid, dim3(256), 0, stream, t, op.nc[0], op.nc[1],
                        op.n_cells, op.nd[0], op.nd[1], cm, ks0, ks1, ks2, kmode, (Number)op.kappa, karr, op.metric,
-                       diag);
+                       cellwise);
   else
     hipLaunchKernelGGL((diagonal_kernel<Number, N1, false>), grid, dim3(256), 0, stream, t, op.nc[0], op.nc[1],
                        op.n_cells, op.nd[0], op.nd[1], cm, ks0, ks1, ks2, kmode, (Number)op.kappa, karr, op.metric,
-                       diag);
+                       cellwise);
+  hipLaunchKernelGGL((diag_gather_kernel<Number, N1>), dim3((op.n_dofs + 255) / 256), dim3(256), 0, stream, cellwise,
+                     op.nc[0], op.nc[1], op.nc[2], diag);
 }
 
-// kq: scratch of n_cells (k+1)^3 values, used for a conductivity law
+// scratch: 2 n_cells (k+1)^3 values (the cells' entries, and k at the points for a conductivity law)
 template <typename Number>
-int launch_diagonal(const OperatorData<Number> &op_in, Number *diag, Number *kq, hipStream_t stream) {
+int launch_diagonal(const OperatorData<Number> &op_in, Number *diag, Number *scratch, hipStream_t stream) {
+  const long ncq = op_in.n_cells * (long)((op_in.degree + 1) * (op_in.degree + 1) * (op_in.degree + 1));
+  Number *cellwise = scratch, *kq = scratch + ncq;
   const OperatorData<Number> op = kernel_view(op_in);
   if (op.metric == nullptr && op_in.metric != nullptr) return MPDG_ERR_INVALID;
   int kmode = 0;
@@ -511,12 +547,11 @@ int launch_diagonal(const OperatorData<Number> &op_in, Number *diag, Number *kq,
     kmode = 1;
     karr = kq;
   }
-  if (hipMemsetAsync(diag, 0, sizeof(Number) * op.n_dofs, stream) != hipSuccess) return MPDG_ERR_DEVICE;
   switch (op.degree) {
-    case 1: launch_diagonal_n1<Number, 2>(op, kmode, karr, diag, stream); break;
-    case 2: launch_diagonal_n1<Number, 3>(op, kmode, karr, diag, stream); break;
-    case 3: launch_diagonal_n1<Number, 4>(op, kmode, karr, diag, stream); break;
-    case 4: launch_diagonal_n1<Number, 5>(op, kmode, karr, diag, stream); break;
+    case 1: launch_diagonal_n1<Number, 2>(op, kmode, karr, cellwise, diag, stream); break;
+    case 2: launch_diagonal_n1<Number, 3>(op, kmode, karr, cellwise, diag, stream); break;
+    case 3: launch_diagonal_n1<Number, 4>(op, kmode, karr, cellwise, diag, stream); break;
+    case 4: launch_diagonal_n1<Number, 5>(op, kmode, karr, cellwise, diag, stream); break;
     default: return MPDG_ERR_INVALID;
   }
   if (op.dmask)

@@ -99,8 +99,8 @@ int mpdg_heat_op::diagonal64(double *diag, hipStream_t s) const {
   }
   const int n1 = d64.degree + 1;
   MPDG_TRY(refresh_metric_k(s));
-  MPDG_TRY(kq.resize((size_t)d64.n_cells * n1 * n1 * n1));
-  MPDG_TRY(launch_diagonal<double>(d64, diag, kq.get(), s));
+  MPDG_TRY(diag_scratch.resize(2 * (size_t)d64.n_cells * n1 * n1 * n1));
+  MPDG_TRY(launch_diagonal<double>(d64, diag, diag_scratch.get(), s));
   if (d64.robin_mask) MPDG_TRY(launch_boundary(d64, 1, nullptr, diag, nullptr, T_robin, s));
   return MPDG_OK;
 }
@@ -300,7 +300,7 @@ struct Interp1DHost {
 template <typename T>
 struct Level {
   OperatorData<T> od;
-  DeviceArray<T> x, b, r, d, Ax, inv, kcell, kqp, Tlev, kq_scratch;
+  DeviceArray<T> x, b, r, d, Ax, inv, kcell, kqp, Tlev, kq_scratch, diag_scratch;
   double lambda_max = 0.0;
   // transfer from the next coarser level (unused on the coarsest)
   DeviceArray<int> pcol[3], rcol[3];
@@ -437,7 +437,7 @@ struct GMGPrecond : mpdg_precond {
     if (op->coords) MPDG_TRY(build_geometry());
     MPDG_TRY(red.init());
     MPDG_TRY(eig_sc.resize(4));
-    MPDG_TRY(eig_hist.resize(2 * std::max(prm.eig_cg_n_iterations, 1) + 2));
+    MPDG_TRY(eig_hist.resize(std::max<size_t>(lev.size(), 2) * hist_stride()));
     nc_coarse_dofs = (int)lev[0]->od.n_dofs;
     MPDG_TRY(coarse_inv.resize((size_t)nc_coarse_dofs * nc_coarse_dofs));
     MPDG_TRY(cvec_in.resize(nc_coarse_dofs));
@@ -558,38 +558,55 @@ struct GMGPrecond : mpdg_precond {
     for (size_t l = 1; l < lev.size(); ++l) {
       Level<T> &L = *lev[l];
       const int n1 = L.od.degree + 1;
-      MPDG_TRY(L.kq_scratch.resize((size_t)L.od.n_cells * n1 * n1 * n1));
-      MPDG_TRY(launch_diagonal<T>(L.od, L.inv.get(), L.kq_scratch.get(), s));
+      MPDG_TRY(L.diag_scratch.resize(2 * (size_t)L.od.n_cells * n1 * n1 * n1));
+      MPDG_TRY(launch_diagonal<T>(L.od, L.inv.get(), L.diag_scratch.get(), s));
       MPDG_TRY(launch_invert_diagonal<T>(L.inv.get(), L.od.n_dofs, s));
       // smoothing leaves the constrained DoFs at zero
       MPDG_TRY(launch_zero_constrained<T>(L.inv.get(), L.od.nd, L.od.dmask, s));
-      L.kq_scratch.release();
-      MPDG_TRY(estimate_eigenvalue(L, s));
-      set_chebyshev(L);
+      MPDG_TRY(run_eigenvalue_cg(L, eig_hist.get() + (l - 1) * hist_stride(), s));
+    }
+    // one copy of every level's CG coefficients, then lambda_max and the Chebyshev coefficients
+    const size_t nh = (lev.size() - 1) * hist_stride();
+    std::vector<double> h(nh);
+    if (nh) {
+      if (hipMemcpyAsync(h.data(), eig_hist.get(), nh * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
+        return MPDG_ERR_DEVICE;
+      if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+    }
+    for (size_t l = 1; l < lev.size(); ++l) {
+      lev[l]->lambda_max = lambda_max_from(h.data() + (l - 1) * hist_stride());
+      set_chebyshev(*lev[l]);
     }
     return MPDG_OK;
   }
 
-  int estimate_eigenvalue(Level<T> &L, hipStream_t s) {
+  size_t hist_stride() const { return 2 * (size_t)std::max(prm.eig_cg_n_iterations, 1) + 2; }
+
+  // eig_cg_n_iterations CG steps on L x = start vector with the inverse diagonal; the step
+  // coefficients (alpha_k, beta_k) go to hist[2k], hist[2k + 1]
+  int run_eigenvalue_cg(Level<T> &L, double *hist, hipStream_t s) {
     const long n = L.od.n_dofs;
     const int it = prm.eig_cg_n_iterations;
     // r = start vector (b, x0 = 0), z = inv r, p = z
     MPDG_TRY(launch_eig_start<T>(L.r.get(), L.od.nd, L.od.dmask, s));
     MPDG_TRY(launch_eig_step<T>(0, L.r.get(), L.b.get(), L.d.get(), nullptr, L.inv.get(), n, red.red(), eig_sc.get(),
-                                eig_hist.get(), 0, s));
+                                hist, 0, s));
     for (int k = 0; k < it; ++k) {
       MPDG_TRY(level_apply<T>(L.od, L.d.get(), L.Ax.get(), s));
       MPDG_TRY(launch_eig_step<T>(1, L.r.get(), L.b.get(), L.d.get(), L.Ax.get(), L.inv.get(), n, red.red(),
-                                  eig_sc.get(), eig_hist.get(), k, s));
+                                  eig_sc.get(), hist, k, s));
       MPDG_TRY(launch_eig_step<T>(2, L.r.get(), L.b.get(), L.d.get(), L.Ax.get(), L.inv.get(), n, red.red(),
-                                  eig_sc.get(), eig_hist.get(), k, s));
+                                  eig_sc.get(), hist, k, s));
       MPDG_TRY(launch_eig_step<T>(3, L.r.get(), L.b.get(), L.d.get(), L.Ax.get(), L.inv.get(), n, red.red(),
-                                  eig_sc.get(), eig_hist.get(), k, s));
+                                  eig_sc.get(), hist, k, s));
     }
-    std::vector<double> h(2 * it);
-    if (hipMemcpyAsync(h.data(), eig_hist.get(), h.size() * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
-      return MPDG_ERR_DEVICE;
-    if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+    return MPDG_OK;
+  }
+
+  // 1.2 x the largest eigenvalue of the Lanczos tridiagonal matrix of the CG coefficients
+  // (deal.II's PreconditionChebyshev estimate)
+  double lambda_max_from(const double *h) const {
+    const int it = prm.eig_cg_n_iterations;
     std::vector<double> alphas, betas;
     for (int k = 0; k < it; ++k) {
       const double a = h[2 * k], b = h[2 * k + 1];
@@ -604,8 +621,7 @@ struct GMGPrecond : mpdg_precond {
       dd[j] = 1.0 / alphas[j] + (j > 0 ? betas[j - 1] / alphas[j - 1] : 0.0);
       if (j + 1 < m) ee[j] = std::sqrt(betas[j]) / alphas[j];
     }
-    L.lambda_max = 1.2 * tridiag_max_eig(dd, ee);
-    return MPDG_OK;
+    return 1.2 * tridiag_max_eig(dd, ee);
   }
 
   // PreconditionChebyshev coefficients on [lambda_max/range, lambda_max]
@@ -1275,9 +1291,9 @@ int mpdg_heat_op_compute_diagonal(const mpdg_heat_op *op, void *diag, void *stre
   if (!op || !diag) return MPDG_ERR_INVALID;
   if (op->precision == MPDG_FP64) return op->diagonal64((double *)diag, S(stream));
   const int n1 = op->d32.degree + 1;
-  DeviceArray<float> kq;
-  MPDG_TRY(kq.resize((size_t)op->d32.n_cells * n1 * n1 * n1));
-  MPDG_TRY(launch_diagonal<float>(op->d32, (float *)diag, kq.get(), S(stream)));
+  DeviceArray<float> scratch;
+  MPDG_TRY(scratch.resize(2 * (size_t)op->d32.n_cells * n1 * n1 * n1));
+  MPDG_TRY(launch_diagonal<float>(op->d32, (float *)diag, scratch.get(), S(stream)));
   return hipStreamSynchronize(S(stream)) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 int mpdg_heat_op_create_rhs(const mpdg_heat_op *op, double *dst, const double *T, const double *T_old,

@@ -145,6 +145,7 @@ struct mpdg_heat_op {
   double flux[6] = {0, 0, 0, 0, 0, 0};
   const double *T_robin = nullptr;
   mutable mpdg::DeviceArray<double> kq, tmp1, tmp2, scal, tables, metric64, metric64_k;
+  mutable mpdg::DeviceArray<double> diag_scratch;  // launch_diagonal: the cells' entries, k at the points
   mpdg::DeviceArray<float> metric32;
   mutable mpdg::DeviceArray<float> metric32_k, kq32;
   const double *coords = nullptr;

@@ -128,8 +128,9 @@ int launch_apply(const OperatorData<Number> &op, const Number *src, Number *dst,
 size_t apply_edge_elems(int degree, const int *n_cells, bool fp32, int mapped = -1);
 
 // heat_aux.hip: diagonal, boundary faces, residual, norms, conductivity
+// diag = the operator's diagonal (cells only; constrained rows 1); scratch: 2 n_cells (k+1)^3 values
 template <typename Number>
-int launch_diagonal(const OperatorData<Number> &op, Number *diag, Number *kq_scratch, hipStream_t stream);
+int launch_diagonal(const OperatorData<Number> &op, Number *diag, Number *scratch, hipStream_t stream);
 // boundary faces: mode 0 Robin tangent on src, 1 its diagonal, 2 boundary residual (+ surface flux)
 int launch_boundary(const OperatorData<double> &op, int mode, const double *flux, double *dst,
                     const double *src, const double *T, hipStream_t stream);
