@@ -96,15 +96,14 @@ __global__ __launch_bounds__(256) void conv3_kernel(const unsigned short *__rest
 // weight gradient: dW[tap][co][ci] = sum_v g[v][co] in[v+off(tap)][ci], db[co] = sum_v g[v][co]
 // (g = gradient of the layer's pre-activation). MFMA with the voxels as the
 // contraction: A (M = co, K = voxel): lane holds g[v = x0+4(l/16)+j][co = l%16];
-// B (K = voxel, N = ci): lane holds in[v+off][ci = l%16]. Block partials are
-// written to part[block][27*256 + 16] and summed in fixed order by wgrad_reduce.
+// B (K = voxel, N = ci): lane holds in[v+off][ci = l%16]. Every wavefront walks its own tiles
+// in a fixed order and stores its partial sums to part[wavefront][27*256 + 16] (no two lanes of a
+// wavefront share an entry); wgrad_reduce sums the wavefronts' partials in a fixed order, so the
+// gradient is bitwise repeatable.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void wgrad_kernel(const unsigned short *__restrict__ g,
                                                     const unsigned short *__restrict__ in, int nx, int ny, int nz,
                                                     float *__restrict__ part) {
-  __shared__ float red[27 * 256 + 16];
-  for (int i = threadIdx.x; i < 27 * 256 + 16; i += 256) red[i] = 0.f;
-  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
   const int nwaves = (gridDim.x * 256) >> 6;
@@ -142,24 +141,23 @@ __global__ __launch_bounds__(256) void wgrad_kernel(const unsigned short *__rest
     }
   }
   // D[m = co = 4 kg + i][n = ci]
+  float *mine = part + (long)wave * (27 * 256 + 16);
 #pragma unroll
   for (int t = 0; t < 27; ++t)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) atomicAdd(&red[t * 256 + (4 * kg + i) * 16 + n], acc[t][i]);
+    for (int i = 0; i < 4; ++i) mine[t * 256 + (4 * kg + i) * 16 + n] = acc[t][i];
   // bias: lanes with the same co (= n here, since a holds g[.][co = n]) summed over kg
   bsum += __shfl_xor(bsum, 16);
   bsum += __shfl_xor(bsum, 32);
-  if (kg == 0) atomicAdd(&red[27 * 256 + n], bsum);
-  __syncthreads();
-  for (int i = threadIdx.x; i < 27 * 256 + 16; i += 256) part[(long)blockIdx.x * (27 * 256 + 16) + i] = red[i];
+  if (kg == 0) mine[27 * 256 + n] = bsum;
 }
 
-__global__ __launch_bounds__(256) void wgrad_reduce_kernel(const float *__restrict__ part, int nblocks,
+__global__ __launch_bounds__(256) void wgrad_reduce_kernel(const float *__restrict__ part, int nparts,
                                                            float *__restrict__ dW, float *__restrict__ db) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= 27 * 256 + 16) return;
   float s = 0.f;
-  for (int b = 0; b < nblocks; ++b) s += part[(long)b * (27 * 256 + 16) + i];
+  for (int b = 0; b < nparts; ++b) s += part[(long)b * (27 * 256 + 16) + i];
   if (i < 27 * 256) dW[i] = s;
   else db[i - 27 * 256] = s;
 }
@@ -294,15 +292,23 @@ __global__ __launch_bounds__(256) void stats_partial_kernel(const double *__rest
     for (int q = 0; q < 4; ++q) part[blockIdx.x * 4 + q] = sh[q][0];
 }
 // s = rms(Tn - Tnm1) (+ tiny), mean(Tn), rms(Tn - mean) (+ tiny), sz = rms(z) (+ tiny) or s without z
-__global__ void stats_final_kernel(const double *__restrict__ part, int nb, long n, int has_z,
-                                   double *__restrict__ stats) {
-  double a = 0, b = 0, c = 0, e = 0;
-  for (int i = 0; i < nb; ++i) {
-    a += part[4 * i];
-    b += part[4 * i + 1];
-    c += part[4 * i + 2];
-    e += part[4 * i + 3];
+// one workgroup: thread t sums the partials t, t + 256, ..., then a fixed tree over the threads
+__global__ __launch_bounds__(256) void stats_final_kernel(const double *__restrict__ part, int nb, long n, int has_z,
+                                                          double *__restrict__ stats) {
+  __shared__ double red[4][256];
+  const int t = threadIdx.x;
+  double v[4] = {0, 0, 0, 0};
+  for (int i = t; i < nb; i += 256)
+    for (int j = 0; j < 4; ++j) v[j] += part[4 * i + j];
+  for (int j = 0; j < 4; ++j) red[j][t] = v[j];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+      for (int j = 0; j < 4; ++j) red[j][t] += red[j][t + w];
+    __syncthreads();
   }
+  if (t != 0) return;
+  const double a = red[0][0], b = red[1][0], c = red[2][0], e = red[3][0];
   const double mean = b / n;
   stats[0] = sqrt(a / n) + 1e-30;
   stats[1] = mean;
@@ -328,7 +334,8 @@ int cnn_conv(const unsigned short *in, unsigned short *out, const unsigned short
 int cnn_wgrad(const unsigned short *g, const unsigned short *in, const int *dims, float *part, int nblocks, float *dW,
               float *db, hipStream_t s) {
   hipLaunchKernelGGL(wgrad_kernel, dim3(nblocks), dim3(256), 0, s, g, in, dims[0], dims[1], dims[2], part);
-  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((27 * 256 + 16 + 255) / 256), dim3(256), 0, s, part, nblocks, dW, db);
+  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((27 * 256 + 16 + 255) / 256), dim3(256), 0, s, part, 4 * nblocks, dW,
+                     db);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
@@ -350,7 +357,7 @@ int cnn_stats(const double *Tnm1, const double *Tn, const double *z, long n, dou
               hipStream_t s) {
   const int nb = (int)std::min<long>(1024, (n + 255) / 256);
   hipLaunchKernelGGL(stats_partial_kernel, dim3(nb), dim3(256), 0, s, Tnm1, Tn, z, n, part);
-  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(1), 0, s, part, nb, n, z != nullptr ? 1 : 0, stats);
+  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(256), 0, s, part, nb, n, z != nullptr ? 1 : 0, stats);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 

@@ -7,6 +7,7 @@ namespace mpdg {
 
 int cnn_conv(const unsigned short *in, unsigned short *out, const unsigned short *W, const float *bias, const int *dims,
              int mode, const unsigned short *mask, hipStream_t s);
+// part: 4 nblocks (27 * 256 + 16) floats (one slot per wavefront)
 int cnn_wgrad(const unsigned short *g, const unsigned short *in, const int *dims, float *part, int nblocks, float *dW,
               float *db, hipStream_t s);
 int cnn_pack(const float *master, int cin, int cout, unsigned short *W, unsigned short *Wt, hipStream_t s);

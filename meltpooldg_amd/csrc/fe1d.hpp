@@ -90,4 +90,14 @@ struct FE1D {
   }
 };
 
+// The tables of degrees 1 .. kMaxN1 - 1, built once per process (the Newton iterations above
+// cost tens of microseconds, which a kernel launch should not pay)
+inline const FE1D &fe1d(int degree) {
+  struct Table {
+    FE1D e[kMaxN1 - 1] = {FE1D(1), FE1D(2), FE1D(3), FE1D(4), FE1D(5), FE1D(6), FE1D(7)};
+  };
+  static const Table t;
+  return t.e[std::min(std::max(degree, 1), kMaxN1 - 1) - 1];
+}
+
 }  // namespace mpdg

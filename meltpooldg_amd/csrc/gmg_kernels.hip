@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void coarse_dirichlet_kernel(double *A, long n
 template <typename T>
 int launch_coarse_matrix(const OperatorData<T> &od, double c_mass, const double *kc, double kconst, double *A,
                          hipStream_t s) {
-  FE1D fe(od.degree);
+  const FE1D &fe = fe1d(od.degree);
   CoarseTables t{};
   for (int a = 0; a < fe.n1; ++a) {
     t.w[a] = fe.qw[a];

@@ -17,7 +17,7 @@ struct AuxTables {
 };
 template <typename Number>
 AuxTables<Number> make_aux_tables(int k) {
-  FE1D fe(k);
+  const FE1D &fe = fe1d(k);
   AuxTables<Number> t{};
   for (int a = 0; a < fe.n1; ++a) {
     t.w[a] = (Number)fe.qw[a];
@@ -473,7 +473,7 @@ int launch_boundary(const OperatorData<double> &op, int mode, const double *flux
                     const double *src, const double *T, hipStream_t stream) {
   FaceArgs A{};
   A.t = make_aux_tables<double>(op.degree);
-  FE1D fe(op.degree);
+  const FE1D &fe = fe1d(op.degree);
   for (int i = 0; i < fe.n1; ++i) A.qp[i] = fe.qp[i];
   A.n1 = op.degree + 1;
   for (int d = 0; d < 3; ++d) {
@@ -691,7 +691,7 @@ __global__ __launch_bounds__(256) void surface_flux_field_kernel(double *__restr
 }
 int launch_surface_flux_field(const OperatorData<double> &op, const double *p6, double *out, hipStream_t s) {
   FluxNodes fn{};
-  FE1D fe(op.degree);
+  const FE1D &fe = fe1d(op.degree);
   for (int i = 0; i < fe.n1; ++i) fn.nodes[i] = fe.nodes[i];
   for (int i = 0; i < 6; ++i) fn.p[i] = p6[i];
   hipLaunchKernelGGL(surface_flux_field_kernel, dim3(std::min<long>((op.n_dofs + 255) / 256, 8192)), dim3(256), 0, s,

@@ -173,7 +173,7 @@ int mpdg_io_write_vtu(const char *filename, int degree, const int *n_cells, cons
     ok = ok && write_array(f, "UInt8", "types", 1, type);
   }
   std::fprintf(f, "      </Cells>\n      <PointData>\n");
-  FE1D fe(degree);
+  const FE1D &fe = mpdg::fe1d(degree);
   for (int i = 0; i < n_fields && ok; ++i) {
     std::vector<double> v;
     to_patch_vertices(fe, nd, fields[i], v);
@@ -208,7 +208,7 @@ int mpdg_io_dof_permutation(int degree, const int *n_cells, const double *extent
     h[d] = extent[d] / n_cells[d];
   }
   if (n != (long)nd[0] * nd[1] * nd[2]) return MPDG_ERR_INVALID;
-  FE1D fe(degree);
+  const FE1D &fe = mpdg::fe1d(degree);
   std::vector<char> seen(n, 0);
   for (long i = 0; i < n; ++i) {
     long idx[3];

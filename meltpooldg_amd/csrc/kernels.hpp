@@ -93,7 +93,7 @@ struct ApplyArgs {
 
 template <typename Number, int N1>
 void fill_apply_args(const OperatorData<Number> &op, ApplyArgs<Number, N1> &A) {
-  FE1D fe(N1 - 1);
+  const FE1D &fe = fe1d(N1 - 1);
   for (int a = 0; a < N1; ++a) A.w[a] = (Number)fe.qw[a];
   A.S.set(fe.S);
   A.D.set(fe.D);

@@ -113,7 +113,7 @@ struct mpdg_learned {
     for (int l = 0; l <= kLayers; ++l) MPDG_TRY(act[l].resize((size_t)n * 16));
     MPDG_TRY(g0.resize((size_t)n * 16));
     MPDG_TRY(g1.resize((size_t)n * 16));
-    MPDG_TRY(part.resize((size_t)kWgradBlocks * (27 * 256 + 16)));
+    MPDG_TRY(part.resize((size_t)kWgradBlocks * 4 * (27 * 256 + 16)));  // one slot per wavefront
     MPDG_TRY(stats.resize(4));
     MPDG_TRY(spart.resize(4 * 1024));
     MPDG_TRY(lpart.resize(1024));

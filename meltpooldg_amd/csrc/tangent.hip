@@ -30,7 +30,7 @@ struct Tables1D {
   double S[kMaxN1][kMaxN1], G[kMaxN1][kMaxN1], w[kMaxN1];
 };
 Tables1D make_tables(int k) {
-  FE1D fe(k);
+  const FE1D &fe = fe1d(k);
   Tables1D t{};
   for (int a = 0; a < fe.n1; ++a) {
     t.w[a] = fe.qw[a];

@@ -103,6 +103,25 @@ def test_training_reduces_loss(cuda):
     assert last < 0.2 * first
 
 
+def test_training_is_bitwise_repeatable(cuda):
+    """Two networks with the same seed trained on the same samples end with identical weights and
+    losses: the weight gradient sums its wavefronts' partials in a fixed order (no atomics)."""
+    torch = cuda
+    import meltpooldg_amd as M
+    shape = (17, 17, 33)
+    runs = []
+    for _ in range(2):
+        net, _ = make_net(M, shape, degree=2, rho=2, qscale=0.5, seed=7)
+        losses = []
+        for s in range(3):
+            d = [to_dev(torch, a) for a in sample(shape, seed=10 + s)]
+            for _ in range(4):
+                losses.append(net.train_step(*d, 2e-3))
+        runs.append((losses, net.get_weights()))
+    assert runs[0][0] == runs[1][0]
+    assert np.array_equal(runs[0][1], runs[1][1])
+
+
 def test_output_scale_is_set_by_the_first_sample(cuda):
     """The first train step sets the output scale to the target's rms in units of rms(z), and
     predictions use it as the oracle does."""
