@@ -19,14 +19,15 @@ __device__ __forceinline__ bool constrained(long g, int nd0, int nd1, int nd2, i
 // One thread per (cell, I, J) entry of the element matrix
 //   A_e[I][J] = sum_q m_q phi_I phi_J + k_cell grad phi_I . G_q grad phi_J,
 // Cartesian cells: m_q = c_m vol W_q, G_q = diag(vol/h_d^2) W_q; mapped cells: m_q = c_m JxW_q,
-// G_q = JxW_q B B^T. Summed into the dense row-major matrix A; rows and columns of
-// constrained DoFs are left out (their identity diagonal is set separately).
+// G_q = JxW_q B B^T, stored per cell in E[cell][I][J]; coarse_gather_kernel sums them into the
+// dense row-major matrix A in a fixed order (no atomics); rows and columns of constrained DoFs
+// are left out (their identity diagonal is set separately).
 template <typename T>
 __global__ __launch_bounds__(256) void coarse_matrix_kernel(CoarseTables t, int n1, int nc0, int nc1, long ncells,
                                                             int nd0, int nd1, int nd2, int dmask, double cm,
                                                             double ks0, double ks1, double ks2, double c_mass,
                                                             const double *__restrict__ kc, double kconst,
-                                                            const T *__restrict__ metric, double *__restrict__ A) {
+                                                            const T *__restrict__ metric, double *__restrict__ E) {
   const int nloc = n1 * n1 * n1, K = n1 - 1;
   const long gid = (long)blockIdx.x * 256 + threadIdx.x;
   if (gid >= ncells * nloc * nloc) return;
@@ -35,9 +36,6 @@ __global__ __launch_bounds__(256) void coarse_matrix_kernel(CoarseTables t, int 
   const int cx = cell % nc0, cy = (cell / nc0) % nc1, cz = cell / ((long)nc0 * nc1);
   const int i0 = I % n1, i1 = (I / n1) % n1, i2 = I / (n1 * n1);
   const int j0 = J % n1, j1 = (J / n1) % n1, j2 = J / (n1 * n1);
-  const long gI = (long)(K * cx + i0) + (long)nd0 * ((K * cy + i1) + (long)nd1 * (K * cz + i2));
-  const long gJ = (long)(K * cx + j0) + (long)nd0 * ((K * cy + j1) + (long)nd1 * (K * cz + j2));
-  if (dmask && (constrained(gI, nd0, nd1, nd2, dmask) || constrained(gJ, nd0, nd1, nd2, dmask))) return;
   const double kap = kc ? kc[cell] : kconst;
   double acc = 0.0;
   for (int qz = 0; qz < n1; ++qz)
@@ -62,7 +60,43 @@ __global__ __launch_bounds__(256) void coarse_matrix_kernel(CoarseTables t, int 
           acc += W * (cm * pI * pJ + kap * (ks0 * a[0] * b[0] + ks1 * a[1] * b[1] + ks2 * a[2] * b[2]));
         }
       }
-  atomicAdd(A + gI * (nd0 * (long)nd1 * nd2) + gJ, acc);
+  E[gid] = acc;
+}
+
+// A[I][J] = sum over the cells holding both DoFs (z, y, x order) of E[cell][I_loc][J_loc];
+// along each direction the cells holding node index x are c with K c <= x <= K c + K
+__global__ __launch_bounds__(256) void coarse_gather_kernel(const double *__restrict__ E, int n1, int nc0, int nc1,
+                                                            int nc2, int nd0, int nd1, int nd2, int dmask,
+                                                            double *__restrict__ A) {
+  const int K = n1 - 1, nloc = n1 * n1 * n1;
+  const long n = (long)nd0 * nd1 * nd2;
+  const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+  if (gid >= n * n) return;
+  const long gI = gid / n, gJ = gid - gI * n;
+  double acc = 0.0;
+  if (!(dmask && (constrained(gI, nd0, nd1, nd2, dmask) || constrained(gJ, nd0, nd1, nd2, dmask)))) {
+    const int ii[3] = {(int)(gI % nd0), (int)((gI / nd0) % nd1), (int)(gI / ((long)nd0 * nd1))};
+    const int jj[3] = {(int)(gJ % nd0), (int)((gJ / nd0) % nd1), (int)(gJ / ((long)nd0 * nd1))};
+    const int nc[3] = {nc0, nc1, nc2};
+    int lo[3], hi[3];
+    bool empty = false;
+    for (int d = 0; d < 3; ++d) {
+      const int mx = max(ii[d], jj[d]), mn = min(ii[d], jj[d]);
+      lo[d] = max(0, (mx + K - 1) / K - 1);
+      hi[d] = min(nc[d] - 1, mn / K);
+      empty = empty || lo[d] > hi[d];
+    }
+    if (!empty)
+      for (int cz = lo[2]; cz <= hi[2]; ++cz)
+        for (int cy = lo[1]; cy <= hi[1]; ++cy)
+          for (int cx = lo[0]; cx <= hi[0]; ++cx) {
+            const long cell = cx + (long)nc0 * (cy + (long)nc1 * cz);
+            const int I = (ii[0] - K * cx) + n1 * ((ii[1] - K * cy) + n1 * (ii[2] - K * cz));
+            const int J = (jj[0] - K * cx) + n1 * ((jj[1] - K * cy) + n1 * (jj[2] - K * cz));
+            acc += E[(cell * nloc + I) * nloc + J];
+          }
+  }
+  A[gid] = acc;
 }
 
 __global__ __launch_bounds__(256) void coarse_dirichlet_kernel(double *A, long n, int nd0, int nd1, int nd2,
@@ -73,7 +107,7 @@ __global__ __launch_bounds__(256) void coarse_dirichlet_kernel(double *A, long n
 
 template <typename T>
 int launch_coarse_matrix(const OperatorData<T> &od, double c_mass, const double *kc, double kconst, double *A,
-                         hipStream_t s) {
+                         double *E, hipStream_t s) {
   const FE1D &fe = fe1d(od.degree);
   CoarseTables t{};
   for (int a = 0; a < fe.n1; ++a) {
@@ -85,12 +119,12 @@ int launch_coarse_matrix(const OperatorData<T> &od, double c_mass, const double 
   }
   const long n = od.n_dofs, nloc = (long)fe.n1 * fe.n1 * fe.n1, work = od.n_cells * nloc * nloc;
   const double vol = od.h[0] * od.h[1] * od.h[2];
-  const int rc = launch_fill<double>(A, 0.0, n * n, s);
-  if (rc) return rc;
   hipLaunchKernelGGL(coarse_matrix_kernel<T>, dim3((work + 255) / 256), dim3(256), 0, s, t, fe.n1, od.nc[0],
                      od.nc[1], od.n_cells, od.nd[0], od.nd[1], od.nd[2], od.dmask, c_mass * vol,
                      vol / (od.h[0] * od.h[0]), vol / (od.h[1] * od.h[1]), vol / (od.h[2] * od.h[2]), c_mass, kc,
-                     kconst, od.metric, A);
+                     kconst, od.metric, E);
+  hipLaunchKernelGGL(coarse_gather_kernel, dim3((n * n + 255) / 256), dim3(256), 0, s, E, fe.n1, od.nc[0], od.nc[1],
+                     od.nc[2], od.nd[0], od.nd[1], od.nd[2], od.dmask, A);
   if (od.dmask)
     hipLaunchKernelGGL(coarse_dirichlet_kernel, dim3((n + 255) / 256), dim3(256), 0, s, A, n, od.nd[0], od.nd[1],
                        od.nd[2], od.dmask);
@@ -243,8 +277,8 @@ int launch_gauss_jordan_inverse(double *A, int n, hipStream_t s) {
 }
 
 template int launch_coarse_matrix<double>(const OperatorData<double> &, double, const double *, double, double *,
-                                          hipStream_t);
+                                          double *, hipStream_t);
 template int launch_coarse_matrix<float>(const OperatorData<float> &, double, const double *, double, double *,
-                                         hipStream_t);
+                                         double *, hipStream_t);
 
 }  // namespace mpdg

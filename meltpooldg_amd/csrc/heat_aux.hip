@@ -359,7 +359,7 @@ __global__ __launch_bounds__(256) void diag_dirichlet_kernel(Number *__restrict_
   if (con) d[gid] = Number(1);
 }
 
-// Boundary faces, thread per (boundary cell face, face DoF I).
+// Boundary faces (Robin terms, surface flux); the sums over faces and cells per DoF are below.
 // mode 0: Robin tangent applied to src: dst_I += sum_q phi_I (alpha + 4 eps sigma T_q^3) u_q JxW_q
 //         (heat_diffuse_operator.cpp:1090-1129)
 // mode 1: its diagonal
@@ -378,95 +378,133 @@ struct FaceArgs {
   long n_dofs;
 };
 
-__global__ __launch_bounds__(256) void boundary_face_kernel(FaceArgs A, const double *__restrict__ src,
-                                                            const double *__restrict__ T, double *__restrict__ dst) {
-  const int n1 = A.n1, K = n1 - 1, nf = n1 * n1;
-  long gid = (long)blockIdx.x * 256 + threadIdx.x;
-  for (int f = 0; f < 6; ++f) {
-    if (!((A.face_mask >> f) & 1)) continue;
-    const int d = f / 2, s = f % 2;
-    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
-    const long count = (long)A.nc[ta] * A.nc[tb] * nf;
-    if (gid >= count) { gid -= count; continue; }
-    const long fc = gid / nf;
-    const int a = gid - fc * nf;
-    const int ia = a % n1, ib = a / n1;
-    int c[3];
-    c[d] = s ? A.nc[d] - 1 : 0;
-    c[ta] = fc % A.nc[ta];
-    c[tb] = fc / A.nc[ta];
-    auto dof = [&](int xa, int xb) {
-      int idx[3];
-      idx[d] = K * c[d] + s * K;
-      idx[ta] = K * c[ta] + xa;
-      idx[tb] = K * c[tb] + xb;
-      return (long)idx[0] + (long)A.nd[0] * (idx[1] + (long)A.nd[1] * idx[2]);
-    };
-    auto constrained = [&](long g) {
-      const int ix = g % A.nd[0], iy = (g / A.nd[0]) % A.nd[1], iz = g / ((long)A.nd[0] * A.nd[1]);
-      const int m = A.dmask;
-      return ((m & 1) && ix == 0) || ((m & 2) && ix == A.nd[0] - 1) || ((m & 4) && iy == 0) ||
-             ((m & 8) && iy == A.nd[1] - 1) || ((m & 16) && iz == 0) || ((m & 32) && iz == A.nd[2] - 1);
-    };
-    const long gI = dof(ia, ib);
-    if (constrained(gI)) return;
-    const bool robin = (A.robin_mask >> f) & 1;
-    double acc = 0.0;
-    for (int qb = 0; qb < n1; ++qb)
-      for (int qa = 0; qa < n1; ++qa) {
-        double Tq = 0.0, uq = 0.0;
-        double dxa[3] = {0.0, 0.0, 0.0}, dxb[3] = {0.0, 0.0, 0.0}, xq[3] = {0.0, 0.0, 0.0};
-        for (int xb = 0; xb < n1; ++xb)
-          for (int xa = 0; xa < n1; ++xa) {
-            const double phi = A.t.S[qa][xa] * A.t.S[qb][xb];
-            const long g = dof(xa, xb);
-            if (T) Tq += phi * T[g];
-            if (A.mode == 0 && !constrained(g)) uq += phi * src[g];
-            if (A.coords) {
-              const double ga = A.t.G[qa][xa] * A.t.S[qb][xb], gb = A.t.S[qa][xa] * A.t.G[qb][xb];
-              for (int c = 0; c < 3; ++c) {
-                const double X = A.coords[c * A.n_dofs + g];
-                dxa[c] += ga * X;
-                dxb[c] += gb * X;
-                xq[c] += phi * X;
-              }
+__device__ __forceinline__ bool face_dof_constrained(const FaceArgs &A, long g) {
+  const int ix = g % A.nd[0], iy = (g / A.nd[0]) % A.nd[1], iz = g / ((long)A.nd[0] * A.nd[1]);
+  const int m = A.dmask;
+  return ((m & 1) && ix == 0) || ((m & 2) && ix == A.nd[0] - 1) || ((m & 4) && iy == 0) ||
+         ((m & 8) && iy == A.nd[1] - 1) || ((m & 16) && iz == 0) || ((m & 32) && iz == A.nd[2] - 1);
+}
+
+// contribution of face f of cell c to the face DoF (ia, ib) of that cell face
+__device__ double face_contribution(const FaceArgs &A, const double *__restrict__ src, const double *__restrict__ T,
+                                    int f, const int *c, int ia, int ib) {
+  const int n1 = A.n1, K = n1 - 1;
+  const int d = f / 2, s = f % 2;
+  const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+  auto dof = [&](int xa, int xb) {
+    int idx[3];
+    idx[d] = K * c[d] + s * K;
+    idx[ta] = K * c[ta] + xa;
+    idx[tb] = K * c[tb] + xb;
+    return (long)idx[0] + (long)A.nd[0] * (idx[1] + (long)A.nd[1] * idx[2]);
+  };
+  const bool robin = (A.robin_mask >> f) & 1;
+  double acc = 0.0;
+  for (int qb = 0; qb < n1; ++qb)
+    for (int qa = 0; qa < n1; ++qa) {
+      double Tq = 0.0, uq = 0.0;
+      double dxa[3] = {0.0, 0.0, 0.0}, dxb[3] = {0.0, 0.0, 0.0}, xq[3] = {0.0, 0.0, 0.0};
+      for (int xb = 0; xb < n1; ++xb)
+        for (int xa = 0; xa < n1; ++xa) {
+          const double phi = A.t.S[qa][xa] * A.t.S[qb][xb];
+          const long g = dof(xa, xb);
+          if (T) Tq += phi * T[g];
+          if (A.mode == 0 && !face_dof_constrained(A, g)) uq += phi * src[g];
+          if (A.coords) {
+            const double ga = A.t.G[qa][xa] * A.t.S[qb][xb], gb = A.t.S[qa][xa] * A.t.G[qb][xb];
+            for (int e = 0; e < 3; ++e) {
+              const double X = A.coords[e * A.n_dofs + g];
+              dxa[e] += ga * X;
+              dxb[e] += gb * X;
+              xq[e] += phi * X;
             }
           }
-        double JxW = A.h[ta] * A.h[tb] * A.t.w[qa] * A.t.w[qb];
-        if (A.coords) {
-          const double c0 = dxa[1] * dxb[2] - dxa[2] * dxb[1], c1 = dxa[2] * dxb[0] - dxa[0] * dxb[2],
-                       c2 = dxa[0] * dxb[1] - dxa[1] * dxb[0];
-          JxW = sqrt(c0 * c0 + c1 * c1 + c2 * c2) * A.t.w[qa] * A.t.w[qb];
         }
-        const double phiI = A.t.S[qa][ia] * A.t.S[qb][ib];
-        double val = 0.0;
-        if (A.mode == 2) {
-          if (robin && A.emissivity > 0) {
-            const double T4 = Tq * Tq * Tq * Tq, Ti = A.T_inf_rad;
-            val -= A.emissivity * kStefanBoltzmann * (T4 - Ti * Ti * Ti * Ti);
-          }
-          if (robin && A.alpha > 0) val -= A.alpha * (Tq - A.T_inf_conv);
-          if (A.flux_on && f == 5) {
-            double pos[3];
-            pos[d] = (c[d] + s) * A.h[d];
-            pos[ta] = (c[ta] + A.qp[qa]) * A.h[ta];
-            pos[tb] = (c[tb] + A.qp[qb]) * A.h[tb];
-            if (A.coords)
-              for (int e = 0; e < 3; ++e) pos[e] = xq[e];
-            const double dx = pos[0] - A.flux[2] - A.flux[4] * A.flux[5], dy = pos[1] - A.flux[3];
-            val += A.flux[0] * exp(-(dx * dx + dy * dy) / (2.0 * A.flux[1] * A.flux[1]));
-          }
-        } else {
-          double coef = 0.0;
-          if (A.alpha > 0) coef += A.alpha;
-          if (A.emissivity > 0) coef += 4.0 * A.emissivity * kStefanBoltzmann * Tq * Tq * Tq;
-          val = coef * (A.mode == 0 ? uq : phiI);
-        }
-        acc += phiI * val * JxW;
+      double JxW = A.h[ta] * A.h[tb] * A.t.w[qa] * A.t.w[qb];
+      if (A.coords) {
+        const double c0 = dxa[1] * dxb[2] - dxa[2] * dxb[1], c1 = dxa[2] * dxb[0] - dxa[0] * dxb[2],
+                     c2 = dxa[0] * dxb[1] - dxa[1] * dxb[0];
+        JxW = sqrt(c0 * c0 + c1 * c1 + c2 * c2) * A.t.w[qa] * A.t.w[qb];
       }
-    atomicAdd(dst + gI, acc);
-    return;
+      const double phiI = A.t.S[qa][ia] * A.t.S[qb][ib];
+      double val = 0.0;
+      if (A.mode == 2) {
+        if (robin && A.emissivity > 0) {
+          const double T4 = Tq * Tq * Tq * Tq, Ti = A.T_inf_rad;
+          val -= A.emissivity * kStefanBoltzmann * (T4 - Ti * Ti * Ti * Ti);
+        }
+        if (robin && A.alpha > 0) val -= A.alpha * (Tq - A.T_inf_conv);
+        if (A.flux_on && f == 5) {
+          double pos[3];
+          pos[d] = (c[d] + s) * A.h[d];
+          pos[ta] = (c[ta] + A.qp[qa]) * A.h[ta];
+          pos[tb] = (c[tb] + A.qp[qb]) * A.h[tb];
+          if (A.coords)
+            for (int e = 0; e < 3; ++e) pos[e] = xq[e];
+          const double dx = pos[0] - A.flux[2] - A.flux[4] * A.flux[5], dy = pos[1] - A.flux[3];
+          val += A.flux[0] * exp(-(dx * dx + dy * dy) / (2.0 * A.flux[1] * A.flux[1]));
+        }
+      } else {
+        double coef = 0.0;
+        if (A.alpha > 0) coef += A.alpha;
+        if (A.emissivity > 0) coef += 4.0 * A.emissivity * kStefanBoltzmann * Tq * Tq * Tq;
+        val = coef * (A.mode == 0 ? uq : phiI);
+      }
+      acc += phiI * val * JxW;
+    }
+  return acc;
+}
+
+// One thread per boundary DoF: the DoFs of the six face grids, each handled by the first face
+// (in the order x-, x+, y-, y+, z-, z+) that holds it. The thread sums the contributions of every
+// selected face holding the DoF and, on each, of the (up to four) cells whose face holds it, in
+// that fixed order, and adds the total to dst once: no atomics, so the result is bitwise
+// repeatable.
+__global__ __launch_bounds__(256) void boundary_face_kernel(FaceArgs A, const double *__restrict__ src,
+                                                            const double *__restrict__ T, double *__restrict__ dst) {
+  const int K = A.n1 - 1;
+  long gid = (long)blockIdx.x * 256 + threadIdx.x;
+  int idx[3];
+  int owner = -1;
+  for (int f = 0; f < 6 && owner < 0; ++f) {
+    const int d = f / 2, s = f % 2, ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    const long count = (long)A.nd[ta] * A.nd[tb];
+    if (gid >= count) {
+      gid -= count;
+      continue;
+    }
+    idx[d] = s ? A.nd[d] - 1 : 0;
+    idx[ta] = (int)(gid % A.nd[ta]);
+    idx[tb] = (int)(gid / A.nd[ta]);
+    owner = f;
   }
+  if (owner < 0) return;
+  auto on_face = [&](int f) { return idx[f / 2] == ((f % 2) ? A.nd[f / 2] - 1 : 0); };
+  for (int f = 0; f < owner; ++f)
+    if (on_face(f)) return;  // an earlier face owns this DoF
+  const long gI = (long)idx[0] + (long)A.nd[0] * (idx[1] + (long)A.nd[1] * idx[2]);
+  if (face_dof_constrained(A, gI)) return;
+  double total = 0.0;
+  bool any = false;
+  for (int f = 0; f < 6; ++f) {
+    if (!((A.face_mask >> f) & 1) || !on_face(f)) continue;
+    const int d = f / 2, s = f % 2, ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    // cells along ta and tb whose face holds the DoF: the one at or below it, and the one
+    // before when the DoF sits on their shared edge
+    const int ca = min(idx[ta] / K, A.nc[ta] - 1), cb = min(idx[tb] / K, A.nc[tb] - 1);
+    const int la = idx[ta] - K * ca, lb = idx[tb] - K * cb;
+    const int two_a = (la == 0 && ca > 0) ? 1 : 0, two_b = (lb == 0 && cb > 0) ? 1 : 0;
+    for (int sb = two_b; sb >= 0; --sb)
+      for (int sa = two_a; sa >= 0; --sa) {
+        int c[3];
+        c[d] = s ? A.nc[d] - 1 : 0;
+        c[ta] = ca - sa;
+        c[tb] = cb - sb;
+        total += face_contribution(A, src, T, f, c, sa ? K : la, sb ? K : lb);
+        any = true;
+      }
+  }
+  if (any) dst[gI] += total;
 }
 
 int launch_boundary(const OperatorData<double> &op, int mode, const double *flux, double *dst,
@@ -498,12 +536,11 @@ int launch_boundary(const OperatorData<double> &op, int mode, const double *flux
   A.T_inf_rad = op.T_inf_rad;
   if (flux)
     for (int i = 0; i < 6; ++i) A.flux[i] = flux[i];
-  long total = 0;
-  for (int f = 0; f < 6; ++f)
-    if ((A.face_mask >> f) & 1) {
-      const int d = f / 2, ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
-      total += (long)op.nc[ta] * op.nc[tb] * A.n1 * A.n1;
-    }
+  long total = 0;  // the DoFs of all six face grids (each boundary DoF has one owner among them)
+  for (int f = 0; f < 6; ++f) {
+    const int d = f / 2, ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    total += (long)op.nd[ta] * op.nd[tb];
+  }
   hipLaunchKernelGGL(boundary_face_kernel, dim3((total + 255) / 256), dim3(256), 0, stream, A, src, T, dst);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }

@@ -345,7 +345,7 @@ template <typename T>
 struct GMGPrecond : mpdg_precond {
   mpdg_gmg_params prm{};
   std::vector<std::unique_ptr<Level<T>>> lev;  // 0 = coarsest
-  DeviceArray<double> coarse_inv, cvec_in, cvec_out, cvec_part, eig_sc, eig_hist, kcell64, kcell_pp[2];
+  DeviceArray<double> coarse_inv, coarse_elem, cvec_in, cvec_out, cvec_part, eig_sc, eig_hist, kcell64, kcell_pp[2];
   DeviceArray<T> edge;  // apply scratch shared by the levels (one stream, zero between applies)
   ReductionScratch red;
   int nc_coarse_dofs = 0;
@@ -650,7 +650,9 @@ struct GMGPrecond : mpdg_precond {
       MPDG_TRY(launch_cell_average<double>(f, kcell64.get(), s));
       kc = kcell64.get();
     }
-    MPDG_TRY(launch_coarse_matrix<T>(od, f.c_mass, kc, f.kappa, coarse_inv.get(), s));
+    const size_t n1 = od.degree + 1;
+    MPDG_TRY(coarse_elem.resize((size_t)od.n_cells * n1 * n1 * n1 * n1 * n1 * n1));
+    MPDG_TRY(launch_coarse_matrix<T>(od, f.c_mass, kc, f.kappa, coarse_inv.get(), coarse_elem.get(), s));
     // the inverse of a symmetric matrix is symmetric: row-major == column-major for the gemv
     return launch_gauss_jordan_inverse(coarse_inv.get(), (int)od.n_dofs, s);
   }

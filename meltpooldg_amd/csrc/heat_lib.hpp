@@ -16,10 +16,11 @@
 namespace mpdg {
 
 // dense coarse-grid matrix (row-major, n_dofs^2) of the level `od` with cell conductivities kc
-// (nullptr: kconst everywhere); constrained DoFs get identity rows and columns
+// (nullptr: kconst everywhere); constrained DoFs get identity rows and columns. E: scratch of
+// n_cells (k+1)^6 element-matrix entries
 template <typename T>
 int launch_coarse_matrix(const OperatorData<T> &od, double c_mass, const double *kc, double kconst, double *A,
-                         hipStream_t s);
+                         double *E, hipStream_t s);
 int launch_gauss_jordan_inverse(double *A, int n, hipStream_t s);
 
 #define MPDG_TRY_INNER(...)         \

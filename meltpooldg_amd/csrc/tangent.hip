@@ -15,10 +15,12 @@
 // Coefficients a problem does not have (no velocity, no solidification) are neither written
 // nor read. The apply, residual and diagonal kernels are cell-local sum factorization with one
 // thread per line of the cell's N1^3 tensor (N1^2 threads per cell) and LDS transposes between
-// the directions. When a block's cells lie in one x-row of cells (the usual case), the block
-// loads and stores its DoFs as N1^2 lines along x with consecutive lanes on consecutive
-// addresses, sums the shared x-faces in LDS, stores the DoFs only it touches and adds the rest
-// with FP64 atomics; otherwise every cell adds its rows with atomics.
+// the directions. A block holds a segment of an x-row of cells; it loads and stores its DoFs as
+// N1^2 lines along x with consecutive lanes on consecutive addresses, sums the shared x-faces in
+// LDS, stores the DoFs only it touches and adds the rest to dst. The kernels that add run in
+// eight launches, one per colour (parity of the segment, of the row's y and of its z index):
+// blocks of one colour share no DoF, so the adds need no atomics, and the colours follow each
+// other in a fixed order, which makes the result bitwise repeatable.
 #include "blas.hpp"
 #include "tangent.hpp"
 
@@ -46,7 +48,8 @@ template <int N1>
 constexpr int kCells = 256 / (N1 * N1);
 
 struct Geometry {
-  int nc0, nc1;
+  int nc0, nc1, nc2;
+  int color;  // 0..7: the blocks of one colour (segment, y, z parities); -1: all blocks
   long ncells;
   int nd[3];
   int dmask;
@@ -65,9 +68,12 @@ __device__ __forceinline__ bool is_constrained(int m, const int *nd, int ix, int
 template <int N1>
 constexpr int kLine = kCells<N1> * (N1 - 1) + 1;
 
-inline long tangent_blocks(int n1, const int *nc) {
+inline long tangent_blocks(int n1, const int *nc, int color = -1) {
   const int cpb = 256 / (n1 * n1);
-  return (long)nc[1] * nc[2] * ((nc[0] + cpb - 1) / cpb);
+  const long nseg = (nc[0] + cpb - 1) / cpb;
+  if (color < 0) return (long)nc[1] * nc[2] * nseg;
+  const int px = color & 1, py = (color >> 1) & 1, pz = (color >> 2) & 1;
+  return ((nseg - px + 1) / 2) * (long)((nc[1] - py + 1) / 2) * ((nc[2] - pz + 1) / 2);
 }
 
 // The thread's place: cell `cell` (valid when `active`), line (a, b) of the cell's tensor;
@@ -84,12 +90,26 @@ struct Place {
     a = r / N1;
     b = r - a * N1;
     const int nseg = (g.nc0 + CPB - 1) / CPB;
-    const long row = blockIdx.x / nseg;
-    const int x0 = (int)(blockIdx.x - row * nseg) * CPB;
+    int seg, cy, cz;
+    if (g.color < 0) {
+      const long row = blockIdx.x / nseg;
+      seg = (int)(blockIdx.x - row * nseg);
+      cy = (int)(row % g.nc1);
+      cz = (int)(row / g.nc1);
+    } else {
+      const int px = g.color & 1, py = (g.color >> 1) & 1, pz = (g.color >> 2) & 1;
+      const int nseg_c = (nseg - px + 1) / 2, ny_c = (g.nc1 - py + 1) / 2;
+      const long rest = blockIdx.x / nseg_c;
+      seg = 2 * (int)(blockIdx.x - rest * nseg_c) + px;
+      cy = 2 * (int)(rest % ny_c) + py;
+      cz = 2 * (int)(rest / ny_c) + pz;
+    }
+    const long row = cy + (long)g.nc1 * cz;
+    const int x0 = seg * CPB;
     ncb = min(CPB, g.nc0 - x0);
     ix0 = K * x0;
-    iy0 = K * (int)(row % g.nc1);
-    iz0 = K * (int)(row / g.nc1);
+    iy0 = K * cy;
+    iz0 = K * cz;
     in_block = lc < CPB;
     active = lc < ncb;
     cell = row * g.nc0 + x0 + lc;
@@ -280,7 +300,8 @@ __device__ void gather(const Geometry &g, const Place<N1> &p, const double *src,
 // lines in LDS (shared x-faces of neighbouring cells added there), then written with
 // consecutive lanes on consecutive addresses. With ZEROED (dst cleared before the launch) the
 // DoFs that only this block touches -- y, z interior to the cells, x interior to the block --
-// are stored, the others added with atomics. Constrained rows are left alone.
+// are stored, the others added (no other block of the launch's colour touches them).
+// Constrained rows are left alone.
 template <int N1, bool ZEROED>
 __device__ void scatter(const Geometry &g, const Place<N1> &p, double *dst, const double *out, double *line) {
   constexpr int K = N1 - 1, NE = N1 * N1 * kLine<N1>;
@@ -304,7 +325,7 @@ __device__ void scatter(const Geometry &g, const Place<N1> &p, double *dst, cons
     if (own)
       dst[gi] = line[e];
     else
-      atomicAdd(dst + gi, line[e]);
+      dst[gi] += line[e];
   }
 }
 
@@ -556,6 +577,8 @@ Geometry geometry(const OperatorData<double> &op, double dt, int conv = 0, int s
   g.solid = solid ? 1 : 0;
   g.nc0 = op.nc[0];
   g.nc1 = op.nc[1];
+  g.nc2 = op.nc[2];
+  g.color = -1;
   g.ncells = op.n_cells;
   for (int d = 0; d < 3; ++d) {
     g.nd[d] = op.nd[d];
@@ -570,17 +593,19 @@ Geometry geometry(const OperatorData<double> &op, double dt, int conv = 0, int s
 
 int finish(hipStream_t) { return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE; }
 
-#define MPDG_TANGENT_DISPATCH(KERNEL, ...)                                                                       \
-  switch (op.degree) {                                                                                           \
-    case 1: hipLaunchKernelGGL(KERNEL<2>, dim3(tangent_blocks(2, op.nc)), dim3(256), 0, s,                       \
-                               __VA_ARGS__); break;                                                              \
-    case 2: hipLaunchKernelGGL(KERNEL<3>, dim3(tangent_blocks(3, op.nc)), dim3(256), 0, s,                       \
-                               __VA_ARGS__); break;                                                              \
-    case 3: hipLaunchKernelGGL(KERNEL<4>, dim3(tangent_blocks(4, op.nc)), dim3(256), 0, s,                       \
-                               __VA_ARGS__); break;                                                              \
-    case 4: hipLaunchKernelGGL(KERNEL<5>, dim3(tangent_blocks(5, op.nc)), dim3(256), 0, s,                       \
-                               __VA_ARGS__); break;                                                              \
-    default: return MPDG_ERR_INVALID;                                                                            \
+// one launch over all blocks (GEO.color = -1), or eight, one per colour (COLORS = 8)
+#define MPDG_TANGENT_DISPATCH(KERNEL, GEO, COLORS, ...)                                                           \
+  if (op.degree < 1 || op.degree > 4) return MPDG_ERR_INVALID;                                                   \
+  for (int color_ = (COLORS) > 1 ? 0 : -1; color_ < ((COLORS) > 1 ? (COLORS) : 0); ++color_) {                  \
+    GEO.color = color_;                                                                                          \
+    const long nb_ = tangent_blocks(op.degree + 1, op.nc, color_);                                               \
+    if (nb_ == 0) continue;                                                                                      \
+    switch (op.degree) {                                                                                         \
+      case 1: hipLaunchKernelGGL(KERNEL<2>, dim3(nb_), dim3(256), 0, s, __VA_ARGS__); break;                     \
+      case 2: hipLaunchKernelGGL(KERNEL<3>, dim3(nb_), dim3(256), 0, s, __VA_ARGS__); break;                     \
+      case 3: hipLaunchKernelGGL(KERNEL<4>, dim3(nb_), dim3(256), 0, s, __VA_ARGS__); break;                     \
+      default: hipLaunchKernelGGL(KERNEL<5>, dim3(nb_), dim3(256), 0, s, __VA_ARGS__); break;                    \
+    }                                                                                                            \
   }
 
 }  // namespace
@@ -595,16 +620,16 @@ int launch_tangent_data(const OperatorData<double> &op, double dt, const mpdg_he
   A.vf = vfield;
   A.solid = solid;
   const Tables1D t = make_tables(op.degree);
-  MPDG_TANGENT_DISPATCH(tangent_data_kernel, t, A, T, T_old, td)
+  MPDG_TANGENT_DISPATCH(tangent_data_kernel, A.g, 1, t, A, T, T_old, td)
   return finish(s);
 }
 
 int launch_tangent_apply(const OperatorData<double> &op, const double *td, int conv, int solid,
                          const double *src, double *dst, hipStream_t s) {
-  const Geometry g = geometry(op, 1.0, conv, solid);
+  Geometry g = geometry(op, 1.0, conv, solid);
   const Tables1D t = make_tables(op.degree);
   if (hipMemsetAsync(dst, 0, sizeof(double) * op.n_dofs, s) != hipSuccess) return MPDG_ERR_DEVICE;
-  MPDG_TANGENT_DISPATCH(tangent_apply_kernel, t, g, td, src, dst)
+  MPDG_TANGENT_DISPATCH(tangent_apply_kernel, g, 8, t, g, td, src, dst)
   return finish(s);
 }
 
@@ -617,16 +642,16 @@ int launch_tangent_residual(const OperatorData<double> &op, double dt, const mpd
   for (int d = 0; d < 3; ++d) A.vel[d] = vel[d];
   A.vf = vfield;
   const Tables1D t = make_tables(op.degree);
-  MPDG_TANGENT_DISPATCH(tangent_residual_kernel, t, A, T, T_old, q, dst)
+  MPDG_TANGENT_DISPATCH(tangent_residual_kernel, A.g, 8, t, A, T, T_old, q, dst)
   return finish(s);
 }
 
 int launch_tangent_diagonal(const OperatorData<double> &op, const double *td, int conv, int solid,
                             double *diag, hipStream_t s) {
-  const Geometry g = geometry(op, 1.0, conv, solid);
+  Geometry g = geometry(op, 1.0, conv, solid);
   const Tables1D t = make_tables(op.degree);
   if (hipMemsetAsync(diag, 0, sizeof(double) * op.n_dofs, s) != hipSuccess) return MPDG_ERR_DEVICE;
-  MPDG_TANGENT_DISPATCH(tangent_diagonal_kernel, t, g, td, diag)
+  MPDG_TANGENT_DISPATCH(tangent_diagonal_kernel, g, 8, t, g, td, diag)
   if (op.dmask)
     hipLaunchKernelGGL(unit_constrained_kernel, dim3((op.n_dofs + 255) / 256), dim3(256), 0, s, diag, g, op.n_dofs);
   return finish(s);

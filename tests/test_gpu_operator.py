@@ -201,6 +201,36 @@ def test_mapped_robin_faces_and_flux_match_oracle(cuda, degree, nc, mapname):
     assert rel_err(out.cpu().numpy(), q) < 1e-14
 
 
+def test_robin_faces_and_flux_are_bitwise_repeatable(cuda):
+    """Robin apply, diagonal and residual with the surface flux, applied repeatedly: identical bits
+    (each boundary DoF sums its faces and cells in a fixed order, without atomics)."""
+    torch = cuda
+    import meltpooldg_amd as M
+    nc, L = (24, 20, 12), (1.0, 0.8, 0.5)
+    rng = np.random.default_rng(41)
+    gop = M.HeatDiffuseMultiPhaseOperator(3, nc, L, rho_cp=3.6e6)
+    gop.reset_time_increment(1e-2)
+    gop.set_conductivity_constant(55.563)
+    n = gop.n_dofs
+    Td = torch.tensor(1000.0 + 50.0 * rng.uniform(-1, 1, n), device="cuda")
+    gop.set_robin(0b011111, alpha=25.0, T_inf_conv=300.0, emissivity=0.98, T_inf_rad=310.0, T=Td)
+    gop.set_surface_flux(*LASER)
+    x = torch.tensor(rng.uniform(-1, 1, n), device="cuda")
+    outs = []
+    for _ in range(3):
+        y = torch.empty(n, dtype=torch.float64, device="cuda")
+        gop.vmult(y, x)
+        d = torch.empty_like(y)
+        gop.compute_inverse_diagonal_from_matrixfree(d)
+        r = torch.zeros_like(y)
+        gop.create_rhs(r, Td, Td - 1.0)
+        torch.cuda.synchronize()
+        outs.append([v.cpu().numpy().copy() for v in (y, d, r)])
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
+
+
 # Meshes with 64 columns of chunks (a multiple of the 8 XCDs): the launch then gives each XCD
 # a contiguous range of columns, a path the smaller meshes above do not take.
 XCD_CASES = [(1, (8, 32, 32), "const", 0, 0), (2, (14, 24, 16), "qp", 0, 0), (3, (8, 32, 16), "qp", 0b000101, 0),

@@ -78,6 +78,29 @@ def test_gmg_vcycle_fp32(cuda, degree, nc, mode, mapped):
     assert rel_err(out.cpu().numpy(), og.vcycle(b)) < 1e-4
 
 
+@pytest.mark.parametrize("precision", ["FP64", "FP32"])
+def test_gmg_vcycle_is_bitwise_repeatable(cuda, precision):
+    """Two multigrid preconditioners built on the same operator, each updated twice, give the same
+    V-cycle bits (diagonals, eigenvalue estimates, coarse matrix and its inverse, smoothers and
+    transfers have no order-dependent sums)."""
+    torch = cuda
+    import meltpooldg_amd as M
+    orc, gop, _ = make_pair(torch, 3, (16, 16, 12), mode="qp", seed=21, dirichlet=0b000011)
+    b = torch.tensor(np.random.default_rng(22).uniform(-1, 1, orc.n_dofs), device="cuda")
+    outs = []
+    for _ in range(2):
+        pc = M.GMGPreconditioner(gop, precision=getattr(M, precision), n_cells_coarse=4, coarse_degree=2)
+        for _ in range(2):
+            pc.update()
+            out = torch.empty(orc.n_dofs, dtype=torch.float64, device="cuda")
+            pc.vmult(out, b)
+            torch.cuda.synchronize()
+            outs.append(out.cpu().numpy().copy())
+        del pc
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
+
+
 @pytest.mark.parametrize("prec,solver,degree,nc,mode,mapped", [
     (0, 0, 3, (16, 16, 16), "qp", None), (1, 1, 3, (16, 16, 16), "qp", None),
     (1, 1, 4, (12, 12, 12), "qp", None), (0, 0, 4, (12, 12, 12), "law", None),   # Q4 (the Q4 96^3 solve)

@@ -130,6 +130,32 @@ def test_velocity_field_needs_the_tangent(cuda):
         op.set_velocity_field(torch.zeros(op.n_dofs, dtype=torch.float64, device="cuda"))
 
 
+def test_tangent_is_bitwise_repeatable(cuda):
+    """Tangent apply, diagonal and residual applied repeatedly give identical bits (the blocks add
+    their shared rows in eight colour classes, one after the other, without atomics)."""
+    torch = cuda
+    import meltpooldg_amd as M
+    orc, op, T, T_old, Td, Tod = pair(torch, M, 2, (40, 12, 10), (0.8, 0.24, 0.2), 0b000011, True,
+                                      (0.2, -0.4, 0.1), seed=11)
+    src = torch.tensor(np.random.default_rng(12).uniform(-1, 1, orc.n_dofs), device="cuda")
+    q = torch.tensor(np.random.default_rng(13).uniform(0, 100, orc.n_dofs), device="cuda")
+    outs = []
+    for _ in range(3):
+        dst = torch.empty(orc.n_dofs, dtype=torch.float64, device="cuda")
+        op.vmult(dst, src)
+        d = torch.empty_like(dst)
+        op.compute_diagonal(d)
+        r = torch.full_like(dst, 0.5)
+        op.create_rhs(r, Td, Tod, q)
+        torch.cuda.synchronize()
+        outs.append([v.cpu().numpy().copy() for v in (dst, d, r)])
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
+    ref = orc.vmult(src.cpu().numpy(), threads=1)
+    assert np.linalg.norm(outs[0][0] - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
 @pytest.mark.parametrize("restart", [0, 5])
 def test_gmres_matches_oracle(cuda, restart):
     """J x = b with the non-symmetric tangent: GMRES + inverse diagonal on the GPU and in the oracle reach
