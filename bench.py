@@ -270,10 +270,10 @@ def scan_timings(torch, M, k, N, n_train, n_eval, epochs, rho, only="all", law=F
         if wanted is None or name in wanted:
             net = M.LearnedPredictor((N, N, N), degree=k, coarsening=rho, qscale=dt * N, seed=11)
             t0 = time.perf_counter()
-            losses = train(net, Ts_o, qs, zs, epochs=epochs, lr=2e-3, bs=bs)
+            losses = train(net, Ts_o, qs, zs, epochs=epochs, lr=2e-3, bs=bs, lr_decay=0.7)
             torch.cuda.synchronize()
             nets[order] = net
-            training[name] = {"pairs": n_train - 2, "epochs": epochs, "coarsening": rho,
+            training[name] = {"pairs": n_train - 2, "epochs": epochs, "lr": 2e-3, "lr_decay": 0.7, "coarsening": rho,
                               "seconds": round(time.perf_counter() - t0, 2), "loss_first_epoch": losses[0],
                               "loss_last_epoch": losses[-1]}
         if Ts_o is not Ts:
@@ -373,7 +373,8 @@ def learned_c4(torch, M, n_steps=200, n_train=150, epochs=8, cells=64, rho=2, sm
         t_traj = time.perf_counter() - t0
         net = M.LearnedPredictor((cells,) * 3, degree=k, coarsening=rho, qscale=dt * cells, seed=11)
         t0 = time.perf_counter()
-        losses = train(net, Ts[:n_train + 1], qs[:n_train], zs[:n_train], epochs=epochs, lr=2e-3, bs=bs[:n_train])
+        losses = train(net, Ts[:n_train + 1], qs[:n_train], zs[:n_train], epochs=epochs, lr=2e-3, bs=bs[:n_train],
+                       lr_decay=0.7)
         torch.cuda.synchronize()
         t_train = time.perf_counter() - t0
         del qs, zs, bs
@@ -398,7 +399,8 @@ def learned_c4(torch, M, n_steps=200, n_train=150, epochs=8, cells=64, rho=2, sm
             del case
         name = "base_linear" if order == 1 else "base_extrapolation2"
         out[name] = {"trajectory_seconds": round(t_traj, 2), "training_seconds": round(t_train, 2),
-                     "training_pairs": n_train - 2, "epochs": epochs, "loss_first_epoch": losses[0],
+                     "training_pairs": n_train - 2, "epochs": epochs, "lr": 2e-3, "lr_decay": 0.7,
+                     "loss_first_epoch": losses[0],
                      "loss_last_epoch": losses[-1], "without_network": rows["base"], "with_network": rows["learned"],
                      "time_ratio": round(rows["base"]["ms_per_step"] / rows["learned"]["ms_per_step"], 3)}
         del Ts, net

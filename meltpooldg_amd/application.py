@@ -110,9 +110,11 @@ def generate_trajectory(degree, cells, n_steps, T_init=None, with_base=False, **
     return case, Ts, qs, zs, stats
 
 
-def train(net, Ts, qs, zs, epochs=4, lr=1e-3, first=2, seed=0, log=None, bs=None):
+def train(net, Ts, qs, zs, epochs=4, lr=1e-3, first=2, seed=0, log=None, bs=None, lr_decay=1.0):
     """Adam over the solver's own pairs ((T_{n-1}, T_n, q_{n+1}, z_n[, T_b,n]) -> T_{n+1}),
-    n >= first; bs (the base guesses) None: the linear extrapolation."""
+    n >= first, in a shuffled order per epoch; bs (the base guesses) None: the linear
+    extrapolation. The step size of epoch e is lr * lr_decay**e (a decay of 0.7 keeps the loss
+    falling from epoch to epoch where a constant rate lets it jump back up)."""
     import random
     rng = random.Random(seed)
     idx = list(range(first, len(Ts) - 1))
@@ -120,8 +122,10 @@ def train(net, Ts, qs, zs, epochs=4, lr=1e-3, first=2, seed=0, log=None, bs=None
     for e in range(epochs):
         rng.shuffle(idx)
         tot = 0.0
+        lr_e = lr * lr_decay ** e
         for n in idx:
-            tot += net.train_step(Ts[n - 1], Ts[n], qs[n], zs[n], Ts[n + 1], lr, base=None if bs is None else bs[n])
+            tot += net.train_step(Ts[n - 1], Ts[n], qs[n], zs[n], Ts[n + 1], lr_e,
+                                  base=None if bs is None else bs[n])
         losses.append(tot / len(idx))
         if log:
             log("epoch %d loss %.4e" % (e, losses[-1]))

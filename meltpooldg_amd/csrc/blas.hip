@@ -307,15 +307,16 @@ __global__ __launch_bounds__(256) void cg_direction_kernel(double *p, const doub
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = z[i] + beta * p[i];
 }
 
-// initial: r = b - Ax (Ax in q), z = P r (Jacobi / identity), rr, rz, bb; p = z
+// initial: r = b - Ax (Ax in q; or q is the residual itself, q_is_r), z = P r (Jacobi /
+// identity), rr, rz, bb; p = z
 template <bool JACOBI>
 __global__ __launch_bounds__(256) void cg_init_kernel(const double *b, double *r, const double *q, double *z,
                                                       double *p, const double *inv, long n, double *partials,
                                                       unsigned *counter, CGState *st, double rel_tol,
-                                                      double abs_tol, int criterion, float *r32) {
+                                                      double abs_tol, int criterion, float *r32, int q_is_r) {
   double v[3] = {0.0, 0.0, 0.0};
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-    const double ri = b[i] - q[i];
+    const double ri = q_is_r ? q[i] : b[i] - q[i];
     r[i] = ri;
     if (r32) r32[i] = (float)ri;
     v[0] += ri * ri;
@@ -357,13 +358,14 @@ __global__ __launch_bounds__(256) void cg_rz0_kernel(const double *r, const doub
 
 int cg_init(const double *b, double *r, const double *q, double *z, double *p, const double *inv, bool jacobi,
             long n, const Reduction &red, CGState *st, double rel_tol, double abs_tol, int criterion, hipStream_t s,
-            float *r32) {
+            float *r32, bool q_is_residual) {
+  const int qr = q_is_residual ? 1 : 0;
   if (jacobi)
     hipLaunchKernelGGL(cg_init_kernel<true>, dim3(red_blocks(n)), dim3(256), 0, s, b, r, q, z, p, inv, n,
-                       red.partials, red.counter, st, rel_tol, abs_tol, criterion, (float *)nullptr);
+                       red.partials, red.counter, st, rel_tol, abs_tol, criterion, (float *)nullptr, qr);
   else
     hipLaunchKernelGGL(cg_init_kernel<false>, dim3(red_blocks(n)), dim3(256), 0, s, b, r, q, z, p, inv, n,
-                       red.partials, red.counter, st, rel_tol, abs_tol, criterion, r32);
+                       red.partials, red.counter, st, rel_tol, abs_tol, criterion, r32, qr);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
@@ -549,8 +551,7 @@ constexpr int kEllLine = 1024;  // longest x line staged in LDS (DoF grids up to
 template <typename T> struct EllStage { static constexpr int rows = 32; };  // 32 KB of LDS either way
 template <> struct EllStage<double> { static constexpr int rows = 16; };
 template <typename T, int MAXW>
-__global__ __launch_bounds__(256) void ell_dir0_kernel(const T *__restrict__ in, const T *__restrict__ in2,
-                                                       T *__restrict__ out, int o0, int i0, long nlines, int width,
+__global__ __launch_bounds__(256) void ell_dir0_kernel(EllFields<T> F, int o0, int i0, long nlines, int width,
                                                        const int *__restrict__ col, const T *__restrict__ val,
                                                        int add) {
   __shared__ T lines[4][kEllLine];
@@ -559,28 +560,36 @@ __global__ __launch_bounds__(256) void ell_dir0_kernel(const T *__restrict__ in,
   if (line_id >= nlines) return;  // whole wavefront; only wavefront-local LDS use below
   T *line = lines[wave];
   const long lin = (long)i0 * line_id, lout = (long)o0 * line_id;
-  for (int x = lane; x < i0; x += 64) line[x] = in2 ? in[lin + x] - in2[lin + x] : in[lin + x];
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (int x = lane; x < o0; x += 64) {
-    T acc = 0;
-#pragma unroll
-    for (int e = 0; e < MAXW; ++e) {
-      if (e < width) {
-        const int c = col[x * width + e];
-        if (c >= 0) acc += val[x * width + e] * line[c];
-      }
+  for (int f = 0; f < F.nf; ++f) {
+    const T *in = F.in[f], *in2 = F.in2[f];
+    T *out = F.out[f];
+    if (f > 0) {  // every lane is done with the previous field's line
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    out[lout + x] = add ? out[lout + x] + acc : acc;
+    for (int x = lane; x < i0; x += 64) line[x] = in2 ? in[lin + x] - in2[lin + x] : in[lin + x];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int x = lane; x < o0; x += 64) {
+      T acc = 0;
+#pragma unroll
+      for (int e = 0; e < MAXW; ++e) {
+        if (e < width) {
+          const int c = col[x * width + e];
+          if (c >= 0) acc += val[x * width + e] * line[c];
+        }
+      }
+      out[lout + x] = add ? out[lout + x] + acc : acc;
+    }
   }
 }
 // DIR 1: thread index = x, blockIdx.y = z; DIR 2: thread index = position in the x-y plane
 // (consecutive threads read consecutive addresses of a plane). Each thread walks `walk`
 // output rows along DIR.
 template <typename T, int DIR, int MAXW>
-__global__ __launch_bounds__(256) void ell_dir12_kernel(const T *__restrict__ in, const T *__restrict__ in2,
-                                                        T *__restrict__ out, int o0, int o1, int o2, int i0, int i1,
+__global__ __launch_bounds__(256) void ell_dir12_kernel(EllFields<T> F, int o0, int o1, int o2, int i0, int i1,
                                                         int width, const int *__restrict__ col,
                                                         const T *__restrict__ val, int add, int walk) {
   constexpr int SR = EllStage<T>::rows;
@@ -617,53 +626,58 @@ __global__ __launch_bounds__(256) void ell_dir12_kernel(const T *__restrict__ in
     ostride = plane;
   }
   const bool staged = cmax >= 0 && cmax - cmin < SR;
-  if (staged) {
-    for (int j = 0; j <= cmax - cmin; ++j) {
-      const long src = base + stride * (cmin + j);
-      stage[j][tx] = in2 ? in[src] - in2[src] : in[src];
-    }
-  }
-  for (int row = r0; row < r1; ++row) {
-    // the row's ELL entries first (uniform), then all reads, then the sum
-    int cs[MAXW];
-    T vs[MAXW], xs[MAXW];
-#pragma unroll
-    for (int e = 0; e < MAXW; ++e) {
-      cs[e] = e < width ? col[row * width + e] : -1;
-      vs[e] = e < width ? val[row * width + e] : T(0);
-    }
-#pragma unroll
-    for (int e = 0; e < MAXW; ++e) {
-      const int c = cs[e];
-      T v = 0;
-      if (c >= 0) {
-        if (staged) {
-          v = stage[c - cmin][tx];
-        } else {
-          const long src = base + stride * c;
-          v = in2 ? in[src] - in2[src] : in[src];
-        }
+  // each thread stages and reads only its own LDS column, so the fields can reuse it in turn
+  for (int f = 0; f < F.nf; ++f) {
+    const T *in = F.in[f], *in2 = F.in2[f];
+    T *out = F.out[f];
+    if (staged) {
+      for (int j = 0; j <= cmax - cmin; ++j) {
+        const long src = base + stride * (cmin + j);
+        stage[j][tx] = in2 ? in[src] - in2[src] : in[src];
       }
-      xs[e] = v;
     }
-    T acc = 0;
+    for (int row = r0; row < r1; ++row) {
+      // the row's ELL entries first (uniform), then all reads, then the sum
+      int cs[MAXW];
+      T vs[MAXW], xs[MAXW];
 #pragma unroll
-    for (int e = 0; e < MAXW; ++e) acc += vs[e] * xs[e];
-    const long g = obase + ostride * row;
-    out[g] = add ? out[g] + acc : acc;
+      for (int e = 0; e < MAXW; ++e) {
+        cs[e] = e < width ? col[row * width + e] : -1;
+        vs[e] = e < width ? val[row * width + e] : T(0);
+      }
+#pragma unroll
+      for (int e = 0; e < MAXW; ++e) {
+        const int c = cs[e];
+        T v = 0;
+        if (c >= 0) {
+          if (staged) {
+            v = stage[c - cmin][tx];
+          } else {
+            const long src = base + stride * c;
+            v = in2 ? in[src] - in2[src] : in[src];
+          }
+        }
+        xs[e] = v;
+      }
+      T acc = 0;
+#pragma unroll
+      for (int e = 0; e < MAXW; ++e) acc += vs[e] * xs[e];
+      const long g = obase + ostride * row;
+      out[g] = add ? out[g] + acc : acc;
+    }
   }
 }
 template <typename T>
-int launch_ell_dir(const T *in, T *out, const int *odims, const int *idims, int dir, int width, const int *col,
-                   const T *val, hipStream_t s, const T *in2, bool add) {
-  if (width > 32) return MPDG_ERR_INVALID;
+int launch_ell_dir_fields(const EllFields<T> &F, const int *odims, const int *idims, int dir, int width,
+                          const int *col, const T *val, hipStream_t s, bool add) {
+  if (width > 32 || F.nf < 1 || F.nf > kEllMaxFields) return MPDG_ERR_INVALID;
   if (dir == 0) {
     if (idims[0] > kEllLine) return MPDG_ERR_INVALID;
     const long nlines = (long)odims[1] * odims[2];
     const dim3 grid((unsigned)((nlines + 3) / 4));
 #define MPDG_ELL0(W)                                                                                             \
-  hipLaunchKernelGGL((ell_dir0_kernel<T, W>), grid, dim3(256), 0, s, in, in2, out, odims[0], idims[0], nlines,  \
-                     width, col, val, add ? 1 : 0)
+  hipLaunchKernelGGL((ell_dir0_kernel<T, W>), grid, dim3(256), 0, s, F, odims[0], idims[0], nlines, width, col, \
+                     val, add ? 1 : 0)
     if (width <= 4) MPDG_ELL0(4);
     else if (width <= 8) MPDG_ELL0(8);
     else if (width <= 16) MPDG_ELL0(16);
@@ -680,8 +694,8 @@ int launch_ell_dir(const T *in, T *out, const int *odims, const int *idims, int 
   if (dir == 1) grid = dim3((odims[0] + 255) / 256, odims[2], (odims[1] + walk - 1) / walk);
   else grid = dim3((unsigned)(((long)odims[0] * odims[1] + 255) / 256), 1, (odims[2] + walk - 1) / walk);
 #define MPDG_ELL12(D, W)                                                                                        \
-  hipLaunchKernelGGL((ell_dir12_kernel<T, D, W>), grid, dim3(256), 0, s, in, in2, out, odims[0], odims[1],      \
-                     odims[2], idims[0], idims[1], width, col, val, add ? 1 : 0, walk)
+  hipLaunchKernelGGL((ell_dir12_kernel<T, D, W>), grid, dim3(256), 0, s, F, odims[0], odims[1], odims[2],       \
+                     idims[0], idims[1], width, col, val, add ? 1 : 0, walk)
   if (dir == 1) {
     if (width <= 4) MPDG_ELL12(1, 4);
     else if (width <= 8) MPDG_ELL12(1, 8);
@@ -896,10 +910,22 @@ template int launch_htransfer<float>(bool, int, const float *, const float *, fl
 template int launch_htransfer<double>(bool, int, const double *, const double *, double *, const int *, const int *,
                                       int, const double *, const double *, int, int, bool, hipStream_t);
 
+template <typename T>
+int launch_ell_dir(const T *in, T *out, const int *odims, const int *idims, int dir, int width, const int *col,
+                   const T *val, hipStream_t s, const T *in2, bool add) {
+  EllFields<T> F{};
+  F.in[0] = in;
+  F.in2[0] = in2;
+  F.out[0] = out;
+  F.nf = 1;
+  return launch_ell_dir_fields<T>(F, odims, idims, dir, width, col, val, s, add);
+}
 template int launch_ell_dir<float>(const float *, float *, const int *, const int *, int, int, const int *,
                                    const float *, hipStream_t, const float *, bool);
 template int launch_ell_dir<double>(const double *, double *, const int *, const int *, int, int, const int *,
                                     const double *, hipStream_t, const double *, bool);
+template int launch_ell_dir_fields<double>(const EllFields<double> &, const int *, const int *, int, int,
+                                           const int *, const double *, hipStream_t, bool);
 
 // Coarse-grid solve with the stored inverse (dense, column-major, FP64):
 // y_i = sum_j A[j n + i] x_j in two deterministic stages: segment k of the j range

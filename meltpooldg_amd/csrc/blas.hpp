@@ -29,9 +29,10 @@ template <typename T> int launch_invert_diagonal(T *d, long n, hipStream_t s);
 template <typename TA, typename TB>
 int launch_dot(const TA *a, const TB *b, long n, const Reduction &r, double *out, hipStream_t s);
 
+// q_is_residual: q already holds b - A x (the initial apply is skipped by the caller)
 int cg_init(const double *b, double *r, const double *q, double *z, double *p, const double *inv, bool jacobi,
             long n, const Reduction &red, CGState *st, double rel_tol, double abs_tol, int criterion, hipStream_t s,
-            float *r32 = nullptr);
+            float *r32 = nullptr, bool q_is_residual = false);
 // CG / flexible CG with an FP32 preconditioner run in place: r32 its input, z its output
 int fcg32_start(const double *r, const float *z, double *p, long n, const Reduction &red, CGState *st,
                 hipStream_t s);
@@ -69,6 +70,18 @@ int launch_mul(double *x, const double *y, long n, hipStream_t s);
 template <typename T>
 int launch_ell_dir(const T *in, T *out, const int *odims, const int *idims, int dir, int width, const int *col,
                    const T *val, hipStream_t s, const T *in2 = nullptr, bool add = false);
+// the same 1D operator applied to up to kEllMaxFields fields of equal shape in one launch
+// (out[f] = A in[f], or A (in[f] - in2[f]) where in2[f] is set)
+constexpr int kEllMaxFields = 4;
+template <typename T> struct EllFields {
+  const T *in[kEllMaxFields];
+  const T *in2[kEllMaxFields];
+  T *out[kEllMaxFields];
+  int nf;
+};
+template <typename T>
+int launch_ell_dir_fields(const EllFields<T> &F, const int *odims, const int *idims, int dir, int width,
+                          const int *col, const T *val, hipStream_t s, bool add = false);
 // structured 2:1 h-transfer of Q_k along `dir` (see blas.hip): restrict_mode false: out (+)= P in
 // (fine <- coarse, accumulating when add); true: out = P^T (in - in2) (in2 may be null).
 // tab_host / tab_dev: the (2k+1) x (k+1) table of one coarse cell; lo / hi: Dirichlet ends

@@ -910,15 +910,16 @@ int gmres_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &pr
 // (flexible) CG with an FP32 preconditioner run in place: the iteration of krylov_solve below
 // with r also written in FP32 (the preconditioner's input) and z read in FP32 (its output)
 int fcg32_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &prm, const double *b, double *x,
-                KrylovWork &w, int *iterations, double *res, float *r32, const float *z32, hipStream_t s) {
+                KrylovWork &w, int *iterations, double *res, float *r32, const float *z32, const double *r_init,
+                hipStream_t s) {
   const long n = op->n_dofs();
   CGState h{};
   h.max_it = prm.max_iterations;
   h.flexible = prm.solver_type == MPDG_SOLVER_FCG ? 1 : 0;
   if (hipMemcpyAsync(w.st.get(), &h, sizeof(CGState), hipMemcpyHostToDevice, s) != hipSuccess) return MPDG_ERR_DEVICE;
-  MPDG_TRY(op->vmult(w.q.get(), x, s));
-  MPDG_TRY(cg_init(b, w.r.get(), w.q.get(), nullptr, nullptr, nullptr, false, n, w.red.red(), w.st.get(),
-                   prm.rel_tolerance, prm.abs_tolerance, prm.criterion, s, r32));
+  if (!r_init) MPDG_TRY(op->vmult(w.q.get(), x, s));
+  MPDG_TRY(cg_init(b, w.r.get(), r_init ? r_init : w.q.get(), nullptr, nullptr, nullptr, false, n, w.red.red(),
+                   w.st.get(), prm.rel_tolerance, prm.abs_tolerance, prm.criterion, s, r32, r_init != nullptr));
   MPDG_TRY(pc->vmult_f32_in_place(s));
   MPDG_TRY(fcg32_start(w.r.get(), z32, w.p.get(), n, w.red.red(), w.st.get(), s));
   const int batch = n > 2000000 ? 1 : 8;
@@ -944,8 +945,9 @@ int fcg32_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &pr
   return MPDG_OK;
 }
 
+// r_init (may be nullptr): b - A x, already computed by the caller (CG / FCG only)
 int krylov_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &prm, const double *b, double *x,
-                 KrylovWork &w, int *iterations, double *res, hipStream_t s) {
+                 KrylovWork &w, int *iterations, double *res, hipStream_t s, const double *r_init = nullptr) {
   const long n = op->n_dofs();
   MPDG_TRY(w.init(n));
   if (prm.solver_type == MPDG_SOLVER_GMRES) {
@@ -962,11 +964,11 @@ int krylov_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &p
   // FP32 multigrid: its input and output are used in place, in FP32
   float *r32 = external ? pc->f32_input() : nullptr;
   const float *z32 = external ? pc->f32_output() : nullptr;
-  if (r32 && z32) return fcg32_solve(op, pc, prm, b, x, w, iterations, res, r32, z32, s);
+  if (r32 && z32) return fcg32_solve(op, pc, prm, b, x, w, iterations, res, r32, z32, r_init, s);
   // r = b - A x
-  MPDG_TRY(op->vmult(w.q.get(), x, s));
-  MPDG_TRY(cg_init(b, w.r.get(), w.q.get(), w.z.get(), w.p.get(), inv, jacobi, n, w.red.red(), w.st.get(),
-                   prm.rel_tolerance, prm.abs_tolerance, prm.criterion, s));
+  if (!r_init) MPDG_TRY(op->vmult(w.q.get(), x, s));
+  MPDG_TRY(cg_init(b, w.r.get(), r_init ? r_init : w.q.get(), w.z.get(), w.p.get(), inv, jacobi, n, w.red.red(),
+                   w.st.get(), prm.rel_tolerance, prm.abs_tolerance, prm.criterion, s, nullptr, r_init != nullptr));
   if (external) {
     MPDG_TRY(pc->vmult(w.z.get(), w.r.get(), s));
     MPDG_TRY(cg_after_external_precond_init(w.r.get(), w.z.get(), w.p.get(), n, w.red.red(), w.st.get(), s));
@@ -1554,6 +1556,7 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
   MPDG_TRY(launch_fill<double>(h->bvec.get(), 0.0, n, s));
   MPDG_TRY(op->create_rhs(h->bvec.get(), h->zero.get(), T_old, nullptr, s, true));
   MPDG_TRY(learned_guess(h, dt, h->bvec.get(), T, T_old, s));
+  const double *r_init = nullptr;
   if (h->learned_applied && h->prm.guess_check) {
     // the learned guess (T) against the base guess it corrects (its residual norm came with the
     // network's input); the residual of T is kept in alt. With guess_smoothing the network's
@@ -1575,13 +1578,16 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
     if (!(rr[0] == rr[0]) || !(rr[0] < rr[1])) {
       MPDG_TRY(launch_convert<double, double>(h->tbase.get(), T, n, s));
       h->fell_back = 1;
+    } else {
+      r_init = h->alt.get();  // b - J T of the accepted guess: the solve starts from it
     }
   }
   mpdg_solver_params lp = h->prm.linear;
   lp.criterion = MPDG_CRIT_RHS;
   int li = 0;
   double res = 0.0;
-  const int rc = krylov_solve(op, h->pc, lp, h->bvec.get(), T, h->work, &li, &res, s);
+  const int rc = krylov_solve(op, h->pc, lp, h->bvec.get(), T, h->work, &li, &res, s,
+                              lp.solver_type == MPDG_SOLVER_GMRES ? nullptr : r_init);
   CGState st{};
   if (hipMemcpyAsync(&st, h->work.st.get(), sizeof(CGState), hipMemcpyDeviceToHost, s) != hipSuccess)
     return MPDG_ERR_DEVICE;

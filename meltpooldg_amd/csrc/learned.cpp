@@ -76,6 +76,7 @@ struct mpdg_learned {
   DeviceArray<double> wval[3], pval[3];
   int ww[3] = {0, 0, 0}, pw[3] = {0, 0, 0};
   DeviceArray<double> cf[6], tmpa, tmpb, corr, corr_f;
+  long tmpa_n = 0, tmpb_n = 0;  // per field: the transfers carry up to kEllMaxFields fields at once
 
   int init(const int *cells, int k, int coarsening, double qs, unsigned long long seed) {
     degree = k;
@@ -142,8 +143,10 @@ struct mpdg_learned {
       for (auto &a : cf) MPDG_TRY(a.resize(n));
       const long a1 = std::max((long)dims[0] * fdims[1] * fdims[2], (long)fdims[0] * fdims[1] * dims[2]);
       const long a2 = std::max((long)dims[0] * dims[1] * fdims[2], (long)fdims[0] * dims[1] * dims[2]);
-      MPDG_TRY(tmpa.resize(a1));
-      MPDG_TRY(tmpb.resize(a2));
+      tmpa_n = a1;
+      tmpb_n = a2;
+      MPDG_TRY(tmpa.resize((size_t)kEllMaxFields * a1));
+      MPDG_TRY(tmpb.resize((size_t)kEllMaxFields * a2));
       MPDG_TRY(corr_f.resize(nf));
     }
     return pack(nullptr);
@@ -152,25 +155,34 @@ struct mpdg_learned {
     for (int l = 0; l < kLayers; ++l) MPDG_TRY(cnn_pack(w[l].get(), kCin[l], kCout[l], W[l].get(), Wt[l].get(), s));
     return MPDG_OK;
   }
-  // a DoF-grid field on the network grid (the field itself when rho = 1, nullptr stays nullptr)
-  const double *to_net(const double *f, int slot, hipStream_t s, int *rc) {
-    if (!f || rho == 1) return f;
+  // DoF-grid fields f[0..count) on the network grid, into the network-grid slots slot0, slot0+1,
+  // ... (the fields themselves when rho = 1; nullptr stays nullptr): the three 1D full weightings
+  // each carry all the fields in one launch
+  int to_net(const double *const *f, int slot0, int count, const double **net, hipStream_t s) {
+    EllFields<double> F[3] = {};
+    for (int i = 0; i < count; ++i) {
+      net[i] = f[i];
+      if (!f[i] || rho == 1) continue;
+      const int k = F[0].nf;
+      F[0].in[k] = f[i];
+      F[0].out[k] = tmpa.get() + k * tmpa_n;
+      F[1].in[k] = F[0].out[k];
+      F[1].out[k] = tmpb.get() + k * tmpb_n;
+      F[2].in[k] = F[1].out[k];
+      F[2].out[k] = cf[slot0 + i].get();
+      for (auto &g : F) g.nf = k + 1;
+      net[i] = cf[slot0 + i].get();
+    }
+    if (F[0].nf == 0) return MPDG_OK;
     const int o0[3] = {dims[0], fdims[1], fdims[2]}, o1[3] = {dims[0], dims[1], fdims[2]};
-    *rc = launch_ell_dir<double>(f, tmpa.get(), o0, fdims, 0, ww[0], wcol[0].get(), wval[0].get(), s);
-    if (!*rc) *rc = launch_ell_dir<double>(tmpa.get(), tmpb.get(), o1, o0, 1, ww[1], wcol[1].get(), wval[1].get(), s);
-    if (!*rc) *rc = launch_ell_dir<double>(tmpb.get(), cf[slot].get(), dims, o1, 2, ww[2], wcol[2].get(), wval[2].get(), s);
-    return cf[slot].get();
+    MPDG_TRY(launch_ell_dir_fields<double>(F[0], o0, fdims, 0, ww[0], wcol[0].get(), wval[0].get(), s));
+    MPDG_TRY(launch_ell_dir_fields<double>(F[1], o1, o0, 1, ww[1], wcol[1].get(), wval[1].get(), s));
+    return launch_ell_dir_fields<double>(F[2], dims, o1, 2, ww[2], wcol[2].get(), wval[2].get(), s);
   }
   int forward(const double *Tnm1, const double *Tn, const double *q, const double *z, hipStream_t s) {
-    int rc = MPDG_OK;
-    const double *a = to_net(Tnm1, 0, s, &rc);
-    MPDG_TRY(rc);
-    const double *t = to_net(Tn, 1, s, &rc);
-    MPDG_TRY(rc);
-    const double *qq = to_net(q, 2, s, &rc);
-    MPDG_TRY(rc);
-    const double *zz = to_net(z, 3, s, &rc);
-    MPDG_TRY(rc);
+    const double *in[4] = {Tnm1, Tn, q, z}, *net[4];
+    MPDG_TRY(to_net(in, 0, 4, net, s));
+    const double *a = net[0], *t = net[1], *qq = net[2], *zz = net[3];
     last_Tnm1 = a;
     last_Tn = t;
     MPDG_TRY(cnn_stats(a, t, zz, n, spart.get(), stats.get(), s));
@@ -182,12 +194,9 @@ struct mpdg_learned {
   }
   // rms of the target (T_{n+1} - T_b) / rms(z) of a sample after forward()
   int target_rms(const double *base, const double *Tnp1, double *rms, hipStream_t s) {
-    int rc = MPDG_OK;
-    const double *tp = to_net(Tnp1, 4, s, &rc);
-    MPDG_TRY(rc);
-    const double *bn = to_net(base, 5, s, &rc);
-    MPDG_TRY(rc);
-    return target_rms_net(bn, tp, rms, 512, s);
+    const double *in[2] = {Tnp1, base}, *net[2];
+    MPDG_TRY(to_net(in, 4, 2, net, s));
+    return target_rms_net(net[1], net[0], rms, 512, s);
   }
   const double *last_Tnm1 = nullptr, *last_Tn = nullptr;  // network-grid inputs of the last forward()
   int target_rms_net(const double *bn, const double *tp, double *rms, int nb, hipStream_t s) {
@@ -205,12 +214,10 @@ struct mpdg_learned {
   int backward(const double *Tnm1, const double *Tn, const double *base, const double *Tnp1, double *loss,
                hipStream_t s) {
     const int nb = 512;
-    int rc = MPDG_OK;
     const double *a = rho == 1 ? Tnm1 : cf[0].get(), *t = rho == 1 ? Tn : cf[1].get();
-    const double *tp = to_net(Tnp1, 4, s, &rc);
-    MPDG_TRY(rc);
-    const double *bn = to_net(base, 5, s, &rc);
-    MPDG_TRY(rc);
+    const double *in[2] = {Tnp1, base}, *net[2];
+    MPDG_TRY(to_net(in, 4, 2, net, s));
+    const double *tp = net[0], *bn = net[1];
     MPDG_TRY(cnn_loss(act[kLayers].get(), a, t, bn, tp, n, stats.get(), oscale, 1.0f, g0.get(), lpart.get(), nb, s));
     unsigned short *g = g0.get(), *gn = g1.get();
     for (int l = kLayers - 1; l >= 0; --l) {
