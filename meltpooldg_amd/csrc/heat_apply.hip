@@ -36,7 +36,8 @@ namespace mpdg {
 // slower everywhere (one wavefront, Q3 4x2x2: 0.70 vs 0.64 ms; its barriers are free, but its
 // 2x2-cell columns share 36 % of their rows with neighbours). Q3 (FP64 per-qp): 4x4x4 0.694 ms, 16x2x2 0.652, 8x4x2 0.645, 8x2x4 0.667,
 // 16x4x1 0.878, 8x8x1 0.945; FP32 Q3 (4 wavefronts per SIMD with constant k): 16x2x2 is 1-1.5 %
-// faster than 8x4x2 for every conductivity mode, 8x2x4 equal, 4x4x4 9 % slower. Q2: 14x3x2 beats
+// faster than 8x4x2 for every conductivity mode, 8x2x4 equal, 4x4x4 9 % slower, 8x4x2 with 4x2x2-cell
+// wavefront boxes (40.3 KB of LDS: four workgroups per CU instead of three) within 2 %. Q2: 14x3x2 beats
 // 7x4x3 by 5 % (FP64) and 25 % (FP32). Q4: FP64 keeps 4x4x3 (the longer shapes are 1-4 % slower),
 // FP32 gains 11 % with 12x2x2.
 // WX x WY x WZ: how the chunk is split into the NW wavefronts' boxes of cells (each box holds
