@@ -315,6 +315,11 @@ struct Level {
   std::vector<double> cheb_c1, cheb_c2;
   DeviceArray<double> coords;  // mapping node coordinates of this level (mapped meshes)
   DeviceArray<T> metric, metric_k;
+  // the level's own apply scratch and reduction scratch: update() runs the levels' diagonals
+  // and eigenvalue estimates concurrently, one stream per level
+  DeviceArray<T> edge;
+  ReductionScratch red;
+  DeviceArray<double> eig_sc;
 };
 
 template <typename T>
@@ -345,10 +350,19 @@ template <typename T>
 struct GMGPrecond : mpdg_precond {
   mpdg_gmg_params prm{};
   std::vector<std::unique_ptr<Level<T>>> lev;  // 0 = coarsest
-  DeviceArray<double> coarse_inv, coarse_elem, cvec_in, cvec_out, cvec_part, eig_sc, eig_hist, kcell64, kcell_pp[2];
-  DeviceArray<T> edge;  // apply scratch shared by the levels (one stream, zero between applies)
+  DeviceArray<double> coarse_inv, coarse_elem, cvec_in, cvec_out, cvec_part, eig_hist, kcell64, kcell_pp[2];
   ReductionScratch red;
   int nc_coarse_dofs = 0;
+  // update(): the coarse matrix and the levels below the finest run on streams of their own
+  // (default priority), joined to the caller's stream by events; the finest level stays on it
+  std::vector<hipStream_t> lstreams;
+  std::vector<hipEvent_t> levents;
+  hipEvent_t ev_setup = nullptr;
+  ~GMGPrecond() override {
+    for (hipStream_t st : lstreams) (void)hipStreamDestroy(st);
+    for (hipEvent_t e : levents) (void)hipEventDestroy(e);
+    if (ev_setup) (void)hipEventDestroy(ev_setup);
+  }
 
   int build() {
     const auto &f = op->d64;
@@ -427,16 +441,25 @@ struct GMGPrecond : mpdg_precond {
       MPDG_TRY(L.t1.resize(t1));
       MPDG_TRY(L.t2.resize(t2));
     }
-    size_t ne = 0;
-    for (auto &L : lev) ne = std::max(ne, apply_edge_elems(L->od.degree, L->od.nc, std::is_same<T, float>::value));
-    MPDG_TRY(edge.resize_zeroed(ne));
     for (auto &L : lev) {
-      L->od.edge = edge.get();
+      const size_t ne = apply_edge_elems(L->od.degree, L->od.nc, std::is_same<T, float>::value);
+      MPDG_TRY(L->edge.resize_zeroed(ne));
+      L->od.edge = L->edge.get();
       L->od.edge_elems = ne;
+      MPDG_TRY(L->red.init());
+      MPDG_TRY(L->eig_sc.resize(4));
+    }
+    if (hipEventCreateWithFlags(&ev_setup, hipEventDisableTiming) != hipSuccess) return MPDG_ERR_DEVICE;
+    for (size_t l = 0; l + 1 < lev.size(); ++l) {
+      hipStream_t st;
+      hipEvent_t e;
+      if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return MPDG_ERR_DEVICE;
+      lstreams.push_back(st);
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return MPDG_ERR_DEVICE;
+      levents.push_back(e);
     }
     if (op->coords) MPDG_TRY(build_geometry());
     MPDG_TRY(red.init());
-    MPDG_TRY(eig_sc.resize(4));
     MPDG_TRY(eig_hist.resize(std::max<size_t>(lev.size(), 2) * hist_stride()));
     nc_coarse_dofs = (int)lev[0]->od.n_dofs;
     MPDG_TRY(coarse_inv.resize((size_t)nc_coarse_dofs * nc_coarse_dofs));
@@ -549,22 +572,36 @@ struct GMGPrecond : mpdg_precond {
         }
         MPDG_TRY(launch_convert<double, T>(kf, L.kcell.get(), L.od.n_cells, s));
         L.od.kappa_arr = L.kcell.get();
-        if (l == 0) MPDG_TRY(assemble_coarse(kf, s));
       }
+      // the coarse matrix and its inverse, concurrently with the levels' set-up below (kf: the
+      // coarsest cells' conductivities, not written again in this update)
+      if (hipEventRecord(ev_setup, s) != hipSuccess) return MPDG_ERR_DEVICE;
+      if (hipStreamWaitEvent(lstreams[0], ev_setup, 0) != hipSuccess) return MPDG_ERR_DEVICE;
+      MPDG_TRY(assemble_coarse(kf, lstreams[0]));
+      if (hipEventRecord(levents[0], lstreams[0]) != hipSuccess) return MPDG_ERR_DEVICE;
     } else {
       MPDG_TRY(assemble_coarse(nullptr, s));
     }
-    // inverse diagonals and eigenvalue estimates
+    // inverse diagonals and eigenvalue estimates, level l > 0 on its own stream (the finest on
+    // the caller's): the levels share no data here, and the coarse ones are short, launch-bound
+    // chains that run beside the finest level's
     for (size_t l = 1; l < lev.size(); ++l) {
       Level<T> &L = *lev[l];
+      const bool own = l + 1 < lev.size();
+      hipStream_t sl = own ? lstreams[l] : s;
+      if (own && hipStreamWaitEvent(sl, ev_setup, 0) != hipSuccess) return MPDG_ERR_DEVICE;
       const int n1 = L.od.degree + 1;
       MPDG_TRY(L.diag_scratch.resize(2 * (size_t)L.od.n_cells * n1 * n1 * n1));
-      MPDG_TRY(launch_diagonal<T>(L.od, L.inv.get(), L.diag_scratch.get(), s));
-      MPDG_TRY(launch_invert_diagonal<T>(L.inv.get(), L.od.n_dofs, s));
+      MPDG_TRY(launch_diagonal<T>(L.od, L.inv.get(), L.diag_scratch.get(), sl));
+      MPDG_TRY(launch_invert_diagonal<T>(L.inv.get(), L.od.n_dofs, sl));
       // smoothing leaves the constrained DoFs at zero
-      MPDG_TRY(launch_zero_constrained<T>(L.inv.get(), L.od.nd, L.od.dmask, s));
-      MPDG_TRY(run_eigenvalue_cg(L, eig_hist.get() + (l - 1) * hist_stride(), s));
+      MPDG_TRY(launch_zero_constrained<T>(L.inv.get(), L.od.nd, L.od.dmask, sl));
+      MPDG_TRY(run_eigenvalue_cg(L, eig_hist.get() + (l - 1) * hist_stride(), sl));
+      if (own && hipEventRecord(levents[l], sl) != hipSuccess) return MPDG_ERR_DEVICE;
     }
+    if (lev.size() > 1)
+      for (size_t l = 0; l + 1 < lev.size(); ++l)
+        if (hipStreamWaitEvent(s, levents[l], 0) != hipSuccess) return MPDG_ERR_DEVICE;
     // one copy of every level's CG coefficients, then lambda_max and the Chebyshev coefficients
     const size_t nh = (lev.size() - 1) * hist_stride();
     std::vector<double> h(nh);
@@ -589,16 +626,16 @@ struct GMGPrecond : mpdg_precond {
     const int it = prm.eig_cg_n_iterations;
     // r = start vector (b, x0 = 0), z = inv r, p = z
     MPDG_TRY(launch_eig_start<T>(L.r.get(), L.od.nd, L.od.dmask, s));
-    MPDG_TRY(launch_eig_step<T>(0, L.r.get(), L.b.get(), L.d.get(), nullptr, L.inv.get(), n, red.red(), eig_sc.get(),
-                                hist, 0, s));
+    MPDG_TRY(launch_eig_step<T>(0, L.r.get(), L.b.get(), L.d.get(), nullptr, L.inv.get(), n, L.red.red(),
+                                L.eig_sc.get(), hist, 0, s));
     for (int k = 0; k < it; ++k) {
       MPDG_TRY(level_apply<T>(L.od, L.d.get(), L.Ax.get(), s));
-      MPDG_TRY(launch_eig_step<T>(1, L.r.get(), L.b.get(), L.d.get(), L.Ax.get(), L.inv.get(), n, red.red(),
-                                  eig_sc.get(), hist, k, s));
-      MPDG_TRY(launch_eig_step<T>(2, L.r.get(), L.b.get(), L.d.get(), L.Ax.get(), L.inv.get(), n, red.red(),
-                                  eig_sc.get(), hist, k, s));
-      MPDG_TRY(launch_eig_step<T>(3, L.r.get(), L.b.get(), L.d.get(), L.Ax.get(), L.inv.get(), n, red.red(),
-                                  eig_sc.get(), hist, k, s));
+      MPDG_TRY(launch_eig_step<T>(1, L.r.get(), L.b.get(), L.d.get(), L.Ax.get(), L.inv.get(), n, L.red.red(),
+                                  L.eig_sc.get(), hist, k, s));
+      MPDG_TRY(launch_eig_step<T>(2, L.r.get(), L.b.get(), L.d.get(), L.Ax.get(), L.inv.get(), n, L.red.red(),
+                                  L.eig_sc.get(), hist, k, s));
+      MPDG_TRY(launch_eig_step<T>(3, L.r.get(), L.b.get(), L.d.get(), L.Ax.get(), L.inv.get(), n, L.red.red(),
+                                  L.eig_sc.get(), hist, k, s));
     }
     return MPDG_OK;
   }
