@@ -43,6 +43,9 @@ HBM_PEAK_GBS = 8000.0
 # profiler runs, so the bench reports the profiled figure for these workloads and null for others.
 PROFILED_TRAFFIC = {(3, 128, "qp", False): 2.506e+09, (4, 96, "qp", False): 2.360e+09,
                     (4, 96, "law", False): 2.292e+09, (3, 96, "const", True): 3.759e+09}
+# learned predictor: after PAUSE[0] consecutive steps whose learned guess loses the guess check, the
+# network is not evaluated for PAUSE[1] steps, then tried again
+PAUSE = (2, 8)
 
 
 def parse():
@@ -274,7 +277,7 @@ def scan_timings(torch, M, k, N, n_train, n_eval, epochs, rho, only="all", law=F
             torch.cuda.synchronize()
             nets[order] = net
             training[name] = {"pairs": n_train - 2, "epochs": epochs, "lr": 2e-3, "lr_decay": 0.7, "coarsening": rho,
-                              "seconds": round(time.perf_counter() - t0, 2), "loss_first_epoch": losses[0],
+                              "guess_pause": list(PAUSE), "seconds": round(time.perf_counter() - t0, 2), "loss_first_epoch": losses[0],
                               "loss_last_epoch": losses[-1]}
         if Ts_o is not Ts:
             del Ts_o
@@ -300,10 +303,12 @@ def scan_timings(torch, M, k, N, n_train, n_eval, epochs, rho, only="all", law=F
     for name, pc, pred, order, pre in variants:
         # _update4: the multigrid is rebuilt every 4th step only (any fixed SPD preconditioner
         # keeps CG exact; the FP64 residual test decides)
+        # the learned predictor pauses for PAUSE[1] steps after PAUSE[0] consecutive rejections
         case = LaserScan(k, N, dt=dt, precond=pc, predictor=pred,
                          learned=nets.get(order) if pred == M.PREDICTOR_LEARNED else None,
                          pre_smoother_degree=pre, extrapolation_order=order, guess_smoothing=smoothing,
-                         preconditioner_update_interval=4 if name.endswith("_update4") else 1, **kw)
+                         preconditioner_update_interval=4 if name.endswith("_update4") else 1,
+                         guess_pause_after=PAUSE[0], guess_pause_steps=PAUSE[1], **kw)
         # restart from the recorded state; the replayed steps rebuild the history the predictor uses
         case.t = (n_train - order) * dt
         case.heat.set_initial_condition(Ts[n_train - order])
@@ -324,7 +329,8 @@ def scan_timings(torch, M, k, N, n_train, n_eval, epochs, rho, only="all", law=F
                 errs.append(float(torch.linalg.norm(T - T_ref[j]) / torch.linalg.norm(T_ref[j])))
         row = {"ms_per_step": round(float(np.mean(ms)), 3), "iterations": float(np.mean(its)),
                "log10_initial_rel_residual": round(float(np.mean(np.log10(r0s))), 3),
-               "fallbacks": int(sum(fbs)), "final_rel_residual_max": float(max(rs))}
+               "fallbacks": fbs.count(M.GUESS_REJECTED), "paused": fbs.count(M.GUESS_PAUSED),
+               "final_rel_residual_max": float(max(rs))}
         if errs:
             row["rel_error_vs_fp64_max"] = float(max(errs))
         out[name] = row
@@ -363,7 +369,7 @@ def learned_c4(torch, M, n_steps=200, n_train=150, epochs=8, cells=64, rho=2, sm
                        "steps 2-%d, evaluated on steps %d-%d; FP32 multigrid V(1,3), coarse Q2 4^3, flexible CG "
                        "to ||b-JT|| <= 1e-10 ||b||" % (cells, (2 * cells + 1) ** 3, n_steps, n_train, n_train + 1,
                                                        n_steps)}
-    out["network"] = {"working_grid_coarsening": rho, "guess_smoothing": smoothing}
+    out["network"] = {"working_grid_coarsening": rho, "guess_smoothing": smoothing, "guess_pause": list(PAUSE)}
     for order in orders:
         pred = M.PREDICTOR_LINEAR if order == 1 else M.PREDICTOR_EXTRAPOLATION
         t0 = time.perf_counter()
@@ -381,7 +387,8 @@ def learned_c4(torch, M, n_steps=200, n_train=150, epochs=8, cells=64, rho=2, sm
         rows = {}
         for name, p in (("base", pred), ("learned", M.PREDICTOR_LEARNED)):
             case = LaserScan(k, cells, dt=dt, predictor=p, learned=net if p == M.PREDICTOR_LEARNED else None,
-                             extrapolation_order=order, pre_smoother_degree=1, guess_smoothing=smoothing)
+                             extrapolation_order=order, pre_smoother_degree=1, guess_smoothing=smoothing,
+                             guess_pause_after=PAUSE[0], guess_pause_steps=PAUSE[1])
             case.t = (n_train - order) * dt
             case.heat.set_initial_condition(Ts[n_train - order])
             for _ in range(order):
@@ -395,7 +402,7 @@ def learned_c4(torch, M, n_steps=200, n_train=150, epochs=8, cells=64, rho=2, sm
                 fbs.append(fb)
             rows[name] = {"ms_per_step": round(float(np.mean(ms)), 3), "iterations": float(np.mean(its)),
                           "log10_initial_rel_residual": round(float(np.mean(np.log10(r0s))), 3),
-                          "fallbacks": int(sum(fbs))}
+                          "fallbacks": fbs.count(M.GUESS_REJECTED), "paused": fbs.count(M.GUESS_PAUSED)}
             del case
         name = "base_linear" if order == 1 else "base_extrapolation2"
         out[name] = {"trajectory_seconds": round(t_traj, 2), "training_seconds": round(t_train, 2),

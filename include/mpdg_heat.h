@@ -251,6 +251,11 @@ typedef struct mpdg_heat_operation_params {
   int preconditioner_update_interval; /* linear step mode with a conductivity law: the
                                  preconditioner is rebuilt every this many steps (0 or 1: every
                                  step, as the reference; a change of dt always rebuilds it) */
+  int guess_pause_after;  /* learned predictor with guess_check in linear step mode: after this many
+                             consecutive steps whose learned guess the check rejected, the network
+                             is not evaluated for guess_pause_steps steps (the base guess is used
+                             as is), then tried again (0: never pause) */
+  int guess_pause_steps;
 } mpdg_heat_operation_params;
 int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p,
                                const mpdg_heat_operation_params *params,
@@ -274,8 +279,11 @@ int mpdg_heat_operation_set_learned(mpdg_heat_operation *h, mpdg_learned *net, c
    itself; z = NULL stops recording. Training pairs for mpdg_learned_train_step come from a run
    with this set. */
 int mpdg_heat_operation_record_guess_feature(mpdg_heat_operation *h, double *z, double *base);
-/* statistics of the last step: 1 if the learned guess was replaced by T_n, initial and final
-   relative residuals ||b - J T||/||b|| of the step's linear solve (MPDG_STEP_LINEAR) */
+/* statistics of the last step: guess status (0: the predictor's guess was used; 1: the learned
+   guess was rejected by the guess check and replaced by its base guess; 2: the learned predictor
+   was paused after repeated rejections and the base guess used without evaluating the network),
+   initial and final relative residuals ||b - J T||/||b|| of the step's linear solve
+   (MPDG_STEP_LINEAR) */
 int mpdg_heat_operation_last_step_info(const mpdg_heat_operation *h, int *fell_back, double *rel_res0,
                                        double *rel_res);
 /* device pointers to the current / previous solution (owned by the handle) */

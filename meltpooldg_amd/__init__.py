@@ -76,10 +76,12 @@ class OperationParams(C.Structure):
                 ("residual_tolerance", C.c_double), ("field_correction_tolerance", C.c_double),
                 ("step_mode", C.c_int), ("guess_check", C.c_int), ("guess_smoothing", C.c_int),
                 ("extrapolation_order", C.c_int), ("n_old_solution_vectors", C.c_int),
-                ("preconditioner_update_interval", C.c_int)]
+                ("preconditioner_update_interval", C.c_int), ("guess_pause_after", C.c_int),
+                ("guess_pause_steps", C.c_int)]
 
 
 STEP_NEWTON, STEP_LINEAR = 0, 1
+GUESS_USED, GUESS_REJECTED, GUESS_PAUSED = 0, 1, 2
 
 
 _V, _I, _D, _L = C.c_void_p, C.c_int, C.c_double, C.c_long
@@ -493,7 +495,8 @@ class HeatDiffuseOperation:
 
     def __init__(self, op, preconditioner=None, predictor=PREDICTOR_LINEAR, linear=None, nlsolve=None,
                  step_mode=STEP_NEWTON, learned=None, q_field=None, guess_check=True, guess_smoothing=0,
-                 extrapolation_order=2, n_old_solution_vectors=2, preconditioner_update_interval=1):
+                 extrapolation_order=2, n_old_solution_vectors=2, preconditioner_update_interval=1,
+                 guess_pause_after=0, guess_pause_steps=0):
         self.op = op
         self.lib = op.lib
         self.pc = preconditioner
@@ -512,6 +515,8 @@ class HeatDiffuseOperation:
         p.extrapolation_order = extrapolation_order
         p.n_old_solution_vectors = n_old_solution_vectors
         p.preconditioner_update_interval = preconditioner_update_interval
+        p.guess_pause_after = guess_pause_after
+        p.guess_pause_steps = guess_pause_steps
         h = C.c_void_p()
         _check(self.lib.mpdg_heat_operation_create(op.h, preconditioner.h if preconditioner else None,
                                                    C.byref(p), C.byref(h)), "heat_operation_create")
@@ -528,10 +533,13 @@ class HeatDiffuseOperation:
                "record_guess_feature")
 
     def last_step_info(self):
-        """(learned guess replaced by its base guess, initial and final ||b - J T||/||b|| of the step)"""
+        """(guess status, initial and final ||b - J T||/||b|| of the step); the status is
+        GUESS_USED (0), GUESS_REJECTED (1: the learned guess lost the guess check and its base
+        guess was used) or GUESS_PAUSED (2: the network was not evaluated after repeated
+        rejections, see guess_pause_after)"""
         fb, r0, r = C.c_int(), C.c_double(), C.c_double()
         _check(self.lib.mpdg_heat_operation_last_step_info(self.h, C.byref(fb), C.byref(r0), C.byref(r)), "info")
-        return bool(fb.value), r0.value, r.value
+        return fb.value, r0.value, r.value
 
     def old_solution(self):
         torch = _torch()

@@ -20,7 +20,7 @@ class LaserScan:
                  x0=0.3, y0=0.5, precond="gmg32", predictor=M.PREDICTOR_NONE, learned=None, rel_tol=1e-10,
                  n_cells_coarse=4, coarse_degree=2, guess_check=True, smoother_degree=3, pre_smoother_degree=0,
                  guess_smoothing=0, extrapolation_order=2, n_old_solution_vectors=2,
-                 preconditioner_update_interval=1):
+                 preconditioner_update_interval=1, guess_pause_after=0, guess_pause_steps=0):
         import torch
         self.torch = torch
         self.dt, self.peak, self.sigma, self.speed, self.x0, self.y0 = dt, peak, sigma, speed, x0, y0
@@ -56,7 +56,8 @@ class LaserScan:
             step_mode=M.STEP_LINEAR, learned=learned, q_field=self.q, guess_check=guess_check,
             guess_smoothing=guess_smoothing, extrapolation_order=extrapolation_order,
             n_old_solution_vectors=n_old_solution_vectors,
-            preconditioner_update_interval=preconditioner_update_interval)
+            preconditioner_update_interval=preconditioner_update_interval,
+            guess_pause_after=guess_pause_after, guess_pause_steps=guess_pause_steps)
         self.heat.set_initial_condition(self.T0)
         self.t = 0.0
         self.dims = tuple(k * degree + 1 for k in nc)

@@ -1072,7 +1072,9 @@ struct mpdg_heat_operation {
   DeviceArray<double> kq_step;       // k(T_n) per quadrature point, frozen for a linear step
   DeviceArray<double> lsq_x, lsq_q, lsq_part, lsq_h;  // least-squares projection: bases and scalars
   bool law_frozen = false;
-  int fell_back = 0;
+  int fell_back = 0;            // guess status of the last step (0 used, 1 rejected, 2 paused)
+  int guess_rejections = 0;     // consecutive steps whose learned guess the check rejected
+  int guess_paused = 0;         // steps left on which the network is not evaluated
   int steps_since_update = 0;  // linear steps since the preconditioner was rebuilt
   int n_calls = 0;             // predictor calls since set_initial (Predictor::n_calls, predictor.hpp:97)
   double rel_res0 = 0.0, rel_res = 0.0;
@@ -1430,6 +1432,7 @@ int mpdg_heat_operation_create(mpdg_heat_op *op, mpdg_precond *p, const mpdg_hea
     return MPDG_ERR_INVALID;
   if (prm->step_mode != MPDG_STEP_NEWTON && prm->step_mode != MPDG_STEP_LINEAR) return MPDG_ERR_INVALID;
   if (!valid_solver(prm->linear)) return MPDG_ERR_INVALID;
+  if (prm->guess_pause_after < 0 || prm->guess_pause_steps < 0) return MPDG_ERR_INVALID;
   if ((prm->predictor == MPDG_PREDICTOR_EXTRAPOLATION || prm->predictor == MPDG_PREDICTOR_LEARNED) &&
       (prm->extrapolation_order < (prm->predictor == MPDG_PREDICTOR_LEARNED ? 0 : 1) ||
        prm->extrapolation_order > mpdg_heat_operation::kMaxOlder + 1))
@@ -1475,6 +1478,8 @@ int mpdg_heat_operation_set_initial(mpdg_heat_operation *h, const double *T0, vo
   h->dt_old = 0.0;
   h->n_older = 0;
   h->n_calls = 0;
+  h->guess_rejections = 0;
+  h->guess_paused = 0;
   return hipStreamSynchronize(S(stream)) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
@@ -1506,11 +1511,18 @@ static int guess_feature(mpdg_heat_operation *h, const double *b, const double *
 // Learned predictor after the base guess T (the extrapolation) is in place: records the
 // residual input (and the base) and, with equal steps and a network, replaces T by the
 // network's correction of it; the base stays in tbase as the fallback, and the base's squared
-// residual norm |b - J T|^2 in scal[1] for the guess check (linear step).
+// residual norm |b - J T|^2 in scal[1] for the guess check (linear step). While the predictor
+// is paused (after repeated rejections by the check) the base guess stays and the network is
+// not evaluated.
 static int learned_guess(mpdg_heat_operation *h, double dt, const double *b, double *T, const double *T_old,
                          hipStream_t s) {
-  const bool learned = h->prm.predictor == MPDG_PREDICTOR_LEARNED;
+  bool learned = h->prm.predictor == MPDG_PREDICTOR_LEARNED;
   h->learned_applied = false;
+  if (learned && b && h->prm.guess_check && h->guess_paused > 0) {
+    --h->guess_paused;
+    h->fell_back = 2;
+    learned = false;
+  }
   if (!learned && !h->z_record) return MPDG_OK;
   const long n = h->op->n_dofs();
   double *z = h->z_record;
@@ -1615,8 +1627,15 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
     if (!(rr[0] == rr[0]) || !(rr[0] < rr[1])) {
       MPDG_TRY(launch_convert<double, double>(h->tbase.get(), T, n, s));
       h->fell_back = 1;
+      // a run of rejections pauses the network (its evaluation and check cost about three
+      // operator applies and a smoothing per step for nothing), then it is tried again
+      if (h->prm.guess_pause_after > 0 && ++h->guess_rejections >= h->prm.guess_pause_after) {
+        h->guess_paused = h->prm.guess_pause_steps;
+        h->guess_rejections = 0;
+      }
     } else {
       r_init = h->alt.get();  // b - J T of the accepted guess: the solve starts from it
+      h->guess_rejections = 0;
     }
   }
   mpdg_solver_params lp = h->prm.linear;

@@ -58,3 +58,30 @@ def test_python_mirror_fails_loudly_without_gpu():
         pytest.skip("GPU present")
     with pytest.raises(RuntimeError):
         M.HeatDiffuseMultiPhaseOperator(2, (2, 2, 2))
+
+
+def test_python_struct_mirrors_match_the_header(tmp_path):
+    """Every ctypes mirror of a parameter struct has the header's size and field offsets (a C
+    program built with gcc from include/mpdg_heat.h prints them)."""
+    import subprocess
+    import meltpooldg_amd as M
+    mirrors = {"mpdg_heat_desc": M.HeatDesc, "mpdg_gmg_params": M.GMGParams, "mpdg_solver_params": M.SolverParams,
+               "mpdg_heat_material": M.Material, "mpdg_heat_operation_params": M.OperationParams}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mpdg_heat.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append('  printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('  printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                        check=True).stdout.splitlines())
+    for cname, py in mirrors.items():
+        assert int(got["%s size" % cname]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got["%s %s" % (cname, f)]) == getattr(py, f).offset, (cname, f)

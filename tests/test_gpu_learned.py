@@ -225,3 +225,33 @@ def test_learned_guess_beats_its_base_predictor_q2_32(cuda, order):
     assert fl == 0
     assert ll.mean() < lb.mean() - 0.3 and np.all(ll < lb)
     assert il.mean() < ib.mean()
+
+
+def test_rejected_network_is_paused_and_leaves_the_base_run_unchanged(cuda):
+    """A network whose guesses never pass the guess check (all weights zero: the guess is its
+    base, whose residual is not smaller) is paused for guess_pause_steps steps after
+    guess_pause_after consecutive rejections, then tried again. Every step then uses the base
+    guess, so the run is bitwise that of the linear predictor alone."""
+    torch = cuda
+    import meltpooldg_amd as M
+    from meltpooldg_amd.application import LaserScan
+    k, cells, dt, n_steps = 2, 8, 1e-3, 12
+    net = M.LearnedPredictor((cells,) * 3, degree=k, qscale=dt * cells, seed=3)
+    net.set_weights(np.zeros_like(net.get_weights()))
+    runs = {}
+    for name, pred in (("linear", M.PREDICTOR_LINEAR), ("learned", M.PREDICTOR_LEARNED)):
+        case = LaserScan(k, cells, dt=dt, predictor=pred, learned=net if pred == M.PREDICTOR_LEARNED else None,
+                         extrapolation_order=1, guess_pause_after=2, guess_pause_steps=3)
+        sols, status = [], []
+        for _ in range(n_steps):
+            _, _, (st, r0, r) = case.step()
+            assert r <= 1.0001e-10
+            sols.append(case.solution().clone())
+            status.append(st)
+        runs[name] = (sols, status)
+    # step 1 has no step history (no network); then rejected, rejected, paused x3, ...
+    R, P = M.GUESS_REJECTED, M.GUESS_PAUSED
+    assert runs["linear"][1] == [M.GUESS_USED] * n_steps
+    assert runs["learned"][1] == [M.GUESS_USED, R, R, P, P, P, R, R, P, P, P, R]
+    for a, b in zip(runs["linear"][0], runs["learned"][0]):
+        assert torch.equal(a, b)
