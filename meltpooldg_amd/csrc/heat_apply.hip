@@ -60,6 +60,13 @@ template <> struct ChunkShapeOf<4, double, true> {
   struct type { static constexpr int BX = 16, BY = 2, BZ = 2, NW = 4, WX = 1, WY = 2, WZ = 2; };
 };
 
+// Brick rows per batch of the write-out, whose LDS reads are all issued before any of its sums
+// (0: one row at a time, each read waited for in turn). Batches pay where the write-out's LDS
+// latency is exposed: the FP64 Q4 apply (one wavefront per SIMD) runs 6 % faster with batches
+// of 5 rows; at Q3 they change nothing in FP64 and cost FP32 its occupancy (spills).
+template <int N1, typename Number, bool MAPPED> struct WriteOutBatch { static constexpr int value = 0; };
+template <> struct WriteOutBatch<5, double, false> { static constexpr int value = 5; };
+
 template <int N1, typename Number = double, bool MAPPED = false> struct ApplyGeom {
   using Shape = typename ChunkShapeOf<N1, Number, MAPPED>::type;
   static constexpr int K = N1 - 1;
@@ -88,11 +95,20 @@ template <int N1, typename Number = double, bool MAPPED = false> struct ApplyGeo
   static constexpr int TBUF_DOUBLES = (TBUF * (int)sizeof(Number) + 7) / 8;
   static constexpr int CARRY = AY * AZ;
   static constexpr int ROWS = AY * AZ;                 // x rows of the DoF brick
-  // LDS: transpose slots, sub-bricks, two carry planes, then per brick row its global offset
-  // (8 B) and its write-out code (4 B)
-  static constexpr int LDS_BYTES = 8 * (TBUF_DOUBLES + ACC + 2 * CARRY + ROWS) + 4 * ROWS;
+  // LDS: transpose slots, sub-bricks, two carry planes, a zero entry, then per brick row its
+  // global offset (8 B) and its write-out code (4 B)
+  static constexpr int LDS_BYTES = 8 * (TBUF_DOUBLES + ACC + 2 * CARRY + 1 + ROWS) + 4 * ROWS;
+  // the zero entry, relative to the sub-bricks: what the write-out reads (and clears) for the
+  // sub-brick entries a brick position does not have
+  static constexpr int ZERO_ENTRY = ACC + 2 * CARRY;
   // x rows of the DoF brick written per pass of the write-out loop (lane = x in the row)
   static constexpr int ROWS_PER_PASS = NT / AX;
+  // brick rows per thread, and per batch of the write-out (all LDS reads of a batch are issued
+  // before any of its sums)
+  static constexpr int ROWS_PER_THREAD = (ROWS + ROWS_PER_PASS - 1) / ROWS_PER_PASS;
+  static constexpr int WO_BATCH = ROWS_PER_THREAD < WriteOutBatch<N1, Number, MAPPED>::value
+                                     ? ROWS_PER_THREAD : WriteOutBatch<N1, Number, MAPPED>::value;
+  static constexpr int WO_ENTRIES = (WX > 1 ? 2 : 1) * (WY > 1 ? 2 : 1) * (WZ > 1 ? 2 : 1);
 };
 
 // ---- register-level 1D contractions on an N1 x N1 array a[row][col] ----
@@ -335,7 +351,7 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
   double *const lds_raw = dyn_lds;
   double *const acc = dyn_lds + G::TBUF_DOUBLES;
   double *const carry = acc + G::ACC;
-  long *const row_off = reinterpret_cast<long *>(carry + 2 * G::CARRY);
+  long *const row_off = reinterpret_cast<long *>(carry + 2 * G::CARRY + 1);
   int *const row_code = reinterpret_cast<int *>(row_off + G::ROWS);
   Number *const lds = reinterpret_cast<Number *>(lds_raw);  // transpose slots
 
@@ -363,7 +379,7 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
   const int nchunks = (P.nc[0] + G::BX - 1) / G::BX;
   // the brick starts at zero; the write-out of every chunk zeroes what it reads
   for (int idx = threadIdx.x; idx < G::ACC; idx += G::NT) acc[idx] = 0.0;
-  for (int idx = threadIdx.x; idx < 2 * G::CARRY; idx += G::NT) carry[idx] = 0.0;
+  for (int idx = threadIdx.x; idx < 2 * G::CARRY + 1; idx += G::NT) carry[idx] = 0.0;  // and the zero entry
   // what the write-out needs of each x row (y, z) of the brick, the same for every chunk of
   // the column: its global offset (in dst, or in the edge buffer for the upper layer's rows
   // where four columns meet) and a code: y/z part of its sub-brick entry, whether it lies on a
@@ -594,43 +610,114 @@ void heat_apply_kernel(const ApplyArgs<Number, N1> P,
     const int wr = tid / G::AX, wx = tid - wr * G::AX;
     if (wr < G::ROWS_PER_PASS && wx < DX) {
       const int ix = X0 + wx;
-      int xe[2];
+      int xe[2] = {0, 0};
       const int nxe = brick_parts<G::WX, K * G::EX, G::SUB, 1>(wx, xe);
       const bool con_x = ((P.dmask & 1) && ix == 0) || ((P.dmask & 2) && ix == P.nd[0] - 1);
       const bool carry_out = wx == DX - 1 && !last, carry_in = wx == 0 && chunk > 0;
       double *const carry_rd = carry + ((chunk + 1) & 1) * G::CARRY;
       double *const carry_wr = carry + (chunk & 1) * G::CARRY;
-      for (int r = wr; r < G::ROWS; r += G::ROWS_PER_PASS) {
-        const int code = row_code[r];
-        if (code & WO_SKIP) continue;
-        const int yz = code & WO_PART;
+      if constexpr (G::WO_BATCH > 0) {
         // the previous box's entry in y / z: one box back, at the far end of its sub-brick
         constexpr int DYP = -G::WX * G::SUB + K * G::EY * G::SX;
         constexpr int DZP = -G::WX * G::WY * G::SUB + K * G::EZ * G::SX * G::SY;
-        const int y2 = (code & WO_Y2) ? 1 : 0, z2 = (code & WO_Z2) ? 1 : 0;
-        double sum = carry_in ? carry_rd[r] : 0.0;
+        constexpr int RB = G::WO_BATCH, NE = G::WO_ENTRIES;
 #pragma unroll
-        for (int c = 0; c < (G::WZ > 1 ? 2 : 1); ++c)
+        for (int k0 = 0; k0 < G::ROWS_PER_THREAD; k0 += RB) {
+          // 1. the rows' codes and offsets, 2. their sub-brick entries (lower boxes first; an
+          // absent one reads the zero entry) and carries, 3. clear the entries, 4. sums in that
+          // order and the global stores / adds. All LDS reads of a batch are in flight together.
+          int code[RB], rr[RB];
+          long off[RB];
 #pragma unroll
-          for (int b = 0; b < (G::WY > 1 ? 2 : 1); ++b)
+          for (int k = 0; k < RB; ++k) {
+            const int r = wr + (k0 + k) * G::ROWS_PER_PASS;
+            rr[k] = min(r, G::ROWS - 1);
+            const int c = row_code[rr[k]];
+            code[k] = (k0 + k < G::ROWS_PER_THREAD && r < G::ROWS) ? c : WO_SKIP;
+            off[k] = row_off[rr[k]];
+          }
+          int ent[RB][NE];
 #pragma unroll
-            for (int a = 0; a < (G::WX > 1 ? 2 : 1); ++a)
-              if (c <= z2 && b <= y2 && a < nxe) {
-                const int e = xe[a] + yz + (c < z2 ? DZP : 0) + (b < y2 ? DYP : 0);
-                sum += acc[e];
-                acc[e] = 0.0;
-              }
-        if (carry_out) {
-          carry_wr[r] = sum;
-          continue;
+          for (int k = 0; k < RB; ++k) {
+            const int c = code[k];
+            const int yz = c & WO_PART, y2 = (c & WO_Y2) ? 1 : 0, z2 = (c & WO_Z2) ? 1 : 0;
+            const bool live = !(c & WO_SKIP);
+            int i = 0;
+#pragma unroll
+            for (int cz = 0; cz < (G::WZ > 1 ? 2 : 1); ++cz)
+#pragma unroll
+              for (int by = 0; by < (G::WY > 1 ? 2 : 1); ++by)
+#pragma unroll
+                for (int ax = 0; ax < (G::WX > 1 ? 2 : 1); ++ax, ++i) {
+                  const bool present = live && cz <= z2 && by <= y2 && ax < nxe;
+                  ent[k][i] = present ? xe[ax] + yz + (cz < z2 ? DZP : 0) + (by < y2 ? DYP : 0) : G::ZERO_ENTRY;
+                }
+          }
+          double v[RB][NE], cr[RB];
+#pragma unroll
+          for (int k = 0; k < RB; ++k) {
+            const double c = carry_rd[rr[k]];
+            cr[k] = carry_in ? c : 0.0;
+#pragma unroll
+            for (int i = 0; i < NE; ++i) v[k][i] = acc[ent[k][i]];
+          }
+#pragma unroll
+          for (int k = 0; k < RB; ++k)
+#pragma unroll
+            for (int i = 0; i < NE; ++i) acc[ent[k][i]] = 0.0;
+#pragma unroll
+          for (int k = 0; k < RB; ++k) {
+            if (code[k] & WO_SKIP) continue;
+            double sum = cr[k];
+#pragma unroll
+            for (int i = 0; i < NE; ++i) sum += v[k][i];
+            if (carry_out) {
+              carry_wr[rr[k]] = sum;
+              continue;
+            }
+            const Number val = (Number)sum;
+            const bool con = con_x || (code[k] & WO_CON);
+            const long g = off[k] + ix;
+            if (code[k] & WO_FACE) {
+              if (!con) atomicAdd(((code[k] & WO_EDGE) ? P.edge : dst) + g, val);
+            } else {
+              dst[g] = con ? Number(0) : val;
+            }
+          }
         }
-        const Number val = (Number)sum;
-        const bool con = con_x || (code & WO_CON);
-        const long g = row_off[r] + ix;
-        if (code & WO_FACE) {
-          if (!con) atomicAdd(((code & WO_EDGE) ? P.edge : dst) + g, val);
-        } else {
-          dst[g] = con ? Number(0) : val;
+      } else {
+        for (int r = wr; r < G::ROWS; r += G::ROWS_PER_PASS) {
+          const int code = row_code[r];
+          if (code & WO_SKIP) continue;
+          const int yz = code & WO_PART;
+          // the previous box's entry in y / z: one box back, at the far end of its sub-brick
+          constexpr int DYP = -G::WX * G::SUB + K * G::EY * G::SX;
+          constexpr int DZP = -G::WX * G::WY * G::SUB + K * G::EZ * G::SX * G::SY;
+          const int y2 = (code & WO_Y2) ? 1 : 0, z2 = (code & WO_Z2) ? 1 : 0;
+          double sum = carry_in ? carry_rd[r] : 0.0;
+#pragma unroll
+          for (int c = 0; c < (G::WZ > 1 ? 2 : 1); ++c)
+#pragma unroll
+            for (int b = 0; b < (G::WY > 1 ? 2 : 1); ++b)
+#pragma unroll
+              for (int a = 0; a < (G::WX > 1 ? 2 : 1); ++a)
+                if (c <= z2 && b <= y2 && a < nxe) {
+                  const int e = xe[a] + yz + (c < z2 ? DZP : 0) + (b < y2 ? DYP : 0);
+                  sum += acc[e];
+                  acc[e] = 0.0;
+                }
+          if (carry_out) {
+            carry_wr[r] = sum;
+            continue;
+          }
+          const Number val = (Number)sum;
+          const bool con = con_x || (code & WO_CON);
+          const long g = row_off[r] + ix;
+          if (code & WO_FACE) {
+            if (!con) atomicAdd(((code & WO_EDGE) ? P.edge : dst) + g, val);
+          } else {
+            dst[g] = con ? Number(0) : val;
+          }
         }
       }
     }
