@@ -180,6 +180,9 @@ typedef struct mpdg_gmg_params {
   int eig_cg_n_iterations;/* CG steps of the eigenvalue estimate */
   int pre_smoother_degree;/* Chebyshev degree before the coarse correction (0: smoother_degree);
                              the first step from a zero guess needs no operator apply */
+  int vcycle_graph;       /* 1: the V-cycle's kernels are captured into a HIP graph after each
+                             update and replayed by vmult (one launch instead of one per kernel);
+                             0: launched one by one. Same results either way. */
 } mpdg_gmg_params;
 
 int mpdg_precond_create(mpdg_heat_op *op, int type, const mpdg_gmg_params *gmg, mpdg_precond **p);

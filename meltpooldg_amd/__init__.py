@@ -57,7 +57,7 @@ class HeatDesc(C.Structure):
 class GMGParams(C.Structure):
     _fields_ = [("precision", C.c_int), ("n_cells_coarse", C.c_int), ("coarse_degree", C.c_int),
                 ("smoother_degree", C.c_int), ("smoothing_range", C.c_double),
-                ("eig_cg_n_iterations", C.c_int), ("pre_smoother_degree", C.c_int)]
+                ("eig_cg_n_iterations", C.c_int), ("pre_smoother_degree", C.c_int), ("vcycle_graph", C.c_int)]
 
 
 class SolverParams(C.Structure):
@@ -433,12 +433,13 @@ class GMGPreconditioner(Preconditioner):
     """Geometric multigrid V-cycle (recipe of unit_tests/gmg.cc) with FP32 or FP64 levels."""
 
     def __init__(self, op, precision=FP32, n_cells_coarse=4, coarse_degree=0, smoother_degree=3,
-                 smoothing_range=20.0, eig_cg_n_iterations=20, pre_smoother_degree=0):
+                 smoothing_range=20.0, eig_cg_n_iterations=20, pre_smoother_degree=0, vcycle_graph=True):
         p = GMGParams()
         p.precision, p.n_cells_coarse, p.coarse_degree = precision, n_cells_coarse, coarse_degree
         p.smoother_degree, p.smoothing_range, p.eig_cg_n_iterations = smoother_degree, smoothing_range, \
             eig_cg_n_iterations
         p.pre_smoother_degree = pre_smoother_degree
+        p.vcycle_graph = 1 if vcycle_graph else 0
         super().__init__(op, PRECOND_GMG, p)
 
     def levels(self):

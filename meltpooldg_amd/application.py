@@ -20,7 +20,7 @@ class LaserScan:
                  x0=0.3, y0=0.5, precond="gmg32", predictor=M.PREDICTOR_NONE, learned=None, rel_tol=1e-10,
                  n_cells_coarse=4, coarse_degree=2, guess_check=True, smoother_degree=3, pre_smoother_degree=0,
                  guess_smoothing=0, extrapolation_order=2, n_old_solution_vectors=2,
-                 preconditioner_update_interval=1, guess_pause_after=0, guess_pause_steps=0):
+                 preconditioner_update_interval=1, guess_pause_after=0, guess_pause_steps=0, vcycle_graph=True):
         import torch
         self.torch = torch
         self.dt, self.peak, self.sigma, self.speed, self.x0, self.y0 = dt, peak, sigma, speed, x0, y0
@@ -36,14 +36,14 @@ class LaserScan:
         if precond == "gmg32":
             self.pc = M.GMGPreconditioner(self.op, precision=M.FP32, n_cells_coarse=n_cells_coarse,
                                           coarse_degree=coarse_degree, smoother_degree=smoother_degree,
-                                          pre_smoother_degree=pre_smoother_degree)
+                                          pre_smoother_degree=pre_smoother_degree, vcycle_graph=vcycle_graph)
             solver = M.SOLVER_FCG
         elif precond in ("gmg64", "gmg64_fcg"):
             # FP64 multigrid inside CG, or inside flexible CG (for non-symmetric cycles, e.g. fewer
             # pre- than post-smoothing steps)
             self.pc = M.GMGPreconditioner(self.op, precision=M.FP64, n_cells_coarse=n_cells_coarse,
                                           coarse_degree=coarse_degree, smoother_degree=smoother_degree,
-                                          pre_smoother_degree=pre_smoother_degree)
+                                          pre_smoother_degree=pre_smoother_degree, vcycle_graph=vcycle_graph)
             solver = M.SOLVER_CG if precond == "gmg64" else M.SOLVER_FCG
         else:
             self.pc = M.JacobiPreconditioner(self.op)

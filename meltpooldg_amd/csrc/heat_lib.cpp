@@ -358,10 +358,44 @@ struct GMGPrecond : mpdg_precond {
   std::vector<hipStream_t> lstreams;
   std::vector<hipEvent_t> levents;
   hipEvent_t ev_setup = nullptr;
+  // V-cycle replay (vcycle_graph): the kernels of one V-cycle read fixed level buffers, and their
+  // arguments (level data, smoother coefficients) change only in update(); so the cycle is
+  // captured on a stream of its own at the first vmult after an update and then replayed on the
+  // caller's stream as one graph launch. update() drops the graph.
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t vgraph = nullptr;
   ~GMGPrecond() override {
+    drop_vcycle_graph();
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
     for (hipStream_t st : lstreams) (void)hipStreamDestroy(st);
     for (hipEvent_t e : levents) (void)hipEventDestroy(e);
     if (ev_setup) (void)hipEventDestroy(ev_setup);
+  }
+  void drop_vcycle_graph() {
+    if (vgraph) (void)hipGraphExecDestroy(vgraph);
+    vgraph = nullptr;
+  }
+  // one V-cycle from the finest level's b into its x, on stream s
+  int run_vcycle(hipStream_t s) {
+    const int top = (int)lev.size() - 1;
+    if (!prm.vcycle_graph) return vcycle(top, s);
+    if (!vgraph) {
+      if (!cap_stream && hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking) != hipSuccess)
+        return MPDG_ERR_DEVICE;
+      if (hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return MPDG_ERR_DEVICE;
+      const int rc = vcycle(top, cap_stream);
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(cap_stream, &g);
+      hipError_t ei = hipErrorUnknown;
+      if (rc == MPDG_OK && ec == hipSuccess && g) ei = hipGraphInstantiate(&vgraph, g, nullptr, nullptr, 0);
+      if (g) (void)hipGraphDestroy(g);
+      if (rc != MPDG_OK) return rc;
+      if (ei != hipSuccess) {
+        vgraph = nullptr;
+        return MPDG_ERR_DEVICE;
+      }
+    }
+    return hipGraphLaunch(vgraph, s) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
   }
 
   int build() {
@@ -520,6 +554,7 @@ struct GMGPrecond : mpdg_precond {
   // level data that depends on the fine operator's coefficient and time increment
   int update(hipStream_t s) override {
     if (op->tangent) return MPDG_ERR_INVALID;
+    drop_vcycle_graph();
     const auto &f = op->d64;
     Level<T> &top = *lev.back();
     for (auto &L : lev) L->od.c_mass = f.c_mass;
@@ -773,7 +808,7 @@ struct GMGPrecond : mpdg_precond {
     Level<T> &top = *lev.back();
     const long n = top.od.n_dofs;
     MPDG_TRY(launch_convert<double, T>(src, top.b.get(), n, s));
-    MPDG_TRY(vcycle((int)lev.size() - 1, s));
+    MPDG_TRY(run_vcycle(s));
     return launch_convert<T, double>(top.x.get(), dst, n, s);
   }
   float *f32_input() override {
@@ -786,7 +821,7 @@ struct GMGPrecond : mpdg_precond {
   }
   int vmult_f32_in_place(hipStream_t s) override {
     if constexpr (!std::is_same<T, float>::value) return MPDG_ERR_INVALID;
-    return vcycle((int)lev.size() - 1, s);
+    return run_vcycle(s);
   }
   int smooth(double *dst, const double *src, int degree, hipStream_t s) override {
     Level<T> &top = *lev.back();

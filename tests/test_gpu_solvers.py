@@ -78,6 +78,39 @@ def test_gmg_vcycle_fp32(cuda, degree, nc, mode, mapped):
     assert rel_err(out.cpu().numpy(), og.vcycle(b)) < 1e-4
 
 
+@pytest.mark.parametrize("precision,degree,nc,mapped", [("FP32", 3, (16, 16, 12), None), ("FP64", 3, (16, 16, 12), None),
+                                                     ("FP32", 2, (16, 12, 8), None), ("FP32", 3, (12, 12, 12), 0.05)])
+def test_gmg_vcycle_graph_replays_the_eager_cycle(cuda, precision, degree, nc, mapped):
+    """The V-cycle replayed from its captured HIP graph gives the bits of the cycle launched
+    kernel by kernel: on several replays with other work of the operator and of the eager
+    preconditioner between them, and again after a new conductivity and update() (the graph is
+    captured anew)."""
+    torch = cuda
+    import meltpooldg_amd as M
+    orc, gop, data = make_pair(torch, degree, nc, mode="qp", seed=31, dirichlet=0b000011 if mapped is None else 0,
+                               mapped=mapped)
+    rng = np.random.default_rng(32)
+    kw = dict(precision=getattr(M, precision), n_cells_coarse=4, coarse_degree=2, pre_smoother_degree=1)
+    pcg = M.GMGPreconditioner(gop, vcycle_graph=True, **kw)
+    pce = M.GMGPreconditioner(gop, vcycle_graph=False, **kw)
+    y = torch.empty(orc.n_dofs, dtype=torch.float64, device="cuda")
+    for rebuild in range(2):
+        if rebuild:
+            # a new conductivity: both preconditioners are rebuilt, the graph is captured again
+            data["k"].copy_(torch.tensor(rng.uniform(0.5, 1.5, data["k"].numel()), device="cuda"))
+        pcg.update()
+        pce.update()
+        for rep in range(4):
+            b = torch.tensor(rng.uniform(-1, 1, orc.n_dofs), device="cuda")
+            og, oe = torch.empty_like(b), torch.empty_like(b)
+            pcg.vmult(og, b)
+            gop.vmult(y, og)  # other work on the stream between replays
+            pce.vmult(oe, b)
+            torch.cuda.synchronize()
+            assert torch.isfinite(og).all()
+            assert torch.equal(og, oe), (rebuild, rep)
+
+
 @pytest.mark.parametrize("precision", ["FP64", "FP32"])
 def test_gmg_vcycle_is_bitwise_repeatable(cuda, precision):
     """Two multigrid preconditioners built on the same operator, each updated twice, give the same
