@@ -236,6 +236,46 @@ int launch_dot(const TA *a, const TB *b, long n, const Reduction &r, double *out
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 template int launch_dot<double, double>(const double *, const double *, long, const Reduction &, double *, hipStream_t);
+
+// z = b - z, |z|^2 into *rr (the dot's grid and order), then z *= scale (if given): a residual,
+// its norm and its diagonal scaling in one pass
+__global__ __launch_bounds__(256) void residual_norm_kernel(double *z, const double *b, const double *scale, long n,
+                                                            double *partials, unsigned *counter, double *rr,
+                                                            double *raw) {
+  double v[1] = {0.0};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const double r = b[i] - z[i];
+    v[0] += r * r;
+    if (raw) raw[i] = r;
+    z[i] = scale ? r * scale[i] : r;
+  }
+  if (reduce_to_last_block<1>(v, partials, counter) && threadIdx.x == 0) *rr = v[0];
+}
+int launch_residual_norm(double *z, const double *b, const double *scale, long n, const Reduction &r, double *rr,
+                         hipStream_t s, double *raw) {
+  hipLaunchKernelGGL(residual_norm_kernel, dim3(red_blocks(n)), dim3(256), 0, s, z, b, scale, n, r.partials, r.counter,
+                     rr, raw);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+// Guess check on the device: the guess (in x, its residual in r) is kept when its squared
+// residual norm sc[0] is finite and below the base guess's sc[1]; otherwise x and r are
+// replaced by the base guess and its residual. The host reads sc later, without waiting here.
+__global__ __launch_bounds__(256) void guess_select_kernel(const double *sc, double *x, const double *xb, double *r,
+                                                           const double *rb, long n) {
+  const double g = sc[0], base = sc[1];
+  if (g == g && g < base) return;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    x[i] = xb[i];
+    r[i] = rb[i];
+  }
+}
+int launch_guess_select(const double *sc, double *x, const double *xb, double *r, const double *rb, long n,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(guess_select_kernel, dim3(std::max<long>(1, std::min<long>((n + 255) / 256, 8192))), dim3(256), 0,
+                     s, sc, x, xb, r, rb, n);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
 template int launch_dot<float, float>(const float *, const float *, long, const Reduction &, double *, hipStream_t);
 
 // ================= preconditioned CG =================

@@ -67,6 +67,13 @@ int launch_chebyshev_step(T *x, T *d, const T *b, const T *Ax, const T *inv, lon
 template <typename T> int launch_residual(T *r, const T *b, const T *Ax, long n, hipStream_t s);
 template <typename T> int launch_add(T *x, const T *y, long n, hipStream_t s);
 int launch_mul(double *x, const double *y, long n, hipStream_t s);
+// z = b - z; *rr = |z|^2 (device scalar); raw (if given) = z; then z *= scale pointwise when
+// scale is given
+int launch_residual_norm(double *z, const double *b, const double *scale, long n, const Reduction &r, double *rr,
+                         hipStream_t s, double *raw = nullptr);
+// x, r <- xb, rb unless sc[0] (the guess's squared residual norm) is finite and below sc[1]
+int launch_guess_select(const double *sc, double *x, const double *xb, double *r, const double *rb, long n,
+                        hipStream_t s);
 template <typename T>
 int launch_ell_dir(const T *in, T *out, const int *odims, const int *idims, int dir, int width, const int *col,
                    const T *val, hipStream_t s, const T *in2 = nullptr, bool add = false);

@@ -57,11 +57,14 @@ __global__ __launch_bounds__(256) void conv3_kernel(const unsigned short *__rest
   float bv[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) bv[i] = bias ? bias[4 * kg + i] : 0.0f;
-  const int tx = (nx + 15) >> 4;
-  const long ntiles = (long)tx * ny * nz;
+  // tiles of 16 consecutive voxels in the lexicographic order (a tile may wrap into the next x
+  // row: every lane finds its own neighbours), so no lanes idle at the ends of short x rows
+  const long nvox = (long)nx * ny * nz;
+  const long ntiles = (nvox + 15) >> 4;
   for (long tile = wave; tile < ntiles; tile += nwaves) {
-    const int x0 = (int)(tile % tx) * 16, y = (int)((tile / tx) % ny), z = (int)(tile / ((long)tx * ny));
-    const int x = x0 + n;
+    const long v = tile * 16 + n;
+    const long vc = v < nvox ? v : nvox - 1;
+    const int x = (int)(vc % nx), y = (int)((vc / nx) % ny), z = (int)(vc / ((long)nx * ny));
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int t = 0; t < 27; ++t) {
@@ -72,8 +75,7 @@ __global__ __launch_bounds__(256) void conv3_kernel(const unsigned short *__rest
         b = load4(in + ((long)xs + (long)nx * (ys + (long)ny * zs)) * 16 + 4 * kg);
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], b, acc, 0, 0, 0);
     }
-    if (x < nx) {
-      const long v = (long)x + (long)nx * (y + (long)ny * z);
+    if (v < nvox) {
       unsigned short o[4];
       uint2 mk = make_uint2(0, 0);
       if (mode == 2) mk = *reinterpret_cast<const uint2 *>(mask + v * 16 + 4 * kg);
@@ -266,54 +268,82 @@ __global__ void extrapolate_add_kernel(const double *__restrict__ Tnm1, const do
 }
 
 // statistics of a sample: sums of (Tn - Tnm1)^2, Tn, Tn^2, z^2
-__global__ __launch_bounds__(256) void stats_partial_kernel(const double *__restrict__ Tnm1, const double *__restrict__ Tn,
-                                                            const double *__restrict__ z, long n,
-                                                            double *__restrict__ part) {
+// Input statistics in one pass: every workgroup stores its four partial sums, and the last
+// workgroup to finish (counted by an atomic counter, which it resets) adds the partials in a
+// fixed order: each thread sums the partials t, t + 256, ... (all loads in flight together),
+// then a fixed tree over the threads. Outputs: s = rms(Tn - Tnm1) (+ tiny), mean(Tn),
+// rms(Tn - mean) (+ tiny), sz = rms(z) (+ tiny) or s without z.
+constexpr int kStatsBlocks = 1024;
+__global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ Tnm1, const double *__restrict__ Tn,
+                                                    const double *__restrict__ z, long n, double *__restrict__ part,
+                                                    unsigned *__restrict__ counter, double *__restrict__ stats) {
   __shared__ double sh[4][256];
-  double a = 0, b = 0, c = 0, e = 0;
-  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-    const double d = Tn[i] - Tnm1[i];
-    a += d * d;
-    b += Tn[i];
-    c += Tn[i] * Tn[i];
-    if (z) e += z[i] * z[i];
-  }
-  sh[0][threadIdx.x] = a;
-  sh[1][threadIdx.x] = b;
-  sh[2][threadIdx.x] = c;
-  sh[3][threadIdx.x] = e;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if (threadIdx.x < k)
-      for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + k];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0)
-    for (int q = 0; q < 4; ++q) part[blockIdx.x * 4 + q] = sh[q][0];
-}
-// s = rms(Tn - Tnm1) (+ tiny), mean(Tn), rms(Tn - mean) (+ tiny), sz = rms(z) (+ tiny) or s without z
-// one workgroup: thread t sums the partials t, t + 256, ..., then a fixed tree over the threads
-__global__ __launch_bounds__(256) void stats_final_kernel(const double *__restrict__ part, int nb, long n, int has_z,
-                                                          double *__restrict__ stats) {
-  __shared__ double red[4][256];
+  __shared__ bool is_last;
   const int t = threadIdx.x;
   double v[4] = {0, 0, 0, 0};
-  for (int i = t; i < nb; i += 256)
-    for (int j = 0; j < 4; ++j) v[j] += part[4 * i + j];
-  for (int j = 0; j < 4; ++j) red[j][t] = v[j];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w)
-      for (int j = 0; j < 4; ++j) red[j][t] += red[j][t + w];
-    __syncthreads();
+  for (long i = (long)blockIdx.x * 256 + t; i < n; i += (long)gridDim.x * 256) {
+    const double d = Tn[i] - Tnm1[i];
+    v[0] += d * d;
+    v[1] += Tn[i];
+    v[2] += Tn[i] * Tn[i];
+    if (z) v[3] += z[i] * z[i];
   }
+  auto tree = [&]() {
+    for (int j = 0; j < 4; ++j) sh[j][t] = v[j];
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+      if (t < k)
+        for (int j = 0; j < 4; ++j) sh[j][t] += sh[j][t + k];
+      __syncthreads();
+    }
+  };
+  tree();
+  if (t == 0) {
+    for (int j = 0; j < 4; ++j) part[j * kStatsBlocks + blockIdx.x] = sh[j][0];
+    __threadfence();
+    is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  constexpr int PER = kStatsBlocks / 256;
+  double w[4][PER];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      const int blk = t + 256 * r;
+      w[j][r] = blk < (int)gridDim.x ? __builtin_nontemporal_load(part + j * kStatsBlocks + blk) : 0.0;
+    }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    v[j] = 0.0;
+#pragma unroll
+    for (int r = 0; r < PER; ++r) v[j] += w[j][r];
+  }
+  __syncthreads();
+  tree();
   if (t != 0) return;
-  const double a = red[0][0], b = red[1][0], c = red[2][0], e = red[3][0];
+  *counter = 0u;
+  const int has_z = z != nullptr;
+  const double a = sh[0][0], b = sh[1][0], c = sh[2][0], e = sh[3][0];
   const double mean = b / n;
   stats[0] = sqrt(a / n) + 1e-30;
   stats[1] = mean;
   stats[2] = sqrt(fmax(c / n - mean * mean, 0.0)) + 1e-12 + 1e-6 * fabs(mean);
   stats[3] = has_z ? sqrt(e / n) + 1e-30 : stats[0];
+}
+
+// f <- (sx[ix] sy[iy] sz[iz]) f on an n0 x n1 x n2 grid, for up to four fields: the row scaling
+// that turns three passes of P^T into full weighting
+__global__ __launch_bounds__(256) void scale3_kernel(CnnFields F, int n0, int n1, int n2, const double *__restrict__ sx,
+                                                     const double *__restrict__ sy, const double *__restrict__ sz) {
+  const long n = (long)n0 * n1 * n2;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const int ix = (int)(i % n0), iy = (int)((i / n0) % n1), iz = (int)(i / ((long)n0 * n1));
+    const double w = (sx[ix] * sy[iy]) * sz[iz];
+    for (int f = 0; f < F.nf; ++f) F.f[f][i] *= w;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -325,7 +355,7 @@ static inline int cnn_grid(long ntiles) {
 
 int cnn_conv(const unsigned short *in, unsigned short *out, const unsigned short *W, const float *bias, const int *dims,
              int mode, const unsigned short *mask, hipStream_t s) {
-  const long ntiles = (long)((dims[0] + 15) / 16) * dims[1] * dims[2];
+  const long ntiles = ((long)dims[0] * dims[1] * dims[2] + 15) / 16;
   hipLaunchKernelGGL(conv3_kernel, dim3(cnn_grid(ntiles)), dim3(256), 0, s, in, out, W, bias, dims[0], dims[1],
                      dims[2], mode, mask);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
@@ -353,11 +383,19 @@ int cnn_adam(float *w, float *m, float *v, const float *dW, const float *db, flo
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
-int cnn_stats(const double *Tnm1, const double *Tn, const double *z, long n, double *part, double *stats,
-              hipStream_t s) {
-  const int nb = (int)std::min<long>(1024, (n + 255) / 256);
-  hipLaunchKernelGGL(stats_partial_kernel, dim3(nb), dim3(256), 0, s, Tnm1, Tn, z, n, part);
-  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(256), 0, s, part, nb, n, z != nullptr ? 1 : 0, stats);
+int cnn_stats(const double *Tnm1, const double *Tn, const double *z, long n, double *part, unsigned *counter,
+              double *stats, hipStream_t s) {
+  const int nb = (int)std::min<long>(kStatsBlocks, (n + 255) / 256);
+  hipLaunchKernelGGL(stats_kernel, dim3(nb), dim3(256), 0, s, Tnm1, Tn, z, n, part, counter, stats);
+  return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
+}
+
+int cnn_scale3(const CnnFields &F, const int *dims, const double *sx, const double *sy, const double *sz,
+               hipStream_t s) {
+  if (F.nf < 1 || F.nf > 4) return MPDG_ERR_INVALID;
+  const long n = (long)dims[0] * dims[1] * dims[2];
+  hipLaunchKernelGGL(scale3_kernel, dim3((int)std::min<long>(8192, (n + 255) / 256)), dim3(256), 0, s, F, dims[0],
+                     dims[1], dims[2], sx, sy, sz);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 

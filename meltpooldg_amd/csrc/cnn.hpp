@@ -13,8 +13,17 @@ int cnn_wgrad(const unsigned short *g, const unsigned short *in, const int *dims
 int cnn_pack(const float *master, int cin, int cout, unsigned short *W, unsigned short *Wt, hipStream_t s);
 int cnn_adam(float *w, float *m, float *v, const float *dW, const float *db, float *b, float *mb, float *vb, int cin,
              int cout, float lr, float b1, float b2, float eps, int step, float gscale, hipStream_t s);
-int cnn_stats(const double *Tnm1, const double *Tn, const double *z, long n, double *part, double *stats,
-              hipStream_t s);
+// up to four fields on one grid
+struct CnnFields {
+  double *f[4];
+  int nf;
+};
+// f *= sx[ix] sy[iy] sz[iz] for every field (dims = grid sizes along x, y, z)
+int cnn_scale3(const CnnFields &F, const int *dims, const double *sx, const double *sy, const double *sz,
+               hipStream_t s);
+// input statistics (part: 4 x 1024 partial sums; counter: zero before the first call, left at zero)
+int cnn_stats(const double *Tnm1, const double *Tn, const double *z, long n, double *part, unsigned *counter,
+              double *stats, hipStream_t s);
 int cnn_features(const double *Tnm1, const double *Tn, const double *q, const double *z, long n,
                  const double *stats, double qscale, unsigned short *x, hipStream_t s);
 // base: the field the network corrects (nullptr: the linear extrapolation 2 Tn - Tnm1)

@@ -1102,6 +1102,11 @@ struct mpdg_heat_operation {
   double *base_record = nullptr;     // caller's vector for the base guess of each step
   DeviceArray<double> zbuf, invd;    // residual input and inverse diagonal (learned predictor)
   DeviceArray<double> tbase;         // base guess the network corrected (the fallback)
+  DeviceArray<double> rbase;         // its residual b - J T_b (the solve's start when it is kept)
+  double *scal_host = nullptr;       // pinned copy of the guess check's two norms, read after the solve
+  ~mpdg_heat_operation() {
+    if (scal_host) (void)hipHostFree(scal_host);
+  }
   bool learned_applied = false;      // this step's guess came from the network
   double invd_dt = 0.0;              // time step the cached inverse diagonal was computed for
   DeviceArray<double> kq_step;       // k(T_n) per quadrature point, frozen for a linear step
@@ -1522,7 +1527,7 @@ int mpdg_heat_operation_set_initial(mpdg_heat_operation *h, const double *T0, vo
 // z = D^-1 (b - J T) with b given (linear step), or z = D^-1 r_Newton(T, T_old) (b == nullptr).
 // rr (device scalar, may be nullptr): |b - J T|^2 before the scaling.
 static int guess_feature(mpdg_heat_operation *h, const double *b, const double *T, const double *T_old, double *z,
-                         double *rr, hipStream_t s) {
+                         double *rr, hipStream_t s, double *raw = nullptr) {
   mpdg_heat_op *op = h->op;
   const long n = op->n_dofs();
   // the inverse diagonal is kept while dt is unchanged and the conductivity does not follow T
@@ -1531,6 +1536,11 @@ static int guess_feature(mpdg_heat_operation *h, const double *b, const double *
     MPDG_TRY(h->invd.resize(n));
     MPDG_TRY(op->inverse_diagonal(h->invd.get(), s));
     h->invd_dt = t_dependent ? 0.0 : op->dt;
+  }
+  if (b && rr) {
+    // residual, its norm and the diagonal scaling in one pass
+    MPDG_TRY(op->vmult(z, T, s));
+    return launch_residual_norm(z, b, h->invd.get(), n, h->red.red(), rr, s, raw);
   }
   if (b) {
     MPDG_TRY(op->vmult(z, T, s));
@@ -1567,7 +1577,8 @@ static int learned_guess(mpdg_heat_operation *h, double dt, const double *b, dou
   }
   if (h->base_record) MPDG_TRY(launch_convert<double, double>(T, h->base_record, n, s));
   const bool check = learned && b && h->prm.guess_check;
-  MPDG_TRY(guess_feature(h, b, T, T_old, z, check ? h->scal.get() + 1 : nullptr, s));
+  if (check) MPDG_TRY(h->rbase.resize(n));
+  MPDG_TRY(guess_feature(h, b, T, T_old, z, check ? h->scal.get() + 1 : nullptr, s, check ? h->rbase.get() : nullptr));
   // the network extrapolates with equal steps (as in its training trajectories)
   const bool equal_steps = h->dt_old > 0.0 && std::fabs(dt - h->dt_old) <= 1e-12 * dt;
   if (learned && equal_steps) {
@@ -1647,31 +1658,24 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
     // guess is first smoothed (the correction is prolonged from its working grid and carries
     // high-frequency error, which the residual and CG weigh most), then checked.
     MPDG_TRY(op->vmult(h->alt.get(), T, s));
-    MPDG_TRY(launch_axpby(h->alt.get(), 1.0, h->bvec.get(), -1.0, n, s));
     if (h->prm.guess_smoothing > 0 && h->pc) {
+      MPDG_TRY(launch_axpby(h->alt.get(), 1.0, h->bvec.get(), -1.0, n, s));
       MPDG_TRY(h->pc->smooth(h->du.get(), h->alt.get(), h->prm.guess_smoothing, s));
       MPDG_TRY(launch_axpby(T, 1.0, h->du.get(), 1.0, n, s));
       MPDG_TRY(op->vmult(h->alt.get(), T, s));
-      MPDG_TRY(launch_axpby(h->alt.get(), 1.0, h->bvec.get(), -1.0, n, s));
     }
-    MPDG_TRY(launch_dot<double, double>(h->alt.get(), h->alt.get(), n, h->red.red(), h->scal.get(), s));
-    double rr[2];
-    if (hipMemcpyAsync(rr, h->scal.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
+    // alt = b - J T and its squared norm; on the device, the guess and its residual or the base
+    // guess and its residual (kept from the network's input) become the solve's start, so the
+    // host does not wait for the decision
+    MPDG_TRY(launch_residual_norm(h->alt.get(), h->bvec.get(), nullptr, n, h->red.red(), h->scal.get(), s));
+    MPDG_TRY(launch_guess_select(h->scal.get(), T, h->tbase.get(), h->alt.get(), h->rbase.get(), n, s));
+    if (!h->scal_host && hipHostMalloc((void **)&h->scal_host, 2 * sizeof(double), 0) != hipSuccess) {
+      h->scal_host = nullptr;
+      return MPDG_ERR_NOMEM;
+    }
+    if (hipMemcpyAsync(h->scal_host, h->scal.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
       return MPDG_ERR_DEVICE;
-    if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
-    if (!(rr[0] == rr[0]) || !(rr[0] < rr[1])) {
-      MPDG_TRY(launch_convert<double, double>(h->tbase.get(), T, n, s));
-      h->fell_back = 1;
-      // a run of rejections pauses the network (its evaluation and check cost about three
-      // operator applies and a smoothing per step for nothing), then it is tried again
-      if (h->prm.guess_pause_after > 0 && ++h->guess_rejections >= h->prm.guess_pause_after) {
-        h->guess_paused = h->prm.guess_pause_steps;
-        h->guess_rejections = 0;
-      }
-    } else {
-      r_init = h->alt.get();  // b - J T of the accepted guess: the solve starts from it
-      h->guess_rejections = 0;
-    }
+    r_init = h->alt.get();
   }
   mpdg_solver_params lp = h->prm.linear;
   lp.criterion = MPDG_CRIT_RHS;
@@ -1683,6 +1687,21 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
   if (hipMemcpyAsync(&st, h->work.st.get(), sizeof(CGState), hipMemcpyDeviceToHost, s) != hipSuccess)
     return MPDG_ERR_DEVICE;
   if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
+  if (r_init) {
+    // the check's outcome (copied before the solve): a run of rejections pauses the network (its
+    // evaluation and check cost about three operator applies and a smoothing per step for
+    // nothing), then it is tried again
+    const double g = h->scal_host[0], base = h->scal_host[1];
+    if (!(g == g) || !(g < base)) {
+      h->fell_back = 1;
+      if (h->prm.guess_pause_after > 0 && ++h->guess_rejections >= h->prm.guess_pause_after) {
+        h->guess_paused = h->prm.guess_pause_steps;
+        h->guess_rejections = 0;
+      }
+    } else {
+      h->guess_rejections = 0;
+    }
+  }
   h->rel_res0 = std::sqrt(st.rr0 / st.bb);
   h->rel_res = std::sqrt(st.rr / st.bb);
   commit_time_increment(h, dt);

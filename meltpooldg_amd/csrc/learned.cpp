@@ -70,6 +70,7 @@ struct mpdg_learned {
   DeviceArray<unsigned short> W[kLayers], Wt[kLayers], act[kLayers + 1], g0, g1;
   DeviceArray<float> dW[kLayers], db[kLayers], part;
   DeviceArray<double> stats, spart, lpart;
+  DeviceArray<unsigned> scount;  // workgroups done in the statistics pass (back at zero after it)
   // rho > 1: 1D full weighting (coarse <- fine) and prolongation (fine <- coarse) per direction,
   // the fields on the network grid and the intermediate arrays of the direction passes
   DeviceArray<int> wcol[3], pcol[3];
@@ -77,6 +78,12 @@ struct mpdg_learned {
   int ww[3] = {0, 0, 0}, pw[3] = {0, 0, 0};
   DeviceArray<double> cf[6], tmpa, tmpb, corr, corr_f;
   long tmpa_n = 0, tmpb_n = 0;  // per field: the transfers carry up to kEllMaxFields fields at once
+  // rho = 2 (and x lines of at most 1024 nodes): the structured 2:1 transfers of the multigrid
+  // (one coarse cell's table, no index arrays), the full weighting as three passes of P^T and
+  // the inverse row sums per direction
+  bool hstruct = false;
+  std::vector<double> htab;
+  DeviceArray<double> htab_dev, isum[3];
 
   int init(const int *cells, int k, int coarsening, double qs, unsigned long long seed) {
     degree = k;
@@ -117,6 +124,8 @@ struct mpdg_learned {
     MPDG_TRY(part.resize((size_t)kWgradBlocks * 4 * (27 * 256 + 16)));  // one slot per wavefront
     MPDG_TRY(stats.resize(4));
     MPDG_TRY(spart.resize(4 * 1024));
+    MPDG_TRY(scount.resize(1));
+    if (hipMemset(scount.get(), 0, sizeof(unsigned)) != hipSuccess) return MPDG_ERR_DEVICE;
     MPDG_TRY(lpart.resize(1024));
     MPDG_TRY(corr.resize(n));
     if (rho > 1) {
@@ -140,6 +149,29 @@ struct mpdg_learned {
         hipMemcpy(pcol[d].get(), t.pcol.data(), t.pcol.size() * sizeof(int), hipMemcpyHostToDevice);
         hipMemcpy(pval[d].get(), t.pval.data(), t.pval.size() * sizeof(double), hipMemcpyHostToDevice);
       }
+      if (rho == 2 && fdims[0] <= 1024) {
+        hstruct = true;
+        Transfer1DHost cell(k, 1, k, 2, false, false);  // one coarse cell: rows m = 0..2k, cols j = 0..k
+        htab.assign((size_t)(2 * k + 1) * (k + 1), 0.0);
+        for (int m = 0; m <= 2 * k; ++m)
+          for (int e = 0; e < cell.pw; ++e) {
+            const int j = cell.pcol[(size_t)m * cell.pw + e];
+            if (j >= 0) htab[(size_t)m * (k + 1) + j] = cell.pval[(size_t)m * cell.pw + e];
+          }
+        MPDG_TRY(htab_dev.resize(htab.size()));
+        hipMemcpy(htab_dev.get(), htab.data(), htab.size() * sizeof(double), hipMemcpyHostToDevice);
+        for (int d = 0; d < 3; ++d) {
+          Transfer1DHost t(k, cells[d] / rho, k, cells[d], false, false);
+          std::vector<double> inv(t.nc);
+          for (int c = 0; c < t.nc; ++c) {
+            double sum = 0.0;
+            for (int e = 0; e < t.rw; ++e) sum += t.rval[(size_t)c * t.rw + e];
+            inv[c] = 1.0 / sum;
+          }
+          MPDG_TRY(isum[d].resize(inv.size()));
+          hipMemcpy(isum[d].get(), inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice);
+        }
+      }
       for (auto &a : cf) MPDG_TRY(a.resize(n));
       const long a1 = std::max((long)dims[0] * fdims[1] * fdims[2], (long)fdims[0] * fdims[1] * dims[2]);
       const long a2 = std::max((long)dims[0] * dims[1] * fdims[2], (long)fdims[0] * dims[1] * dims[2]);
@@ -159,6 +191,26 @@ struct mpdg_learned {
   // ... (the fields themselves when rho = 1; nullptr stays nullptr): the three 1D full weightings
   // each carry all the fields in one launch
   int to_net(const double *const *f, int slot0, int count, const double **net, hipStream_t s) {
+    if (hstruct) {
+      // per field three passes of P^T (x, y, z), then the row scaling of all of them at once
+      const int o0[3] = {dims[0], fdims[1], fdims[2]}, o1[3] = {dims[0], dims[1], fdims[2]};
+      CnnFields C{};
+      for (int i = 0; i < count; ++i) {
+        net[i] = f[i];
+        if (!f[i]) continue;
+        double *out = cf[slot0 + i].get();
+        MPDG_TRY(launch_htransfer<double>(true, 0, f[i], nullptr, tmpa.get(), fdims, o0, degree, htab.data(),
+                                          htab_dev.get(), 0, 0, false, s));
+        MPDG_TRY(launch_htransfer<double>(true, 1, tmpa.get(), nullptr, tmpb.get(), o0, o1, degree, htab.data(),
+                                          htab_dev.get(), 0, 0, false, s));
+        MPDG_TRY(launch_htransfer<double>(true, 2, tmpb.get(), nullptr, out, o1, dims, degree, htab.data(),
+                                          htab_dev.get(), 0, 0, false, s));
+        C.f[C.nf++] = out;
+        net[i] = out;
+      }
+      if (C.nf == 0) return MPDG_OK;
+      return cnn_scale3(C, dims, isum[0].get(), isum[1].get(), isum[2].get(), s);
+    }
     EllFields<double> F[3] = {};
     for (int i = 0; i < count; ++i) {
       net[i] = f[i];
@@ -185,7 +237,7 @@ struct mpdg_learned {
     const double *a = net[0], *t = net[1], *qq = net[2], *zz = net[3];
     last_Tnm1 = a;
     last_Tn = t;
-    MPDG_TRY(cnn_stats(a, t, zz, n, spart.get(), stats.get(), s));
+    MPDG_TRY(cnn_stats(a, t, zz, n, spart.get(), scount.get(), stats.get(), s));
     MPDG_TRY(cnn_features(a, t, qq, zz, n, stats.get(), qscale, act[0].get(), s));
     for (int l = 0; l < kLayers; ++l)
       MPDG_TRY(cnn_conv(act[l].get(), act[l + 1].get(), W[l].get(), b[l].get(), dims, l == kLayers - 1 ? 0 : 1,
@@ -244,7 +296,16 @@ struct mpdg_learned {
   int predict(const double *Tnm1, const double *Tn, const double *base, double *out, hipStream_t s) {
     MPDG_TRY(cnn_correction(act[kLayers].get(), n, stats.get(), oscale, corr.get(), s));
     const double *c = corr.get();
-    if (rho > 1) {
+    if (hstruct) {
+      const int o0[3] = {fdims[0], dims[1], dims[2]}, o1[3] = {fdims[0], fdims[1], dims[2]};
+      MPDG_TRY(launch_htransfer<double>(false, 0, corr.get(), nullptr, tmpb.get(), dims, o0, degree, htab.data(),
+                                        htab_dev.get(), 0, 0, false, s));
+      MPDG_TRY(launch_htransfer<double>(false, 1, tmpb.get(), nullptr, tmpa.get(), o0, o1, degree, htab.data(),
+                                        htab_dev.get(), 0, 0, false, s));
+      MPDG_TRY(launch_htransfer<double>(false, 2, tmpa.get(), nullptr, corr_f.get(), o1, fdims, degree, htab.data(),
+                                        htab_dev.get(), 0, 0, false, s));
+      c = corr_f.get();
+    } else if (rho > 1) {
       const int o0[3] = {fdims[0], dims[1], dims[2]}, o1[3] = {fdims[0], fdims[1], dims[2]};
       MPDG_TRY(launch_ell_dir<double>(corr.get(), tmpb.get(), o0, dims, 0, pw[0], pcol[0].get(), pval[0].get(), s));
       MPDG_TRY(launch_ell_dir<double>(tmpb.get(), tmpa.get(), o1, o0, 1, pw[1], pcol[1].get(), pval[1].get(), s));
