@@ -204,12 +204,23 @@ __device__ __forceinline__ bool reduce_to_last_block(double (&v)[NQ], double *pa
   __syncthreads();
   if (!is_last) return false;
   __threadfence();
+  // thread t adds the partials t, t + 256, ... in that order; all its loads are issued first
+  constexpr int PER = kRedBlocks / 256;
+  double part[NQ][PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int b = threadIdx.x + 256 * r;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      part[q][r] = b < (int)gridDim.x ? __builtin_nontemporal_load(partials + q * kRedBlocks + b) : 0.0;
+  }
   double w[NQ];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) w[q] = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += 256)
+  for (int q = 0; q < NQ; ++q) {
+    w[q] = 0.0;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) w[q] += __builtin_nontemporal_load(partials + q * kRedBlocks + b);
+    for (int r = 0; r < PER; ++r) w[q] += part[q][r];
+  }
   // fixed-order tree over the 256 thread sums (deterministic for a fixed grid)
   block_reduce<NQ>(w, sh);
 #pragma unroll

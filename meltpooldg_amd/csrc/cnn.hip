@@ -28,15 +28,8 @@ __device__ __forceinline__ unsigned short f32_to_bf16(float f) {
 }
 __device__ __forceinline__ float bf16_to_f32(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
 
-__device__ __forceinline__ bf16x4 load4(const unsigned short *p) {
-  const uint2 v = *reinterpret_cast<const uint2 *>(p);
-  bf16x4 r;
-  r[0] = (short)(v.x & 0xffff);
-  r[1] = (short)(v.x >> 16);
-  r[2] = (short)(v.y & 0xffff);
-  r[3] = (short)(v.y >> 16);
-  return r;
-}
+// four consecutive bf16 values (8-byte aligned) as the MFMA operand, in one 8-byte load
+__device__ __forceinline__ bf16x4 load4(const unsigned short *p) { return *reinterpret_cast<const bf16x4 *>(p); }
 
 // ---------------------------------------------------------------------------
 // forward / data-gradient convolution: out[v][co] = act(bias[co] + sum_tap,ci W[tap][co][ci] in[v+off(tap)][ci])
@@ -61,18 +54,24 @@ __global__ __launch_bounds__(256) void conv3_kernel(const unsigned short *__rest
   // row: every lane finds its own neighbours), so no lanes idle at the ends of short x rows
   const long nvox = (long)nx * ny * nz;
   const long ntiles = (nvox + 15) >> 4;
+  const int sy = 16 * nx, sz = 16 * nx * ny;  // element strides of y and z neighbours (grids < 2^26 voxels)
   for (long tile = wave; tile < ntiles; tile += nwaves) {
     const long v = tile * 16 + n;
     const long vc = v < nvox ? v : nvox - 1;
     const int x = (int)(vc % nx), y = (int)((vc / nx) % ny), z = (int)(vc / ((long)nx * ny));
+    // which neighbours exist; a missing one is read at the voxel itself and replaced by zero,
+    // so the 27 loads carry no branches and only one address add each
+    const bool xl = x > 0, xh = x < nx - 1, yl = y > 0, yh = y < ny - 1, zl = z > 0, zh = z < nz - 1;
+    const unsigned short *c = in + vc * 16 + 4 * kg;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int t = 0; t < 27; ++t) {
       const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
-      const int xs = x + dx, ys = y + dy, zs = z + dz;
-      bf16x4 b = {0, 0, 0, 0};
-      if (xs >= 0 && xs < nx && ys >= 0 && ys < ny && zs >= 0 && zs < nz)
-        b = load4(in + ((long)xs + (long)nx * (ys + (long)ny * zs)) * 16 + 4 * kg);
+      const bool ok = (dx < 0 ? xl : (dx > 0 ? xh : true)) && (dy < 0 ? yl : (dy > 0 ? yh : true)) &&
+                      (dz < 0 ? zl : (dz > 0 ? zh : true));
+      const int off = 16 * dx + sy * dy + sz * dz;
+      bf16x4 b = load4(c + (ok ? off : 0));
+      if (!ok) b = bf16x4{0, 0, 0, 0};
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], b, acc, 0, 0, 0);
     }
     if (v < nvox) {
@@ -356,6 +355,7 @@ static inline int cnn_grid(long ntiles) {
 int cnn_conv(const unsigned short *in, unsigned short *out, const unsigned short *W, const float *bias, const int *dims,
              int mode, const unsigned short *mask, hipStream_t s) {
   const long ntiles = ((long)dims[0] * dims[1] * dims[2] + 15) / 16;
+  if ((long)dims[0] * dims[1] * dims[2] >= (1L << 26)) return MPDG_ERR_INVALID;  // 32-bit neighbour offsets
   hipLaunchKernelGGL(conv3_kernel, dim3(cnn_grid(ntiles)), dim3(256), 0, s, in, out, W, bias, dims[0], dims[1],
                      dims[2], mode, mask);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
