@@ -41,8 +41,8 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by the library's streams of known size;
 # the summaries with the commands are under profiles/ (heat_apply_*). Counters need their own
 # profiler runs, so the bench reports the profiled figure for these workloads and null for others.
-PROFILED_TRAFFIC = {(3, 128, "qp", False): 2.506e+09, (4, 96, "qp", False): 2.360e+09,
-                    (4, 96, "law", False): 2.292e+09, (3, 96, "const", True): 3.759e+09}
+PROFILED_TRAFFIC = {(3, 128, "qp", False): 2.506e+09, (4, 96, "qp", False): 2.354e+09,
+                    (4, 96, "law", False): 2.308e+09, (3, 96, "const", True): 3.759e+09}
 # learned predictor: after PAUSE[0] consecutive steps whose learned guess loses the guess check, the
 # network is not evaluated for PAUSE[1] steps, then tried again
 PAUSE = (2, 8)
