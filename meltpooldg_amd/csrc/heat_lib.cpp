@@ -983,7 +983,7 @@ int gmres_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &pr
 // with r also written in FP32 (the preconditioner's input) and z read in FP32 (its output)
 int fcg32_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &prm, const double *b, double *x,
                 KrylovWork &w, int *iterations, double *res, float *r32, const float *z32, const double *r_init,
-                hipStream_t s) {
+                hipStream_t s, int first_batch) {
   const long n = op->n_dofs();
   CGState h{};
   h.max_it = prm.max_iterations;
@@ -995,9 +995,9 @@ int fcg32_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &pr
   MPDG_TRY(pc->vmult_f32_in_place(s));
   MPDG_TRY(fcg32_start(w.r.get(), z32, w.p.get(), n, w.red.red(), w.st.get(), s));
   const int batch = n > 2000000 ? 1 : 8;
-  int it = 0;
+  int it = 0, k_batch = std::max(batch, first_batch);
   while (true) {
-    for (int k = 0; k < batch; ++k) {
+    for (int k = 0; k < k_batch; ++k) {
       MPDG_TRY(op->vmult(w.q.get(), w.p.get(), s));
       MPDG_TRY(cg_alpha(w.p.get(), w.q.get(), n, w.red.red(), w.st.get(), s));
       MPDG_TRY(fcg32_update(x, w.r.get(), w.p.get(), w.q.get(), z32, r32, n, w.red.red(), w.st.get(), s));
@@ -1005,7 +1005,8 @@ int fcg32_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &pr
       MPDG_TRY(fcg32_beta(w.r.get(), z32, n, w.red.red(), w.st.get(), s));
       MPDG_TRY(fcg32_direction(w.p.get(), z32, n, w.st.get(), s));
     }
-    it += batch;
+    it += k_batch;
+    k_batch = batch;
     if (hipMemcpyAsync(&h, w.st.get(), sizeof(CGState), hipMemcpyDeviceToHost, s) != hipSuccess) return MPDG_ERR_DEVICE;
     if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
     if (h.done || it > prm.max_iterations + batch) break;
@@ -1018,8 +1019,12 @@ int fcg32_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &pr
 }
 
 // r_init (may be nullptr): b - A x, already computed by the caller (CG / FCG only)
+// first_batch: iterations enqueued before the first host check (at least the usual batch); a
+// caller that expects the solve to take about as many iterations as its previous one passes one
+// fewer than those, which saves that many host round trips
 int krylov_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &prm, const double *b, double *x,
-                 KrylovWork &w, int *iterations, double *res, hipStream_t s, const double *r_init = nullptr) {
+                 KrylovWork &w, int *iterations, double *res, hipStream_t s, const double *r_init = nullptr,
+                 int first_batch = 0) {
   const long n = op->n_dofs();
   MPDG_TRY(w.init(n));
   if (prm.solver_type == MPDG_SOLVER_GMRES) {
@@ -1036,7 +1041,7 @@ int krylov_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &p
   // FP32 multigrid: its input and output are used in place, in FP32
   float *r32 = external ? pc->f32_input() : nullptr;
   const float *z32 = external ? pc->f32_output() : nullptr;
-  if (r32 && z32) return fcg32_solve(op, pc, prm, b, x, w, iterations, res, r32, z32, r_init, s);
+  if (r32 && z32) return fcg32_solve(op, pc, prm, b, x, w, iterations, res, r32, z32, r_init, s, first_batch);
   // r = b - A x
   if (!r_init) MPDG_TRY(op->vmult(w.q.get(), x, s));
   MPDG_TRY(cg_init(b, w.r.get(), r_init ? r_init : w.q.get(), w.z.get(), w.p.get(), inv, jacobi, n, w.red.red(),
@@ -1050,9 +1055,9 @@ int krylov_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &p
   // large systems: one iteration per host check (an iteration far outweighs the check; a
   // batch would run whole iterations after convergence); small ones: batches hide launch latency
   const int batch = n > 2000000 ? 1 : 8;
-  int it = 0;
+  int it = 0, k_batch = std::max(batch, first_batch);
   while (true) {
-    for (int k = 0; k < batch; ++k) {
+    for (int k = 0; k < k_batch; ++k) {
       MPDG_TRY(op->vmult(w.q.get(), w.p.get(), s));
       MPDG_TRY(cg_alpha(w.p.get(), w.q.get(), n, w.red.red(), w.st.get(), s));
       MPDG_TRY(cg_update(x, w.r.get(), w.p.get(), w.q.get(), w.z.get(), inv, jacobi, n, w.red.red(), w.st.get(), s));
@@ -1062,7 +1067,8 @@ int krylov_solve(mpdg_heat_op *op, mpdg_precond *pc, const mpdg_solver_params &p
       }
       MPDG_TRY(cg_direction(w.p.get(), w.z.get(), n, w.st.get(), s));
     }
-    it += batch;
+    it += k_batch;
+    k_batch = batch;
     if (hipMemcpyAsync(&h, w.st.get(), sizeof(CGState), hipMemcpyDeviceToHost, s) != hipSuccess) return MPDG_ERR_DEVICE;
     if (hipStreamSynchronize(s) != hipSuccess) return MPDG_ERR_DEVICE;
     if (h.done || it > prm.max_iterations + batch) break;
@@ -1116,6 +1122,7 @@ struct mpdg_heat_operation {
   int guess_rejections = 0;     // consecutive steps whose learned guess the check rejected
   int guess_paused = 0;         // steps left on which the network is not evaluated
   int steps_since_update = 0;  // linear steps since the preconditioner was rebuilt
+  int last_lin_iters = 0;       // iterations of the previous linear step's solve
   int n_calls = 0;             // predictor calls since set_initial (Predictor::n_calls, predictor.hpp:97)
   double rel_res0 = 0.0, rel_res = 0.0;
 };
@@ -1520,6 +1527,7 @@ int mpdg_heat_operation_set_initial(mpdg_heat_operation *h, const double *T0, vo
   h->n_calls = 0;
   h->guess_rejections = 0;
   h->guess_paused = 0;
+  h->last_lin_iters = 0;
   return hipStreamSynchronize(S(stream)) == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 
@@ -1669,10 +1677,7 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
     // host does not wait for the decision
     MPDG_TRY(launch_residual_norm(h->alt.get(), h->bvec.get(), nullptr, n, h->red.red(), h->scal.get(), s));
     MPDG_TRY(launch_guess_select(h->scal.get(), T, h->tbase.get(), h->alt.get(), h->rbase.get(), n, s));
-    if (!h->scal_host && hipHostMalloc((void **)&h->scal_host, 2 * sizeof(double), 0) != hipSuccess) {
-      h->scal_host = nullptr;
-      return MPDG_ERR_NOMEM;
-    }
+    if (!h->scal_host) return MPDG_ERR_INVALID;  // allocated with the network (set_learned)
     if (hipMemcpyAsync(h->scal_host, h->scal.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
       return MPDG_ERR_DEVICE;
     r_init = h->alt.get();
@@ -1681,8 +1686,11 @@ static int linear_step(mpdg_heat_operation *h, double dt, double *T, const doubl
   lp.criterion = MPDG_CRIT_RHS;
   int li = 0;
   double res = 0.0;
+  // consecutive linear steps take similar iteration counts: all but one of the previous step's
+  // iterations are enqueued before the first host check
   const int rc = krylov_solve(op, h->pc, lp, h->bvec.get(), T, h->work, &li, &res, s,
-                              lp.solver_type == MPDG_SOLVER_GMRES ? nullptr : r_init);
+                              lp.solver_type == MPDG_SOLVER_GMRES ? nullptr : r_init, h->last_lin_iters - 1);
+  h->last_lin_iters = li;
   CGState st{};
   if (hipMemcpyAsync(&st, h->work.st.get(), sizeof(CGState), hipMemcpyDeviceToHost, s) != hipSuccess)
     return MPDG_ERR_DEVICE;
@@ -1977,6 +1985,11 @@ int mpdg_heat_operation_set_learned(mpdg_heat_operation *h, mpdg_learned *net, c
     MPDG_TRY(mpdg_learned_grid(net, fd, nullptr));
     for (int d = 0; d < 3; ++d)
       if (fd[d] != h->op->d64.nd[d]) return MPDG_ERR_INVALID;  // network made for another DoF grid
+  }
+  // pinned host copy of the guess check's norms, read after each step's solve
+  if (net && !h->scal_host && hipHostMalloc((void **)&h->scal_host, 2 * sizeof(double), 0) != hipSuccess) {
+    h->scal_host = nullptr;
+    return MPDG_ERR_NOMEM;
   }
   h->net = net;
   h->q_field = q_field;
