@@ -982,8 +982,11 @@ template int launch_ell_dir_fields<double>(const EllFields<double> &, const int 
 // y_i = sum_j A[j n + i] x_j in two deterministic stages: segment k of the j range
 // (kGemvSeg segments) gives part[k][i], then the segments are summed in order.
 constexpr int kGemvSeg = 32;
+// x and y in the level precision TX / TY (read and written as doubles and rounded on the way
+// out, the same values a conversion pass before and after would give)
+template <typename TX>
 __global__ __launch_bounds__(256) void dense_gemv_part_kernel(const double *__restrict__ A,
-                                                              const double *__restrict__ x,
+                                                              const TX *__restrict__ x,
                                                               double *__restrict__ part, int n) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int k = blockIdx.y;
@@ -991,23 +994,27 @@ __global__ __launch_bounds__(256) void dense_gemv_part_kernel(const double *__re
   const int len = (n + kGemvSeg - 1) / kGemvSeg;
   const int j0 = k * len, j1 = min(n, j0 + len);
   double acc = 0.0;
-  for (int j = j0; j < j1; ++j) acc += A[(long)j * n + i] * x[j];
+  for (int j = j0; j < j1; ++j) acc += A[(long)j * n + i] * (double)x[j];
   part[(long)k * n + i] = acc;
 }
-__global__ __launch_bounds__(256) void dense_gemv_sum_kernel(const double *__restrict__ part, double *__restrict__ y,
+template <typename TY>
+__global__ __launch_bounds__(256) void dense_gemv_sum_kernel(const double *__restrict__ part, TY *__restrict__ y,
                                                              int n) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   double acc = 0.0;
   for (int k = 0; k < kGemvSeg; ++k) acc += part[(long)k * n + i];
-  y[i] = acc;
+  y[i] = (TY)acc;
 }
 long dense_gemv_scratch(int n) { return (long)kGemvSeg * n; }
-int launch_dense_gemv(const double *A, const double *x, double *y, double *part, int n, hipStream_t s) {
-  hipLaunchKernelGGL(dense_gemv_part_kernel, dim3((n + 255) / 256, kGemvSeg), dim3(256), 0, s, A, x, part, n);
-  hipLaunchKernelGGL(dense_gemv_sum_kernel, dim3((n + 255) / 256), dim3(256), 0, s, part, y, n);
+template <typename T>
+int launch_dense_gemv(const double *A, const T *x, T *y, double *part, int n, hipStream_t s) {
+  hipLaunchKernelGGL(dense_gemv_part_kernel<T>, dim3((n + 255) / 256, kGemvSeg), dim3(256), 0, s, A, x, part, n);
+  hipLaunchKernelGGL(dense_gemv_sum_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, s, part, y, n);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
+template int launch_dense_gemv<double>(const double *, const double *, double *, double *, int, hipStream_t);
+template int launch_dense_gemv<float>(const double *, const float *, float *, double *, int, hipStream_t);
 
 // v = 0 on the DoFs of homogeneous Dirichlet faces
 template <typename T>

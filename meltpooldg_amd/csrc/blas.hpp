@@ -96,9 +96,11 @@ template <typename T>
 int launch_htransfer(bool restrict_mode, int dir, const T *in, const T *in2, T *out, const int *idims,
                      const int *odims, int k, const double *tab_host, const T *tab_dev, int lo, int hi, bool add,
                      hipStream_t s);
-// y = A^T x for a dense column-major n x n matrix (coarse-grid inverse); part: dense_gemv_scratch(n) doubles
+// y = A^T x for a dense column-major n x n matrix (coarse-grid inverse), accumulated in FP64 with
+// x and y in precision T; part: dense_gemv_scratch(n) doubles
 long dense_gemv_scratch(int n);
-int launch_dense_gemv(const double *A, const double *x, double *y, double *part, int n, hipStream_t s);
+template <typename T>
+int launch_dense_gemv(const double *A, const T *x, T *y, double *part, int n, hipStream_t s);
 template <typename T> int launch_zero_constrained(T *v, const int *nd, int mask, hipStream_t s);
 template <typename T> int launch_eig_start(T *v, const int *nd, int mask, hipStream_t s);
 template <typename T>

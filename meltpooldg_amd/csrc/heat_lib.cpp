@@ -350,7 +350,7 @@ template <typename T>
 struct GMGPrecond : mpdg_precond {
   mpdg_gmg_params prm{};
   std::vector<std::unique_ptr<Level<T>>> lev;  // 0 = coarsest
-  DeviceArray<double> coarse_inv, coarse_elem, cvec_in, cvec_out, cvec_part, eig_hist, kcell64, kcell_pp[2];
+  DeviceArray<double> coarse_inv, coarse_elem, cvec_part, eig_hist, kcell64, kcell_pp[2];
   ReductionScratch red;
   int nc_coarse_dofs = 0;
   // update(): the coarse matrix and the levels below the finest run on streams of their own
@@ -497,8 +497,6 @@ struct GMGPrecond : mpdg_precond {
     MPDG_TRY(eig_hist.resize(std::max<size_t>(lev.size(), 2) * hist_stride()));
     nc_coarse_dofs = (int)lev[0]->od.n_dofs;
     MPDG_TRY(coarse_inv.resize((size_t)nc_coarse_dofs * nc_coarse_dofs));
-    MPDG_TRY(cvec_in.resize(nc_coarse_dofs));
-    MPDG_TRY(cvec_out.resize(nc_coarse_dofs));
     MPDG_TRY(cvec_part.resize(dense_gemv_scratch((int)nc_coarse_dofs)));
     MPDG_TRY(kcell64.resize(f.n_cells));
     for (auto &k : kcell_pp) MPDG_TRY(k.resize(std::max<long>(1, f.n_cells / 8)));
@@ -791,9 +789,8 @@ struct GMGPrecond : mpdg_precond {
   int vcycle(int l, hipStream_t s) {
     Level<T> &L = *lev[l];
     if (l == 0) {
-      MPDG_TRY(launch_convert<T, double>(L.b.get(), cvec_in.get(), L.od.n_dofs, s));
-      MPDG_TRY(launch_dense_gemv(coarse_inv.get(), cvec_in.get(), cvec_out.get(), cvec_part.get(), (int)L.od.n_dofs, s));
-      return launch_convert<double, T>(cvec_out.get(), L.x.get(), L.od.n_dofs, s);
+      // the level's b and x directly (FP64 accumulation, one rounding to T on the way out)
+      return launch_dense_gemv<T>(coarse_inv.get(), L.b.get(), L.x.get(), cvec_part.get(), (int)L.od.n_dofs, s);
     }
     MPDG_TRY(smooth(L, true, s));
     MPDG_TRY(level_apply<T>(L.od, L.x.get(), L.Ax.get(), s));
@@ -1585,7 +1582,10 @@ static int learned_guess(mpdg_heat_operation *h, double dt, const double *b, dou
   }
   if (h->base_record) MPDG_TRY(launch_convert<double, double>(T, h->base_record, n, s));
   const bool check = learned && b && h->prm.guess_check;
-  if (check) MPDG_TRY(h->rbase.resize(n));
+  if (check) {  // before the first step the network is applied, so that no step allocates
+    MPDG_TRY(h->rbase.resize(n));
+    MPDG_TRY(h->tbase.resize(n));
+  }
   MPDG_TRY(guess_feature(h, b, T, T_old, z, check ? h->scal.get() + 1 : nullptr, s, check ? h->rbase.get() : nullptr));
   // the network extrapolates with equal steps (as in its training trajectories)
   const bool equal_steps = h->dt_old > 0.0 && std::fabs(dt - h->dt_old) <= 1e-12 * dt;
