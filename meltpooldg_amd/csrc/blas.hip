@@ -810,9 +810,9 @@ __global__ __launch_bounds__(256) void hprolong_kernel(const T *__restrict__ in,
     }
   }
 }
+// the fields of FS in turn (each with the same lines)
 template <typename T, int K>
-__global__ __launch_bounds__(256) void hrestrict_kernel(const T *__restrict__ in, const T *__restrict__ in2,
-                                                        T *__restrict__ out, long nl, long ibstride, long obstride,
+__global__ __launch_bounds__(256) void hrestrict_kernel(EllFields<T> FS, long nl, long ibstride, long obstride,
                                                         long istride, long ostride, int ncc, HTab<T, K> P, int lo,
                                                         int hi) {
   const long t = (long)blockIdx.x * 256 + threadIdx.x;
@@ -820,6 +820,10 @@ __global__ __launch_bounds__(256) void hrestrict_kernel(const T *__restrict__ in
   const long ib = t + ibstride * blockIdx.y, ob = t + obstride * blockIdx.y;
   const int nfine = 2 * K * ncc + 1, ncoarse = K * ncc + 1;
   const int c0 = blockIdx.z * kHSeg, c1 = min(ncc, c0 + kHSeg);
+  for (int fi = 0; fi < FS.nf; ++fi) {
+  const T *__restrict__ in = FS.in[fi];
+  const T *__restrict__ in2 = FS.in2[fi];
+  T *__restrict__ out = FS.out[fi];
   auto F = [&](int g) -> T {
     if ((lo && g == 0) || (hi && g == nfine - 1)) return T(0);
     const long a = ib + istride * g;
@@ -847,11 +851,11 @@ __global__ __launch_bounds__(256) void hrestrict_kernel(const T *__restrict__ in
     for (int m = 1; m < 2 * K; ++m) carry += P.v[m][K] * f[m];
     if (cc == ncc - 1) out[ob + ostride * (ncoarse - 1)] = hi ? T(0) : carry + F(nfine - 1);
   }
+  }
 }
 // x lines: one wavefront per line, the input line in LDS, the table read from global memory
 template <typename T, int K, bool RESTRICT>
-__global__ __launch_bounds__(256) void htransfer_x_kernel(const T *__restrict__ in, const T *__restrict__ in2,
-                                                          T *__restrict__ out, long nlines, int ncc,
+__global__ __launch_bounds__(256) void htransfer_x_kernel(EllFields<T> FS, long nlines, int ncc,
                                                           const T *__restrict__ tab, int lo, int hi, int add) {
   __shared__ T lines[4][kEllLine];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -861,6 +865,15 @@ __global__ __launch_bounds__(256) void htransfer_x_kernel(const T *__restrict__ 
   const int ni = RESTRICT ? nfine : ncoarse, no = RESTRICT ? ncoarse : nfine;
   T *line = lines[wave];
   const long lin = (long)ni * line_id, lout = (long)no * line_id;
+  for (int fi = 0; fi < FS.nf; ++fi) {
+  const T *__restrict__ in = FS.in[fi];
+  const T *__restrict__ in2 = FS.in2[fi];
+  T *__restrict__ out = FS.out[fi];
+  if (fi > 0) {  // every lane is done with the previous field's line
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
   for (int x = lane; x < ni; x += 64) {
     const bool drop = (lo && x == 0) || (hi && x == ni - 1);
     line[x] = drop ? T(0) : (in2 ? in[lin + x] - in2[lin + x] : in[lin + x]);
@@ -896,11 +909,14 @@ __global__ __launch_bounds__(256) void htransfer_x_kernel(const T *__restrict__ 
       out[lout + x] = a;
     }
   }
+  }
 }
 template <typename T, int K>
-static int htransfer_k(bool restrict_mode, int dir, const T *in, const T *in2, T *out, const int *idims,
-                       const int *odims, const double *tab_host, const T *tab_dev, int lo, int hi, bool add,
-                       hipStream_t s) {
+static int htransfer_k(bool restrict_mode, int dir, const EllFields<T> &FS, const int *idims, const int *odims,
+                       const double *tab_host, const T *tab_dev, int lo, int hi, bool add, hipStream_t s) {
+  if (FS.nf < 1 || FS.nf > kEllMaxFields || (!restrict_mode && FS.nf != 1)) return MPDG_ERR_INVALID;
+  const T *in = FS.in[0];
+  T *out = FS.out[0];
   HTab<T, K> P;
   for (int m = 0; m <= 2 * K; ++m)
     for (int j = 0; j <= K; ++j) P.v[m][j] = (T)tab_host[m * (K + 1) + j];
@@ -913,11 +929,10 @@ static int htransfer_k(bool restrict_mode, int dir, const T *in, const T *in2, T
     const long nlines = (long)odims[1] * odims[2];
     const dim3 grid((unsigned)((nlines + 3) / 4));
     if (restrict_mode)
-      hipLaunchKernelGGL((htransfer_x_kernel<T, K, true>), grid, dim3(256), 0, s, in, in2, out, nlines, ncc, tab_dev,
-                         lo, hi, 0);
+      hipLaunchKernelGGL((htransfer_x_kernel<T, K, true>), grid, dim3(256), 0, s, FS, nlines, ncc, tab_dev, lo, hi, 0);
     else
-      hipLaunchKernelGGL((htransfer_x_kernel<T, K, false>), grid, dim3(256), 0, s, in, in2, out, nlines, ncc,
-                         tab_dev, lo, hi, add ? 1 : 0);
+      hipLaunchKernelGGL((htransfer_x_kernel<T, K, false>), grid, dim3(256), 0, s, FS, nlines, ncc, tab_dev, lo, hi,
+                         add ? 1 : 0);
     return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
   }
   long nl, ibs, obs, is, os;
@@ -937,25 +952,36 @@ static int htransfer_k(bool restrict_mode, int dir, const T *in, const T *in2, T
   }
   const dim3 grid((unsigned)((nl + 255) / 256), gy, (ncc + kHSeg - 1) / kHSeg);
   if (restrict_mode)
-    hipLaunchKernelGGL((hrestrict_kernel<T, K>), grid, dim3(256), 0, s, in, in2, out, nl, ibs, obs, is, os, ncc, P, lo,
-                       hi);
+    hipLaunchKernelGGL((hrestrict_kernel<T, K>), grid, dim3(256), 0, s, FS, nl, ibs, obs, is, os, ncc, P, lo, hi);
   else
     hipLaunchKernelGGL((hprolong_kernel<T, K>), grid, dim3(256), 0, s, in, out, nl, ibs, obs, is, os, ncc, P, lo, hi,
                        add ? 1 : 0);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 template <typename T>
-int launch_htransfer(bool restrict_mode, int dir, const T *in, const T *in2, T *out, const int *idims,
-                     const int *odims, int k, const double *tab_host, const T *tab_dev, int lo, int hi, bool add,
-                     hipStream_t s) {
+int launch_htransfer_fields(bool restrict_mode, int dir, const EllFields<T> &FS, const int *idims, const int *odims,
+                            int k, const double *tab_host, const T *tab_dev, int lo, int hi, bool add, hipStream_t s) {
   switch (k) {
-    case 1: return htransfer_k<T, 1>(restrict_mode, dir, in, in2, out, idims, odims, tab_host, tab_dev, lo, hi, add, s);
-    case 2: return htransfer_k<T, 2>(restrict_mode, dir, in, in2, out, idims, odims, tab_host, tab_dev, lo, hi, add, s);
-    case 3: return htransfer_k<T, 3>(restrict_mode, dir, in, in2, out, idims, odims, tab_host, tab_dev, lo, hi, add, s);
-    case 4: return htransfer_k<T, 4>(restrict_mode, dir, in, in2, out, idims, odims, tab_host, tab_dev, lo, hi, add, s);
+    case 1: return htransfer_k<T, 1>(restrict_mode, dir, FS, idims, odims, tab_host, tab_dev, lo, hi, add, s);
+    case 2: return htransfer_k<T, 2>(restrict_mode, dir, FS, idims, odims, tab_host, tab_dev, lo, hi, add, s);
+    case 3: return htransfer_k<T, 3>(restrict_mode, dir, FS, idims, odims, tab_host, tab_dev, lo, hi, add, s);
+    case 4: return htransfer_k<T, 4>(restrict_mode, dir, FS, idims, odims, tab_host, tab_dev, lo, hi, add, s);
     default: return MPDG_ERR_INVALID;
   }
 }
+template <typename T>
+int launch_htransfer(bool restrict_mode, int dir, const T *in, const T *in2, T *out, const int *idims,
+                     const int *odims, int k, const double *tab_host, const T *tab_dev, int lo, int hi, bool add,
+                     hipStream_t s) {
+  EllFields<T> FS{};
+  FS.in[0] = in;
+  FS.in2[0] = in2;
+  FS.out[0] = out;
+  FS.nf = 1;
+  return launch_htransfer_fields<T>(restrict_mode, dir, FS, idims, odims, k, tab_host, tab_dev, lo, hi, add, s);
+}
+template int launch_htransfer_fields<double>(bool, int, const EllFields<double> &, const int *, const int *, int,
+                                             const double *, const double *, int, int, bool, hipStream_t);
 template int launch_htransfer<float>(bool, int, const float *, const float *, float *, const int *, const int *, int,
                                      const double *, const float *, int, int, bool, hipStream_t);
 template int launch_htransfer<double>(bool, int, const double *, const double *, double *, const int *, const int *,

@@ -96,6 +96,10 @@ template <typename T>
 int launch_htransfer(bool restrict_mode, int dir, const T *in, const T *in2, T *out, const int *idims,
                      const int *odims, int k, const double *tab_host, const T *tab_dev, int lo, int hi, bool add,
                      hipStream_t s);
+// restriction of up to kEllMaxFields fields on the same grids in one launch (in2[f] as above)
+template <typename T>
+int launch_htransfer_fields(bool restrict_mode, int dir, const EllFields<T> &FS, const int *idims, const int *odims,
+                            int k, const double *tab_host, const T *tab_dev, int lo, int hi, bool add, hipStream_t s);
 // y = A^T x for a dense column-major n x n matrix (coarse-grid inverse), accumulated in FP64 with
 // x and y in precision T; part: dense_gemv_scratch(n) doubles
 long dense_gemv_scratch(int n);

@@ -273,20 +273,16 @@ __global__ void extrapolate_add_kernel(const double *__restrict__ Tnm1, const do
 // then a fixed tree over the threads. Outputs: s = rms(Tn - Tnm1) (+ tiny), mean(Tn),
 // rms(Tn - mean) (+ tiny), sz = rms(z) (+ tiny) or s without z.
 constexpr int kStatsBlocks = 1024;
-__global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ Tnm1, const double *__restrict__ Tn,
-                                                    const double *__restrict__ z, long n, double *__restrict__ part,
-                                                    unsigned *__restrict__ counter, double *__restrict__ stats) {
+// The four sums v (of every workgroup's threads) to the statistics: tree over the workgroup,
+// partial of the workgroup to part, and the last workgroup to finish adds the partials in a
+// fixed order (each thread the partials t, t + 256, ... with all loads in flight, then the tree)
+// and writes s = rms(Tn - Tnm1) (+ tiny), mean(Tn), rms(Tn - mean) (+ tiny), sz = rms(z)
+// (+ tiny) or s without z. The counter is back at zero afterwards.
+__device__ void stats_finish(double (&v)[4], long n, int has_z, double *__restrict__ part,
+                             unsigned *__restrict__ counter, double *__restrict__ stats) {
   __shared__ double sh[4][256];
   __shared__ bool is_last;
   const int t = threadIdx.x;
-  double v[4] = {0, 0, 0, 0};
-  for (long i = (long)blockIdx.x * 256 + t; i < n; i += (long)gridDim.x * 256) {
-    const double d = Tn[i] - Tnm1[i];
-    v[0] += d * d;
-    v[1] += Tn[i];
-    v[2] += Tn[i] * Tn[i];
-    if (z) v[3] += z[i] * z[i];
-  }
   auto tree = [&]() {
     for (int j = 0; j < 4; ++j) sh[j][t] = v[j];
     __syncthreads();
@@ -324,7 +320,6 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ T
   tree();
   if (t != 0) return;
   *counter = 0u;
-  const int has_z = z != nullptr;
   const double a = sh[0][0], b = sh[1][0], c = sh[2][0], e = sh[3][0];
   const double mean = b / n;
   stats[0] = sqrt(a / n) + 1e-30;
@@ -332,17 +327,48 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ T
   stats[2] = sqrt(fmax(c / n - mean * mean, 0.0)) + 1e-12 + 1e-6 * fabs(mean);
   stats[3] = has_z ? sqrt(e / n) + 1e-30 : stats[0];
 }
+__global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ Tnm1, const double *__restrict__ Tn,
+                                                    const double *__restrict__ z, long n, double *__restrict__ part,
+                                                    unsigned *__restrict__ counter, double *__restrict__ stats) {
+  double v[4] = {0, 0, 0, 0};
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const double d = Tn[i] - Tnm1[i];
+    v[0] += d * d;
+    v[1] += Tn[i];
+    v[2] += Tn[i] * Tn[i];
+    if (z) v[3] += z[i] * z[i];
+  }
+  stats_finish(v, n, z != nullptr, part, counter, stats);
+}
 
 // f <- (sx[ix] sy[iy] sz[iz]) f on an n0 x n1 x n2 grid, for up to four fields: the row scaling
-// that turns three passes of P^T into full weighting
+// that turns three passes of P^T into full weighting. With STATS, also the input statistics of
+// the scaled fields F.f[ia] (T_{n-1}), F.f[it] (T_n) and F.f[iz] (z; iz < 0: none), as
+// stats_kernel computes them.
+template <bool STATS>
 __global__ __launch_bounds__(256) void scale3_kernel(CnnFields F, int n0, int n1, int n2, const double *__restrict__ sx,
-                                                     const double *__restrict__ sy, const double *__restrict__ sz) {
+                                                     const double *__restrict__ sy, const double *__restrict__ sz,
+                                                     int ia, int it, int iz, double *__restrict__ part,
+                                                     unsigned *__restrict__ counter, double *__restrict__ stats) {
   const long n = (long)n0 * n1 * n2;
+  double v[4] = {0, 0, 0, 0};
   for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-    const int ix = (int)(i % n0), iy = (int)((i / n0) % n1), iz = (int)(i / ((long)n0 * n1));
-    const double w = (sx[ix] * sy[iy]) * sz[iz];
-    for (int f = 0; f < F.nf; ++f) F.f[f][i] *= w;
+    const int ix = (int)(i % n0), iy = (int)((i / n0) % n1), izz = (int)(i / ((long)n0 * n1));
+    const double w = (sx[ix] * sy[iy]) * sz[izz];
+    double val[4] = {0, 0, 0, 0};
+    for (int f = 0; f < F.nf; ++f) {
+      val[f] = F.f[f][i] * w;
+      F.f[f][i] = val[f];
+    }
+    if (STATS) {
+      const double a = val[ia], t = val[it], d = t - a;
+      v[0] += d * d;
+      v[1] += t;
+      v[2] += t * t;
+      if (iz >= 0) v[3] += val[iz] * val[iz];
+    }
   }
+  if (STATS) stats_finish(v, n, iz >= 0, part, counter, stats);
 }
 
 // ---------------------------------------------------------------------------
@@ -391,11 +417,19 @@ int cnn_stats(const double *Tnm1, const double *Tn, const double *z, long n, dou
 }
 
 int cnn_scale3(const CnnFields &F, const int *dims, const double *sx, const double *sy, const double *sz,
-               hipStream_t s) {
+               hipStream_t s, const int *stat_fields, double *part, unsigned *counter, double *stats) {
   if (F.nf < 1 || F.nf > 4) return MPDG_ERR_INVALID;
   const long n = (long)dims[0] * dims[1] * dims[2];
-  hipLaunchKernelGGL(scale3_kernel, dim3((int)std::min<long>(8192, (n + 255) / 256)), dim3(256), 0, s, F, dims[0],
-                     dims[1], dims[2], sx, sy, sz);
+  if (stat_fields) {
+    const int ia = stat_fields[0], it = stat_fields[1], iz = stat_fields[2];
+    if (ia < 0 || ia >= F.nf || it < 0 || it >= F.nf || iz >= F.nf || !part || !counter || !stats)
+      return MPDG_ERR_INVALID;
+    hipLaunchKernelGGL(scale3_kernel<true>, dim3((int)std::min<long>(kStatsBlocks, (n + 255) / 256)), dim3(256), 0, s,
+                       F, dims[0], dims[1], dims[2], sx, sy, sz, ia, it, iz, part, counter, stats);
+  } else {
+    hipLaunchKernelGGL(scale3_kernel<false>, dim3((int)std::min<long>(8192, (n + 255) / 256)), dim3(256), 0, s, F,
+                       dims[0], dims[1], dims[2], sx, sy, sz, 0, 0, -1, nullptr, nullptr, nullptr);
+  }
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
 

@@ -18,9 +18,12 @@ struct CnnFields {
   double *f[4];
   int nf;
 };
-// f *= sx[ix] sy[iy] sz[iz] for every field (dims = grid sizes along x, y, z)
+// f *= sx[ix] sy[iy] sz[iz] for every field (dims = grid sizes along x, y, z); with stat_fields
+// = {T_{n-1}, T_n, z (or -1)} indices into F, also the input statistics of the scaled fields
+// (part, counter and stats as cnn_stats takes them)
 int cnn_scale3(const CnnFields &F, const int *dims, const double *sx, const double *sy, const double *sz,
-               hipStream_t s);
+               hipStream_t s, const int *stat_fields = nullptr, double *part = nullptr, unsigned *counter = nullptr,
+               double *stats = nullptr);
 // input statistics (part: 4 x 1024 partial sums; counter: zero before the first call, left at zero)
 int cnn_stats(const double *Tnm1, const double *Tn, const double *z, long n, double *part, unsigned *counter,
               double *stats, hipStream_t s);

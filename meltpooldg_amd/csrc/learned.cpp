@@ -190,25 +190,46 @@ struct mpdg_learned {
   // DoF-grid fields f[0..count) on the network grid, into the network-grid slots slot0, slot0+1,
   // ... (the fields themselves when rho = 1; nullptr stays nullptr): the three 1D full weightings
   // each carry all the fields in one launch
-  int to_net(const double *const *f, int slot0, int count, const double **net, hipStream_t s) {
+  // with_stats (forward's inputs T_{n-1}, T_n, q, z): the input statistics come with the
+  // transfer (done is set), otherwise they are left to cnn_stats
+  int to_net(const double *const *f, int slot0, int count, const double **net, hipStream_t s,
+             bool with_stats = false, bool *done = nullptr) {
+    if (done) *done = false;
     if (hstruct) {
-      // per field three passes of P^T (x, y, z), then the row scaling of all of them at once
+      // three passes of P^T (x, y, z), each carrying all the fields, then the row scaling of all
+      // of them at once
       const int o0[3] = {dims[0], fdims[1], fdims[2]}, o1[3] = {dims[0], dims[1], fdims[2]};
+      EllFields<double> F[3] = {};
       CnnFields C{};
       for (int i = 0; i < count; ++i) {
         net[i] = f[i];
         if (!f[i]) continue;
-        double *out = cf[slot0 + i].get();
-        MPDG_TRY(launch_htransfer<double>(true, 0, f[i], nullptr, tmpa.get(), fdims, o0, degree, htab.data(),
-                                          htab_dev.get(), 0, 0, false, s));
-        MPDG_TRY(launch_htransfer<double>(true, 1, tmpa.get(), nullptr, tmpb.get(), o0, o1, degree, htab.data(),
-                                          htab_dev.get(), 0, 0, false, s));
-        MPDG_TRY(launch_htransfer<double>(true, 2, tmpb.get(), nullptr, out, o1, dims, degree, htab.data(),
-                                          htab_dev.get(), 0, 0, false, s));
-        C.f[C.nf++] = out;
-        net[i] = out;
+        const int k = F[0].nf;
+        F[0].in[k] = f[i];
+        F[0].out[k] = tmpa.get() + k * tmpa_n;
+        F[1].in[k] = F[0].out[k];
+        F[1].out[k] = tmpb.get() + k * tmpb_n;
+        F[2].in[k] = F[1].out[k];
+        F[2].out[k] = cf[slot0 + i].get();
+        for (auto &g : F) g.nf = k + 1;
+        C.f[C.nf++] = cf[slot0 + i].get();
+        net[i] = cf[slot0 + i].get();
       }
       if (C.nf == 0) return MPDG_OK;
+      MPDG_TRY(launch_htransfer_fields<double>(true, 0, F[0], fdims, o0, degree, htab.data(), htab_dev.get(), 0, 0,
+                                               false, s));
+      MPDG_TRY(launch_htransfer_fields<double>(true, 1, F[1], o0, o1, degree, htab.data(), htab_dev.get(), 0, 0,
+                                               false, s));
+      MPDG_TRY(launch_htransfer_fields<double>(true, 2, F[2], o1, dims, degree, htab.data(), htab_dev.get(), 0, 0,
+                                               false, s));
+      if (with_stats && f[0] && f[1]) {
+        // T_{n-1} and T_n are the first two fields carried; z, when given, the last
+        const int sf[3] = {0, 1, f[3] ? C.nf - 1 : -1};
+        MPDG_TRY(cnn_scale3(C, dims, isum[0].get(), isum[1].get(), isum[2].get(), s, sf, spart.get(), scount.get(),
+                            stats.get()));
+        if (done) *done = true;
+        return MPDG_OK;
+      }
       return cnn_scale3(C, dims, isum[0].get(), isum[1].get(), isum[2].get(), s);
     }
     EllFields<double> F[3] = {};
@@ -233,11 +254,12 @@ struct mpdg_learned {
   }
   int forward(const double *Tnm1, const double *Tn, const double *q, const double *z, hipStream_t s) {
     const double *in[4] = {Tnm1, Tn, q, z}, *net[4];
-    MPDG_TRY(to_net(in, 0, 4, net, s));
+    bool have_stats = false;
+    MPDG_TRY(to_net(in, 0, 4, net, s, true, &have_stats));
     const double *a = net[0], *t = net[1], *qq = net[2], *zz = net[3];
     last_Tnm1 = a;
     last_Tn = t;
-    MPDG_TRY(cnn_stats(a, t, zz, n, spart.get(), scount.get(), stats.get(), s));
+    if (!have_stats) MPDG_TRY(cnn_stats(a, t, zz, n, spart.get(), scount.get(), stats.get(), s));
     MPDG_TRY(cnn_features(a, t, qq, zz, n, stats.get(), qscale, act[0].get(), s));
     for (int l = 0; l < kLayers; ++l)
       MPDG_TRY(cnn_conv(act[l].get(), act[l + 1].get(), W[l].get(), b[l].get(), dims, l == kLayers - 1 ? 0 : 1,
