@@ -821,36 +821,36 @@ __global__ __launch_bounds__(256) void hrestrict_kernel(EllFields<T> FS, long nl
   const int nfine = 2 * K * ncc + 1, ncoarse = K * ncc + 1;
   const int c0 = blockIdx.z * kHSeg, c1 = min(ncc, c0 + kHSeg);
   for (int fi = 0; fi < FS.nf; ++fi) {
-  const T *__restrict__ in = FS.in[fi];
-  const T *__restrict__ in2 = FS.in2[fi];
-  T *__restrict__ out = FS.out[fi];
-  auto F = [&](int g) -> T {
-    if ((lo && g == 0) || (hi && g == nfine - 1)) return T(0);
-    const long a = ib + istride * g;
-    return in2 ? in[a] - in2[a] : in[a];
-  };
-  T carry = 0;
-  if (c0 > 0) {
+    const T *__restrict__ in = FS.in[fi];
+    const T *__restrict__ in2 = FS.in2[fi];
+    T *__restrict__ out = FS.out[fi];
+    auto F = [&](int g) -> T {
+      if ((lo && g == 0) || (hi && g == nfine - 1)) return T(0);
+      const long a = ib + istride * g;
+      return in2 ? in[a] - in2[a] : in[a];
+    };
+    T carry = 0;
+    if (c0 > 0) {
 #pragma unroll
-    for (int m = 1; m < 2 * K; ++m) carry += P.v[m][K] * F(2 * K * (c0 - 1) + m);
-  }
-  for (int cc = c0; cc < c1; ++cc) {
-    T f[2 * K];
-#pragma unroll
-    for (int m = 0; m < 2 * K; ++m) f[m] = F(2 * K * cc + m);
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      const int n = K * cc + j;
-      T a = j == 0 ? carry : T(0);
-#pragma unroll
-      for (int m = 0; m < 2 * K; ++m) a += P.v[m][j] * f[m];
-      out[ob + ostride * n] = (lo && n == 0) ? T(0) : a;
+      for (int m = 1; m < 2 * K; ++m) carry += P.v[m][K] * F(2 * K * (c0 - 1) + m);
     }
-    carry = 0;
+    for (int cc = c0; cc < c1; ++cc) {
+      T f[2 * K];
 #pragma unroll
-    for (int m = 1; m < 2 * K; ++m) carry += P.v[m][K] * f[m];
-    if (cc == ncc - 1) out[ob + ostride * (ncoarse - 1)] = hi ? T(0) : carry + F(nfine - 1);
-  }
+      for (int m = 0; m < 2 * K; ++m) f[m] = F(2 * K * cc + m);
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const int n = K * cc + j;
+        T a = j == 0 ? carry : T(0);
+#pragma unroll
+        for (int m = 0; m < 2 * K; ++m) a += P.v[m][j] * f[m];
+        out[ob + ostride * n] = (lo && n == 0) ? T(0) : a;
+      }
+      carry = 0;
+#pragma unroll
+      for (int m = 1; m < 2 * K; ++m) carry += P.v[m][K] * f[m];
+      if (cc == ncc - 1) out[ob + ostride * (ncoarse - 1)] = hi ? T(0) : carry + F(nfine - 1);
+    }
   }
 }
 // x lines: one wavefront per line, the input line in LDS, the table read from global memory
@@ -866,49 +866,49 @@ __global__ __launch_bounds__(256) void htransfer_x_kernel(EllFields<T> FS, long 
   T *line = lines[wave];
   const long lin = (long)ni * line_id, lout = (long)no * line_id;
   for (int fi = 0; fi < FS.nf; ++fi) {
-  const T *__restrict__ in = FS.in[fi];
-  const T *__restrict__ in2 = FS.in2[fi];
-  T *__restrict__ out = FS.out[fi];
-  if (fi > 0) {  // every lane is done with the previous field's line
+    const T *__restrict__ in = FS.in[fi];
+    const T *__restrict__ in2 = FS.in2[fi];
+    T *__restrict__ out = FS.out[fi];
+    if (fi > 0) {  // every lane is done with the previous field's line
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    for (int x = lane; x < ni; x += 64) {
+      const bool drop = (lo && x == 0) || (hi && x == ni - 1);
+      line[x] = drop ? T(0) : (in2 ? in[lin + x] - in2[lin + x] : in[lin + x]);
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  }
-  for (int x = lane; x < ni; x += 64) {
-    const bool drop = (lo && x == 0) || (hi && x == ni - 1);
-    line[x] = drop ? T(0) : (in2 ? in[lin + x] - in2[lin + x] : in[lin + x]);
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  constexpr int W = K + 1;
-  for (int x = lane; x < no; x += 64) {
-    T a = 0;
-    if (!RESTRICT) {
-      // fine node x: cell cc, row m (the last node is the last cell's right vertex)
-      const int cc = min(x / (2 * K), ncc - 1), m = x - 2 * K * cc;
+    constexpr int W = K + 1;
+    for (int x = lane; x < no; x += 64) {
+      T a = 0;
+      if (!RESTRICT) {
+        // fine node x: cell cc, row m (the last node is the last cell's right vertex)
+        const int cc = min(x / (2 * K), ncc - 1), m = x - 2 * K * cc;
 #pragma unroll
-      for (int j = 0; j <= K; ++j) a += tab[m * W + j] * line[K * cc + j];
-      if ((lo && x == 0) || (hi && x == no - 1)) a = 0;
-      out[lout + x] = add ? out[lout + x] + a : a;
-    } else {
-      // coarse node x = K cc + j: this cell's rows 0..2K-1, the previous cell's for a vertex,
-      // and the last fine node for the last vertex
-      const int cc = min(x / K, ncc - 1), j = x - K * cc;
-      if (j < K) {
+        for (int j = 0; j <= K; ++j) a += tab[m * W + j] * line[K * cc + j];
+        if ((lo && x == 0) || (hi && x == no - 1)) a = 0;
+        out[lout + x] = add ? out[lout + x] + a : a;
+      } else {
+        // coarse node x = K cc + j: this cell's rows 0..2K-1, the previous cell's for a vertex,
+        // and the last fine node for the last vertex
+        const int cc = min(x / K, ncc - 1), j = x - K * cc;
+        if (j < K) {
 #pragma unroll
-        for (int m = 0; m < 2 * K; ++m) a += tab[m * W + j] * line[2 * K * cc + m];
+          for (int m = 0; m < 2 * K; ++m) a += tab[m * W + j] * line[2 * K * cc + m];
+        }
+        const int cp = j == K ? cc : cc - 1;  // cell whose right vertex x is
+        if ((j == 0 && cc > 0) || j == K) {
+#pragma unroll
+          for (int m = 1; m < 2 * K; ++m) a += tab[m * W + K] * line[2 * K * cp + m];
+        }
+        if (j == K) a += line[nfine - 1];
+        if ((lo && x == 0) || (hi && x == no - 1)) a = 0;
+        out[lout + x] = a;
       }
-      const int cp = j == K ? cc : cc - 1;  // cell whose right vertex x is
-      if ((j == 0 && cc > 0) || j == K) {
-#pragma unroll
-        for (int m = 1; m < 2 * K; ++m) a += tab[m * W + K] * line[2 * K * cp + m];
-      }
-      if (j == K) a += line[nfine - 1];
-      if ((lo && x == 0) || (hi && x == no - 1)) a = 0;
-      out[lout + x] = a;
     }
-  }
   }
 }
 template <typename T, int K>
