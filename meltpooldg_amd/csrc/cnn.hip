@@ -374,6 +374,9 @@ __global__ __launch_bounds__(256) void scale3_kernel(CnnFields F, int n0, int n1
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// workgroups of the statistics pass: one per 4096 points (each ends with a device-scope release
+// and a counter atomic, which dominate on the network's small grids), at most kStatsBlocks
+static inline int stats_blocks(long n) { return (int)std::max<long>(1, std::min<long>(kStatsBlocks, (n + 4095) / 4096)); }
 static inline int cnn_grid(long ntiles) {
   return (int)std::max<long>(1, std::min<long>((ntiles + 3) / 4, 2048));
 }
@@ -411,7 +414,7 @@ int cnn_adam(float *w, float *m, float *v, const float *dW, const float *db, flo
 
 int cnn_stats(const double *Tnm1, const double *Tn, const double *z, long n, double *part, unsigned *counter,
               double *stats, hipStream_t s) {
-  const int nb = (int)std::min<long>(kStatsBlocks, (n + 255) / 256);
+  const int nb = stats_blocks(n);
   hipLaunchKernelGGL(stats_kernel, dim3(nb), dim3(256), 0, s, Tnm1, Tn, z, n, part, counter, stats);
   return hipGetLastError() == hipSuccess ? MPDG_OK : MPDG_ERR_DEVICE;
 }
@@ -424,7 +427,7 @@ int cnn_scale3(const CnnFields &F, const int *dims, const double *sx, const doub
     const int ia = stat_fields[0], it = stat_fields[1], iz = stat_fields[2];
     if (ia < 0 || ia >= F.nf || it < 0 || it >= F.nf || iz >= F.nf || !part || !counter || !stats)
       return MPDG_ERR_INVALID;
-    hipLaunchKernelGGL(scale3_kernel<true>, dim3((int)std::min<long>(kStatsBlocks, (n + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(scale3_kernel<true>, dim3(stats_blocks(n)), dim3(256), 0, s,
                        F, dims[0], dims[1], dims[2], sx, sy, sz, ia, it, iz, part, counter, stats);
   } else {
     hipLaunchKernelGGL(scale3_kernel<false>, dim3((int)std::min<long>(8192, (n + 255) / 256)), dim3(256), 0, s, F,
