@@ -598,7 +598,8 @@ int launch_mul(double *x, const double *y, long n, hipStream_t s) {
 // in2 (may be null) fuses a residual b - Ax into the first pass of a restriction; add fuses
 // x += P e into the last pass of a prolongation.
 // DIR 0: one wavefront per (y, z) line; the input line is staged in LDS and the wavefront
-//        forms the outputs of the line from it.
+//        forms the outputs of the line from it, one output row per lane. The table is stored
+//        entry-major (col[e * o0 + row]), so that the lanes read consecutive addresses.
 // DIR 1, 2: each thread walks a few output rows along DIR; the ELL rows are uniform over the
 //        workgroup (scalar loads). The input rows those outputs use (a contiguous range that
 //        fits the LDS stage for the multigrid transfers) are read once into a per-thread LDS
@@ -634,8 +635,8 @@ __global__ __launch_bounds__(256) void ell_dir0_kernel(EllFields<T> F, int o0, i
 #pragma unroll
       for (int e = 0; e < MAXW; ++e) {
         if (e < width) {
-          const int c = col[x * width + e];
-          if (c >= 0) acc += val[x * width + e] * line[c];
+          const int c = col[e * o0 + x];
+          if (c >= 0) acc += val[e * o0 + x] * line[c];
         }
       }
       out[lout + x] = add ? out[lout + x] + acc : acc;

@@ -74,6 +74,9 @@ int launch_residual_norm(double *z, const double *b, const double *scale, long n
 // x, r <- xb, rb unless sc[0] (the guess's squared residual norm) is finite and below sc[1]
 int launch_guess_select(const double *sc, double *x, const double *xb, double *r, const double *rb, long n,
                         hipStream_t s);
+// 1D ELL matrix (rows = odims[dir], `width` entries each, col < 0 marks an empty entry) along
+// direction dir; the tables of dir 1 and 2 are row-major (entry e of row r at r * width + e),
+// those of dir 0 entry-major (at e * odims[0] + r, see ell_entry_major)
 template <typename T>
 int launch_ell_dir(const T *in, T *out, const int *odims, const int *idims, int dir, int width, const int *col,
                    const T *val, hipStream_t s, const T *in2 = nullptr, bool add = false);

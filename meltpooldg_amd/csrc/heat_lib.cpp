@@ -450,10 +450,17 @@ struct GMGPrecond : mpdg_precond {
                           (f.dmask >> (2 * d + 1)) & 1);
         L.pw[d] = tr.pw;
         L.rw[d] = tr.rw;
-        MPDG_TRY(upload_int(L.pcol[d], tr.pcol));
-        MPDG_TRY(upload_int(L.rcol[d], tr.rcol));
-        MPDG_TRY(upload<T>(L.pval[d], tr.pval));
-        MPDG_TRY(upload<T>(L.rval[d], tr.rval));
+        if (d == 0) {  // entry-major along x
+          MPDG_TRY(upload_int(L.pcol[d], ell_entry_major(tr.pcol, tr.pw)));
+          MPDG_TRY(upload_int(L.rcol[d], ell_entry_major(tr.rcol, tr.rw)));
+          MPDG_TRY(upload<T>(L.pval[d], ell_entry_major(tr.pval, tr.pw)));
+          MPDG_TRY(upload<T>(L.rval[d], ell_entry_major(tr.rval, tr.rw)));
+        } else {
+          MPDG_TRY(upload_int(L.pcol[d], tr.pcol));
+          MPDG_TRY(upload_int(L.rcol[d], tr.rcol));
+          MPDG_TRY(upload<T>(L.pval[d], tr.pval));
+          MPDG_TRY(upload<T>(L.rval[d], tr.rval));
+        }
         L.lo[d] = (f.dmask >> (2 * d)) & 1;
         L.hi[d] = (f.dmask >> (2 * d + 1)) & 1;
       }
@@ -520,8 +527,8 @@ struct GMGPrecond : mpdg_precond {
       for (int d = 0; d < 3; ++d) {
         Interp1DHost e(P.od.degree, P.od.nc[d], L.od.degree, L.od.nc[d]);
         w[d] = e.w;
-        MPDG_TRY(upload_int(col[d], e.col));
-        MPDG_TRY(upload<double>(val[d], e.val));
+        MPDG_TRY(upload_int(col[d], d == 0 ? ell_entry_major(e.col, e.w) : e.col));
+        MPDG_TRY(upload<double>(val[d], d == 0 ? ell_entry_major(e.val, e.w) : e.val));
       }
       MPDG_TRY(L.coords.resize(3 * (size_t)L.od.n_dofs));
       DeviceArray<double> a, b;

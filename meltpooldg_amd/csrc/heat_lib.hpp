@@ -131,6 +131,17 @@ struct Transfer1DHost {
   }
 };
 
+// A row-major ELL table (entry e of row r at r * width + e) in the entry-major order of the
+// direction-0 transfers (at e * rows + r), whose lanes each form one output row
+template <typename V>
+std::vector<V> ell_entry_major(const std::vector<V> &t, int width) {
+  const size_t rows = t.size() / width;
+  std::vector<V> o(t.size());
+  for (size_t r = 0; r < rows; ++r)
+    for (int e = 0; e < width; ++e) o[(size_t)e * rows + r] = t[r * width + e];
+  return o;
+}
+
 }  // namespace mpdg
 
 // ---------------------------------------------------------------------------

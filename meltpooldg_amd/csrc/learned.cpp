@@ -140,14 +140,19 @@ struct mpdg_learned {
         }
         ww[d] = t.rw;
         pw[d] = t.pw;
-        MPDG_TRY(wcol[d].resize(t.rcol.size()));
+        // the x tables entry-major, as the direction-0 transfer reads them
+        std::vector<int> rc = d == 0 ? ell_entry_major(t.rcol, t.rw) : t.rcol;
+        std::vector<int> pc = d == 0 ? ell_entry_major(t.pcol, t.pw) : t.pcol;
+        std::vector<double> pv = d == 0 ? ell_entry_major(t.pval, t.pw) : t.pval;
+        if (d == 0) rv = ell_entry_major(rv, t.rw);
+        MPDG_TRY(wcol[d].resize(rc.size()));
         MPDG_TRY(wval[d].resize(rv.size()));
-        MPDG_TRY(pcol[d].resize(t.pcol.size()));
-        MPDG_TRY(pval[d].resize(t.pval.size()));
-        hipMemcpy(wcol[d].get(), t.rcol.data(), t.rcol.size() * sizeof(int), hipMemcpyHostToDevice);
+        MPDG_TRY(pcol[d].resize(pc.size()));
+        MPDG_TRY(pval[d].resize(pv.size()));
+        hipMemcpy(wcol[d].get(), rc.data(), rc.size() * sizeof(int), hipMemcpyHostToDevice);
         hipMemcpy(wval[d].get(), rv.data(), rv.size() * sizeof(double), hipMemcpyHostToDevice);
-        hipMemcpy(pcol[d].get(), t.pcol.data(), t.pcol.size() * sizeof(int), hipMemcpyHostToDevice);
-        hipMemcpy(pval[d].get(), t.pval.data(), t.pval.size() * sizeof(double), hipMemcpyHostToDevice);
+        hipMemcpy(pcol[d].get(), pc.data(), pc.size() * sizeof(int), hipMemcpyHostToDevice);
+        hipMemcpy(pval[d].get(), pv.data(), pv.size() * sizeof(double), hipMemcpyHostToDevice);
       }
       if (rho == 2 && fdims[0] <= 1024) {
         hstruct = true;
