@@ -326,9 +326,10 @@ int mpdg_io_permute(const long *lex_index, const double *src, double *dst, long 
        DoF grid, BF16 MFMA kernels, FP32 master weights, Adam. Enters the predictor slot of
        include/meltpooldg/linear_algebra/predictor.hpp:118-160 as a new predictor type. --- */
 /* network for the DoF grid of Q_degree on n_cells[3] cells; it works on the DoF grid of
-   Q_degree on n_cells / coarsening cells (coarsening 1, 2, 4 or 8, dividing n_cells): fields are
-   carried there by full weighting and the correction back by the FE prolongation. qscale
-   multiplies the heat-source channel (e.g. dt / (rho c_p h)). */
+   Q_degree on n_cells / coarsening cells (coarsening 1, 2, 4 or 8, dividing n_cells; with
+   coarsening > 1 at most 1024 DoFs along x): fields are carried there by full weighting and the
+   correction back by the FE prolongation. qscale multiplies the heat-source channel (e.g.
+   dt / (rho c_p h)). MPDG_ERR_INVALID otherwise. */
 int mpdg_learned_create(const int *n_cells, int degree, int coarsening, double qscale,
                         unsigned long long seed, mpdg_learned **net);
 /* output scale: the correction is oscale * rms(z) * y; the first train step sets it from its

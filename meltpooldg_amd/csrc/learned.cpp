@@ -19,7 +19,7 @@
 // Working grid: the network runs on the DoF grid of Q_k on (n_cells / rho) cells, rho in
 // {1, 2, 4, 8}. The fields are carried there by full weighting W = diag(P^T 1)^-1 P^T and
 // the correction back by the FE prolongation P (Q_k on n_cells/rho -> Q_k on n_cells), both
-// applied direction by direction. The correction of the extrapolation is smooth on the
+// applied direction by direction as log2(rho) structured 2:1 transfers. The correction of the extrapolation is smooth on the
 // diffusion length sqrt(k dt / rho c_p); when that spans more nodes than the 9^3 receptive
 // field (stiff steps, dt k / h^2 >> 1) a coarser working grid lets the network see it, at
 // rho^3 less work.
@@ -71,19 +71,20 @@ struct mpdg_learned {
   DeviceArray<float> dW[kLayers], db[kLayers], part;
   DeviceArray<double> stats, spart, lpart;
   DeviceArray<unsigned> scount;  // workgroups done in the statistics pass (back at zero after it)
-  // rho > 1: 1D full weighting (coarse <- fine) and prolongation (fine <- coarse) per direction,
-  // the fields on the network grid and the intermediate arrays of the direction passes
-  DeviceArray<int> wcol[3], pcol[3];
-  DeviceArray<double> wval[3], pval[3];
-  int ww[3] = {0, 0, 0}, pw[3] = {0, 0, 0};
+  // rho > 1: the fields on the network grid and the intermediate arrays of the direction passes
   DeviceArray<double> cf[6], tmpa, tmpb, corr, corr_f;
   long tmpa_n = 0, tmpb_n = 0;  // per field: the transfers carry up to kEllMaxFields fields at once
-  // rho = 2 (and x lines of at most 1024 nodes): the structured 2:1 transfers of the multigrid
-  // (one coarse cell's table, no index arrays), the full weighting as three passes of P^T and
-  // the inverse row sums per direction
-  bool hstruct = false;
+  // rho = 2, 4 or 8: the structured 2:1 transfers of the multigrid (one coarse cell's table, no
+  // index arrays), log2(rho) times through the DoF grids of n_cells / 2 and n_cells / 4 (the Q_k
+  // spaces are nested, so P of rho is the product of the 2:1 prolongations); the full weighting
+  // as passes of P^T and the inverse row sums of the whole transfer per direction
+  int hlev = 0;                  // 2:1 levels
+  int mdims[2][3] = {};          // DoF grids between them
+  long mid_n[2] = {0, 0};
   std::vector<double> htab;
-  DeviceArray<double> htab_dev, isum[3];
+  DeviceArray<double> htab_dev, isum[3], mid[2];
+  // DoF grid l of the 2:1 levels: 0 the fields', hlev the network's
+  const int *hgrid(int l) const { return l == 0 ? fdims : l == hlev ? dims : mdims[l - 1]; }
 
   int init(const int *cells, int k, int coarsening, double qs, unsigned long long seed) {
     degree = k;
@@ -129,57 +130,43 @@ struct mpdg_learned {
     MPDG_TRY(lpart.resize(1024));
     MPDG_TRY(corr.resize(n));
     if (rho > 1) {
+      // rho = 2^hlev; the transfers stage x lines of at most 1024 nodes (create checks both)
+      hlev = rho == 2 ? 1 : rho == 4 ? 2 : 3;
+      Transfer1DHost cell(k, 1, k, 2, false, false);  // one coarse cell: rows m = 0..2k, cols j = 0..k
+      htab.assign((size_t)(2 * k + 1) * (k + 1), 0.0);
+      for (int m = 0; m <= 2 * k; ++m)
+        for (int e = 0; e < cell.pw; ++e) {
+          const int j = cell.pcol[(size_t)m * cell.pw + e];
+          if (j >= 0) htab[(size_t)m * (k + 1) + j] = cell.pval[(size_t)m * cell.pw + e];
+        }
+      MPDG_TRY(htab_dev.resize(htab.size()));
+      hipMemcpy(htab_dev.get(), htab.data(), htab.size() * sizeof(double), hipMemcpyHostToDevice);
+      // full weighting: the rows of P^T (of the whole transfer) scaled to sum 1
       for (int d = 0; d < 3; ++d) {
         Transfer1DHost t(k, cells[d] / rho, k, cells[d], false, false);
-        // full weighting: rows of P^T scaled to sum 1
-        std::vector<double> rv = t.rval;
+        std::vector<double> inv(t.nc);
         for (int c = 0; c < t.nc; ++c) {
           double sum = 0.0;
-          for (int e = 0; e < t.rw; ++e) sum += rv[(size_t)c * t.rw + e];
-          for (int e = 0; e < t.rw; ++e) rv[(size_t)c * t.rw + e] /= sum;
+          for (int e = 0; e < t.rw; ++e) sum += t.rval[(size_t)c * t.rw + e];
+          inv[c] = 1.0 / sum;
         }
-        ww[d] = t.rw;
-        pw[d] = t.pw;
-        // the x tables entry-major, as the direction-0 transfer reads them
-        std::vector<int> rc = d == 0 ? ell_entry_major(t.rcol, t.rw) : t.rcol;
-        std::vector<int> pc = d == 0 ? ell_entry_major(t.pcol, t.pw) : t.pcol;
-        std::vector<double> pv = d == 0 ? ell_entry_major(t.pval, t.pw) : t.pval;
-        if (d == 0) rv = ell_entry_major(rv, t.rw);
-        MPDG_TRY(wcol[d].resize(rc.size()));
-        MPDG_TRY(wval[d].resize(rv.size()));
-        MPDG_TRY(pcol[d].resize(pc.size()));
-        MPDG_TRY(pval[d].resize(pv.size()));
-        hipMemcpy(wcol[d].get(), rc.data(), rc.size() * sizeof(int), hipMemcpyHostToDevice);
-        hipMemcpy(wval[d].get(), rv.data(), rv.size() * sizeof(double), hipMemcpyHostToDevice);
-        hipMemcpy(pcol[d].get(), pc.data(), pc.size() * sizeof(int), hipMemcpyHostToDevice);
-        hipMemcpy(pval[d].get(), pv.data(), pv.size() * sizeof(double), hipMemcpyHostToDevice);
-      }
-      if (rho == 2 && fdims[0] <= 1024) {
-        hstruct = true;
-        Transfer1DHost cell(k, 1, k, 2, false, false);  // one coarse cell: rows m = 0..2k, cols j = 0..k
-        htab.assign((size_t)(2 * k + 1) * (k + 1), 0.0);
-        for (int m = 0; m <= 2 * k; ++m)
-          for (int e = 0; e < cell.pw; ++e) {
-            const int j = cell.pcol[(size_t)m * cell.pw + e];
-            if (j >= 0) htab[(size_t)m * (k + 1) + j] = cell.pval[(size_t)m * cell.pw + e];
-          }
-        MPDG_TRY(htab_dev.resize(htab.size()));
-        hipMemcpy(htab_dev.get(), htab.data(), htab.size() * sizeof(double), hipMemcpyHostToDevice);
-        for (int d = 0; d < 3; ++d) {
-          Transfer1DHost t(k, cells[d] / rho, k, cells[d], false, false);
-          std::vector<double> inv(t.nc);
-          for (int c = 0; c < t.nc; ++c) {
-            double sum = 0.0;
-            for (int e = 0; e < t.rw; ++e) sum += t.rval[(size_t)c * t.rw + e];
-            inv[c] = 1.0 / sum;
-          }
-          MPDG_TRY(isum[d].resize(inv.size()));
-          hipMemcpy(isum[d].get(), inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice);
-        }
+        MPDG_TRY(isum[d].resize(inv.size()));
+        hipMemcpy(isum[d].get(), inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice);
       }
       for (auto &a : cf) MPDG_TRY(a.resize(n));
-      const long a1 = std::max((long)dims[0] * fdims[1] * fdims[2], (long)fdims[0] * fdims[1] * dims[2]);
-      const long a2 = std::max((long)dims[0] * dims[1] * fdims[2], (long)fdims[0] * dims[1] * dims[2]);
+      for (int l = 0; l + 1 < hlev; ++l) {
+        for (int d = 0; d < 3; ++d) mdims[l][d] = k * (cells[d] >> (l + 1)) + 1;
+        mid_n[l] = (long)mdims[l][0] * mdims[l][1] * mdims[l][2];
+        MPDG_TRY(mid[l].resize((size_t)kEllMaxFields * mid_n[l]));
+      }
+      // x then y then z, every pass into tmpa, tmpb, out (restriction) or tmpb, tmpa, out
+      // (prolongation): per field the largest intermediate of either, over the levels
+      long a1 = 0, a2 = 0;
+      for (int l = 0; l < hlev; ++l) {
+        const int *f = hgrid(l), *c = hgrid(l + 1);
+        a1 = std::max({a1, (long)c[0] * f[1] * f[2], (long)f[0] * f[1] * c[2]});
+        a2 = std::max({a2, (long)c[0] * c[1] * f[2], (long)f[0] * c[1] * c[2]});
+      }
       tmpa_n = a1;
       tmpb_n = a2;
       MPDG_TRY(tmpa.resize((size_t)kEllMaxFields * a1));
@@ -193,40 +180,49 @@ struct mpdg_learned {
     return MPDG_OK;
   }
   // DoF-grid fields f[0..count) on the network grid, into the network-grid slots slot0, slot0+1,
-  // ... (the fields themselves when rho = 1; nullptr stays nullptr): the three 1D full weightings
-  // each carry all the fields in one launch
+  // ... (the fields themselves when rho = 1; nullptr stays nullptr): every 1D pass of the full
+  // weighting carries all the fields in one launch
   // with_stats (forward's inputs T_{n-1}, T_n, q, z): the input statistics come with the
   // transfer (done is set), otherwise they are left to cnn_stats
   int to_net(const double *const *f, int slot0, int count, const double **net, hipStream_t s,
              bool with_stats = false, bool *done = nullptr) {
     if (done) *done = false;
-    if (hstruct) {
-      // three passes of P^T (x, y, z), each carrying all the fields, then the row scaling of all
-      // of them at once
-      const int o0[3] = {dims[0], fdims[1], fdims[2]}, o1[3] = {dims[0], dims[1], fdims[2]};
-      EllFields<double> F[3] = {};
+    if (rho > 1) {
+      // per 2:1 level three passes of P^T (x, y, z), each carrying all the fields, then the row
+      // scaling of all of them at once
+      EllFields<double> F[9] = {};
       CnnFields C{};
       for (int i = 0; i < count; ++i) {
         net[i] = f[i];
         if (!f[i]) continue;
         const int k = F[0].nf;
-        F[0].in[k] = f[i];
-        F[0].out[k] = tmpa.get() + k * tmpa_n;
-        F[1].in[k] = F[0].out[k];
-        F[1].out[k] = tmpb.get() + k * tmpb_n;
-        F[2].in[k] = F[1].out[k];
-        F[2].out[k] = cf[slot0 + i].get();
+        const double *src = f[i];
+        for (int l = 0; l < hlev; ++l) {
+          EllFields<double> *G = F + 3 * l;
+          double *dst = l + 1 < hlev ? mid[l].get() + k * mid_n[l] : cf[slot0 + i].get();
+          G[0].in[k] = src;
+          G[0].out[k] = tmpa.get() + k * tmpa_n;
+          G[1].in[k] = G[0].out[k];
+          G[1].out[k] = tmpb.get() + k * tmpb_n;
+          G[2].in[k] = G[1].out[k];
+          G[2].out[k] = dst;
+          src = dst;
+        }
         for (auto &g : F) g.nf = k + 1;
         C.f[C.nf++] = cf[slot0 + i].get();
         net[i] = cf[slot0 + i].get();
       }
       if (C.nf == 0) return MPDG_OK;
-      MPDG_TRY(launch_htransfer_fields<double>(true, 0, F[0], fdims, o0, degree, htab.data(), htab_dev.get(), 0, 0,
-                                               false, s));
-      MPDG_TRY(launch_htransfer_fields<double>(true, 1, F[1], o0, o1, degree, htab.data(), htab_dev.get(), 0, 0,
-                                               false, s));
-      MPDG_TRY(launch_htransfer_fields<double>(true, 2, F[2], o1, dims, degree, htab.data(), htab_dev.get(), 0, 0,
-                                               false, s));
+      for (int l = 0; l < hlev; ++l) {
+        const int *a = hgrid(l), *c = hgrid(l + 1);
+        const int o0[3] = {c[0], a[1], a[2]}, o1[3] = {c[0], c[1], a[2]};
+        MPDG_TRY(launch_htransfer_fields<double>(true, 0, F[3 * l], a, o0, degree, htab.data(), htab_dev.get(), 0,
+                                                 0, false, s));
+        MPDG_TRY(launch_htransfer_fields<double>(true, 1, F[3 * l + 1], o0, o1, degree, htab.data(), htab_dev.get(),
+                                                 0, 0, false, s));
+        MPDG_TRY(launch_htransfer_fields<double>(true, 2, F[3 * l + 2], o1, c, degree, htab.data(), htab_dev.get(),
+                                                 0, 0, false, s));
+      }
       if (with_stats && f[0] && f[1]) {
         // T_{n-1} and T_n are the first two fields carried; z, when given, the last
         const int sf[3] = {0, 1, f[3] ? C.nf - 1 : -1};
@@ -237,25 +233,8 @@ struct mpdg_learned {
       }
       return cnn_scale3(C, dims, isum[0].get(), isum[1].get(), isum[2].get(), s);
     }
-    EllFields<double> F[3] = {};
-    for (int i = 0; i < count; ++i) {
-      net[i] = f[i];
-      if (!f[i] || rho == 1) continue;
-      const int k = F[0].nf;
-      F[0].in[k] = f[i];
-      F[0].out[k] = tmpa.get() + k * tmpa_n;
-      F[1].in[k] = F[0].out[k];
-      F[1].out[k] = tmpb.get() + k * tmpb_n;
-      F[2].in[k] = F[1].out[k];
-      F[2].out[k] = cf[slot0 + i].get();
-      for (auto &g : F) g.nf = k + 1;
-      net[i] = cf[slot0 + i].get();
-    }
-    if (F[0].nf == 0) return MPDG_OK;
-    const int o0[3] = {dims[0], fdims[1], fdims[2]}, o1[3] = {dims[0], dims[1], fdims[2]};
-    MPDG_TRY(launch_ell_dir_fields<double>(F[0], o0, fdims, 0, ww[0], wcol[0].get(), wval[0].get(), s));
-    MPDG_TRY(launch_ell_dir_fields<double>(F[1], o1, o0, 1, ww[1], wcol[1].get(), wval[1].get(), s));
-    return launch_ell_dir_fields<double>(F[2], dims, o1, 2, ww[2], wcol[2].get(), wval[2].get(), s);
+    for (int i = 0; i < count; ++i) net[i] = f[i];  // rho = 1: the fields themselves
+    return MPDG_OK;
   }
   int forward(const double *Tnm1, const double *Tn, const double *q, const double *z, hipStream_t s) {
     const double *in[4] = {Tnm1, Tn, q, z}, *net[4];
@@ -323,20 +302,21 @@ struct mpdg_learned {
   int predict(const double *Tnm1, const double *Tn, const double *base, double *out, hipStream_t s) {
     MPDG_TRY(cnn_correction(act[kLayers].get(), n, stats.get(), oscale, corr.get(), s));
     const double *c = corr.get();
-    if (hstruct) {
-      const int o0[3] = {fdims[0], dims[1], dims[2]}, o1[3] = {fdims[0], fdims[1], dims[2]};
-      MPDG_TRY(launch_htransfer<double>(false, 0, corr.get(), nullptr, tmpb.get(), dims, o0, degree, htab.data(),
-                                        htab_dev.get(), 0, 0, false, s));
-      MPDG_TRY(launch_htransfer<double>(false, 1, tmpb.get(), nullptr, tmpa.get(), o0, o1, degree, htab.data(),
-                                        htab_dev.get(), 0, 0, false, s));
-      MPDG_TRY(launch_htransfer<double>(false, 2, tmpa.get(), nullptr, corr_f.get(), o1, fdims, degree, htab.data(),
-                                        htab_dev.get(), 0, 0, false, s));
-      c = corr_f.get();
-    } else if (rho > 1) {
-      const int o0[3] = {fdims[0], dims[1], dims[2]}, o1[3] = {fdims[0], fdims[1], dims[2]};
-      MPDG_TRY(launch_ell_dir<double>(corr.get(), tmpb.get(), o0, dims, 0, pw[0], pcol[0].get(), pval[0].get(), s));
-      MPDG_TRY(launch_ell_dir<double>(tmpb.get(), tmpa.get(), o1, o0, 1, pw[1], pcol[1].get(), pval[1].get(), s));
-      MPDG_TRY(launch_ell_dir<double>(tmpa.get(), corr_f.get(), fdims, o1, 2, pw[2], pcol[2].get(), pval[2].get(), s));
+    if (rho > 1) {
+      // per 2:1 level (coarsest first) three passes of P (x, y, z)
+      const double *src = corr.get();
+      for (int l = hlev - 1; l >= 0; --l) {  // from grid l + 1 to grid l
+        const int *a = hgrid(l), *b = hgrid(l + 1);
+        double *dst = l > 0 ? mid[l - 1].get() : corr_f.get();
+        const int o0[3] = {a[0], b[1], b[2]}, o1[3] = {a[0], a[1], b[2]};
+        MPDG_TRY(launch_htransfer<double>(false, 0, src, nullptr, tmpb.get(), b, o0, degree, htab.data(),
+                                          htab_dev.get(), 0, 0, false, s));
+        MPDG_TRY(launch_htransfer<double>(false, 1, tmpb.get(), nullptr, tmpa.get(), o0, o1, degree, htab.data(),
+                                          htab_dev.get(), 0, 0, false, s));
+        MPDG_TRY(launch_htransfer<double>(false, 2, tmpa.get(), nullptr, dst, o1, a, degree, htab.data(),
+                                          htab_dev.get(), 0, 0, false, s));
+        src = dst;
+      }
       c = corr_f.get();
     }
     return cnn_extrapolate_add(Tnm1, Tn, base, c, nf, out, s);
@@ -351,6 +331,8 @@ int mpdg_learned_create(const int *n_cells, int degree, int coarsening, double q
   if (coarsening != 1 && coarsening != 2 && coarsening != 4 && coarsening != 8) return MPDG_ERR_INVALID;
   for (int d = 0; d < 3; ++d)
     if (n_cells[d] < coarsening || n_cells[d] % coarsening != 0) return MPDG_ERR_INVALID;
+  // the 2:1 transfers stage x lines of at most 1024 DoFs
+  if (coarsening > 1 && (long)degree * n_cells[0] + 1 > 1024) return MPDG_ERR_INVALID;
   auto *h = new (std::nothrow) mpdg_learned;
   if (!h) return MPDG_ERR_NOMEM;
   const int rc = h->init(n_cells, degree, coarsening, qscale, seed);

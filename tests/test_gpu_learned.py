@@ -22,7 +22,8 @@ def make_net(M, shape, degree=1, rho=1, **kw):
 
 
 @pytest.mark.parametrize("shape,degree,rho", [((5, 6, 7), 1, 1), ((4, 9, 21), 1, 1), ((17, 13, 9), 2, 2),
-                                              ((13, 25, 13), 3, 4), ((9, 13, 17), 4, 1), ((9, 17, 17), 4, 2)])
+                                              ((13, 25, 13), 3, 4), ((9, 13, 17), 4, 1), ((9, 17, 17), 4, 2),
+                                              ((17, 17, 33), 2, 8)])
 def test_forward_matches_oracle(cuda, shape, degree, rho):
     torch = cuda
     import meltpooldg_amd as M
@@ -40,7 +41,8 @@ def test_forward_matches_oracle(cuda, shape, degree, rho):
     assert np.linalg.norm(got - exp) <= 2e-2 * np.linalg.norm(exp)
 
 
-@pytest.mark.parametrize("shape,degree,rho", [((6, 7, 18), 1, 1), ((17, 13, 9), 2, 2), ((9, 17, 17), 4, 2)])
+@pytest.mark.parametrize("shape,degree,rho", [((6, 7, 18), 1, 1), ((17, 13, 9), 2, 2), ((9, 17, 17), 4, 2),
+                                              ((13, 25, 13), 3, 4)])
 def test_gradients_match_oracle(cuda, shape, degree, rho):
     torch = cuda
     import meltpooldg_amd as M
@@ -58,6 +60,15 @@ def test_gradients_match_oracle(cuda, shape, degree, rho):
         o += n
         assert np.linalg.norm(a - b) <= 5e-2 * np.linalg.norm(b)
         assert a @ b / (np.linalg.norm(a) * np.linalg.norm(b)) > 0.995
+
+
+def test_create_rejects_what_the_transfers_cannot_carry(cuda):
+    import meltpooldg_amd as M
+    with pytest.raises(M.MpdgError):
+        M.LearnedPredictor((1024, 2, 2), degree=1, coarsening=2)  # 1025 DoFs along x
+    with pytest.raises(M.MpdgError):
+        M.LearnedPredictor((12, 12, 12), degree=1, coarsening=3)
+    M.LearnedPredictor((1024, 2, 2), degree=1, coarsening=1)
 
 
 def test_base_guess_is_what_the_network_corrects(cuda):
