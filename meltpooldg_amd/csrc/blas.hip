@@ -1095,11 +1095,29 @@ template int launch_eig_start<double>(double *, const int *, int, hipStream_t);
 // CG step kernels of the eigenvalue estimate in the level precision, scalars in FP64:
 // alpha_k = rz/(p.Ap) recorded in hist[2k]; x += alpha p; r -= alpha Ap; z = inv r;
 // beta_k = rz_new/rz recorded in hist[2k+1]; p = z + beta p.
+// In FP32 the two reductions take four grid strides per trip, their loads issued together (a
+// 4-byte load per lane leaves too few bytes in flight otherwise; in FP64 the one-stride loop is
+// faster); each thread still adds its elements in index order, so the sums are those of the
+// one-stride loop.
+template <typename T> constexpr int kEigStrides = sizeof(T) == 4 ? 4 : 1;
 template <typename T>
 __global__ __launch_bounds__(256) void eig_alpha_kernel(const T *p, const T *q, long n, double *partials,
                                                         unsigned *counter, double *sc, double *hist, int k) {
+  constexpr int U = kEigStrides<T>;
   double v[1] = {0.0};
-  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) v[0] += (double)p[i] * q[i];
+  const long st = (long)gridDim.x * 256;
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  for (; i + (U - 1) * st < n; i += U * st) {
+    T a[U], b[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      a[u] = p[i + u * st];
+      b[u] = q[i + u * st];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[0] += (double)a[u] * b[u];
+  }
+  for (; i < n; i += st) v[0] += (double)p[i] * q[i];
   if (reduce_to_last_block<1>(v, partials, counter) && threadIdx.x == 0) {
     sc[1] = sc[0] / v[0];
     hist[2 * k] = sc[1];
@@ -1109,9 +1127,29 @@ template <typename T>
 __global__ __launch_bounds__(256) void eig_update_kernel(T *r, T *z, const T *q, const T *inv, long n,
                                                          double *partials, unsigned *counter, double *sc,
                                                          double *hist, int k) {
+  constexpr int U = kEigStrides<T>;
   const T alpha = (T)sc[1];
   double v[1] = {0.0};
-  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+  const long st = (long)gridDim.x * 256;
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  for (; i + (U - 1) * st < n; i += U * st) {
+    T a[U], b[U], c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      a[u] = r[i + u * st];
+      b[u] = q[i + u * st];
+      c[u] = inv[i + u * st];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const T ri = a[u] - alpha * b[u];
+      const T zi = c[u] * ri;
+      r[i + u * st] = ri;
+      z[i + u * st] = zi;
+      v[0] += (double)ri * zi;
+    }
+  }
+  for (; i < n; i += st) {
     const T ri = r[i] - alpha * q[i];
     r[i] = ri;
     const T zi = inv[i] * ri;
