@@ -1589,7 +1589,7 @@ static int learned_guess(mpdg_heat_operation *h, double dt, const double *b, dou
   }
   if (h->base_record) MPDG_TRY(launch_convert<double, double>(T, h->base_record, n, s));
   const bool check = learned && b && h->prm.guess_check;
-  if (check) {  // before the first step the network is applied, so that no step allocates
+  if (check) {  // allocated with the network (no-ops then)
     MPDG_TRY(h->rbase.resize(n));
     MPDG_TRY(h->tbase.resize(n));
   }
@@ -1997,6 +1997,15 @@ int mpdg_heat_operation_set_learned(mpdg_heat_operation *h, mpdg_learned *net, c
   if (net && !h->scal_host && hipHostMalloc((void **)&h->scal_host, 2 * sizeof(double), 0) != hipSuccess) {
     h->scal_host = nullptr;
     return MPDG_ERR_NOMEM;
+  }
+  if (net) {
+    // the learned step's vectors (residual input, inverse diagonal, base guess and its
+    // residual), so that no step allocates
+    const size_t n = (size_t)h->op->n_dofs();
+    MPDG_TRY(h->zbuf.resize(n));
+    MPDG_TRY(h->invd.resize(n));
+    MPDG_TRY(h->rbase.resize(n));
+    MPDG_TRY(h->tbase.resize(n));
   }
   h->net = net;
   h->q_field = q_field;
