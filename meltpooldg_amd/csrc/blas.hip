@@ -259,8 +259,25 @@ template int launch_dot<double, double>(const double *, const double *, long, co
 __global__ __launch_bounds__(256) void residual_norm_kernel(double *z, const double *b, const double *scale, long n,
                                                             double *partials, unsigned *counter, double *rr,
                                                             double *raw) {
+  // four grid strides per trip, their loads issued together; each thread adds in index order
   double v[1] = {0.0};
-  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+  const long st = (long)gridDim.x * 256;
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  for (; i + 3 * st < n; i += 4 * st) {
+    double r[4], c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      r[u] = b[i + u * st] - z[i + u * st];
+      c[u] = scale ? scale[i + u * st] : 1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      v[0] += r[u] * r[u];
+      if (raw) raw[i + u * st] = r[u];
+      z[i + u * st] = scale ? r[u] * c[u] : r[u];
+    }
+  }
+  for (; i < n; i += st) {
     const double r = b[i] - z[i];
     v[0] += r * r;
     if (raw) raw[i] = r;

@@ -350,22 +350,34 @@ __global__ __launch_bounds__(256) void scale3_kernel(CnnFields F, int n0, int n1
                                                      const double *__restrict__ sy, const double *__restrict__ sz,
                                                      int ia, int it, int iz, double *__restrict__ part,
                                                      unsigned *__restrict__ counter, double *__restrict__ stats) {
-  const long n = (long)n0 * n1 * n2;
+  // grids below 2^26 points (as the convolution): 32-bit index arithmetic; the field loop and
+  // the selection of the statistics' fields are unrolled, so that nothing is indexed at run
+  // time in private memory
+  const int n = n0 * n1 * n2, n01 = n0 * n1;
   double v[4] = {0, 0, 0, 0};
-  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-    const int ix = (int)(i % n0), iy = (int)((i / n0) % n1), izz = (int)(i / ((long)n0 * n1));
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const int izz = i / n01, rem = i - izz * n01, iy = rem / n0, ix = rem - iy * n0;
     const double w = (sx[ix] * sy[iy]) * sz[izz];
     double val[4] = {0, 0, 0, 0};
-    for (int f = 0; f < F.nf; ++f) {
-      val[f] = F.f[f][i] * w;
-      F.f[f][i] = val[f];
-    }
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+      if (f < F.nf) {
+        val[f] = F.f[f][i] * w;
+        F.f[f][i] = val[f];
+      }
     if (STATS) {
-      const double a = val[ia], t = val[it], d = t - a;
+      double a = 0.0, t = 0.0, zz = 0.0;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        a = f == ia ? val[f] : a;
+        t = f == it ? val[f] : t;
+        zz = f == iz ? val[f] : zz;
+      }
+      const double d = t - a;
       v[0] += d * d;
       v[1] += t;
       v[2] += t * t;
-      if (iz >= 0) v[3] += val[iz] * val[iz];
+      v[3] += zz * zz;
     }
   }
   if (STATS) stats_finish(v, n, iz >= 0, part, counter, stats);
@@ -374,9 +386,9 @@ __global__ __launch_bounds__(256) void scale3_kernel(CnnFields F, int n0, int n1
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-// workgroups of the statistics pass: one per 4096 points (each ends with a device-scope release
+// workgroups of the statistics pass: one per 1024 points (each ends with a device-scope release
 // and a counter atomic, which dominate on the network's small grids), at most kStatsBlocks
-static inline int stats_blocks(long n) { return (int)std::max<long>(1, std::min<long>(kStatsBlocks, (n + 4095) / 4096)); }
+static inline int stats_blocks(long n) { return (int)std::max<long>(1, std::min<long>(kStatsBlocks, (n + 1023) / 1024)); }
 static inline int cnn_grid(long ntiles) {
   return (int)std::max<long>(1, std::min<long>((ntiles + 3) / 4, 2048));
 }
@@ -423,6 +435,7 @@ int cnn_scale3(const CnnFields &F, const int *dims, const double *sx, const doub
                hipStream_t s, const int *stat_fields, double *part, unsigned *counter, double *stats) {
   if (F.nf < 1 || F.nf > 4) return MPDG_ERR_INVALID;
   const long n = (long)dims[0] * dims[1] * dims[2];
+  if (n >= (1L << 26)) return MPDG_ERR_INVALID;  // 32-bit point indices, as the convolution
   if (stat_fields) {
     const int ia = stat_fields[0], it = stat_fields[1], iz = stat_fields[2];
     if (ia < 0 || ia >= F.nf || it < 0 || it >= F.nf || iz >= F.nf || !part || !counter || !stats)
